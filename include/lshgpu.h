@@ -1,0 +1,236 @@
+/*
+ * lshgpu.h -- C ABI of the MI355X-native k-NN search engine.
+ *
+ * The first block mirrors the reference's launch/result surface
+ * (LinearScan/kernel.cu, point.h, resultDTO.h, launchHelper.cuh) so a caller
+ * of `linearScans` / `LSH` can switch by renaming the call.  Inputs are host
+ * float32 row-major arrays that are never modified; results are allocated by
+ * the library with malloc (the reference's main() frees them with free(),
+ * kernel.cu:191).  Every entry point returns a status code (0 = success) and
+ * never exits the process; lsh_last_error() gives the message of the last
+ * failure on the calling thread.
+ *
+ * The second block exposes the per-stage kernels of the path (sketch
+ * builders, key builders, sketch-distance top-C) so that the integer stages
+ * can be checked bit for bit.  The third block is the resident index used for
+ * batched search and serving; it has no counterpart in the reference, whose
+ * every call resets the device (cudaHelpers.cu:12-14).
+ */
+#ifndef LSHGPU_H
+#define LSHGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ types */
+
+/* point.h:3-7 -- {int ID; float distance;}, 8 bytes. */
+typedef struct lsh_point {
+    int32_t ID;
+    float distance;
+} lsh_point_t;
+
+/* resultDTO.h:5-32 on LP64 (clock_t is long).  The three times are in
+ * milliseconds, as the reference stores them (time_lapsed*1000/CLOCKS_PER_SEC);
+ * here they are wall-clock times measured with HIP events.  recall and
+ * avgDistance are NaN until a validation step fills them. */
+typedef struct lsh_result {
+    lsh_point_t* results;   /* [N_query][k], k already rounded by lsh_calculate_k */
+    long preprocessTime;
+    long constructionTime;
+    long scanTime;
+    float recall;
+    float avgDistance;
+} lsh_result_t;
+
+/* status codes */
+enum {
+    LSH_OK = 0,
+    LSH_ERR_INVALID = 1,     /* bad id, size or argument */
+    LSH_ERR_UNSUPPORTED = 2, /* valid combination the engine does not provide */
+    LSH_ERR_DEVICE = 3,      /* HIP runtime error */
+    LSH_ERR_NOMEM = 4,       /* host or device allocation failed / over budget */
+    LSH_ERR_INTERNAL = 5
+};
+
+/* Optional knobs of the _ex entry points.  lsh_options_default() fills the
+ * values the plain entry points use. */
+typedef struct lsh_options {
+    int candidates;     /* C: size of the sketch-distance candidate set that is
+                           reranked exactly.  Default 4096 = 32 lanes x
+                           THREAD_QUEUE_SIZE 128, the width of the reference's
+                           rerank queue (constants.cuh:6, candidateSetScanner.cuh:21). */
+    int device;         /* HIP device ordinal, default 0 (cudaHelpers.cu:16-18) */
+    int rescan_margin;  /* extra FP32 candidates kept past k for the FP64 recheck
+                           (default 32) */
+    int reserved[8];
+} lsh_options_t;
+
+/* -------------------------------------------------- reference surface */
+
+/* launchHelper.cuh:21-25 calculateK: round k up to a multiple of 32. */
+int lsh_calculate_k(int k);
+
+void lsh_options_default(lsh_options_t* opt);
+
+/* kernel.cu:27 linearScans(implementation, k, d, N_query, N_data, data,
+ * queries, sketchDim, distanceFunc).
+ *   implementation 1: exact scan, distance -cos, padding (-1, 2.0f)
+ *                    (optimizedLinearScan.cu:22-91)
+ *   implementation 2: exact scan with distanceFunc 1 angular, 2 generalized
+ *                    Jaccard (after shift+normalise), 3 Euclidean
+ *                    (memOptimizedLinearScan.cu:129-191)
+ *   implementation 3: SimHash, 32*sketchDim bits, Hamming top-C, angular rerank
+ *                    (simHash.cu:61-144)
+ *   implementation 4: weighted MinHash, sketchDim hashes, match-count top-C,
+ *                    generalized-Jaccard rerank (weightedMinHash.cuh:344-429)
+ *   implementation 5: 1-bit weighted MinHash, 32*sketchDim bits, Hamming top-C,
+ *                    generalized-Jaccard rerank
+ *   implementation 6: SimHash-JL, sketchDim floats, squared-L2 top-C, angular
+ *                    rerank (simHashJL.cuh:238-281)
+ * k is rounded with lsh_calculate_k; out->results is [N_query][k] ascending by
+ * (distance, ID), padded with (-1, 2147483648.0f). */
+int lsh_linear_scans(int implementation, int k, int d, int N_query, int N_data,
+                     const float* data, const float* queries, int sketchDim,
+                     int distanceFunc, lsh_result_t* out);
+
+/* kernel.cu:91 LSH(...): (K,L) bucket framework.  keysImplementation 3 SimHash
+ * (bucketKeyBits bits per table), 4 weighted MinHash red count, 5 1-bit
+ * weighted MinHash (bucketKeyBits bits), 7 cross-polytope; implementation
+ * picks the in-bucket distance as for lsh_linear_scans (2 = exact).  The
+ * candidate set of a query is the union of its L buckets without duplicates;
+ * with runWithSketchedData it is cut to the top-C by sketch distance and
+ * reranked exactly, otherwise all of it is ranked exactly
+ * (lshFramework.cuh:142-280). */
+int lsh_framework(int implementation, int keysImplementation, int k, int d,
+                  int N_query, int N_data, const float* data,
+                  const float* queries, int sketchDim, int distanceFunc,
+                  int bucketKeyBits, int tables, int runWithSketchedData,
+                  lsh_result_t* out);
+
+int lsh_linear_scans_ex(int implementation, int k, int d, int N_query,
+                        int N_data, const float* data, const float* queries,
+                        int sketchDim, int distanceFunc,
+                        const lsh_options_t* opt, lsh_result_t* out);
+
+int lsh_framework_ex(int implementation, int keysImplementation, int k, int d,
+                     int N_query, int N_data, const float* data,
+                     const float* queries, int sketchDim, int distanceFunc,
+                     int bucketKeyBits, int tables, int runWithSketchedData,
+                     const lsh_options_t* opt, lsh_result_t* out);
+
+/* Frees out->results (allocated with malloc) and clears the pointer. */
+void lsh_result_free(lsh_result_t* r);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* lsh_last_error(void);
+
+/* Library version string. */
+const char* lsh_version(void);
+
+/* ------------------------------------------------------ stage entry points
+ * Host pointers in, host pointers out; each runs its kernels on the device. */
+
+/* randomVectorGenerator.cpp:32-53 generateRandomVectors(float*, N): N draws of
+ * std::normal_distribution<double>(0, 1) on a default-seeded
+ * std::default_random_engine, cast to float.  stddev != 1 gives the
+ * (N, sketchDim) variant of randomVectorGenerator.cpp:56-74 with
+ * stddev = 1/sketchDim. */
+int lsh_random_normals(float* out, int64_t n, double stddev);
+
+/* randomVectorGenerator.cpp:76-87 generateRandomOnePlusMinusVector. */
+int lsh_random_signs(int32_t* out, int64_t n);
+
+/* simHash.cuh:20-46 sketchDataGeneric / simHash.cu:22-47 sketch on the GPU:
+ * word w, bit b of point i is dot(x_i, planes[(w*hashBits+b)*d ...]) >= 0 with
+ * the dot product a sequential FP32 fused multiply-add chain over dimensions
+ * in index order; bits are packed LSB first.  planes == NULL uses the
+ * reference stream (lsh_random_normals with stddev 1). */
+int lsh_simhash_sketch(const float* x, int64_t n, int d, int words,
+                       int hashBits, const float* planes, uint32_t* out);
+
+/* simHashJL.cuh:34-51: float JL projections with planes drawn from
+ * N(0, 1/sketchDim) (planes == NULL uses the reference stream). */
+int lsh_jl_sketch(const float* x, int64_t n, int d, int sketchDim,
+                  const float* planes, float* out);
+
+/* processingUtils.cuh:12-67 as used by memOptimizedLinearScan.cu:149-163:
+ * shift every dimension by |min(0, floor(min over data and queries))|, then
+ * scale every row to unit L2 norm (FP32, fused multiply-add norm).  In place. */
+int lsh_gj_transform(float* data, int64_t N_data, float* queries,
+                     int64_t N_query, int d);
+
+/* Top-C candidate ids per query by (Hamming distance over `words` packed
+ * words, id), ascending.  distances may be NULL. */
+int lsh_hamming_topc(const uint32_t* data_sketch, int64_t N_data,
+                     const uint32_t* query_sketch, int64_t N_query, int words,
+                     int C, int32_t* ids, int32_t* distances);
+
+/* crossPolyTope.cuh:25-85 generateHashCrossPoly: pad to P = next power of two
+ * >= d, three rounds of (+-1 diagonal, unnormalised Walsh-Hadamard
+ * butterfly), key = argmax |v| encoded as i (negative) or i + P.  Rows are
+ * L2-normalised first (crossPolyTope.cuh:17-21).  signs == NULL uses the
+ * reference stream (lsh_random_signs over tables*3*P values). */
+int lsh_crosspoly_keys(const float* x, int64_t n, int d, int tables,
+                       const int32_t* signs, uint16_t* keys);
+
+/* ---------------------------------------------------------- resident index
+ * No counterpart in the reference.  The index keeps the data, its sketches
+ * and bucket tables resident in HBM; searches take host or device query
+ * pointers and write host or device result arrays. */
+
+typedef struct lsh_index lsh_index_t;
+
+typedef struct lsh_index_config {
+    int implementation;      /* as lsh_linear_scans / lsh_framework */
+    int distanceFunc;        /* 1, 2, 3 (implementation 2) */
+    int d;
+    int64_t N_data;
+    int sketchDim;
+    int framework;           /* 0 linear scan, 1 (K,L) buckets */
+    int keysImplementation;
+    int bucketKeyBits;
+    int tables;
+    int runWithSketchedData;
+    int candidates;          /* C; 0 = default */
+    int device;
+    int rescan_margin;       /* 0 = default 32 */
+    int reserved[8];
+} lsh_index_config_t;
+
+void lsh_index_config_default(lsh_index_config_t* cfg);
+
+/* data_on_device != 0: `data` is a device pointer on cfg->device. */
+int lsh_index_create(const lsh_index_config_t* cfg, const float* data,
+                     int data_on_device, lsh_index_t** out);
+
+/* Search N_query queries for their k nearest (k rounded up to a multiple of
+ * 32, as lsh_calculate_k).  results is [N_query][lsh_calculate_k(k)].
+ * stream: a hipStream_t (NULL = the index's own stream); the call returns
+ * when the results are complete. */
+int lsh_index_search(lsh_index_t* index, const float* queries, int N_query,
+                     int k, int queries_on_device, lsh_point_t* results,
+                     int results_on_device, void* stream);
+
+/* Per-phase device times (ms) of the last search: [0] query sketch,
+ * [1] candidate scan (all rounds), [2] candidate select, [3] rerank/finalise,
+ * [4] total; launch counts in [5]; average duration (ms) of one launch of the
+ * dominant scan kernel in [6] and the bytes it read per launch in [7].
+ * Filled only while profiling is on. */
+int lsh_index_set_profiling(lsh_index_t* index, int on);
+int lsh_index_phase_times(const lsh_index_t* index, double* out8);
+
+/* Device bytes held by the index. */
+int64_t lsh_index_device_bytes(const lsh_index_t* index);
+
+int lsh_index_destroy(lsh_index_t* index);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LSHGPU_H */

@@ -1,0 +1,423 @@
+"""lshongpu_amd -- MI355X-native k-NN search engine (Python side of the C ABI).
+
+The compute path is liblshgpu.so (hand-written HIP for gfx950, built by
+lshongpu_amd.build); this module only loads it with ctypes and mirrors the
+reference's dispatch layer (LinearScan/kernel.cu:27-117):
+
+    linearScans(implementation, k, d, N_query, N_data, data, queries,
+                sketchDim, distanceFunc) -> Result
+    LSH(implementation, keysImplementation, k, d, N_query, N_data, data,
+        queries, sketchDim, distanceFunc, bucketKeyBits, tables,
+        runWithSketchedData) -> Result
+
+with the same argument meanings.  Failures raise LshError (the reference
+prints and exits).  Index is the resident index for batched search and
+serving.  There is no CPU fallback: if the library is missing, importing the
+compute entry points raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+__all__ = [
+    "LshError", "Result", "Index", "POINT_DTYPE", "load", "calculateK", "linearScans", "LSH",
+    "linear_scans", "lsh_framework", "random_normals", "random_signs", "simhash_sketch",
+    "jl_sketch", "gj_transform", "hamming_topc", "crosspoly_keys", "library_path",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liblshgpu.so")
+_lib: Optional[C.CDLL] = None
+
+# point.h:3-7 {int ID; float distance;}
+POINT_DTYPE = np.dtype([("ID", np.int32), ("distance", np.float32)])
+
+
+class LshError(RuntimeError):
+    """A nonzero status from the C ABI."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+        self.message = message
+
+
+class _Point(C.Structure):
+    _fields_ = [("ID", C.c_int32), ("distance", C.c_float)]
+
+
+class _Result(C.Structure):
+    # resultDTO.h:5-32 on LP64
+    _fields_ = [
+        ("results", C.POINTER(_Point)),
+        ("preprocessTime", C.c_long),
+        ("constructionTime", C.c_long),
+        ("scanTime", C.c_long),
+        ("recall", C.c_float),
+        ("avgDistance", C.c_float),
+    ]
+
+
+class _Options(C.Structure):
+    _fields_ = [
+        ("candidates", C.c_int),
+        ("device", C.c_int),
+        ("rescan_margin", C.c_int),
+        ("reserved", C.c_int * 8),
+    ]
+
+
+class _IndexConfig(C.Structure):
+    _fields_ = [
+        ("implementation", C.c_int),
+        ("distanceFunc", C.c_int),
+        ("d", C.c_int),
+        ("N_data", C.c_int64),
+        ("sketchDim", C.c_int),
+        ("framework", C.c_int),
+        ("keysImplementation", C.c_int),
+        ("bucketKeyBits", C.c_int),
+        ("tables", C.c_int),
+        ("runWithSketchedData", C.c_int),
+        ("candidates", C.c_int),
+        ("device", C.c_int),
+        ("rescan_margin", C.c_int),
+        ("reserved", C.c_int * 8),
+    ]
+
+
+_FP = C.POINTER(C.c_float)
+_SIGS = {
+    "lsh_calculate_k": (C.c_int, [C.c_int]),
+    "lsh_options_default": (None, [C.POINTER(_Options)]),
+    "lsh_linear_scans": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _FP, _FP, C.c_int, C.c_int,
+                                   C.POINTER(_Result)]),
+    "lsh_linear_scans_ex": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _FP, _FP, C.c_int, C.c_int,
+                                      C.POINTER(_Options), C.POINTER(_Result)]),
+    "lsh_framework": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _FP, _FP, C.c_int,
+                                C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_Result)]),
+    "lsh_framework_ex": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _FP, _FP, C.c_int,
+                                   C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_Options),
+                                   C.POINTER(_Result)]),
+    "lsh_result_free": (None, [C.POINTER(_Result)]),
+    "lsh_last_error": (C.c_char_p, []),
+    "lsh_version": (C.c_char_p, []),
+    "lsh_random_normals": (C.c_int, [_FP, C.c_int64, C.c_double]),
+    "lsh_random_signs": (C.c_int, [C.POINTER(C.c_int32), C.c_int64]),
+    "lsh_simhash_sketch": (C.c_int, [_FP, C.c_int64, C.c_int, C.c_int, C.c_int, _FP, C.POINTER(C.c_uint32)]),
+    "lsh_jl_sketch": (C.c_int, [_FP, C.c_int64, C.c_int, C.c_int, _FP, _FP]),
+    "lsh_gj_transform": (C.c_int, [_FP, C.c_int64, _FP, C.c_int64, C.c_int]),
+    "lsh_hamming_topc": (C.c_int, [C.POINTER(C.c_uint32), C.c_int64, C.POINTER(C.c_uint32), C.c_int64, C.c_int,
+                                   C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "lsh_crosspoly_keys": (C.c_int, [_FP, C.c_int64, C.c_int, C.c_int, C.POINTER(C.c_int32),
+                                     C.POINTER(C.c_uint16)]),
+    "lsh_index_config_default": (None, [C.POINTER(_IndexConfig)]),
+    "lsh_index_create": (C.c_int, [C.POINTER(_IndexConfig), C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "lsh_index_search": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int,
+                                   C.c_void_p]),
+    "lsh_index_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "lsh_index_phase_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "lsh_index_device_bytes": (C.c_int64, [C.c_void_p]),
+    "lsh_index_destroy": (C.c_int, [C.c_void_p]),
+}
+
+EXPORTED_SYMBOLS = tuple(_SIGS)
+
+
+def library_path() -> str:
+    return _LIB_PATH
+
+
+def load() -> C.CDLL:
+    """Loads liblshgpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise LshError(-1, f"{_LIB_PATH} is missing: build it with `python -m lshongpu_amd.build`")
+    lib = C.CDLL(_LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        msg = load().lsh_last_error().decode(errors="replace")
+        raise LshError(status, msg)
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(_FP)
+
+
+@dataclass
+class Result:
+    """resultDTO.h:5-32: results is a (N_query, k) array of POINT_DTYPE."""
+
+    results: np.ndarray
+    preprocessTime: int
+    constructionTime: int
+    scanTime: int
+    recall: float
+    avgDistance: float
+
+    @property
+    def ids(self) -> np.ndarray:
+        return self.results["ID"]
+
+    @property
+    def distances(self) -> np.ndarray:
+        return self.results["distance"]
+
+
+def calculateK(k: int) -> int:
+    """launchHelper.cuh:21-25: k rounded up to a multiple of 32."""
+    return int(load().lsh_calculate_k(int(k)))
+
+
+def _options(candidates=None, device=0, rescan_margin=None) -> _Options:
+    o = _Options()
+    load().lsh_options_default(C.byref(o))
+    if candidates is not None:
+        o.candidates = int(candidates)
+    o.device = int(device)
+    if rescan_margin is not None:
+        o.rescan_margin = int(rescan_margin)
+    return o
+
+
+def _take_result(r: _Result, n_query: int, k: int) -> Result:
+    kq = calculateK(k)
+    n = n_query * kq
+    if n > 0:
+        buf = np.ctypeslib.as_array(C.cast(r.results, C.POINTER(C.c_byte)), shape=(n * 8,)).copy()
+        pts = buf.view(POINT_DTYPE).reshape(n_query, kq)
+    else:
+        pts = np.zeros((n_query, kq), dtype=POINT_DTYPE)
+    res = Result(pts, int(r.preprocessTime), int(r.constructionTime), int(r.scanTime), float(r.recall),
+                 float(r.avgDistance))
+    load().lsh_result_free(C.byref(r))
+    return res
+
+
+def linearScans(implementation, k, d, N_query, N_data, data, queries, sketchDim, distanceFunc, *,
+                candidates=None, device=0, rescan_margin=None) -> Result:
+    """kernel.cu:27 linearScans (see include/lshgpu.h for the implementation ids)."""
+    lib = load()
+    data = _f32(data).reshape(-1)
+    queries = _f32(queries).reshape(-1)
+    if data.size < int(N_data) * int(d) or queries.size < int(N_query) * int(d):
+        raise LshError(1, "data/queries are smaller than N x d")
+    r = _Result()
+    o = _options(candidates, device, rescan_margin)
+    _check(lib.lsh_linear_scans_ex(int(implementation), int(k), int(d), int(N_query), int(N_data), _fp(data),
+                                   _fp(queries), int(sketchDim), int(distanceFunc), C.byref(o), C.byref(r)))
+    return _take_result(r, int(N_query), int(k))
+
+
+def LSH(implementation, keysImplementation, k, d, N_query, N_data, data, queries, sketchDim, distanceFunc,
+        bucketKeyBits, tables, runWithSketchedData, *, candidates=None, device=0, rescan_margin=None) -> Result:
+    """kernel.cu:91 LSH: the (K,L) bucket framework."""
+    lib = load()
+    data = _f32(data).reshape(-1)
+    queries = _f32(queries).reshape(-1)
+    if data.size < int(N_data) * int(d) or queries.size < int(N_query) * int(d):
+        raise LshError(1, "data/queries are smaller than N x d")
+    r = _Result()
+    o = _options(candidates, device, rescan_margin)
+    _check(lib.lsh_framework_ex(int(implementation), int(keysImplementation), int(k), int(d), int(N_query),
+                                int(N_data), _fp(data), _fp(queries), int(sketchDim), int(distanceFunc),
+                                int(bucketKeyBits), int(tables), int(bool(runWithSketchedData)), C.byref(o),
+                                C.byref(r)))
+    return _take_result(r, int(N_query), int(k))
+
+
+linear_scans = linearScans
+lsh_framework = LSH
+
+
+# ------------------------------------------------------------- stage entry points
+
+def random_normals(n: int, stddev: float = 1.0) -> np.ndarray:
+    out = np.empty(int(n), dtype=np.float32)
+    _check(load().lsh_random_normals(_fp(out), int(n), float(stddev)))
+    return out
+
+
+def random_signs(n: int) -> np.ndarray:
+    out = np.empty(int(n), dtype=np.int32)
+    _check(load().lsh_random_signs(out.ctypes.data_as(C.POINTER(C.c_int32)), int(n)))
+    return out
+
+
+def simhash_sketch(x, words: int, hash_bits: int = 32, planes=None) -> np.ndarray:
+    x = _f32(x)
+    n, d = x.shape
+    out = np.empty((n, int(words)), dtype=np.uint32)
+    pp = None
+    if planes is not None:
+        planes = _f32(planes)
+        pp = _fp(planes)
+    _check(load().lsh_simhash_sketch(_fp(x), n, d, int(words), int(hash_bits), pp,
+                                     out.ctypes.data_as(C.POINTER(C.c_uint32))))
+    return out
+
+
+def jl_sketch(x, sketch_dim: int, planes=None) -> np.ndarray:
+    x = _f32(x)
+    n, d = x.shape
+    out = np.empty((n, int(sketch_dim)), dtype=np.float32)
+    pp = None
+    if planes is not None:
+        planes = _f32(planes)
+        pp = _fp(planes)
+    _check(load().lsh_jl_sketch(_fp(x), n, d, int(sketch_dim), pp, _fp(out)))
+    return out
+
+
+def gj_transform(data, queries):
+    data = _f32(data).copy()
+    queries = _f32(queries).copy()
+    d = data.shape[1]
+    _check(load().lsh_gj_transform(_fp(data), data.shape[0], _fp(queries), queries.shape[0], d))
+    return data, queries
+
+
+def hamming_topc(data_sketch, query_sketch, C_: int):
+    ds = np.ascontiguousarray(data_sketch, dtype=np.uint32)
+    qs = np.ascontiguousarray(query_sketch, dtype=np.uint32)
+    words = ds.shape[1]
+    ids = np.empty((qs.shape[0], int(C_)), dtype=np.int32)
+    dist = np.empty((qs.shape[0], int(C_)), dtype=np.int32)
+    _check(load().lsh_hamming_topc(ds.ctypes.data_as(C.POINTER(C.c_uint32)), ds.shape[0],
+                                   qs.ctypes.data_as(C.POINTER(C.c_uint32)), qs.shape[0], words, int(C_),
+                                   ids.ctypes.data_as(C.POINTER(C.c_int32)),
+                                   dist.ctypes.data_as(C.POINTER(C.c_int32))))
+    return ids, dist
+
+
+def crosspoly_keys(x, tables: int, signs=None) -> np.ndarray:
+    x = _f32(x)
+    n, d = x.shape
+    keys = np.empty((n, int(tables)), dtype=np.uint16)
+    sp = None
+    if signs is not None:
+        signs = np.ascontiguousarray(signs, dtype=np.int32)
+        sp = signs.ctypes.data_as(C.POINTER(C.c_int32))
+    _check(load().lsh_crosspoly_keys(_fp(x), n, d, int(tables), sp, keys.ctypes.data_as(C.POINTER(C.c_uint16))))
+    return keys
+
+
+# ------------------------------------------------------------------ resident index
+
+def _ptr_of(a):
+    """(pointer, on_device) of a numpy array or a torch tensor."""
+    if hasattr(a, "data_ptr") and hasattr(a, "is_cuda"):
+        return C.c_void_p(a.data_ptr()), bool(a.is_cuda)
+    return C.c_void_p(a.ctypes.data), False
+
+
+class Index:
+    """Resident index: data, sketches and tables stay in HBM between searches.
+
+    data may be a float32 numpy array (host) or a contiguous float32 torch
+    tensor on the device.
+    """
+
+    def __init__(self, data, *, implementation=2, distanceFunc=1, sketchDim=0, framework=0,
+                 keysImplementation=0, bucketKeyBits=0, tables=0, runWithSketchedData=False, candidates=None,
+                 device=0, rescan_margin=None):
+        lib = load()
+        cfg = _IndexConfig()
+        lib.lsh_index_config_default(C.byref(cfg))
+        if hasattr(data, "data_ptr"):
+            if not data.is_contiguous():
+                raise LshError(1, "data tensor must be contiguous")
+            n, d = data.shape
+        else:
+            data = _f32(data)
+            n, d = data.shape
+        cfg.implementation = int(implementation)
+        cfg.distanceFunc = int(distanceFunc)
+        cfg.d = int(d)
+        cfg.N_data = int(n)
+        cfg.sketchDim = int(sketchDim)
+        cfg.framework = int(framework)
+        cfg.keysImplementation = int(keysImplementation)
+        cfg.bucketKeyBits = int(bucketKeyBits)
+        cfg.tables = int(tables)
+        cfg.runWithSketchedData = int(bool(runWithSketchedData))
+        if candidates is not None:
+            cfg.candidates = int(candidates)
+        if rescan_margin is not None:
+            cfg.rescan_margin = int(rescan_margin)
+        cfg.device = int(device)
+        self.d = int(d)
+        self.n = int(n)
+        self._h = C.c_void_p()
+        ptr, on_dev = _ptr_of(data)
+        _check(lib.lsh_index_create(C.byref(cfg), ptr, int(on_dev), C.byref(self._h)))
+
+    def search(self, queries, k: int, out=None, stream=None):
+        """Returns (N_query, calculateK(k)) POINT_DTYPE results (numpy), or
+        fills `out` (a device int64/float tensor view of the same bytes)."""
+        lib = load()
+        kq = calculateK(k)
+        if hasattr(queries, "data_ptr"):
+            nq = int(queries.shape[0])
+        else:
+            queries = _f32(queries)
+            nq = int(queries.shape[0])
+        qp, q_dev = _ptr_of(queries)
+        if out is None:
+            res = np.empty((nq, kq), dtype=POINT_DTYPE)
+            rp, r_dev = C.c_void_p(res.ctypes.data), False
+        else:
+            res = out
+            rp, r_dev = _ptr_of(out)
+        sp = C.c_void_p(stream) if stream else None
+        _check(lib.lsh_index_search(self._h, qp, nq, int(k), int(q_dev), rp, int(r_dev), sp))
+        return res
+
+    def set_profiling(self, on: bool = True) -> None:
+        _check(load().lsh_index_set_profiling(self._h, int(bool(on))))
+
+    def phase_times(self) -> dict:
+        arr = (C.c_double * 8)()
+        _check(load().lsh_index_phase_times(self._h, arr))
+        v = list(arr)
+        return {"prep_ms": v[0], "scan_ms": v[1], "select_ms": v[2], "final_ms": v[3], "total_ms": v[4],
+                "scan_launches": int(v[5]), "scan_avg_ms": v[6], "scan_bytes_per_launch": v[7]}
+
+    def device_bytes(self) -> int:
+        return int(load().lsh_index_device_bytes(self._h))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            load().lsh_index_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

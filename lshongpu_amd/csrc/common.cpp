@@ -1,0 +1,88 @@
+// common.cpp -- error state, launch checks, phase timers and the host random
+// streams of the reference.
+#include <random>
+
+#include "lshgpu_internal.hpp"
+
+namespace lshgpu {
+
+void fail(int code, const std::string& msg) { throw Error(code, msg); }
+
+void check_launch(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) fail(LSH_ERR_DEVICE, std::string("launch of ") + what + " failed: " + hipGetErrorString(e));
+}
+
+// randomVectorGenerator.cpp:32-53 and :56-74: a default-seeded
+// std::default_random_engine feeding std::normal_distribution<double>(0,
+// stddev), every draw cast to float.  Calling the same library functions as
+// the reference (libstdc++ on Linux) reproduces its hyperplanes bit for bit.
+std::vector<float> normal_stream(int64_t n, double stddev) {
+    std::vector<float> v((size_t)n);
+    std::default_random_engine gen;
+    std::normal_distribution<double> dist(0.0, stddev);
+    for (int64_t i = 0; i < n; ++i) v[i] = (float)dist(gen);
+    return v;
+}
+
+// randomVectorGenerator.cpp:76-87: uniform_int_distribution<int>(0, 2),
+// 0 -> -1 and 1, 2 -> +1.
+std::vector<int32_t> sign_stream(int64_t n) {
+    std::vector<int32_t> v((size_t)n);
+    std::default_random_engine gen;
+    std::uniform_int_distribution<int> dist(0, 2);
+    for (int64_t i = 0; i < n; ++i) v[i] = dist(gen) == 0 ? -1 : 1;
+    return v;
+}
+
+// weightedMinHash.cuh:231-250: uniform_int_distribution<int>(0, 1).
+std::vector<uint8_t> bit_stream(int64_t n) {
+    std::vector<uint8_t> v((size_t)n);
+    std::default_random_engine gen;
+    std::uniform_int_distribution<int> dist(0, 1);
+    for (int64_t i = 0; i < n; ++i) v[i] = (uint8_t)dist(gen);
+    return v;
+}
+
+PhaseTimer::~PhaseTimer() {
+    for (auto& r : recs_) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (auto& o : open_) (void)hipEventDestroy(o.second);
+}
+
+void PhaseTimer::begin(int phase, hipStream_t s) {
+    if (!on_) return;
+    hipEvent_t e;
+    LSH_HIP(hipEventCreate(&e));
+    LSH_HIP(hipEventRecord(e, s));
+    open_.push_back({phase, e});
+}
+
+void PhaseTimer::end(int phase, hipStream_t s) {
+    if (!on_) return;
+    for (size_t i = open_.size(); i-- > 0;) {
+        if (open_[i].first == phase) {
+            hipEvent_t e;
+            LSH_HIP(hipEventCreate(&e));
+            LSH_HIP(hipEventRecord(e, s));
+            recs_.push_back({phase, open_[i].second, e});
+            open_.erase(open_.begin() + (long)i);
+            return;
+        }
+    }
+}
+
+void PhaseTimer::collect(double* ms, int nphase) {
+    for (int p = 0; p < nphase; ++p) ms[p] = 0.0;
+    if (!on_) return;
+    for (auto& r : recs_) {
+        LSH_HIP(hipEventSynchronize(r.b));
+        float t = 0.0f;
+        LSH_HIP(hipEventElapsedTime(&t, r.a, r.b));
+        if (r.phase >= 0 && r.phase < nphase) ms[r.phase] += t;
+    }
+}
+
+}  // namespace lshgpu

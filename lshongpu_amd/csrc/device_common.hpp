@@ -1,0 +1,137 @@
+// device_common.hpp -- device helpers shared by the scan, select and rerank
+// kernels (header-only so every translation unit inlines them).
+#pragma once
+
+#include "lshgpu_internal.hpp"
+
+namespace lshgpu {
+
+// Appends `key` for the lanes with `pass` to query row `qrow` of the top-K
+// state: one atomic per wave reserves the slots, each lane writes at its rank
+// among the passing lanes.  Slots past newcap are counted but not written
+// (the select kernel records the overflow).
+__device__ inline void append_key(const TopKView& v, int qrow, bool pass, uint64_t key) {
+    const unsigned long long pm = __ballot(pass);
+    if (pm == 0) return;
+    const int leader = __ffsll((long long)pm) - 1;
+    const int lane = __lane_id();
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&v.cnt[qrow], (uint32_t)__popcll(pm));
+    base = __shfl(base, leader);
+    if (pass) {
+        const uint32_t rank = (uint32_t)__popcll(pm & ((1ull << lane) - 1ull));
+        const uint32_t slot = base + rank;
+        if (slot < (uint32_t)v.newcap) v.keys[(size_t)qrow * v.stride + v.K + slot] = key;
+    }
+}
+
+// FP64 distance with the fixed operation order shared with the oracle
+// (distanceFunctions.cuh:10-71 in double precision): sequential fused
+// multiply-adds of the float inputs promoted to double, correctly rounded
+// square root and division.
+__device__ inline double dist64_dev(int kind, const float* __restrict__ q,
+                                    const float* __restrict__ x, int d) {
+    if (kind == kNegCos || kind == kAngular) {
+        double dot = 0, nq = 0, nx = 0;
+        for (int j = 0; j < d; ++j) {
+            const double a = q[j], b = x[j];
+            dot = __builtin_fma(a, b, dot);
+            nq = __builtin_fma(a, a, nq);
+            nx = __builtin_fma(b, b, nx);
+        }
+        const double c = dot / (__builtin_sqrt(nq) * __builtin_sqrt(nx));
+        return kind == kNegCos ? -c : 1.0 - c;
+    } else if (kind == kGenJaccard) {
+        double smin = 0, smax = 0;
+        for (int j = 0; j < d; ++j) {
+            const double a = q[j], b = x[j];
+            smin = smin + ((b < a) ? b : a);
+            smax = smax + ((a < b) ? b : a);
+        }
+        return 1.0 - smin / smax;
+    } else {
+        double s = 0;
+        for (int j = 0; j < d; ++j) {
+            const double t = (double)q[j] - (double)x[j];
+            s = __builtin_fma(t, t, s);
+        }
+        return __builtin_sqrt(s);
+    }
+}
+
+// FP32 distance as the reference's device code writes it
+// (distanceFunctions.cuh:23-52; kNegCos per optimizedLinearScan.cu:45-57).
+__device__ inline float dist32_dev(int kind, const float* __restrict__ q,
+                                   const float* __restrict__ x, int d, float q_norm) {
+    if (kind == kNegCos || kind == kAngular) {
+        float dot = 0.0f, nx = 0.0f;
+        for (int j = 0; j < d; ++j) {
+            dot = __builtin_fmaf(q[j], x[j], dot);
+            nx = __builtin_fmaf(x[j], x[j], nx);
+        }
+        const float c = dot / (q_norm * __builtin_sqrtf(nx));
+        return kind == kNegCos ? -c : 1.0f - c;
+    } else if (kind == kGenJaccard) {
+        float smin = 0.0f, smax = 0.0f;
+        for (int j = 0; j < d; ++j) {
+            smin += fminf(x[j], q[j]);
+            smax += fmaxf(x[j], q[j]);
+        }
+        return 1.0f - smin / smax;
+    } else {
+        float s = 0.0f;
+        for (int j = 0; j < d; ++j) {
+            const float t = q[j] - x[j];
+            s = __builtin_fmaf(t, t, s);
+        }
+        return s;  // squared; kEuclid ranks by the square
+    }
+}
+
+__device__ inline void cmpx_u64(uint64_t* a, int i, int j, bool asc) {
+    const uint64_t x = a[i], y = a[j];
+    if ((x > y) == asc) {
+        a[i] = y;
+        a[j] = x;
+    }
+}
+
+// Ascending bitonic sort of n (power of two) keys in LDS by the whole block.
+__device__ inline void bitonic_sort_lds(uint64_t* a, int n) {
+    for (int size = 2; size <= n; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            __syncthreads();
+            for (int t = threadIdx.x; t < (n >> 1); t += blockDim.x) {
+                const int i = 2 * t - (t & (stride - 1));
+                cmpx_u64(a, i, i + stride, (i & size) == 0);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Ascending sort of (key, id) pairs (n a power of two) in LDS.
+__device__ inline void bitonic_sort_pairs(uint64_t* k, int32_t* id, int n) {
+    for (int size = 2; size <= n; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            __syncthreads();
+            for (int t = threadIdx.x; t < (n >> 1); t += blockDim.x) {
+                const int i = 2 * t - (t & (stride - 1));
+                const int j = i + stride;
+                const bool asc = (i & size) == 0;
+                const uint64_t ki = k[i], kj = k[j];
+                const int32_t ii = id[i], ij = id[j];
+                const bool gt = ki > kj || (ki == kj && ii > ij);
+                if (gt == asc) {
+                    k[i] = kj;
+                    k[j] = ki;
+                    id[i] = ij;
+                    id[j] = ii;
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+}  // namespace lshgpu

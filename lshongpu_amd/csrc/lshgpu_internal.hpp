@@ -1,0 +1,281 @@
+// lshgpu_internal.hpp -- shared declarations of the engine (host and device).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/lshgpu.h"
+
+namespace lshgpu {
+
+// ------------------------------------------------------------------ errors
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] void fail(int code, const std::string& msg);
+
+#define LSH_HIP(expr)                                                              \
+    do {                                                                           \
+        hipError_t _e = (expr);                                                    \
+        if (_e != hipSuccess)                                                      \
+            ::lshgpu::fail(LSH_ERR_DEVICE, std::string(#expr) + ": " +             \
+                                               hipGetErrorString(_e) + " at " +    \
+                                               __FILE__ + ":" + std::to_string(__LINE__)); \
+    } while (0)
+
+#define LSH_REQUIRE(cond, code, msg) \
+    do {                             \
+        if (!(cond)) ::lshgpu::fail(code, msg); \
+    } while (0)
+
+// Checks the launch just issued on `s` (configuration errors only; faults
+// surface at the next synchronisation).
+void check_launch(const char* what);
+
+// ------------------------------------------------------------ device memory
+
+template <class T>
+class DevBuf {
+   public:
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { alloc(n); }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p_ = o.p_;
+            n_ = o.n_;
+            o.p_ = nullptr;
+            o.n_ = 0;
+        }
+        return *this;
+    }
+    void alloc(size_t n) {
+        release();
+        if (n == 0) return;
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, n * sizeof(T));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            fail(LSH_ERR_NOMEM, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed: " +
+                                    hipGetErrorString(e));
+        }
+        p_ = static_cast<T*>(p);
+        n_ = n;
+    }
+    // Grow-only reuse.
+    void ensure(size_t n) {
+        if (n > n_) alloc(n);
+    }
+    void release() {
+        if (p_) (void)hipFree(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    T* get() const { return p_; }
+    size_t size() const { return n_; }
+    size_t bytes() const { return n_ * sizeof(T); }
+
+   private:
+    T* p_ = nullptr;
+    size_t n_ = 0;
+};
+
+// ------------------------------------------------------------ ordered keys
+// A candidate is a 64-bit key: an order-preserving image of its distance in
+// the high 32 bits and its ID in the low 32 bits, so that comparing keys as
+// unsigned integers is the (distance, ID) order.  kEmptyKey marks no candidate.
+
+constexpr uint64_t kEmptyKey = ~0ull;
+constexpr float kPadDistance = 2147483648.0f;  // (float)INT_MAX, candidateSetScanner.cuh:30
+
+__host__ __device__ inline uint32_t f32_key(float f) {
+    if (f != f) return 0xFFFFFFFFu;  // every NaN ranks last
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__host__ __device__ inline float key_f32(uint32_t k) {
+    uint32_t u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+    return __builtin_bit_cast(float, u);
+}
+__host__ __device__ inline uint64_t f64_key(double f) {
+    if (f != f) return ~0ull;  // every NaN ranks last
+    uint64_t u = __builtin_bit_cast(uint64_t, f);
+    return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+__host__ __device__ inline double key_f64(uint64_t k) {
+    uint64_t u = (k & 0x8000000000000000ull) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+    return __builtin_bit_cast(double, u);
+}
+__host__ __device__ inline uint64_t make_key(uint32_t dkey, uint32_t id) {
+    return ((uint64_t)dkey << 32) | id;
+}
+
+// --------------------------------------------------------- distance kinds
+// Exact distances of distanceFunctions.cuh:10-71.  kNegCos is the -cos of
+// implementation 1 (optimizedLinearScan.cu:56-57); kSqL2 the squared L2 of
+// the JL sketch scan (distanceFunctions.cuh:87-99).
+enum DistKind : int { kNegCos = 0, kAngular = 1, kGenJaccard = 2, kEuclid = 3, kSqL2 = 4 };
+
+// Bound on |FP32 distance - exact distance| used to decide when the FP32
+// ranking may differ from the FP64 one (absolute for kNegCos/kAngular/
+// kGenJaccard, relative for kEuclid).  d is the dimension.
+double fp32_error_bound(int kind, int d);
+
+// ------------------------------------------------------------- top-K state
+// Per query: keys[q*stride .. +K) hold the best K keys so far in ascending
+// order (kEmptyKey padded); keys[q*stride + K .. + newcap) receive the
+// candidates appended by a scan round; cnt[q] counts the appends (it may
+// exceed newcap, which is recorded as an overflow); thr[q] is the K-th best
+// key or kEmptyKey while fewer than K are known.
+struct TopKView {
+    uint64_t* keys;
+    uint32_t* cnt;
+    uint64_t* thr;
+    uint32_t* ovf;
+    int K;
+    int newcap;
+    int stride;  // K + newcap
+    int Q;
+};
+
+class TopKState {
+   public:
+    void reset(int Q, int K, int newcap, hipStream_t s);
+    TopKView view() const { return v_; }
+    size_t bytes() const { return keys_.bytes() + cnt_.bytes() + thr_.bytes() + ovf_.bytes(); }
+
+   private:
+    DevBuf<uint64_t> keys_;
+    DevBuf<uint32_t> cnt_;
+    DevBuf<uint64_t> thr_;
+    DevBuf<uint32_t> ovf_;
+    TopKView v_{};
+};
+
+// Largest K the select kernels support (keys of the best list plus the sort
+// buffer must fit the 160 KiB LDS of a CDNA4 workgroup).
+constexpr int kMaxTopK = 4096;
+
+int pow2_ceil(int x);
+int newcap_for(int K);
+
+// ----------------------------------------------------------- kernel entry
+// (defined in the .hip files; all launch on `s` and do not synchronise)
+
+// select.hip
+void launch_select_merge(const TopKView& v, hipStream_t s);
+void launch_count_overflow(const TopKView& v, uint32_t* out_count, hipStream_t s);
+
+// scan_exact.hip: distances of queries [0,Q) to points [p0,p1) of data.
+struct ExactScanArgs {
+    const float* data;       // [N][d]
+    const float* data_inv;   // [N] 1/||x|| (FP32) for kNegCos/kAngular, else null
+    const float* qT;         // [ceil(Q/QG)][d][QG] transposed query groups
+    const float* q_inv;      // [Q] 1/||q|| (FP32) for kNegCos/kAngular
+    int d;
+    int Q;
+    int64_t p0, p1;
+    int kind;
+};
+int exact_scan_query_group();
+void launch_exact_scan(const ExactScanArgs& a, const TopKView& v, hipStream_t s);
+void launch_transpose_queries(const float* q, int Q, int d, float* qT, hipStream_t s);
+void launch_row_inv_norms(const float* x, int64_t n, int d, float* inv, hipStream_t s);
+
+// FP64 finalisation of an exact scan: keys hold the best K' by FP32; writes
+// k results per query (id, (float)FP64 distance) in (FP64 distance, id)
+// order, and flags[q] = 1 when the FP32 margin does not prove the set exact.
+void launch_exact_finalize(const float* data, const float* queries, int d, int kind,
+                           const TopKView& v, int k, double eps, lsh_point_t* out,
+                           uint32_t* flags, hipStream_t s);
+
+// rerank.hip: candidates = low 32 bits of the first C keys of each query
+// (or `cand` ids when non-null, [Q][C]); exact FP32 distances, best K' kept,
+// FP64 recheck (all C in FP64 when the margin fails), k written.
+void launch_rerank(const float* data, const float* queries, int d, int kind, const uint64_t* keys,
+                   int key_stride, const int32_t* cand, int C, int Q, int k, int kprime, double eps,
+                   lsh_point_t* out, hipStream_t s);
+
+// sketch.hip (out rows have `out_stride` words; planesT is [words][d][hashBits])
+void launch_simhash(const float* x, int64_t n, int d, const float* planesT, int words,
+                    int hashBits, uint32_t* out, int out_stride, hipStream_t s);
+void launch_jl(const float* x, int64_t n, int d, const float* planesT, int sdim, float* out,
+               hipStream_t s);
+void launch_normalize_rows(float* x, int64_t n, int d, hipStream_t s);
+void launch_gj_mins(const float* x, int64_t n, int d, int* mins, hipStream_t s);
+void launch_gj_shift(float* x, int64_t n, int d, const int* mins, hipStream_t s);
+void launch_crosspoly(const float* x, int64_t n, int d, int tables, const int32_t* signs,
+                      uint16_t* keys, hipStream_t s);
+
+// scan_sketch.hip: integer sketch distances of queries [0,Q) to points
+// [p0,p1).  kind 0: Hamming over `words` words; kind 1: words minus the
+// number of equal words (weighted-MinHash match count).
+struct SketchScanArgs {
+    const uint32_t* data;   // [N][words]
+    const uint32_t* qs;     // [ceil(Q/64)*64][words], rows past Q zero
+    int words;
+    int Q;
+    int64_t p0, p1;
+    int kind;
+};
+void launch_sketch_scan(const SketchScanArgs& a, const TopKView& v, hipStream_t s);
+
+// minhash.hip
+void launch_wmh_bounds(const float* x, int64_t n, int d, int* bounds, hipStream_t s);
+void launch_wmh_sketch(const float* x, int64_t n, int d, const int* owner, const int* start,
+                       int M, int H, int mode, const uint8_t* bitmap, int bitmap_len,
+                       const float* draws, const int* draw_owner, int R, uint32_t* out,
+                       hipStream_t s);
+
+// util.hip
+void launch_gather_rows_f32(const float* src, int cols, const int32_t* idx, int m, float* dst,
+                            hipStream_t s);
+void launch_gather_rows_u32(const uint32_t* src, int cols, const int32_t* idx, int m, uint32_t* dst,
+                            hipStream_t s);
+void launch_scatter_rows_u64(const uint64_t* src, int src_stride, int cols, const int32_t* idx, int m,
+                             uint64_t* dst, int dst_stride, hipStream_t s);
+void launch_scatter_rows_pts(const lsh_point_t* src, int cols, const int32_t* idx, int m,
+                             lsh_point_t* dst, hipStream_t s);
+void launch_pad_rows_u32(const uint32_t* src, int64_t n, int cols, int64_t npad, int pcols,
+                         uint32_t* dst, hipStream_t s);
+void launch_fill_i32(int32_t* p, size_t n, int32_t v, hipStream_t s);
+
+// ---------------------------------------------------------- host helpers
+
+std::vector<float> normal_stream(int64_t n, double stddev);
+std::vector<int32_t> sign_stream(int64_t n);
+std::vector<uint8_t> bit_stream(int64_t n);
+
+// Phase timing with HIP events on one stream.
+class PhaseTimer {
+   public:
+    explicit PhaseTimer(bool on) : on_(on) {}
+    ~PhaseTimer();
+    void begin(int phase, hipStream_t s);
+    void end(int phase, hipStream_t s);
+    // Sum of the recorded intervals per phase (ms); synchronises the events.
+    void collect(double* ms, int nphase);
+
+   private:
+    bool on_;
+    struct Rec {
+        int phase;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs_;
+    std::vector<std::pair<int, hipEvent_t>> open_;
+};
+
+}  // namespace lshgpu

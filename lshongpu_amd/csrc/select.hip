@@ -1,0 +1,126 @@
+// select.hip -- per-query top-K maintenance between scan rounds.
+//
+// A scan round appends every candidate whose key beats the query's current
+// threshold (the K-th best key so far).  This kernel folds those appends into
+// the sorted best-K list: the new keys are bitonic-sorted in LDS, the K
+// smallest of (best list, new keys) are taken as the element-wise minimum of
+// the best list and the reversed sorted new keys (a bitonic sequence), and a
+// half-cleaner network sorts them.  This replaces the reference's per-lane
+// insertion sort plus warp bitonic network (sortingFramework.cuh:163-254)
+// with exact (distance, ID) keys instead of its EPSILON comparator
+// (pointExtensions.cuh:8-33).
+#include "device_common.hpp"
+
+namespace lshgpu {
+
+namespace {
+
+constexpr int kSelThreads = 256;
+
+__global__ __launch_bounds__(kSelThreads) void select_merge_kernel(TopKView v) {
+    extern __shared__ uint64_t sm[];
+    const int q = blockIdx.x;
+    const uint32_t n_raw = v.cnt[q];
+    if (n_raw == 0) return;
+    const int K = v.K;
+    const int n = (int)min(n_raw, (uint32_t)v.newcap);
+    int P2 = K;
+    while (P2 < n) P2 <<= 1;
+    uint64_t* A = sm;
+    uint64_t* B = sm + K;
+    uint64_t* row = v.keys + (size_t)q * v.stride;
+    for (int i = threadIdx.x; i < P2; i += blockDim.x) B[i] = i < n ? row[K + i] : kEmptyKey;
+    for (int i = threadIdx.x; i < K; i += blockDim.x) A[i] = row[i];
+    bitonic_sort_lds(B, P2);
+    for (int i = threadIdx.x; i < K; i += blockDim.x) {
+        uint64_t a = A[i], b = B[K - 1 - i];
+        A[i] = a < b ? a : b;
+    }
+    for (int stride = K >> 1; stride > 0; stride >>= 1) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < (K >> 1); t += blockDim.x) {
+            int i = 2 * t - (t & (stride - 1));
+            cmpx_u64(A, i, i + stride, true);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < K; i += blockDim.x) row[i] = A[i];
+    if (threadIdx.x == 0) {
+        v.thr[q] = A[K - 1];
+        v.cnt[q] = 0;
+        if (n_raw > (uint32_t)v.newcap) v.ovf[q] = 1;
+    }
+}
+
+__global__ void fill_u64(uint64_t* p, size_t n, uint64_t val) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t st = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += st) p[i] = val;
+}
+
+__global__ void fill_u32(uint32_t* p, size_t n, uint32_t val) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t st = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += st) p[i] = val;
+}
+
+__global__ void count_flags_kernel(const uint32_t* f, int n, uint32_t* out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t c = (i < n && f[i]) ? 1u : 0u;
+    unsigned long long m = __ballot(c);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(out, (uint32_t)__popcll(m));
+}
+
+unsigned grid_for(size_t n) {
+    size_t b = (n + 255) / 256;
+    return (unsigned)std::min<size_t>(std::max<size_t>(b, 1), 65535 * 8);
+}
+
+}  // namespace
+
+int pow2_ceil(int x) {
+    int p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+int newcap_for(int K) { return K <= 1024 ? 2 * K + 256 : K + 512; }
+
+void TopKState::reset(int Q, int K, int newcap, hipStream_t s) {
+    LSH_REQUIRE(K >= 1 && K <= kMaxTopK && (K & (K - 1)) == 0, LSH_ERR_INVALID,
+                "top-K size must be a power of two <= 4096");
+    const int stride = K + newcap;
+    keys_.ensure((size_t)Q * stride);
+    cnt_.ensure((size_t)Q);
+    thr_.ensure((size_t)Q);
+    ovf_.ensure((size_t)Q);
+    v_ = TopKView{keys_.get(), cnt_.get(), thr_.get(), ovf_.get(), K, newcap, stride, Q};
+    // Fill kernels rather than memsets keep the sequence capturable in a graph.
+    size_t nk = (size_t)Q * stride;
+    hipLaunchKernelGGL(fill_u64, dim3(grid_for(nk)), dim3(256), 0, s, keys_.get(), nk, kEmptyKey);
+    hipLaunchKernelGGL(fill_u64, dim3(grid_for(Q)), dim3(256), 0, s, thr_.get(), (size_t)Q, kEmptyKey);
+    hipLaunchKernelGGL(fill_u32, dim3(grid_for(Q)), dim3(256), 0, s, cnt_.get(), (size_t)Q, 0u);
+    hipLaunchKernelGGL(fill_u32, dim3(grid_for(Q)), dim3(256), 0, s, ovf_.get(), (size_t)Q, 0u);
+    check_launch("top-K state reset");
+}
+
+void launch_select_merge(const TopKView& v, hipStream_t s) {
+    const size_t lds = (size_t)(v.K + pow2_ceil(std::max(v.newcap, v.K))) * sizeof(uint64_t);
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute((const void*)select_merge_kernel,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(select_merge_kernel, dim3(v.Q), dim3(kSelThreads), lds, s, v);
+    check_launch("select_merge_kernel");
+}
+
+void launch_count_overflow(const TopKView& v, uint32_t* out_count, hipStream_t s) {
+    hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, s, out_count, (size_t)1, 0u);
+    hipLaunchKernelGGL(count_flags_kernel, dim3((v.Q + 255) / 256), dim3(256), 0, s, v.ovf, v.Q,
+                       out_count);
+    check_launch("count_flags_kernel");
+}
+
+}  // namespace lshgpu

@@ -1,0 +1,225 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Integer stages must be bit-exact (sketch bits, Hamming top-C IDs, keys);
+returned IDs must equal the oracle's (FP64 distance, ID) order; returned
+distances must be within 1e-5 relative of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-5
+
+
+def _close(a, b, rtol=RTOL):
+    np.testing.assert_allclose(a, b, rtol=rtol, atol=1e-6)
+
+
+def _unit(rng, n, d):
+    x = rng.standard_normal((n, d)).astype(np.float32)
+    return (x / np.linalg.norm(x, axis=1, keepdims=True)).astype(np.float32)
+
+
+# --------------------------------------------------------------- exact scan
+
+@pytest.mark.parametrize("impl,func", [(2, 1), (2, 2), (2, 3), (1, 1)])
+def test_exact_scan_matches_oracle(lsh, orc, impl, func):
+    rng = np.random.default_rng(100 + 10 * impl + func)
+    N, Q, d, k = 5000, 70, 100, 32
+    data = rng.standard_normal((N, d)).astype(np.float32)
+    queries = rng.standard_normal((Q, d)).astype(np.float32)
+    res = lsh.linearScans(impl, k, d, Q, N, data, queries, 0, func)
+    if impl == 1:
+        ids, dist = orc.exact_knn(data, queries, k, 0)
+    elif func == 2:
+        td, tq = orc.gj_transform(data, queries)
+        ids, dist = orc.exact_knn(td, tq, k, 2)
+    else:
+        ids, dist = orc.exact_knn(data, queries, k, func)
+    np.testing.assert_array_equal(res.ids, ids)
+    _close(res.distances, dist)
+
+
+def test_exact_scan_config0_size(lsh, orc):
+    from lshongpu_amd import synth
+
+    data, queries = synth.config(0)
+    res = lsh.linearScans(2, 32, 100, len(queries), len(data), data, queries, 0, 1)
+    ids, dist = orc.exact_knn(data, queries, 32, 1)
+    np.testing.assert_array_equal(res.ids, ids)
+    _close(res.distances, dist)
+
+
+def test_exact_scan_odd_dims_padding_and_k_rounding(lsh, orc):
+    rng = np.random.default_rng(7)
+    N, Q, d = 20, 5, 37
+    data = rng.standard_normal((N, d)).astype(np.float32)
+    queries = rng.standard_normal((Q, d)).astype(np.float32)
+    res = lsh.linearScans(2, 10, d, Q, N, data, queries, 0, 1)  # k=10 -> 32
+    assert res.results.shape == (Q, 32)
+    ids, dist = orc.exact_knn(data, queries, 32, 1)
+    np.testing.assert_array_equal(res.ids, ids)
+    assert (res.ids[:, N:] == -1).all()
+    assert (res.distances[:, N:] == np.float32(2147483648.0)).all()
+    _close(res.distances[:, :N], dist[:, :N])
+
+
+def test_exact_scan_duplicates_break_ties_by_index(lsh, orc):
+    rng = np.random.default_rng(8)
+    base = _unit(rng, 50, 16)
+    data = np.concatenate([base[rng.integers(0, 50, 3000)], _unit(rng, 1000, 16)])
+    queries = base[:9].copy()
+    res = lsh.linearScans(2, 64, 16, 9, len(data), data, queries, 0, 1)
+    ids, dist = orc.exact_knn(data, queries, 64, 1)
+    np.testing.assert_array_equal(res.ids, ids)
+    _close(res.distances, dist)
+
+
+def test_exact_scan_large_k(lsh, orc):
+    rng = np.random.default_rng(9)
+    data = _unit(rng, 6000, 24)
+    queries = _unit(rng, 3, 24)
+    res = lsh.linearScans(2, 1000, 24, 3, 6000, data, queries, 0, 3)
+    ids, dist = orc.exact_knn(data, queries, 1024, 3)
+    np.testing.assert_array_equal(res.ids, ids)
+    _close(res.distances, dist)
+
+
+def test_exact_scan_overflow_fallback_sorted_data(lsh, orc):
+    # Data ordered from farthest to nearest for every query: each scan round
+    # appends its whole range, which overflows and takes the re-run path.
+    rng = np.random.default_rng(10)
+    q = _unit(rng, 1, 32)
+    x = _unit(rng, 60000, 32)
+    data = x[np.argsort(x @ q[0])].copy()  # cos ascending = distance descending
+    queries = np.concatenate([q, q * 0.5 + _unit(rng, 1, 32) * 0.1]).astype(np.float32)
+    res = lsh.linearScans(2, 32, 32, 2, len(data), data, queries, 0, 1)
+    ids, dist = orc.exact_knn(data, queries, 32, 1)
+    np.testing.assert_array_equal(res.ids, ids)
+    _close(res.distances, dist)
+
+
+def test_exact_scan_near_ties_escalate(lsh, orc):
+    # 200 points within 1e-7 of each other around the 32nd neighbour: the
+    # FP32 margin cannot separate them, so the FP64 recheck has to decide.
+    rng = np.random.default_rng(11)
+    d = 64
+    q = _unit(rng, 1, d)[0]
+    far = _unit(rng, 5000, d)
+    near = q + 0.05 * _unit(rng, 31, d)
+    p = q + 0.2 * _unit(rng, 1, d)[0]
+    tied = np.stack([p + 1e-7 * rng.standard_normal(d) for _ in range(200)])
+    data = np.concatenate([far, near, tied]).astype(np.float32)
+    data = data[rng.permutation(len(data))]
+    res = lsh.linearScans(2, 64, d, 1, len(data), data, q[None].astype(np.float32), 0, 1)
+    ids, dist = orc.exact_knn(data, q[None].astype(np.float32), 64, 1)
+    np.testing.assert_array_equal(res.ids, ids)
+    _close(res.distances, dist)
+
+
+# ------------------------------------------------------------ sketch stages
+
+@pytest.mark.parametrize("words,bits", [(32, 32), (3, 32), (12, 10)])
+def test_simhash_bit_exact(lsh, orc, words, bits):
+    rng = np.random.default_rng(words * 100 + bits)
+    x = rng.standard_normal((3001, 100)).astype(np.float32)
+    planes = orc.normal_stream(words * bits * 100)
+    np.testing.assert_array_equal(lsh.simhash_sketch(x, words, bits), orc.simhash(x, planes, words, bits))
+
+
+def test_jl_bit_exact(lsh, orc):
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((1000, 50)).astype(np.float32)
+    planes = orc.normal_stream(40 * 50, 1 / 40)
+    np.testing.assert_array_equal(lsh.jl_sketch(x, 40), orc.jl(x, planes, 40))
+
+
+def test_gj_transform_bit_exact(lsh, orc):
+    rng = np.random.default_rng(4)
+    data = (3 * rng.standard_normal((2000, 33))).astype(np.float32)
+    queries = (3 * rng.standard_normal((17, 33))).astype(np.float32)
+    gd, gq = lsh.gj_transform(data, queries)
+    od, oq = orc.gj_transform(data, queries)
+    np.testing.assert_array_equal(gd, od)
+    np.testing.assert_array_equal(gq, oq)
+
+
+@pytest.mark.parametrize("d,tables", [(100, 8), (128, 3), (17, 5)])
+def test_crosspoly_keys_bit_exact(lsh, orc, d, tables):
+    rng = np.random.default_rng(d + tables)
+    x = rng.standard_normal((2000, d)).astype(np.float32)
+    P = 1 << int(np.ceil(np.log2(d)))
+    signs = orc.sign_stream(tables * 3 * P)
+    keys = lsh.crosspoly_keys(x, tables)
+    np.testing.assert_array_equal(keys, orc.crosspoly(orc.normalize_rows(x), tables, signs))
+
+
+@pytest.mark.parametrize("C", [1, 100, 1024])
+def test_hamming_topc_bit_exact(lsh, orc, C):
+    rng = np.random.default_rng(C)
+    ds = (rng.integers(0, 2 ** 32, size=(20000, 5), dtype=np.uint64) & 0x0F0F0F0F).astype(np.uint32)
+    qs = (rng.integers(0, 2 ** 32, size=(67, 5), dtype=np.uint64) & 0x0F0F0F0F).astype(np.uint32)
+    ids, dist = lsh.hamming_topc(ds, qs, C)
+    oids, odist = orc.sketch_topc(0, ds, qs, C)
+    np.testing.assert_array_equal(ids, oids)
+    np.testing.assert_array_equal(dist, odist)
+
+
+# ------------------------------------------------------------ full pipelines
+
+def test_simhash_pipeline_matches_oracle(lsh, orc):
+    rng = np.random.default_rng(21)
+    N, Q, d, words, C, k = 20000, 40, 64, 8, 256, 32
+    data = _unit(rng, N, d)
+    queries = _unit(rng, Q, d)
+    res = lsh.linearScans(3, k, d, Q, N, data, queries, words, 1, candidates=C)
+    planes = orc.normal_stream(words * 32 * d)
+    cand, _ = orc.sketch_topc(0, orc.simhash(data, planes, words), orc.simhash(queries, planes, words), C)
+    ids, dist = orc.rerank(data, queries, cand, k, 1)
+    np.testing.assert_array_equal(res.ids, ids)
+    _close(res.distances, dist)
+
+
+def test_jl_pipeline_matches_oracle(lsh, orc):
+    rng = np.random.default_rng(22)
+    N, Q, d, sdim, C, k = 8000, 20, 48, 24, 128, 32
+    data = rng.standard_normal((N, d)).astype(np.float32)
+    queries = rng.standard_normal((Q, d)).astype(np.float32)
+    res = lsh.linearScans(6, k, d, Q, N, data, queries, sdim, 1, candidates=C)
+    nd, nq = orc.normalize_rows(data), orc.normalize_rows(queries)
+    planes = orc.normal_stream(sdim * d, 1 / sdim)
+    cand = orc.jl_topc(orc.jl(nd, planes, sdim), orc.jl(nq, planes, sdim), C)
+    ids, dist = orc.rerank(nd, nq, cand, k, 1)
+    np.testing.assert_array_equal(res.ids, ids)
+    _close(res.distances, dist)
+
+
+def test_index_search_repeatable_and_batch_independent(lsh):
+    rng = np.random.default_rng(23)
+    data = _unit(rng, 30000, 100)
+    queries = _unit(rng, 300, 100)
+    with lsh.Index(data, implementation=3, sketchDim=16, candidates=512) as ix:
+        a = ix.search(queries, 32)
+        b = ix.search(queries, 32)
+        c = ix.search(queries[:7], 32)
+        one = ix.search(queries[5:6], 32)
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(a[:7], c)
+    np.testing.assert_array_equal(a[5:6], one)
+
+
+def test_index_device_pointers(lsh):
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(24)
+    data = _unit(rng, 10000, 64)
+    queries = _unit(rng, 50, 64)
+    host = lsh.Index(data, implementation=2, distanceFunc=1).search(queries, 32)
+    dd = torch.from_numpy(data).cuda()
+    dq = torch.from_numpy(queries).cuda()
+    out = torch.empty((50, 32 * 2), dtype=torch.int32, device="cuda")
+    with lsh.Index(dd, implementation=2, distanceFunc=1) as ix:
+        ix.search(dq, 32, out=out)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().view(lsh.POINT_DTYPE).reshape(50, 32)
+    np.testing.assert_array_equal(got, host)

@@ -1,0 +1,353 @@
+"""Pins of the CPU oracle (runs without a GPU).
+
+Each oracle function is checked against something that is not itself: the
+reference's own code compiled into oracle/_ref/ and the fixtures it wrote
+(tests/golden/), an independent restatement of a published algorithm, or
+brute force / closed forms in numpy.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+
+# ----------------------------------------------------------------- calculateK
+
+@pytest.mark.parametrize("k,expect", [(1, 32), (31, 32), (32, 32), (33, 64), (64, 64), (100, 128), (1024, 1024),
+                                      (1025, 1056)])
+def test_calculate_k_closed_form(orc, k, expect):
+    # launchHelper.cuh:21-25: smallest multiple of 32 that is >= k
+    assert orc.calculate_k(k) == expect == int(math.ceil(k / 32) * 32)
+
+
+# ------------------------------------------------------------- random streams
+# Independent restatement of libstdc++'s minstd_rand0 + generate_canonical +
+# Marsaglia polar normal_distribution and the downscaling
+# uniform_int_distribution, in plain Python.
+
+class _MinStd0:
+    def __init__(self):
+        self.x = 1
+
+    def __call__(self):
+        self.x = (16807 * self.x) % 2147483647
+        return self.x
+
+
+def _canonical(g):
+    r = 2147483646.0
+    s = float(g() - 1)
+    s = s + float(g() - 1) * r
+    t = r * r
+    v = s / t
+    return v if v < 1.0 else math.nextafter(1.0, 0.0)
+
+
+def _normals(n, stddev=1.0):
+    g, out, saved = _MinStd0(), [], None
+    while len(out) < n:
+        if saved is not None:
+            v, saved = saved, None
+        else:
+            while True:
+                x = 2.0 * _canonical(g) - 1.0
+                y = 2.0 * _canonical(g) - 1.0
+                r2 = x * x + y * y
+                if not (r2 > 1.0 or r2 == 0.0):
+                    break
+            mult = math.sqrt(-2 * math.log(r2) / r2)
+            saved = x * mult
+            v = y * mult
+        out.append(np.float32(v * stddev + 0.0))
+    return np.array(out, dtype=np.float32)
+
+
+def _uniform_int(n, hi):
+    g, out = _MinStd0(), []
+    uerange = hi + 1
+    scaling = 2147483645 // uerange
+    past = uerange * scaling
+    while len(out) < n:
+        u = g() - 1
+        if u >= past:
+            continue
+        out.append(u // scaling)
+    return np.array(out)
+
+
+def test_canonical_tmp_is_the_long_double_product():
+    # __tmp *= __r is computed in long double and rounded to double once
+    assert 2147483646.0 * 2147483646.0 == float(2147483646 ** 2)
+
+
+def test_normal_stream_restated(orc):
+    np.testing.assert_array_equal(orc.normal_stream(2000), _normals(2000))
+    np.testing.assert_array_equal(orc.normal_stream(300, 1 / 32), _normals(300, 1 / 32))
+
+
+def test_sign_and_bit_streams_restated(orc):
+    s = _uniform_int(1000, 2)
+    np.testing.assert_array_equal(orc.sign_stream(1000), np.where(s == 0, -1, 1))
+    np.testing.assert_array_equal(orc.bit_stream(1000), _uniform_int(1000, 1))
+
+
+def test_streams_match_reference_fixture(orc):
+    fx = json.load(open(os.path.join(GOLDEN, "rng_streams.json")))
+    np.testing.assert_array_equal(orc.normal_stream(64), np.array(fx["normal_stddev1_first64"], np.float32))
+    np.testing.assert_array_equal(orc.normal_stream(64, 1 / 32),
+                                  np.array(fx["normal_stddev_1_over_32_first64"], np.float32))
+    np.testing.assert_array_equal(orc.sign_stream(64), np.array(fx["signs_first64"]))
+
+
+def test_streams_match_reference_build(orc):
+    if not orc.ref_available():
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    np.testing.assert_array_equal(orc.normal_stream(5000), orc.ref_random_vectors(5000))
+    np.testing.assert_array_equal(orc.sign_stream(5000), orc.ref_random_signs(5000))
+
+
+# --------------------------------------------------------------- exact k-NN
+
+def _brute(data, queries, k, func):
+    d64 = data.astype(np.float64)
+    q64 = queries.astype(np.float64)
+    if func in (0, 1):
+        c = (q64 @ d64.T) / (np.linalg.norm(q64, axis=1)[:, None] * np.linalg.norm(d64, axis=1)[None, :])
+        D = -c if func == 0 else 1 - c
+    elif func == 2:
+        mn = np.minimum(q64[:, None, :], d64[None, :, :]).sum(-1)
+        mx = np.maximum(q64[:, None, :], d64[None, :, :]).sum(-1)
+        D = 1 - mn / mx
+    else:
+        D = np.sqrt(((q64[:, None, :] - d64[None, :, :]) ** 2).sum(-1))
+    ids = np.stack([np.lexsort((np.arange(len(data)), row))[:k] for row in D])
+    return ids, np.take_along_axis(D, ids, 1)
+
+
+@pytest.mark.parametrize("func", [0, 1, 2, 3])
+def test_exact_knn_bruteforce(orc, func):
+    rng = np.random.default_rng(func)
+    data = rng.standard_normal((700, 9)).astype(np.float32)
+    queries = rng.standard_normal((11, 9)).astype(np.float32)
+    if func == 2:
+        data, queries = np.abs(data), np.abs(queries)
+    ids, dist = orc.exact_knn(data, queries, 32, func)
+    bids, bd = _brute(data, queries, 32, func)
+    np.testing.assert_array_equal(ids, bids)
+    np.testing.assert_allclose(dist, bd, rtol=1e-6, atol=1e-7)
+
+
+def test_exact_knn_index_tie_break_and_padding(orc):
+    data = np.array([[1, 0], [1, 0], [0, 1], [1, 0]], dtype=np.float32)
+    queries = np.array([[1, 0]], dtype=np.float32)
+    ids, dist = orc.exact_knn(data, queries, 32, 1)
+    assert list(ids[0, :4]) == [0, 1, 3, 2]
+    assert (ids[0, 4:] == -1).all() and (dist[0, 4:] == np.float32(2147483648.0)).all()
+    ids0, dist0 = orc.exact_knn(data, queries, 32, 0)
+    assert (dist0[0, 4:] == 2.0).all()
+
+
+def test_exact_knn_matches_reference_goldstandard(orc):
+    fx = np.load(os.path.join(GOLDEN, "goldstandard_small.npz"))
+    K = int(fx["K"])
+    ids, dist = orc.exact_knn(fx["data"], fx["queries"], K, 0)
+    np.testing.assert_array_equal(ids, fx["truth_ids"])
+    # the truth file prints -cos with 6 significant digits
+    np.testing.assert_allclose(dist, fx["truth_dist"], rtol=2e-6, atol=2e-6)
+
+
+def test_goldstandard_port_matches_reference(orc):
+    fx = np.load(os.path.join(GOLDEN, "goldstandard_small.npz"))
+    K = int(fx["K"])
+    _, ids = orc.goldstandard_port(fx["data"], fx["queries"], K, threads=2)
+    np.testing.assert_array_equal(ids, fx["truth_ids"])
+
+
+# ------------------------------------------------------------- transforms
+
+def test_gj_transform_properties(orc):
+    rng = np.random.default_rng(3)
+    data = (rng.standard_normal((200, 7)) * 3).astype(np.float32)
+    queries = (rng.standard_normal((5, 7)) * 3).astype(np.float32)
+    td, tq = orc.gj_transform(data, queries)
+    mins = np.minimum(0, np.floor(np.minimum(data.min(0), queries.min(0)))).astype(np.int64)
+    shifted = (data + np.abs(mins).astype(np.float32)).astype(np.float64)
+    assert (td >= 0).all() and (tq >= 0).all()
+    np.testing.assert_allclose(np.linalg.norm(td.astype(np.float64), axis=1), 1.0, atol=3e-7)
+    np.testing.assert_allclose(td, shifted / np.linalg.norm(shifted, axis=1)[:, None], rtol=4e-7, atol=1e-8)
+
+
+# ------------------------------------------------------------------ SimHash
+
+def test_simhash_bits_and_packing(orc):
+    rng = np.random.default_rng(11)
+    d, words = 20, 3
+    x = rng.standard_normal((64, d)).astype(np.float32)
+    planes = rng.standard_normal((words * 32, d)).astype(np.float32)
+    bits = orc.simhash(x, planes, words)
+    dots = x.astype(np.float64) @ planes.astype(np.float64).T  # [n][w*32+b]
+    sure = np.abs(dots) > 1e-4 * np.linalg.norm(x, axis=1)[:, None] * np.linalg.norm(planes, axis=1)[None, :]
+    unpacked = np.stack([(bits[:, w][:, None] >> np.arange(32)) & 1 for w in range(words)], 1).reshape(64, -1)
+    assert sure.mean() > 0.99
+    np.testing.assert_array_equal(unpacked[sure], (dots[sure] >= 0).astype(np.uint32))
+
+
+def test_simhash_generic_bits_per_word(orc):
+    # simHash.cuh:20-46 with hashBits = 10: word t holds hyperplanes t*10 .. t*10+9
+    rng = np.random.default_rng(12)
+    x = rng.standard_normal((50, 8)).astype(np.float32)
+    planes = rng.standard_normal((4 * 10, 8)).astype(np.float32)
+    keys = orc.simhash(x, planes, 4, 10)
+    full = orc.simhash(x, np.concatenate([planes, np.zeros((88, 8), np.float32)]), 4, 32)
+    assert (keys < 1024).all()
+    flat = np.stack([(keys[:, t][:, None] >> np.arange(10)) & 1 for t in range(4)], 1).reshape(50, -1)
+    ref = (x.astype(np.float64) @ planes.T.astype(np.float64) >= 0).astype(np.uint32)
+    sure = np.abs(x.astype(np.float64) @ planes.T.astype(np.float64)) > 1e-4
+    np.testing.assert_array_equal(flat[sure], ref[sure])
+    assert full.shape == (50, 4)
+
+
+def _popcount(a):
+    a = a.astype(np.uint32)
+    return np.unpackbits(a.view(np.uint8).reshape(a.shape + (4,)), axis=-1).sum(-1).sum(-1)
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_sketch_topc_bruteforce(orc, kind):
+    rng = np.random.default_rng(5 + kind)
+    hi = 2 ** 32 if kind == 0 else 3  # few distinct values -> many ties
+    ds = rng.integers(0, hi, size=(3000, 6), dtype=np.uint64).astype(np.uint32)
+    qs = rng.integers(0, hi, size=(7, 6), dtype=np.uint64).astype(np.uint32)
+    ids, dist = orc.sketch_topc(kind, ds, qs, 100)
+    for q in range(7):
+        if kind == 0:
+            h = _popcount(ds ^ qs[q])
+        else:
+            h = (ds != qs[q]).sum(1)
+        order = np.lexsort((np.arange(3000), h))[:100]
+        np.testing.assert_array_equal(ids[q], order)
+        np.testing.assert_array_equal(dist[q], h[order])
+
+
+def test_rerank_bruteforce(orc):
+    rng = np.random.default_rng(9)
+    data = rng.standard_normal((400, 10)).astype(np.float32)
+    queries = rng.standard_normal((6, 10)).astype(np.float32)
+    cand = np.stack([rng.permutation(400)[:90] for _ in range(6)]).astype(np.int32)
+    cand[0, 10:20] = -1
+    ids, dist = orc.rerank(data, queries, cand, 32, 1)
+    for q in range(6):
+        c = cand[q][cand[q] >= 0]
+        bids, bd = _brute(data[c], queries[q:q + 1], 32, 1)
+        np.testing.assert_array_equal(ids[q], c[bids[0]])
+
+
+# --------------------------------------------------------------- validation
+
+def test_recall_and_ratio_match_reference_fixture(orc):
+    fx = json.load(open(os.path.join(GOLDEN, "validation_small.json")))
+    t = np.array(fx["truth_ids"], np.int32)
+    r = np.array(fx["result_ids"], np.int32)
+    rd = np.array(fx["result_dist"], np.float32)
+    td = np.array(fx["truth_dist"], np.float32)
+    assert orc.recall(t, r, fx["report_k"]) == pytest.approx(fx["recall"], abs=1e-6)
+    assert orc.distance_ratio(td, r, rd, fx["report_k"]) == pytest.approx(fx["distance_ratio"], rel=1e-5)
+
+
+def test_recall_closed_form(orc):
+    truth = np.arange(64, dtype=np.int32).reshape(2, 32)
+    res = truth.copy()
+    res[1, :8] = 1000 + np.arange(8)
+    assert orc.recall(truth, res, 32) == pytest.approx((1.0 + 24 / 32) / 2)
+
+
+# ---------------------------------------------------------- cross-polytope
+
+def test_crosspoly_matches_hadamard(orc):
+    from scipy.linalg import hadamard
+
+    rng = np.random.default_rng(21)
+    d, tables, P = 12, 5, 16
+    x = orc.normalize_rows(rng.standard_normal((300, d)).astype(np.float32))
+    signs = np.where(rng.random(tables * 3 * P) < 1 / 3, -1, 1).astype(np.int32)
+    keys = orc.crosspoly(x, tables, signs)
+    H = hadamard(P).astype(np.float64)
+    ok = 0
+    for t in range(tables):
+        v = np.zeros((300, P))
+        v[:, :d] = x
+        for r in range(3):
+            v = (v * signs[(t * 3 + r) * P:(t * 3 + r + 1) * P]) @ H.T
+        a = np.abs(v)
+        top = np.argsort(-a, axis=1, kind="stable")
+        gap = a[np.arange(300), top[:, 0]] - a[np.arange(300), top[:, 1]]
+        i = top[:, 0]
+        expect = np.where(v[np.arange(300), i] < 0, i, i + P)
+        sure = gap > 1e-4
+        np.testing.assert_array_equal(keys[sure, t], expect[sure])
+        ok += sure.sum()
+    assert ok > 0.98 * 300 * tables
+
+
+def test_hadamard_involution():
+    from scipy.linalg import hadamard
+
+    H = hadamard(128)
+    np.testing.assert_array_equal(H @ H, 128 * np.eye(128, dtype=H.dtype))
+
+
+# -------------------------------------------------------- weighted MinHash
+
+@pytest.mark.parametrize("ctr,key,expect", [
+    ((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+    ((0xffffffff,) * 4, (0xffffffff,) * 2, (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+    ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+     (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
+])
+def test_philox_known_answers(orc, ctr, key, expect):
+    # Random123 known-answer vectors for Philox4x32-10
+    np.testing.assert_array_equal(orc.philox4x32_10(ctr, key), np.array(expect, np.uint32))
+
+
+def test_wmh_collision_probability_tracks_generalized_jaccard(orc):
+    rng = np.random.default_rng(31)
+    d, H = 16, 2048
+    base = rng.standard_exponential(d).astype(np.float32)
+    pairs = []
+    for s in (0.05, 0.3, 0.8):
+        other = np.abs(base + s * rng.standard_normal(d)).astype(np.float32)
+        pairs.append(other)
+    x = orc.normalize_rows(np.stack([base] + pairs))
+    bounds, start, M = orc.wmh_bounds(x)
+    assert M == int(np.ceil(x.max(0)).clip(min=0).sum())
+    sk = orc.wmh_sketch(x, bounds, start, M, H, 0, np.zeros(1, np.uint8))
+    for i in range(1, 4):
+        gj = np.minimum(x[0], x[i]).sum() / np.maximum(x[0], x[i]).sum()
+        est = (sk[0] == sk[i]).mean()
+        assert abs(est - gj) < 4 * math.sqrt(gj * (1 - gj) / H) + 0.01
+
+
+def test_wmh_counts_are_geometric(orc):
+    rng = np.random.default_rng(32)
+    x = orc.normalize_rows(rng.standard_exponential((40, 12)).astype(np.float32))
+    bounds, start, M = orc.wmh_bounds(x)
+    sk = orc.wmh_sketch(x, bounds, start, M, 400, 0, np.zeros(1, np.uint8))
+    # P(green) = sum(x)/M, so the mean number of reds is M/sum(x) - 1
+    expect = M / x.sum(1) - 1
+    np.testing.assert_allclose(sk.mean(1), expect, rtol=0.25)
+
+
+# ---------------------------------------------------------------------- LSH
+
+def test_lsh_candidates_bruteforce(orc):
+    rng = np.random.default_rng(41)
+    dk = rng.integers(0, 4, size=(500, 3)).astype(np.uint32)
+    qk = rng.integers(0, 4, size=(2, 3)).astype(np.uint32)
+    for q in range(2):
+        c = orc.lsh_candidates(dk, qk, q)
+        expect = np.nonzero((dk == qk[q]).any(1))[0]
+        np.testing.assert_array_equal(c, expect)
