@@ -140,6 +140,12 @@ def load() -> C.CDLL:
         return _lib
     if not os.path.exists(_LIB_PATH):
         raise LshError(-1, f"{_LIB_PATH} is missing: build it with `python -m lshongpu_amd.build`")
+    # torch ships its own HIP runtime under the same soname; only one can be
+    # loaded per process, and torch only works with its own, so it loads first.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(_LIB_PATH)
     for name, (res, args) in _SIGS.items():
         fn = getattr(lib, name)

@@ -13,6 +13,12 @@ void check_launch(const char* what) {
     if (e != hipSuccess) fail(LSH_ERR_DEVICE, std::string("launch of ") + what + " failed: " + hipGetErrorString(e));
 }
 
+void set_dynamic_lds(const void* kernel, size_t bytes) {
+    if (bytes <= 64 * 1024) return;
+    LSH_REQUIRE(bytes <= 160 * 1024, LSH_ERR_UNSUPPORTED, "workgroup LDS request above 160 KiB");
+    LSH_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
 // randomVectorGenerator.cpp:32-53 and :56-74: a default-seeded
 // std::default_random_engine feeding std::normal_distribution<double>(0,
 // stddev), every draw cast to float.  Calling the same library functions as

@@ -161,8 +161,9 @@ void lsh_index::build(const float* src, bool on_device, const float* extra_host,
     DeviceGuard g(device);
     hipStream_t s = stream;
     data.alloc((size_t)N * d);
-    LSH_HIP(hipMemcpyAsync(data.get(), src, sizeof(float) * (size_t)N * d,
-                           on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    if (N > 0)
+        LSH_HIP(hipMemcpyAsync(data.get(), src, sizeof(float) * (size_t)N * d,
+                               on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
     DevBuf<float> extra;
     if (extra_host && n_extra > 0) {
         extra.alloc((size_t)n_extra * d);
@@ -207,7 +208,7 @@ void lsh_index::build(const float* src, bool on_device, const float* extra_host,
         planesT.alloc(pt.size());
         LSH_HIP(hipMemcpyAsync(planesT.get(), pt.data(), sizeof(float) * pt.size(), hipMemcpyHostToDevice, s));
         dsk.alloc((size_t)N * wpad);
-        LSH_HIP(hipMemsetAsync(dsk.get(), 0, dsk.bytes(), s));
+        if (N > 0) LSH_HIP(hipMemsetAsync(dsk.get(), 0, dsk.bytes(), s));
         launch_simhash(data.get(), N, d, planesT.get(), words, 32, dsk.get(), wpad, s);
         LSH_HIP(hipStreamSynchronize(s));
     } else if (framework == 0 && impl == 6) {

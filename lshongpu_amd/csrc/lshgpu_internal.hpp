@@ -40,6 +40,10 @@ struct Error : std::runtime_error {
 // surface at the next synchronisation).
 void check_launch(const char* what);
 
+// Raises the dynamic LDS limit of `kernel` to `bytes` when it exceeds the
+// default 64 KiB (CDNA4 workgroups may use up to 160 KiB).
+void set_dynamic_lds(const void* kernel, size_t bytes);
+
 // ------------------------------------------------------------ device memory
 
 template <class T>

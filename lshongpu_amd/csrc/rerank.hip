@@ -15,6 +15,21 @@ namespace lshgpu {
 
 namespace {
 
+// Partial sums of one lane: (dot, |x|^2) for the angular kinds,
+// (sum min, sum max) for generalized Jaccard, (sum of squares, -) for L2.
+__device__ inline void group_accumulate(int kind, float x, float y, float& a, float& b) {
+    if (kind == kNegCos || kind == kAngular) {
+        a = __builtin_fmaf(x, y, a);
+        b = __builtin_fmaf(x, x, b);
+    } else if (kind == kGenJaccard) {
+        a += fminf(x, y);
+        b += fmaxf(x, y);
+    } else {
+        const float t = y - x;
+        a = __builtin_fmaf(t, t, a);
+    }
+}
+
 __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ data,
                                                      const float* __restrict__ queries, int d,
                                                      int kind, const uint64_t* __restrict__ keys,
@@ -34,7 +49,12 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
         for (int j = 0; j < d; ++j) qn = __builtin_fmaf(qv[j], qv[j], qn);
         qn = __builtin_sqrtf(qn);
     }
-    for (int t = threadIdx.x; t < Cp; t += blockDim.x) {
+    // FP32 distances: a group of 16 lanes reads one candidate row with
+    // coalesced float4 loads and reduces its partial sums with shuffles.
+    const int gl = threadIdx.x & 15;
+    const int ngroups = blockDim.x >> 4;
+    const bool vec4 = (d & 3) == 0;
+    for (int t = threadIdx.x >> 4; t < Cp; t += ngroups) {
         int32_t id = -1;
         if (t < C) {
             if (cand) {
@@ -44,9 +64,39 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
                 id = key == kEmptyKey ? -1 : (int32_t)(uint32_t)key;
             }
         }
-        k32[t] = id < 0 ? kEmptyKey
-                        : make_key(f32_key(dist32_dev(kind, qv, data + (size_t)id * d, d, qn)),
-                                   (uint32_t)id);
+        float a = 0.0f, b = 0.0f;
+        if (id >= 0) {
+            const float* __restrict__ x = data + (size_t)id * d;
+            if (vec4) {
+                for (int j = gl * 4; j < d; j += 64) {
+                    const float4 xv = *reinterpret_cast<const float4*>(x + j);
+                    const float4 yv = *reinterpret_cast<const float4*>(qv + j);
+                    group_accumulate(kind, xv.x, yv.x, a, b);
+                    group_accumulate(kind, xv.y, yv.y, a, b);
+                    group_accumulate(kind, xv.z, yv.z, a, b);
+                    group_accumulate(kind, xv.w, yv.w, a, b);
+                }
+            } else {
+                for (int j = gl; j < d; j += 16) group_accumulate(kind, x[j], qv[j], a, b);
+            }
+        }
+#pragma unroll
+        for (int o = 8; o > 0; o >>= 1) {
+            a += __shfl_xor(a, o, 16);
+            b += __shfl_xor(b, o, 16);
+        }
+        if (gl == 0) {
+            float dist;
+            if (kind == kNegCos || kind == kAngular) {
+                const float c = a / (qn * __builtin_sqrtf(b));
+                dist = kind == kNegCos ? -c : 1.0f - c;
+            } else if (kind == kGenJaccard) {
+                dist = 1.0f - a / b;
+            } else {
+                dist = a;  // squared; kEuclid ranks by the square
+            }
+            k32[t] = id < 0 ? kEmptyKey : make_key(f32_key(dist), (uint32_t)id);
+        }
     }
     bitonic_sort_lds(k32, Cp);
 
@@ -119,12 +169,7 @@ void launch_rerank(const float* data, const float* queries, int d, int kind, con
     LSH_REQUIRE(Cp <= kMaxTopK, LSH_ERR_INVALID, "candidate set larger than 4096");
     const int kp = pow2_ceil(std::max(kprime, k));
     const size_t lds = (size_t)Cp * (2 * sizeof(uint64_t) + sizeof(int32_t));
-    static bool attr_set = false;
-    if (!attr_set) {
-        (void)hipFuncSetAttribute((const void*)rerank_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  160 * 1024);
-        attr_set = true;
-    }
+    set_dynamic_lds((const void*)rerank_kernel, lds);
     hipLaunchKernelGGL(rerank_kernel, dim3(Q), dim3(256), lds, s, data, queries, d, kind, keys, key_stride,
                        cand, C, Cp, k, kp, eps, out);
     check_launch("rerank_kernel");

@@ -242,12 +242,7 @@ void launch_exact_finalize(const float* data, const float* queries, int d, int k
                            const TopKView& v, int k, double eps, lsh_point_t* out,
                            uint32_t* flags, hipStream_t s) {
     const size_t lds = (size_t)v.K * (sizeof(uint64_t) + sizeof(int32_t));
-    static bool attr_set = false;
-    if (!attr_set) {
-        (void)hipFuncSetAttribute((const void*)exact_finalize_kernel,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attr_set = true;
-    }
+    set_dynamic_lds((const void*)exact_finalize_kernel, lds);
     hipLaunchKernelGGL(exact_finalize_kernel, dim3(v.Q), dim3(256), lds, s, data, queries, d, kind, v,
                        k, eps, out, flags);
     check_launch("exact_finalize_kernel");

@@ -106,12 +106,7 @@ void TopKState::reset(int Q, int K, int newcap, hipStream_t s) {
 
 void launch_select_merge(const TopKView& v, hipStream_t s) {
     const size_t lds = (size_t)(v.K + pow2_ceil(std::max(v.newcap, v.K))) * sizeof(uint64_t);
-    static bool attr_set = false;
-    if (!attr_set) {
-        (void)hipFuncSetAttribute((const void*)select_merge_kernel,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attr_set = true;
-    }
+    set_dynamic_lds((const void*)select_merge_kernel, lds);
     hipLaunchKernelGGL(select_merge_kernel, dim3(v.Q), dim3(kSelThreads), lds, s, v);
     check_launch("select_merge_kernel");
 }

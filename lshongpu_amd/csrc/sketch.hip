@@ -19,30 +19,31 @@ namespace {
 
 constexpr int TPB = 256;
 
+// One lane per (point, word); blockIdx.y is the word, so the hyperplane
+// values a wave reads are uniform across it.
 __global__ __launch_bounds__(TPB) void simhash_kernel(const float* __restrict__ x, int64_t n, int d,
                                                       const float* __restrict__ pT, int words, int hb,
                                                       uint32_t* __restrict__ out, int ostride) {
     const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    const int w = blockIdx.y;
     if (i >= n) return;
     const float* __restrict__ row = x + (size_t)i * d;
-    for (int w = 0; w < words; ++w) {
-        float acc[32];
+    float acc[32];
 #pragma unroll
-        for (int b = 0; b < 32; ++b) acc[b] = 0.0f;
-        const float* __restrict__ p = pT + (size_t)w * d * hb;
-        for (int j = 0; j < d; ++j) {
-            const float xj = row[j];
-            const float* __restrict__ pj = p + (size_t)j * hb;
-#pragma unroll
-            for (int b = 0; b < 32; ++b)
-                if (b < hb) acc[b] = __builtin_fmaf(xj, pj[b], acc[b]);
-        }
-        uint32_t s = 0;
+    for (int b = 0; b < 32; ++b) acc[b] = 0.0f;
+    const float* __restrict__ p = pT + (size_t)w * d * hb;
+    for (int j = 0; j < d; ++j) {
+        const float xj = row[j];
+        const float* __restrict__ pj = p + (size_t)j * hb;
 #pragma unroll
         for (int b = 0; b < 32; ++b)
-            if (b < hb) s |= (acc[b] >= 0.0f ? 1u : 0u) << b;
-        out[(size_t)i * ostride + w] = s;
+            if (b < hb) acc[b] = __builtin_fmaf(xj, pj[b], acc[b]);
     }
+    uint32_t s = 0;
+#pragma unroll
+    for (int b = 0; b < 32; ++b)
+        if (b < hb) s |= (acc[b] >= 0.0f ? 1u : 0u) << b;
+    out[(size_t)i * ostride + w] = s;
 }
 
 // pT is [groups of 32 sketch values][d][32].
@@ -155,7 +156,8 @@ void launch_simhash(const float* x, int64_t n, int d, const float* planesT, int 
                     uint32_t* out, int out_stride, hipStream_t s) {
     if (n <= 0) return;
     LSH_REQUIRE(hashBits >= 1 && hashBits <= 32, LSH_ERR_INVALID, "hashBits must be in 1..32");
-    hipLaunchKernelGGL(simhash_kernel, dim3(blocks_for(n, TPB)), dim3(TPB), 0, s, x, n, d, planesT,
+    LSH_REQUIRE(words <= 65535, LSH_ERR_UNSUPPORTED, "at most 65535 sketch words");
+    hipLaunchKernelGGL(simhash_kernel, dim3(blocks_for(n, TPB), words), dim3(TPB), 0, s, x, n, d, planesT,
                        words, hashBits, out, out_stride);
     check_launch("simhash_kernel");
 }
