@@ -28,7 +28,7 @@ sys.path.insert(0, ROOT)
 
 # MI355X published peaks.
 HBM_PEAK_GBS = 8000.0
-VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # 32-bit lane-ops: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz
+MFMA_FP4_PEAK_TFLOPS = 10100.0  # dense FP4 matrix peak
 
 
 def cpu_model() -> str:
@@ -94,7 +94,7 @@ def main() -> int:
         step()
     ix.set_profiling(True)
     phases = {"prep_ms": 0.0, "scan_ms": 0.0, "select_ms": 0.0, "final_ms": 0.0, "total_ms": 0.0,
-              "scan_launches": 0}
+              "mfma_scan_ms": 0.0, "mfma_launches": 0, "mfma_points": 0}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -109,18 +109,20 @@ def main() -> int:
     qps = Q * args.steps / elapsed
     res = out.cpu().numpy().view(lsh.POINT_DTYPE).reshape(Q, kq)
 
-    # ---- dominant kernel: the Hamming scan (all scan rounds of one step)
-    scan_ms = phases["scan_ms"] / args.steps
-    launches = phases["scan_launches"] / args.steps
-    wpad = (args.words + 7) // 8 * 8
-    # Algorithmic bytes of one step's scan rounds: every data sketch once and
-    # the query sketches once per round.
-    alg_bytes_step = args.n * args.words * 4 + launches * Q * args.words * 4
-    alg_bytes_launch = alg_bytes_step / launches
-    avg_launch_ms = scan_ms / launches
-    achieved_gbs = alg_bytes_launch / (avg_launch_ms * 1e-3) / 1e9
-    lane_ops = 2.0 * args.words * Q * args.n  # xor + bit-count-accumulate per word per pair
-    valu_tops = lane_ops / (scan_ms * 1e-3) / 1e12
+    # ---- dominant kernel: the matrix-core Hamming scan (rounds past the first few)
+    steps = args.steps
+    mfma_ms = phases["mfma_scan_ms"] / steps
+    mfma_launches = phases["mfma_launches"] / steps
+    mfma_points = phases["mfma_points"] / steps
+    bits = 32 * ((args.words + 7) // 8 * 8)
+    # Algorithmic work: one FP4 multiply-add per sketch bit and (query, point) pair.
+    flop_step = 2.0 * bits * Q * mfma_points
+    flop_launch = flop_step / max(mfma_launches, 1)
+    avg_launch_ms = mfma_ms / max(mfma_launches, 1)
+    achieved_tflops = flop_launch / (avg_launch_ms * 1e-3) / 1e12 if avg_launch_ms > 0 else 0.0
+    # Algorithmic HBM bytes of those launches: the packed data sketches of the
+    # range once plus the query sketches once per launch.
+    bytes_launch = (mfma_points * args.words * 4 + mfma_launches * Q * args.words * 4) / max(mfma_launches, 1)
 
     # ---- plain FP32 exact scan on the same GPU (and the truth for recall)
     exact = {}
@@ -176,19 +178,20 @@ def main() -> int:
         "index_build_ms": build_ms,
         "phase_ms_per_step": {key: phases[key] / args.steps for key in phases},
         "roofline": {
-            "kernel": "sketch_scan_kernel (Hamming scan rounds)",
-            "bound": "hbm",
-            "achieved": achieved_gbs,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved_gbs / HBM_PEAK_GBS,
+            "kernel": "hamming_mfma_kernel (FP4 +-1 v_mfma_scale_f32_32x32x64_f8f6f4)",
+            "bound": "mfma",
+            "achieved": achieved_tflops,
+            "peak": MFMA_FP4_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved_tflops / MFMA_FP4_PEAK_TFLOPS,
             "traffic": None,
-            "algorithmic_bytes_per_launch": alg_bytes_launch,
+            "algorithmic_flop_per_launch": flop_launch,
             "avg_launch_ms": avg_launch_ms,
-            "launches_per_step": launches,
-            "valu_lane_ops_tops": valu_tops,
-            "valu_peak_tops": VALU_PEAK_TOPS,
-            "valu_frac": valu_tops / VALU_PEAK_TOPS,
+            "launches_per_step": mfma_launches,
+            "points_per_step": mfma_points,
+            "algorithmic_bytes_per_launch": bytes_launch,
+            "hbm_frac_at_algorithmic_bytes": bytes_launch / (avg_launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+            if avg_launch_ms > 0 else 0.0,
         },
         "cpu_baseline": cpu,
         "data_gen_s": gen_s,

@@ -216,11 +216,12 @@ int lsh_index_search(lsh_index_t* index, const float* queries, int N_query,
                      int k, int queries_on_device, lsh_point_t* results,
                      int results_on_device, void* stream);
 
-/* Per-phase device times (ms) of the last search: [0] query sketch,
- * [1] candidate scan (all rounds), [2] candidate select, [3] rerank/finalise,
- * [4] total; launch counts in [5]; average duration (ms) of one launch of the
- * dominant scan kernel in [6] and the bytes it read per launch in [7].
- * Filled only while profiling is on. */
+/* Per-phase device times (ms) of the last search: [0] query preparation and
+ * sketch, [1] candidate scans (all rounds), [2] candidate selection, [3]
+ * rerank / finalisation, [4] total, [5] the matrix-core Hamming scan
+ * launches among [1]; [6] the number of those launches and [7] the data
+ * points they covered (summed over launches).  Filled only while profiling
+ * is on. */
 int lsh_index_set_profiling(lsh_index_t* index, int on);
 int lsh_index_phase_times(const lsh_index_t* index, double* out8);
 

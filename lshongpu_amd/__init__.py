@@ -406,7 +406,7 @@ class Index:
         _check(load().lsh_index_phase_times(self._h, arr))
         v = list(arr)
         return {"prep_ms": v[0], "scan_ms": v[1], "select_ms": v[2], "final_ms": v[3], "total_ms": v[4],
-                "scan_launches": int(v[5]), "scan_avg_ms": v[6], "scan_bytes_per_launch": v[7]}
+                "mfma_scan_ms": v[5], "mfma_launches": int(v[6]), "mfma_points": int(v[7])}
 
     def device_bytes(self) -> int:
         return int(load().lsh_index_device_bytes(self._h))

@@ -24,6 +24,7 @@ SOURCES = [
     "select.hip",
     "scan_exact.hip",
     "scan_sketch.hip",
+    "scan_mfma.hip",
     "rerank.hip",
     "sketch.hip",
     "minhash.hip",

@@ -11,12 +11,13 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 using namespace lshgpu;
 
 namespace {
 
-enum Phase { PH_PREP = 0, PH_SCAN = 1, PH_SELECT = 2, PH_FINAL = 3, PH_TOTAL = 4 };
+enum Phase { PH_PREP = 0, PH_SCAN = 1, PH_SELECT = 2, PH_FINAL = 3, PH_TOTAL = 4, PH_MFMA = 5 };
 
 struct DeviceGuard {
     int prev = 0;
@@ -30,6 +31,16 @@ struct DeviceGuard {
 int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
 bool angular_kind(int kind) { return kind == kNegCos || kind == kAngular; }
+
+// LSHGPU_VALU_HAMMING=1 selects the VALU Hamming scan instead of the matrix
+// cores (both give identical results; used to compare them).
+bool force_valu_scan() {
+    static const bool on = [] {
+        const char* e = std::getenv("LSHGPU_VALU_HAMMING");
+        return e && e[0] == '1';
+    }();
+    return on;
+}
 
 // Indices of the nonzero entries of a device flag array.
 std::vector<int32_t> flagged(const uint32_t* dflags, int Q, hipStream_t s) {
@@ -46,9 +57,9 @@ std::vector<int32_t> flagged(const uint32_t* dflags, int Q, hipStream_t s) {
 // launches the scan kernel of one range.
 template <class ScanFn>
 void run_rounds(TopKState& st, int Q, int K, int64_t N, bool fixed, ScanFn&& scan, hipStream_t s,
-                PhaseTimer& pt, int64_t& launches) {
+                PhaseTimer& pt, int64_t& launches, bool int_keys = false) {
     const int newcap = newcap_for(K);
-    st.reset(Q, K, newcap, s);
+    st.reset(Q, K, newcap, s, int_keys);
     const TopKView v = st.view();
     int64_t p = 0;
     int64_t chunk = std::min<int64_t>(N, newcap);
@@ -101,19 +112,38 @@ void exact_topk(const float* data, const float* data_inv, int64_t N, int d, int 
     LSH_HIP(hipStreamSynchronize(s));
 }
 
-// Integer sketch scan: qs has ceil(Q/64)*64 rows of wpad words.
+// Rows the query sketch buffer of an integer sketch scan must have.
+int sketch_query_rows(int Q) { return round_up(std::max(Q, 1), 256); }
+
+// Integer sketch scan: qs has sketch_query_rows(Q) rows of wpad words (zero
+// past Q).  Hamming scans of up to 1024 bits run on the matrix cores.
 void sketch_topk(const uint32_t* ds, int64_t N, int wpad, int kind, const uint32_t* qs, int Q, int K,
-                 bool fixed, TopKState& st, hipStream_t s, PhaseTimer& pt, int64_t& launches) {
+                 bool fixed, TopKState& st, hipStream_t s, PhaseTimer& pt, int64_t& launches,
+                 int64_t* mfma_launches = nullptr, int64_t* mfma_points = nullptr) {
+    const bool mfma = kind == 0 && hamming_mfma_supported(wpad) && !force_valu_scan();
     auto scan = [&](int64_t p0, int64_t p1, const TopKView& v) {
-        SketchScanArgs a{ds, qs, wpad, Q, p0, p1, kind};
-        launch_sketch_scan(a, v, s);
+        // The matrix-core scan queues candidates in LDS, which suits ranges
+        // where few pairs pass: once 16*K points are behind every threshold,
+        // a 256-query x 32-point tile yields about 256*32*K/p0 <= 512 of them.
+        // The re-run of overflowed queries (fixed) appends straight to global
+        // memory, whose capacity per round it never exceeds.
+        if (mfma && !fixed && p0 >= 16 * (int64_t)K) {
+            pt.begin(PH_MFMA, s);
+            launch_hamming_mfma(ds, qs, wpad, Q, p0, p1, v, s);
+            pt.end(PH_MFMA, s);
+            if (mfma_launches) ++*mfma_launches;
+            if (mfma_points) *mfma_points += p1 - p0;
+        } else {
+            SketchScanArgs a{ds, qs, wpad, Q, p0, p1, kind};
+            launch_sketch_scan(a, v, s);
+        }
     };
-    run_rounds(st, Q, K, N, fixed, scan, s, pt, launches);
+    run_rounds(st, Q, K, N, fixed, scan, s, pt, launches, K <= kMaxIntTopK);
     if (fixed) return;
     const std::vector<int32_t> ovf = flagged(st.view().ovf, Q, s);
     if (ovf.empty()) return;
     const int m = (int)ovf.size();
-    const int mpad = round_up(m, 64);
+    const int mpad = sketch_query_rows(m);
     std::vector<int32_t> hidx(ovf);
     hidx.resize((size_t)mpad, ovf[0]);
     DevBuf<int32_t> idx((size_t)mpad);
@@ -238,6 +268,8 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
     hipStream_t s = user_stream ? user_stream : stream;
     PhaseTimer pt(profiling);
     scan_launches = 0;
+    mfma_launches = 0;
+    mfma_points = 0;
     if (Q == 0) return;
     pt.begin(PH_TOTAL, s);
     pt.begin(PH_PREP, s);
@@ -287,16 +319,17 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
             LSH_HIP(hipStreamSynchronize(s));
         }
     } else if (framework == 0 && (impl == 3 || impl == 5 || impl == 4)) {
-        const int Qpad = round_up(Q, 64);
+        const int Qpad = sketch_query_rows(Q);
         qsk.ensure((size_t)Qpad * wpad);
         pt.begin(PH_PREP, s);
         LSH_HIP(hipMemsetAsync(qsk.get(), 0, sizeof(uint32_t) * (size_t)Qpad * wpad, s));
         if (impl == 3) launch_simhash(q.get(), Q, d, planesT.get(), words, 32, qsk.get(), wpad, s);
         pt.end(PH_PREP, s);
-        const int Ceff = (int)std::min<int64_t>(candidates, N);
-        const int K = pow2_ceil(std::max(Ceff, 1));
+        const int Ceff = (int)std::max<int64_t>(1, std::min<int64_t>(candidates, N));
+        const int K = Ceff <= kMaxIntTopK ? Ceff : pow2_ceil(Ceff);
         LSH_REQUIRE(K <= kMaxTopK, LSH_ERR_UNSUPPORTED, "candidate sets larger than 4096 are not supported");
-        sketch_topk(dsk.get(), N, wpad, impl == 4 ? 1 : 0, qsk.get(), Q, K, false, st, s, pt, scan_launches);
+        sketch_topk(dsk.get(), N, wpad, impl == 4 ? 1 : 0, qsk.get(), Q, K, false, st, s, pt, scan_launches,
+                    &mfma_launches, &mfma_points);
         const int kind = impl == 3 ? kAngular : kGenJaccard;
         pt.begin(PH_FINAL, s);
         launch_rerank(data.get(), q.get(), d, kind, st.view().keys, st.view().stride, nullptr, Ceff, Q, kq,
@@ -324,9 +357,9 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
                            r_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
     pt.end(PH_TOTAL, s);
     LSH_HIP(hipStreamSynchronize(s));
-    pt.collect(phase_ms, 5);
-    phase_ms[5] = (double)scan_launches;
-    phase_ms[6] = scan_launches > 0 ? phase_ms[PH_SCAN] / (double)scan_launches : 0.0;
+    pt.collect(phase_ms, 6);
+    phase_ms[6] = (double)mfma_launches;
+    phase_ms[7] = (double)mfma_points;
 }
 
 // ---------------------------------------------------------- stage helpers
@@ -400,11 +433,11 @@ void hamming_topc_host(const uint32_t* ds, int64_t N, const uint32_t* qs, int64_
                        int32_t* ids, int32_t* dists, int device) {
     DeviceGuard g(device);
     LSH_REQUIRE(N >= 0 && Q >= 0 && words > 0 && C >= 1, LSH_ERR_INVALID, "bad Hamming top-C arguments");
-    const int K = pow2_ceil(C);
+    const int K = C <= kMaxIntTopK ? C : pow2_ceil(C);
     LSH_REQUIRE(K <= kMaxTopK, LSH_ERR_UNSUPPORTED, "C must be <= 4096");
     if (Q == 0) return;
     const int wpad = round_up(words, 8);
-    const int Qpad = round_up((int)Q, 64);
+    const int Qpad = sketch_query_rows((int)Q);
     DevBuf<uint32_t> draw((size_t)std::max<int64_t>(N, 1) * words), qraw((size_t)Q * words);
     DevBuf<uint32_t> dpad((size_t)std::max<int64_t>(N, 1) * wpad), qpad((size_t)Qpad * wpad);
     if (N > 0) LSH_HIP(hipMemcpy(draw.get(), ds, sizeof(uint32_t) * (size_t)N * words, hipMemcpyHostToDevice));
@@ -415,6 +448,7 @@ void hamming_topc_host(const uint32_t* ds, int64_t N, const uint32_t* qs, int64_
     PhaseTimer pt(false);
     int64_t launches = 0;
     sketch_topk(dpad.get(), N, wpad, 0, qpad.get(), (int)Q, K, false, st, nullptr, pt, launches);
+    launch_sort_rows(st.view(), nullptr);
     LSH_HIP(hipDeviceSynchronize());
     const TopKView v = st.view();
     std::vector<uint64_t> keys((size_t)Q * v.stride);

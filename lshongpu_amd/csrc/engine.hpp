@@ -56,6 +56,8 @@ struct lsh_index {
     lshgpu::DevBuf<uint32_t> flags;
     lshgpu::TopKState st;
     int64_t scan_launches = 0;
+    int64_t mfma_launches = 0;   // matrix-core Hamming scan launches of the last search
+    int64_t mfma_points = 0;     // data points those launches covered
 
     size_t device_bytes() const;
     void build(const float* src, bool on_device, const float* extra_host, int64_t n_extra);

@@ -152,11 +152,14 @@ struct TopKView {
     int newcap;
     int stride;  // K + newcap
     int Q;
+    int int_keys;  // 1: integer distances, best list unordered (select_hist_kernel)
 };
 
 class TopKState {
    public:
-    void reset(int Q, int K, int newcap, hipStream_t s);
+    // int_keys: the distances are small integers; the best list is then an
+    // unordered set of exactly K keys and K need not be a power of two.
+    void reset(int Q, int K, int newcap, hipStream_t s, bool int_keys = false);
     TopKView view() const { return v_; }
     size_t bytes() const { return keys_.bytes() + cnt_.bytes() + thr_.bytes() + ovf_.bytes(); }
 
@@ -169,8 +172,10 @@ class TopKState {
 };
 
 // Largest K the select kernels support (keys of the best list plus the sort
-// buffer must fit the 160 KiB LDS of a CDNA4 workgroup).
+// buffer must fit the 160 KiB LDS of a CDNA4 workgroup); integer-key lists
+// above kMaxIntTopK use the float-key (sorted) kernel.
 constexpr int kMaxTopK = 4096;
+constexpr int kMaxIntTopK = 2048;
 
 int pow2_ceil(int x);
 int newcap_for(int K);
@@ -180,6 +185,7 @@ int newcap_for(int K);
 
 // select.hip
 void launch_select_merge(const TopKView& v, hipStream_t s);
+void launch_sort_rows(const TopKView& v, hipStream_t s);  // orders the best list of every query
 void launch_count_overflow(const TopKView& v, uint32_t* out_count, hipStream_t s);
 
 // scan_exact.hip: distances of queries [0,Q) to points [p0,p1) of data.
@@ -235,6 +241,13 @@ struct SketchScanArgs {
     int kind;
 };
 void launch_sketch_scan(const SketchScanArgs& a, const TopKView& v, hipStream_t s);
+
+// scan_mfma.hip: Hamming scan on the matrix cores (FP4 +-1 operands).  qs
+// has ceil(Q/hamming_mfma_query_rows())*hamming_mfma_query_rows() rows.
+int hamming_mfma_query_rows();
+bool hamming_mfma_supported(int words);
+void launch_hamming_mfma(const uint32_t* ds, const uint32_t* qs, int words, int Q, int64_t p0, int64_t p1,
+                         const TopKView& v, hipStream_t s);
 
 // minhash.hip
 void launch_wmh_bounds(const float* x, int64_t n, int d, int* bounds, hipStream_t s);

@@ -52,6 +52,109 @@ __global__ __launch_bounds__(kSelThreads) void select_merge_kernel(TopKView v) {
     }
 }
 
+// Integer-key variant (sketch distances are small integers): the best list
+// is kept as an unordered set of exactly K keys (K need not be a power of
+// two).  A histogram of the distances of (best list + new keys) in LDS gives
+// the distance T of the K-th key; every key below T is kept, and the keys at
+// exactly T are sorted by ID to keep the lowest.  thr becomes the largest
+// kept key.  Distances at or above kSelBins-1 share the last bin, which the
+// tie step orders exactly.
+constexpr int kSelBins = 4096;
+
+__global__ __launch_bounds__(kSelThreads) void select_hist_kernel(TopKView v, int tie_cap) {
+    extern __shared__ uint64_t sm[];
+    __shared__ uint32_t hist[kSelBins];
+    __shared__ uint32_t s_T, s_below, s_nkeep, s_ntie;
+    const int q = blockIdx.x;
+    const uint32_t n_raw = v.cnt[q];
+    if (n_raw == 0) return;
+    const int K = v.K;
+    uint64_t* kept = sm;       // K
+    uint64_t* ties = sm + K;   // tie_cap (power of two >= K + newcap)
+    const int n = (int)min(n_raw, (uint32_t)v.newcap);
+    const int total = K + n;
+    uint64_t* row = v.keys + (size_t)q * v.stride;
+    for (int i = threadIdx.x; i < kSelBins; i += blockDim.x) hist[i] = 0;
+    if (threadIdx.x == 0) {
+        s_nkeep = 0;
+        s_ntie = 0;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < total; i += blockDim.x) {
+        const uint64_t key = row[i];
+        if (key != kEmptyKey) atomicAdd(&hist[min((uint32_t)(key >> 32), (uint32_t)(kSelBins - 1))], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        // Wave 0 finds the first bin at which the running count reaches K.
+        const int lane = threadIdx.x;
+        constexpr int per = kSelBins / 64;
+        uint32_t local = 0;
+        for (int b = 0; b < per; ++b) local += hist[lane * per + b];
+        uint32_t incl = local;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(incl, o);
+            if (lane >= o) incl += t;
+        }
+        const uint32_t excl = incl - local;
+        const uint32_t all = __shfl(incl, 63);
+        const unsigned long long has = __ballot(incl >= (uint32_t)K && excl < (uint32_t)K);
+        if (all < (uint32_t)K) {
+            if (lane == 0) {
+                s_T = kSelBins;  // fewer than K keys in total: keep them all
+                s_below = all;
+            }
+        } else {
+            const int owner = __ffsll((long long)has) - 1;
+            if (lane == owner) {
+                uint32_t c = excl;
+                int b = lane * per;
+                while (c + hist[b] < (uint32_t)K) c += hist[b++];
+                s_T = (uint32_t)b;
+                s_below = c;
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t T = s_T;
+    for (int i = threadIdx.x; i < total; i += blockDim.x) {
+        const uint64_t key = row[i];
+        if (key == kEmptyKey) continue;
+        const uint32_t b = min((uint32_t)(key >> 32), (uint32_t)(kSelBins - 1));
+        if (b < T)
+            kept[atomicAdd(&s_nkeep, 1u)] = key;
+        else if (b == T)
+            ties[atomicAdd(&s_ntie, 1u)] = key;
+    }
+    __syncthreads();
+    const uint32_t below = s_below;  // == s_nkeep
+    const uint32_t ntie = s_ntie;
+    int P = 1;
+    while (P < (int)ntie) P <<= 1;
+    for (int i = (int)ntie + threadIdx.x; i < P; i += blockDim.x) ties[i] = kEmptyKey;
+    bitonic_sort_lds(ties, P);
+    const uint32_t take = T == kSelBins ? 0u : min((uint32_t)K - below, ntie);
+    for (uint32_t i = threadIdx.x; i < below; i += blockDim.x) row[i] = kept[i];
+    for (uint32_t i = threadIdx.x; i < take; i += blockDim.x) row[below + i] = ties[i];
+    for (uint32_t i = below + take + threadIdx.x; i < (uint32_t)K; i += blockDim.x) row[i] = kEmptyKey;
+    if (threadIdx.x == 0) {
+        v.thr[q] = (below + take == (uint32_t)K) ? ties[take - 1] : kEmptyKey;
+        v.cnt[q] = 0;
+        if (n_raw > (uint32_t)v.newcap) v.ovf[q] = 1;
+    }
+    (void)tie_cap;
+}
+
+// Sorts the first K keys of every row ascending (K <= 4096).
+__global__ __launch_bounds__(kSelThreads) void sort_rows_kernel(uint64_t* keys, int stride, int K, int P) {
+    extern __shared__ uint64_t sm[];
+    uint64_t* row = keys + (size_t)blockIdx.x * stride;
+    for (int i = threadIdx.x; i < P; i += blockDim.x) sm[i] = i < K ? row[i] : kEmptyKey;
+    bitonic_sort_lds(sm, P);
+    for (int i = threadIdx.x; i < K; i += blockDim.x) row[i] = sm[i];
+}
+
 __global__ void fill_u64(uint64_t* p, size_t n, uint64_t val) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t st = (size_t)gridDim.x * blockDim.x;
@@ -86,15 +189,15 @@ int pow2_ceil(int x) {
 
 int newcap_for(int K) { return K <= 1024 ? 2 * K + 256 : K + 512; }
 
-void TopKState::reset(int Q, int K, int newcap, hipStream_t s) {
-    LSH_REQUIRE(K >= 1 && K <= kMaxTopK && (K & (K - 1)) == 0, LSH_ERR_INVALID,
+void TopKState::reset(int Q, int K, int newcap, hipStream_t s, bool int_keys) {
+    LSH_REQUIRE(K >= 1 && K <= kMaxTopK && (int_keys || (K & (K - 1)) == 0), LSH_ERR_INVALID,
                 "top-K size must be a power of two <= 4096");
     const int stride = K + newcap;
     keys_.ensure((size_t)Q * stride);
     cnt_.ensure((size_t)Q);
     thr_.ensure((size_t)Q);
     ovf_.ensure((size_t)Q);
-    v_ = TopKView{keys_.get(), cnt_.get(), thr_.get(), ovf_.get(), K, newcap, stride, Q};
+    v_ = TopKView{keys_.get(), cnt_.get(), thr_.get(), ovf_.get(), K, newcap, stride, Q, int_keys ? 1 : 0};
     // Fill kernels rather than memsets keep the sequence capturable in a graph.
     size_t nk = (size_t)Q * stride;
     hipLaunchKernelGGL(fill_u64, dim3(grid_for(nk)), dim3(256), 0, s, keys_.get(), nk, kEmptyKey);
@@ -105,10 +208,26 @@ void TopKState::reset(int Q, int K, int newcap, hipStream_t s) {
 }
 
 void launch_select_merge(const TopKView& v, hipStream_t s) {
+    if (v.int_keys) {
+        const int tie_cap = pow2_ceil(v.K + v.newcap);
+        const size_t lds = (size_t)(v.K + tie_cap) * sizeof(uint64_t);
+        set_dynamic_lds((const void*)select_hist_kernel, lds + sizeof(uint32_t) * (kSelBins + 4));
+        hipLaunchKernelGGL(select_hist_kernel, dim3(v.Q), dim3(kSelThreads), lds, s, v, tie_cap);
+        check_launch("select_hist_kernel");
+        return;
+    }
     const size_t lds = (size_t)(v.K + pow2_ceil(std::max(v.newcap, v.K))) * sizeof(uint64_t);
     set_dynamic_lds((const void*)select_merge_kernel, lds);
     hipLaunchKernelGGL(select_merge_kernel, dim3(v.Q), dim3(kSelThreads), lds, s, v);
     check_launch("select_merge_kernel");
+}
+
+void launch_sort_rows(const TopKView& v, hipStream_t s) {
+    const int P = pow2_ceil(std::max(v.K, 2));
+    const size_t lds = (size_t)P * sizeof(uint64_t);
+    set_dynamic_lds((const void*)sort_rows_kernel, lds);
+    hipLaunchKernelGGL(sort_rows_kernel, dim3(v.Q), dim3(kSelThreads), lds, s, v.keys, v.stride, v.K, P);
+    check_launch("sort_rows_kernel");
 }
 
 void launch_count_overflow(const TopKView& v, uint32_t* out_count, hipStream_t s) {

@@ -166,6 +166,22 @@ def test_hamming_topc_bit_exact(lsh, orc, C):
     np.testing.assert_array_equal(dist, odist)
 
 
+def test_hamming_topc_adversarial_order_overflows_and_recovers(lsh, orc):
+    # Points ordered from the farthest to the nearest (in Hamming distance) of
+    # query 0: every round finds better candidates than all before it, the
+    # per-round capacity overflows, and the affected queries are re-run.
+    rng = np.random.default_rng(77)
+    ds = rng.integers(0, 2 ** 32, size=(60000, 8), dtype=np.uint64).astype(np.uint32)
+    qs = rng.integers(0, 2 ** 32, size=(300, 8), dtype=np.uint64).astype(np.uint32)
+    qs[1:40] = qs[0] ^ (1 << rng.integers(0, 32, size=(39, 8))).astype(np.uint32)
+    h0 = np.unpackbits((ds ^ qs[0]).view(np.uint8), axis=1).sum(1)
+    ds = ds[np.argsort(-h0, kind="stable")].copy()
+    ids, dist = lsh.hamming_topc(ds, qs, 256)
+    oids, odist = orc.sketch_topc(0, ds, qs, 256)
+    np.testing.assert_array_equal(ids, oids)
+    np.testing.assert_array_equal(dist, odist)
+
+
 # ------------------------------------------------------------ full pipelines
 
 def test_simhash_pipeline_matches_oracle(lsh, orc):
