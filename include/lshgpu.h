@@ -170,6 +170,17 @@ int lsh_hamming_topc(const uint32_t* data_sketch, int64_t N_data,
                      const uint32_t* query_sketch, int64_t N_query, int words,
                      int C, int32_t* ids, int32_t* distances);
 
+/* weightedMinHash.cuh:41-166 on rows that are already shifted and
+ * normalised: bounds from x, then per hash h (seed h*1234 + 92138) the number
+ * of red draws before the first green one.  oneBit == 0: out is [n][sketchDim]
+ * counts; oneBit != 0: out is [n][sketchDim] words whose bit b of word w is
+ * bitmap[count of hash w*32+b] (sketchDataOneBit).  The draws come from
+ * Philox4x32-10 (key (seed, 0x5EED5EED), counter t/4, word t%4, U = (word>>8)
+ * * 2^-24), not the reference's cuRAND XORWOW stream; rows with no green draw
+ * in 2^16 draws record 2^16. */
+int lsh_wmh_sketch(const float* x, int64_t n, int d, int sketchDim, int oneBit,
+                   uint32_t* out);
+
 /* crossPolyTope.cuh:25-85 generateHashCrossPoly: pad to P = next power of two
  * >= d, three rounds of (+-1 diagonal, unnormalised Walsh-Hadamard
  * butterfly), key = argmax |v| encoded as i (negative) or i + P.  Rows are

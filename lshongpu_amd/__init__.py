@@ -27,7 +27,7 @@ import numpy as np
 __all__ = [
     "LshError", "Result", "Index", "POINT_DTYPE", "load", "calculateK", "linearScans", "LSH",
     "linear_scans", "lsh_framework", "random_normals", "random_signs", "simhash_sketch",
-    "jl_sketch", "gj_transform", "hamming_topc", "crosspoly_keys", "library_path",
+    "jl_sketch", "gj_transform", "hamming_topc", "wmh_sketch", "crosspoly_keys", "library_path",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -114,6 +114,7 @@ _SIGS = {
     "lsh_gj_transform": (C.c_int, [_FP, C.c_int64, _FP, C.c_int64, C.c_int]),
     "lsh_hamming_topc": (C.c_int, [C.POINTER(C.c_uint32), C.c_int64, C.POINTER(C.c_uint32), C.c_int64, C.c_int,
                                    C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "lsh_wmh_sketch": (C.c_int, [_FP, C.c_int64, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint32)]),
     "lsh_crosspoly_keys": (C.c_int, [_FP, C.c_int64, C.c_int, C.c_int, C.POINTER(C.c_int32),
                                      C.POINTER(C.c_uint16)]),
     "lsh_index_config_default": (None, [C.POINTER(_IndexConfig)]),
@@ -313,6 +314,16 @@ def hamming_topc(data_sketch, query_sketch, C_: int):
                                    ids.ctypes.data_as(C.POINTER(C.c_int32)),
                                    dist.ctypes.data_as(C.POINTER(C.c_int32))))
     return ids, dist
+
+
+def wmh_sketch(x, sketch_dim: int, one_bit: bool = False) -> np.ndarray:
+    """Weighted MinHash of rows that are already shifted and normalised."""
+    x = _f32(x)
+    n, d = x.shape
+    out = np.empty((n, int(sketch_dim)), dtype=np.uint32)
+    _check(load().lsh_wmh_sketch(_fp(x), n, d, int(sketch_dim), int(bool(one_bit)),
+                                 out.ctypes.data_as(C.POINTER(C.c_uint32))))
+    return out
 
 
 def crosspoly_keys(x, tables: int, signs=None) -> np.ndarray:

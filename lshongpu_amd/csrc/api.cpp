@@ -270,6 +270,14 @@ int lsh_hamming_topc(const uint32_t* data_sketch, int64_t N_data, const uint32_t
     });
 }
 
+int lsh_wmh_sketch(const float* x, int64_t n, int d, int sketchDim, int oneBit, uint32_t* out) {
+    return guarded([&] {
+        LSH_REQUIRE((x && out) || n == 0, LSH_ERR_INVALID, "null pointer");
+        if (n == 0) return;
+        lshgpu::wmh_host(x, n, d, sketchDim, oneBit, out, 0);
+    });
+}
+
 int lsh_crosspoly_keys(const float* x, int64_t n, int d, int tables, const int32_t* signs, uint16_t* keys) {
     return guarded([&] {
         LSH_REQUIRE((x && keys) || n == 0, LSH_ERR_INVALID, "null pointer");

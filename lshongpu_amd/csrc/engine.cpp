@@ -252,11 +252,65 @@ void lsh_index::build(const float* src, bool on_device, const float* extra_host,
         launch_jl(data.get(), N, d, planesT.get(), sdim, djl.get(), s);
         LSH_HIP(hipStreamSynchronize(s));
     } else if (framework == 0 && (impl == 4 || impl == 5)) {
-        fail(LSH_ERR_UNSUPPORTED, "weighted MinHash is not built yet");
+        // weightedMinHash.cuh:344-416: bounds, index map, sketches (the bounds
+        // include the one-shot queries, as the reference computes them).
+        wmh_mode = impl == 5 ? 1 : 0;
+        words = sketchDim;
+        wpad = round_up(std::max(words, 1), 8);
+        build_wmh_tables(data.get(), N, extra.get(), n_extra, s);
+        dsk.alloc((size_t)N * wpad);
+        if (N > 0) LSH_HIP(hipMemsetAsync(dsk.get(), 0, dsk.bytes(), s));
+        wmh_sketch_rows(data.get(), N, dsk.get(), s);
+        LSH_HIP(hipStreamSynchronize(s));
     } else if (framework == 1) {
         fail(LSH_ERR_UNSUPPORTED, "the (K,L) framework is not built yet");
     }
     LSH_HIP(hipStreamSynchronize(s));
+}
+
+void lsh_index::build_wmh_tables(const float* x, int64_t n, const float* extra, int64_t n_extra, hipStream_t s) {
+    DevBuf<int> b((size_t)d);
+    launch_fill_i32(b.get(), (size_t)d, 0, s);
+    launch_wmh_bounds(x, n, d, b.get(), s);
+    if (extra && n_extra > 0) launch_wmh_bounds(extra, n_extra, d, b.get(), s);
+    std::vector<int> hb((size_t)d);
+    LSH_HIP(hipMemcpyAsync(hb.data(), b.get(), sizeof(int) * (size_t)d, hipMemcpyDeviceToHost, s));
+    LSH_HIP(hipStreamSynchronize(s));
+    // weightedMinHash.cuh:62-90: M = sum of bounds, start = exclusive prefix
+    // sum, owner[r] = the dimension whose interval holds r.
+    std::vector<int> start((size_t)d);
+    int64_t M = 0;
+    for (int j = 0; j < d; ++j) {
+        start[j] = (int)M;
+        M += hb[j];
+    }
+    LSH_REQUIRE(M < (int64_t)1 << 30, LSH_ERR_UNSUPPORTED, "weighted MinHash bounds sum too large");
+    wmh_M = (int)M;
+    std::vector<int> owner((size_t)std::max<int64_t>(M, 1), 0);
+    for (int j = 0; j < d; ++j)
+        for (int r = start[j]; r < start[j] + hb[j]; ++r) owner[r] = j;
+    wmh_owner.alloc(owner.size());
+    wmh_start.alloc((size_t)d);
+    LSH_HIP(hipMemcpyAsync(wmh_owner.get(), owner.data(), sizeof(int) * owner.size(), hipMemcpyHostToDevice, s));
+    LSH_HIP(hipMemcpyAsync(wmh_start.get(), start.data(), sizeof(int) * (size_t)d, hipMemcpyHostToDevice, s));
+    // weightedMinHash.cuh:231-250 generates 255 bits of the default-seeded
+    // stream; the table continues that stream so counts past 254 stay defined.
+    std::vector<uint8_t> bm = bit_stream(4096);
+    wmh_bitmap_len = (int)bm.size();
+    wmh_bitmap.alloc(bm.size());
+    LSH_HIP(hipMemcpyAsync(wmh_bitmap.get(), bm.data(), bm.size(), hipMemcpyHostToDevice, s));
+    const int nhash = wmh_mode == 1 ? 32 * words : words;
+    wmh_R = 256;
+    wmh_draws.alloc((size_t)nhash * wmh_R);
+    wmh_draw_owner.alloc((size_t)nhash * wmh_R);
+    launch_wmh_draws(nhash, wmh_R, wmh_M, wmh_owner.get(), wmh_draws.get(), wmh_draw_owner.get(), s);
+    LSH_HIP(hipStreamSynchronize(s));
+}
+
+void lsh_index::wmh_sketch_rows(const float* x, int64_t n, uint32_t* out, hipStream_t s) {
+    const int nhash = wmh_mode == 1 ? 32 * words : words;
+    launch_wmh_sketch(x, n, d, wmh_owner.get(), wmh_start.get(), wmh_M, nhash, wmh_mode, wmh_bitmap.get(),
+                      wmh_bitmap_len, wmh_draws.get(), wmh_draw_owner.get(), wmh_R, out, wpad, s);
 }
 
 void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh_point_t* results,
@@ -324,6 +378,7 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
         pt.begin(PH_PREP, s);
         LSH_HIP(hipMemsetAsync(qsk.get(), 0, sizeof(uint32_t) * (size_t)Qpad * wpad, s));
         if (impl == 3) launch_simhash(q.get(), Q, d, planesT.get(), words, 32, qsk.get(), wpad, s);
+        else wmh_sketch_rows(q.get(), Q, qsk.get(), s);
         pt.end(PH_PREP, s);
         const int Ceff = (int)std::max<int64_t>(1, std::min<int64_t>(candidates, N));
         const int K = Ceff <= kMaxIntTopK ? Ceff : pow2_ceil(Ceff);
@@ -459,6 +514,25 @@ void hamming_topc_host(const uint32_t* ds, int64_t N, const uint32_t* qs, int64_
             ids[q * C + c] = key == kEmptyKey ? -1 : (int32_t)(uint32_t)key;
             if (dists) dists[q * C + c] = key == kEmptyKey ? -1 : (int32_t)(key >> 32);
         }
+}
+
+void wmh_host(const float* x, int64_t n, int d, int sketchDim, int one_bit, uint32_t* out, int device) {
+    DeviceGuard g(device);
+    LSH_REQUIRE(n >= 0 && d > 0 && sketchDim > 0, LSH_ERR_INVALID, "bad weighted MinHash arguments");
+    lsh_index ix;
+    ix.d = d;
+    ix.N = n;
+    ix.device = device;
+    ix.wmh_mode = one_bit ? 1 : 0;
+    ix.words = sketchDim;
+    ix.wpad = sketchDim;
+    DevBuf<float> dx((size_t)n * d);
+    DevBuf<uint32_t> dout((size_t)n * sketchDim);
+    LSH_HIP(hipMemcpy(dx.get(), x, sizeof(float) * (size_t)n * d, hipMemcpyHostToDevice));
+    ix.build_wmh_tables(dx.get(), n, nullptr, 0, nullptr);
+    ix.wmh_sketch_rows(dx.get(), n, dout.get(), nullptr);
+    LSH_HIP(hipDeviceSynchronize());
+    LSH_HIP(hipMemcpy(out, dout.get(), sizeof(uint32_t) * (size_t)n * sketchDim, hipMemcpyDeviceToHost));
 }
 
 void crosspoly_host(const float* x, int64_t n, int d, int tables, const int32_t* signs, uint16_t* keys,

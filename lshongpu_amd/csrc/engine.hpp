@@ -60,6 +60,8 @@ struct lsh_index {
     int64_t mfma_points = 0;     // data points those launches covered
 
     size_t device_bytes() const;
+    void build_wmh_tables(const float* x, int64_t n, const float* extra, int64_t n_extra, hipStream_t s);
+    void wmh_sketch_rows(const float* x, int64_t n, uint32_t* out, hipStream_t s);
     void build(const float* src, bool on_device, const float* extra_host, int64_t n_extra);
     void search(const float* queries, int Q, int k, bool q_on_device, lsh_point_t* results,
                 bool r_on_device, hipStream_t s);
@@ -77,6 +79,7 @@ void jl_host(const float* x, int64_t n, int d, int sdim, const float* planes, fl
 void gj_transform_host(float* data, int64_t N, float* queries, int64_t Q, int d, int device);
 void hamming_topc_host(const uint32_t* ds, int64_t N, const uint32_t* qs, int64_t Q, int words, int C,
                        int32_t* ids, int32_t* dists, int device);
+void wmh_host(const float* x, int64_t n, int d, int sketchDim, int one_bit, uint32_t* out, int device);
 void crosspoly_host(const float* x, int64_t n, int d, int tables, const int32_t* signs, uint16_t* keys,
                     int device);
 

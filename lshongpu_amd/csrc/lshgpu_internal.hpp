@@ -250,11 +250,12 @@ void launch_hamming_mfma(const uint32_t* ds, const uint32_t* qs, int words, int 
                          const TopKView& v, hipStream_t s);
 
 // minhash.hip
+int wmh_max_draws();
 void launch_wmh_bounds(const float* x, int64_t n, int d, int* bounds, hipStream_t s);
-void launch_wmh_sketch(const float* x, int64_t n, int d, const int* owner, const int* start,
-                       int M, int H, int mode, const uint8_t* bitmap, int bitmap_len,
-                       const float* draws, const int* draw_owner, int R, uint32_t* out,
-                       hipStream_t s);
+void launch_wmh_draws(int nhash, int R, int M, const int* owner, float* draws, int* downer, hipStream_t s);
+void launch_wmh_sketch(const float* x, int64_t n, int d, const int* owner, const int* start, int M, int nhash,
+                       int mode, const uint8_t* bitmap, int bitmap_len, const float* draws, const int* draw_owner,
+                       int R, uint32_t* out, int out_stride, hipStream_t s);
 
 // util.hip
 void launch_gather_rows_f32(const float* src, int cols, const int32_t* idx, int m, float* dst,

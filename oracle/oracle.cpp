@@ -514,15 +514,16 @@ EXPORT void orc_wmh_sketch(const float* x, int64_t n, int d, const int32_t* boun
     for (int j = 0; j < d; ++j)
         for (int r = start[j]; r < start[j] + bounds[j]; ++r) owner[r] = j;
     const float Mf = (float)M;
+    // A row that sees no green draw in 2^16 draws (a zero or NaN row) records 2^16.
     auto count = [&](const float* row, uint32_t seed) -> uint32_t {
         if (M <= 0) return 0;
-        for (uint32_t t = 0;; ++t) {
+        for (uint32_t t = 0; t < 65536u; ++t) {
             float r = Mf * orc_wmh_uniform(seed, t);
             int ri = std::min((int)r, M - 1);
             int c = owner[ri];
             if (r <= (float)start[c] + row[c]) return t;
-            if (t == 0xFFFFFFFFu) return t;
         }
+        return 65536u;
     };
     parallel_for(n, [&](int64_t i) {
         const float* row = x + i * d;

@@ -239,3 +239,50 @@ def test_index_device_pointers(lsh):
     torch.cuda.synchronize()
     got = out.cpu().numpy().view(lsh.POINT_DTYPE).reshape(50, 32)
     np.testing.assert_array_equal(got, host)
+
+
+# ------------------------------------------------------------ weighted MinHash
+
+def _wmh_rows(rng, n, d, density):
+    x = rng.exponential(1.0, (n, d)).astype(np.float32)
+    x *= rng.random((n, d)) < density
+    x[np.arange(n), rng.integers(0, d, n)] += 0.5  # no empty rows
+    return x
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("one_bit", [False, True])
+@pytest.mark.parametrize("d,density", [(40, 0.6), (300, 0.01)])
+def test_wmh_sketch_bit_exact(lsh, orc, one_bit, d, density):
+    """Sparse rows make most counts exceed the precomputed draw table."""
+    rng = np.random.default_rng(31 + d)
+    raw = _wmh_rows(rng, 700, d, density)
+    x, _ = orc.gj_transform(raw, raw[:0])
+    sk = 3 if one_bit else 40
+    got = lsh.wmh_sketch(x, sk, one_bit)
+    bounds, start, M = orc.wmh_bounds(x)
+    want = orc.wmh_sketch(x, bounds, start, M, sk, int(one_bit), orc.bit_stream(4096))
+    np.testing.assert_array_equal(got, want)
+    if not one_bit and density < 0.1:
+        assert (want > 256).mean() > 0.2
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("impl", [4, 5])
+def test_wmh_pipeline_matches_oracle(lsh, orc, impl):
+    rng = np.random.default_rng(40 + impl)
+    N, Q, d, C, k = 6000, 24, 64, 200, 32
+    sk = 24 if impl == 4 else 8
+    data = _wmh_rows(rng, N, d, 0.3)
+    queries = _wmh_rows(rng, Q, d, 0.3)
+    res = lsh.linearScans(impl, k, d, Q, N, data, queries, sk, 2, candidates=C)
+    td, tq = orc.gj_transform(data, queries)
+    bounds, start, M = orc.wmh_bounds(td, tq)
+    bm = orc.bit_stream(4096)
+    mode = impl - 4
+    ds = orc.wmh_sketch(td, bounds, start, M, sk, mode, bm)
+    qs = orc.wmh_sketch(tq, bounds, start, M, sk, mode, bm)
+    cand, _ = orc.sketch_topc(1 if impl == 4 else 0, ds, qs, C)
+    ids, dist = orc.rerank(td, tq, cand, k, 2)
+    np.testing.assert_array_equal(res.ids, ids)
+    _close(res.distances, dist)
