@@ -26,6 +26,11 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# HBM bytes per hamming_mfma_kernel launch measured with FETCH_SIZE/WRITE_SIZE
+# counters at the default workload, recorded in this summary.
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hamming_mfma_kernel_simhash1024_N1.2M_Q10k.md")
+DEFAULT_WORKLOAD = dict(n=1_200_000, queries=10_000, words=32, candidates=1024, k=32)
+
 # MI355X published peaks.
 HBM_PEAK_GBS = 8000.0
 MFMA_FP4_PEAK_TFLOPS = 10100.0  # dense FP4 matrix peak
@@ -39,6 +44,19 @@ def cpu_model() -> str:
     except OSError:
         pass
     return platform.processor() or "unknown"
+
+
+def measured_traffic(args):
+    """Counter-measured bytes per launch, when this run is the profiled workload."""
+    if any(getattr(args, key) != val for key, val in DEFAULT_WORKLOAD.items()):
+        return None
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith("TRAFFIC_BYTES_PER_LAUNCH:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def main() -> int:
@@ -55,6 +73,9 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-exact", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="after the run, copy MIB MiB on the device 3 times with an elementwise kernel (a "
+                         "plain copy of known size that scales FETCH_SIZE/WRITE_SIZE counter readings)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -184,7 +205,9 @@ def main() -> int:
             "peak": MFMA_FP4_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / MFMA_FP4_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": measured_traffic(args),
+            "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a plain copy kernel; "
+                              "profiles/hamming_mfma_kernel_simhash1024_N1.2M_Q10k.md",
             "algorithmic_flop_per_launch": flop_launch,
             "avg_launch_ms": avg_launch_ms,
             "launches_per_step": mfma_launches,
@@ -197,6 +220,14 @@ def main() -> int:
         "data_gen_s": gen_s,
     }
     ix.close()
+    if args.calibrate_copy > 0:
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.arange(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)  # an elementwise kernel: reads and writes n*4 bytes
+        torch.cuda.synchronize()
+        line["copy_calibration_bytes"] = n * 4
     print(json.dumps(line))
     return 0
 
