@@ -88,6 +88,33 @@ __device__ inline float dist32_dev(int kind, const float* __restrict__ q,
     }
 }
 
+// Partial sums of one lane of a group that splits a row over its lanes:
+// (dot, |x|^2) for the angular kinds, (sum min, sum max) for generalized
+// Jaccard, (sum of squares, -) for L2.  x is the data value, y the query's.
+__device__ inline void group_accumulate(int kind, float x, float y, float& a, float& b) {
+    if (kind == kNegCos || kind == kAngular) {
+        a = __builtin_fmaf(x, y, a);
+        b = __builtin_fmaf(x, x, b);
+    } else if (kind == kGenJaccard) {
+        a += fminf(x, y);
+        b += fmaxf(x, y);
+    } else {
+        const float t = y - x;
+        a = __builtin_fmaf(t, t, a);
+    }
+}
+
+// FP32 distance from the reduced partial sums (q_norm = ||q|| in FP32).
+__device__ inline float group_distance(int kind, float a, float b, float q_norm) {
+    if (kind == kNegCos || kind == kAngular) {
+        const float c = a / (q_norm * __builtin_sqrtf(b));
+        return kind == kNegCos ? -c : 1.0f - c;
+    } else if (kind == kGenJaccard) {
+        return 1.0f - a / b;
+    }
+    return a;  // squared; kEuclid ranks by the square
+}
+
 __device__ inline void cmpx_u64(uint64_t* a, int i, int j, bool asc) {
     const uint64_t x = a[i], y = a[j];
     if ((x > y) == asc) {

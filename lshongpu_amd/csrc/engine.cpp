@@ -180,11 +180,131 @@ std::vector<float> transpose_jl(const std::vector<float>& planes, int sdim, int 
 
 // ---------------------------------------------------------------- index
 
+namespace lshgpu {
+
+void WmhHasher::build(const float* x, int64_t n, const float* extra, int64_t n_extra, int d, int mode_,
+                      int nhash_, int hbits_, hipStream_t s) {
+    mode = mode_;
+    nhash = nhash_;
+    hbits = hbits_;
+    DevBuf<int> b((size_t)d);
+    launch_fill_i32(b.get(), (size_t)d, 0, s);
+    launch_wmh_bounds(x, n, d, b.get(), s);
+    if (extra && n_extra > 0) launch_wmh_bounds(extra, n_extra, d, b.get(), s);
+    std::vector<int> hb((size_t)d);
+    LSH_HIP(hipMemcpyAsync(hb.data(), b.get(), sizeof(int) * (size_t)d, hipMemcpyDeviceToHost, s));
+    LSH_HIP(hipStreamSynchronize(s));
+    // weightedMinHash.cuh:62-90: M = sum of bounds, start = exclusive prefix
+    // sum, owner[r] = the dimension whose interval holds r.
+    std::vector<int> st((size_t)d);
+    int64_t m = 0;
+    for (int j = 0; j < d; ++j) {
+        st[j] = (int)m;
+        m += hb[j];
+    }
+    LSH_REQUIRE(m < (int64_t)1 << 30, LSH_ERR_UNSUPPORTED, "weighted MinHash bounds sum too large");
+    M = (int)m;
+    std::vector<int> own((size_t)std::max<int64_t>(m, 1), 0);
+    for (int j = 0; j < d; ++j)
+        for (int r = st[j]; r < st[j] + hb[j]; ++r) own[r] = j;
+    owner.alloc(own.size());
+    start.alloc((size_t)d);
+    LSH_HIP(hipMemcpyAsync(owner.get(), own.data(), sizeof(int) * own.size(), hipMemcpyHostToDevice, s));
+    LSH_HIP(hipMemcpyAsync(start.get(), st.data(), sizeof(int) * (size_t)d, hipMemcpyHostToDevice, s));
+    // weightedMinHash.cuh:231-250 generates 255 bits of the default-seeded
+    // stream; the table continues that stream so counts past 254 stay defined.
+    std::vector<uint8_t> bm = bit_stream(4096);
+    bitmap_len = (int)bm.size();
+    bitmap.alloc(bm.size());
+    LSH_HIP(hipMemcpyAsync(bitmap.get(), bm.data(), bm.size(), hipMemcpyHostToDevice, s));
+    R = 256;
+    draws.alloc((size_t)nhash * R);
+    draw_owner.alloc((size_t)nhash * R);
+    launch_wmh_draws(nhash, R, M, owner.get(), draws.get(), draw_owner.get(), s);
+    LSH_HIP(hipStreamSynchronize(s));
+}
+
+void WmhHasher::sketch(const float* x, int64_t n, int d, uint32_t* out, int out_stride, hipStream_t s) const {
+    launch_wmh_sketch(x, n, d, owner.get(), start.get(), M, nhash, mode, hbits, bitmap.get(), bitmap_len,
+                      draws.get(), draw_owner.get(), R, out, out_stride, s);
+}
+
+}  // namespace lshgpu
+
+namespace {
+
+// Applies `steps` to x (and extra) at build time, freezing the shifts.
+void apply_steps_build(std::vector<Transform>& steps, float* x, int64_t n, float* extra, int64_t n_extra, int d,
+                       hipStream_t s) {
+    for (Transform& t : steps) {
+        if (t.kind == kGjStep) {
+            t.mins.alloc((size_t)d);
+            launch_fill_i32(t.mins.get(), (size_t)d, 0, s);
+            launch_gj_mins(x, n, d, t.mins.get(), s);
+            if (extra && n_extra > 0) launch_gj_mins(extra, n_extra, d, t.mins.get(), s);
+            launch_gj_shift(x, n, d, t.mins.get(), s);
+            if (extra && n_extra > 0) launch_gj_shift(extra, n_extra, d, t.mins.get(), s);
+        }
+        launch_normalize_rows(x, n, d, s);
+        if (extra && n_extra > 0) launch_normalize_rows(extra, n_extra, d, s);
+    }
+}
+
+void apply_steps_query(const std::vector<Transform>& steps, float* q, int64_t n, int d, hipStream_t s) {
+    for (const Transform& t : steps) {
+        if (t.kind == kGjStep) launch_gj_shift(q, n, d, t.mins.get(), s);
+        launch_normalize_rows(q, n, d, s);
+    }
+}
+
+std::vector<Transform> one_step(int kind) {
+    std::vector<Transform> v(1);
+    v[0].kind = kind;
+    return v;
+}
+
+// Top-K over the unique candidate sets of the rows of `a` (rows = a.Q).
+// hmap: host copy of a.qmap (empty when a.qmap is null).  Queries whose
+// appends overflowed a round are re-run on their own with ranges no larger
+// than the append capacity.
+void lsh_topk(const LshScanArgs& base, const std::vector<int32_t>& hmap, int64_t maxS, int K, bool fixed,
+              bool int_keys, TopKState& st, hipStream_t s, PhaseTimer& pt, int64_t& launches) {
+    auto scan = [&](int64_t p0, int64_t p1, const TopKView& v) {
+        LshScanArgs a = base;
+        a.p0 = p0;
+        a.p1 = p1;
+        launch_lsh_scan(a, v, s);
+    };
+    run_rounds(st, base.Q, K, std::max<int64_t>(maxS, 1), fixed, scan, s, pt, launches, int_keys);
+    if (fixed) return;
+    const std::vector<int32_t> ovf = flagged(st.view().ovf, base.Q, s);
+    if (ovf.empty()) return;
+    const int m = (int)ovf.size();
+    std::vector<int32_t> sub_map((size_t)m);
+    for (int i = 0; i < m; ++i) sub_map[i] = hmap.empty() ? ovf[i] : hmap[ovf[i]];
+    DevBuf<int32_t> dmap((size_t)m), drows((size_t)m);
+    LSH_HIP(hipMemcpyAsync(dmap.get(), sub_map.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, s));
+    LSH_HIP(hipMemcpyAsync(drows.get(), ovf.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, s));
+    LshScanArgs a = base;
+    a.qmap = dmap.get();
+    a.Q = m;
+    TopKState sst;
+    lsh_topk(a, sub_map, maxS, K, true, int_keys, sst, s, pt, launches);
+    const TopKView sv = sst.view(), v = st.view();
+    launch_scatter_rows_u64(sv.keys, sv.stride, K, drows.get(), m, v.keys, v.stride, s);
+    LSH_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
 size_t lsh_index::device_bytes() const {
-    return data.bytes() + data_inv.bytes() + mins.bytes() + dsk.bytes() + planesT.bytes() + djl.bytes() +
-           wmh_owner.bytes() + wmh_start.bytes() + wmh_bitmap.bytes() + q.bytes() + qT.bytes() +
-           q_inv.bytes() + qjl.bytes() + qjlT.bytes() + qsk.bytes() + out.bytes() + flags.bytes() +
-           st.bytes();
+    size_t b = data.bytes() + data_inv.bytes() + dsk.bytes() + planesT.bytes() + djl.bytes() + wmh.bytes() +
+               kwmh.bytes() + kplanesT.bytes() + ksigns.bytes() + dkeys.bytes() + boff.bytes() + bperm.bytes() +
+               qkeys.bytes() + qlo.bytes() + qpre.bytes() + qmax.bytes() + q.bytes() + qT.bytes() +
+               q_inv.bytes() + qjl.bytes() + qjlT.bytes() + qsk.bytes() + out.bytes() + flags.bytes() + st.bytes();
+    for (const Transform& t : key_steps) b += t.mins.bytes();
+    for (const Transform& t : sketch_steps) b += t.mins.bytes();
+    return b;
 }
 
 void lsh_index::build(const float* src, bool on_device, const float* extra_host, int64_t n_extra) {
@@ -200,36 +320,29 @@ void lsh_index::build(const float* src, bool on_device, const float* extra_host,
         LSH_HIP(hipMemcpyAsync(extra.get(), extra_host, sizeof(float) * (size_t)n_extra * d,
                                hipMemcpyHostToDevice, s));
     }
-
-    gj_transform = (framework == 0 && impl == 2 && dfunc == 2) || impl == 4 || impl == 5 ||
-                   (framework == 1 && (keysImpl == 4 || keysImpl == 5));
-    normalize = impl == 6 || (framework == 1 && keysImpl == 7);
-    if (gj_transform) {
-        // processingUtils.cuh:12-46 then :48-67 (memOptimizedLinearScan.cu:149-163,
-        // weightedMinHash.cuh:201-213).  The shift is frozen at build time; the
-        // one-shot entry points include the queries, as the reference does.
-        mins.alloc((size_t)d);
-        launch_fill_i32(mins.get(), (size_t)d, 0, s);
-        launch_gj_mins(data.get(), N, d, mins.get(), s);
-        if (extra.get()) launch_gj_mins(extra.get(), n_extra, d, mins.get(), s);
-        launch_gj_shift(data.get(), N, d, mins.get(), s);
-        launch_normalize_rows(data.get(), N, d, s);
-        if (extra.get()) {
-            launch_gj_shift(extra.get(), n_extra, d, mins.get(), s);
-            launch_normalize_rows(extra.get(), n_extra, d, s);
-        }
-    } else if (normalize) {
-        // simHashJL.cuh:28-32 / crossPolyTope.cuh:17-21.
-        launch_normalize_rows(data.get(), N, d, s);
+    if (framework == 1) {
+        build_lsh(extra.get(), n_extra, s);
+        LSH_HIP(hipStreamSynchronize(s));
+        return;
     }
 
+    // processingUtils.cuh:12-67 as memOptimizedLinearScan.cu:149-163 and
+    // weightedMinHash.cuh:201-213 apply it; simHashJL.cuh:28-32 normalises.
+    // The shifts are frozen at build time; the one-shot entry points include
+    // the queries, as the reference does.
+    if ((impl == 2 && dfunc == 2) || impl == 4 || impl == 5)
+        sketch_steps = one_step(kGjStep);
+    else if (impl == 6)
+        sketch_steps = one_step(kNormStep);
+    apply_steps_build(sketch_steps, data.get(), N, extra.get(), n_extra, d, s);
+
     const int kind = impl == 1 ? kNegCos : (impl == 2 ? dfunc : (impl == 4 || impl == 5 ? kGenJaccard : kAngular));
-    if (angular_kind(kind)) {
+    if (impl <= 2 && angular_kind(kind)) {
         data_inv.alloc((size_t)N);
         launch_row_inv_norms(data.get(), N, d, data_inv.get(), s);
     }
 
-    if (framework == 0 && impl == 3) {
+    if (impl == 3) {
         // simHash.cu:61-99: 32*sketchDim hyperplanes of the reference stream.
         words = sketchDim;
         wpad = round_up(std::max(words, 1), 8);
@@ -240,8 +353,7 @@ void lsh_index::build(const float* src, bool on_device, const float* extra_host,
         dsk.alloc((size_t)N * wpad);
         if (N > 0) LSH_HIP(hipMemsetAsync(dsk.get(), 0, dsk.bytes(), s));
         launch_simhash(data.get(), N, d, planesT.get(), words, 32, dsk.get(), wpad, s);
-        LSH_HIP(hipStreamSynchronize(s));
-    } else if (framework == 0 && impl == 6) {
+    } else if (impl == 6) {
         // simHashJL.cuh:238-262: planes from N(0, 1/sketchDim).
         sdim = sketchDim;
         std::vector<float> planes = normal_stream((int64_t)sdim * d, 1.0 / sdim);
@@ -250,67 +362,120 @@ void lsh_index::build(const float* src, bool on_device, const float* extra_host,
         LSH_HIP(hipMemcpyAsync(planesT.get(), pt.data(), sizeof(float) * pt.size(), hipMemcpyHostToDevice, s));
         djl.alloc((size_t)N * sdim);
         launch_jl(data.get(), N, d, planesT.get(), sdim, djl.get(), s);
-        LSH_HIP(hipStreamSynchronize(s));
-    } else if (framework == 0 && (impl == 4 || impl == 5)) {
+    } else if (impl == 4 || impl == 5) {
         // weightedMinHash.cuh:344-416: bounds, index map, sketches (the bounds
         // include the one-shot queries, as the reference computes them).
-        wmh_mode = impl == 5 ? 1 : 0;
         words = sketchDim;
         wpad = round_up(std::max(words, 1), 8);
-        build_wmh_tables(data.get(), N, extra.get(), n_extra, s);
+        wmh.build(data.get(), N, extra.get(), n_extra, d, impl == 5 ? 1 : 0, impl == 5 ? 32 * words : words, 32, s);
         dsk.alloc((size_t)N * wpad);
         if (N > 0) LSH_HIP(hipMemsetAsync(dsk.get(), 0, dsk.bytes(), s));
-        wmh_sketch_rows(data.get(), N, dsk.get(), s);
+        wmh.sketch(data.get(), N, d, dsk.get(), wpad, s);
+    }
+    LSH_HIP(hipStreamSynchronize(s));
+}
+
+// lshFramework.cuh:312-409 runLsh up to the scan: keys (generateHashes with
+// isHashKeys), bucket tables, then the sketches of the in-bucket distance.
+// Each hash family preprocesses the rows in place before hashing, exactly as
+// the reference's does (weightedMinHash.cuh:451, crossPolyTope.cuh:109,
+// simHashJL.cuh:59), so a second family sees the first one's output.
+void lsh_index::build_lsh(const float* extra_in, int64_t n_extra, hipStream_t s) {
+    float* extra = const_cast<float*>(extra_in);
+    const int L = tables;
+    LSH_REQUIRE(bucketKeyBits >= 1 && bucketKeyBits <= 24, LSH_ERR_UNSUPPORTED, "bucketKeyBits must be in 1..24");
+    LSH_REQUIRE((int64_t)L * std::max<int64_t>(N, 1) < 0xFFFFFFFFll, LSH_ERR_UNSUPPORTED,
+                "tables * N_data must be below 2^32");
+    tableSize = 1u << bucketKeyBits;
+    if (keysImpl == 4 || keysImpl == 5) key_steps = one_step(kGjStep);
+    else if (keysImpl == 7) key_steps = one_step(kNormStep);
+    apply_steps_build(key_steps, data.get(), N, extra, n_extra, d, s);
+
+    // The reference's key types (kernel.cu:67-80): unsigned short for 3, 5
+    // and 7, unsigned char for 4.
+    keyBitsMask = keysImpl == 4 ? 0xFF : 0xFFFF;
+    if (keysImpl == 3) {
+        // lshFramework.cuh:22-45: tables*d*bucketKeyBits hyperplanes; table t
+        // takes bits b < bucketKeyBits from hyperplane t*bucketKeyBits + b.
+        LSH_REQUIRE(bucketKeyBits <= 32, LSH_ERR_UNSUPPORTED, "SimHash keys take at most 32 bits");
+        std::vector<float> planes = normal_stream((int64_t)L * bucketKeyBits * d, 1.0);
+        std::vector<float> pt = transpose_planes(planes, L, bucketKeyBits, d);
+        kplanesT.alloc(pt.size());
+        LSH_HIP(hipMemcpyAsync(kplanesT.get(), pt.data(), sizeof(float) * pt.size(), hipMemcpyHostToDevice, s));
+    } else if (keysImpl == 4) {
+        // weightedMinHashGeneric with sketchDim = tables, original counts.
+        LSH_REQUIRE(bucketKeyBits >= 8, LSH_ERR_UNSUPPORTED,
+                    "weighted MinHash count keys are 8-bit: bucketKeyBits must be >= 8");
+        kwmh.build(data.get(), N, extra, n_extra, d, 0, L, 32, s);
+    } else if (keysImpl == 5) {
+        // weightedMinHashGeneric with sketchDim = tables, hashBits = bucketKeyBits.
+        kwmh.build(data.get(), N, extra, n_extra, d, 1, L * bucketKeyBits, bucketKeyBits, s);
+    } else {
+        // crossPolyTope.cuh:106-130: one cross-polytope hash per table.
+        int P = 1;
+        while (P < d) P *= 2;
+        LSH_REQUIRE(P <= 256, LSH_ERR_UNSUPPORTED, "cross-polytope keys support d <= 256");
+        LSH_REQUIRE((uint32_t)(2 * P) <= tableSize, LSH_ERR_UNSUPPORTED,
+                    "cross-polytope keys need 2^bucketKeyBits >= 2 * (d rounded up to a power of two)");
+        std::vector<int32_t> sg = sign_stream((int64_t)L * 3 * P);
+        ksigns.alloc(sg.size());
+        LSH_HIP(hipMemcpyAsync(ksigns.get(), sg.data(), sizeof(int32_t) * sg.size(), hipMemcpyHostToDevice, s));
+    }
+    dkeys.alloc((size_t)N * L);
+    lsh_keys(data.get(), N, dkeys.get(), s);
+    boff.alloc((size_t)L * (tableSize + 1));
+    bperm.alloc((size_t)L * N);
+    {
+        DevBuf<uint32_t> cursor((size_t)L * (tableSize + 1));
+        launch_bucket_build(dkeys.get(), N, L, tableSize, boff.get(), cursor.get(), bperm.get(), s);
         LSH_HIP(hipStreamSynchronize(s));
-    } else if (framework == 1) {
-        fail(LSH_ERR_UNSUPPORTED, "the (K,L) framework is not built yet");
     }
-    LSH_HIP(hipStreamSynchronize(s));
+
+    // generateHashes(isHashKeys = false), lshFramework.cuh:391.
+    if (impl == 4 || impl == 5) sketch_steps = one_step(kGjStep);
+    else if (impl == 6) sketch_steps = one_step(kNormStep);
+    apply_steps_build(sketch_steps, data.get(), N, extra, n_extra, d, s);
+    if (!withSketch || impl == 2) return;
+    if (impl == 3) {
+        words = sketchDim;
+        wpad = words;
+        std::vector<float> planes = normal_stream((int64_t)words * 32 * d, 1.0);
+        std::vector<float> pt = transpose_planes(planes, words, 32, d);
+        planesT.alloc(pt.size());
+        LSH_HIP(hipMemcpyAsync(planesT.get(), pt.data(), sizeof(float) * pt.size(), hipMemcpyHostToDevice, s));
+        dsk.alloc((size_t)N * wpad);
+        launch_simhash(data.get(), N, d, planesT.get(), words, 32, dsk.get(), wpad, s);
+    } else if (impl == 4 || impl == 5) {
+        words = sketchDim;
+        wpad = words;
+        wmh.build(data.get(), N, extra, n_extra, d, impl == 5 ? 1 : 0, impl == 5 ? 32 * words : words, 32, s);
+        dsk.alloc((size_t)N * wpad);
+        wmh.sketch(data.get(), N, d, dsk.get(), wpad, s);
+    } else {
+        sdim = sketchDim;
+        std::vector<float> planes = normal_stream((int64_t)sdim * d, 1.0 / sdim);
+        std::vector<float> pt = transpose_jl(planes, sdim, d);
+        planesT.alloc(pt.size());
+        LSH_HIP(hipMemcpyAsync(planesT.get(), pt.data(), sizeof(float) * pt.size(), hipMemcpyHostToDevice, s));
+        djl.alloc((size_t)N * sdim);
+        launch_jl(data.get(), N, d, planesT.get(), sdim, djl.get(), s);
+    }
 }
 
-void lsh_index::build_wmh_tables(const float* x, int64_t n, const float* extra, int64_t n_extra, hipStream_t s) {
-    DevBuf<int> b((size_t)d);
-    launch_fill_i32(b.get(), (size_t)d, 0, s);
-    launch_wmh_bounds(x, n, d, b.get(), s);
-    if (extra && n_extra > 0) launch_wmh_bounds(extra, n_extra, d, b.get(), s);
-    std::vector<int> hb((size_t)d);
-    LSH_HIP(hipMemcpyAsync(hb.data(), b.get(), sizeof(int) * (size_t)d, hipMemcpyDeviceToHost, s));
-    LSH_HIP(hipStreamSynchronize(s));
-    // weightedMinHash.cuh:62-90: M = sum of bounds, start = exclusive prefix
-    // sum, owner[r] = the dimension whose interval holds r.
-    std::vector<int> start((size_t)d);
-    int64_t M = 0;
-    for (int j = 0; j < d; ++j) {
-        start[j] = (int)M;
-        M += hb[j];
+void lsh_index::lsh_keys(const float* x, int64_t n, uint32_t* keys, hipStream_t s) {
+    const int L = tables;
+    if (n <= 0) return;
+    if (keysImpl == 3) {
+        launch_simhash(x, n, d, kplanesT.get(), L, bucketKeyBits, keys, L, s);
+    } else if (keysImpl == 4 || keysImpl == 5) {
+        kwmh.sketch(x, n, d, keys, L, s);
+    } else {
+        DevBuf<uint16_t> k16((size_t)n * L);
+        launch_crosspoly(x, n, d, L, ksigns.get(), k16.get(), s);
+        launch_widen_keys(k16.get(), keys, (size_t)n * L, s);
+        LSH_HIP(hipStreamSynchronize(s));
     }
-    LSH_REQUIRE(M < (int64_t)1 << 30, LSH_ERR_UNSUPPORTED, "weighted MinHash bounds sum too large");
-    wmh_M = (int)M;
-    std::vector<int> owner((size_t)std::max<int64_t>(M, 1), 0);
-    for (int j = 0; j < d; ++j)
-        for (int r = start[j]; r < start[j] + hb[j]; ++r) owner[r] = j;
-    wmh_owner.alloc(owner.size());
-    wmh_start.alloc((size_t)d);
-    LSH_HIP(hipMemcpyAsync(wmh_owner.get(), owner.data(), sizeof(int) * owner.size(), hipMemcpyHostToDevice, s));
-    LSH_HIP(hipMemcpyAsync(wmh_start.get(), start.data(), sizeof(int) * (size_t)d, hipMemcpyHostToDevice, s));
-    // weightedMinHash.cuh:231-250 generates 255 bits of the default-seeded
-    // stream; the table continues that stream so counts past 254 stay defined.
-    std::vector<uint8_t> bm = bit_stream(4096);
-    wmh_bitmap_len = (int)bm.size();
-    wmh_bitmap.alloc(bm.size());
-    LSH_HIP(hipMemcpyAsync(wmh_bitmap.get(), bm.data(), bm.size(), hipMemcpyHostToDevice, s));
-    const int nhash = wmh_mode == 1 ? 32 * words : words;
-    wmh_R = 256;
-    wmh_draws.alloc((size_t)nhash * wmh_R);
-    wmh_draw_owner.alloc((size_t)nhash * wmh_R);
-    launch_wmh_draws(nhash, wmh_R, wmh_M, wmh_owner.get(), wmh_draws.get(), wmh_draw_owner.get(), s);
-    LSH_HIP(hipStreamSynchronize(s));
-}
-
-void lsh_index::wmh_sketch_rows(const float* x, int64_t n, uint32_t* out, hipStream_t s) {
-    const int nhash = wmh_mode == 1 ? 32 * words : words;
-    launch_wmh_sketch(x, n, d, wmh_owner.get(), wmh_start.get(), wmh_M, nhash, wmh_mode, wmh_bitmap.get(),
-                      wmh_bitmap_len, wmh_draws.get(), wmh_draw_owner.get(), wmh_R, out, wpad, s);
+    launch_mask_keys(keys, (size_t)n * L, (uint32_t)keyBitsMask, s);
 }
 
 void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh_point_t* results,
@@ -324,23 +489,21 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
     scan_launches = 0;
     mfma_launches = 0;
     mfma_points = 0;
+    last_candidates = 0;
     if (Q == 0) return;
     pt.begin(PH_TOTAL, s);
     pt.begin(PH_PREP, s);
     q.ensure((size_t)Q * d);
     LSH_HIP(hipMemcpyAsync(q.get(), queries, sizeof(float) * (size_t)Q * d,
                            q_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
-    if (gj_transform) {
-        launch_gj_shift(q.get(), Q, d, mins.get(), s);
-        launch_normalize_rows(q.get(), Q, d, s);
-    } else if (normalize) {
-        launch_normalize_rows(q.get(), Q, d, s);
-    }
+    if (framework == 0) apply_steps_query(sketch_steps, q.get(), Q, d, s);
     pt.end(PH_PREP, s);
     out.ensure((size_t)Q * kq);
     flags.ensure((size_t)Q);
 
-    if (framework == 0 && (impl == 1 || impl == 2)) {
+    if (framework == 1) {
+        search_lsh(Q, kq, s, pt);
+    } else if (impl == 1 || impl == 2) {
         const int kind = impl == 1 ? kNegCos : dfunc;
         LSH_REQUIRE(kq <= kMaxTopK, LSH_ERR_UNSUPPORTED, "exact scan supports k <= 4096");
         int K = std::min(kMaxTopK, pow2_ceil(kq + margin));
@@ -372,13 +535,13 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
             launch_scatter_rows_pts(sout.get(), kq, idx.get(), m, out.get(), s);
             LSH_HIP(hipStreamSynchronize(s));
         }
-    } else if (framework == 0 && (impl == 3 || impl == 5 || impl == 4)) {
+    } else if (impl >= 3 && impl <= 5) {
         const int Qpad = sketch_query_rows(Q);
         qsk.ensure((size_t)Qpad * wpad);
         pt.begin(PH_PREP, s);
         LSH_HIP(hipMemsetAsync(qsk.get(), 0, sizeof(uint32_t) * (size_t)Qpad * wpad, s));
         if (impl == 3) launch_simhash(q.get(), Q, d, planesT.get(), words, 32, qsk.get(), wpad, s);
-        else wmh_sketch_rows(q.get(), Q, qsk.get(), s);
+        else wmh.sketch(q.get(), Q, d, qsk.get(), wpad, s);
         pt.end(PH_PREP, s);
         const int Ceff = (int)std::max<int64_t>(1, std::min<int64_t>(candidates, N));
         const int K = Ceff <= kMaxIntTopK ? Ceff : pow2_ceil(Ceff);
@@ -390,7 +553,7 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
         launch_rerank(data.get(), q.get(), d, kind, st.view().keys, st.view().stride, nullptr, Ceff, Q, kq,
                       kq + margin, fp32_error_bound(kind, d), out.get(), s);
         pt.end(PH_FINAL, s);
-    } else if (framework == 0 && impl == 6) {
+    } else {
         qjl.ensure((size_t)Q * sdim);
         pt.begin(PH_PREP, s);
         launch_jl(q.get(), Q, d, planesT.get(), sdim, qjl.get(), s);
@@ -404,8 +567,6 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
         launch_rerank(data.get(), q.get(), d, kAngular, st.view().keys, st.view().stride, nullptr, Ceff, Q, kq,
                       kq + margin, fp32_error_bound(kAngular, d), out.get(), s);
         pt.end(PH_FINAL, s);
-    } else {
-        fail(LSH_ERR_UNSUPPORTED, "this implementation / framework combination is not built yet");
     }
 
     LSH_HIP(hipMemcpyAsync(results, out.get(), sizeof(lsh_point_t) * (size_t)Q * kq,
@@ -415,6 +576,104 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
     pt.collect(phase_ms, 6);
     phase_ms[6] = (double)mfma_launches;
     phase_ms[7] = (double)mfma_points;
+}
+
+// lshFramework.cuh:142-280 scan + removeDuplicates over the unique candidate
+// set of each query: either all of it ranked by the exact distance
+// (distanceFunc), or its top-C by sketch distance reranked exactly
+// (runWithSketchedData, candidateSetScanner.cuh:10-32).
+void lsh_index::search_lsh(int Q, int kq, hipStream_t s, PhaseTimer& pt) {
+    const int L = tables;
+    pt.begin(PH_PREP, s);
+    apply_steps_query(key_steps, q.get(), Q, d, s);
+    qkeys.ensure((size_t)Q * L);
+    lsh_keys(q.get(), Q, qkeys.get(), s);
+    apply_steps_query(sketch_steps, q.get(), Q, d, s);
+    qlo.ensure((size_t)Q * L);
+    qpre.ensure((size_t)Q * (L + 1));
+    qmax.ensure(1);
+    launch_fill_i32(reinterpret_cast<int32_t*>(qmax.get()), 1, 0, s);
+    launch_query_ranges(qkeys.get(), Q, L, tableSize, boff.get(), qlo.get(), qpre.get(), qmax.get(), s);
+    const bool sketched = withSketch && impl >= 3;
+    if (sketched) {
+        if (impl == 6) {
+            qjl.ensure((size_t)Q * sdim);
+            launch_jl(q.get(), Q, d, planesT.get(), sdim, qjl.get(), s);
+        } else {
+            qsk.ensure((size_t)Q * wpad);
+            if (impl == 3) launch_simhash(q.get(), Q, d, planesT.get(), words, 32, qsk.get(), wpad, s);
+            else wmh.sketch(q.get(), Q, d, qsk.get(), wpad, s);
+        }
+    }
+    uint32_t maxS = 0;
+    LSH_HIP(hipMemcpyAsync(&maxS, qmax.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    LSH_HIP(hipStreamSynchronize(s));
+    pt.end(PH_PREP, s);
+
+    LshScanArgs a{};
+    a.perm = bperm.get();
+    a.dkeys = dkeys.get();
+    a.qkeys = qkeys.get();
+    a.qlo = qlo.get();
+    a.qpre = qpre.get();
+    a.qmap = nullptr;
+    a.L = L;
+    a.Q = Q;
+    a.data = data.get();
+    a.queries = q.get();
+    a.d = d;
+    a.dsk = dsk.get();
+    a.qsk = qsk.get();
+    a.words = wpad;
+    a.djl = djl.get();
+    a.qjl = qjl.get();
+    a.sdim = sdim;
+    const std::vector<int32_t> none;
+    const int kind = dfunc;  // candidateSetScan / runDistanceFunction use distanceFunc
+    const double eps = fp32_error_bound(kind, d);
+    if (sketched) {
+        a.sketch = true;
+        a.kind = impl == 6 ? kSqL2 : (impl == 4 ? 1 : 0);
+        const bool int_keys = impl != 6;
+        const int Ceff = (int)std::max<int64_t>(1, std::min<int64_t>(candidates, N));
+        const int K = int_keys && Ceff <= kMaxIntTopK ? Ceff : pow2_ceil(Ceff);
+        LSH_REQUIRE(K <= kMaxTopK, LSH_ERR_UNSUPPORTED, "candidate sets larger than 4096 are not supported");
+        lsh_topk(a, none, maxS, K, false, int_keys && K <= kMaxIntTopK, st, s, pt, scan_launches);
+        pt.begin(PH_FINAL, s);
+        launch_rerank(data.get(), q.get(), d, kind, st.view().keys, st.view().stride, nullptr, Ceff, Q, kq,
+                      kq + margin, eps, out.get(), s);
+        pt.end(PH_FINAL, s);
+        return;
+    }
+    a.sketch = false;
+    a.kind = kind;
+    LSH_REQUIRE(kq <= kMaxTopK, LSH_ERR_UNSUPPORTED, "k must be <= 4096");
+    const int K = std::min(kMaxTopK, pow2_ceil(kq + margin));
+    lsh_topk(a, none, maxS, K, false, false, st, s, pt, scan_launches);
+    pt.begin(PH_FINAL, s);
+    launch_exact_finalize(data.get(), q.get(), d, kind, st.view(), kq, eps, out.get(), flags.get(), s);
+    pt.end(PH_FINAL, s);
+    std::vector<int32_t> bad = flagged(flags.get(), Q, s);
+    if (bad.empty()) return;
+    LSH_REQUIRE(K < kMaxTopK, LSH_ERR_UNSUPPORTED,
+                "more than 4096 candidates within the FP32 error band of the k-th distance");
+    const int m = (int)bad.size();
+    DevBuf<int32_t> idx((size_t)m);
+    LSH_HIP(hipMemcpyAsync(idx.get(), bad.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, s));
+    DevBuf<float> sub((size_t)m * d);
+    launch_gather_rows_f32(q.get(), d, idx.get(), m, sub.get(), s);
+    LshScanArgs b = a;
+    b.qmap = idx.get();
+    b.Q = m;
+    TopKState sst;
+    lsh_topk(b, bad, maxS, kMaxTopK, false, false, sst, s, pt, scan_launches);
+    DevBuf<lsh_point_t> sout((size_t)m * kq);
+    DevBuf<uint32_t> sflags((size_t)m);
+    launch_exact_finalize(data.get(), sub.get(), d, kind, sst.view(), kq, eps, sout.get(), sflags.get(), s);
+    LSH_REQUIRE(flagged(sflags.get(), m, s).empty(), LSH_ERR_UNSUPPORTED,
+                "more than 4096 candidates within the FP32 error band of the k-th distance");
+    launch_scatter_rows_pts(sout.get(), kq, idx.get(), m, out.get(), s);
+    LSH_HIP(hipStreamSynchronize(s));
 }
 
 // ---------------------------------------------------------- stage helpers
@@ -519,18 +778,12 @@ void hamming_topc_host(const uint32_t* ds, int64_t N, const uint32_t* qs, int64_
 void wmh_host(const float* x, int64_t n, int d, int sketchDim, int one_bit, uint32_t* out, int device) {
     DeviceGuard g(device);
     LSH_REQUIRE(n >= 0 && d > 0 && sketchDim > 0, LSH_ERR_INVALID, "bad weighted MinHash arguments");
-    lsh_index ix;
-    ix.d = d;
-    ix.N = n;
-    ix.device = device;
-    ix.wmh_mode = one_bit ? 1 : 0;
-    ix.words = sketchDim;
-    ix.wpad = sketchDim;
     DevBuf<float> dx((size_t)n * d);
     DevBuf<uint32_t> dout((size_t)n * sketchDim);
     LSH_HIP(hipMemcpy(dx.get(), x, sizeof(float) * (size_t)n * d, hipMemcpyHostToDevice));
-    ix.build_wmh_tables(dx.get(), n, nullptr, 0, nullptr);
-    ix.wmh_sketch_rows(dx.get(), n, dout.get(), nullptr);
+    WmhHasher h;
+    h.build(dx.get(), n, nullptr, 0, d, one_bit ? 1 : 0, one_bit ? 32 * sketchDim : sketchDim, 32, nullptr);
+    h.sketch(dx.get(), n, d, dout.get(), sketchDim, nullptr);
     LSH_HIP(hipDeviceSynchronize());
     LSH_HIP(hipMemcpy(out, dout.get(), sizeof(uint32_t) * (size_t)n * sketchDim, hipMemcpyDeviceToHost));
 }

@@ -3,6 +3,45 @@
 
 #include "lshgpu_internal.hpp"
 
+namespace lshgpu {
+
+// One preprocessing step applied in place to data and queries.
+// kGjStep: shift by |min(0, floor(min))| per dimension, then unit L2 rows
+// (processingUtils.cuh:12-67); kNormStep: unit L2 rows only.
+enum StepKind : int { kGjStep = 0, kNormStep = 1 };
+struct Transform {
+    int kind = kNormStep;
+    DevBuf<int> mins;  // kGjStep: per-dimension floor minimum (frozen at build)
+};
+
+// Weighted MinHash tables (weightedMinHash.cuh:41-90, 231-262) and the
+// sketch of rows with them.  mode 0: one red count per hash; mode 1: words of
+// hbits bits, bit b of word w from hash w*hbits + b.
+struct WmhHasher {
+    int mode = 0;
+    int nhash = 0;
+    int hbits = 32;
+    int M = 0;
+    int R = 0;
+    int bitmap_len = 0;
+    DevBuf<int> owner;       // [M] dimension of each unit of mass
+    DevBuf<int> start;       // [d] first unit of each dimension
+    DevBuf<uint8_t> bitmap;  // bit of each red count (one-bit mode)
+    DevBuf<float> draws;     // [nhash][R] first draws r = M*U of each hash
+    DevBuf<int> draw_owner;  // [nhash][R] dimension owning floor(r)
+
+    // Bounds from the rows of x (and extra, when given).
+    void build(const float* x, int64_t n, const float* extra, int64_t n_extra, int d, int mode_, int nhash_,
+               int hbits_, hipStream_t s);
+    // out[i][*] with row stride out_stride (words for mode 1, hashes for mode 0).
+    void sketch(const float* x, int64_t n, int d, uint32_t* out, int out_stride, hipStream_t s) const;
+    size_t bytes() const {
+        return owner.bytes() + start.bytes() + bitmap.bytes() + draws.bytes() + draw_owner.bytes();
+    }
+};
+
+}  // namespace lshgpu
+
 struct lsh_index {
     // Resolved configuration.
     int impl = 2;
@@ -26,9 +65,6 @@ struct lsh_index {
     // Working copy of the data after the path's preprocessing.
     lshgpu::DevBuf<float> data;
     lshgpu::DevBuf<float> data_inv;  // 1/||x|| for the angular kinds
-    bool gj_transform = false;       // shift + normalise (generalized Jaccard, weighted MinHash)
-    bool normalize = false;          // unit rows (SimHash-JL, cross-polytope keys)
-    lshgpu::DevBuf<int> mins;        // shift of the generalized-Jaccard transform
 
     // Sketches.
     int words = 0;   // uint32 words per point
@@ -38,16 +74,20 @@ struct lsh_index {
     int sdim = 0;                    // JL sketch length
     lshgpu::DevBuf<float> djl;       // [N][sdim]
 
-    // Weighted MinHash.
-    int wmh_M = 0;
-    int wmh_mode = 0;                // 0 counts (impl 4), 1 one-bit words (impl 5)
-    lshgpu::DevBuf<int> wmh_owner;   // [M] component of each unit of mass
-    lshgpu::DevBuf<int> wmh_start;   // [d] first unit of each component
-    lshgpu::DevBuf<uint8_t> wmh_bitmap;
-    int wmh_bitmap_len = 0;
-    lshgpu::DevBuf<float> wmh_draws;     // [hashes][R] draw values r = M*U
-    lshgpu::DevBuf<int> wmh_draw_owner;  // [hashes][R] owner component of each draw
-    int wmh_R = 0;
+    // Weighted MinHash hashers: `wmh` for the sketches, `kwmh` for bucket keys.
+    lshgpu::WmhHasher wmh, kwmh;
+
+    // (K,L) bucket tables (framework 1).
+    int keyBitsMask = 0;                  // key width of the reference's key type (8 or 16 bits)
+    uint32_t tableSize = 0;               // 2^bucketKeyBits
+    std::vector<lshgpu::Transform> key_steps, sketch_steps;  // preprocessing before keys / sketches
+    lshgpu::DevBuf<float> kplanesT;       // SimHash key hyperplanes [tables][d][bucketKeyBits]
+    lshgpu::DevBuf<int32_t> ksigns;       // cross-polytope diagonals [tables][3][P]
+    lshgpu::DevBuf<uint32_t> dkeys;       // [N][tables]
+    lshgpu::DevBuf<uint32_t> boff;        // [tables][tableSize+1] absolute positions into bperm
+    lshgpu::DevBuf<uint32_t> bperm;       // [tables][N] point IDs grouped by key
+    lshgpu::DevBuf<uint32_t> qkeys, qlo, qpre, qmax;
+    int64_t last_candidates = 0;          // sum over queries of bucket entries walked (last search)
 
     // Per-search scratch (grow-only).
     lshgpu::DevBuf<float> q, qT, q_inv, qjl, qjlT;
@@ -60,11 +100,14 @@ struct lsh_index {
     int64_t mfma_points = 0;     // data points those launches covered
 
     size_t device_bytes() const;
-    void build_wmh_tables(const float* x, int64_t n, const float* extra, int64_t n_extra, hipStream_t s);
-    void wmh_sketch_rows(const float* x, int64_t n, uint32_t* out, hipStream_t s);
     void build(const float* src, bool on_device, const float* extra_host, int64_t n_extra);
     void search(const float* queries, int Q, int k, bool q_on_device, lsh_point_t* results,
                 bool r_on_device, hipStream_t s);
+
+   private:
+    void build_lsh(const float* extra, int64_t n_extra, hipStream_t s);
+    void lsh_keys(const float* x, int64_t n, uint32_t* keys, hipStream_t s);
+    void search_lsh(int Q, int kq, hipStream_t s, lshgpu::PhaseTimer& pt);
 };
 
 namespace lshgpu {

@@ -1,0 +1,377 @@
+// lsh.hip -- (K,L) bucket tables and the scan of a query's candidate set
+// (lshFramework.cuh:102-280).
+//
+// Build: for every table a counting sort of the point IDs by bucket key.  The
+// reference counts with atomics, scans the counts serially per table and
+// scatters with one warp per table (lshFramework.cuh:102-140); here the count
+// runs over (point, table) pairs with per-workgroup LDS histograms when the
+// table is small, one workgroup per table computes the exclusive scan, and the
+// scatter reserves slots with atomics.  off[t][key] is the absolute start of
+// the bucket in perm, as the reference's hashKeys array holds it
+// (lshFramework.cuh:116-124), with one sentinel entry per table so that the
+// end of the last bucket needs no special case (:183-184).
+//
+// Query: a query's candidates are the points of its L buckets.  They form a
+// virtual list of positions 0 .. S-1 (bucket of table 0, then table 1, ...);
+// a scan round covers positions [p0, p1) of every query.  A point that shares
+// the query's key in several tables appears several times in that list; it is
+// scored only at its first table (a table t' < t with equal keys means it was
+// already seen), which visits the union once without any shared state.  The
+// reference instead scores every occurrence and drops adjacent duplicates
+// after the final sort (lshFramework.cuh:255-280).
+#include "device_common.hpp"
+
+namespace lshgpu {
+
+namespace {
+
+constexpr int kTPB = 256;
+constexpr int kItems = 4;                 // positions per thread per launch tile
+constexpr int kTile = kTPB * kItems;      // positions per workgroup
+constexpr uint32_t kLdsHistMax = 16384;   // table sizes counted in LDS
+
+unsigned grid_for(size_t n, int tpb) {
+    size_t b = (n + tpb - 1) / tpb;
+    return (unsigned)std::min<size_t>(std::max<size_t>(b, 1), 65535u * 16u);
+}
+
+__global__ void mask_keys_kernel(uint32_t* __restrict__ k, size_t n, uint32_t mask) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        k[i] &= mask;
+}
+
+__global__ void widen_keys_kernel(const uint16_t* __restrict__ in, uint32_t* __restrict__ out, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = in[i];
+}
+
+// Counts of small tables: grid.y = table, each workgroup a slice of points.
+__global__ __launch_bounds__(kTPB) void bucket_count_lds_kernel(const uint32_t* __restrict__ keys, int64_t N,
+                                                                int L, uint32_t T1, int64_t per_block,
+                                                                uint32_t* __restrict__ cnt) {
+    extern __shared__ uint32_t hist[];
+    const int t = blockIdx.y;
+    for (uint32_t j = threadIdx.x; j < T1; j += kTPB) hist[j] = 0;
+    __syncthreads();
+    const int64_t i0 = (int64_t)blockIdx.x * per_block;
+    const int64_t i1 = min(N, i0 + per_block);
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += kTPB) {
+        const uint32_t key = keys[(size_t)i * L + t];
+        if (key < T1 - 1) atomicAdd(&hist[key], 1u);  // keys are below T by construction
+    }
+    __syncthreads();
+    uint32_t* row = cnt + (size_t)t * T1;
+    for (uint32_t j = threadIdx.x; j < T1; j += kTPB)
+        if (hist[j]) atomicAdd(&row[j], hist[j]);
+}
+
+__global__ void bucket_count_kernel(const uint32_t* __restrict__ keys, int64_t N, int L, uint32_t T1,
+                                    uint32_t* __restrict__ cnt) {
+    const int64_t total = N * L;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int t = (int)(i % L);
+        const uint32_t key = keys[i];
+        if (key < T1 - 1) atomicAdd(&cnt[(size_t)t * T1 + key], 1u);
+    }
+}
+
+// One workgroup of 1024 per table: in-place exclusive scan of T1 counts
+// (the last is zero) plus the table's base t*N.
+__global__ __launch_bounds__(1024) void bucket_offsets_kernel(uint32_t* __restrict__ a, int64_t N, uint32_t T1) {
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t total_s;
+    uint32_t* row = a + (size_t)blockIdx.x * T1;
+    uint32_t carry = (uint32_t)((int64_t)blockIdx.x * N);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (uint32_t c0 = 0; c0 < T1; c0 += 4096) {
+        uint32_t v[4];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const uint32_t j = c0 + threadIdx.x * 4 + e;
+            v[e] = j < T1 ? row[j] : 0u;
+            sum += v[e];
+        }
+        uint32_t inc = sum;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(inc, o);
+            if (lane >= o) inc += t;
+        }
+        if (lane == 63) wsum[w] = inc;
+        __syncthreads();
+        if (w == 0) {
+            const uint32_t x = lane < 16 ? wsum[lane] : 0u;
+            uint32_t xi = x;
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+                const uint32_t t = __shfl_up(xi, o);
+                if (lane >= o) xi += t;
+            }
+            if (lane < 16) wsum[lane] = xi - x;
+            if (lane == 15) total_s = xi;
+        }
+        __syncthreads();
+        uint32_t base = carry + wsum[w] + inc - sum;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const uint32_t j = c0 + threadIdx.x * 4 + e;
+            if (j < T1) row[j] = base;
+            base += v[e];
+        }
+        carry += total_s;
+        __syncthreads();
+    }
+}
+
+__global__ void bucket_scatter_kernel(const uint32_t* __restrict__ keys, int64_t N, int L, uint32_t T1,
+                                      uint32_t* __restrict__ cursor, uint32_t* __restrict__ perm) {
+    const int64_t total = N * L;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int t = (int)(i % L);
+        const uint32_t key = keys[i];
+        if (key >= T1 - 1) continue;
+        const uint32_t pos = atomicAdd(&cursor[(size_t)t * T1 + key], 1u);
+        perm[pos] = (uint32_t)(i / L);
+    }
+}
+
+__global__ void query_ranges_kernel(const uint32_t* __restrict__ qkeys, int Q, int L, uint32_t T,
+                                    const uint32_t* __restrict__ off, uint32_t* __restrict__ lo,
+                                    uint32_t* __restrict__ pre, uint32_t* __restrict__ maxS) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Q) return;
+    uint32_t acc = 0;
+    pre[(size_t)q * (L + 1)] = 0;
+    for (int t = 0; t < L; ++t) {
+        const uint32_t key = qkeys[(size_t)q * L + t];
+        const uint32_t* o = off + (size_t)t * (T + 1);
+        const uint32_t a = key < T ? o[key] : o[T];
+        const uint32_t b = key < T ? o[key + 1] : o[T];
+        lo[(size_t)q * L + t] = a;
+        acc += b - a;
+        pre[(size_t)q * (L + 1) + t + 1] = acc;
+    }
+    atomicMax(maxS, acc);
+}
+
+// Position of the n-th (from 0) set bit of m; m has more than n set bits.
+__device__ inline int nth_set_bit(uint64_t m, int n) {
+    int pos = 0;
+#pragma unroll
+    for (int w = 32; w > 0; w >>= 1) {
+        const uint64_t low = m & ((1ull << w) - 1ull);
+        const int c = __popcll(low);
+        if (n >= c) {
+            n -= c;
+            m >>= w;
+            pos += w;
+        } else {
+            m = low;
+        }
+    }
+    return pos;
+}
+
+// MODE 0: exact distance (kind), 16-lane groups per candidate row.
+// MODE 1: integer sketch distance (kind 0 Hamming, 1 unequal words).
+// MODE 2: squared L2 over JL sketches.
+template <int MODE>
+__global__ __launch_bounds__(kTPB) void lsh_scan_kernel(LshScanArgs a, TopKView v) {
+    extern __shared__ uint32_t lds[];
+    const int L = a.L;
+    uint32_t* s_pre = lds;              // L+1
+    uint32_t* s_lo = lds + (L + 1);     // L
+    uint32_t* s_qk = s_lo + L;          // L
+    float* s_qf = reinterpret_cast<float*>(s_qk + L);  // query vector (d / sdim floats)
+    uint32_t* s_qw = s_qk + L;                          // query sketch (words)
+    const int lane = threadIdx.x & 63;
+
+    for (int r = blockIdx.y; r < a.Q; r += gridDim.y) {
+        const int q = a.qmap ? a.qmap[r] : r;
+        const uint32_t* pre = a.qpre + (size_t)q * (L + 1);
+        const int64_t S = pre[L];
+        const int64_t start = a.p0 + (int64_t)blockIdx.x * kTile;
+        const int64_t end = min(min(a.p1, S), start + (int64_t)kTile);
+        if (start >= end) continue;  // uniform over the workgroup
+        __syncthreads();             // previous row's LDS reads are done
+        for (int t = threadIdx.x; t <= L; t += kTPB) s_pre[t] = pre[t];
+        for (int t = threadIdx.x; t < L; t += kTPB) {
+            s_lo[t] = a.qlo[(size_t)q * L + t];
+            s_qk[t] = a.qkeys[(size_t)q * L + t];
+        }
+        if (MODE == 0) {
+            for (int j = threadIdx.x; j < a.d; j += kTPB) s_qf[j] = a.queries[(size_t)q * a.d + j];
+        } else if (MODE == 1) {
+            for (int j = threadIdx.x; j < a.words; j += kTPB) s_qw[j] = a.qsk[(size_t)q * a.words + j];
+        } else {
+            for (int j = threadIdx.x; j < a.sdim; j += kTPB) s_qf[j] = a.qjl[(size_t)q * a.sdim + j];
+        }
+        __syncthreads();
+        const uint64_t thr = v.thr[r];
+        float qn = 0.0f;
+        if (MODE == 0 && (a.kind == kAngular || a.kind == kNegCos)) {
+            for (int j = 0; j < a.d; ++j) qn = __builtin_fmaf(s_qf[j], s_qf[j], qn);
+            qn = __builtin_sqrtf(qn);
+        }
+        for (int64_t base = start; base < end; base += kTPB) {
+            const int64_t pos = base + threadIdx.x;
+            bool owned = false;
+            uint32_t id = 0;
+            if (pos < end) {
+                int lo = 0, hi = L;  // pre[lo] <= pos < pre[hi]
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if ((int64_t)s_pre[mid] <= pos) lo = mid;
+                    else hi = mid;
+                }
+                id = a.perm[s_lo[lo] + (uint32_t)(pos - s_pre[lo])];
+                owned = true;
+                const uint32_t* dk = a.dkeys + (size_t)id * L;
+                for (int t2 = 0; t2 < lo; ++t2)
+                    if (dk[t2] == s_qk[t2]) {
+                        owned = false;
+                        break;
+                    }
+            }
+            if (MODE == 0) {
+                const uint64_t m = __ballot(owned);
+                const int n = __popcll(m);
+                const int g = lane >> 4, gl = lane & 15;
+                for (int it = 0; it * 4 < n; ++it) {
+                    const int j = it * 4 + g;
+                    const bool has = j < n;
+                    const int src = has ? nth_set_bit(m, j) : 0;
+                    const uint32_t cid = (uint32_t)__shfl((int)id, src);
+                    float x = 0.0f, y = 0.0f;
+                    if (has) {
+                        const float* __restrict__ row = a.data + (size_t)cid * a.d;
+                        if ((a.d & 3) == 0) {
+                            for (int jj = gl * 4; jj < a.d; jj += 64) {
+                                const float4 xv = *reinterpret_cast<const float4*>(row + jj);
+                                group_accumulate(a.kind, xv.x, s_qf[jj], x, y);
+                                group_accumulate(a.kind, xv.y, s_qf[jj + 1], x, y);
+                                group_accumulate(a.kind, xv.z, s_qf[jj + 2], x, y);
+                                group_accumulate(a.kind, xv.w, s_qf[jj + 3], x, y);
+                            }
+                        } else {
+                            for (int jj = gl; jj < a.d; jj += 16) group_accumulate(a.kind, row[jj], s_qf[jj], x, y);
+                        }
+                    }
+#pragma unroll
+                    for (int o = 8; o > 0; o >>= 1) {
+                        x += __shfl_xor(x, o, 16);
+                        y += __shfl_xor(y, o, 16);
+                    }
+                    const float dist = group_distance(a.kind, x, y, qn);
+                    const uint64_t key = make_key(f32_key(dist), cid);
+                    append_key(v, r, has && gl == 0 && key < thr, key);
+                }
+            } else if (MODE == 1) {
+                uint32_t dist = 0;
+                if (owned) {
+                    const uint32_t* __restrict__ row = a.dsk + (size_t)id * a.words;
+                    if (a.kind == 0) {
+                        for (int w = 0; w < a.words; ++w) dist += __popc(row[w] ^ s_qw[w]);
+                    } else {
+                        for (int w = 0; w < a.words; ++w) dist += row[w] != s_qw[w] ? 1u : 0u;
+                    }
+                }
+                const uint64_t key = make_key(dist, id);
+                append_key(v, r, owned && key < thr, key);
+            } else {
+                float dist = 0.0f;
+                if (owned) {
+                    const float* __restrict__ row = a.djl + (size_t)id * a.sdim;
+                    for (int j = 0; j < a.sdim; ++j) {
+                        const float t = s_qf[j] - row[j];
+                        dist = __builtin_fmaf(t, t, dist);
+                    }
+                }
+                const uint64_t key = make_key(f32_key(dist), id);
+                append_key(v, r, owned && key < thr, key);
+            }
+        }
+    }
+}
+
+}  // namespace
+
+void launch_mask_keys(uint32_t* keys, size_t n, uint32_t mask, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(mask_keys_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, keys, n, mask);
+    check_launch("mask_keys_kernel");
+}
+
+void launch_widen_keys(const uint16_t* in, uint32_t* out, size_t n, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(widen_keys_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, in, out, n);
+    check_launch("widen_keys_kernel");
+}
+
+void launch_bucket_build(const uint32_t* keys, int64_t N, int L, uint32_t T, uint32_t* off, uint32_t* cursor,
+                         uint32_t* perm, hipStream_t s) {
+    const uint32_t T1 = T + 1;
+    const size_t nrow = (size_t)L * T1;
+    launch_fill_i32(reinterpret_cast<int32_t*>(off), nrow, 0, s);
+    if (N > 0) {
+        if (T1 <= kLdsHistMax) {
+            const int64_t per = std::max<int64_t>(4096, (N + 1023) / 1024);
+            const unsigned gx = (unsigned)((N + per - 1) / per);
+            const size_t lds = sizeof(uint32_t) * T1;
+            set_dynamic_lds((const void*)bucket_count_lds_kernel, lds);
+            hipLaunchKernelGGL(bucket_count_lds_kernel, dim3(gx, (unsigned)L), dim3(kTPB), lds, s, keys, N, L, T1,
+                               per, off);
+            check_launch("bucket_count_lds_kernel");
+        } else {
+            hipLaunchKernelGGL(bucket_count_kernel, dim3(grid_for((size_t)N * L, 256)), dim3(256), 0, s, keys, N,
+                               L, T1, off);
+            check_launch("bucket_count_kernel");
+        }
+    }
+    hipLaunchKernelGGL(bucket_offsets_kernel, dim3((unsigned)L), dim3(1024), 0, s, off, N, T1);
+    check_launch("bucket_offsets_kernel");
+    if (N > 0) {
+        LSH_HIP(hipMemcpyAsync(cursor, off, sizeof(uint32_t) * nrow, hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(bucket_scatter_kernel, dim3(grid_for((size_t)N * L, 256)), dim3(256), 0, s, keys, N, L,
+                           T1, cursor, perm);
+        check_launch("bucket_scatter_kernel");
+    }
+}
+
+void launch_query_ranges(const uint32_t* qkeys, int Q, int L, uint32_t T, const uint32_t* off, uint32_t* lo,
+                         uint32_t* pre, uint32_t* maxS, hipStream_t s) {
+    if (Q <= 0) return;
+    hipLaunchKernelGGL(query_ranges_kernel, dim3((unsigned)((Q + 255) / 256)), dim3(256), 0, s, qkeys, Q, L, T,
+                       off, lo, pre, maxS);
+    check_launch("query_ranges_kernel");
+}
+
+void launch_lsh_scan(const LshScanArgs& a, const TopKView& v, hipStream_t s) {
+    if (a.p1 <= a.p0 || a.Q <= 0) return;
+    const int64_t tiles = (a.p1 - a.p0 + kTile - 1) / kTile;
+    LSH_REQUIRE(tiles < (int64_t)1 << 31, LSH_ERR_UNSUPPORTED, "candidate range too large");
+    const dim3 grid((unsigned)tiles, (unsigned)std::min(a.Q, 65535));
+    const int mode = !a.sketch ? 0 : (a.kind == kSqL2 ? 2 : 1);
+    const int qlen = mode == 0 ? a.d : (mode == 1 ? a.words : a.sdim);
+    const size_t lds = sizeof(uint32_t) * ((size_t)3 * a.L + 1 + (size_t)qlen);
+    switch (mode) {
+        case 0:
+            set_dynamic_lds((const void*)lsh_scan_kernel<0>, lds);
+            hipLaunchKernelGGL(lsh_scan_kernel<0>, grid, dim3(kTPB), lds, s, a, v);
+            break;
+        case 1:
+            set_dynamic_lds((const void*)lsh_scan_kernel<1>, lds);
+            hipLaunchKernelGGL(lsh_scan_kernel<1>, grid, dim3(kTPB), lds, s, a, v);
+            break;
+        default:
+            set_dynamic_lds((const void*)lsh_scan_kernel<2>, lds);
+            hipLaunchKernelGGL(lsh_scan_kernel<2>, grid, dim3(kTPB), lds, s, a, v);
+            break;
+    }
+    check_launch("lsh_scan_kernel");
+}
+
+}  // namespace lshgpu

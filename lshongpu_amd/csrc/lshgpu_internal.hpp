@@ -253,9 +253,10 @@ void launch_hamming_mfma(const uint32_t* ds, const uint32_t* qs, int words, int 
 int wmh_max_draws();
 void launch_wmh_bounds(const float* x, int64_t n, int d, int* bounds, hipStream_t s);
 void launch_wmh_draws(int nhash, int R, int M, const int* owner, float* draws, int* downer, hipStream_t s);
+// mode 0: one count per hash; mode 1: words of hbits bits (nhash = words*hbits).
 void launch_wmh_sketch(const float* x, int64_t n, int d, const int* owner, const int* start, int M, int nhash,
-                       int mode, const uint8_t* bitmap, int bitmap_len, const float* draws, const int* draw_owner,
-                       int R, uint32_t* out, int out_stride, hipStream_t s);
+                       int mode, int hbits, const uint8_t* bitmap, int bitmap_len, const float* draws,
+                       const int* draw_owner, int R, uint32_t* out, int out_stride, hipStream_t s);
 
 // util.hip
 void launch_gather_rows_f32(const float* src, int cols, const int32_t* idx, int m, float* dst,
@@ -269,6 +270,49 @@ void launch_scatter_rows_pts(const lsh_point_t* src, int cols, const int32_t* id
 void launch_pad_rows_u32(const uint32_t* src, int64_t n, int cols, int64_t npad, int pcols,
                          uint32_t* dst, hipStream_t s);
 void launch_fill_i32(int32_t* p, size_t n, int32_t v, hipStream_t s);
+
+// lsh.hip: (K,L) bucket tables and the candidate scan.
+// keys [n][L] -> keys & mask (in place); u16 keys widened to u32.
+void launch_mask_keys(uint32_t* keys, size_t n, uint32_t mask, hipStream_t s);
+void launch_widen_keys(const uint16_t* in, uint32_t* out, size_t n, hipStream_t s);
+// Counting sort of the N points of every table by key: off is [L][T+1] with
+// off[t][key] the absolute start of the bucket in perm ([L][N], table t at
+// t*N) and off[t][T] = t*N + N; perm holds the IDs of each bucket (in no
+// particular order).  cursor is [L][T+1] scratch.
+void launch_bucket_build(const uint32_t* keys, int64_t N, int L, uint32_t T, uint32_t* off, uint32_t* cursor,
+                         uint32_t* perm, hipStream_t s);
+// Per query: lo[q][t] = start of the query's bucket in table t, pre[q][0..L]
+// = prefix sums of the bucket sizes; maxS = max over queries of pre[q][L]
+// (maxS must be zero on entry).
+void launch_query_ranges(const uint32_t* qkeys, int Q, int L, uint32_t T, const uint32_t* off, uint32_t* lo,
+                         uint32_t* pre, uint32_t* maxS, hipStream_t s);
+
+// Candidate positions [p0,p1) of every query row's concatenated L buckets;
+// a point is scored only under the first table in which it shares the
+// query's key, so the union is visited without duplicates.
+struct LshScanArgs {
+    const uint32_t* perm;   // [L][N]
+    const uint32_t* dkeys;  // [N][L]
+    const uint32_t* qkeys;  // [Qall][L]
+    const uint32_t* qlo;    // [Qall][L]
+    const uint32_t* qpre;   // [Qall][L+1]
+    const int32_t* qmap;    // [Q] row -> query, or null for identity
+    int L;
+    int Q;
+    int64_t p0, p1;
+    int kind;               // kAngular/kGenJaccard/kEuclid exact; 0/1 integer sketch (with sketch); kSqL2 JL
+    bool sketch;            // score by sketch distance
+    const float* data;      // exact: [N][d]
+    const float* queries;   // exact: [Qall][d]
+    int d;
+    const uint32_t* dsk;    // integer sketch [N][words]
+    const uint32_t* qsk;    // [Qall][words]
+    int words;
+    const float* djl;       // JL [N][sdim]
+    const float* qjl;       // [Qall][sdim]
+    int sdim;
+};
+void launch_lsh_scan(const LshScanArgs& a, const TopKView& v, hipStream_t s);
 
 // ---------------------------------------------------------- host helpers
 

@@ -77,13 +77,14 @@ __global__ void wmh_draws_kernel(int nhash, int R, int M, const int* __restrict_
     downer[i] = owner[min((int)r, M - 1)];
 }
 
-// Block: 64 points (one wave) x 32 hashes (blockIdx.y selects the hashes
-// 32*y .. 32*y+31).  mode 0: out[i][h] = red count; mode 1: bit h%32 of
-// out[i][h/32] = bitmap[count % bitmap_len].
+// Block: 64 points (one wave) x `per` hashes (blockIdx.y selects the hashes
+// per*y .. per*y+per-1).  mode 0 (per = 32): out[i][h] = red count; mode 1
+// (per = hbits): bit b of out[i][y] = bitmap[count of hash y*hbits + b %
+// bitmap_len] (sketchDataOneBit, weightedMinHash.cuh:115-136).
 __global__ __launch_bounds__(64) void wmh_sketch_kernel(const float* __restrict__ x, int64_t n, int d,
                                                         const int* __restrict__ start,
                                                         const int* __restrict__ owner, int M, int nhash,
-                                                        int mode, const uint8_t* __restrict__ bitmap,
+                                                        int mode, int per, const uint8_t* __restrict__ bitmap,
                                                         int bitmap_len, const float* __restrict__ draws,
                                                         const int* __restrict__ downer, int R,
                                                         uint32_t* __restrict__ out, int ostride) {
@@ -100,8 +101,8 @@ __global__ __launch_bounds__(64) void wmh_sketch_kernel(const float* __restrict_
     const float* row = rows + lane * ld;
     const float Mf = (float)M;
     uint32_t word = 0;
-    for (int hb = 0; hb < 32; ++hb) {
-        const int h = blockIdx.y * 32 + hb;
+    for (int hb = 0; hb < per; ++hb) {
+        const int h = blockIdx.y * per + hb;
         if (h >= nhash) break;
         const uint32_t seed = (uint32_t)(h * 1234 + 92138);
         int count = (valid && M > 0) ? -1 : 0;
@@ -149,15 +150,18 @@ void launch_wmh_draws(int nhash, int R, int M, const int* owner, float* draws, i
 }
 
 void launch_wmh_sketch(const float* x, int64_t n, int d, const int* owner, const int* start, int M, int nhash,
-                       int mode, const uint8_t* bitmap, int bitmap_len, const float* draws, const int* draw_owner,
-                       int R, uint32_t* out, int out_stride, hipStream_t s) {
+                       int mode, int hbits, const uint8_t* bitmap, int bitmap_len, const float* draws,
+                       const int* draw_owner, int R, uint32_t* out, int out_stride, hipStream_t s) {
     if (n <= 0 || nhash <= 0) return;
+    LSH_REQUIRE(mode == 0 || (hbits >= 1 && hbits <= 32 && nhash % hbits == 0), LSH_ERR_INVALID,
+                "one-bit MinHash needs 1..32 hashes per word");
     const size_t lds = (size_t)64 * (d + 1) * sizeof(float);
     set_dynamic_lds((const void*)wmh_sketch_kernel, lds);
-    const unsigned gy = (unsigned)((nhash + 31) / 32);
+    const int per = mode == 1 ? hbits : 32;
+    const unsigned gy = (unsigned)((nhash + per - 1) / per);
     LSH_REQUIRE(gy <= 65535, LSH_ERR_UNSUPPORTED, "too many MinHash hashes");
     hipLaunchKernelGGL(wmh_sketch_kernel, dim3((unsigned)((n + 63) / 64), gy), dim3(64), lds, s, x, n, d, start,
-                       owner, M, nhash, mode, bitmap, bitmap_len, draws, draw_owner, R, out, out_stride);
+                       owner, M, nhash, mode, per, bitmap, bitmap_len, draws, draw_owner, R, out, out_stride);
     check_launch("wmh_sketch_kernel");
 }
 

@@ -15,21 +15,6 @@ namespace lshgpu {
 
 namespace {
 
-// Partial sums of one lane: (dot, |x|^2) for the angular kinds,
-// (sum min, sum max) for generalized Jaccard, (sum of squares, -) for L2.
-__device__ inline void group_accumulate(int kind, float x, float y, float& a, float& b) {
-    if (kind == kNegCos || kind == kAngular) {
-        a = __builtin_fmaf(x, y, a);
-        b = __builtin_fmaf(x, x, b);
-    } else if (kind == kGenJaccard) {
-        a += fminf(x, y);
-        b += fmaxf(x, y);
-    } else {
-        const float t = y - x;
-        a = __builtin_fmaf(t, t, a);
-    }
-}
-
 __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ data,
                                                      const float* __restrict__ queries, int d,
                                                      int kind, const uint64_t* __restrict__ keys,
@@ -86,15 +71,7 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
             b += __shfl_xor(b, o, 16);
         }
         if (gl == 0) {
-            float dist;
-            if (kind == kNegCos || kind == kAngular) {
-                const float c = a / (qn * __builtin_sqrtf(b));
-                dist = kind == kNegCos ? -c : 1.0f - c;
-            } else if (kind == kGenJaccard) {
-                dist = 1.0f - a / b;
-            } else {
-                dist = a;  // squared; kEuclid ranks by the square
-            }
+            const float dist = group_distance(kind, a, b, qn);
             k32[t] = id < 0 ? kEmptyKey : make_key(f32_key(dist), (uint32_t)id);
         }
     }

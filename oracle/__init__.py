@@ -64,7 +64,7 @@ def load() -> C.CDLL:
         "orc_philox4x32_10": (None, [_U32, _U32, _U32]),
         "orc_wmh_uniform": (C.c_float, [C.c_uint32, C.c_uint32]),
         "orc_wmh_bounds": (C.c_int, [_FP, C.c_int64, _FP, C.c_int64, C.c_int, _I32, _I32]),
-        "orc_wmh_sketch": (None, [_FP, C.c_int64, C.c_int, _I32, _I32, C.c_int, C.c_int, C.c_int,
+        "orc_wmh_sketch": (None, [_FP, C.c_int64, C.c_int, _I32, _I32, C.c_int, C.c_int, C.c_int, C.c_int,
                                   C.POINTER(C.c_uint8), C.c_int, _U32]),
         "orc_lsh_candidates": (C.c_int64, [_U32, C.c_int64, _U32, C.c_int, C.c_int64, _I32, C.c_int64]),
         "orc_goldstandard_port": (C.c_double, [_FP, C.c_int64, _FP, C.c_int64, C.c_int, C.c_int, C.c_int, _I32]),
@@ -238,13 +238,14 @@ def wmh_bounds(x, y=None):
     return bounds, start, int(M)
 
 
-def wmh_sketch(x, bounds, start, M: int, H: int, mode: int, bitmap):
+def wmh_sketch(x, bounds, start, M: int, H: int, mode: int, bitmap, hbits: int = 32):
+    """H counts (mode 0) or H words of `hbits` one-bit hashes (mode 1) per row."""
     x, xp = _f(x)
     b, bp = _i(bounds)
     s, sp = _i(start)
     bm = np.ascontiguousarray(bitmap, dtype=np.uint8)
     out = np.empty((x.shape[0], H), dtype=np.uint32)
-    load().orc_wmh_sketch(xp, x.shape[0], x.shape[1], bp, sp, int(M), int(H), int(mode),
+    load().orc_wmh_sketch(xp, x.shape[0], x.shape[1], bp, sp, int(M), int(H), int(mode), int(hbits),
                           bm.ctypes.data_as(C.POINTER(C.c_uint8)), bm.shape[0], out.ctypes.data_as(_U32))
     return out
 
@@ -256,6 +257,86 @@ def lsh_candidates(dkeys, qkeys, q: int):
     out = np.empty(n, dtype=np.int32)
     c = load().orc_lsh_candidates(dp, n, qp, tables, int(q), out.ctypes.data_as(_I32), n)
     return out[:c].copy()
+
+
+def lsh_keys(x, keys_impl: int, tables: int, bits: int, wmh=None):
+    """Bucket keys [n][tables] of rows that have had the key family's
+    preprocessing (lshFramework.cuh:61-100 with isHashKeys): 3 SimHash bit
+    slices (16-bit key type), 4 weighted-MinHash counts (8-bit key type),
+    5 one-bit weighted MinHash (16-bit), 7 cross-polytope.  wmh = (bounds,
+    start, M) for 4 and 5."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    n, d = x.shape
+    if keys_impl == 3:
+        planes = normal_stream(tables * bits * d)
+        return simhash(x, planes, tables, bits) & np.uint32(0xFFFF)
+    if keys_impl in (4, 5):
+        bounds, start, M = wmh
+        if keys_impl == 4:
+            return wmh_sketch(x, bounds, start, M, tables, 0, bit_stream(4096)) & np.uint32(0xFF)
+        return wmh_sketch(x, bounds, start, M, tables, 1, bit_stream(4096), hbits=bits) & np.uint32(0xFFFF)
+    if keys_impl == 7:
+        P = 1
+        while P < d:
+            P *= 2
+        return crosspoly(x, tables, sign_stream(tables * 3 * P)).astype(np.uint32)
+    raise ValueError("keys implementation must be 3, 4, 5 or 7")
+
+
+def lsh_search(data, queries, k: int, implementation: int, keys_impl: int, sketch_dim: int, func: int,
+               bits: int, tables: int, with_sketch: bool, C_: int = 4096):
+    """(K,L) framework search (lshFramework.cuh:312-460) as sets: every
+    query's candidates are the distinct points that share its key in at least
+    one table; they are ranked by exact distance `func`, or (with_sketch and
+    implementation >= 3) cut to the top C_ by sketch distance with ID ties and
+    then ranked exactly.  Each hash family first preprocesses data and queries
+    together, in the reference's order (keys, then sketches).
+    Returns (ids, distances) [Q][calculate_k(k)]."""
+    data = np.array(data, dtype=np.float32, copy=True, order="C")
+    queries = np.array(queries, dtype=np.float32, copy=True, order="C")
+    kq = calculate_k(k)
+
+    def preprocess(family):
+        nonlocal data, queries
+        if family in (4, 5):
+            data, queries = gj_transform(data, queries)
+        elif family in (6, 7):
+            data, queries = normalize_rows(data), normalize_rows(queries)
+
+    preprocess(keys_impl)
+    wmh = wmh_bounds(data, queries) if keys_impl in (4, 5) else None
+    dk = lsh_keys(data, keys_impl, tables, bits, wmh)
+    qk = lsh_keys(queries, keys_impl, tables, bits, wmh)
+    preprocess(implementation)
+    sketched = with_sketch and implementation >= 3
+    if sketched:
+        if implementation == 3:
+            planes = normal_stream(sketch_dim * 32 * data.shape[1])
+            ds, qs = simhash(data, planes, sketch_dim), simhash(queries, planes, sketch_dim)
+        elif implementation in (4, 5):
+            b, st, M = wmh_bounds(data, queries)
+            mode, H = implementation - 4, sketch_dim
+            ds = wmh_sketch(data, b, st, M, H, mode, bit_stream(4096))
+            qs = wmh_sketch(queries, b, st, M, H, mode, bit_stream(4096))
+        else:
+            planes = normal_stream(sketch_dim * data.shape[1], 1.0 / sketch_dim)
+            ds, qs = jl(data, planes, sketch_dim), jl(queries, planes, sketch_dim)
+    ids = np.empty((queries.shape[0], kq), dtype=np.int32)
+    dist = np.empty((queries.shape[0], kq), dtype=np.float32)
+    for q in range(queries.shape[0]):
+        cand = lsh_candidates(dk, qk, q)
+        if sketched and cand.size:
+            c = min(C_, data.shape[0])
+            if implementation == 6:
+                sub = jl_topc(ds[cand], qs[q:q + 1], c)[0]
+            else:
+                sub = sketch_topc(1 if implementation == 4 else 0, ds[cand], qs[q:q + 1], c)[0][0]
+            cand = cand[sub[sub >= 0]]
+        row = np.full((1, max(cand.size, 1)), -1, dtype=np.int32)
+        row[0, :cand.size] = cand
+        i, dd = rerank(data, queries[q:q + 1], row, kq, func)
+        ids[q], dist[q] = i[0], dd[0]
+    return ids, dist
 
 
 def goldstandard_port(data, queries, K: int, threads: int = 1):

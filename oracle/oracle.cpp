@@ -506,9 +506,9 @@ EXPORT int orc_wmh_bounds(const float* x, int64_t n, const float* y, int64_t m, 
 // floor(r); the hash is the number of red draws.  Seeds are
 // i*1234 + 92138 (weightedMinHash.cuh:252-262).  mode 0: one uint32 count per
 // hash (H hashes); mode 1 (weightedMinHash.cuh:109-140): word w bit b is
-// bitmap[count % bitmap_len] of seed index w*32 + b (H words).
+// bitmap[count % bitmap_len] of seed index w*hbits + b, b < hbits (H words).
 EXPORT void orc_wmh_sketch(const float* x, int64_t n, int d, const int32_t* bounds,
-                           const int32_t* start, int M, int H, int mode,
+                           const int32_t* start, int M, int H, int mode, int hbits,
                            const uint8_t* bitmap, int bitmap_len, uint32_t* out) {
     std::vector<int32_t> owner((size_t)std::max(M, 1));
     for (int j = 0; j < d; ++j)
@@ -532,8 +532,8 @@ EXPORT void orc_wmh_sketch(const float* x, int64_t n, int d, const int32_t* boun
         } else {
             for (int w = 0; w < H; ++w) {
                 uint32_t s = 0;
-                for (int b = 0; b < 32; ++b) {
-                    uint32_t c = count(row, (uint32_t)((w * 32 + b) * 1234 + 92138));
+                for (int b = 0; b < hbits; ++b) {
+                    uint32_t c = count(row, (uint32_t)((w * hbits + b) * 1234 + 92138));
                     s |= (uint32_t)bitmap[c % (uint32_t)bitmap_len] << b;
                 }
                 out[i * H + w] = s;

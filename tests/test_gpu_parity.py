@@ -286,3 +286,63 @@ def test_wmh_pipeline_matches_oracle(lsh, orc, impl):
     ids, dist = orc.rerank(td, tq, cand, k, 2)
     np.testing.assert_array_equal(res.ids, ids)
     _close(res.distances, dist)
+
+
+# ------------------------------------------------------------ (K,L) framework
+
+LSH_CASES = [
+    # implementation, keys, runWithSketchedData, distanceFunc, bucketKeyBits, tables, k, data kind
+    (2, 3, False, 1, 10, 8, 32, "unit"),
+    (2, 3, False, 3, 4, 3, 100, "unit"),        # 16 buckets per table: large candidate sets
+    (2, 7, False, 1, 8, 6, 32, "unit"),
+    (2, 5, False, 2, 6, 5, 32, "wmh"),
+    (2, 4, False, 2, 8, 4, 32, "wmh"),
+    (3, 3, True, 1, 12, 10, 32, "unit"),
+    (4, 4, True, 2, 8, 6, 32, "wmh"),           # both families shift and normalise
+    (5, 7, True, 2, 8, 4, 32, "wmh"),
+    (6, 3, True, 3, 10, 6, 32, "unit"),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("impl,keys,sk,func,bits,tables,k,kind", LSH_CASES)
+def test_lsh_framework_matches_oracle(lsh, orc, impl, keys, sk, func, bits, tables, k, kind):
+    rng = np.random.default_rng(50 + impl * 7 + keys + bits)
+    N, Q, d = 5000, 24, 64
+    if kind == "unit":
+        data, queries = _unit(rng, N, d), _unit(rng, Q, d)
+    else:
+        data, queries = _wmh_rows(rng, N, d, 0.3), _wmh_rows(rng, Q, d, 0.3)
+    sketch_dim = {3: 4, 4: 16, 5: 2, 6: 16}.get(impl, 1)
+    C = 300
+    res = lsh.LSH(impl, keys, k, d, Q, N, data, queries, sketch_dim, func, bits, tables, sk, candidates=C)
+    ids, dist = orc.lsh_search(data, queries, k, impl, keys, sketch_dim, func, bits, tables, sk, C)
+    np.testing.assert_array_equal(res.ids, ids)
+    _close(res.distances, dist)
+
+
+@pytest.mark.gpu
+def test_lsh_index_matches_one_shot_and_is_batch_independent(lsh, orc):
+    rng = np.random.default_rng(61)
+    N, Q, d = 20000, 64, 48
+    data, queries = _unit(rng, N, d), _unit(rng, Q, d)
+    cfg = dict(implementation=2, distanceFunc=1, framework=1, keysImplementation=3, bucketKeyBits=11, tables=12)
+    with lsh.Index(data, **cfg) as ix:
+        a = ix.search(queries, 32)
+        b = ix.search(queries[10:11], 32)
+    np.testing.assert_array_equal(a[10:11], b)
+    # Without preprocessing the index and the one-shot call agree exactly.
+    one = lsh.LSH(2, 3, 32, d, Q, N, data, queries, 1, 1, 11, 12, False)
+    np.testing.assert_array_equal(a["ID"], one.ids)
+    ids, _ = orc.lsh_search(data, queries, 32, 2, 3, 1, 1, 11, 12, False)
+    np.testing.assert_array_equal(a["ID"], ids)
+
+
+@pytest.mark.gpu
+def test_lsh_rejects_keys_wider_than_the_table(lsh):
+    rng = np.random.default_rng(62)
+    data, queries = _unit(rng, 100, 64), _unit(rng, 4, 64)
+    with pytest.raises(lsh.LshError):
+        lsh.LSH(2, 7, 32, 64, 4, 100, data, queries, 1, 1, 6, 4, False)  # 2P = 128 > 2^6
+    with pytest.raises(lsh.LshError):
+        lsh.LSH(2, 4, 32, 64, 4, 100, data, queries, 1, 1, 6, 4, False)  # 8-bit counts > 2^6
