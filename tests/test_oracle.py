@@ -351,3 +351,74 @@ def test_lsh_candidates_bruteforce(orc):
         c = orc.lsh_candidates(dk, qk, q)
         expect = np.nonzero((dk == qk[q]).any(1))[0]
         np.testing.assert_array_equal(c, expect)
+
+
+def test_wmh_one_bit_words_pack_the_counts(orc):
+    """Mode 1 word w, bit b is bitmap[count of hash w*hbits + b] (the counts
+    themselves are pinned above), for the 32-bit words of the linear scan and
+    the bucketKeyBits-wide words of the LSH keys."""
+    rng = np.random.default_rng(33)
+    x = orc.normalize_rows(rng.standard_exponential((30, 10)).astype(np.float32))
+    bounds, start, M = orc.wmh_bounds(x)
+    bm = orc.bit_stream(4096)
+    for hbits, words in ((32, 2), (7, 5)):
+        counts = orc.wmh_sketch(x, bounds, start, M, words * hbits, 0, bm)
+        packed = orc.wmh_sketch(x, bounds, start, M, words, 1, bm, hbits=hbits)
+        bits = bm[counts % 4096].astype(np.uint64).reshape(30, words, hbits)
+        expect = (bits << np.arange(hbits, dtype=np.uint64)).sum(2).astype(np.uint32)
+        np.testing.assert_array_equal(packed, expect)
+
+
+def _angular64(x, q):
+    x = x.astype(np.float64)
+    q = q.astype(np.float64)
+    return 1.0 - (x @ q) / (np.linalg.norm(x, axis=1) * np.linalg.norm(q))
+
+
+def test_lsh_search_bruteforce_exact(orc):
+    """Exact mode: the k best of the points sharing a key with the query in
+    at least one table, by (distance, ID); padding when fewer exist."""
+    rng = np.random.default_rng(42)
+    N, Q, d, L, bits = 800, 6, 12, 3, 6
+    data = rng.standard_normal((N, d)).astype(np.float32)
+    queries = rng.standard_normal((Q, d)).astype(np.float32)
+    ids, dist = orc.lsh_search(data, queries, 40, 2, 3, 1, 1, bits, L, False)
+    assert ids.shape == (Q, 64)
+    dk = orc.lsh_keys(data, 3, L, bits)
+    qk = orc.lsh_keys(queries, 3, L, bits)
+    planes = orc.normal_stream(L * bits * d).reshape(L * bits, d)
+    proj = (data.astype(np.float64) @ planes.T.astype(np.float64)) >= 0
+    weights = (1 << np.arange(bits)).astype(np.uint32)
+    approx = (proj.reshape(N, L, bits) * weights).sum(2)
+    assert (approx != dk).mean() < 0.01  # the FP32 bits agree with FP64 signs away from zero
+    for q in range(Q):
+        cand = np.nonzero((dk == qk[q]).any(1))[0]
+        dd = _angular64(data[cand], queries[q])
+        order = np.lexsort((cand, dd))[:64]
+        want = np.full(64, -1)
+        want[:order.size] = cand[order]
+        np.testing.assert_array_equal(ids[q], want)
+        np.testing.assert_allclose(dist[q, :order.size], dd[order], rtol=1e-6, atol=1e-7)
+        assert np.all(dist[q, order.size:] == np.float32(2147483648.0))
+
+
+def test_lsh_search_bruteforce_sketched(orc):
+    """Sketched mode: the top C of the candidate union by (Hamming distance,
+    ID), reranked exactly."""
+    rng = np.random.default_rng(43)
+    N, Q, d, L, bits, W, C = 600, 5, 16, 4, 4, 2, 25
+    data = rng.standard_normal((N, d)).astype(np.float32)
+    queries = rng.standard_normal((Q, d)).astype(np.float32)
+    ids, _ = orc.lsh_search(data, queries, 10, 3, 3, W, 1, bits, L, True, C)
+    dk, qk = orc.lsh_keys(data, 3, L, bits), orc.lsh_keys(queries, 3, L, bits)
+    planes = orc.normal_stream(W * 32 * d)
+    ds, qs = orc.simhash(data, planes, W), orc.simhash(queries, planes, W)
+    for q in range(Q):
+        cand = np.nonzero((dk == qk[q]).any(1))[0]
+        ham = np.array([sum(bin(int(a) ^ int(b)).count("1") for a, b in zip(ds[i], qs[q])) for i in cand])
+        top = cand[np.lexsort((cand, ham))[:C]]
+        dd = _angular64(data[top], queries[q])
+        order = np.lexsort((top, dd))[:32]
+        want = np.full(32, -1)
+        want[:order.size] = top[order]
+        np.testing.assert_array_equal(ids[q], want)
