@@ -36,8 +36,9 @@ typedef struct lsh_point {
 
 /* resultDTO.h:5-32 on LP64 (clock_t is long).  The three times are in
  * milliseconds, as the reference stores them (time_lapsed*1000/CLOCKS_PER_SEC);
- * here they are wall-clock times measured with HIP events.  recall and
- * avgDistance are NaN until a validation step fills them. */
+ * here they are host wall-clock times around the device work (the index
+ * build, then the search).  recall and avgDistance are NaN until a
+ * validation step fills them. */
 typedef struct lsh_result {
     lsh_point_t* results;   /* [N_query][k], k already rounded by lsh_calculate_k */
     long preprocessTime;
@@ -99,13 +100,19 @@ int lsh_linear_scans(int implementation, int k, int d, int N_query, int N_data,
                      int distanceFunc, lsh_result_t* out);
 
 /* kernel.cu:91 LSH(...): (K,L) bucket framework.  keysImplementation 3 SimHash
- * (bucketKeyBits bits per table), 4 weighted MinHash red count, 5 1-bit
- * weighted MinHash (bucketKeyBits bits), 7 cross-polytope; implementation
- * picks the in-bucket distance as for lsh_linear_scans (2 = exact).  The
- * candidate set of a query is the union of its L buckets without duplicates;
- * with runWithSketchedData it is cut to the top-C by sketch distance and
- * reranked exactly, otherwise all of it is ranked exactly
- * (lshFramework.cuh:142-280). */
+ * (bucketKeyBits bits per table, 16-bit key type), 4 weighted MinHash red
+ * count (8-bit key type; needs bucketKeyBits >= 8), 5 1-bit weighted MinHash
+ * (bucketKeyBits bits, 16-bit key type), 7 cross-polytope (one hash per
+ * table; needs 2^bucketKeyBits >= 2*P, P = d rounded up to a power of two,
+ * d <= 256); bucketKeyBits 1..24.  Each key family preprocesses the rows
+ * first (4, 5: shift + normalise; 7: normalise), then the sketch family of
+ * `implementation` does (4, 5: shift + normalise; 6: normalise), in that
+ * order (lshFramework.cuh:330, 391).  The candidate set of a query is the
+ * union of its L buckets without duplicates.  With runWithSketchedData and
+ * implementation 3..6 it is cut to the top-C by sketch distance (ID ties)
+ * and reranked exactly by distanceFunc; otherwise all of it is ranked by
+ * distanceFunc (lshFramework.cuh:142-280).  Rows with fewer candidates than
+ * k are padded with (-1, 2147483648.0f). */
 int lsh_framework(int implementation, int keysImplementation, int k, int d,
                   int N_query, int N_data, const float* data,
                   const float* queries, int sketchDim, int distanceFunc,
