@@ -112,15 +112,70 @@ void exact_topk(const float* data, const float* data_inv, int64_t N, int d, int 
     LSH_HIP(hipStreamSynchronize(s));
 }
 
+// Packed integer sketches [N][wpad] and, for Hamming scans, their FP4 image.
+struct SketchSet {
+    const uint32_t* ds;
+    const uint32_t* fp4;
+    int64_t N;
+};
+
 // Rows the query sketch buffer of an integer sketch scan must have.
 int sketch_query_rows(int Q) { return round_up(std::max(Q, 1), 256); }
 
 // Integer sketch scan: qs has sketch_query_rows(Q) rows of wpad words (zero
 // past Q).  Hamming scans of up to 1024 bits run on the matrix cores.
-void sketch_topk(const uint32_t* ds, int64_t N, int wpad, int kind, const uint32_t* qs, int Q, int K,
-                 bool fixed, TopKState& st, hipStream_t s, PhaseTimer& pt, int64_t& launches,
+//
+// With a strided sample of the data (every stride-th row), a Hamming scan
+// first finds the exact Ks-th nearest sample distance T of every query
+// (Ks = 1.45*K/stride), then makes one pass over all points appending the
+// pairs at distance <= T (about 1.45*K per query instead of the ~K*ln(N/K)
+// of the geometric rounds), and selects the K best of them.  A query whose
+// pass overflowed, or found fewer than K pairs, is redone with the rounds,
+// so the result is the exact top-K either way.
+void sketch_topk(const SketchSet& d, const SketchSet* sample, int wpad, int kind, const uint32_t* qs, int Q,
+                 int K, bool fixed, TopKState& st, hipStream_t s, PhaseTimer& pt, int64_t& launches,
                  int64_t* mfma_launches = nullptr, int64_t* mfma_points = nullptr) {
     const bool mfma = kind == 0 && hamming_mfma_supported(wpad) && !force_valu_scan();
+    const bool int_keys = K <= kMaxIntTopK;
+    if (mfma && !fixed && sample && sample->N > 0 && d.fp4 && int_keys) {
+        const int stride = (int)((d.N + sample->N - 1) / sample->N);
+        const int Ks = std::min<int>(kMaxIntTopK, std::max(1, (int)std::ceil(1.45 * K / stride)));
+        TopKState sa;
+        sketch_topk(*sample, nullptr, wpad, kind, qs, Q, Ks, false, sa, s, pt, launches, mfma_launches,
+                    mfma_points);
+        st.reset(Q, K, newcap_for(K), s, true);
+        launch_threshold_above(sa.view(), st.view(), s);
+        pt.begin(PH_SCAN, s);
+        pt.begin(PH_MFMA, s);
+        launch_hamming_mfma(d.ds, d.fp4, qs, wpad, Q, 0, d.N, st.view(), s);
+        pt.end(PH_MFMA, s);
+        pt.end(PH_SCAN, s);
+        ++launches;
+        if (mfma_launches) ++*mfma_launches;
+        if (mfma_points) *mfma_points += d.N;
+        pt.begin(PH_SELECT, s);
+        launch_select_merge(st.view(), s);
+        pt.end(PH_SELECT, s);
+        DevBuf<uint32_t> flags((size_t)Q);
+        launch_incomplete_flags(st.view(), flags.get(), s);
+        const std::vector<int32_t> bad = flagged(flags.get(), Q, s);
+        if (bad.empty()) return;
+        const int m = (int)bad.size();
+        const int mpad = sketch_query_rows(m);
+        std::vector<int32_t> hidx(bad);
+        hidx.resize((size_t)mpad, bad[0]);
+        DevBuf<int32_t> idx((size_t)mpad);
+        LSH_HIP(hipMemcpyAsync(idx.get(), hidx.data(), sizeof(int32_t) * mpad, hipMemcpyHostToDevice, s));
+        DevBuf<uint32_t> sub((size_t)mpad * wpad);
+        launch_gather_rows_u32(qs, wpad, idx.get(), mpad, sub.get(), s);
+        TopKState sst;
+        sketch_topk(d, nullptr, wpad, kind, sub.get(), m, K, false, sst, s, pt, launches, mfma_launches,
+                    mfma_points);
+        const TopKView sv = sst.view(), v = st.view();
+        launch_scatter_rows_u64(sv.keys, sv.stride, K, idx.get(), m, v.keys, v.stride, s);
+        LSH_HIP(hipStreamSynchronize(s));
+        return;
+    }
     auto scan = [&](int64_t p0, int64_t p1, const TopKView& v) {
         // The matrix-core scan queues candidates in LDS, which suits ranges
         // where few pairs pass: once 16*K points are behind every threshold,
@@ -129,16 +184,16 @@ void sketch_topk(const uint32_t* ds, int64_t N, int wpad, int kind, const uint32
         // memory, whose capacity per round it never exceeds.
         if (mfma && !fixed && p0 >= 16 * (int64_t)K) {
             pt.begin(PH_MFMA, s);
-            launch_hamming_mfma(ds, qs, wpad, Q, p0, p1, v, s);
+            launch_hamming_mfma(d.ds, d.fp4, qs, wpad, Q, p0, p1, v, s);
             pt.end(PH_MFMA, s);
             if (mfma_launches) ++*mfma_launches;
             if (mfma_points) *mfma_points += p1 - p0;
         } else {
-            SketchScanArgs a{ds, qs, wpad, Q, p0, p1, kind};
+            SketchScanArgs a{d.ds, qs, wpad, Q, p0, p1, kind};
             launch_sketch_scan(a, v, s);
         }
     };
-    run_rounds(st, Q, K, N, fixed, scan, s, pt, launches, K <= kMaxIntTopK);
+    run_rounds(st, Q, K, d.N, fixed, scan, s, pt, launches, int_keys);
     if (fixed) return;
     const std::vector<int32_t> ovf = flagged(st.view().ovf, Q, s);
     if (ovf.empty()) return;
@@ -151,9 +206,31 @@ void sketch_topk(const uint32_t* ds, int64_t N, int wpad, int kind, const uint32
     DevBuf<uint32_t> sub((size_t)mpad * wpad);
     launch_gather_rows_u32(qs, wpad, idx.get(), mpad, sub.get(), s);
     TopKState sst;
-    sketch_topk(ds, N, wpad, kind, sub.get(), m, K, true, sst, s, pt, launches);
+    sketch_topk(SketchSet{d.ds, nullptr, d.N}, nullptr, wpad, kind, sub.get(), m, K, true, sst, s, pt, launches);
     const TopKView sv = sst.view(), v = st.view();
     launch_scatter_rows_u64(sv.keys, sv.stride, K, idx.get(), m, v.keys, v.stride, s);
+    LSH_HIP(hipStreamSynchronize(s));
+}
+
+// Every stride-th row of a packed sketch array and its FP4 image, when the
+// data are large enough for the sampled threshold (at least 32*C points per
+// sample row; stride >= 4).
+int sample_stride(int64_t N, int C) {
+    const int64_t st = N / (32 * (int64_t)std::max(C, 1));
+    return st < 4 ? 0 : (int)std::min<int64_t>(16, st);
+}
+
+void build_sample(const uint32_t* ds, int64_t N, int wpad, int stride, DevBuf<uint32_t>& sds,
+                  DevBuf<uint32_t>& sfp4, int64_t& S, hipStream_t s) {
+    S = (N + stride - 1) / stride;
+    std::vector<int32_t> rows((size_t)S);
+    for (int64_t i = 0; i < S; ++i) rows[i] = (int32_t)(i * stride);
+    DevBuf<int32_t> idx((size_t)S);
+    LSH_HIP(hipMemcpyAsync(idx.get(), rows.data(), sizeof(int32_t) * rows.size(), hipMemcpyHostToDevice, s));
+    sds.alloc((size_t)S * wpad);
+    launch_gather_rows_u32(ds, wpad, idx.get(), (int)S, sds.get(), s);
+    sfp4.alloc(hamming_fp4_words(S, wpad));
+    launch_expand_fp4(sds.get(), S, wpad, sfp4.get(), s);
     LSH_HIP(hipStreamSynchronize(s));
 }
 
@@ -298,7 +375,8 @@ void lsh_topk(const LshScanArgs& base, const std::vector<int32_t>& hmap, int64_t
 }  // namespace
 
 size_t lsh_index::device_bytes() const {
-    size_t b = data.bytes() + data_inv.bytes() + dsk.bytes() + planesT.bytes() + djl.bytes() + wmh.bytes() +
+    size_t b = data.bytes() + data_inv.bytes() + dsk.bytes() + dfp4.bytes() + dsk_s.bytes() + dfp4_s.bytes() +
+               planesT.bytes() + djl.bytes() + wmh.bytes() +
                kwmh.bytes() + kplanesT.bytes() + ksigns.bytes() + dkeys.bytes() + boff.bytes() + bperm.bytes() +
                qkeys.bytes() + qlo.bytes() + qpre.bytes() + qmax.bytes() + q.bytes() + qT.bytes() +
                q_inv.bytes() + qjl.bytes() + qjlT.bytes() + qsk.bytes() + out.bytes() + flags.bytes() + st.bytes();
@@ -353,6 +431,7 @@ void lsh_index::build(const float* src, bool on_device, const float* extra_host,
         dsk.alloc((size_t)N * wpad);
         if (N > 0) LSH_HIP(hipMemsetAsync(dsk.get(), 0, dsk.bytes(), s));
         launch_simhash(data.get(), N, d, planesT.get(), words, 32, dsk.get(), wpad, s);
+        build_fp4(s);
     } else if (impl == 6) {
         // simHashJL.cuh:238-262: planes from N(0, 1/sketchDim).
         sdim = sketchDim;
@@ -371,8 +450,18 @@ void lsh_index::build(const float* src, bool on_device, const float* extra_host,
         dsk.alloc((size_t)N * wpad);
         if (N > 0) LSH_HIP(hipMemsetAsync(dsk.get(), 0, dsk.bytes(), s));
         wmh.sketch(data.get(), N, d, dsk.get(), wpad, s);
+        if (impl == 5) build_fp4(s);
     }
     LSH_HIP(hipStreamSynchronize(s));
+}
+
+// FP4 operand image of the packed sketches for the matrix-core Hamming scan.
+void lsh_index::build_fp4(hipStream_t s) {
+    if (!hamming_mfma_supported(wpad) || N < 16 * 1024) return;
+    dfp4.alloc(hamming_fp4_words(N, wpad));
+    launch_expand_fp4(dsk.get(), N, wpad, dfp4.get(), s);
+    const int stride = sample_stride(N, candidates);
+    if (stride) build_sample(dsk.get(), N, wpad, stride, dsk_s, dfp4_s, S_sample, s);
 }
 
 // lshFramework.cuh:312-409 runLsh up to the scan: keys (generateHashes with
@@ -546,7 +635,9 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
         const int Ceff = (int)std::max<int64_t>(1, std::min<int64_t>(candidates, N));
         const int K = Ceff <= kMaxIntTopK ? Ceff : pow2_ceil(Ceff);
         LSH_REQUIRE(K <= kMaxTopK, LSH_ERR_UNSUPPORTED, "candidate sets larger than 4096 are not supported");
-        sketch_topk(dsk.get(), N, wpad, impl == 4 ? 1 : 0, qsk.get(), Q, K, false, st, s, pt, scan_launches,
+        const SketchSet full{dsk.get(), dfp4.get(), N}, samp{dsk_s.get(), dfp4_s.get(), S_sample};
+        sketch_topk(full, S_sample > 0 ? &samp : nullptr, wpad, impl == 4 ? 1 : 0, qsk.get(), Q, K, false, st, s,
+                    pt, scan_launches,
                     &mfma_launches, &mfma_points);
         const int kind = impl == 3 ? kAngular : kGenJaccard;
         pt.begin(PH_FINAL, s);
@@ -761,7 +852,16 @@ void hamming_topc_host(const uint32_t* ds, int64_t N, const uint32_t* qs, int64_
     TopKState st;
     PhaseTimer pt(false);
     int64_t launches = 0;
-    sketch_topk(dpad.get(), N, wpad, 0, qpad.get(), (int)Q, K, false, st, nullptr, pt, launches);
+    DevBuf<uint32_t> fp4, sds, sfp4;
+    int64_t S = 0;
+    if (hamming_mfma_supported(wpad)) {
+        fp4.alloc(hamming_fp4_words(N, wpad));
+        launch_expand_fp4(dpad.get(), N, wpad, fp4.get(), nullptr);
+        const int stride = sample_stride(N, C);
+        if (stride) build_sample(dpad.get(), N, wpad, stride, sds, sfp4, S, nullptr);
+    }
+    const SketchSet full{dpad.get(), fp4.get(), N}, samp{sds.get(), sfp4.get(), S};
+    sketch_topk(full, S > 0 ? &samp : nullptr, wpad, 0, qpad.get(), (int)Q, K, false, st, nullptr, pt, launches);
     launch_sort_rows(st.view(), nullptr);
     LSH_HIP(hipDeviceSynchronize());
     const TopKView v = st.view();

@@ -70,6 +70,10 @@ struct lsh_index {
     int words = 0;   // uint32 words per point
     int wpad = 0;    // words padded to a multiple of 8
     lshgpu::DevBuf<uint32_t> dsk;    // [N][wpad]
+    lshgpu::DevBuf<uint32_t> dfp4;   // FP4 operand image of dsk for the matrix-core Hamming scan
+    lshgpu::DevBuf<uint32_t> dsk_s;  // every stride-th row of dsk (threshold sample)
+    lshgpu::DevBuf<uint32_t> dfp4_s; // its FP4 image
+    int64_t S_sample = 0;
     lshgpu::DevBuf<float> planesT;   // SimHash [words][d][32], JL [groups][d][32]
     int sdim = 0;                    // JL sketch length
     lshgpu::DevBuf<float> djl;       // [N][sdim]
@@ -105,6 +109,7 @@ struct lsh_index {
                 bool r_on_device, hipStream_t s);
 
    private:
+    void build_fp4(hipStream_t s);
     void build_lsh(const float* extra, int64_t n_extra, hipStream_t s);
     void lsh_keys(const float* x, int64_t n, uint32_t* keys, hipStream_t s);
     void search_lsh(int Q, int kq, hipStream_t s, lshgpu::PhaseTimer& pt);

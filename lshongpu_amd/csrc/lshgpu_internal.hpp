@@ -187,6 +187,11 @@ int newcap_for(int K);
 void launch_select_merge(const TopKView& v, hipStream_t s);
 void launch_sort_rows(const TopKView& v, hipStream_t s);  // orders the best list of every query
 void launch_count_overflow(const TopKView& v, uint32_t* out_count, hipStream_t s);
+// dst.thr[q] = (integer distance of src.thr[q]) + 1 with ID 0: a scan with it
+// appends exactly the pairs at or below that distance.
+void launch_threshold_above(const TopKView& src, const TopKView& dst, hipStream_t s);
+// flags[q] = 1 when query q overflowed or holds fewer than K keys.
+void launch_incomplete_flags(const TopKView& v, uint32_t* flags, hipStream_t s);
 
 // scan_exact.hip: distances of queries [0,Q) to points [p0,p1) of data.
 struct ExactScanArgs {
@@ -243,10 +248,16 @@ struct SketchScanArgs {
 void launch_sketch_scan(const SketchScanArgs& a, const TopKView& v, hipStream_t s);
 
 // scan_mfma.hip: Hamming scan on the matrix cores (FP4 +-1 operands).  qs
-// has ceil(Q/hamming_mfma_query_rows())*hamming_mfma_query_rows() rows.
+// has ceil(Q/hamming_mfma_query_rows())*hamming_mfma_query_rows() rows; the
+// data sketches are the FP4 operand image written by launch_expand_fp4
+// (hamming_fp4_words(N, words) uint32 words) from the packed [N][words].
 int hamming_mfma_query_rows();
 bool hamming_mfma_supported(int words);
-void launch_hamming_mfma(const uint32_t* ds, const uint32_t* qs, int words, int Q, int64_t p0, int64_t p1,
+size_t hamming_fp4_words(int64_t N, int words);
+void launch_expand_fp4(const uint32_t* ds, int64_t N, int words, uint32_t* frag, hipStream_t s);
+// ds: packed [N][words]; frag: the FP4 image, or null to expand ds on the fly.
+void launch_hamming_mfma(const uint32_t* ds, const uint32_t* frag, const uint32_t* qs, int words, int Q, int64_t p0,
+                         int64_t p1,
                          const TopKView& v, hipStream_t s);
 
 // minhash.hip

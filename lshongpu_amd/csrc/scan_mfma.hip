@@ -10,19 +10,27 @@
 // Operand layout: for K-step s (bits 64s .. 64s+63), lane l supplies row
 // l%32 and the 32 bits of word 2s + l/32 of that row, one nibble per bit,
 // in the low four registers of the operand.  A (queries) and B (points) use
-// the same mapping, so the dot product pairs equal bit positions.
+// the same mapping, so the dot product pairs equal bit positions.  The
+// accumulator register r of lane l holds row (r/4)*8 + (l/32)*4 + r%4 of A
+// and row l%32 of B.
 //
-// Workgroup: 4 waves; each wave keeps the expanded fragments of MQ = 2 query
-// tiles (64 queries) in VGPRs for the whole sketch, so a workgroup covers 256
-// queries.  The workgroup walks point tiles of 32 points: the packed words
-// of a tile (32 x W words, one coalesced read) are expanded once into LDS in
-// fragment order and every wave reads its B fragments from there.  The
-// epilogue compares each accumulator with the query's threshold (one float
-// compare: dot >= B - 2*threshold); the rare survivors are checked against
-// the exact 64-bit threshold and queued in an LDS pool with per-query LDS
-// counters, and the pool is flushed to the global top-K state with one
-// atomic per query, so no global round trip sits on the tile loop.  Ranges
-// whose candidates are still dense (the first rounds) use the VALU scan.
+// The data sketches are kept in HBM already expanded to FP4 and laid out in
+// operand order, [tile of 32 points][K-step][lane][16 bytes], so a wave loads
+// the B operand of one K-step with one coalesced 1 KiB read and no
+// arithmetic (4x the bytes of the packed sketch; 512 B per point at 1024
+// bits).  Each wave owns 64 queries (two A tiles resident in VGPRs for the
+// whole launch) and walks the workgroup's slice of point tiles two tiles at
+// a time, independently of the other waves: no LDS staging and no
+// workgroup barrier.  Workgroups are numbered so that the query blocks of
+// one slice of points run on the same XCD (blocks are dealt to the 8 XCDs
+// round-robin), which keeps one copy of the slice in that XCD's L2.
+//
+// Epilogue: a pair may pass when its dot product is >= B - 2*h_thr (one
+// float compare against the query's threshold distance); for the registers
+// some lane flags, the exact (distance, ID) key is checked against the
+// 64-bit threshold and appended to the wave's LDS pool, which is flushed to
+// the global top-K state with one atomic per query.  Ranges whose
+// candidates are still dense (the first rounds) use the VALU scan.
 #include "device_common.hpp"
 
 namespace lshgpu {
@@ -32,10 +40,6 @@ namespace {
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v8i __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
-
-constexpr int MQ = 2;                     // query tiles per wave
-constexpr int WG_QUERIES = 4 * MQ * 32;   // queries per workgroup
-constexpr int TPB = 256;
 
 // 8 bits -> 8 FP4 nibbles: bit i -> nibble i = 0x2 (+1.0) or 0xA (-1.0).
 __device__ inline uint32_t spread8(uint32_t x) {
@@ -58,125 +62,156 @@ __device__ inline v4i expand_word(uint32_t w) {
 // FP4 operands occupy the low four registers; the upper half is not read.
 __device__ inline v8i widen(v4i a) { return __builtin_shufflevector(a, a, 0, 1, 2, 3, -1, -1, -1, -1); }
 
-constexpr int POOL = 2048;  // queued candidates per workgroup
+constexpr int MQ = 2;                      // query tiles per wave
+constexpr int WAVES = 4;
+constexpr int WAVE_QUERIES = MQ * 32;      // 64: one query per lane in the flush
+constexpr int WG_QUERIES = WAVES * WAVE_QUERIES;
+constexpr int TPB = WAVES * 64;
+constexpr int POOLW = 512;                 // queued candidates per wave
 
-// KS = sketch words / 2 (K-steps of 64 bits).
-template <int KS>
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// KS = sketch words / 2 (K-steps of 64 bits).  FP4IMG: stage the B operands
+// from the FP4 operand image with direct global-to-LDS loads into a double
+// buffer; otherwise load the packed words and expand them in registers.
+template <int KS, bool FP4IMG>
 __global__ __launch_bounds__(TPB, 2) void hamming_mfma_kernel(const uint32_t* __restrict__ ds,
+                                                              const v4i* __restrict__ frag,
                                                               const uint32_t* __restrict__ qs, int Q,
-                                                              int64_t p0, int64_t p1, int tiles_per_wg,
-                                                              TopKView v) {
+                                                              int64_t p0, int64_t p1, int64_t tile_begin,
+                                                              int64_t tile_end, int tiles_per_wg, int qblocks,
+                                                              int slices, TopKView v) {
     constexpr int W = 2 * KS;
-    constexpr float BITS = 32.0f * W;
-    __shared__ v4i bfrag[KS * 64];          // expanded B fragments of the current tile
-    __shared__ float spill[4][MQ * 16][64]; // per-wave accumulators for the append path
-    __shared__ float tfs[WG_QUERIES];       // coarse thresholds: dot >= tfs[q] may pass
-    __shared__ uint64_t lthr[WG_QUERIES];   // exact thresholds (keys)
-    __shared__ uint32_t lcnt[WG_QUERIES];   // queued candidates per query since the last flush
-    __shared__ uint32_t lbase[WG_QUERIES];  // global slot of the first of them
-    __shared__ uint64_t pkey[POOL];         // queued keys
-    __shared__ uint16_t pq[POOL];           // query (within the workgroup) of each queued key
-    __shared__ uint32_t npool;
+    constexpr int BITS = 32 * W;
+    constexpr int SPW = (KS + WAVES - 1) / WAVES;  // K-steps staged per wave
+    constexpr int NBUF = FP4IMG ? 2 : 1;
+    __shared__ v4i bfrag[NBUF][KS * 64];           // B operands of the tile(s) in flight
+    __shared__ float tf[WG_QUERIES];               // pass iff dot > tf (Hamming < threshold distance)
+    __shared__ uint16_t spill[WAVES][MQ * 16][64]; // Hamming distances of a tile with candidates
+    __shared__ uint32_t lcnt[WAVES][WAVE_QUERIES];
+    __shared__ uint32_t lbase[WAVES][WAVE_QUERIES];
+    __shared__ uint64_t pkey[WAVES][POOLW];
+    __shared__ uint8_t pq[WAVES][POOLW];
 
+    // XCD-aware numbering: block L runs on XCD L % 8; the blocks of one XCD
+    // take the query blocks of slices xcd, xcd + 8, ...
+    const int L = blockIdx.x;
+    const int j = L >> 3;
+    const int qb = j % qblocks;
+    const int slice = (L & 7) + 8 * (j / qblocks);
+    if (slice >= slices) return;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int qblock = blockIdx.x * WG_QUERIES;
+    const int qwave = qb * WG_QUERIES + wave * WAVE_QUERIES;
 
-    // Resident A fragments of this wave's query tiles.
+    {
+        const int q = qwave + lane;
+        float f = __builtin_inff();  // rows past Q never pass
+        if (q < Q) {
+            const uint32_t h = (uint32_t)(v.thr[q] >> 32);
+            f = h == 0xFFFFFFFFu ? -__builtin_inff() : (float)BITS - 2.0f * (float)h;
+        }
+        tf[threadIdx.x] = f;
+        lcnt[wave][lane] = 0;
+    }
     v4i afrag[MQ][KS];
 #pragma unroll
     for (int t = 0; t < MQ; ++t) {
-        const int qt = qblock + (wave * MQ + t) * 32;
-        const uint32_t* __restrict__ qr = qs + (size_t)(qt + (lane & 31)) * W + (lane >> 5);
+        const uint32_t* __restrict__ qr = qs + (size_t)(qwave + t * 32 + (lane & 31)) * W + (lane >> 5);
 #pragma unroll
         for (int s = 0; s < KS; ++s) afrag[t][s] = expand_word(qr[2 * s]);
     }
-    {
-        const int qi = qblock + (int)threadIdx.x;
-        float f = __builtin_inff();  // rows past Q never pass
-        uint64_t th = 0;
-        if (qi < Q) {
-            th = v.thr[qi];
-            const uint32_t hi = (uint32_t)(th >> 32);
-            f = hi == 0xFFFFFFFFu ? -__builtin_inff() : BITS - 2.0f * (float)hi;
-        }
-        tfs[threadIdx.x] = f;
-        lthr[threadIdx.x] = th;
-        lcnt[threadIdx.x] = 0;
-        if (threadIdx.x == 0) npool = 0;
-    }
+    __syncthreads();
+    // Threshold of accumulator register r of query tile t of this lane.
+    auto thr_of = [&](int t, int r) -> float {
+        return tf[wave * WAVE_QUERIES + t * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3)];
+    };
 
-    // Moves the queued candidates to the global top-K state (all threads):
-    // per-query ranks from LDS atomics, one global atomic per query, scatter.
+    // The wave's queue: pool entries [0, queued); moved to the global top-K
+    // state with LDS ranks per query and one global atomic per query.
+    uint32_t queued = 0;
     auto flush = [&]() {
-        const uint32_t n = min(npool, (uint32_t)POOL);
-        uint32_t rank[POOL / TPB];
+        wave_sync();
+        const uint32_t n = min(queued, (uint32_t)POOLW);
+        uint32_t rank[POOLW / 64];
 #pragma unroll
-        for (int j = 0; j < POOL / TPB; ++j) {
-            const uint32_t e = threadIdx.x + j * TPB;
-            rank[j] = e < n ? atomicAdd(&lcnt[pq[e]], 1u) : 0u;
+        for (int k = 0; k < POOLW / 64; ++k) {
+            const uint32_t e = lane + 64 * k;
+            rank[k] = e < n ? atomicAdd(&lcnt[wave][pq[wave][e]], 1u) : 0u;
         }
-        __syncthreads();
-        const uint32_t c = lcnt[threadIdx.x];
-        if (c) lbase[threadIdx.x] = atomicAdd(&v.cnt[qblock + threadIdx.x], c);
-        __syncthreads();
+        wave_sync();
+        const uint32_t c = lcnt[wave][lane];
+        if (c) lbase[wave][lane] = atomicAdd(&v.cnt[qwave + lane], c);
+        wave_sync();
 #pragma unroll
-        for (int j = 0; j < POOL / TPB; ++j) {
-            const uint32_t e = threadIdx.x + j * TPB;
+        for (int k = 0; k < POOLW / 64; ++k) {
+            const uint32_t e = lane + 64 * k;
             if (e < n) {
-                const uint32_t q = pq[e];
-                const uint32_t slot = lbase[q] + rank[j];
-                if (slot < (uint32_t)v.newcap) v.keys[(size_t)(qblock + q) * v.stride + v.K + slot] = pkey[e];
+                const uint32_t q = pq[wave][e];
+                const uint32_t slot = lbase[wave][q] + rank[k];
+                if (slot < (uint32_t)v.newcap) v.keys[(size_t)(qwave + q) * v.stride + v.K + slot] = pkey[wave][e];
             }
         }
-        __syncthreads();
-        lcnt[threadIdx.x] = 0;
-        if (threadIdx.x == 0) npool = 0;
-        __syncthreads();
+        lcnt[wave][lane] = 0;
+        queued = 0;
+        wave_sync();
     };
 
-    const int64_t ntiles = (p1 - p0 + 31) / 32;
-    const int64_t t0 = (int64_t)blockIdx.y * tiles_per_wg;
-    const int64_t t1 = min(ntiles, t0 + tiles_per_wg);
-    // Staging: wave w fills the fragment rows of K-steps w, w+4, ...; lane l
-    // of row s holds word 2s + l/32 of point l%32, so the LDS writes of a wave
-    // are 64 consecutive 16-byte slots.  The words of the next tile are
-    // loaded into registers while the current tile is multiplied.
-    constexpr int SPW = (KS + 3) / 4;  // K-steps staged per wave
+    const int64_t t0 = tile_begin + (int64_t)slice * tiles_per_wg;
+    const int64_t t1 = min(tile_end, t0 + tiles_per_wg);
+    // Wave w stages K-steps w, w+4, ...; lane l of K-step s holds word
+    // 2s + l/32 of point l%32 of the tile.
     const int sp = lane & 31, sw = lane >> 5;
     uint32_t next[SPW];
-    auto load_tile = [&](int64_t tile) {
-        const int64_t pi = p0 + tile * 32 + sp;
+    auto stage = [&](int64_t tile, int buf) {
+        if (FP4IMG) {
 #pragma unroll
-        for (int j = 0; j < SPW; ++j) {
-            const int st = wave + 4 * j;
-            next[j] = (st < KS && pi < p1) ? ds[(size_t)pi * W + 2 * st + sw] : 0u;
+            for (int k = 0; k < SPW; ++k) {
+                const int st = wave + WAVES * k;
+                if (st < KS)
+                    __builtin_amdgcn_global_load_lds(
+                        (const void*)(frag + ((size_t)tile * KS + st) * 64 + lane),
+                        (__attribute__((address_space(3))) void*)&bfrag[buf][st * 64], 16, 0, 0);
+            }
+        } else {
+            const int64_t pi = tile * 32 + sp;
+#pragma unroll
+            for (int k = 0; k < SPW; ++k) {
+                const int st = wave + WAVES * k;
+                next[k] = (st < KS && pi < p1) ? ds[(size_t)pi * W + 2 * st + sw] : 0u;
+            }
         }
     };
-    if (t0 < t1) load_tile(t0);
+    if (t0 < t1) stage(t0, 0);
     for (int64_t tile = t0; tile < t1; ++tile) {
-        const int64_t pbase = p0 + tile * 32;
-        __syncthreads();  // the previous tile's reads of bfrag and its queueing are complete
-        const uint32_t queued = npool;
+        const int buf = FP4IMG ? (int)((tile - t0) & 1) : 0;
+        if (FP4IMG) {
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __syncthreads();  // the tile's operands are in LDS; the previous tile is done
+            if (tile + 1 < t1) stage(tile + 1, buf ^ 1);
+        } else {
+            __syncthreads();  // the previous tile's reads of bfrag are complete
 #pragma unroll
-        for (int j = 0; j < SPW; ++j) {
-            const int st = wave + 4 * j;
-            if (st < KS) bfrag[st * 64 + lane] = expand_word(next[j]);
+            for (int k = 0; k < SPW; ++k) {
+                const int st = wave + WAVES * k;
+                if (st < KS) bfrag[0][st * 64 + lane] = expand_word(next[k]);
+            }
+            if (tile + 1 < t1) stage(tile + 1, 0);
+            __syncthreads();
         }
-        if (tile + 1 < t1) load_tile(tile + 1);
-        __syncthreads();
-        if (queued > POOL / 2) flush();
 
         v16f acc[MQ];
 #pragma unroll
         for (int t = 0; t < MQ; ++t) acc[t] = (v16f){};
-        // Four K-steps per group; the scheduling fence keeps the B fragments
-        // of one group live at a time.
 #pragma unroll
         for (int s0 = 0; s0 < KS; s0 += 4) {
             v4i b[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) b[i] = bfrag[(s0 + i) * 64 + lane];
+            for (int i = 0; i < 4; ++i) b[i] = bfrag[buf][(s0 + i) * 64 + lane];
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -186,79 +221,81 @@ __global__ __launch_bounds__(TPB, 2) void hamming_mfma_kernel(const uint32_t* __
             __builtin_amdgcn_sched_barrier(0);
         }
 
-        const int64_t pi = pbase + (lane & 31);
-        const bool pvalid = pi < p1;
-        // Bit t*16+r of `cand`: accumulator (t, r) of this lane may beat its
-        // query's threshold.
-        uint32_t cand = 0;
+        // A pair passes when its Hamming distance is below the query's
+        // threshold distance.  Equal distances never pass: every key of the
+        // best list has a smaller ID than the points of this range.
+        const int64_t pi = tile * 32 + sp;
+        const bool pvalid = pi >= p0 && pi < p1;
+        uint32_t pm = 0;  // bit t*16+r: accumulator (t, r) passes
 #pragma unroll
         for (int t = 0; t < MQ; ++t)
 #pragma unroll
-            for (int r = 0; r < 16; ++r)
-                cand |= (acc[t][r] >= tfs[(wave * MQ + t) * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3)] ? 1u : 0u)
-                        << (t * 16 + r);
-        if (!pvalid) cand = 0;
-        if (__ballot(cand != 0)) {
-            // Rare: visit the accumulator registers some lane flagged (their
-            // union over the wave), in wave-uniform order through LDS.
+            for (int r = 0; r < 16; ++r) pm |= (acc[t][r] > thr_of(t, r) ? 1u : 0u) << (t * 16 + r);
+        if (!pvalid) pm = 0;
+        if (__ballot(pm != 0)) {
 #pragma unroll
             for (int t = 0; t < MQ; ++t)
 #pragma unroll
-                for (int r = 0; r < 16; ++r) spill[wave][t * 16 + r][lane] = acc[t][r];
-            uint32_t any = cand;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) any |= (uint32_t)__shfl_xor((int)any, o);
-            // Exact check against the 64-bit threshold, then one pool
-            // reservation for the whole wave.
-            uint32_t pass = 0;
-            for (uint32_t m = any; m; m &= m - 1) {
-                const int bit = __ffs((int)m) - 1;
-                const int qrel = (wave * MQ + (bit >> 4)) * 32 + ((bit & 15) >> 2) * 8 + (lane >> 5) * 4 + (bit & 3);
-                const uint32_t h = (uint32_t)(((int)BITS - (int)spill[wave][bit][lane]) >> 1);
-                if (((cand >> bit) & 1u) && make_key(h, (uint32_t)pi) < lthr[qrel]) pass |= 1u << bit;
-            }
-            const uint32_t mine = (uint32_t)__popc(pass);
+                for (int r = 0; r < 16; ++r) spill[wave][t * 16 + r][lane] = (uint16_t)((BITS - (int)acc[t][r]) >> 1);
+            const uint32_t mine = (uint32_t)__popc(pm);
             uint32_t incl = mine;
 #pragma unroll
             for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t t = __shfl_up(incl, o);
-                if (lane >= o) incl += t;
+                const uint32_t x = __shfl_up(incl, o);
+                if (lane >= o) incl += x;
             }
-            const uint32_t wtotal = __shfl(incl, 63);
-            uint32_t base = 0;
-            if (lane == 0 && wtotal) base = atomicAdd(&npool, wtotal);
-            base = __shfl(base, 0);
-            uint32_t e = base + incl - mine;
-            for (uint32_t m = pass; m; m &= m - 1, ++e) {
+            const uint32_t total = (uint32_t)__shfl((int)incl, 63);
+            wave_sync();
+            uint32_t e = queued + incl - mine;
+            for (uint32_t m = pm; m; m &= m - 1, ++e) {
                 const int bit = __ffs((int)m) - 1;
-                const int qrel = (wave * MQ + (bit >> 4)) * 32 + ((bit & 15) >> 2) * 8 + (lane >> 5) * 4 + (bit & 3);
-                if (e < (uint32_t)POOL) {
-                    const uint32_t h = (uint32_t)(((int)BITS - (int)spill[wave][bit][lane]) >> 1);
-                    pkey[e] = make_key(h, (uint32_t)pi);
-                    pq[e] = (uint16_t)qrel;
+                const int qrel = (bit >> 4) * 32 + ((bit & 15) >> 2) * 8 + sw * 4 + (bit & 3);
+                if (e < (uint32_t)POOLW) {
+                    pkey[wave][e] = make_key(spill[wave][bit][lane], (uint32_t)pi);
+                    pq[wave][e] = (uint8_t)qrel;
                 } else {
-                    v.ovf[qblock + qrel] = 1u;  // dropped: the query is redone
+                    v.ovf[qwave + qrel] = 1u;  // dropped: the query is redone
                 }
             }
+            queued += total;
+            if (queued > POOLW / 2) flush();
         }
     }
-    __syncthreads();
     flush();
 }
 
 template <int KS>
-void launch_ks(const uint32_t* ds, const uint32_t* qs, int Q, int64_t p0, int64_t p1, const TopKView& v,
-               hipStream_t s) {
+void launch_ks(const uint32_t* ds, const v4i* frag, const uint32_t* qs, int Q, int64_t p0, int64_t p1,
+               const TopKView& v, hipStream_t s) {
     const int qblocks = (Q + WG_QUERIES - 1) / WG_QUERIES;
-    const int64_t ntiles = (p1 - p0 + 31) / 32;
-    // Enough workgroups to fill 256 CUs twice over, at least 4 tiles each.
+    const int64_t tile_begin = p0 / 32, tile_end = (p1 + 31) / 32;
+    const int64_t ntiles = tile_end - tile_begin;
+    // About 2048 workgroups, at least 4 tiles each.
     int64_t slices = std::max<int64_t>(1, std::min<int64_t>(ntiles / 4, (2048 + qblocks - 1) / qblocks));
-    slices = std::min<int64_t>(slices, 65535);
-    const int tiles_per_wg = (int)((ntiles + slices - 1) / slices);
-    slices = (ntiles + tiles_per_wg - 1) / tiles_per_wg;
-    hipLaunchKernelGGL((hamming_mfma_kernel<KS>), dim3(qblocks, (unsigned)slices), dim3(TPB), 0, s, ds, qs, Q,
-                       p0, p1, tiles_per_wg, v);
+    const int64_t per = (ntiles + slices - 1) / slices;
+    slices = (ntiles + per - 1) / per;
+    const int64_t groups = (slices + 7) / 8;
+    const int64_t blocks = 8 * (int64_t)qblocks * groups;
+    LSH_REQUIRE(blocks < (int64_t)1 << 31, LSH_ERR_UNSUPPORTED, "matrix-core scan grid too large");
+    if (frag)
+        hipLaunchKernelGGL((hamming_mfma_kernel<KS, true>), dim3((unsigned)blocks), dim3(TPB), 0, s, ds, frag, qs, Q,
+                           p0, p1, tile_begin, tile_end, (int)per, qblocks, (int)slices, v);
+    else
+        hipLaunchKernelGGL((hamming_mfma_kernel<KS, false>), dim3((unsigned)blocks), dim3(TPB), 0, s, ds, frag, qs,
+                           Q, p0, p1, tile_begin, tile_end, (int)per, qblocks, (int)slices, v);
     check_launch("hamming_mfma_kernel");
+}
+
+__global__ void expand_fp4_kernel(const uint32_t* __restrict__ ds, int64_t N, int wpad, int KS, int64_t total,
+                                  v4i* __restrict__ frag) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int lane = (int)(i & 63);
+    const int s = (int)((i >> 6) % KS);
+    const int64_t T = (i >> 6) / KS;
+    const int64_t row = T * 32 + (lane & 31);
+    const uint32_t x = row < N ? ds[(size_t)row * wpad + 2 * s + (lane >> 5)] : 0u;
+    frag[i] = expand_word(x);
 }
 
 }  // namespace
@@ -267,14 +304,28 @@ int hamming_mfma_query_rows() { return WG_QUERIES; }
 
 bool hamming_mfma_supported(int words) { return words == 8 || words == 16 || words == 24 || words == 32; }
 
-void launch_hamming_mfma(const uint32_t* ds, const uint32_t* qs, int words, int Q, int64_t p0, int64_t p1,
-                         const TopKView& v, hipStream_t s) {
+size_t hamming_fp4_words(int64_t N, int words) {
+    const int64_t tiles = (N + 31) / 32 + 2;  // the scan reads one tile past a range
+    return (size_t)tiles * (size_t)(words / 2) * 64 * 4;
+}
+
+void launch_expand_fp4(const uint32_t* ds, int64_t N, int words, uint32_t* frag, hipStream_t s) {
+    const int KS = words / 2;
+    const int64_t total = (int64_t)hamming_fp4_words(N, words) / 4;
+    hipLaunchKernelGGL(expand_fp4_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, ds, N, words, KS,
+                       total, reinterpret_cast<v4i*>(frag));
+    check_launch("expand_fp4_kernel");
+}
+
+void launch_hamming_mfma(const uint32_t* ds, const uint32_t* frag, const uint32_t* qs, int words, int Q, int64_t p0,
+                         int64_t p1, const TopKView& v, hipStream_t s) {
     if (p1 <= p0 || Q <= 0) return;
+    const v4i* f = reinterpret_cast<const v4i*>(frag);
     switch (words) {
-        case 8: launch_ks<4>(ds, qs, Q, p0, p1, v, s); break;
-        case 16: launch_ks<8>(ds, qs, Q, p0, p1, v, s); break;
-        case 24: launch_ks<12>(ds, qs, Q, p0, p1, v, s); break;
-        case 32: launch_ks<16>(ds, qs, Q, p0, p1, v, s); break;
+        case 8: launch_ks<4>(ds, f, qs, Q, p0, p1, v, s); break;
+        case 16: launch_ks<8>(ds, f, qs, Q, p0, p1, v, s); break;
+        case 24: launch_ks<12>(ds, f, qs, Q, p0, p1, v, s); break;
+        case 32: launch_ks<16>(ds, f, qs, Q, p0, p1, v, s); break;
         default: fail(LSH_ERR_INTERNAL, "MFMA Hamming scan needs 8, 16, 24 or 32 words");
     }
 }

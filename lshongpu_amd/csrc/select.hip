@@ -174,6 +174,19 @@ __global__ void count_flags_kernel(const uint32_t* f, int n, uint32_t* out) {
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(out, (uint32_t)__popcll(m));
 }
 
+__global__ void threshold_above_kernel(TopKView src, TopKView dst) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= dst.Q) return;
+    const uint64_t t = src.thr[q];
+    dst.thr[q] = t == kEmptyKey ? kEmptyKey : make_key((uint32_t)(t >> 32) + 1u, 0u);
+}
+
+__global__ void incomplete_flags_kernel(TopKView v, uint32_t* flags) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= v.Q) return;
+    flags[q] = (v.ovf[q] != 0u || v.thr[q] == kEmptyKey) ? 1u : 0u;
+}
+
 unsigned grid_for(size_t n) {
     size_t b = (n + 255) / 256;
     return (unsigned)std::min<size_t>(std::max<size_t>(b, 1), 65535 * 8);
@@ -235,6 +248,16 @@ void launch_count_overflow(const TopKView& v, uint32_t* out_count, hipStream_t s
     hipLaunchKernelGGL(count_flags_kernel, dim3((v.Q + 255) / 256), dim3(256), 0, s, v.ovf, v.Q,
                        out_count);
     check_launch("count_flags_kernel");
+}
+
+void launch_threshold_above(const TopKView& src, const TopKView& dst, hipStream_t s) {
+    hipLaunchKernelGGL(threshold_above_kernel, dim3((dst.Q + 255) / 256), dim3(256), 0, s, src, dst);
+    check_launch("threshold_above_kernel");
+}
+
+void launch_incomplete_flags(const TopKView& v, uint32_t* flags, hipStream_t s) {
+    hipLaunchKernelGGL(incomplete_flags_kernel, dim3((v.Q + 255) / 256), dim3(256), 0, s, v, flags);
+    check_launch("incomplete_flags_kernel");
 }
 
 }  // namespace lshgpu

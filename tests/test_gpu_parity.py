@@ -182,6 +182,38 @@ def test_hamming_topc_adversarial_order_overflows_and_recovers(lsh, orc):
     np.testing.assert_array_equal(dist, odist)
 
 
+@pytest.mark.gpu
+def test_hamming_topc_sampled_threshold(lsh, orc):
+    # Large enough for the sampled threshold (every 16th row): one matrix-core
+    # pass appends the pairs at or below the sample's threshold distance.
+    rng = np.random.default_rng(78)
+    ds = rng.integers(0, 2 ** 32, size=(60000, 8), dtype=np.uint64).astype(np.uint32)
+    qs = rng.integers(0, 2 ** 32, size=(100, 8), dtype=np.uint64).astype(np.uint32)
+    qs[50:] = ds[rng.integers(0, 60000, 50)] ^ (1 << rng.integers(0, 32, size=(50, 8))).astype(np.uint32)
+    ids, dist = lsh.hamming_topc(ds, qs, 64)
+    oids, odist = orc.sketch_topc(0, ds, qs, 64)
+    np.testing.assert_array_equal(ids, oids)
+    np.testing.assert_array_equal(dist, odist)
+
+
+@pytest.mark.gpu
+def test_hamming_topc_sampled_threshold_falls_back(lsh, orc):
+    # Query 0: the sampled rows are its only near points, so the sample's
+    # threshold admits fewer than C points.  Query 1: 600 unsampled copies of
+    # it overflow the pass.  Both are redone with the rounds.
+    rng = np.random.default_rng(79)
+    ds = rng.integers(0, 2 ** 32, size=(60000, 8), dtype=np.uint64).astype(np.uint32)
+    qs = rng.integers(0, 2 ** 32, size=(40, 8), dtype=np.uint64).astype(np.uint32)
+    ds[np.arange(8) * 16] = qs[0] ^ np.array([1, 0, 0, 0, 0, 0, 0, 0], np.uint32)
+    copies = np.arange(600) * 16 + 7
+    ds[copies] = qs[1]
+    ids, dist = lsh.hamming_topc(ds, qs, 64)
+    oids, odist = orc.sketch_topc(0, ds, qs, 64)
+    np.testing.assert_array_equal(ids, oids)
+    np.testing.assert_array_equal(dist, odist)
+    assert (dist[1] == 0).all() and (ids[1] == copies[:64]).all()
+
+
 # ------------------------------------------------------------ full pipelines
 
 def test_simhash_pipeline_matches_oracle(lsh, orc):
