@@ -147,7 +147,7 @@ void sketch_topk(const SketchSet& d, const SketchSet* sample, int wpad, int kind
         launch_threshold_above(sa.view(), st.view(), s);
         pt.begin(PH_SCAN, s);
         pt.begin(PH_MFMA, s);
-        launch_hamming_mfma(d.ds, d.fp4, qs, wpad, Q, 0, d.N, st.view(), s);
+        launch_hamming_mfma(d.fp4, qs, wpad, Q, 0, d.N, st.view(), s);
         pt.end(PH_MFMA, s);
         pt.end(PH_SCAN, s);
         ++launches;
@@ -182,9 +182,9 @@ void sketch_topk(const SketchSet& d, const SketchSet* sample, int wpad, int kind
         // a 256-query x 32-point tile yields about 256*32*K/p0 <= 512 of them.
         // The re-run of overflowed queries (fixed) appends straight to global
         // memory, whose capacity per round it never exceeds.
-        if (mfma && !fixed && p0 >= 16 * (int64_t)K) {
+        if (mfma && d.fp4 && !fixed && p0 >= 16 * (int64_t)K) {
             pt.begin(PH_MFMA, s);
-            launch_hamming_mfma(d.ds, d.fp4, qs, wpad, Q, p0, p1, v, s);
+            launch_hamming_mfma(d.fp4, qs, wpad, Q, p0, p1, v, s);
             pt.end(PH_MFMA, s);
             if (mfma_launches) ++*mfma_launches;
             if (mfma_points) *mfma_points += p1 - p0;

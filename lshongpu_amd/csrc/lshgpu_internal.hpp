@@ -255,9 +255,7 @@ int hamming_mfma_query_rows();
 bool hamming_mfma_supported(int words);
 size_t hamming_fp4_words(int64_t N, int words);
 void launch_expand_fp4(const uint32_t* ds, int64_t N, int words, uint32_t* frag, hipStream_t s);
-// ds: packed [N][words]; frag: the FP4 image, or null to expand ds on the fly.
-void launch_hamming_mfma(const uint32_t* ds, const uint32_t* frag, const uint32_t* qs, int words, int Q, int64_t p0,
-                         int64_t p1,
+void launch_hamming_mfma(const uint32_t* frag, const uint32_t* qs, int words, int Q, int64_t p0, int64_t p1,
                          const TopKView& v, hipStream_t s);
 
 // minhash.hip

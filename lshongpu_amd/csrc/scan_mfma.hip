@@ -68,6 +68,7 @@ constexpr int WAVE_QUERIES = MQ * 32;      // 64: one query per lane in the flus
 constexpr int WG_QUERIES = WAVES * WAVE_QUERIES;
 constexpr int TPB = WAVES * 64;
 constexpr int POOLW = 512;                 // queued candidates per wave
+constexpr int NBUF = 2;                    // staged point tiles in flight
 
 __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -75,12 +76,9 @@ __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// KS = sketch words / 2 (K-steps of 64 bits).  FP4IMG: stage the B operands
-// from the FP4 operand image with direct global-to-LDS loads into a double
-// buffer; otherwise load the packed words and expand them in registers.
-template <int KS, bool FP4IMG>
-__global__ __launch_bounds__(TPB, 2) void hamming_mfma_kernel(const uint32_t* __restrict__ ds,
-                                                              const v4i* __restrict__ frag,
+// KS = sketch words / 2 (K-steps of 64 bits).
+template <int KS>
+__global__ __launch_bounds__(TPB, 2) void hamming_mfma_kernel(const v4i* __restrict__ frag,
                                                               const uint32_t* __restrict__ qs, int Q,
                                                               int64_t p0, int64_t p1, int64_t tile_begin,
                                                               int64_t tile_end, int tiles_per_wg, int qblocks,
@@ -88,10 +86,8 @@ __global__ __launch_bounds__(TPB, 2) void hamming_mfma_kernel(const uint32_t* __
     constexpr int W = 2 * KS;
     constexpr int BITS = 32 * W;
     constexpr int SPW = (KS + WAVES - 1) / WAVES;  // K-steps staged per wave
-    constexpr int NBUF = FP4IMG ? 2 : 1;
-    __shared__ v4i bfrag[NBUF][KS * 64];           // B operands of the tile(s) in flight
+    __shared__ v4i bfrag[NBUF][KS * 64];           // point tiles in flight (A operands)
     __shared__ float tf[WG_QUERIES];               // pass iff dot > tf (Hamming < threshold distance)
-    __shared__ uint16_t spill[WAVES][MQ * 16][64]; // Hamming distances of a tile with candidates
     __shared__ uint32_t lcnt[WAVES][WAVE_QUERIES];
     __shared__ uint32_t lbase[WAVES][WAVE_QUERIES];
     __shared__ uint64_t pkey[WAVES][POOLW];
@@ -118,21 +114,24 @@ __global__ __launch_bounds__(TPB, 2) void hamming_mfma_kernel(const uint32_t* __
         tf[threadIdx.x] = f;
         lcnt[wave][lane] = 0;
     }
-    v4i afrag[MQ][KS];
+    // Query fragments (B operands), resident for the whole launch.
+    v4i qfrag[MQ][KS];
 #pragma unroll
     for (int t = 0; t < MQ; ++t) {
         const uint32_t* __restrict__ qr = qs + (size_t)(qwave + t * 32 + (lane & 31)) * W + (lane >> 5);
 #pragma unroll
-        for (int s = 0; s < KS; ++s) afrag[t][s] = expand_word(qr[2 * s]);
+        for (int s = 0; s < KS; ++s) qfrag[t][s] = expand_word(qr[2 * s]);
     }
     __syncthreads();
-    // Threshold of accumulator register r of query tile t of this lane.
-    auto thr_of = [&](int t, int r) -> float {
-        return tf[wave * WAVE_QUERIES + t * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3)];
-    };
+    // Lane l holds column l%32 (a query) of each 32x32 product; register r
+    // holds row (r/4)*8 + (l/32)*4 + r%4 (a point of the tile).
+    float thr[MQ];
+#pragma unroll
+    for (int t = 0; t < MQ; ++t) thr[t] = tf[wave * WAVE_QUERIES + t * 32 + (lane & 31)];
+    const int rowbase = (lane >> 5) * 4;
 
-    // The wave's queue: pool entries [0, queued); moved to the global top-K
-    // state with LDS ranks per query and one global atomic per query.
+    // The wave's queue holds `queued` entries; it is moved to the global
+    // top-K state with LDS ranks per query and one global atomic per query.
     uint32_t queued = 0;
     auto flush = [&]() {
         wave_sync();
@@ -163,110 +162,103 @@ __global__ __launch_bounds__(TPB, 2) void hamming_mfma_kernel(const uint32_t* __
 
     const int64_t t0 = tile_begin + (int64_t)slice * tiles_per_wg;
     const int64_t t1 = min(tile_end, t0 + tiles_per_wg);
-    // Wave w stages K-steps w, w+4, ...; lane l of K-step s holds word
-    // 2s + l/32 of point l%32 of the tile.
-    const int sp = lane & 31, sw = lane >> 5;
-    uint32_t next[SPW];
+    // Wave w stages K-steps w, w+4, ... of a point tile with direct
+    // global-to-LDS loads (16 bytes per lane, the operand layout).
     auto stage = [&](int64_t tile, int buf) {
-        if (FP4IMG) {
 #pragma unroll
-            for (int k = 0; k < SPW; ++k) {
-                const int st = wave + WAVES * k;
-                if (st < KS)
-                    __builtin_amdgcn_global_load_lds(
-                        (const void*)(frag + ((size_t)tile * KS + st) * 64 + lane),
-                        (__attribute__((address_space(3))) void*)&bfrag[buf][st * 64], 16, 0, 0);
-            }
-        } else {
-            const int64_t pi = tile * 32 + sp;
-#pragma unroll
-            for (int k = 0; k < SPW; ++k) {
-                const int st = wave + WAVES * k;
-                next[k] = (st < KS && pi < p1) ? ds[(size_t)pi * W + 2 * st + sw] : 0u;
-            }
+        for (int k = 0; k < SPW; ++k) {
+            const int st = wave + WAVES * k;
+            if (st < KS)
+                __builtin_amdgcn_global_load_lds((const void*)(frag + ((size_t)tile * KS + st) * 64 + lane),
+                                                 (__attribute__((address_space(3))) void*)&bfrag[buf][st * 64], 16,
+                                                 0, 0);
         }
     };
     if (t0 < t1) stage(t0, 0);
     for (int64_t tile = t0; tile < t1; ++tile) {
-        const int buf = FP4IMG ? (int)((tile - t0) & 1) : 0;
-        if (FP4IMG) {
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-            __syncthreads();  // the tile's operands are in LDS; the previous tile is done
-            if (tile + 1 < t1) stage(tile + 1, buf ^ 1);
-        } else {
-            __syncthreads();  // the previous tile's reads of bfrag are complete
-#pragma unroll
-            for (int k = 0; k < SPW; ++k) {
-                const int st = wave + WAVES * k;
-                if (st < KS) bfrag[0][st * 64 + lane] = expand_word(next[k]);
-            }
-            if (tile + 1 < t1) stage(tile + 1, 0);
-            __syncthreads();
-        }
+        const int buf = (int)((tile - t0) % NBUF);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();  // the tile's operands are in LDS; the previous tile's reads are done
+        if (tile + 1 < t1) stage(tile + 1, (buf + 1) % NBUF);
 
         v16f acc[MQ];
 #pragma unroll
         for (int t = 0; t < MQ; ++t) acc[t] = (v16f){};
 #pragma unroll
         for (int s0 = 0; s0 < KS; s0 += 4) {
-            v4i b[4];
+            v4i a[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) b[i] = bfrag[buf][(s0 + i) * 64 + lane];
+            for (int i = 0; i < 4; ++i) a[i] = bfrag[buf][(s0 + i) * 64 + lane];
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int t = 0; t < MQ; ++t)
                     acc[t] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
-                        widen(afrag[t][s0 + i]), widen(b[i]), acc[t], 4, 4, 0, 0x7F7F7F7F, 0, 0x7F7F7F7F);
+                        widen(a[i]), widen(qfrag[t][s0 + i]), acc[t], 4, 4, 0, 0x7F7F7F7F, 0, 0x7F7F7F7F);
             __builtin_amdgcn_sched_barrier(0);
         }
 
         // A pair passes when its Hamming distance is below the query's
         // threshold distance.  Equal distances never pass: every key of the
-        // best list has a smaller ID than the points of this range.
-        const int64_t pi = tile * 32 + sp;
-        const bool pvalid = pi >= p0 && pi < p1;
-        uint32_t pm = 0;  // bit t*16+r: accumulator (t, r) passes
-#pragma unroll
-        for (int t = 0; t < MQ; ++t)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) pm |= (acc[t][r] > thr_of(t, r) ? 1u : 0u) << (t * 16 + r);
-        if (!pvalid) pm = 0;
-        if (__ballot(pm != 0)) {
+        // best list has a smaller ID than the points of this range.  Each
+        // register is one compare and one ballot; the rare passing lanes
+        // take slots of the wave's queue in lane order.
+        const uint32_t pbase = (uint32_t)(tile * 32);
+        auto epilogue = [&](uint32_t vmask) {
+            // All compares first (their lane masks stay in scalar
+            // registers), then a scalar walk over the registers with hits.
+            uint64_t m[MQ][16];
+            uint64_t any = 0;
 #pragma unroll
             for (int t = 0; t < MQ; ++t)
 #pragma unroll
-                for (int r = 0; r < 16; ++r) spill[wave][t * 16 + r][lane] = (uint16_t)((BITS - (int)acc[t][r]) >> 1);
-            const uint32_t mine = (uint32_t)__popc(pm);
-            uint32_t incl = mine;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t x = __shfl_up(incl, o);
-                if (lane >= o) incl += x;
-            }
-            const uint32_t total = (uint32_t)__shfl((int)incl, 63);
-            wave_sync();
-            uint32_t e = queued + incl - mine;
-            for (uint32_t m = pm; m; m &= m - 1, ++e) {
-                const int bit = __ffs((int)m) - 1;
-                const int qrel = (bit >> 4) * 32 + ((bit & 15) >> 2) * 8 + sw * 4 + (bit & 3);
-                if (e < (uint32_t)POOLW) {
-                    pkey[wave][e] = make_key(spill[wave][bit][lane], (uint32_t)pi);
-                    pq[wave][e] = (uint8_t)qrel;
-                } else {
-                    v.ovf[qwave + qrel] = 1u;  // dropped: the query is redone
+                for (int r = 0; r < 16; ++r) {
+                    m[t][r] = __ballot(acc[t][r] > thr[t] && ((vmask >> r) & 1u));
+                    any |= m[t][r];
                 }
+            if (!any) return;
+#pragma unroll
+            for (int t = 0; t < MQ; ++t)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const uint64_t mr = m[t][r];
+                    if (!mr) continue;
+                    if ((mr >> lane) & 1ull) {
+                        const uint32_t e = queued + (uint32_t)__popcll(mr & ((1ull << lane) - 1ull));
+                        const int qrel = t * 32 + (lane & 31);
+                        if (e < (uint32_t)POOLW) {
+                            const uint32_t pi = pbase + (uint32_t)((r >> 2) * 8 + rowbase + (r & 3));
+                            pkey[wave][e] = make_key((uint32_t)((BITS - (int)acc[t][r]) >> 1), pi);
+                            pq[wave][e] = (uint8_t)qrel;
+                        } else {
+                            v.ovf[qwave + qrel] = 1u;  // dropped: the query is redone
+                        }
+                    }
+                    queued += (uint32_t)__popcll(mr);
+                }
+        };
+        if ((int64_t)pbase >= p0 && (int64_t)pbase + 32 <= p1) {
+            epilogue(0xFFFFu);
+        } else {
+            // Rows of a tile cut by the range: bit r set when row
+            // (r/4)*8 + rowbase + r%4 lies in [p0, p1).
+            uint32_t vm = 0;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int64_t pi = (int64_t)pbase + (r >> 2) * 8 + rowbase + (r & 3);
+                vm |= (pi >= p0 && pi < p1 ? 1u : 0u) << r;
             }
-            queued += total;
-            if (queued > POOLW / 2) flush();
+            asm volatile("" : "+v"(vm));  // keep the mask; do not re-derive it per register
+            epilogue(vm);
         }
+        if (queued > POOLW / 2) flush();
     }
     flush();
 }
 
 template <int KS>
-void launch_ks(const uint32_t* ds, const v4i* frag, const uint32_t* qs, int Q, int64_t p0, int64_t p1,
-               const TopKView& v, hipStream_t s) {
+void launch_ks(const v4i* frag, const uint32_t* qs, int Q, int64_t p0, int64_t p1, const TopKView& v,
+               hipStream_t s) {
     const int qblocks = (Q + WG_QUERIES - 1) / WG_QUERIES;
     const int64_t tile_begin = p0 / 32, tile_end = (p1 + 31) / 32;
     const int64_t ntiles = tile_end - tile_begin;
@@ -277,12 +269,8 @@ void launch_ks(const uint32_t* ds, const v4i* frag, const uint32_t* qs, int Q, i
     const int64_t groups = (slices + 7) / 8;
     const int64_t blocks = 8 * (int64_t)qblocks * groups;
     LSH_REQUIRE(blocks < (int64_t)1 << 31, LSH_ERR_UNSUPPORTED, "matrix-core scan grid too large");
-    if (frag)
-        hipLaunchKernelGGL((hamming_mfma_kernel<KS, true>), dim3((unsigned)blocks), dim3(TPB), 0, s, ds, frag, qs, Q,
-                           p0, p1, tile_begin, tile_end, (int)per, qblocks, (int)slices, v);
-    else
-        hipLaunchKernelGGL((hamming_mfma_kernel<KS, false>), dim3((unsigned)blocks), dim3(TPB), 0, s, ds, frag, qs,
-                           Q, p0, p1, tile_begin, tile_end, (int)per, qblocks, (int)slices, v);
+    hipLaunchKernelGGL((hamming_mfma_kernel<KS>), dim3((unsigned)blocks), dim3(TPB), 0, s, frag, qs, Q, p0, p1,
+                       tile_begin, tile_end, (int)per, qblocks, (int)slices, v);
     check_launch("hamming_mfma_kernel");
 }
 
@@ -317,15 +305,16 @@ void launch_expand_fp4(const uint32_t* ds, int64_t N, int words, uint32_t* frag,
     check_launch("expand_fp4_kernel");
 }
 
-void launch_hamming_mfma(const uint32_t* ds, const uint32_t* frag, const uint32_t* qs, int words, int Q, int64_t p0,
-                         int64_t p1, const TopKView& v, hipStream_t s) {
+void launch_hamming_mfma(const uint32_t* frag, const uint32_t* qs, int words, int Q, int64_t p0, int64_t p1,
+                         const TopKView& v, hipStream_t s) {
     if (p1 <= p0 || Q <= 0) return;
+    LSH_REQUIRE(frag, LSH_ERR_INTERNAL, "matrix-core scan needs the FP4 image");
     const v4i* f = reinterpret_cast<const v4i*>(frag);
     switch (words) {
-        case 8: launch_ks<4>(ds, f, qs, Q, p0, p1, v, s); break;
-        case 16: launch_ks<8>(ds, f, qs, Q, p0, p1, v, s); break;
-        case 24: launch_ks<12>(ds, f, qs, Q, p0, p1, v, s); break;
-        case 32: launch_ks<16>(ds, f, qs, Q, p0, p1, v, s); break;
+        case 8: launch_ks<4>(f, qs, Q, p0, p1, v, s); break;
+        case 16: launch_ks<8>(f, qs, Q, p0, p1, v, s); break;
+        case 24: launch_ks<12>(f, qs, Q, p0, p1, v, s); break;
+        case 32: launch_ks<16>(f, qs, Q, p0, p1, v, s); break;
         default: fail(LSH_ERR_INTERNAL, "MFMA Hamming scan needs 8, 16, 24 or 32 words");
     }
 }
