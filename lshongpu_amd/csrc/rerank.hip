@@ -39,40 +39,64 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
     const int gl = threadIdx.x & 15;
     const int ngroups = blockDim.x >> 4;
     const bool vec4 = (d & 3) == 0;
-    for (int t = threadIdx.x >> 4; t < Cp; t += ngroups) {
-        int32_t id = -1;
-        if (t < C) {
-            if (cand) {
-                id = cand[(size_t)q * C + t];
-            } else {
-                const uint64_t key = keys[(size_t)q * key_stride + t];
-                id = key == kEmptyKey ? -1 : (int32_t)(uint32_t)key;
-            }
-        }
-        float a = 0.0f, b = 0.0f;
-        if (id >= 0) {
-            const float* __restrict__ x = data + (size_t)id * d;
-            if (vec4) {
-                for (int j = gl * 4; j < d; j += 64) {
-                    const float4 xv = *reinterpret_cast<const float4*>(x + j);
-                    const float4 yv = *reinterpret_cast<const float4*>(qv + j);
-                    group_accumulate(kind, xv.x, yv.x, a, b);
-                    group_accumulate(kind, xv.y, yv.y, a, b);
-                    group_accumulate(kind, xv.z, yv.z, a, b);
-                    group_accumulate(kind, xv.w, yv.w, a, b);
+    // Each group takes U rows per step so that U row reads are in flight.
+    constexpr int U = 4;
+    for (int t0 = threadIdx.x >> 4; t0 < Cp; t0 += ngroups * U) {
+        int32_t id[U];
+        float a[U], b[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int t = t0 + u * ngroups;
+            id[u] = -1;
+            if (t < C) {
+                if (cand) {
+                    id[u] = cand[(size_t)q * C + t];
+                } else {
+                    const uint64_t key = keys[(size_t)q * key_stride + t];
+                    id[u] = key == kEmptyKey ? -1 : (int32_t)(uint32_t)key;
                 }
-            } else {
-                for (int j = gl; j < d; j += 16) group_accumulate(kind, x[j], qv[j], a, b);
+            }
+            a[u] = 0.0f;
+            b[u] = 0.0f;
+        }
+        if (vec4) {
+            for (int j = gl * 4; j < d; j += 64) {
+                const float4 yv = *reinterpret_cast<const float4*>(qv + j);
+                float4 xv[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    xv[u] = id[u] >= 0 ? *reinterpret_cast<const float4*>(data + (size_t)id[u] * d + j)
+                                       : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    group_accumulate(kind, xv[u].x, yv.x, a[u], b[u]);
+                    group_accumulate(kind, xv[u].y, yv.y, a[u], b[u]);
+                    group_accumulate(kind, xv[u].z, yv.z, a[u], b[u]);
+                    group_accumulate(kind, xv[u].w, yv.w, a[u], b[u]);
+                }
+            }
+        } else {
+            for (int j = gl; j < d; j += 16) {
+                const float y = qv[j];
+                float xs[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) xs[u] = id[u] >= 0 ? data[(size_t)id[u] * d + j] : 0.0f;
+#pragma unroll
+                for (int u = 0; u < U; ++u) group_accumulate(kind, xs[u], y, a[u], b[u]);
             }
         }
 #pragma unroll
-        for (int o = 8; o > 0; o >>= 1) {
-            a += __shfl_xor(a, o, 16);
-            b += __shfl_xor(b, o, 16);
-        }
-        if (gl == 0) {
-            const float dist = group_distance(kind, a, b, qn);
-            k32[t] = id < 0 ? kEmptyKey : make_key(f32_key(dist), (uint32_t)id);
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int o = 8; o > 0; o >>= 1) {
+                a[u] += __shfl_xor(a[u], o, 16);
+                b[u] += __shfl_xor(b[u], o, 16);
+            }
+            const int t = t0 + u * ngroups;
+            if (gl == 0 && t < Cp) {
+                const float dist = group_distance(kind, a[u], b[u], qn);
+                k32[t] = id[u] < 0 ? kEmptyKey : make_key(f32_key(dist), (uint32_t)id[u]);
+            }
         }
     }
     bitonic_sort_lds(k32, Cp);

@@ -67,8 +67,8 @@ constexpr int WAVES = 4;
 constexpr int WAVE_QUERIES = MQ * 32;      // 64: one query per lane in the flush
 constexpr int WG_QUERIES = WAVES * WAVE_QUERIES;
 constexpr int TPB = WAVES * 64;
-constexpr int POOLW = 512;                 // queued candidates per wave
-constexpr int NBUF = 2;                    // staged point tiles in flight
+constexpr int POOLW = 256;                 // queued candidates per wave
+constexpr int TPS = 2;                     // point tiles per staging step
 
 __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -86,7 +86,8 @@ __global__ __launch_bounds__(TPB, 2) void hamming_mfma_kernel(const v4i* __restr
     constexpr int W = 2 * KS;
     constexpr int BITS = 32 * W;
     constexpr int SPW = (KS + WAVES - 1) / WAVES;  // K-steps staged per wave
-    __shared__ v4i bfrag[NBUF][KS * 64];           // point tiles in flight (A operands)
+    __shared__ v4i bufA[TPS * KS * 64];            // point tiles in flight (A operands); two
+    __shared__ v4i bufB[TPS * KS * 64];            // arrays so reads of one overlap loads of the other
     __shared__ float tf[WG_QUERIES];               // pass iff dot > tf (Hamming < threshold distance)
     __shared__ uint32_t lcnt[WAVES][WAVE_QUERIES];
     __shared__ uint32_t lbase[WAVES][WAVE_QUERIES];
@@ -164,23 +165,24 @@ __global__ __launch_bounds__(TPB, 2) void hamming_mfma_kernel(const v4i* __restr
     const int64_t t1 = min(tile_end, t0 + tiles_per_wg);
     // Wave w stages K-steps w, w+4, ... of a point tile with direct
     // global-to-LDS loads (16 bytes per lane, the operand layout).
-    auto stage = [&](int64_t tile, int buf) {
+    // Staging step u holds tiles t0 + u*TPS .. + TPS-1 (those below t1).
+    auto stage = [&](int64_t u, v4i* buf) {
 #pragma unroll
-        for (int k = 0; k < SPW; ++k) {
-            const int st = wave + WAVES * k;
-            if (st < KS)
-                __builtin_amdgcn_global_load_lds((const void*)(frag + ((size_t)tile * KS + st) * 64 + lane),
-                                                 (__attribute__((address_space(3))) void*)&bfrag[buf][st * 64], 16,
-                                                 0, 0);
+        for (int i = 0; i < TPS; ++i) {
+            const int64_t tile = t0 + u * TPS + i;
+            if (tile >= t1) break;
+#pragma unroll
+            for (int k = 0; k < SPW; ++k) {
+                const int st = wave + WAVES * k;
+                if (st < KS)
+                    __builtin_amdgcn_global_load_lds(
+                        (const void*)(frag + ((size_t)tile * KS + st) * 64 + lane),
+                        (__attribute__((address_space(3))) void*)&buf[(i * KS + st) * 64], 16, 0, 0);
+            }
         }
     };
-    if (t0 < t1) stage(t0, 0);
-    for (int64_t tile = t0; tile < t1; ++tile) {
-        const int buf = (int)((tile - t0) % NBUF);
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        __syncthreads();  // the tile's operands are in LDS; the previous tile's reads are done
-        if (tile + 1 < t1) stage(tile + 1, (buf + 1) % NBUF);
-
+    // One point tile: products from the staged operands, then the epilogue.
+    auto process = [&](int64_t tile, const v4i* buf) {
         v16f acc[MQ];
 #pragma unroll
         for (int t = 0; t < MQ; ++t) acc[t] = (v16f){};
@@ -188,7 +190,7 @@ __global__ __launch_bounds__(TPB, 2) void hamming_mfma_kernel(const v4i* __restr
         for (int s0 = 0; s0 < KS; s0 += 4) {
             v4i a[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = bfrag[buf][(s0 + i) * 64 + lane];
+            for (int i = 0; i < 4; ++i) a[i] = buf[(s0 + i) * 64 + lane];
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -226,12 +228,17 @@ __global__ __launch_bounds__(TPB, 2) void hamming_mfma_kernel(const v4i* __restr
                     if ((mr >> lane) & 1ull) {
                         const uint32_t e = queued + (uint32_t)__popcll(mr & ((1ull << lane) - 1ull));
                         const int qrel = t * 32 + (lane & 31);
+                        const uint32_t pi = pbase + (uint32_t)((r >> 2) * 8 + rowbase + (r & 3));
+                        const uint64_t key = make_key((uint32_t)((BITS - (int)acc[t][r]) >> 1), pi);
                         if (e < (uint32_t)POOLW) {
-                            const uint32_t pi = pbase + (uint32_t)((r >> 2) * 8 + rowbase + (r & 3));
-                            pkey[wave][e] = make_key((uint32_t)((BITS - (int)acc[t][r]) >> 1), pi);
+                            pkey[wave][e] = key;
                             pq[wave][e] = (uint8_t)qrel;
                         } else {
-                            v.ovf[qwave + qrel] = 1u;  // dropped: the query is redone
+                            // Queue full (a dense tile): append straight to the
+                            // query's global slots; the select kernel counts
+                            // any excess as an overflow.
+                            const uint32_t slot = atomicAdd(&v.cnt[qwave + qrel], 1u);
+                            if (slot < (uint32_t)v.newcap) v.keys[(size_t)(qwave + qrel) * v.stride + v.K + slot] = key;
                         }
                     }
                     queued += (uint32_t)__popcll(mr);
@@ -252,8 +259,44 @@ __global__ __launch_bounds__(TPB, 2) void hamming_mfma_kernel(const v4i* __restr
             epilogue(vm);
         }
         if (queued > POOLW / 2) flush();
+    };
+
+    // Staging steps alternate between the two buffers: after the barrier
+    // that publishes step u, the loads of step u+1 go to the other buffer and
+    // overlap the products of step u.
+    const int64_t steps = (t1 - t0 + TPS - 1) / TPS;
+    auto process_step = [&](int64_t u, const v4i* buf) {
+#pragma unroll
+        for (int i = 0; i < TPS; ++i) {
+            const int64_t tile = t0 + u * TPS + i;
+            if (tile < t1) process(tile, buf + i * KS * 64);
+        }
+    };
+    if (steps > 0) stage(0, bufA);
+    for (int64_t u = 0; u < steps; u += 2) {
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();  // step u is in bufA; bufB is free
+        if (u + 1 < steps) stage(u + 1, bufB);
+        process_step(u, bufA);
+        if (u + 1 >= steps) break;
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();  // step u+1 is in bufB; bufA is free
+        if (u + 2 < steps) stage(u + 2, bufA);
+        process_step(u + 1, bufB);
     }
     flush();
+}
+
+int wg_slots() {
+    // Two workgroups of this kernel fit a CU (LDS and registers).
+    static const int slots = [] {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        return 2 * cus;
+    }();
+    return slots;
 }
 
 template <int KS>
@@ -262,11 +305,27 @@ void launch_ks(const v4i* frag, const uint32_t* qs, int Q, int64_t p0, int64_t p
     const int qblocks = (Q + WG_QUERIES - 1) / WG_QUERIES;
     const int64_t tile_begin = p0 / 32, tile_end = (p1 + 31) / 32;
     const int64_t ntiles = tile_end - tile_begin;
-    // About 2048 workgroups, at least 4 tiles each.
-    int64_t slices = std::max<int64_t>(1, std::min<int64_t>(ntiles / 4, (2048 + qblocks - 1) / qblocks));
-    const int64_t per = (ntiles + slices - 1) / slices;
+    // At least ~2048 workgroups of at least 2*TPS tiles each; among the
+    // group counts up to twice the minimum, the one whose workgroups fill
+    // the last wave of the machine best.
+    const int64_t gmax = std::max<int64_t>(1, ntiles / (2 * TPS * 8));
+    const int64_t gmin = std::min<int64_t>(gmax, std::max<int64_t>(1, (2048 + 8 * qblocks - 1) / (8 * qblocks)));
+    int64_t groups = gmin;
+    double best = -1.0;
+    const int slots = wg_slots();
+    for (int64_t g = gmin; g <= std::min<int64_t>(gmax, 2 * gmin); ++g) {
+        const int64_t b = 8 * (int64_t)qblocks * g;
+        const double eff = (double)b / (double)(((b + slots - 1) / slots) * slots);
+        if (eff > best + 1e-9) {
+            best = eff;
+            groups = g;
+        }
+    }
+    int64_t slices = 8 * groups;
+    int64_t per = (ntiles + slices - 1) / slices;
+    per = (per + TPS - 1) / TPS * TPS;
     slices = (ntiles + per - 1) / per;
-    const int64_t groups = (slices + 7) / 8;
+    groups = (slices + 7) / 8;
     const int64_t blocks = 8 * (int64_t)qblocks * groups;
     LSH_REQUIRE(blocks < (int64_t)1 << 31, LSH_ERR_UNSUPPORTED, "matrix-core scan grid too large");
     hipLaunchKernelGGL((hamming_mfma_kernel<KS>), dim3((unsigned)blocks), dim3(TPB), 0, s, frag, qs, Q, p0, p1,
