@@ -54,15 +54,21 @@ std::vector<int32_t> flagged(const uint32_t* dflags, int Q, hipStream_t s) {
 }
 
 // Rounds over points [0, N) (see the file comment).  `scan(p0, p1, view)`
-// launches the scan kernel of one range.
+// launches the scan kernel of one range.  kNormal sizes the ranges for
+// about newcap/2 appends per query, kCautious for newcap/8 (the re-run of
+// queries that overflowed), kFixed keeps every range at newcap points, which
+// cannot overflow.
+enum RoundsMode : int { kNormal = 0, kCautious = 1, kFixed = 2 };
+
 template <class ScanFn>
-void run_rounds(TopKState& st, int Q, int K, int64_t N, bool fixed, ScanFn&& scan, hipStream_t s,
+void run_rounds(TopKState& st, int Q, int K, int64_t N, int mode, ScanFn&& scan, hipStream_t s,
                 PhaseTimer& pt, int64_t& launches, bool int_keys = false) {
     const int newcap = newcap_for(K);
     st.reset(Q, K, newcap, s, int_keys);
     const TopKView v = st.view();
     int64_t p = 0;
     int64_t chunk = std::min<int64_t>(N, newcap);
+    const double target = mode == kCautious ? 8.0 : 2.0;
     while (p < N) {
         const int64_t p1 = std::min<int64_t>(N, p + chunk);
         pt.begin(PH_SCAN, s);
@@ -72,7 +78,9 @@ void run_rounds(TopKState& st, int Q, int K, int64_t N, bool fixed, ScanFn&& sca
         pt.begin(PH_SELECT, s);
         launch_select_merge(v, s);
         pt.end(PH_SELECT, s);
-        if (!fixed) chunk = std::max<int64_t>(256, (int64_t)((double)p1 * newcap / (2.0 * K)));
+        if (mode != kFixed)
+            chunk = std::max<int64_t>(std::min<int64_t>(chunk, newcap),
+                                      (int64_t)((double)p1 * newcap / (target * K)));
         p = p1;
     }
 }
@@ -84,32 +92,32 @@ struct ExactQueries {
 };
 
 void exact_topk(const float* data, const float* data_inv, int64_t N, int d, int kind,
-                const ExactQueries& eq, int K, bool fixed, TopKState& st, hipStream_t s, PhaseTimer& pt,
-                int64_t& launches) {
+                const ExactQueries& eq, int K, int mode, TopKState& st, Scratch& sc, int depth, hipStream_t s,
+                PhaseTimer& pt, int64_t& launches) {
     const int QG = exact_scan_query_group();
     const int groups = (eq.Q + QG - 1) / QG;
-    DevBuf<float> qT((size_t)groups * d * QG);
-    DevBuf<float> qinv(angular_kind(kind) ? (size_t)eq.Q : 0);
-    launch_transpose_queries(eq.rows, eq.Q, d, qT.get(), s);
-    if (angular_kind(kind)) launch_row_inv_norms(eq.rows, eq.Q, d, qinv.get(), s);
+    float* qT = sc.get<float>(depth, 0, (size_t)groups * d * QG);
+    float* qinv = sc.get<float>(depth, 1, (size_t)eq.Q);
+    launch_transpose_queries(eq.rows, eq.Q, d, qT, s);
+    if (angular_kind(kind)) launch_row_inv_norms(eq.rows, eq.Q, d, qinv, s);
     auto scan = [&](int64_t p0, int64_t p1, const TopKView& v) {
-        ExactScanArgs a{data, data_inv, qT.get(), qinv.get(), d, eq.Q, p0, p1, kind};
+        ExactScanArgs a{data, data_inv, qT, qinv, d, eq.Q, p0, p1, kind};
         launch_exact_scan(a, v, s);
     };
-    run_rounds(st, eq.Q, K, N, fixed, scan, s, pt, launches);
-    if (fixed) return;
+    run_rounds(st, eq.Q, K, N, mode, scan, s, pt, launches);
+    if (mode == kFixed) return;
     const std::vector<int32_t> ovf = flagged(st.view().ovf, eq.Q, s);
     if (ovf.empty()) return;
     const int m = (int)ovf.size();
-    DevBuf<int32_t> idx((size_t)m);
-    LSH_HIP(hipMemcpyAsync(idx.get(), ovf.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, s));
-    DevBuf<float> sub((size_t)m * d);
-    launch_gather_rows_f32(eq.rows, d, idx.get(), m, sub.get(), s);
-    TopKState sst;
-    exact_topk(data, data_inv, N, d, kind, ExactQueries{sub.get(), m}, K, true, sst, s, pt, launches);
+    int32_t* idx = sc.get<int32_t>(depth, 2, (size_t)m);
+    LSH_HIP(hipMemcpyAsync(idx, ovf.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, s));
+    float* sub = sc.get<float>(depth, 3, (size_t)m * d);
+    launch_gather_rows_f32(eq.rows, d, idx, m, sub, s);
+    TopKState& sst = sc.tk[depth];
+    exact_topk(data, data_inv, N, d, kind, ExactQueries{sub, m}, K, mode == kNormal ? kCautious : kFixed, sst, sc,
+               depth + 1, s, pt, launches);
     const TopKView sv = sst.view(), v = st.view();
-    launch_scatter_rows_u64(sv.keys, sv.stride, K, idx.get(), m, v.keys, v.stride, s);
-    LSH_HIP(hipStreamSynchronize(s));
+    launch_scatter_rows_u64(sv.keys, sv.stride, K, idx, m, v.keys, v.stride, s);
 }
 
 // Packed integer sketches [N][wpad] and, for Hamming scans, their FP4 image.
@@ -119,31 +127,63 @@ struct SketchSet {
     int64_t N;
 };
 
+// Rows of a level per row of its sample.
+int sample_step(int64_t n, int64_t sample_n) { return (int)((n + sample_n - 1) / sample_n); }
+
+// Sample candidates Ks for K over a level sampled every `stride` rows: with
+// Ks*stride points expected at or below the sample's Ks-th distance and a
+// standard deviation of about sqrt(Ks)*stride, Ks solves
+// Ks*stride = K + 4*sqrt(Ks)*stride, so fewer than K points is a 4-sigma
+// event (the query is then redone).
+int sample_k(int K, int stride) {
+    const double r = (double)K / stride;
+    const double x = (4.0 + std::sqrt(16.0 + 4.0 * r)) / 2.0;
+    return std::min(kMaxIntTopK, std::max(1, (int)std::ceil(x * x)));
+}
+
 // Rows the query sketch buffer of an integer sketch scan must have.
 int sketch_query_rows(int Q) { return round_up(std::max(Q, 1), 256); }
 
 // Integer sketch scan: qs has sketch_query_rows(Q) rows of wpad words (zero
 // past Q).  Hamming scans of up to 1024 bits run on the matrix cores.
 //
-// With a strided sample of the data (every stride-th row), a Hamming scan
-// first finds the exact Ks-th nearest sample distance T of every query
-// (Ks = 1.45*K/stride), then makes one pass over all points appending the
-// pairs at distance <= T (about 1.45*K per query instead of the ~K*ln(N/K)
-// of the geometric rounds), and selects the K best of them.  A query whose
-// pass overflowed, or found fewer than K pairs, is redone with the rounds,
-// so the result is the exact top-K either way.
-void sketch_topk(const SketchSet& d, const SketchSet* sample, int wpad, int kind, const uint32_t* qs, int Q,
-                 int K, bool fixed, TopKState& st, hipStream_t s, PhaseTimer& pt, int64_t& launches,
-                 int64_t* mfma_launches = nullptr, int64_t* mfma_points = nullptr) {
+// With strided samples of the data (samples[0] every stride-th row of d,
+// samples[1] of samples[0], ...), a Hamming scan first finds the exact
+// Ks-th nearest sample distance T of every query (Ks from sample_k, the
+// same way one level down), then makes one pass over all points appending
+// the pairs at distance <= T (about Ks*stride per query instead of the
+// ~K*ln(N/K) of the geometric rounds), and selects the K best of them.  A
+// query whose pass overflowed, or found fewer than K pairs, is redone with
+// the rounds, so the result is the exact top-K either way.
+void sketch_topk(const SketchSet& d, const SketchSet* samples, int nsamples, int wpad, int kind,
+                 const uint32_t* qs, int Q, int K, int mode, TopKState& st, Scratch& sc, int depth, hipStream_t s,
+                 PhaseTimer& pt, int64_t& launches, int64_t* mfma_launches = nullptr,
+                 int64_t* mfma_points = nullptr) {
     const bool mfma = kind == 0 && hamming_mfma_supported(wpad) && !force_valu_scan();
     const bool int_keys = K <= kMaxIntTopK;
-    if (mfma && !fixed && sample && sample->N > 0 && d.fp4 && int_keys) {
-        const int stride = (int)((d.N + sample->N - 1) / sample->N);
-        const int Ks = std::min<int>(kMaxIntTopK, std::max(1, (int)std::ceil(1.45 * K / stride)));
-        TopKState sa;
-        sketch_topk(*sample, nullptr, wpad, kind, qs, Q, Ks, false, sa, s, pt, launches, mfma_launches,
-                    mfma_points);
-        st.reset(Q, K, newcap_for(K), s, true);
+    // Re-runs of the queries in `bad` (rows of qs) with the rounds in `rmode`,
+    // scattered back into st.
+    auto redo = [&](const std::vector<int32_t>& bad, int rmode) {
+        const int m = (int)bad.size();
+        const int mpad = sketch_query_rows(m);
+        std::vector<int32_t> hidx(bad);
+        hidx.resize((size_t)mpad, bad[0]);
+        int32_t* idx = sc.get<int32_t>(depth, 1, (size_t)mpad);
+        LSH_HIP(hipMemcpyAsync(idx, hidx.data(), sizeof(int32_t) * mpad, hipMemcpyHostToDevice, s));
+        uint32_t* sub = sc.get<uint32_t>(depth, 2, (size_t)mpad * wpad);
+        launch_gather_rows_u32(qs, wpad, idx, mpad, sub, s);
+        TopKState& sst = sc.tk[depth];
+        sketch_topk(SketchSet{d.ds, rmode == kFixed ? nullptr : d.fp4, d.N}, nullptr, 0, wpad, kind, sub, m, K,
+                    rmode, sst, sc, depth + 1, s, pt, launches, mfma_launches, mfma_points);
+        const TopKView sv = sst.view(), v = st.view();
+        launch_scatter_rows_u64(sv.keys, sv.stride, K, idx, m, v.keys, v.stride, s);
+    };
+    if (mfma && mode == kNormal && nsamples > 0 && samples[0].N > 0 && d.fp4 && int_keys) {
+        const int Ks = sample_k(K, sample_step(d.N, samples[0].N));
+        TopKState& sa = sc.tk[depth];
+        sketch_topk(samples[0], samples + 1, nsamples - 1, wpad, kind, qs, Q, Ks, kNormal, sa, sc, depth + 1, s,
+                    pt, launches, mfma_launches, mfma_points);
+        st.reset(Q, K, 2 * K + 512, s, true);
         launch_threshold_above(sa.view(), st.view(), s);
         pt.begin(PH_SCAN, s);
         pt.begin(PH_MFMA, s);
@@ -156,33 +196,17 @@ void sketch_topk(const SketchSet& d, const SketchSet* sample, int wpad, int kind
         pt.begin(PH_SELECT, s);
         launch_select_merge(st.view(), s);
         pt.end(PH_SELECT, s);
-        DevBuf<uint32_t> flags((size_t)Q);
-        launch_incomplete_flags(st.view(), flags.get(), s);
-        const std::vector<int32_t> bad = flagged(flags.get(), Q, s);
-        if (bad.empty()) return;
-        const int m = (int)bad.size();
-        const int mpad = sketch_query_rows(m);
-        std::vector<int32_t> hidx(bad);
-        hidx.resize((size_t)mpad, bad[0]);
-        DevBuf<int32_t> idx((size_t)mpad);
-        LSH_HIP(hipMemcpyAsync(idx.get(), hidx.data(), sizeof(int32_t) * mpad, hipMemcpyHostToDevice, s));
-        DevBuf<uint32_t> sub((size_t)mpad * wpad);
-        launch_gather_rows_u32(qs, wpad, idx.get(), mpad, sub.get(), s);
-        TopKState sst;
-        sketch_topk(d, nullptr, wpad, kind, sub.get(), m, K, false, sst, s, pt, launches, mfma_launches,
-                    mfma_points);
-        const TopKView sv = sst.view(), v = st.view();
-        launch_scatter_rows_u64(sv.keys, sv.stride, K, idx.get(), m, v.keys, v.stride, s);
-        LSH_HIP(hipStreamSynchronize(s));
+        uint32_t* flags = sc.get<uint32_t>(depth, 0, (size_t)Q);
+        launch_incomplete_flags(st.view(), flags, s);
+        const std::vector<int32_t> bad = flagged(flags, Q, s);
+        if (!bad.empty()) redo(bad, kCautious);
         return;
     }
     auto scan = [&](int64_t p0, int64_t p1, const TopKView& v) {
         // The matrix-core scan queues candidates in LDS, which suits ranges
         // where few pairs pass: once 16*K points are behind every threshold,
         // a 256-query x 32-point tile yields about 256*32*K/p0 <= 512 of them.
-        // The re-run of overflowed queries (fixed) appends straight to global
-        // memory, whose capacity per round it never exceeds.
-        if (mfma && d.fp4 && !fixed && p0 >= 16 * (int64_t)K) {
+        if (mfma && d.fp4 && mode != kFixed && p0 >= 16 * (int64_t)K) {
             pt.begin(PH_MFMA, s);
             launch_hamming_mfma(d.fp4, qs, wpad, Q, p0, p1, v, s);
             pt.end(PH_MFMA, s);
@@ -193,45 +217,54 @@ void sketch_topk(const SketchSet& d, const SketchSet* sample, int wpad, int kind
             launch_sketch_scan(a, v, s);
         }
     };
-    run_rounds(st, Q, K, d.N, fixed, scan, s, pt, launches, int_keys);
-    if (fixed) return;
+    run_rounds(st, Q, K, d.N, mode, scan, s, pt, launches, int_keys);
+    if (mode == kFixed) return;
     const std::vector<int32_t> ovf = flagged(st.view().ovf, Q, s);
-    if (ovf.empty()) return;
-    const int m = (int)ovf.size();
-    const int mpad = sketch_query_rows(m);
-    std::vector<int32_t> hidx(ovf);
-    hidx.resize((size_t)mpad, ovf[0]);
-    DevBuf<int32_t> idx((size_t)mpad);
-    LSH_HIP(hipMemcpyAsync(idx.get(), hidx.data(), sizeof(int32_t) * mpad, hipMemcpyHostToDevice, s));
-    DevBuf<uint32_t> sub((size_t)mpad * wpad);
-    launch_gather_rows_u32(qs, wpad, idx.get(), mpad, sub.get(), s);
-    TopKState sst;
-    sketch_topk(SketchSet{d.ds, nullptr, d.N}, nullptr, wpad, kind, sub.get(), m, K, true, sst, s, pt, launches);
-    const TopKView sv = sst.view(), v = st.view();
-    launch_scatter_rows_u64(sv.keys, sv.stride, K, idx.get(), m, v.keys, v.stride, s);
-    LSH_HIP(hipStreamSynchronize(s));
+    if (!ovf.empty()) redo(ovf, mode == kNormal ? kCautious : kFixed);
 }
 
-// Every stride-th row of a packed sketch array and its FP4 image, when the
-// data are large enough for the sampled threshold (at least 32*C points per
-// sample row; stride >= 4).
-int sample_stride(int64_t N, int C) {
-    const int64_t st = N / (32 * (int64_t)std::max(C, 1));
-    return st < 4 ? 0 : (int)std::min<int64_t>(16, st);
+}  // namespace
+
+// Sample ladder of a packed sketch array: level 0 is every stride-th row of
+// the data, level i+1 every stride-th row of level i, while a level keeps at
+// least 32*8 rows per candidate (stride >= 8).  One level is kept: at
+// 1.2M points a second one (4.7k rows, 28 candidates) cost more in launches
+// and re-runs than it saved.
+void lshgpu::build_sample_levels(const uint32_t* ds, int64_t N, int wpad, int C, std::vector<SampleLevel>& out,
+                                 hipStream_t s) {
+    out.clear();
+    const uint32_t* cur = ds;
+    int64_t n = N;
+    int K = C;
+    constexpr int kMaxLevels = 1;
+    while ((int)out.size() < kMaxLevels) {
+        const int64_t st = n / (32 * (int64_t)std::max(K, 1));
+        if (st < 8) break;
+        const int stride = (int)std::min<int64_t>(16, st);
+        SampleLevel L;
+        L.N = (n + stride - 1) / stride;
+        std::vector<int32_t> rows((size_t)L.N);
+        for (int64_t i = 0; i < L.N; ++i) rows[i] = (int32_t)(i * stride);
+        DevBuf<int32_t> idx((size_t)L.N);
+        LSH_HIP(hipMemcpyAsync(idx.get(), rows.data(), sizeof(int32_t) * rows.size(), hipMemcpyHostToDevice, s));
+        L.ds.alloc((size_t)L.N * wpad);
+        launch_gather_rows_u32(cur, wpad, idx.get(), (int)L.N, L.ds.get(), s);
+        L.fp4.alloc(hamming_fp4_words(L.N, wpad));
+        launch_expand_fp4(L.ds.get(), L.N, wpad, L.fp4.get(), s);
+        LSH_HIP(hipStreamSynchronize(s));
+        K = sample_k(K, sample_step(n, L.N));
+        cur = L.ds.get();
+        n = L.N;
+        out.push_back(std::move(L));
+    }
 }
 
-void build_sample(const uint32_t* ds, int64_t N, int wpad, int stride, DevBuf<uint32_t>& sds,
-                  DevBuf<uint32_t>& sfp4, int64_t& S, hipStream_t s) {
-    S = (N + stride - 1) / stride;
-    std::vector<int32_t> rows((size_t)S);
-    for (int64_t i = 0; i < S; ++i) rows[i] = (int32_t)(i * stride);
-    DevBuf<int32_t> idx((size_t)S);
-    LSH_HIP(hipMemcpyAsync(idx.get(), rows.data(), sizeof(int32_t) * rows.size(), hipMemcpyHostToDevice, s));
-    sds.alloc((size_t)S * wpad);
-    launch_gather_rows_u32(ds, wpad, idx.get(), (int)S, sds.get(), s);
-    sfp4.alloc(hamming_fp4_words(S, wpad));
-    launch_expand_fp4(sds.get(), S, wpad, sfp4.get(), s);
-    LSH_HIP(hipStreamSynchronize(s));
+namespace {
+
+std::vector<SketchSet> sample_views(const std::vector<SampleLevel>& lv) {
+    std::vector<SketchSet> v;
+    for (const SampleLevel& L : lv) v.push_back(SketchSet{L.ds.get(), L.fp4.get(), L.N});
+    return v;
 }
 
 // Hyperplanes of the reference stream, transposed to [words][d][hashBits].
@@ -344,42 +377,44 @@ std::vector<Transform> one_step(int kind) {
 // hmap: host copy of a.qmap (empty when a.qmap is null).  Queries whose
 // appends overflowed a round are re-run on their own with ranges no larger
 // than the append capacity.
-void lsh_topk(const LshScanArgs& base, const std::vector<int32_t>& hmap, int64_t maxS, int K, bool fixed,
-              bool int_keys, TopKState& st, hipStream_t s, PhaseTimer& pt, int64_t& launches) {
+void lsh_topk(const LshScanArgs& base, const std::vector<int32_t>& hmap, int64_t maxS, int K, int mode,
+              bool int_keys, TopKState& st, Scratch& sc, int depth, hipStream_t s, PhaseTimer& pt,
+              int64_t& launches) {
     auto scan = [&](int64_t p0, int64_t p1, const TopKView& v) {
         LshScanArgs a = base;
         a.p0 = p0;
         a.p1 = p1;
         launch_lsh_scan(a, v, s);
     };
-    run_rounds(st, base.Q, K, std::max<int64_t>(maxS, 1), fixed, scan, s, pt, launches, int_keys);
-    if (fixed) return;
+    run_rounds(st, base.Q, K, std::max<int64_t>(maxS, 1), mode, scan, s, pt, launches, int_keys);
+    if (mode == kFixed) return;
     const std::vector<int32_t> ovf = flagged(st.view().ovf, base.Q, s);
     if (ovf.empty()) return;
     const int m = (int)ovf.size();
     std::vector<int32_t> sub_map((size_t)m);
     for (int i = 0; i < m; ++i) sub_map[i] = hmap.empty() ? ovf[i] : hmap[ovf[i]];
-    DevBuf<int32_t> dmap((size_t)m), drows((size_t)m);
-    LSH_HIP(hipMemcpyAsync(dmap.get(), sub_map.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, s));
-    LSH_HIP(hipMemcpyAsync(drows.get(), ovf.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, s));
+    int32_t* dmap = sc.get<int32_t>(depth, 0, (size_t)m);
+    int32_t* drows = sc.get<int32_t>(depth, 1, (size_t)m);
+    LSH_HIP(hipMemcpyAsync(dmap, sub_map.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, s));
+    LSH_HIP(hipMemcpyAsync(drows, ovf.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, s));
     LshScanArgs a = base;
-    a.qmap = dmap.get();
+    a.qmap = dmap;
     a.Q = m;
-    TopKState sst;
-    lsh_topk(a, sub_map, maxS, K, true, int_keys, sst, s, pt, launches);
+    TopKState& sst = sc.tk[depth];
+    lsh_topk(a, sub_map, maxS, K, mode == kNormal ? kCautious : kFixed, int_keys, sst, sc, depth + 1, s, pt,
+             launches);
     const TopKView sv = sst.view(), v = st.view();
-    launch_scatter_rows_u64(sv.keys, sv.stride, K, drows.get(), m, v.keys, v.stride, s);
-    LSH_HIP(hipStreamSynchronize(s));
+    launch_scatter_rows_u64(sv.keys, sv.stride, K, drows, m, v.keys, v.stride, s);
 }
 
 }  // namespace
 
 size_t lsh_index::device_bytes() const {
-    size_t b = data.bytes() + data_inv.bytes() + dsk.bytes() + dfp4.bytes() + dsk_s.bytes() + dfp4_s.bytes() +
-               planesT.bytes() + djl.bytes() + wmh.bytes() +
-               kwmh.bytes() + kplanesT.bytes() + ksigns.bytes() + dkeys.bytes() + boff.bytes() + bperm.bytes() +
+    size_t b = data.bytes() + data_inv.bytes() + dsk.bytes() + dfp4.bytes() + planesT.bytes() + djl.bytes() + wmh.bytes() +
+               kwmh.bytes() + scratch.bytes() + kplanesT.bytes() + ksigns.bytes() + dkeys.bytes() + boff.bytes() + bperm.bytes() +
                qkeys.bytes() + qlo.bytes() + qpre.bytes() + qmax.bytes() + q.bytes() + qT.bytes() +
                q_inv.bytes() + qjl.bytes() + qjlT.bytes() + qsk.bytes() + out.bytes() + flags.bytes() + st.bytes();
+    for (const SampleLevel& L : samples) b += L.ds.bytes() + L.fp4.bytes();
     for (const Transform& t : key_steps) b += t.mins.bytes();
     for (const Transform& t : sketch_steps) b += t.mins.bytes();
     return b;
@@ -460,8 +495,7 @@ void lsh_index::build_fp4(hipStream_t s) {
     if (!hamming_mfma_supported(wpad) || N < 16 * 1024) return;
     dfp4.alloc(hamming_fp4_words(N, wpad));
     launch_expand_fp4(dsk.get(), N, wpad, dfp4.get(), s);
-    const int stride = sample_stride(N, candidates);
-    if (stride) build_sample(dsk.get(), N, wpad, stride, dsk_s, dfp4_s, S_sample, s);
+    build_sample_levels(dsk.get(), N, wpad, candidates, samples, s);
 }
 
 // lshFramework.cuh:312-409 runLsh up to the scan: keys (generateHashes with
@@ -597,8 +631,8 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
         LSH_REQUIRE(kq <= kMaxTopK, LSH_ERR_UNSUPPORTED, "exact scan supports k <= 4096");
         int K = std::min(kMaxTopK, pow2_ceil(kq + margin));
         const double eps = fp32_error_bound(kind, d);
-        exact_topk(data.get(), data_inv.get(), N, d, kind, ExactQueries{q.get(), Q}, K, false, st, s, pt,
-                   scan_launches);
+        exact_topk(data.get(), data_inv.get(), N, d, kind, ExactQueries{q.get(), Q}, K, kNormal, st, scratch, 0, s,
+                   pt, scan_launches);
         pt.begin(PH_FINAL, s);
         launch_exact_finalize(data.get(), q.get(), d, kind, st.view(), kq, eps, out.get(), flags.get(), s);
         pt.end(PH_FINAL, s);
@@ -614,8 +648,8 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
             DevBuf<float> sub((size_t)m * d);
             launch_gather_rows_f32(q.get(), d, idx.get(), m, sub.get(), s);
             TopKState sst;
-            exact_topk(data.get(), data_inv.get(), N, d, kind, ExactQueries{sub.get(), m}, kMaxTopK, false, sst,
-                       s, pt, scan_launches);
+            exact_topk(data.get(), data_inv.get(), N, d, kind, ExactQueries{sub.get(), m}, kMaxTopK, kNormal, sst,
+                       scratch, 0, s, pt, scan_launches);
             DevBuf<lsh_point_t> sout((size_t)m * kq);
             DevBuf<uint32_t> sflags((size_t)m);
             launch_exact_finalize(data.get(), sub.get(), d, kind, sst.view(), kq, eps, sout.get(), sflags.get(), s);
@@ -635,9 +669,10 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
         const int Ceff = (int)std::max<int64_t>(1, std::min<int64_t>(candidates, N));
         const int K = Ceff <= kMaxIntTopK ? Ceff : pow2_ceil(Ceff);
         LSH_REQUIRE(K <= kMaxTopK, LSH_ERR_UNSUPPORTED, "candidate sets larger than 4096 are not supported");
-        const SketchSet full{dsk.get(), dfp4.get(), N}, samp{dsk_s.get(), dfp4_s.get(), S_sample};
-        sketch_topk(full, S_sample > 0 ? &samp : nullptr, wpad, impl == 4 ? 1 : 0, qsk.get(), Q, K, false, st, s,
-                    pt, scan_launches,
+        const SketchSet full{dsk.get(), dfp4.get(), N};
+        const std::vector<SketchSet> sv = sample_views(samples);
+        sketch_topk(full, sv.data(), (int)sv.size(), wpad, impl == 4 ? 1 : 0, qsk.get(), Q, K, kNormal, st,
+                    scratch, 0, s, pt, scan_launches,
                     &mfma_launches, &mfma_points);
         const int kind = impl == 3 ? kAngular : kGenJaccard;
         pt.begin(PH_FINAL, s);
@@ -652,8 +687,8 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
         const int Ceff = (int)std::min<int64_t>(candidates, N);
         const int K = pow2_ceil(std::max(Ceff, 1));
         LSH_REQUIRE(K <= kMaxTopK, LSH_ERR_UNSUPPORTED, "candidate sets larger than 4096 are not supported");
-        exact_topk(djl.get(), nullptr, N, sdim, kSqL2, ExactQueries{qjl.get(), Q}, K, false, st, s, pt,
-                   scan_launches);
+        exact_topk(djl.get(), nullptr, N, sdim, kSqL2, ExactQueries{qjl.get(), Q}, K, kNormal, st, scratch, 0, s,
+                   pt, scan_launches);
         pt.begin(PH_FINAL, s);
         launch_rerank(data.get(), q.get(), d, kAngular, st.view().keys, st.view().stride, nullptr, Ceff, Q, kq,
                       kq + margin, fp32_error_bound(kAngular, d), out.get(), s);
@@ -729,7 +764,7 @@ void lsh_index::search_lsh(int Q, int kq, hipStream_t s, PhaseTimer& pt) {
         const int Ceff = (int)std::max<int64_t>(1, std::min<int64_t>(candidates, N));
         const int K = int_keys && Ceff <= kMaxIntTopK ? Ceff : pow2_ceil(Ceff);
         LSH_REQUIRE(K <= kMaxTopK, LSH_ERR_UNSUPPORTED, "candidate sets larger than 4096 are not supported");
-        lsh_topk(a, none, maxS, K, false, int_keys && K <= kMaxIntTopK, st, s, pt, scan_launches);
+        lsh_topk(a, none, maxS, K, kNormal, int_keys && K <= kMaxIntTopK, st, scratch, 0, s, pt, scan_launches);
         pt.begin(PH_FINAL, s);
         launch_rerank(data.get(), q.get(), d, kind, st.view().keys, st.view().stride, nullptr, Ceff, Q, kq,
                       kq + margin, eps, out.get(), s);
@@ -740,7 +775,7 @@ void lsh_index::search_lsh(int Q, int kq, hipStream_t s, PhaseTimer& pt) {
     a.kind = kind;
     LSH_REQUIRE(kq <= kMaxTopK, LSH_ERR_UNSUPPORTED, "k must be <= 4096");
     const int K = std::min(kMaxTopK, pow2_ceil(kq + margin));
-    lsh_topk(a, none, maxS, K, false, false, st, s, pt, scan_launches);
+    lsh_topk(a, none, maxS, K, kNormal, false, st, scratch, 0, s, pt, scan_launches);
     pt.begin(PH_FINAL, s);
     launch_exact_finalize(data.get(), q.get(), d, kind, st.view(), kq, eps, out.get(), flags.get(), s);
     pt.end(PH_FINAL, s);
@@ -757,7 +792,7 @@ void lsh_index::search_lsh(int Q, int kq, hipStream_t s, PhaseTimer& pt) {
     b.qmap = idx.get();
     b.Q = m;
     TopKState sst;
-    lsh_topk(b, bad, maxS, kMaxTopK, false, false, sst, s, pt, scan_launches);
+    lsh_topk(b, bad, maxS, kMaxTopK, kNormal, false, sst, scratch, 0, s, pt, scan_launches);
     DevBuf<lsh_point_t> sout((size_t)m * kq);
     DevBuf<uint32_t> sflags((size_t)m);
     launch_exact_finalize(data.get(), sub.get(), d, kind, sst.view(), kq, eps, sout.get(), sflags.get(), s);
@@ -852,16 +887,18 @@ void hamming_topc_host(const uint32_t* ds, int64_t N, const uint32_t* qs, int64_
     TopKState st;
     PhaseTimer pt(false);
     int64_t launches = 0;
-    DevBuf<uint32_t> fp4, sds, sfp4;
-    int64_t S = 0;
-    if (hamming_mfma_supported(wpad)) {
+    DevBuf<uint32_t> fp4;
+    std::vector<SampleLevel> lv;
+    if (hamming_mfma_supported(wpad) && N >= 16 * 1024) {
         fp4.alloc(hamming_fp4_words(N, wpad));
         launch_expand_fp4(dpad.get(), N, wpad, fp4.get(), nullptr);
-        const int stride = sample_stride(N, C);
-        if (stride) build_sample(dpad.get(), N, wpad, stride, sds, sfp4, S, nullptr);
+        build_sample_levels(dpad.get(), N, wpad, C, lv, nullptr);
     }
-    const SketchSet full{dpad.get(), fp4.get(), N}, samp{sds.get(), sfp4.get(), S};
-    sketch_topk(full, S > 0 ? &samp : nullptr, wpad, 0, qpad.get(), (int)Q, K, false, st, nullptr, pt, launches);
+    const SketchSet full{dpad.get(), fp4.get(), N};
+    const std::vector<SketchSet> sv = sample_views(lv);
+    Scratch sc;
+    sketch_topk(full, sv.data(), (int)sv.size(), wpad, 0, qpad.get(), (int)Q, K, kNormal, st, sc, 0, nullptr, pt,
+                launches);
     launch_sort_rows(st.view(), nullptr);
     LSH_HIP(hipDeviceSynchronize());
     const TopKView v = st.view();

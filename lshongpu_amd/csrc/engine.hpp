@@ -14,6 +14,16 @@ struct Transform {
     DevBuf<int> mins;  // kGjStep: per-dimension floor minimum (frozen at build)
 };
 
+// Every stride-th row of a packed sketch array (or of the previous level)
+// and its FP4 image.
+struct SampleLevel {
+    DevBuf<uint32_t> ds;
+    DevBuf<uint32_t> fp4;
+    int64_t N = 0;
+};
+void build_sample_levels(const uint32_t* ds, int64_t N, int wpad, int C, std::vector<SampleLevel>& out,
+                         hipStream_t s);
+
 // Weighted MinHash tables (weightedMinHash.cuh:41-90, 231-262) and the
 // sketch of rows with them.  mode 0: one red count per hash; mode 1: words of
 // hbits bits, bit b of word w from hash w*hbits + b.
@@ -37,6 +47,27 @@ struct WmhHasher {
     void sketch(const float* x, int64_t n, int d, uint32_t* out, int out_stride, hipStream_t s) const;
     size_t bytes() const {
         return owner.bytes() + start.bytes() + bitmap.bytes() + draws.bytes() + draw_owner.bytes();
+    }
+};
+
+// Device scratch reused across searches (grow-only).  A call at recursion
+// depth `depth` uses raw slots 4*depth .. 4*depth+3 and top-K state `depth`.
+struct Scratch {
+    static constexpr int kDepth = 8;
+    DevBuf<uint8_t> raw[4 * kDepth];
+    TopKState tk[kDepth];
+    template <class T>
+    T* get(int depth, int slot, size_t n) {
+        LSH_REQUIRE(depth < kDepth, LSH_ERR_INTERNAL, "scratch depth exceeded");
+        DevBuf<uint8_t>& b = raw[4 * depth + slot];
+        b.ensure(std::max<size_t>(n, 1) * sizeof(T));
+        return reinterpret_cast<T*>(b.get());
+    }
+    size_t bytes() const {
+        size_t n = 0;
+        for (const auto& b : raw) n += b.bytes();
+        for (const auto& t : tk) n += t.bytes();
+        return n;
     }
 };
 
@@ -71,9 +102,7 @@ struct lsh_index {
     int wpad = 0;    // words padded to a multiple of 8
     lshgpu::DevBuf<uint32_t> dsk;    // [N][wpad]
     lshgpu::DevBuf<uint32_t> dfp4;   // FP4 operand image of dsk for the matrix-core Hamming scan
-    lshgpu::DevBuf<uint32_t> dsk_s;  // every stride-th row of dsk (threshold sample)
-    lshgpu::DevBuf<uint32_t> dfp4_s; // its FP4 image
-    int64_t S_sample = 0;
+    std::vector<lshgpu::SampleLevel> samples;  // strided samples of dsk (threshold estimates)
     lshgpu::DevBuf<float> planesT;   // SimHash [words][d][32], JL [groups][d][32]
     int sdim = 0;                    // JL sketch length
     lshgpu::DevBuf<float> djl;       // [N][sdim]
@@ -99,6 +128,7 @@ struct lsh_index {
     lshgpu::DevBuf<lsh_point_t> out;
     lshgpu::DevBuf<uint32_t> flags;
     lshgpu::TopKState st;
+    lshgpu::Scratch scratch;
     int64_t scan_launches = 0;
     int64_t mfma_launches = 0;   // matrix-core Hamming scan launches of the last search
     int64_t mfma_points = 0;     // data points those launches covered

@@ -66,6 +66,7 @@ class DevBuf {
         return *this;
     }
     void alloc(size_t n) {
+        if (p_) (void)hipDeviceSynchronize();  // work queued on the old buffer has finished
         release();
         if (n == 0) return;
         void* p = nullptr;

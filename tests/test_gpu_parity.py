@@ -204,7 +204,7 @@ def test_hamming_topc_sampled_threshold_falls_back(lsh, orc):
     rng = np.random.default_rng(79)
     ds = rng.integers(0, 2 ** 32, size=(60000, 8), dtype=np.uint64).astype(np.uint32)
     qs = rng.integers(0, 2 ** 32, size=(40, 8), dtype=np.uint64).astype(np.uint32)
-    ds[np.arange(8) * 16] = qs[0] ^ np.array([1, 0, 0, 0, 0, 0, 0, 0], np.uint32)
+    ds[np.arange(30) * 16] = qs[0] ^ np.array([1, 0, 0, 0, 0, 0, 0, 0], np.uint32)
     copies = np.arange(600) * 16 + 7
     ds[copies] = qs[1]
     ids, dist = lsh.hamming_topc(ds, qs, 64)
@@ -212,6 +212,19 @@ def test_hamming_topc_sampled_threshold_falls_back(lsh, orc):
     np.testing.assert_array_equal(ids, oids)
     np.testing.assert_array_equal(dist, odist)
     assert (dist[1] == 0).all() and (ids[1] == copies[:64]).all()
+
+
+@pytest.mark.gpu
+def test_hamming_topc_two_sample_levels(lsh, orc):
+    # 200k points, C = 16: samples of 12.5k and 782 rows, each level's
+    # threshold coming from the one below.
+    rng = np.random.default_rng(80)
+    ds = rng.integers(0, 2 ** 32, size=(200000, 8), dtype=np.uint64).astype(np.uint32)
+    qs = ds[rng.integers(0, 200000, 60)] ^ (1 << rng.integers(0, 32, size=(60, 8))).astype(np.uint32)
+    ids, dist = lsh.hamming_topc(ds, qs, 16)
+    oids, odist = orc.sketch_topc(0, ds, qs, 16)
+    np.testing.assert_array_equal(ids, oids)
+    np.testing.assert_array_equal(dist, odist)
 
 
 # ------------------------------------------------------------ full pipelines
