@@ -3,34 +3,35 @@
 // A sketch bit b becomes the FP4 (e2m1) value s = +1 (b = 0) or -1 (b = 1);
 // for two sketches of B bits, sum_k s_a[k] s_b[k] = B - 2 * Hamming(a, b),
 // and the FP32 accumulation of these +-1 products is exact.  So the
-// Hamming distances of a 32-query x 32-point tile are one chain of
+// Hamming distances of a 32-point x 32-query tile are one chain of
 // v_mfma_scale_f32_32x32x64_f8f6f4 (FP4 x FP4, unit scales) over the
 // sketch, 64 bits per instruction.
 //
 // Operand layout: for K-step s (bits 64s .. 64s+63), lane l supplies row
 // l%32 and the 32 bits of word 2s + l/32 of that row, one nibble per bit,
-// in the low four registers of the operand.  A (queries) and B (points) use
-// the same mapping, so the dot product pairs equal bit positions.  The
-// accumulator register r of lane l holds row (r/4)*8 + (l/32)*4 + r%4 of A
-// and row l%32 of B.
+// in the low four registers of the operand.  Both operands use the same
+// mapping, so the dot product pairs equal bit positions.  With the points as
+// A and the queries as B, accumulator register r of lane l holds point row
+// (r/4)*8 + (l/32)*4 + r%4 and query l%32: a lane sees one query, whose
+// threshold it keeps in a register.
 //
-// The data sketches are kept in HBM already expanded to FP4 and laid out in
-// operand order, [tile of 32 points][K-step][lane][16 bytes], so a wave loads
-// the B operand of one K-step with one coalesced 1 KiB read and no
-// arithmetic (4x the bytes of the packed sketch; 512 B per point at 1024
-// bits).  Each wave owns 64 queries (two A tiles resident in VGPRs for the
-// whole launch) and walks the workgroup's slice of point tiles two tiles at
-// a time, independently of the other waves: no LDS staging and no
-// workgroup barrier.  Workgroups are numbered so that the query blocks of
-// one slice of points run on the same XCD (blocks are dealt to the 8 XCDs
+// The data sketches are kept in HBM already expanded to FP4 in operand
+// order, [tile of 32 points][K-step][lane][16 bytes] (512 B per point at
+// 1024 bits, 4x the packed sketch), so staging a tile is a copy: each wave
+// moves a quarter of it with direct global-to-LDS loads (16 bytes a lane).
+// A workgroup holds 4 waves of 32 queries each (their B operands resident
+// in VGPRs) and walks a slice of point tiles through two LDS buffers: the
+// loads of the next tile are in flight while the current one is multiplied,
+// with one barrier per tile.  Workgroups are numbered so that the query
+// blocks of one slice run on the same XCD (blocks are dealt to the 8 XCDs
 // round-robin), which keeps one copy of the slice in that XCD's L2.
 //
-// Epilogue: a pair may pass when its dot product is >= B - 2*h_thr (one
-// float compare against the query's threshold distance); for the registers
-// some lane flags, the exact (distance, ID) key is checked against the
-// 64-bit threshold and appended to the wave's LDS pool, which is flushed to
-// the global top-K state with one atomic per query.  Ranges whose
-// candidates are still dense (the first rounds) use the VALU scan.
+// Epilogue: the 16 accumulators of a lane are compared with its query's
+// threshold (dot > B - 2*h_thr, i.e. Hamming < h_thr); the lane masks stay in
+// scalar registers and only the registers with hits append (Hamming, ID) keys
+// to the wave's LDS queue, which is flushed to the global top-K state with
+// one atomic per query (or, when full, spills to the global slots).  Ranges
+// whose candidates are still dense (the first rounds) use the VALU scan.
 #include "device_common.hpp"
 
 namespace lshgpu {
@@ -62,13 +63,14 @@ __device__ inline v4i expand_word(uint32_t w) {
 // FP4 operands occupy the low four registers; the upper half is not read.
 __device__ inline v8i widen(v4i a) { return __builtin_shufflevector(a, a, 0, 1, 2, 3, -1, -1, -1, -1); }
 
-constexpr int MQ = 2;                      // query tiles per wave
+constexpr int MQ = 1;                      // query tiles per wave
 constexpr int WAVES = 4;
-constexpr int WAVE_QUERIES = MQ * 32;      // 64: one query per lane in the flush
+constexpr int WAVE_QUERIES = MQ * 32;      // at most 64: one query per lane in the flush
 constexpr int WG_QUERIES = WAVES * WAVE_QUERIES;
 constexpr int TPB = WAVES * 64;
 constexpr int POOLW = 256;                 // queued candidates per wave
-constexpr int TPS = 2;                     // point tiles per staging step
+constexpr int TPS = 1;                     // point tiles per staging step
+constexpr int OCC = 3;                     // workgroups per CU (registers and LDS)
 
 __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -78,7 +80,7 @@ __device__ inline void wave_sync() {
 
 // KS = sketch words / 2 (K-steps of 64 bits).
 template <int KS>
-__global__ __launch_bounds__(TPB, 2) void hamming_mfma_kernel(const v4i* __restrict__ frag,
+__global__ __launch_bounds__(TPB, OCC) void hamming_mfma_kernel(const v4i* __restrict__ frag,
                                                               const uint32_t* __restrict__ qs, int Q,
                                                               int64_t p0, int64_t p1, int64_t tile_begin,
                                                               int64_t tile_end, int tiles_per_wg, int qblocks,
@@ -105,14 +107,14 @@ __global__ __launch_bounds__(TPB, 2) void hamming_mfma_kernel(const v4i* __restr
     const int wave = threadIdx.x >> 6;
     const int qwave = qb * WG_QUERIES + wave * WAVE_QUERIES;
 
-    {
+    if (lane < WAVE_QUERIES) {
         const int q = qwave + lane;
         float f = __builtin_inff();  // rows past Q never pass
         if (q < Q) {
             const uint32_t h = (uint32_t)(v.thr[q] >> 32);
             f = h == 0xFFFFFFFFu ? -__builtin_inff() : (float)BITS - 2.0f * (float)h;
         }
-        tf[threadIdx.x] = f;
+        tf[wave * WAVE_QUERIES + lane] = f;
         lcnt[wave][lane] = 0;
     }
     // Query fragments (B operands), resident for the whole launch.
@@ -144,8 +146,10 @@ __global__ __launch_bounds__(TPB, 2) void hamming_mfma_kernel(const v4i* __restr
             rank[k] = e < n ? atomicAdd(&lcnt[wave][pq[wave][e]], 1u) : 0u;
         }
         wave_sync();
-        const uint32_t c = lcnt[wave][lane];
-        if (c) lbase[wave][lane] = atomicAdd(&v.cnt[qwave + lane], c);
+        if (lane < WAVE_QUERIES) {
+            const uint32_t c = lcnt[wave][lane];
+            if (c) lbase[wave][lane] = atomicAdd(&v.cnt[qwave + lane], c);
+        }
         wave_sync();
 #pragma unroll
         for (int k = 0; k < POOLW / 64; ++k) {
@@ -156,7 +160,7 @@ __global__ __launch_bounds__(TPB, 2) void hamming_mfma_kernel(const v4i* __restr
                 if (slot < (uint32_t)v.newcap) v.keys[(size_t)(qwave + q) * v.stride + v.K + slot] = pkey[wave][e];
             }
         }
-        lcnt[wave][lane] = 0;
+        if (lane < WAVE_QUERIES) lcnt[wave][lane] = 0;
         queued = 0;
         wave_sync();
     };
@@ -288,13 +292,13 @@ __global__ __launch_bounds__(TPB, 2) void hamming_mfma_kernel(const v4i* __restr
 }
 
 int wg_slots() {
-    // Two workgroups of this kernel fit a CU (LDS and registers).
+    // OCC workgroups of this kernel fit a CU (LDS and registers).
     static const int slots = [] {
         int dev = 0, cus = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
             cus = 256;
-        return 2 * cus;
+        return OCC * cus;
     }();
     return slots;
 }
