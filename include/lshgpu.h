@@ -55,7 +55,8 @@ enum {
     LSH_ERR_UNSUPPORTED = 2, /* valid combination the engine does not provide */
     LSH_ERR_DEVICE = 3,      /* HIP runtime error */
     LSH_ERR_NOMEM = 4,       /* host or device allocation failed / over budget */
-    LSH_ERR_INTERNAL = 5
+    LSH_ERR_INTERNAL = 5,
+    LSH_ERR_IO = 6           /* file cannot be opened, read or written */
 };
 
 /* Optional knobs of the _ex entry points.  lsh_options_default() fills the
@@ -195,6 +196,65 @@ int lsh_wmh_sketch(const float* x, int64_t n, int d, int sketchDim, int oneBit,
  * reference stream (lsh_random_signs over tables*3*P values). */
 int lsh_crosspoly_keys(const float* x, int64_t n, int d, int tables,
                        const int32_t* signs, uint16_t* keys);
+
+/* ------------------------------------- datasets, truth files and reports
+ * The runner's I/O (kernel.cu:119-196): host code, no device needed.  Arrays
+ * returned through pointer arguments are allocated with malloc; release them
+ * with lsh_free. */
+
+void lsh_free(void* p);
+
+/* gloveparser.cu:9-86 parseFile, the format DataParsing/Writer/writer.cpp:
+ * 10-29 writes: "N\nd\n", then one row per line "ID v1 ... vd".  Values are
+ * read as C floats (decimal or exponent form; the reference's digit loop
+ * keeps only the last digit of the integer part and drops a row's last value
+ * when no space follows it).  *data is [N][d]; *ids (may be NULL) receives
+ * the row IDs, which the search does not use. */
+int lsh_read_text_dataset(const char* path, float** data, int64_t* n, int* d,
+                          int32_t** ids);
+
+/* DataParsing/Parsers/siftParser.cpp:8-28: .fvecs records (int32 d, then d
+ * float32), all of one dimension. */
+int lsh_read_fvecs(const char* path, float** data, int64_t* n, int* d);
+
+/* DataParsing/Writer/writer.cpp:10-29: N, d, then "i v1 ... vd " per row
+ * (IDs 0..N-1), values with 9 significant digits so they read back exactly. */
+int lsh_write_text_dataset(const char* path, const float* data, int64_t n,
+                           int d);
+
+/* Truth files, GoldStandard/goldstandard.cpp:123-134 (writer) and
+ * LinearScan/validation.cpp:62-107 (reader): "n\nk\n", then per query
+ * "qid:" and k lines "ID distance".  lsh_write_truth writes the first k of
+ * each result row (rows k_stride apart); query_ids may be NULL (0..n-1). */
+int lsh_read_truth(const char* path, int* n_query, int* k, int32_t** ids,
+                   float** distances);
+int lsh_write_truth(const char* path, const lsh_point_t* results,
+                    const int32_t* query_ids, int n_query, int k_stride,
+                    int k);
+
+/* validation.cpp:129-208 runValidation(truth, container, results, N_query,
+ * k, reportK): recall = mean over the truth file's queries of |first reportK
+ * truth IDs found among the first reportK result IDs| / reportK; avgDistance
+ * = mean of (sum of the distances of the distinct result IDs, in ID order) /
+ * (sum of the first reportK truth distances), in float as the reference
+ * sums them.  results is [N_query][k]. */
+int lsh_validate(const char* truth_path, const lsh_point_t* results,
+                 int N_query, int k, int reportK, float* recall,
+                 float* avgDistance);
+
+/* resultWriter.cpp:7-20 writeResult: "N_query\nreportK\n", then "i:" and
+ * reportK lines "ID distance" (printf %d %f).  path NULL writes to stdout. */
+int lsh_write_results(const char* path, const lsh_point_t* results, int k,
+                      int N_query, int reportK);
+
+/* resultWriter.cpp:37-60 writePerformanceResults: appends one line of 13
+ * comma-separated columns: implementation, keysImplementation, sketchDim, k,
+ * 128 (THREAD_QUEUE_SIZE), bucketKeyBits, tables, 0 (WITH_TQ_OR_BUFFER),
+ * preprocessTime, constructionTime, scanTime, recall, avgDistance. */
+int lsh_write_performance(const char* path, const lsh_result_t* r,
+                          int implementation, int keysImplementation,
+                          int sketchDim, int k, int bucketKeyBits,
+                          int tables);
 
 /* ---------------------------------------------------------- resident index
  * No counterpart in the reference.  The index keeps the data, its sketches

@@ -28,6 +28,8 @@ __all__ = [
     "LshError", "Result", "Index", "POINT_DTYPE", "load", "calculateK", "linearScans", "LSH",
     "linear_scans", "lsh_framework", "random_normals", "random_signs", "simhash_sketch",
     "jl_sketch", "gj_transform", "hamming_topc", "wmh_sketch", "crosspoly_keys", "library_path",
+    "read_text_dataset", "read_fvecs", "write_text_dataset", "read_truth", "write_truth", "validate",
+    "write_results", "write_performance",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -117,6 +119,19 @@ _SIGS = {
     "lsh_wmh_sketch": (C.c_int, [_FP, C.c_int64, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint32)]),
     "lsh_crosspoly_keys": (C.c_int, [_FP, C.c_int64, C.c_int, C.c_int, C.POINTER(C.c_int32),
                                      C.POINTER(C.c_uint16)]),
+    "lsh_free": (None, [C.c_void_p]),
+    "lsh_read_text_dataset": (C.c_int, [C.c_char_p, C.POINTER(_FP), C.POINTER(C.c_int64), C.POINTER(C.c_int),
+                                        C.POINTER(C.POINTER(C.c_int32))]),
+    "lsh_read_fvecs": (C.c_int, [C.c_char_p, C.POINTER(_FP), C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
+    "lsh_write_text_dataset": (C.c_int, [C.c_char_p, _FP, C.c_int64, C.c_int]),
+    "lsh_read_truth": (C.c_int, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                 C.POINTER(C.POINTER(C.c_int32)), C.POINTER(_FP)]),
+    "lsh_write_truth": (C.c_int, [C.c_char_p, C.c_void_p, C.POINTER(C.c_int32), C.c_int, C.c_int, C.c_int]),
+    "lsh_validate": (C.c_int, [C.c_char_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float),
+                               C.POINTER(C.c_float)]),
+    "lsh_write_results": (C.c_int, [C.c_char_p, C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "lsh_write_performance": (C.c_int, [C.c_char_p, C.POINTER(_Result), C.c_int, C.c_int, C.c_int, C.c_int,
+                                        C.c_int, C.c_int]),
     "lsh_index_config_default": (None, [C.POINTER(_IndexConfig)]),
     "lsh_index_create": (C.c_int, [C.POINTER(_IndexConfig), C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
     "lsh_index_search": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int,
@@ -438,3 +453,113 @@ class Index:
 
     def __exit__(self, *exc):
         self.close()
+
+
+# ------------------------------------------------------------------ runner I/O
+# Datasets, truth files, validation and report formats of the reference's
+# runner (include/lshgpu.h documents the formats and their reference lines).
+
+def _path(p) -> bytes:
+    return os.fsencode(p)
+
+
+def _points(results) -> np.ndarray:
+    """(N_query, k) POINT_DTYPE array, or a Result."""
+    if isinstance(results, Result):
+        results = results.results
+    r = np.ascontiguousarray(results)
+    if r.dtype != POINT_DTYPE or r.ndim != 2:
+        raise LshError(1, "results must be a (N_query, k) array of POINT_DTYPE")
+    return r
+
+
+def read_text_dataset(path):
+    """gloveparser.cu:9-86 format; returns (data float32 [N][d], ids int32 [N])."""
+    lib = load()
+    data, ids = _FP(), C.POINTER(C.c_int32)()
+    n, d = C.c_int64(), C.c_int()
+    _check(lib.lsh_read_text_dataset(_path(path), C.byref(data), C.byref(n), C.byref(d), C.byref(ids)))
+    try:
+        x = np.ctypeslib.as_array(data, shape=(max(n.value, 1) * d.value,))[: n.value * d.value].copy()
+        i = np.ctypeslib.as_array(ids, shape=(max(n.value, 1),))[: n.value].copy()
+    finally:
+        lib.lsh_free(C.cast(data, C.c_void_p))
+        lib.lsh_free(C.cast(ids, C.c_void_p))
+    return x.reshape(n.value, d.value), i
+
+
+def read_fvecs(path) -> np.ndarray:
+    """siftParser.cpp:8-28 .fvecs records; returns float32 [N][d]."""
+    lib = load()
+    data = _FP()
+    n, d = C.c_int64(), C.c_int()
+    _check(lib.lsh_read_fvecs(_path(path), C.byref(data), C.byref(n), C.byref(d)))
+    try:
+        x = np.ctypeslib.as_array(data, shape=(n.value * d.value,)).copy()
+    finally:
+        lib.lsh_free(C.cast(data, C.c_void_p))
+    return x.reshape(n.value, d.value)
+
+
+def write_text_dataset(path, data) -> None:
+    """writer.cpp:10-29 format (IDs 0..N-1)."""
+    x = _f32(data)
+    if x.ndim != 2:
+        raise LshError(1, "data must be 2-D")
+    _check(load().lsh_write_text_dataset(_path(path), _fp(x), x.shape[0], x.shape[1]))
+
+
+def read_truth(path):
+    """goldstandard.cpp:123-134 truth file; returns (ids int32 [n][k], distances float32 [n][k])."""
+    lib = load()
+    ids, dist = C.POINTER(C.c_int32)(), _FP()
+    n, k = C.c_int(), C.c_int()
+    _check(lib.lsh_read_truth(_path(path), C.byref(n), C.byref(k), C.byref(ids), C.byref(dist)))
+    m = n.value * k.value
+    try:
+        i = np.ctypeslib.as_array(ids, shape=(max(m, 1),))[:m].copy()
+        dd = np.ctypeslib.as_array(dist, shape=(max(m, 1),))[:m].copy()
+    finally:
+        lib.lsh_free(C.cast(ids, C.c_void_p))
+        lib.lsh_free(C.cast(dist, C.c_void_p))
+    return i.reshape(n.value, k.value), dd.reshape(n.value, k.value)
+
+
+def write_truth(path, results, k: int, query_ids=None) -> None:
+    """Writes the first k entries of every result row as a truth file."""
+    r = _points(results)
+    qid = None
+    if query_ids is not None:
+        qid = np.ascontiguousarray(query_ids, dtype=np.int32)
+    _check(load().lsh_write_truth(_path(path), r.ctypes.data_as(C.c_void_p),
+                                  qid.ctypes.data_as(C.POINTER(C.c_int32)) if qid is not None else None,
+                                  r.shape[0], r.shape[1], int(k)))
+
+
+def validate(truth_path, results, reportK: int):
+    """validation.cpp:129-208 runValidation; returns (recall, avgDistance)."""
+    r = _points(results)
+    rec, avg = C.c_float(), C.c_float()
+    _check(load().lsh_validate(_path(truth_path), r.ctypes.data_as(C.c_void_p), r.shape[0], r.shape[1],
+                               int(reportK), C.byref(rec), C.byref(avg)))
+    return float(rec.value), float(avg.value)
+
+
+def write_results(path, results, reportK: int) -> None:
+    """resultWriter.cpp:7-20 writeResult (path None: stdout)."""
+    r = _points(results)
+    _check(load().lsh_write_results(_path(path) if path is not None else None, r.ctypes.data_as(C.c_void_p),
+                                    r.shape[1], r.shape[0], int(reportK)))
+
+
+def write_performance(path, result: Result, implementation: int, keysImplementation: int, sketchDim: int, k: int,
+                      bucketKeyBits: int, tables: int) -> None:
+    """resultWriter.cpp:37-60 writePerformanceResults: appends a 13-column CSV line."""
+    r = _Result()
+    r.preprocessTime = int(result.preprocessTime)
+    r.constructionTime = int(result.constructionTime)
+    r.scanTime = int(result.scanTime)
+    r.recall = float(result.recall)
+    r.avgDistance = float(result.avgDistance)
+    _check(load().lsh_write_performance(_path(path), C.byref(r), int(implementation), int(keysImplementation),
+                                        int(sketchDim), int(k), int(bucketKeyBits), int(tables)))

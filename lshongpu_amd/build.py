@@ -30,7 +30,12 @@ SOURCES = [
     "minhash.hip",
     "lsh.hip",
     "util.hip",
+    "io.cpp",
 ]
+
+# Command-line runners linked against the library: the reference's knn main
+# (kernel.cu:119-196) and a GPU truth-file generator (goldstandard.cpp).
+PROGRAMS = [("knn_main.cpp", "knn"), ("gold_main.cpp", "goldstandard")]
 
 ARCH = os.environ.get("LSHGPU_ARCH", "gfx950")
 
@@ -80,6 +85,12 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    for main, exe in PROGRAMS:
+        cmd = [hipcc(), "-O2", "-std=c++17", "-o", os.path.join(HERE, exe), os.path.join(CSRC, main),
+               f"-L{HERE}", "-llshgpu", "-Wl,-rpath,$ORIGIN"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"building {exe} failed:\n{r.stderr}")
     if verbose:
         print(f"built {LIB}")
     return LIB

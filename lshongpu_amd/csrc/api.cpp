@@ -10,25 +10,7 @@
 
 namespace {
 
-thread_local std::string g_err;
-
-template <class F>
-int guarded(F&& f) {
-    try {
-        f();
-        g_err.clear();
-        return LSH_OK;
-    } catch (const lshgpu::Error& e) {
-        g_err = e.what();
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        g_err = "host allocation failed";
-        return LSH_ERR_NOMEM;
-    } catch (const std::exception& e) {
-        g_err = e.what();
-        return LSH_ERR_INTERNAL;
-    }
-}
+using lshgpu::guarded;
 
 double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
@@ -148,7 +130,7 @@ void lsh_index_config_default(lsh_index_config_t* cfg) {
     cfg->rescan_margin = 32;
 }
 
-const char* lsh_last_error(void) { return g_err.c_str(); }
+const char* lsh_last_error(void) { return lshgpu::last_error().c_str(); }
 
 const char* lsh_version(void) { return "lshgpu 0.1.0 gfx950"; }
 

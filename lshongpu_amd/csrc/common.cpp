@@ -6,6 +6,11 @@
 
 namespace lshgpu {
 
+std::string& last_error() {
+    static thread_local std::string msg;
+    return msg;
+}
+
 void fail(int code, const std::string& msg) { throw Error(code, msg); }
 
 void check_launch(const char* what) {

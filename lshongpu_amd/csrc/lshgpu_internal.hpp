@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <new>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -21,6 +22,29 @@ struct Error : std::runtime_error {
 };
 
 [[noreturn]] void fail(int code, const std::string& msg);
+
+// Message of the last failed entry point on this thread.
+std::string& last_error();
+
+// Runs an entry point body: exceptions become status codes and the message
+// is kept for lsh_last_error().
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        last_error().clear();
+        return LSH_OK;
+    } catch (const Error& e) {
+        last_error() = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        last_error() = "host allocation failed";
+        return LSH_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        last_error() = e.what();
+        return LSH_ERR_INTERNAL;
+    }
+}
 
 #define LSH_HIP(expr)                                                              \
     do {                                                                           \
