@@ -197,6 +197,22 @@ int lsh_wmh_sketch(const float* x, int64_t n, int d, int sketchDim, int oneBit,
 int lsh_crosspoly_keys(const float* x, int64_t n, int d, int tables,
                        const int32_t* signs, uint16_t* keys);
 
+/* statistics.cuh:10-16 bucketDistribution, as weightedMinHash.cuh:215-229
+ * runBucketStatistics runs it over byte sketches: counts256[v] = the number
+ * of the n bytes of `hashes` equal to v, for every v in 0..255 (the
+ * reference's 255-entry array has no slot for 255). */
+int lsh_bucket_distribution(const uint8_t* hashes, int64_t n, int64_t* counts256);
+
+/* statisticsCpu.cpp:11-28 bucketDistributionFullKey: the distinct keys among
+ * n keys of sketchDim bytes each (`hashes` is [n][sketchDim]) with the number
+ * of keys equal to each, in byte-lexicographic key order (the order of the
+ * reference's std::map over the keys' bit strings).  *keys
+ * ([n_distinct][sketchDim]) and *counts ([n_distinct]) are allocated with
+ * malloc; release them with lsh_free.  n < 2^31. */
+int lsh_bucket_distribution_full_key(const uint8_t* hashes, int64_t n, int sketchDim,
+                                     uint8_t** keys, int64_t** counts,
+                                     int64_t* n_distinct);
+
 /* ------------------------------------- datasets, truth files and reports
  * The runner's I/O (kernel.cu:119-196): host code, no device needed.  Arrays
  * returned through pointer arguments are allocated with malloc; release them
@@ -302,6 +318,22 @@ int lsh_index_search(lsh_index_t* index, const float* queries, int N_query,
  * is on. */
 int lsh_index_set_profiling(lsh_index_t* index, int on);
 int lsh_index_phase_times(const lsh_index_t* index, double* out8);
+
+/* Bucket occupancy of table `table` (0 <= table < tables) of an index built
+ * with framework 1: the figures that guide the choice of bucketKeyBits and
+ * tables (statistics.cuh:10-16 counts hash values; weightedMinHash.cuh:
+ * 215-229 prints them).  hist (may be NULL with hist_len 0) receives, for
+ * s < hist_len - 1, the number of buckets holding exactly s points, and in
+ * hist[hist_len - 1] the number holding hist_len - 1 or more. */
+typedef struct lsh_bucket_stats {
+    int64_t buckets;   /* 2^bucketKeyBits */
+    int64_t nonempty;  /* buckets holding at least one point */
+    int64_t points;    /* entries of the table (N_data) */
+    int64_t max_size;  /* points in the largest bucket */
+} lsh_bucket_stats_t;
+
+int lsh_index_bucket_stats(lsh_index_t* index, int table, lsh_bucket_stats_t* out,
+                           int64_t* hist, int hist_len);
 
 /* Device bytes held by the index. */
 int64_t lsh_index_device_bytes(const lsh_index_t* index);

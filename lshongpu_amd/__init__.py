@@ -94,6 +94,10 @@ class _IndexConfig(C.Structure):
 
 
 _FP = C.POINTER(C.c_float)
+class _BucketStats(C.Structure):
+    _fields_ = [("buckets", C.c_int64), ("nonempty", C.c_int64), ("points", C.c_int64), ("max_size", C.c_int64)]
+
+
 _SIGS = {
     "lsh_calculate_k": (C.c_int, [C.c_int]),
     "lsh_options_default": (None, [C.POINTER(_Options)]),
@@ -119,6 +123,9 @@ _SIGS = {
     "lsh_wmh_sketch": (C.c_int, [_FP, C.c_int64, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint32)]),
     "lsh_crosspoly_keys": (C.c_int, [_FP, C.c_int64, C.c_int, C.c_int, C.POINTER(C.c_int32),
                                      C.POINTER(C.c_uint16)]),
+    "lsh_bucket_distribution": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
+    "lsh_bucket_distribution_full_key": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.POINTER(C.c_void_p),
+                                                   C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]),
     "lsh_free": (None, [C.c_void_p]),
     "lsh_read_text_dataset": (C.c_int, [C.c_char_p, C.POINTER(_FP), C.POINTER(C.c_int64), C.POINTER(C.c_int),
                                         C.POINTER(C.POINTER(C.c_int32))]),
@@ -138,6 +145,8 @@ _SIGS = {
                                    C.c_void_p]),
     "lsh_index_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "lsh_index_phase_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "lsh_index_bucket_stats": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(_BucketStats), C.POINTER(C.c_int64),
+                                         C.c_int]),
     "lsh_index_device_bytes": (C.c_int64, [C.c_void_p]),
     "lsh_index_destroy": (C.c_int, [C.c_void_p]),
 }
@@ -353,6 +362,43 @@ def crosspoly_keys(x, tables: int, signs=None) -> np.ndarray:
     return keys
 
 
+def bucket_distribution(hashes) -> np.ndarray:
+    """statistics.cuh:10-16 bucketDistribution: counts of each byte value 0..255."""
+    h = np.ascontiguousarray(hashes, dtype=np.uint8).reshape(-1)
+    counts = np.zeros(256, dtype=np.int64)
+    _check(load().lsh_bucket_distribution(C.c_void_p(h.ctypes.data), h.size,
+                                          counts.ctypes.data_as(C.POINTER(C.c_int64))))
+    return counts
+
+
+def bucket_distribution_full_key(hashes, sketchDim: int | None = None):
+    """statisticsCpu.cpp:11-28 bucketDistributionFullKey on (n, sketchDim) uint8
+    keys: (distinct keys in byte-lexicographic order, their counts)."""
+    h = np.ascontiguousarray(hashes, dtype=np.uint8)
+    if sketchDim is None:
+        if h.ndim != 2:
+            raise LshError(1, "hashes must be (n, sketchDim) or sketchDim given")
+        sketchDim = h.shape[1]
+    sd = int(sketchDim)
+    if sd < 1 or h.size % sd:
+        raise LshError(1, "hashes do not hold whole keys of sketchDim bytes")
+    n = h.size // sd
+    lib = load()
+    kp, cp, nd = C.c_void_p(), C.c_void_p(), C.c_int64()
+    _check(lib.lsh_bucket_distribution_full_key(C.c_void_p(h.ctypes.data), n, sd, C.byref(kp), C.byref(cp),
+                                                C.byref(nd)))
+    try:
+        m = int(nd.value)
+        keys = np.ctypeslib.as_array(C.cast(kp, C.POINTER(C.c_uint8)), shape=(m * sd,)).copy().reshape(m, sd) \
+            if m else np.zeros((0, sd), np.uint8)
+        counts = np.ctypeslib.as_array(C.cast(cp, C.POINTER(C.c_int64)), shape=(m,)).copy() \
+            if m else np.zeros(0, np.int64)
+    finally:
+        lib.lsh_free(kp)
+        lib.lsh_free(cp)
+    return keys, counts
+
+
 # ------------------------------------------------------------------ resident index
 
 def _ptr_of(a):
@@ -433,6 +479,15 @@ class Index:
         v = list(arr)
         return {"prep_ms": v[0], "scan_ms": v[1], "select_ms": v[2], "final_ms": v[3], "total_ms": v[4],
                 "mfma_scan_ms": v[5], "mfma_launches": int(v[6]), "mfma_points": int(v[7])}
+
+    def bucket_stats(self, table: int, hist_len: int = 0):
+        """Occupancy of one (K,L) table: ({buckets, nonempty, points, max_size},
+        histogram of bucket sizes whose last bin collects the larger ones)."""
+        st = _BucketStats()
+        hist = np.zeros(max(int(hist_len), 0), dtype=np.int64)
+        hp = hist.ctypes.data_as(C.POINTER(C.c_int64)) if hist.size else None
+        _check(load().lsh_index_bucket_stats(self._h, int(table), C.byref(st), hp, int(hist.size)))
+        return {f: int(getattr(st, f)) for f, _ in _BucketStats._fields_}, hist
 
     def device_bytes(self) -> int:
         return int(load().lsh_index_device_bytes(self._h))

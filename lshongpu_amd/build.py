@@ -29,6 +29,7 @@ SOURCES = [
     "sketch.hip",
     "minhash.hip",
     "lsh.hip",
+    "stats.hip",
     "util.hip",
     "io.cpp",
 ]

@@ -269,6 +269,43 @@ int lsh_crosspoly_keys(const float* x, int64_t n, int d, int tables, const int32
     });
 }
 
+int lsh_bucket_distribution(const uint8_t* hashes, int64_t n, int64_t* counts256) {
+    return guarded([&] {
+        LSH_REQUIRE(counts256 && (hashes || n == 0) && n >= 0, LSH_ERR_INVALID, "null pointer or negative size");
+        for (int i = 0; i < 256; ++i) counts256[i] = 0;
+        if (n == 0) return;
+        lshgpu::byte_distribution_host(hashes, n, counts256, 0);
+    });
+}
+
+int lsh_bucket_distribution_full_key(const uint8_t* hashes, int64_t n, int sketchDim, uint8_t** keys,
+                                     int64_t** counts, int64_t* n_distinct) {
+    return guarded([&] {
+        LSH_REQUIRE(keys && counts && n_distinct && (hashes || n == 0), LSH_ERR_INVALID, "null pointer");
+        *keys = nullptr;
+        *counts = nullptr;
+        *n_distinct = 0;
+        LSH_REQUIRE(sketchDim >= 1, LSH_ERR_INVALID, "sketchDim must be >= 1");
+        LSH_REQUIRE(n >= 0 && n < (int64_t(1) << 31), LSH_ERR_INVALID, "n must be in [0, 2^31)");
+        if (n == 0) return;
+        std::vector<uint8_t> k;
+        std::vector<int64_t> c;
+        lshgpu::full_key_distribution_host(hashes, n, sketchDim, k, c, 0);
+        auto* kp = static_cast<uint8_t*>(std::malloc(std::max<size_t>(k.size(), 1)));
+        auto* cp = static_cast<int64_t*>(std::malloc(std::max<size_t>(c.size(), 1) * sizeof(int64_t)));
+        if (!kp || !cp) {
+            std::free(kp);
+            std::free(cp);
+            throw std::bad_alloc();
+        }
+        std::copy(k.begin(), k.end(), kp);
+        std::copy(c.begin(), c.end(), cp);
+        *keys = kp;
+        *counts = cp;
+        *n_distinct = (int64_t)c.size();
+    });
+}
+
 int lsh_index_create(const lsh_index_config_t* cfg, const float* data, int data_on_device, lsh_index_t** out) {
     return guarded([&] {
         LSH_REQUIRE(cfg && out, LSH_ERR_INVALID, "null pointer");
@@ -310,6 +347,18 @@ int lsh_index_phase_times(const lsh_index_t* index, double* out8) {
     return guarded([&] {
         LSH_REQUIRE(index && out8, LSH_ERR_INVALID, "null pointer");
         for (int i = 0; i < 8; ++i) out8[i] = index->phase_ms[i];
+    });
+}
+
+int lsh_index_bucket_stats(lsh_index_t* index, int table, lsh_bucket_stats_t* out, int64_t* hist, int hist_len) {
+    return guarded([&] {
+        LSH_REQUIRE(index && out, LSH_ERR_INVALID, "null pointer");
+        int64_t s4[4];
+        lshgpu::index_bucket_stats(index, table, s4, hist, hist_len);
+        out->buckets = s4[0];
+        out->nonempty = s4[1];
+        out->points = s4[2];
+        out->max_size = s4[3];
     });
 }
 

@@ -161,4 +161,10 @@ void wmh_host(const float* x, int64_t n, int d, int sketchDim, int one_bit, uint
 void crosspoly_host(const float* x, int64_t n, int d, int tables, const int32_t* signs, uint16_t* keys,
                     int device);
 
+// Statistics (stats.hip).  summary4: buckets, non-empty buckets, points, largest bucket.
+void byte_distribution_host(const uint8_t* hashes, int64_t n, int64_t* counts, int device);
+void full_key_distribution_host(const uint8_t* hashes, int64_t n, int sd, std::vector<uint8_t>& keys,
+                                std::vector<int64_t>& counts, int device);
+void index_bucket_stats(lsh_index* idx, int table, int64_t* summary4, int64_t* hist, int hist_len);
+
 }  // namespace lshgpu

@@ -409,3 +409,43 @@ def read_truth_file(path: str):
             dist[q, j] = float(tok[p + 1])
             p += 2
     return ids, dist
+
+
+def bucket_distribution(hashes):
+    """statistics.cuh:10-16 bucketDistribution: res[hashes[i]] += 1 for every
+    byte; all 256 values (the reference's result array has 255 entries)."""
+    counts = [0] * 256
+    for v in np.asarray(hashes, dtype=np.uint8).reshape(-1).tolist():
+        counts[v] += 1
+    return np.array(counts, dtype=np.int64)
+
+
+def bucket_distribution_full_key(hashes, sketch_dim: int):
+    """statisticsCpu.cpp:11-28 bucketDistributionFullKey: a map from each
+    key's concatenated bitset<8> strings to its count, read in the map's
+    string order.  Returns (keys [m][sketch_dim] uint8, counts [m])."""
+    sd = int(sketch_dim)
+    m = {}
+    for row in np.asarray(hashes, dtype=np.uint8).reshape(-1, sd).tolist():
+        key = "".join(format(b, "08b") for b in row)
+        m[key] = m.get(key, 0) + 1
+    order = sorted(m)
+    keys = np.array([[int(k[8 * j:8 * j + 8], 2) for j in range(sd)] for k in order], dtype=np.uint8)
+    return keys.reshape(len(order), sd), np.array([m[k] for k in order], dtype=np.int64)
+
+
+def bucket_occupancy(keys, table_size: int, hist_len: int):
+    """Bucket sizes of one table from its points' keys (the counts of the
+    reference's bucket build, lshFramework.cuh:102-124): (buckets, non-empty
+    buckets, points, largest bucket) and the histogram of sizes whose last
+    bin collects the sizes >= hist_len - 1."""
+    sizes = [0] * int(table_size)
+    for k in np.asarray(keys).reshape(-1).tolist():
+        sizes[k] += 1
+    hist = [0] * max(int(hist_len), 0)
+    for s in sizes:
+        if hist:
+            hist[min(s, len(hist) - 1)] += 1
+    summary = {"buckets": len(sizes), "nonempty": sum(1 for s in sizes if s), "points": sum(sizes),
+               "max_size": max(sizes) if sizes else 0}
+    return summary, np.array(hist, dtype=np.int64)

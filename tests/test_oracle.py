@@ -422,3 +422,37 @@ def test_lsh_search_bruteforce_sketched(orc):
         want = np.full(32, -1)
         want[:order.size] = top[order]
         np.testing.assert_array_equal(ids[q], want)
+
+
+def test_bucket_distribution_pins(orc):
+    # Hand case (255 has its own bin) and the library histogram.
+    c = orc.bucket_distribution(np.array([255, 255, 0, 7], np.uint8))
+    assert c[255] == 2 and c[0] == 1 and c[7] == 1 and c.sum() == 4
+    h = np.random.default_rng(5).integers(0, 256, 10007).astype(np.uint8)
+    np.testing.assert_array_equal(orc.bucket_distribution(h), np.bincount(h, minlength=256))
+
+
+def test_bucket_distribution_full_key_pins(orc):
+    # The first byte decides the order (bit strings compare as the bytes do).
+    k, c = orc.bucket_distribution_full_key(np.array([[1, 0], [0, 255], [1, 0]], np.uint8), 2)
+    np.testing.assert_array_equal(k, [[0, 255], [1, 0]])
+    np.testing.assert_array_equal(c, [1, 2])
+    rng = np.random.default_rng(6)
+    for sd in (1, 3, 9):
+        pool = rng.integers(0, 256, (40, sd)).astype(np.uint8)
+        h = pool[rng.integers(0, 40, 2000)]
+        k, c = orc.bucket_distribution_full_key(h, sd)
+        uk, uc = np.unique(h, axis=0, return_counts=True)
+        np.testing.assert_array_equal(k, uk)
+        np.testing.assert_array_equal(c, uc)
+
+
+def test_bucket_occupancy_pins(orc):
+    s, h = orc.bucket_occupancy(np.array([3, 3, 3, 0, 5]), 8, 3)
+    assert s == {"buckets": 8, "nonempty": 3, "points": 5, "max_size": 3}
+    np.testing.assert_array_equal(h, [5, 2, 1])  # sizes 0 x5, 1 x2, >=2 x1
+    keys = np.random.default_rng(7).integers(0, 1024, 5000)
+    sizes = np.bincount(keys, minlength=1024)
+    s, h = orc.bucket_occupancy(keys, 1024, 12)
+    assert s["nonempty"] == np.count_nonzero(sizes) and s["max_size"] == sizes.max() and s["points"] == 5000
+    np.testing.assert_array_equal(h, np.bincount(np.minimum(sizes, 11), minlength=12))
