@@ -25,6 +25,7 @@ SOURCES = [
     "scan_exact.hip",
     "scan_sketch.hip",
     "scan_mfma.hip",
+    "cells.hip",
     "rerank.hip",
     "sketch.hip",
     "minhash.hip",

@@ -1,0 +1,837 @@
+// cells.hip -- exact pruning of the matrix-core Hamming pass with pivot cells.
+//
+// Build: P pivots (every N/P-th data row) split the data into cells: each
+// point belongs to its nearest pivot by Hamming distance (ties to the lower
+// pivot).  The points of a cell are laid out by ascending distance to the
+// pivot, in 32-point tiles (the last tile of a cell padded), and each tile
+// records the smallest and largest distance of its points to the pivot.
+// The FP4 image of the data is written again in this order.
+//
+// Search: Hamming distance is a metric, so for a query q, a pivot p and a
+// point x, H(q, x) >= |H(q, p) - H(x, p)|.  A tile whose points lie at
+// [lo, hi] from its pivot is at least max(H(q, p) - hi, lo - H(q, p), 0) from
+// q; when that bound is not below q's threshold distance h (the pass keeps
+// pairs with H < h) no point of the tile can be kept, and the tile is
+// skipped.  A search takes the distances of all queries to all pivots (one
+// matrix-core product), then:
+//   1. Threshold.  Every point x of a tile t of cell j satisfies
+//      H(q, x) <= H(q, p_j) + hi_t, so the K-th smallest of these bounds over
+//      the tiles of the query's nearest cells (those whose pivots are nearest
+//      and hold at least K points, plus a margin) bounds the K-th nearest
+//      distance from above: T = that bound + 1 is a valid threshold.
+//   2. Pass.  Per cell, the queries whose bound on the cell is below their
+//      threshold are listed and cut into chunks of 32; one wave per chunk
+//      walks the cell's tiles, tests each tile's bound against the chunk's
+//      queries, and multiplies the tiles that can hold a hit, appending the
+//      pairs below the threshold.
+// Every pair below the threshold is appended, so the K best of the appended
+// pairs are the exact top-K, as after a pass over all points.
+#include <cstdio>
+#include <cstdlib>
+
+#include "engine.hpp"
+#include "fp4.hpp"
+
+namespace lshgpu {
+
+namespace {
+
+using fp4::expand_word;
+using fp4::v16f;
+using fp4::v4i;
+
+constexpr int kKeyShift = 11;  // cell key = pivot << 11 | distance (distance <= 1024)
+constexpr int kMaxPivots = 8192;  // keys below 2^24 (the counting sort's table limit)
+constexpr uint32_t kDistMask = (1u << kKeyShift) - 1u;
+
+// LSHGPU_DEBUG_CELLS=1 prints the radii, thresholds and list sizes of each
+// cell search to stderr.
+bool debug_cells() {
+    static const bool on = [] {
+        const char* e = std::getenv("LSHGPU_DEBUG_CELLS");
+        return e && e[0] == '1';
+    }();
+    return on;
+}
+
+unsigned blocks_for(size_t n, int per_block) {
+    return (unsigned)std::max<size_t>(1, (n + per_block - 1) / per_block);
+}
+
+// Nearest pivot of each row of `rows` (packed [n][W]) among the P pivots
+// of the FP4 image pfrag: best[row] = pivot << 11 | distance, ties to the
+// lower pivot.  One wave per 32 rows.
+template <int KS>
+__global__ __launch_bounds__(256) void pivot_assign_kernel(const uint32_t* __restrict__ rows, int64_t n,
+                                                           const v4i* __restrict__ pfrag, int P,
+                                                           uint32_t* __restrict__ best) {
+    constexpr int W = 2 * KS;
+    constexpr int BITS = 32 * W;
+    const int lane = threadIdx.x & 63;
+    const int64_t row0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 32;
+    if (row0 >= n) return;
+    const int64_t row = row0 + (lane & 31);
+    const bool live = row < n;
+    v4i b[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) b[s] = expand_word(live ? rows[(size_t)row * W + 2 * s + (lane >> 5)] : 0u);
+    float bdot = -__builtin_inff();
+    int bp = 0;
+    const int ptiles = (P + 31) / 32;
+    for (int pt = 0; pt < ptiles; ++pt) {
+        v16f acc = {};
+#pragma unroll
+        for (int s = 0; s < KS; ++s) acc = fp4::mma(pfrag[((size_t)pt * KS + s) * 64 + lane], b[s], acc);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int p = pt * 32 + fp4::acc_row(r, lane);
+            if (p < P && acc[r] > bdot) {
+                bdot = acc[r];
+                bp = p;
+            }
+        }
+    }
+    const float od = __shfl_xor(bdot, 32);
+    const int op = __shfl_xor(bp, 32);
+    if (od > bdot || (od == bdot && op < bp)) {
+        bdot = od;
+        bp = op;
+    }
+    if (lane < 32 && live) best[row] = ((uint32_t)bp << kKeyShift) | (uint32_t)((BITS - (int)bdot) >> 1);
+}
+
+// D[q][p] (row stride Pp, a multiple of 32) = Hamming distance of query q
+// to pivot p.  Each wave takes 32 queries (blockIdx.x, wave) and a group of
+// kGroup pivot tiles (blockIdx.y); a 32x32 block of distances goes through
+// LDS so that every lane stores 32 contiguous bytes of one query's row.
+constexpr int kGroup = 8;
+template <int KS>
+__global__ __launch_bounds__(256) void pivot_matrix_kernel(const uint32_t* __restrict__ qs, int Q,
+                                                           const v4i* __restrict__ pfrag, int P, int Pp,
+                                                           uint16_t* __restrict__ D) {
+    constexpr int W = 2 * KS;
+    constexpr int BITS = 32 * W;
+    __shared__ uint32_t tb[4][32][33];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int q0 = (blockIdx.x * 4 + wave) * 32;
+    if (q0 >= Q) return;
+    const int q = q0 + (lane & 31);
+    v4i b[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) b[s] = expand_word(q < Q ? qs[(size_t)q * W + 2 * s + (lane >> 5)] : 0u);
+    const int ptiles = (P + 31) / 32;
+    const int pt1 = min(ptiles, (int)(blockIdx.y + 1) * kGroup);
+    uint32_t(*t)[33] = tb[wave];
+    for (int pt = blockIdx.y * kGroup; pt < pt1; ++pt) {
+        v16f acc = {};
+#pragma unroll
+        for (int s = 0; s < KS; ++s) acc = fp4::mma(pfrag[((size_t)pt * KS + s) * 64 + lane], b[s], acc);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) t[lane & 31][fp4::acc_row(r, lane)] = (uint32_t)((BITS - (int)acc[r]) >> 1);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        const int half = lane >> 5;
+        uint32_t w[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            w[i] = t[lane & 31][half * 16 + 2 * i] | (t[lane & 31][half * 16 + 2 * i + 1] << 16);
+        if (q < Q) {
+            uint4* dst = reinterpret_cast<uint4*>(D + (size_t)q * Pp + pt * 32 + half * 16);
+            dst[0] = make_uint4(w[0], w[1], w[2], w[3]);
+            dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    }
+}
+
+// Sorted position r (perm from the counting sort by cell key) -> slot in the
+// cell layout.
+__global__ void cell_place_kernel(const int32_t* __restrict__ perm, const uint32_t* __restrict__ ckey, int64_t N,
+                                  const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ ctile,
+                                  int32_t* __restrict__ ids, uint16_t* __restrict__ pdist) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < N; r += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t i = perm[r];
+        const uint32_t key = ckey[i];
+        const uint32_t j = key >> kKeyShift;
+        const int64_t pos = (int64_t)ctile[j] * 32 + (r - (int64_t)cstart[j]);
+        ids[pos] = i;
+        pdist[pos] = (uint16_t)(key & kDistMask);
+    }
+}
+
+// Per tile: min | max << 11 | rows << 22 of the valid rows (a prefix).
+__global__ void cell_tile_info_kernel(const int32_t* __restrict__ ids, const uint16_t* __restrict__ pdist,
+                                      int64_t ntiles, uint32_t* __restrict__ tinfo) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ntiles;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t lo = kDistMask, hi = 0, nv = 0;
+        for (int i = 0; i < 32; ++i) {
+            if (ids[t * 32 + i] < 0) continue;
+            const uint32_t d = pdist[t * 32 + i];
+            lo = d < lo ? d : lo;
+            hi = d > hi ? d : hi;
+            ++nv;
+        }
+        tinfo[t] = nv ? (lo | (hi << kKeyShift) | (nv << 22)) : 0u;
+    }
+}
+
+// Per cell: distance range of its points to the pivot (first and last point
+// of the cell in layout order); an empty cell gets min > max.
+__global__ void cell_range_kernel(const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ ctile,
+                                  const uint16_t* __restrict__ pdist, int P, uint16_t* __restrict__ cmin,
+                                  uint16_t* __restrict__ cmax) {
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < P; j += gridDim.x * blockDim.x) {
+        const uint32_t n = cstart[j + 1] - cstart[j];
+        if (n == 0) {
+            cmin[j] = 1;
+            cmax[j] = 0;
+        } else {
+            cmin[j] = pdist[(size_t)ctile[j] * 32];
+            cmax[j] = pdist[(size_t)ctile[j] * 32 + n - 1];
+        }
+    }
+}
+
+__global__ void cell_starts_kernel(const uint32_t* __restrict__ off, int P, uint32_t* __restrict__ cstart) {
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j <= P; j += gridDim.x * blockDim.x)
+        cstart[j] = off[(size_t)j << kKeyShift];
+}
+
+// Packed sketches in cell order: position pos takes data row ids[pos], its
+// even words first, then its odd words (zero for padding, which the tile
+// row counts exclude), so the matrix-core lane that supplies word 2s + h of
+// a row reads the 4*W/2 contiguous bytes of half h.
+__global__ void pack_cells_kernel(const uint32_t* __restrict__ ds, int wpad, const int32_t* __restrict__ ids,
+                                  int64_t npos, uint32_t* __restrict__ sk) {
+    const int64_t total = npos * wpad;
+    const int KS = wpad / 2;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t pos = i / wpad;
+        const int w = (int)(i - pos * wpad);
+        const int src = w < KS ? 2 * w : 2 * (w - KS) + 1;
+        const int32_t row = ids[pos];
+        sk[i] = row >= 0 ? ds[(size_t)row * wpad + src] : 0u;
+    }
+}
+
+// Float rows in cell order (zero rows for padding) and the position of
+// every data row.
+__global__ void cell_rows_kernel(const float* __restrict__ src, int d, const int32_t* __restrict__ ids,
+                                 int64_t npos, float* __restrict__ dst, int32_t* __restrict__ pos_of) {
+    const int64_t total = npos * d;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t pos = i / d;
+        const int c = (int)(i - pos * d);
+        const int32_t row = ids[pos];
+        dst[i] = row >= 0 ? src[(size_t)row * d + c] : 0.0f;
+        if (c == 0 && row >= 0) pos_of[row] = (int32_t)pos;
+    }
+}
+
+// cmap[c] = the cell of chunk c (choff: exclusive prefix of chunks per cell).
+__global__ void chunk_map_kernel(const uint32_t* __restrict__ choff, int P, uint32_t* __restrict__ cmap) {
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < P; j += gridDim.x * blockDim.x)
+        for (uint32_t c = choff[j]; c < choff[j + 1]; ++c) cmap[c] = (uint32_t)j;
+}
+
+__device__ inline uint32_t lower_bound_to(uint32_t dq, uint32_t lo, uint32_t hi) {
+    return dq > hi ? dq - hi : (lo > dq ? lo - dq : 0u);
+}
+
+// Smallest bin b with sum(hist[0..b]) >= K (nbins when the total is below
+// K), by the first wave of the workgroup.
+__device__ int kth_bin(const uint32_t* hist, int nbins, uint32_t K) {
+    const int lane = threadIdx.x & 63;
+    uint32_t carry = 0;
+    for (int base = 0; base < nbins; base += 64) {
+        const int b = base + lane;
+        uint32_t inc = b < nbins ? hist[b] : 0u;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t x = __shfl_up(inc, o);
+            if (lane >= o) inc += x;
+        }
+        const uint64_t hit = __ballot(b < nbins && carry + inc >= K);
+        if (hit) return base + __ffsll((unsigned long long)hit) - 1;
+        carry += __shfl(inc, 63);
+    }
+    return nbins;
+}
+
+// Threshold of each query (one workgroup per query): the radius R at which
+// the cells nearest to the query hold K points (kept for the first pass),
+// then the K-th smallest
+// upper bound D + hi_t (weighted by the tile's rows) over the tiles of the
+// cells within R + kRadiusMargin; thr = (bound + 1, 0).
+constexpr int kRadiusMargin = 16;
+__global__ __launch_bounds__(256) void cell_threshold_kernel(const uint16_t* __restrict__ D, int Q, int P, int Pp,
+                                                             const uint32_t* __restrict__ csize,
+                                                             const uint32_t* __restrict__ ctile,
+                                                             const uint32_t* __restrict__ tinfo, int bits, int K,
+                                                             uint64_t* __restrict__ thr, uint16_t* __restrict__ R,
+                                                             uint32_t* __restrict__ nearest) {
+    __shared__ uint32_t h1[1025];
+    __shared__ uint32_t h2[2049];
+    __shared__ int rad;
+    __shared__ uint32_t best;
+    const int q = blockIdx.x;
+    const uint16_t* __restrict__ row = D + (size_t)q * Pp;
+    for (int b = threadIdx.x; b < 2049; b += 256) {
+        if (b < 1025) h1[b] = 0;
+        h2[b] = 0;
+    }
+    if (threadIdx.x == 0) best = 0xFFFFFFFFu;
+    __syncthreads();
+    uint32_t mine = 0xFFFFFFFFu;  // (distance, pivot) of the nearest non-empty cell
+    for (int j = threadIdx.x; j < P; j += 256) {
+        const uint32_t n = csize[j];
+        if (n) {
+            atomicAdd(&h1[row[j]], n);
+            mine = min(mine, ((uint32_t)row[j] << 13) | (uint32_t)j);
+        }
+    }
+    atomicMin(&best, mine);
+    __syncthreads();
+    if (threadIdx.x == 0) nearest[q] = best == 0xFFFFFFFFu ? 0u : (best & 0x1FFFu);
+    if (threadIdx.x < 64) {
+        const int r = kth_bin(h1, bits + 1, (uint32_t)K);
+        if (threadIdx.x == 0) {
+            rad = r + kRadiusMargin;
+            R[q] = (uint16_t)min(r, bits);
+        }
+    }
+    __syncthreads();
+    const int rm = rad;
+    for (int j = threadIdx.x; j < P; j += 256) {
+        const int dq = row[j];
+        if (dq > rm || csize[j] == 0) continue;
+        for (uint32_t t = ctile[j]; t < ctile[j + 1]; ++t) {
+            const uint32_t info = tinfo[t];
+            atomicAdd(&h2[dq + ((info >> kKeyShift) & kDistMask)], info >> 22);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int ub = kth_bin(h2, 2 * bits + 1, (uint32_t)K);
+        if (threadIdx.x == 0) thr[q] = ub > 2 * bits ? kEmptyKey : make_key((uint32_t)ub + 1u, 0u);
+    }
+}
+
+// Per-cell query lists.  R == nullptr: the queries whose bound on the cell
+// is below their threshold distance; otherwise those whose distance to the
+// cell's pivot is at most their radius (the first pass).  list == nullptr:
+// count into cnt; otherwise place them.  Grid: x over cells, y (strided)
+// over queries.
+__global__ void cell_need_kernel(const uint16_t* __restrict__ D, int Q, int P, int Pp,
+                                 const uint16_t* __restrict__ cmin, const uint16_t* __restrict__ cmax,
+                                 const uint64_t* __restrict__ thr, const uint16_t* __restrict__ R,
+                                 uint32_t* __restrict__ cnt, const uint32_t* __restrict__ coff,
+                                 int32_t* __restrict__ list) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= P) return;
+    const uint32_t lo = cmin[j], hi = cmax[j];
+    if (lo > hi) return;
+    for (int q = blockIdx.y; q < Q; q += gridDim.y) {
+        const uint32_t dq = D[(size_t)q * Pp + j];
+        if (R ? dq > R[q] : lower_bound_to(dq, lo, hi) >= (uint32_t)(thr[q] >> 32)) continue;
+        const uint32_t slot = atomicAdd(&cnt[j], 1u);
+        if (list) list[coff[j] + slot] = q;
+    }
+}
+
+// coff = exclusive prefix of cnt, choff = exclusive prefix of ceil(cnt/32),
+// both with a final total entry; cursor (= cnt for the placing pass) zeroed.
+__global__ __launch_bounds__(1024) void cell_offsets_kernel(uint32_t* __restrict__ cnt, int P,
+                                                            uint32_t* __restrict__ coff,
+                                                            uint32_t* __restrict__ choff) {
+    __shared__ uint32_t ws[2][16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t carry_a = 0, carry_b = 0;
+    for (int c0 = 0; c0 < P; c0 += 1024) {
+        const int j = c0 + threadIdx.x;
+        const uint32_t a = j < P ? cnt[j] : 0u;
+        const uint32_t b = (a + 31) / 32;
+        uint32_t ia = a, ib = b;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t ta = __shfl_up(ia, o), tb = __shfl_up(ib, o);
+            if (lane >= o) {
+                ia += ta;
+                ib += tb;
+            }
+        }
+        if (lane == 63) {
+            ws[0][w] = ia;
+            ws[1][w] = ib;
+        }
+        __syncthreads();
+        uint32_t pa = 0, pb = 0, ta = 0, tb = 0;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            pa += q < w ? ws[0][q] : 0u;
+            pb += q < w ? ws[1][q] : 0u;
+            ta += ws[0][q];
+            tb += ws[1][q];
+        }
+        if (j < P) {
+            coff[j] = carry_a + pa + ia - a;
+            choff[j] = carry_b + pb + ib - b;
+            cnt[j] = 0;
+        }
+        carry_a += ta;
+        carry_b += tb;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        coff[P] = carry_a;
+        choff[P] = carry_b;
+    }
+}
+
+// Coarse distance histogram of the exact first pass: bin = distance / 8,
+// the last bin also holding the distances past it.  In LDS two 16-bit
+// counters share a word; a wave flushes them to the global histogram at the
+// end of its chunk and after every kFlushTiles tiles (32 pairs per query
+// per tile, so no counter reaches 2^16).
+constexpr int kHistBins = 128;
+constexpr int kFlushTiles = 2047;
+__device__ inline int hist_bin(int d) { return min(d >> 3, kHistBins - 1); }
+
+// One wave per chunk of up to 32 queries of one cell.  kHist: counts the
+// distances of every pair in hist[q][bin] (the first pass); otherwise
+// appends the pairs below each query's threshold to v, skipping the tiles
+// whose bound rules them out for every query of the chunk.  The packed
+// point sketches of the next tile are loaded while the current one is
+// multiplied.
+template <int KS, bool kHist>
+__global__ __launch_bounds__(256) void cell_scan_kernel(const uint32_t* __restrict__ sk,
+                                                        const int32_t* __restrict__ ids,
+                                                        const uint32_t* __restrict__ tinfo,
+                                                        const uint32_t* __restrict__ ctile,
+                                                        const uint32_t* __restrict__ coff,
+                                                        const uint32_t* __restrict__ choff,
+                                                        const uint32_t* __restrict__ cmap, uint32_t nchunks, int Pp,
+                                                        const int32_t* __restrict__ list,
+                                                        const uint16_t* __restrict__ D,
+                                                        const uint32_t* __restrict__ qs, TopKView v,
+                                                        uint32_t* __restrict__ hist) {
+    constexpr int W = 2 * KS;
+    constexpr int BITS = 32 * W;
+    constexpr int HW = kHistBins / 2;  // LDS words per query
+    constexpr int LH = kHist ? 4 * 32 * HW : 1;
+    constexpr int NV = KS / 4;         // uint4 loads per lane per tile
+    __shared__ uint32_t lhist[LH];
+    const int lane = threadIdx.x & 63;
+    const uint32_t c = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (c >= nchunks) return;
+    const uint32_t j = cmap[c];
+    const uint32_t q0 = coff[j] + 32 * (c - choff[j]);
+    const uint32_t m = min(32u, coff[j + 1] - q0);
+    const int col = lane & 31;
+    const int q = (uint32_t)col < m ? list[q0 + col] : -1;
+    uint32_t h = 0;
+    float tf = __builtin_inff();  // idle columns never pass
+    uint32_t dq = 0;
+    if (q >= 0) {
+        h = kHist ? 0xFFFFFFFFu : (uint32_t)(v.thr[q] >> 32);
+        tf = h == 0xFFFFFFFFu ? -__builtin_inff() : (float)BITS - 2.0f * (float)h;
+        dq = D[(size_t)q * Pp + j];
+    }
+    uint32_t* lh = lhist + (kHist ? (threadIdx.x >> 6) * 32 * HW : 0);
+    auto flush = [&]() {
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        for (int e = lane; e < 32 * HW; e += 64) {
+            const uint32_t x = lh[e];
+            const int cq = e / HW;
+            if (x && (uint32_t)cq < m) {
+                uint32_t* row = hist + (size_t)list[q0 + cq] * kHistBins + 2 * (e % HW);
+                if (x & 0xFFFFu) atomicAdd(&row[0], x & 0xFFFFu);
+                if (x >> 16) atomicAdd(&row[1], x >> 16);
+            }
+            lh[e] = 0;
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    };
+    if (kHist) {
+        for (int e = lane; e < 32 * HW; e += 64) lh[e] = 0;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    }
+    v4i b[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) b[s] = expand_word(q >= 0 ? qs[(size_t)q * W + 2 * s + (lane >> 5)] : 0u);
+    const uint32_t t0 = ctile[j], nt = ctile[j + 1] - t0;
+    // Lane l's half of point row l%32 of tile t: W/2 words.
+    auto load = [&](uint32_t t, uint4* x) {
+        const uint4* src = reinterpret_cast<const uint4*>(sk + ((size_t)t * 32 + col) * W + (lane >> 5) * KS);
+#pragma unroll
+        for (int i = 0; i < NV; ++i) x[i] = src[i];
+    };
+    int since_flush = 0;
+    for (uint32_t i0 = 0; i0 < nt; i0 += 64) {
+        const uint32_t n64 = min(64u, nt - i0);
+        const uint32_t myinfo = (uint32_t)lane < n64 ? tinfo[t0 + i0 + lane] : 0u;
+        uint64_t needm = n64 == 64 ? ~0ull : ((1ull << n64) - 1ull);
+        if (!kHist) {
+            uint64_t nm = 0;
+            for (uint32_t tt = 0; tt < n64; ++tt) {
+                const uint32_t info = (uint32_t)__shfl((int)myinfo, (int)tt);
+                const uint32_t tlo = info & kDistMask, thi = (info >> kKeyShift) & kDistMask;
+                if (__ballot(q >= 0 && lower_bound_to(dq, tlo, thi) < h)) nm |= 1ull << tt;
+            }
+            needm = nm;
+        }
+        if (!needm) continue;
+        uint4 nx[NV];
+        int nt_next = __builtin_ctzll(needm);
+        load(t0 + i0 + nt_next, nx);
+        while (true) {
+            const int tt = nt_next;
+            uint4 cur[NV];
+#pragma unroll
+            for (int i = 0; i < NV; ++i) cur[i] = nx[i];
+            needm &= needm - 1ull;
+            if (needm) {
+                nt_next = __builtin_ctzll(needm);
+                load(t0 + i0 + nt_next, nx);
+            }
+            const uint32_t t = t0 + i0 + (uint32_t)tt;
+            const uint32_t nv = (uint32_t)__shfl((int)myinfo, tt) >> 22;
+            const uint32_t* cw = reinterpret_cast<const uint32_t*>(cur);
+            v16f acc = {};
+#pragma unroll
+            for (int s = 0; s < KS; ++s) acc = fp4::mma(expand_word(cw[s]), b[s], acc);
+            uint32_t pass = 0;
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                pass |= (acc[r] > tf && (uint32_t)fp4::acc_row(r, lane) < nv ? 1u : 0u) << r;
+            if (kHist) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    if ((pass >> r) & 1u) {
+                        const int bin = hist_bin((BITS - (int)acc[r]) >> 1);
+                        atomicAdd(&lh[col * HW + (bin >> 1)], 1u << (16 * (bin & 1)));
+                    }
+                if (++since_flush == kFlushTiles) {
+                    flush();
+                    since_flush = 0;
+                }
+            } else {
+                const uint32_t mine = (uint32_t)__popc(pass);
+                const uint32_t other = (uint32_t)__shfl_xor((int)mine, 32);
+                uint32_t base = 0;
+                if (lane < 32 && mine + other > 0) base = atomicAdd(&v.cnt[q], mine + other);
+                base = (uint32_t)__shfl((int)base, col);
+                if (lane >= 32) base += other;
+                if (pass) {
+                    const size_t row0 = (size_t)q * v.stride + v.K;
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        if (!((pass >> r) & 1u)) continue;
+                        if (base < (uint32_t)v.newcap) {
+                            const int32_t id = ids[(size_t)t * 32 + fp4::acc_row(r, lane)];
+                            v.keys[row0 + base] = make_key((uint32_t)((BITS - (int)acc[r]) >> 1), (uint32_t)id);
+                        }
+                        ++base;
+                    }
+                }
+            }
+            if (!needm) break;
+        }
+    }
+    if (kHist && since_flush) flush();
+}
+
+// thr[q] = min(thr[q], (T + 1, 0)) with T the upper edge of the bin at which
+// the query's first-pass counts reach K (the last bin's edge: bits).  One
+// wave per query.
+__global__ __launch_bounds__(256) void hist_threshold_kernel(const uint32_t* __restrict__ hist, int Q, int bits,
+                                                             int K, uint64_t* __restrict__ thr) {
+    const int q = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= Q) return;
+    const int b = kth_bin(hist + (size_t)q * kHistBins, kHistBins, (uint32_t)K);
+    if ((threadIdx.x & 63) == 0 && b < kHistBins) {
+        const uint32_t T = b == kHistBins - 1 ? (uint32_t)bits : (uint32_t)(8 * b + 7);
+        const uint64_t t = make_key(T + 1u, 0u);
+        if (t < thr[q]) thr[q] = t;
+    }
+}
+
+template <int KS>
+void launch_pivots_ks(const uint32_t* rows, int64_t n, const uint32_t* pfrag, int P, int Pp, uint32_t* best,
+                      uint16_t* D, hipStream_t s) {
+    const v4i* f = reinterpret_cast<const v4i*>(pfrag);
+    if (best) {
+        hipLaunchKernelGGL((pivot_assign_kernel<KS>), dim3(blocks_for((size_t)((n + 31) / 32), 4)), dim3(256), 0, s,
+                           rows, n, f, P, best);
+        check_launch("pivot_assign_kernel");
+    } else {
+        const unsigned gy = (unsigned)(((P + 31) / 32 + kGroup - 1) / kGroup);
+        hipLaunchKernelGGL((pivot_matrix_kernel<KS>), dim3(blocks_for((size_t)((n + 31) / 32), 4), gy), dim3(256), 0,
+                           s, rows, (int)n, f, P, Pp, D);
+        check_launch("pivot_matrix_kernel");
+    }
+}
+
+// best != nullptr: nearest pivot of each row; otherwise the distance matrix D.
+void pivots(int words, const uint32_t* rows, int64_t n, const uint32_t* pfrag, int P, int Pp, uint32_t* best,
+            uint16_t* D, hipStream_t s) {
+    switch (words) {
+        case 8: launch_pivots_ks<4>(rows, n, pfrag, P, Pp, best, D, s); break;
+        case 16: launch_pivots_ks<8>(rows, n, pfrag, P, Pp, best, D, s); break;
+        case 24: launch_pivots_ks<12>(rows, n, pfrag, P, Pp, best, D, s); break;
+        case 32: launch_pivots_ks<16>(rows, n, pfrag, P, Pp, best, D, s); break;
+        default: fail(LSH_ERR_INTERNAL, "pivot cells need 8, 16, 24 or 32 sketch words");
+    }
+}
+
+dim3 need_grid(int P, int Q) {
+    return dim3((unsigned)((P + 255) / 256), (unsigned)std::max(1, std::min(Q, 4096)));
+}
+
+template <int KS>
+void launch_cell_scan_ks(const CellIndex& c, unsigned chunks, const uint32_t* coff, const uint32_t* choff,
+                         const uint32_t* cmap, int Pp, const int32_t* list, const uint16_t* D, const uint32_t* qs,
+                         const TopKView& v, uint32_t* hist, hipStream_t s) {
+    if (hist)
+        hipLaunchKernelGGL((cell_scan_kernel<KS, true>), dim3(blocks_for(chunks, 4)), dim3(256), 0, s, c.sk.get(),
+                           c.ids.get(), c.tinfo.get(), c.ctile.get(), coff, choff, cmap, chunks, Pp, list, D, qs, v,
+                           hist);
+    else
+        hipLaunchKernelGGL((cell_scan_kernel<KS, false>), dim3(blocks_for(chunks, 4)), dim3(256), 0, s, c.sk.get(),
+                           c.ids.get(), c.tinfo.get(), c.ctile.get(), coff, choff, cmap, chunks, Pp, list, D, qs, v,
+                           hist);
+    check_launch("cell_scan_kernel");
+}
+
+// The chunk -> cell map (after the lists' offsets), then the scan.
+void cell_scan(const CellIndex& c, unsigned chunks, const uint32_t* coff, const uint32_t* choff, uint32_t* cmap,
+               int Pp, const int32_t* list, const uint16_t* D, const uint32_t* qs, const TopKView& v, uint32_t* hist,
+               hipStream_t s) {
+    hipLaunchKernelGGL(chunk_map_kernel, dim3(blocks_for((size_t)c.P, 256)), dim3(256), 0, s, choff, c.P, cmap);
+    check_launch("chunk_map_kernel");
+    switch (c.words) {
+        case 8: launch_cell_scan_ks<4>(c, chunks, coff, choff, cmap, Pp, list, D, qs, v, hist, s); break;
+        case 16: launch_cell_scan_ks<8>(c, chunks, coff, choff, cmap, Pp, list, D, qs, v, hist, s); break;
+        case 24: launch_cell_scan_ks<12>(c, chunks, coff, choff, cmap, Pp, list, D, qs, v, hist, s); break;
+        case 32: launch_cell_scan_ks<16>(c, chunks, coff, choff, cmap, Pp, list, D, qs, v, hist, s); break;
+        default: fail(LSH_ERR_INTERNAL, "pivot cells need 8, 16, 24 or 32 sketch words");
+    }
+}
+
+struct Lists {
+    uint32_t entries = 0, chunks = 0;
+};
+
+// Counts the per-cell query lists (R: first-pass radii, or nullptr for the
+// bounds of thr) and their chunking; one host synchronisation.
+Lists count_lists(const CellIndex& c, const uint16_t* D, int Q, int Pp, const uint64_t* thr, const uint16_t* R,
+                  uint32_t* cnt, uint32_t* coff, uint32_t* choff, hipStream_t s) {
+    const int P = c.P;
+    LSH_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t) * (P + 1), s));
+    hipLaunchKernelGGL(cell_need_kernel, need_grid(P, Q), dim3(256), 0, s, D, Q, P, Pp, c.cmin.get(), c.cmax.get(),
+                       thr, R, cnt, nullptr, nullptr);
+    check_launch("cell_need_kernel");
+    hipLaunchKernelGGL(cell_offsets_kernel, dim3(1), dim3(1024), 0, s, cnt, P, coff, choff);
+    check_launch("cell_offsets_kernel");
+    uint32_t tot[2] = {0, 0};
+    LSH_HIP(hipMemcpyAsync(&tot[0], coff + P, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    LSH_HIP(hipMemcpyAsync(&tot[1], choff + P, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    LSH_HIP(hipStreamSynchronize(s));
+    return Lists{tot[0], tot[1]};
+}
+
+void place_lists(const CellIndex& c, const uint16_t* D, int Q, int Pp, const uint64_t* thr, const uint16_t* R,
+                 uint32_t* cnt, const uint32_t* coff, int32_t* list, hipStream_t s) {
+    hipLaunchKernelGGL(cell_need_kernel, need_grid(c.P, Q), dim3(256), 0, s, D, Q, c.P, Pp, c.cmin.get(),
+                       c.cmax.get(), thr, R, cnt, coff, list);
+    check_launch("cell_need_kernel");
+}
+
+}  // namespace
+
+int cell_pivots_for(int64_t N) {
+    if (N < kCellMinPoints) return 0;
+    int P = 64;
+    while (P < 8192 && (int64_t)P * 2 * 256 <= N) P *= 2;
+    return P;
+}
+
+void CellIndex::build(const uint32_t* ds, int64_t N, int wpad, hipStream_t s) {
+    P = 0;
+    int np = cell_pivots_for(N);
+    if (np == 0 || !hamming_mfma_supported(wpad)) return;
+    LSH_REQUIRE(N < ((int64_t)1 << 31), LSH_ERR_UNSUPPORTED, "pivot cells need N < 2^31");
+    words = wpad;
+    // Pivots: every (N/np)-th row.  A group of points that drew no pivot of
+    // its own ends up far from the pivot it joins and widens that cell's
+    // range; so points farther than twice the median distance from their
+    // pivot seed more pivots (a stride over them, up to np/8 a round, two
+    // rounds), and the points are assigned again.
+    const int bits = 32 * wpad;
+    std::vector<int32_t> prow((size_t)np);
+    const int64_t step = N / np;
+    for (int i = 0; i < np; ++i) prow[i] = (int32_t)(i * step);
+    DevBuf<uint32_t> ckey((size_t)N);
+    for (int round = 0;; ++round) {
+        const int n = (int)prow.size();
+        DevBuf<int32_t> didx((size_t)n);
+        LSH_HIP(hipMemcpyAsync(didx.get(), prow.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+        DevBuf<uint32_t> psk((size_t)n * wpad);
+        launch_gather_rows_u32(ds, wpad, didx.get(), n, psk.get(), s);
+        pfp4.alloc(hamming_fp4_words(n, wpad));
+        launch_expand_fp4(psk.get(), n, wpad, pfp4.get(), s);
+        // Nearest pivot of every point.
+        pivots(wpad, ds, N, pfp4.get(), n, 0, ckey.get(), nullptr, s);
+        if (round == 2 || n >= kMaxPivots) break;
+        std::vector<uint32_t> hk((size_t)N);
+        LSH_HIP(hipMemcpyAsync(hk.data(), ckey.get(), sizeof(uint32_t) * N, hipMemcpyDeviceToHost, s));
+        LSH_HIP(hipStreamSynchronize(s));
+        std::vector<int64_t> dh((size_t)bits + 1, 0);
+        for (uint32_t k : hk) ++dh[k & kDistMask];
+        int64_t acc = 0;
+        int median = 0;
+        while (median < bits && (acc += dh[median]) * 2 < N) ++median;
+        std::vector<int32_t> far;
+        for (int64_t i = 0; i < N; ++i)
+            if ((int)(hk[i] & kDistMask) > 2 * median) far.push_back((int32_t)i);
+        if (far.empty()) break;
+        const size_t add = std::min<size_t>({far.size(), (size_t)(kMaxPivots - n), (size_t)std::max(1, n / 8)});
+        for (size_t i = 0; i < add; ++i) prow.push_back(far[i * far.size() / add]);
+    }
+    np = (int)prow.size();
+
+    // Counting sort by (pivot, distance).
+    const uint32_t T = (uint32_t)np << kKeyShift;
+    DevBuf<uint32_t> off((size_t)T + 1), cursor((size_t)T + 1), perm((size_t)N);
+    launch_bucket_build(ckey.get(), N, 1, T, off.get(), cursor.get(), perm.get(), s);
+    DevBuf<uint32_t> cstart((size_t)np + 1);
+    hipLaunchKernelGGL(cell_starts_kernel, dim3(blocks_for((size_t)np + 1, 256)), dim3(256), 0, s, off.get(), np,
+                       cstart.get());
+    check_launch("cell_starts_kernel");
+    std::vector<uint32_t> hs((size_t)np + 1), ht((size_t)np + 1);
+    LSH_HIP(hipMemcpyAsync(hs.data(), cstart.get(), sizeof(uint32_t) * hs.size(), hipMemcpyDeviceToHost, s));
+    LSH_HIP(hipStreamSynchronize(s));
+    ht[0] = 0;
+    for (int j = 0; j < np; ++j) ht[j + 1] = ht[j] + (hs[j + 1] - hs[j] + 31) / 32;
+    ntiles = ht[np];
+    ctile.alloc((size_t)np + 1);
+    LSH_HIP(hipMemcpyAsync(ctile.get(), ht.data(), sizeof(uint32_t) * ht.size(), hipMemcpyHostToDevice, s));
+    std::vector<uint32_t> hz((size_t)np);
+    for (int j = 0; j < np; ++j) hz[j] = hs[j + 1] - hs[j];
+    csize.alloc((size_t)np);
+    LSH_HIP(hipMemcpyAsync(csize.get(), hz.data(), sizeof(uint32_t) * hz.size(), hipMemcpyHostToDevice, s));
+
+    const int64_t npos = ntiles * 32;
+    ids.alloc((size_t)npos);
+    launch_fill_i32(ids.get(), (size_t)npos, -1, s);
+    DevBuf<uint16_t> pdist((size_t)npos);
+    hipLaunchKernelGGL(cell_place_kernel, dim3(blocks_for((size_t)N, 256)), dim3(256), 0, s,
+                       reinterpret_cast<const int32_t*>(perm.get()), ckey.get(), N, cstart.get(), ctile.get(),
+                       ids.get(), pdist.get());
+    check_launch("cell_place_kernel");
+    tinfo.alloc((size_t)ntiles);
+    hipLaunchKernelGGL(cell_tile_info_kernel, dim3(blocks_for((size_t)ntiles, 256)), dim3(256), 0, s, ids.get(),
+                       pdist.get(), ntiles, tinfo.get());
+    check_launch("cell_tile_info_kernel");
+    cmin.alloc((size_t)np);
+    cmax.alloc((size_t)np);
+    hipLaunchKernelGGL(cell_range_kernel, dim3(blocks_for((size_t)np, 256)), dim3(256), 0, s, cstart.get(),
+                       ctile.get(), pdist.get(), np, cmin.get(), cmax.get());
+    check_launch("cell_range_kernel");
+    sk.alloc((size_t)npos * wpad);
+    hipLaunchKernelGGL(pack_cells_kernel, dim3(blocks_for((size_t)npos * wpad, 256)), dim3(256), 0, s, ds, wpad,
+                       ids.get(), npos, sk.get());
+    check_launch("pack_cells_kernel");
+    LSH_HIP(hipStreamSynchronize(s));
+    P = np;
+}
+
+void cell_rows(const CellIndex& c, const float* data, int64_t N, int d, DevBuf<float>& rows, DevBuf<int32_t>& pos,
+               hipStream_t s) {
+    const int64_t npos = c.ntiles * 32;
+    rows.alloc((size_t)npos * d);
+    pos.alloc((size_t)N);
+    hipLaunchKernelGGL(cell_rows_kernel, dim3((unsigned)std::min<size_t>(65535u * 16u, blocks_for((size_t)npos * d, 256))),
+                       dim3(256), 0, s, data, d, c.ids.get(), npos, rows.get(), pos.get());
+    check_launch("cell_rows_kernel");
+}
+
+size_t CellIndex::bytes() const {
+    return pfp4.bytes() + sk.bytes() + ids.bytes() + tinfo.bytes() + ctile.bytes() + csize.bytes() + cmin.bytes() +
+           cmax.bytes();
+}
+
+bool cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K, const TopKView& v, Scratch& sc, int depth,
+                 hipStream_t s, int64_t* tiles_out, int32_t* qorder) {
+    if (!c.built() || Q <= 0) return false;
+    const int P = c.P;
+    const int Pp = (P + 31) / 32 * 32;
+    const int bits = 32 * c.words;
+    uint16_t* D = sc.get<uint16_t>(depth, 0, (size_t)Q * Pp);
+    uint32_t* meta = sc.get<uint32_t>(depth, 1, 5 * ((size_t)P + 1) + Q);
+    uint32_t* cnt = meta;
+    uint32_t* coff = meta + (P + 1);
+    uint32_t* choff = meta + 2 * (P + 1);
+    uint32_t* nearest = meta + 5 * (P + 1);
+    const size_t hn = (size_t)Q * kHistBins;
+    uint32_t* hist = sc.get<uint32_t>(depth, 3, hn + ((size_t)Q + 1) / 2);
+    uint16_t* R = reinterpret_cast<uint16_t*>(hist + hn);
+    const double mean_tiles = (double)c.ntiles / P;
+    pivots(c.words, qs, Q, c.pfp4.get(), P, Pp, nullptr, D, s);
+    // Threshold bound from the tile ranges, the first-pass radii and the
+    // nearest cell of each query.
+    hipLaunchKernelGGL(cell_threshold_kernel, dim3((unsigned)Q), dim3(256), 0, s, D, Q, P, Pp, c.csize.get(),
+                       c.ctile.get(), c.tinfo.get(), bits, K, v.thr, R, nearest);
+    check_launch("cell_threshold_kernel");
+    if (qorder) {
+        // Queries grouped by nearest cell (a counting sort): neighbouring
+        // queries then share most candidates, which the rerank can reuse.
+        launch_bucket_build(nearest, Q, 1, (uint32_t)P, meta + 3 * (P + 1), meta + 4 * (P + 1),
+                            reinterpret_cast<uint32_t*>(qorder), s);
+    }
+    // First pass: exact distances over the nearest cells, K-th by histogram.
+    LSH_HIP(hipMemsetAsync(hist, 0, sizeof(uint32_t) * hn, s));
+    const Lists l1 = count_lists(c, D, Q, Pp, nullptr, R, cnt, coff, choff, s);
+    int32_t* list = sc.get<int32_t>(depth, 2, (size_t)l1.entries + l1.chunks + 1);
+    if (l1.chunks) {
+        place_lists(c, D, Q, Pp, nullptr, R, cnt, coff, list, s);
+        cell_scan(c, l1.chunks, coff, choff, reinterpret_cast<uint32_t*>(list + l1.entries), Pp, list, D, qs, v,
+                  hist, s);
+        hipLaunchKernelGGL(hist_threshold_kernel, dim3((unsigned)((Q + 3) / 4)), dim3(256), 0, s, hist, Q, bits, K,
+                           v.thr);
+        check_launch("hist_threshold_kernel");
+    }
+    // Second pass: the tiles the bounds leave.
+    const Lists l2 = count_lists(c, D, Q, Pp, v.thr, nullptr, cnt, coff, choff, s);
+    if (debug_cells()) {
+        std::vector<uint64_t> ht((size_t)Q);
+        LSH_HIP(hipMemcpy(ht.data(), v.thr, sizeof(uint64_t) * Q, hipMemcpyDeviceToHost));
+        double mt = 0;
+        for (int q = 0; q < Q; ++q) mt += ht[q] == kEmptyKey ? 2.0 * bits : (double)(ht[q] >> 32);
+        std::fprintf(stderr,
+                     "cells: P=%d tiles=%lld Q=%d K=%d mean T=%.1f first pass %u entries %u chunks, second %u "
+                     "entries %u chunks\n",
+                     P, (long long)c.ntiles, Q, K, mt / Q, l1.entries, l1.chunks, l2.entries, l2.chunks);
+    }
+    // Work in tiles (chunks x mean tiles per cell) against the full pass
+    // (query tiles x point tiles): too dense, leave it to the full pass.
+    const double pruned = (double)l2.chunks * mean_tiles;
+    const double full = std::ceil(Q / 32.0) * (double)c.ntiles;
+    if (pruned > 0.5 * full) return false;
+    if (tiles_out) *tiles_out += (int64_t)((l1.chunks + (double)l2.chunks) * mean_tiles);
+    if (l2.chunks == 0) return true;
+    list = sc.get<int32_t>(depth, 2, (size_t)l2.entries + l2.chunks);
+    place_lists(c, D, Q, Pp, v.thr, nullptr, cnt, coff, list, s);
+    cell_scan(c, l2.chunks, coff, choff, reinterpret_cast<uint32_t*>(list + l2.entries), Pp, list, D, qs, v,
+              nullptr, s);
+    return true;
+}
+
+}  // namespace lshgpu

@@ -59,6 +59,69 @@ __device__ inline double dist64_dev(int kind, const float* __restrict__ q,
     }
 }
 
+// dist64_dev with the query's squared norm nq (angular kinds) given, and the
+// row read with 16-byte loads when d % 4 == 0 and x is 16-byte aligned; the
+// operations and their order are those of dist64_dev, so the result is the
+// same.
+__device__ inline double dist64_row(int kind, const float* __restrict__ q, const float* __restrict__ x, int d,
+                                    double nq) {
+    if ((d & 3) != 0 || (reinterpret_cast<uintptr_t>(x) & 15) != 0 || (reinterpret_cast<uintptr_t>(q) & 15) != 0)
+        return dist64_dev(kind, q, x, d);
+    const float4* __restrict__ x4 = reinterpret_cast<const float4*>(x);
+    const float4* __restrict__ q4 = reinterpret_cast<const float4*>(q);
+    if (kind == kNegCos || kind == kAngular) {
+        double dot = 0, nx = 0;
+#pragma unroll 5
+        for (int j = 0; j < (d >> 2); ++j) {
+            const float4 a = q4[j], b = x4[j];
+            dot = __builtin_fma((double)a.x, (double)b.x, dot);
+            nx = __builtin_fma((double)b.x, (double)b.x, nx);
+            dot = __builtin_fma((double)a.y, (double)b.y, dot);
+            nx = __builtin_fma((double)b.y, (double)b.y, nx);
+            dot = __builtin_fma((double)a.z, (double)b.z, dot);
+            nx = __builtin_fma((double)b.z, (double)b.z, nx);
+            dot = __builtin_fma((double)a.w, (double)b.w, dot);
+            nx = __builtin_fma((double)b.w, (double)b.w, nx);
+        }
+        const double c = dot / (__builtin_sqrt(nq) * __builtin_sqrt(nx));
+        return kind == kNegCos ? -c : 1.0 - c;
+    } else if (kind == kGenJaccard) {
+        double smin = 0, smax = 0;
+#pragma unroll 5
+        for (int j = 0; j < (d >> 2); ++j) {
+            const float4 a4 = q4[j], b4 = x4[j];
+            const float av[4] = {a4.x, a4.y, a4.z, a4.w}, bv[4] = {b4.x, b4.y, b4.z, b4.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const double a = av[e], b = bv[e];
+                smin = smin + ((b < a) ? b : a);
+                smax = smax + ((a < b) ? b : a);
+            }
+        }
+        return 1.0 - smin / smax;
+    } else {
+        double s2 = 0;
+#pragma unroll 5
+        for (int j = 0; j < (d >> 2); ++j) {
+            const float4 a4 = q4[j], b4 = x4[j];
+            const float av[4] = {a4.x, a4.y, a4.z, a4.w}, bv[4] = {b4.x, b4.y, b4.z, b4.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const double t = (double)av[e] - (double)bv[e];
+                s2 = __builtin_fma(t, t, s2);
+            }
+        }
+        return __builtin_sqrt(s2);
+    }
+}
+
+// Squared norm of q in FP64, summed in index order as dist64_dev does.
+__device__ inline double norm2_64(const float* __restrict__ q, int d) {
+    double nq = 0;
+    for (int j = 0; j < d; ++j) nq = __builtin_fma((double)q[j], (double)q[j], nq);
+    return nq;
+}
+
 // FP32 distance as the reference's device code writes it
 // (distanceFunctions.cuh:23-52; kNegCos per optimizedLinearScan.cu:45-57).
 __device__ inline float dist32_dev(int kind, const float* __restrict__ q,

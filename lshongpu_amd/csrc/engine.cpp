@@ -42,6 +42,16 @@ bool force_valu_scan() {
     return on;
 }
 
+// LSHGPU_FULL_PASS=1 makes the threshold pass cover every point instead of
+// the pivot cells (both give identical results; used to compare them).
+bool force_full_pass() {
+    static const bool on = [] {
+        const char* e = std::getenv("LSHGPU_FULL_PASS");
+        return e && e[0] == '1';
+    }();
+    return on;
+}
+
 // Indices of the nonzero entries of a device flag array.
 std::vector<int32_t> flagged(const uint32_t* dflags, int Q, hipStream_t s) {
     std::vector<uint32_t> h((size_t)Q);
@@ -125,6 +135,9 @@ struct SketchSet {
     const uint32_t* ds;
     const uint32_t* fp4;
     int64_t N;
+    const CellIndex* cells = nullptr;  // pivot cells of ds, for the threshold pass
+    int32_t* qorder = nullptr;         // out: queries grouped by nearest cell, when the cells ran
+    bool* ordered = nullptr;           // out: qorder was written
 };
 
 // Rows of a level per row of its sample.
@@ -179,26 +192,44 @@ void sketch_topk(const SketchSet& d, const SketchSet* samples, int nsamples, int
         launch_scatter_rows_u64(sv.keys, sv.stride, K, idx, m, v.keys, v.stride, s);
     };
     if (mfma && mode == kNormal && nsamples > 0 && samples[0].N > 0 && d.fp4 && int_keys) {
-        const int Ks = sample_k(K, sample_step(d.N, samples[0].N));
-        TopKState& sa = sc.tk[depth];
-        sketch_topk(samples[0], samples + 1, nsamples - 1, wpad, kind, qs, Q, Ks, kNormal, sa, sc, depth + 1, s,
-                    pt, launches, mfma_launches, mfma_points);
         st.reset(Q, K, 2 * K + 512, s, true);
-        launch_threshold_above(sa.view(), st.view(), s);
-        pt.begin(PH_SCAN, s);
-        pt.begin(PH_MFMA, s);
-        launch_hamming_mfma(d.fp4, qs, wpad, Q, 0, d.N, st.view(), s);
-        pt.end(PH_MFMA, s);
-        pt.end(PH_SCAN, s);
+        int64_t tiles = 0;
+        bool pruned = false;
+        if (d.cells && !force_full_pass()) {
+            // Threshold from the nearest pivot cells, then the cells the
+            // triangle inequality leaves.
+            pt.begin(PH_SCAN, s);
+            pt.begin(PH_MFMA, s);
+            pruned = cell_search(*d.cells, qs, Q, K, st.view(), sc, depth, s, &tiles, d.qorder);
+            if (d.ordered) *d.ordered = d.qorder != nullptr;
+            pt.end(PH_MFMA, s);
+            pt.end(PH_SCAN, s);
+        }
+        if (!pruned) {
+            // Threshold from the sample, then one pass over all points.
+            const int Ks = sample_k(K, sample_step(d.N, samples[0].N));
+            TopKState& sa = sc.tk[depth];
+            sketch_topk(samples[0], samples + 1, nsamples - 1, wpad, kind, qs, Q, Ks, kNormal, sa, sc, depth + 1, s,
+                        pt, launches, mfma_launches, mfma_points);
+            launch_threshold_above(sa.view(), st.view(), s);
+            pt.begin(PH_SCAN, s);
+            pt.begin(PH_MFMA, s);
+            launch_hamming_mfma(d.fp4, qs, wpad, Q, 0, d.N, st.view(), s);
+            pt.end(PH_MFMA, s);
+            pt.end(PH_SCAN, s);
+        }
         ++launches;
         if (mfma_launches) ++*mfma_launches;
-        if (mfma_points) *mfma_points += d.N;
+        if (mfma_points) *mfma_points += pruned ? tiles * 32 : d.N;
         pt.begin(PH_SELECT, s);
         launch_select_merge(st.view(), s);
         pt.end(PH_SELECT, s);
         uint32_t* flags = sc.get<uint32_t>(depth, 0, (size_t)Q);
         launch_incomplete_flags(st.view(), flags, s);
         const std::vector<int32_t> bad = flagged(flags, Q, s);
+        if (std::getenv("LSHGPU_DEBUG_CELLS"))
+            std::fprintf(stderr, "threshold pass (%s): %zu of %d queries redone\n", pruned ? "cells" : "full",
+                         bad.size(), Q);
         if (!bad.empty()) redo(bad, kCautious);
         return;
     }
@@ -415,6 +446,7 @@ size_t lsh_index::device_bytes() const {
                qkeys.bytes() + qlo.bytes() + qpre.bytes() + qmax.bytes() + q.bytes() + qT.bytes() +
                q_inv.bytes() + qjl.bytes() + qjlT.bytes() + qsk.bytes() + out.bytes() + flags.bytes() + st.bytes();
     for (const SampleLevel& L : samples) b += L.ds.bytes() + L.fp4.bytes();
+    b += cells.bytes() + qorder.bytes() + cdata.bytes() + cpos.bytes();
     for (const Transform& t : key_steps) b += t.mins.bytes();
     for (const Transform& t : sketch_steps) b += t.mins.bytes();
     return b;
@@ -496,6 +528,10 @@ void lsh_index::build_fp4(hipStream_t s) {
     dfp4.alloc(hamming_fp4_words(N, wpad));
     launch_expand_fp4(dsk.get(), N, wpad, dfp4.get(), s);
     build_sample_levels(dsk.get(), N, wpad, candidates, samples, s);
+    if (!samples.empty() && framework == 0) {
+        cells.build(dsk.get(), N, wpad, s);
+        if (cells.built()) cell_rows(cells, data.get(), N, d, cdata, cpos, s);
+    }
 }
 
 // lshFramework.cuh:312-409 runLsh up to the scan: keys (generateHashes with
@@ -669,15 +705,22 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
         const int Ceff = (int)std::max<int64_t>(1, std::min<int64_t>(candidates, N));
         const int K = Ceff <= kMaxIntTopK ? Ceff : pow2_ceil(Ceff);
         LSH_REQUIRE(K <= kMaxTopK, LSH_ERR_UNSUPPORTED, "candidate sets larger than 4096 are not supported");
-        const SketchSet full{dsk.get(), dfp4.get(), N};
+        bool ordered = false;
+        if (cells.built()) qorder.ensure((size_t)Q);
+        const SketchSet full{dsk.get(), dfp4.get(), N, &cells, cells.built() ? qorder.get() : nullptr, &ordered};
         const std::vector<SketchSet> sv = sample_views(samples);
         sketch_topk(full, sv.data(), (int)sv.size(), wpad, impl == 4 ? 1 : 0, qsk.get(), Q, K, kNormal, st,
                     scratch, 0, s, pt, scan_launches,
                     &mfma_launches, &mfma_points);
         const int kind = impl == 3 ? kAngular : kGenJaccard;
         pt.begin(PH_FINAL, s);
-        launch_rerank(data.get(), q.get(), d, kind, st.view().keys, st.view().stride, nullptr, Ceff, Q, kq,
-                      kq + margin, fp32_error_bound(kind, d), out.get(), s);
+        if (cells.built())
+            launch_rerank(cdata.get(), q.get(), d, kind, st.view().keys, st.view().stride, nullptr, Ceff, Q, kq,
+                          kq + margin, fp32_error_bound(kind, d), out.get(), s, ordered ? qorder.get() : nullptr,
+                          cpos.get());
+        else
+            launch_rerank(data.get(), q.get(), d, kind, st.view().keys, st.view().stride, nullptr, Ceff, Q, kq,
+                          kq + margin, fp32_error_bound(kind, d), out.get(), s);
         pt.end(PH_FINAL, s);
     } else {
         qjl.ensure((size_t)Q * sdim);
@@ -889,12 +932,14 @@ void hamming_topc_host(const uint32_t* ds, int64_t N, const uint32_t* qs, int64_
     int64_t launches = 0;
     DevBuf<uint32_t> fp4;
     std::vector<SampleLevel> lv;
+    CellIndex cells;
     if (hamming_mfma_supported(wpad) && N >= 16 * 1024) {
         fp4.alloc(hamming_fp4_words(N, wpad));
         launch_expand_fp4(dpad.get(), N, wpad, fp4.get(), nullptr);
         build_sample_levels(dpad.get(), N, wpad, C, lv, nullptr);
+        if (!lv.empty()) cells.build(dpad.get(), N, wpad, nullptr);
     }
-    const SketchSet full{dpad.get(), fp4.get(), N};
+    const SketchSet full{dpad.get(), fp4.get(), N, &cells, nullptr, nullptr};
     const std::vector<SketchSet> sv = sample_views(lv);
     Scratch sc;
     sketch_topk(full, sv.data(), (int)sv.size(), wpad, 0, qpad.get(), (int)Q, K, kNormal, st, sc, 0, nullptr, pt,

@@ -71,6 +71,45 @@ struct Scratch {
     }
 };
 
+// Pivot cells of a packed Hamming sketch array (cells.hip): the data in
+// 32-point tiles grouped by nearest pivot and sorted by distance to it, with
+// each tile's distance range, so that a pass can skip the tiles the triangle
+// inequality rules out.  Built for kCellMinPoints points or more.
+constexpr int64_t kCellMinPoints = 65536;
+int cell_pivots_for(int64_t N);
+
+struct CellIndex {
+    int P = 0;       // pivots; 0 when no cells were built
+    int words = 0;   // sketch words per row (8, 16, 24 or 32)
+    int64_t ntiles = 0;
+    DevBuf<uint32_t> pfp4;         // FP4 image of the pivot sketches
+    DevBuf<uint32_t> sk;           // [ntiles*32][W] packed sketches in cell order, even words then odd
+    DevBuf<int32_t> ids;           // [ntiles*32] data row of each position, -1 for padding
+    DevBuf<uint32_t> tinfo;        // per tile: min | max << 11 | rows << 22 (distances to the pivot)
+    DevBuf<uint32_t> ctile;        // [P+1] first tile of each cell
+    DevBuf<uint32_t> csize;        // [P] points of each cell
+    DevBuf<uint16_t> cmin, cmax;   // per cell: distance range to the pivot (min > max: empty)
+    void build(const uint32_t* ds, int64_t N, int wpad, hipStream_t s);
+    size_t bytes() const;
+    bool built() const { return P > 0; }
+};
+
+// The float rows [N][d] in the cells' order (rows: [ntiles*32][d], zero
+// for padding) and pos[id] = the position of data row id.
+void cell_rows(const CellIndex& c, const float* data, int64_t N, int d, DevBuf<float>& rows, DevBuf<int32_t>& pos,
+               hipStream_t s);
+
+// Threshold search over the cells (Q query rows of qs, top-K): sets v.thr
+// to a valid threshold from each query's nearest cells, then appends every
+// pair below it to v, as launch_hamming_mfma over all points would.  Returns
+// false, having appended nothing (v.thr stays valid), when the thresholds
+// leave too many tiles for pruning to pay.  Uses scratch slots 0-3 of
+// `depth`.  tiles_out (may be null) accumulates the tiles multiplied, in
+// 32-query units.  qorder (may be null; Q entries) receives the queries
+// grouped by their nearest cell.
+bool cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K, const TopKView& v, Scratch& sc, int depth,
+                 hipStream_t s, int64_t* tiles_out, int32_t* qorder);
+
 }  // namespace lshgpu
 
 struct lsh_index {
@@ -103,6 +142,9 @@ struct lsh_index {
     lshgpu::DevBuf<uint32_t> dsk;    // [N][wpad]
     lshgpu::DevBuf<uint32_t> dfp4;   // FP4 operand image of dsk for the matrix-core Hamming scan
     std::vector<lshgpu::SampleLevel> samples;  // strided samples of dsk (threshold estimates)
+    lshgpu::CellIndex cells;         // pivot cells of dsk (pruned threshold pass)
+    lshgpu::DevBuf<float> cdata;     // data rows in cell order (rerank reads)
+    lshgpu::DevBuf<int32_t> cpos;    // position of each data row in cdata
     lshgpu::DevBuf<float> planesT;   // SimHash [words][d][32], JL [groups][d][32]
     int sdim = 0;                    // JL sketch length
     lshgpu::DevBuf<float> djl;       // [N][sdim]
@@ -127,6 +169,7 @@ struct lsh_index {
     lshgpu::DevBuf<uint32_t> qsk;
     lshgpu::DevBuf<lsh_point_t> out;
     lshgpu::DevBuf<uint32_t> flags;
+    lshgpu::DevBuf<int32_t> qorder;  // queries grouped by nearest cell (rerank order)
     lshgpu::TopKState st;
     lshgpu::Scratch scratch;
     int64_t scan_launches = 0;

@@ -244,9 +244,13 @@ void launch_exact_finalize(const float* data, const float* queries, int d, int k
 // rerank.hip: candidates = low 32 bits of the first C keys of each query
 // (or `cand` ids when non-null, [Q][C]); exact FP32 distances, best K' kept,
 // FP64 recheck (all C in FP64 when the margin fails), k written.
+// qorder (may be null): a permutation of the queries, taken in that order
+// per XCD so that queries with shared candidates run close together.
+// rowmap (may be null): candidate ID i is row rowmap[i] of data.
 void launch_rerank(const float* data, const float* queries, int d, int kind, const uint64_t* keys,
                    int key_stride, const int32_t* cand, int C, int Q, int k, int kprime, double eps,
-                   lsh_point_t* out, hipStream_t s);
+                   lsh_point_t* out, hipStream_t s, const int32_t* qorder = nullptr,
+                   const int32_t* rowmap = nullptr);
 
 // sketch.hip (out rows have `out_stride` words; planesT is [words][d][hashBits])
 void launch_simhash(const float* x, int64_t n, int d, const float* planesT, int words,

@@ -15,25 +15,142 @@ namespace lshgpu {
 
 namespace {
 
+__device__ inline uint64_t shfl_xor_u64(uint64_t x, int m) {
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)x, m);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(x >> 32), m);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Ascending bitonic sort of one value per lane across a wave.
+__device__ inline uint64_t wave_sort_u64(uint64_t x) {
+    const int lane = threadIdx.x & 63;
+    for (int size = 2; size <= 64; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            const uint64_t y = shfl_xor_u64(x, stride);
+            const bool up = (lane & size) == 0, lower = (lane & stride) == 0;
+            x = (lower == up) ? (x < y ? x : y) : (x < y ? y : x);
+        }
+    return x;
+}
+
+// The kk smallest of the n (a power of two, distinct or kEmptyKey) keys, in
+// ascending order, into out[0..kk): a radix select of the kk-th smallest
+// distance key (four 8-bit digits of the high word), every key at or below
+// it collected, and those sorted (in registers when they fit a wave).
+// Returns the number collected (>= kk).  out has room for n keys.
+__device__ int select_smallest(const uint64_t* keys, int n, int kk, uint64_t* out, uint32_t* hist,
+                               uint32_t* sv) {
+    const int tid = threadIdx.x, bs = blockDim.x;
+    uint32_t prefix = 0, mask = 0, rank = (uint32_t)kk;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        for (int i = tid; i < 256; i += bs) hist[i] = 0;
+        __syncthreads();
+        for (int t = tid; t < n; t += bs) {
+            const uint32_t dk = (uint32_t)(keys[t] >> 32);
+            if ((dk & mask) == prefix) atomicAdd(&hist[(dk >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (tid < 64) {
+            uint32_t c[4], sum = 0;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                c[e] = hist[4 * tid + e];
+                sum += c[e];
+            }
+            uint32_t inc = sum;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t x = __shfl_up(inc, o);
+                if (tid >= o) inc += x;
+            }
+            uint32_t run = inc - sum;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                if (run < rank && run + c[e] >= rank) {
+                    sv[0] = 4u * tid + e;
+                    sv[1] = rank - run;
+                }
+                run += c[e];
+            }
+        }
+        __syncthreads();
+        prefix |= sv[0] << shift;
+        mask |= 0xFFu << shift;
+        rank = sv[1];
+        __syncthreads();
+    }
+    if (tid == 0) sv[2] = 0;
+    __syncthreads();
+    for (int t = tid; t < n; t += bs) {
+        const uint64_t key = keys[t];
+        if ((uint32_t)(key >> 32) <= prefix) out[atomicAdd(&sv[2], 1u)] = key;
+    }
+    __syncthreads();
+    const int m = (int)sv[2];
+    if (m <= 64) {
+        if (tid < 64) {
+            const uint64_t x = wave_sort_u64(tid < m ? out[tid] : kEmptyKey);
+            if (tid < n) out[tid] = x;
+        }
+    } else {
+        int m2 = 1;
+        while (m2 < m) m2 <<= 1;
+        for (int t = m + tid; t < m2; t += bs) out[t] = kEmptyKey;
+        bitonic_sort_lds(out, m2);
+    }
+    __syncthreads();
+    return m;
+}
+
 __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ data,
                                                      const float* __restrict__ queries, int d,
                                                      int kind, const uint64_t* __restrict__ keys,
                                                      int key_stride, const int32_t* __restrict__ cand,
                                                      int C, int Cp, int k, int kprime, double eps,
-                                                     lsh_point_t* __restrict__ out) {
+                                                     lsh_point_t* __restrict__ out,
+                                                     const int32_t* __restrict__ qorder, int Q, int per,
+                                                     const int32_t* __restrict__ rowmap) {
     extern __shared__ uint64_t sm[];
     uint64_t* k32 = sm;                                        // Cp
     uint64_t* k64 = sm + Cp;                                   // Cp
     int32_t* ids = reinterpret_cast<int32_t*>(sm + 2 * Cp);    // Cp
+    int32_t* rows = ids + Cp;                                  // Cp: data row of each candidate
     __shared__ int s_ok;
-    const int q = blockIdx.x;
+    __shared__ uint32_t rs_hist[256];
+    __shared__ uint32_t rs_sv[4];
+    // With an order, XCD x (workgroups x, x + 8, ...) takes the x-th eighth
+    // of it in sequence.
+    int q = blockIdx.x;
+    if (qorder) {
+        const int pos = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+        if (pos >= Q) return;
+        q = qorder[pos];
+    }
     const float* __restrict__ qv = queries + (size_t)q * d;
 
     float qn = 0.0f;
+    double nq64 = 0.0;
     if (kind == kNegCos || kind == kAngular) {
         for (int j = 0; j < d; ++j) qn = __builtin_fmaf(qv[j], qv[j], qn);
         qn = __builtin_sqrtf(qn);
+        nq64 = norm2_64(qv, d);
     }
+    // Candidate IDs first (to LDS), so the row reads below do not wait on
+    // the key reads.
+    for (int t = threadIdx.x; t < Cp; t += blockDim.x) {
+        int32_t id = -1;
+        if (t < C) {
+            if (cand) {
+                id = cand[(size_t)q * C + t];
+            } else {
+                const uint64_t key = keys[(size_t)q * key_stride + t];
+                id = key == kEmptyKey ? -1 : (int32_t)(uint32_t)key;
+            }
+        }
+        ids[t] = id;
+        rows[t] = id < 0 ? -1 : (rowmap ? rowmap[id] : id);
+    }
+    __syncthreads();
     // FP32 distances: a group of 16 lanes reads one candidate row with
     // coalesced float4 loads and reduces its partial sums with shuffles.
     const int gl = threadIdx.x & 15;
@@ -42,20 +159,13 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
     // Each group takes U rows per step so that U row reads are in flight.
     constexpr int U = 4;
     for (int t0 = threadIdx.x >> 4; t0 < Cp; t0 += ngroups * U) {
-        int32_t id[U];
+        int32_t id[U], row[U];
         float a[U], b[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int t = t0 + u * ngroups;
-            id[u] = -1;
-            if (t < C) {
-                if (cand) {
-                    id[u] = cand[(size_t)q * C + t];
-                } else {
-                    const uint64_t key = keys[(size_t)q * key_stride + t];
-                    id[u] = key == kEmptyKey ? -1 : (int32_t)(uint32_t)key;
-                }
-            }
+            id[u] = t < Cp ? ids[t] : -1;
+            row[u] = t < Cp ? rows[t] : -1;
             a[u] = 0.0f;
             b[u] = 0.0f;
         }
@@ -65,7 +175,7 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
                 float4 xv[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u)
-                    xv[u] = id[u] >= 0 ? *reinterpret_cast<const float4*>(data + (size_t)id[u] * d + j)
+                    xv[u] = id[u] >= 0 ? *reinterpret_cast<const float4*>(data + (size_t)row[u] * d + j)
                                        : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
@@ -80,7 +190,7 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
                 const float y = qv[j];
                 float xs[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u) xs[u] = id[u] >= 0 ? data[(size_t)id[u] * d + j] : 0.0f;
+                for (int u = 0; u < U; ++u) xs[u] = id[u] >= 0 ? data[(size_t)row[u] * d + j] : 0.0f;
 #pragma unroll
                 for (int u = 0; u < U; ++u) group_accumulate(kind, xs[u], y, a[u], b[u]);
             }
@@ -99,24 +209,13 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
             }
         }
     }
-    bitonic_sort_lds(k32, Cp);
-
+    // The kp best by FP32 distance, sorted, in k64 (k32 keeps every key).
     const int kp = min(kprime, Cp);
-    for (int t = threadIdx.x; t < kp; t += blockDim.x) {
-        const uint64_t key = k32[t];
-        if (key == kEmptyKey) {
-            k64[t] = kEmptyKey;
-            ids[t] = 0x7FFFFFFF;
-        } else {
-            const int32_t id = (int32_t)(uint32_t)key;
-            k64[t] = f64_key(dist64_dev(kind, qv, data + (size_t)id * d, d));
-            ids[t] = id;
-        }
-    }
+    select_smallest(k32, Cp, kp, k64, rs_hist, rs_sv);
     if (threadIdx.x == 0) {
         bool ok = true;
         if (kp < Cp) {
-            const uint64_t kk = k32[k - 1], kK = k32[kp - 1];
+            const uint64_t kk = k64[k - 1], kK = k64[kp - 1];
             if (kK != kEmptyKey && kk != kEmptyKey) {
                 const double vk = key_f32((uint32_t)(kk >> 32));
                 const double vK = key_f32((uint32_t)(kK >> 32));
@@ -130,16 +229,28 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
     }
     __syncthreads();
     int n = kp;
-    if (!s_ok) {
+    if (s_ok) {
+        // FP64 for the kp best, in place.
+        for (int t = threadIdx.x; t < kp; t += blockDim.x) {
+            const uint64_t key = k64[t];
+            if (key == kEmptyKey) {
+                ids[t] = 0x7FFFFFFF;
+            } else {
+                const int32_t id = (int32_t)(uint32_t)key;
+                k64[t] = f64_key(dist64_row(kind, qv, data + (size_t)(rowmap ? rowmap[id] : id) * d, d, nq64));
+                ids[t] = id;
+            }
+        }
+    } else {
         // FP64 for every candidate.
-        for (int t = threadIdx.x + kp; t < Cp; t += blockDim.x) {
+        for (int t = threadIdx.x; t < Cp; t += blockDim.x) {
             const uint64_t key = k32[t];
             if (key == kEmptyKey) {
                 k64[t] = kEmptyKey;
                 ids[t] = 0x7FFFFFFF;
             } else {
                 const int32_t id = (int32_t)(uint32_t)key;
-                k64[t] = f64_key(dist64_dev(kind, qv, data + (size_t)id * d, d));
+                k64[t] = f64_key(dist64_row(kind, qv, data + (size_t)(rowmap ? rowmap[id] : id) * d, d, nq64));
                 ids[t] = id;
             }
         }
@@ -164,15 +275,16 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
 
 void launch_rerank(const float* data, const float* queries, int d, int kind, const uint64_t* keys,
                    int key_stride, const int32_t* cand, int C, int Q, int k, int kprime, double eps,
-                   lsh_point_t* out, hipStream_t s) {
+                   lsh_point_t* out, hipStream_t s, const int32_t* qorder, const int32_t* rowmap) {
     if (Q <= 0) return;
     const int Cp = pow2_ceil(std::max(C, 2));
     LSH_REQUIRE(Cp <= kMaxTopK, LSH_ERR_INVALID, "candidate set larger than 4096");
     const int kp = pow2_ceil(std::max(kprime, k));
-    const size_t lds = (size_t)Cp * (2 * sizeof(uint64_t) + sizeof(int32_t));
+    const size_t lds = (size_t)Cp * (2 * sizeof(uint64_t) + 2 * sizeof(int32_t));
     set_dynamic_lds((const void*)rerank_kernel, lds);
-    hipLaunchKernelGGL(rerank_kernel, dim3(Q), dim3(256), lds, s, data, queries, d, kind, keys, key_stride,
-                       cand, C, Cp, k, kp, eps, out);
+    const int per = (Q + 7) / 8;
+    hipLaunchKernelGGL(rerank_kernel, dim3(qorder ? 8 * per : Q), dim3(256), lds, s, data, queries, d, kind, keys,
+                       key_stride, cand, C, Cp, k, kp, eps, out, qorder, Q, per, rowmap);
     check_launch("rerank_kernel");
 }
 

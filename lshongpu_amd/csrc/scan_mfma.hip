@@ -32,36 +32,17 @@
 // to the wave's LDS queue, which is flushed to the global top-K state with
 // one atomic per query (or, when full, spills to the global slots).  Ranges
 // whose candidates are still dense (the first rounds) use the VALU scan.
-#include "device_common.hpp"
+#include "fp4.hpp"
 
 namespace lshgpu {
 
 namespace {
 
-typedef int v4i __attribute__((ext_vector_type(4)));
-typedef int v8i __attribute__((ext_vector_type(8)));
-typedef float v16f __attribute__((ext_vector_type(16)));
-
-// 8 bits -> 8 FP4 nibbles: bit i -> nibble i = 0x2 (+1.0) or 0xA (-1.0).
-__device__ inline uint32_t spread8(uint32_t x) {
-    x &= 0xFFu;
-    x = (x | (x << 12)) & 0x000F000Fu;
-    x = (x | (x << 6)) & 0x03030303u;
-    x = (x | (x << 3)) & 0x11111111u;
-    return (x << 3) | 0x22222222u;
-}
-
-__device__ inline v4i expand_word(uint32_t w) {
-    v4i r;
-    r[0] = (int)spread8(w);
-    r[1] = (int)spread8(w >> 8);
-    r[2] = (int)spread8(w >> 16);
-    r[3] = (int)spread8(w >> 24);
-    return r;
-}
-
-// FP4 operands occupy the low four registers; the upper half is not read.
-__device__ inline v8i widen(v4i a) { return __builtin_shufflevector(a, a, 0, 1, 2, 3, -1, -1, -1, -1); }
+using fp4::expand_word;
+using fp4::v16f;
+using fp4::v4i;
+using fp4::v8i;
+using fp4::widen;
 
 constexpr int MQ = 1;                      // query tiles per wave
 constexpr int WAVES = 4;

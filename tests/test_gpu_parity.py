@@ -227,6 +227,55 @@ def test_hamming_topc_two_sample_levels(lsh, orc):
     np.testing.assert_array_equal(dist, odist)
 
 
+def _clustered_sketches(rng, n, words, centres, flip):
+    """Rows near `centres` (each bit flipped with probability `flip`)."""
+    c = centres[rng.integers(0, centres.shape[0], n)]
+    bits = np.unpackbits(c.view(np.uint8), axis=1)
+    bits ^= (rng.random(bits.shape) < flip).astype(np.uint8)
+    return np.packbits(bits, axis=1).view(np.uint32).reshape(n, words)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("words,C", [(32, 256), (32, 64), (8, 100)])
+def test_hamming_topc_pivot_cells(lsh, orc, words, C):
+    # Clustered sketches, 150k points: the threshold pass runs over the pivot
+    # cells and skips the tiles the triangle inequality rules out.  Some
+    # queries are far from every cluster (wide thresholds), duplicates make
+    # ties that cross cells.
+    rng = np.random.default_rng(81 + words + C)
+    centres = rng.integers(0, 2 ** 32, size=(300, words), dtype=np.uint64).astype(np.uint32)
+    ds = _clustered_sketches(rng, 150000, words, centres, 0.08)
+    dup = rng.integers(0, 150000, 3000)
+    ds[rng.integers(0, 150000, 3000)] = ds[dup]
+    qs = _clustered_sketches(rng, 300, words, centres, 0.08)
+    qs[:20] = rng.integers(0, 2 ** 32, size=(20, words), dtype=np.uint64).astype(np.uint32)
+    qs[20:30] = ds[dup[:10]]
+    ids, dist = lsh.hamming_topc(ds, qs, C)
+    oids, odist = orc.sketch_topc(0, ds, qs, C)
+    np.testing.assert_array_equal(ids, oids)
+    np.testing.assert_array_equal(dist, odist)
+
+
+@pytest.mark.gpu
+def test_simhash_index_with_pivot_cells_matches_oracle(lsh, orc):
+    # The resident index at 120k clustered points (pivot cells built) against
+    # the oracle pipeline: exact Hamming top-C, then the exact rerank.
+    rng = np.random.default_rng(82)
+    N, Q, d, words, C, k = 120000, 200, 48, 16, 256, 32
+    centres = rng.standard_normal((400, d)).astype(np.float32)
+    data = centres[rng.integers(0, 400, N)] + 0.3 * rng.standard_normal((N, d)).astype(np.float32)
+    queries = centres[rng.integers(0, 400, Q)] + 0.3 * rng.standard_normal((Q, d)).astype(np.float32)
+    data /= np.linalg.norm(data, axis=1, keepdims=True)
+    queries /= np.linalg.norm(queries, axis=1, keepdims=True)
+    with lsh.Index(data, implementation=3, sketchDim=words, distanceFunc=1, candidates=C) as ix:
+        res = ix.search(queries, k)
+    planes = orc.normal_stream(words * 32 * d)
+    cand, _ = orc.sketch_topc(0, orc.simhash(data, planes, words), orc.simhash(queries, planes, words), C)
+    oids, odist = orc.rerank(data, queries, cand, k, 1)
+    np.testing.assert_array_equal(res["ID"], oids)
+    _close(res["distance"], odist)
+
+
 # ------------------------------------------------------------ full pipelines
 
 def test_simhash_pipeline_matches_oracle(lsh, orc):
