@@ -313,8 +313,8 @@ int lsh_index_search(lsh_index_t* index, const float* queries, int N_query,
 /* Per-phase device times (ms) of the last search: [0] query preparation and
  * sketch, [1] candidate scans (all rounds), [2] candidate selection, [3]
  * rerank / finalisation, [4] total, [5] the matrix-core Hamming scan
- * launches among [1]; [6] the number of those launches and [7] the data
- * points they covered (summed over launches).  Filled only while profiling
+ * launches among [1]; [6] the number of those launches and [7] the (query,
+ * point) pairs they multiplied per query (summed over launches).  Filled only while profiling
  * is on. */
 int lsh_index_set_profiling(lsh_index_t* index, int on);
 int lsh_index_phase_times(const lsh_index_t* index, double* out8);

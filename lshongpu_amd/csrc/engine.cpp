@@ -220,7 +220,8 @@ void sketch_topk(const SketchSet& d, const SketchSet* samples, int nsamples, int
         }
         ++launches;
         if (mfma_launches) ++*mfma_launches;
-        if (mfma_points) *mfma_points += pruned ? tiles * 32 : d.N;
+        // Pairs multiplied per query: the cells' tile products cover 32 x 32 pairs.
+        if (mfma_points) *mfma_points += pruned ? tiles * 1024 / std::max(Q, 1) : d.N;
         pt.begin(PH_SELECT, s);
         launch_select_merge(st.view(), s);
         pt.end(PH_SELECT, s);
