@@ -262,53 +262,64 @@ __device__ int kth_bin(const uint32_t* hist, int nbins, uint32_t K) {
     return nbins;
 }
 
-// Threshold of each query (one workgroup per query): the radius R at which
-// the cells nearest to the query hold K points (kept for the first pass),
-// then the K-th smallest
-// upper bound D + hi_t (weighted by the tile's rows) over the tiles of the
-// cells within R + kRadiusMargin; thr = (bound + 1, 0).
+// Threshold of each query (one workgroup per query):
+//  - its nearest non-empty cell (kept for the query order);
+//  - the radius R at which the nearest cells hold K points, from a histogram
+//    of cell sizes over the 256 distances above the nearest cell's (all
+//    cells when that window holds fewer than K points); the cells within R
+//    are counted into cnt1 for the first pass;
+//  - the K-th smallest upper bound D + hi_t (weighted by the tile's rows)
+//    over the tiles of the cells within R + kRadiusMargin: thr = (bound + 1,
+//    0).
 constexpr int kRadiusMargin = 16;
+constexpr int kWindow = 256;
 __global__ __launch_bounds__(256) void cell_threshold_kernel(const uint16_t* __restrict__ D, int Q, int P, int Pp,
                                                              const uint32_t* __restrict__ csize,
                                                              const uint32_t* __restrict__ ctile,
                                                              const uint32_t* __restrict__ tinfo, int bits, int K,
                                                              uint64_t* __restrict__ thr, uint16_t* __restrict__ R,
-                                                             uint32_t* __restrict__ nearest) {
-    __shared__ uint32_t h1[1025];
+                                                             uint32_t* __restrict__ nearest,
+                                                             uint32_t* __restrict__ cnt1) {
+    __shared__ uint32_t h1[kWindow];
     __shared__ uint32_t h2[2049];
     __shared__ int rad;
     __shared__ uint32_t best;
     const int q = blockIdx.x;
     const uint16_t* __restrict__ row = D + (size_t)q * Pp;
     for (int b = threadIdx.x; b < 2049; b += 256) {
-        if (b < 1025) h1[b] = 0;
+        if (b < kWindow) h1[b] = 0;
         h2[b] = 0;
     }
     if (threadIdx.x == 0) best = 0xFFFFFFFFu;
     __syncthreads();
     uint32_t mine = 0xFFFFFFFFu;  // (distance, pivot) of the nearest non-empty cell
-    for (int j = threadIdx.x; j < P; j += 256) {
-        const uint32_t n = csize[j];
-        if (n) {
-            atomicAdd(&h1[row[j]], n);
-            mine = min(mine, ((uint32_t)row[j] << 13) | (uint32_t)j);
-        }
-    }
+    for (int j = threadIdx.x; j < P; j += 256)
+        if (csize[j]) mine = min(mine, ((uint32_t)row[j] << 13) | (uint32_t)j);
     atomicMin(&best, mine);
     __syncthreads();
+    const uint32_t dmin = best == 0xFFFFFFFFu ? 0u : best >> 13;
     if (threadIdx.x == 0) nearest[q] = best == 0xFFFFFFFFu ? 0u : (best & 0x1FFFu);
+    for (int j = threadIdx.x; j < P; j += 256) {
+        const uint32_t n = csize[j];
+        const uint32_t dd = (uint32_t)row[j] - dmin;
+        if (n && dd < (uint32_t)kWindow) atomicAdd(&h1[dd], n);
+    }
+    __syncthreads();
     if (threadIdx.x < 64) {
-        const int r = kth_bin(h1, bits + 1, (uint32_t)K);
+        const int r = kth_bin(h1, kWindow, (uint32_t)K);
         if (threadIdx.x == 0) {
-            rad = r + kRadiusMargin;
-            R[q] = (uint16_t)min(r, bits);
+            const int rr = r < kWindow ? (int)dmin + r : bits;
+            rad = rr + kRadiusMargin;
+            R[q] = (uint16_t)min(rr, bits);
         }
     }
     __syncthreads();
     const int rm = rad;
+    const int r0 = rm - kRadiusMargin;
     for (int j = threadIdx.x; j < P; j += 256) {
         const int dq = row[j];
         if (dq > rm || csize[j] == 0) continue;
+        if (dq <= r0) atomicAdd(&cnt1[j], 1u);
         for (uint32_t t = ctile[j]; t < ctile[j + 1]; ++t) {
             const uint32_t info = tinfo[t];
             atomicAdd(&h2[dq + ((info >> kKeyShift) & kDistMask)], info >> 22);
@@ -343,18 +354,33 @@ __global__ void cell_need_kernel(const uint16_t* __restrict__ D, int Q, int P, i
     }
 }
 
-// coff = exclusive prefix of cnt, choff = exclusive prefix of ceil(cnt/32),
-// both with a final total entry; cursor (= cnt for the placing pass) zeroed.
+// A chunk is up to 32 queries of one cell over up to kChunkTiles of its
+// tiles, so that no wave walks a long cell alone.
+constexpr int kChunkTiles = 8;
+__device__ inline uint32_t cell_parts(const uint32_t* ctile, int j) {
+    return (ctile[j + 1] - ctile[j] + kChunkTiles - 1) / kChunkTiles;
+}
+
+// coff = exclusive prefix of cnt, choff = exclusive prefix of the cells'
+// chunks (ceil(cnt/32) query groups x cell_parts tile ranges), both with a
+// final total entry; choff[P+1] = the tile products those chunks can make
+// (query groups x tiles, summed); cursor (= cnt for the placing pass) zeroed.
 __global__ __launch_bounds__(1024) void cell_offsets_kernel(uint32_t* __restrict__ cnt, int P,
+                                                            const uint32_t* __restrict__ ctile,
                                                             uint32_t* __restrict__ coff,
                                                             uint32_t* __restrict__ choff) {
-    __shared__ uint32_t ws[2][16];
+    __shared__ uint32_t ws[3][16];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    uint32_t carry_a = 0, carry_b = 0;
+    uint32_t carry_a = 0, carry_b = 0, carry_w = 0;
     for (int c0 = 0; c0 < P; c0 += 1024) {
         const int j = c0 + threadIdx.x;
         const uint32_t a = j < P ? cnt[j] : 0u;
-        const uint32_t b = (a + 31) / 32;
+        const uint32_t groups = (a + 31) / 32;
+        const uint32_t b = groups * (j < P ? cell_parts(ctile, j) : 0u);
+        uint32_t wk = groups ? groups * (ctile[j + 1] - ctile[j]) : 0u;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) wk += __shfl_xor(wk, o);
+        if (lane == 0) ws[2][w] = wk;
         uint32_t ia = a, ib = b;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
@@ -369,13 +395,14 @@ __global__ __launch_bounds__(1024) void cell_offsets_kernel(uint32_t* __restrict
             ws[1][w] = ib;
         }
         __syncthreads();
-        uint32_t pa = 0, pb = 0, ta = 0, tb = 0;
+        uint32_t pa = 0, pb = 0, ta = 0, tb = 0, tw = 0;
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
             pa += q < w ? ws[0][q] : 0u;
             pb += q < w ? ws[1][q] : 0u;
             ta += ws[0][q];
             tb += ws[1][q];
+            tw += ws[2][q];
         }
         if (j < P) {
             coff[j] = carry_a + pa + ia - a;
@@ -384,11 +411,13 @@ __global__ __launch_bounds__(1024) void cell_offsets_kernel(uint32_t* __restrict
         }
         carry_a += ta;
         carry_b += tb;
+        carry_w += tw;
         __syncthreads();
     }
     if (threadIdx.x == 0) {
         coff[P] = carry_a;
         choff[P] = carry_b;
+        choff[P + 1] = carry_w;
     }
 }
 
@@ -429,7 +458,9 @@ __global__ __launch_bounds__(256) void cell_scan_kernel(const uint32_t* __restri
     const uint32_t c = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (c >= nchunks) return;
     const uint32_t j = cmap[c];
-    const uint32_t q0 = coff[j] + 32 * (c - choff[j]);
+    const uint32_t parts = cell_parts(ctile, j);
+    const uint32_t local = c - choff[j];
+    const uint32_t q0 = coff[j] + 32 * (local / parts);
     const uint32_t m = min(32u, coff[j + 1] - q0);
     const int col = lane & 31;
     const int q = (uint32_t)col < m ? list[q0 + col] : -1;
@@ -466,7 +497,8 @@ __global__ __launch_bounds__(256) void cell_scan_kernel(const uint32_t* __restri
     v4i b[KS];
 #pragma unroll
     for (int s = 0; s < KS; ++s) b[s] = expand_word(q >= 0 ? qs[(size_t)q * W + 2 * s + (lane >> 5)] : 0u);
-    const uint32_t t0 = ctile[j], nt = ctile[j + 1] - t0;
+    const uint32_t t0 = ctile[j] + (local % parts) * kChunkTiles;
+    const uint32_t nt = min((uint32_t)kChunkTiles, ctile[j + 1] - t0);
     // Lane l's half of point row l%32 of tile t: W/2 words.
     auto load = [&](uint32_t t, uint4* x) {
         const uint4* src = reinterpret_cast<const uint4*>(sk + ((size_t)t * 32 + col) * W + (lane >> 5) * KS);
@@ -549,17 +581,31 @@ __global__ __launch_bounds__(256) void cell_scan_kernel(const uint32_t* __restri
 }
 
 // thr[q] = min(thr[q], (T + 1, 0)) with T the upper edge of the bin at which
-// the query's first-pass counts reach K (the last bin's edge: bits).  One
-// wave per query.
+// the query's first-pass counts reach K (the last bin's edge: bits); then the
+// cells whose bound is below the final threshold are counted into cnt2 for
+// the second pass.  One wave per query.
 __global__ __launch_bounds__(256) void hist_threshold_kernel(const uint32_t* __restrict__ hist, int Q, int bits,
-                                                             int K, uint64_t* __restrict__ thr) {
+                                                             int K, uint64_t* __restrict__ thr,
+                                                             const uint16_t* __restrict__ D, int P, int Pp,
+                                                             const uint16_t* __restrict__ cmin,
+                                                             const uint16_t* __restrict__ cmax,
+                                                             uint32_t* __restrict__ cnt2) {
     const int q = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (q >= Q) return;
+    const int lane = threadIdx.x & 63;
     const int b = kth_bin(hist + (size_t)q * kHistBins, kHistBins, (uint32_t)K);
-    if ((threadIdx.x & 63) == 0 && b < kHistBins) {
+    uint64_t t = thr[q];
+    if (b < kHistBins) {
         const uint32_t T = b == kHistBins - 1 ? (uint32_t)bits : (uint32_t)(8 * b + 7);
-        const uint64_t t = make_key(T + 1u, 0u);
-        if (t < thr[q]) thr[q] = t;
+        const uint64_t th = make_key(T + 1u, 0u);
+        t = th < t ? th : t;
+    }
+    if (lane == 0) thr[q] = t;
+    const uint32_t h = (uint32_t)(t >> 32);
+    const uint16_t* __restrict__ row = D + (size_t)q * Pp;
+    for (int j = lane; j < P; j += 64) {
+        const uint32_t lo = cmin[j], hi = cmax[j];
+        if (lo <= hi && lower_bound_to(row[j], lo, hi) < h) atomicAdd(&cnt2[j], 1u);
     }
 }
 
@@ -626,25 +672,20 @@ void cell_scan(const CellIndex& c, unsigned chunks, const uint32_t* coff, const 
 }
 
 struct Lists {
-    uint32_t entries = 0, chunks = 0;
+    uint32_t entries = 0, chunks = 0, work = 0;  // work: tile products (32-query units) at most
 };
 
-// Counts the per-cell query lists (R: first-pass radii, or nullptr for the
-// bounds of thr) and their chunking; one host synchronisation.
-Lists count_lists(const CellIndex& c, const uint16_t* D, int Q, int Pp, const uint64_t* thr, const uint16_t* R,
-                  uint32_t* cnt, uint32_t* coff, uint32_t* choff, hipStream_t s) {
+// Offsets and chunking of per-cell query lists already counted in cnt;
+// one host synchronisation for their sizes.
+Lists list_offsets(const CellIndex& c, uint32_t* cnt, uint32_t* coff, uint32_t* choff, hipStream_t s) {
     const int P = c.P;
-    LSH_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t) * (P + 1), s));
-    hipLaunchKernelGGL(cell_need_kernel, need_grid(P, Q), dim3(256), 0, s, D, Q, P, Pp, c.cmin.get(), c.cmax.get(),
-                       thr, R, cnt, nullptr, nullptr);
-    check_launch("cell_need_kernel");
-    hipLaunchKernelGGL(cell_offsets_kernel, dim3(1), dim3(1024), 0, s, cnt, P, coff, choff);
+    hipLaunchKernelGGL(cell_offsets_kernel, dim3(1), dim3(1024), 0, s, cnt, P, c.ctile.get(), coff, choff);
     check_launch("cell_offsets_kernel");
-    uint32_t tot[2] = {0, 0};
+    uint32_t tot[3] = {0, 0, 0};
     LSH_HIP(hipMemcpyAsync(&tot[0], coff + P, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    LSH_HIP(hipMemcpyAsync(&tot[1], choff + P, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    LSH_HIP(hipMemcpyAsync(&tot[1], choff + P, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     LSH_HIP(hipStreamSynchronize(s));
-    return Lists{tot[0], tot[1]};
+    return Lists{tot[0], tot[1], tot[2]};
 }
 
 void place_lists(const CellIndex& c, const uint16_t* D, int Q, int Pp, const uint64_t* thr, const uint16_t* R,
@@ -656,11 +697,11 @@ void place_lists(const CellIndex& c, const uint16_t* D, int Q, int Pp, const uin
 
 }  // namespace
 
+// About kPointsPerPivot points per cell before the outlier rounds.
+constexpr int64_t kPointsPerPivot = 768;
 int cell_pivots_for(int64_t N) {
     if (N < kCellMinPoints) return 0;
-    int P = 64;
-    while (P < 8192 && (int64_t)P * 2 * 256 <= N) P *= 2;
-    return P;
+    return (int)std::min<int64_t>(kMaxPivots * 7 / 8, std::max<int64_t>(64, N / kPointsPerPivot));
 }
 
 void CellIndex::build(const uint32_t* ds, int64_t N, int wpad, hipStream_t s) {
@@ -672,8 +713,9 @@ void CellIndex::build(const uint32_t* ds, int64_t N, int wpad, hipStream_t s) {
     // Pivots: every (N/np)-th row.  A group of points that drew no pivot of
     // its own ends up far from the pivot it joins and widens that cell's
     // range; so points farther than twice the median distance from their
-    // pivot seed more pivots (a stride over them, up to np/8 a round, two
-    // rounds), and the points are assigned again.
+    // pivot seed more pivots (a stride over them, up to a quarter of the
+    // pivots a round), and the points are assigned again, until at most one
+    // point in a thousand is far (or six rounds, or kMaxPivots).
     const int bits = 32 * wpad;
     std::vector<int32_t> prow((size_t)np);
     const int64_t step = N / np;
@@ -689,7 +731,7 @@ void CellIndex::build(const uint32_t* ds, int64_t N, int wpad, hipStream_t s) {
         launch_expand_fp4(psk.get(), n, wpad, pfp4.get(), s);
         // Nearest pivot of every point.
         pivots(wpad, ds, N, pfp4.get(), n, 0, ckey.get(), nullptr, s);
-        if (round == 2 || n >= kMaxPivots) break;
+        if (round == 6 || n >= kMaxPivots) break;
         std::vector<uint32_t> hk((size_t)N);
         LSH_HIP(hipMemcpyAsync(hk.data(), ckey.get(), sizeof(uint32_t) * N, hipMemcpyDeviceToHost, s));
         LSH_HIP(hipStreamSynchronize(s));
@@ -701,8 +743,8 @@ void CellIndex::build(const uint32_t* ds, int64_t N, int wpad, hipStream_t s) {
         std::vector<int32_t> far;
         for (int64_t i = 0; i < N; ++i)
             if ((int)(hk[i] & kDistMask) > 2 * median) far.push_back((int32_t)i);
-        if (far.empty()) break;
-        const size_t add = std::min<size_t>({far.size(), (size_t)(kMaxPivots - n), (size_t)std::max(1, n / 8)});
+        if ((int64_t)far.size() * 1000 <= N) break;
+        const size_t add = std::min<size_t>({far.size(), (size_t)(kMaxPivots - n), (size_t)std::max(1, n / 4)});
         for (size_t i = 0; i < add; ++i) prow.push_back(far[i * far.size() / add]);
     }
     np = (int)prow.size();
@@ -775,41 +817,46 @@ bool cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K, const Top
     const int Pp = (P + 31) / 32 * 32;
     const int bits = 32 * c.words;
     uint16_t* D = sc.get<uint16_t>(depth, 0, (size_t)Q * Pp);
-    uint32_t* meta = sc.get<uint32_t>(depth, 1, 5 * ((size_t)P + 1) + Q);
-    uint32_t* cnt = meta;
-    uint32_t* coff = meta + (P + 1);
-    uint32_t* choff = meta + 2 * (P + 1);
-    uint32_t* nearest = meta + 5 * (P + 1);
+    // meta: cnt1, cnt2 [P+1 each], coff [P+1], choff [P+2], the query sort's
+    // offsets and cursor [P+1 each], nearest cell [Q].
+    uint32_t* meta = sc.get<uint32_t>(depth, 1, 6 * ((size_t)P + 1) + 1 + Q);
+    uint32_t* cnt1 = meta;
+    uint32_t* cnt2 = meta + (P + 1);
+    uint32_t* coff = meta + 2 * (P + 1);
+    uint32_t* choff = meta + 3 * (P + 1);
+    uint32_t* sort_off = meta + 4 * (P + 1) + 1;
+    uint32_t* sort_cur = meta + 5 * (P + 1) + 1;
+    uint32_t* nearest = meta + 6 * (P + 1) + 1;
     const size_t hn = (size_t)Q * kHistBins;
     uint32_t* hist = sc.get<uint32_t>(depth, 3, hn + ((size_t)Q + 1) / 2);
     uint16_t* R = reinterpret_cast<uint16_t*>(hist + hn);
-    const double mean_tiles = (double)c.ntiles / P;
+    LSH_HIP(hipMemsetAsync(cnt1, 0, sizeof(uint32_t) * 2 * (P + 1), s));
+    LSH_HIP(hipMemsetAsync(hist, 0, sizeof(uint32_t) * hn, s));
     pivots(c.words, qs, Q, c.pfp4.get(), P, Pp, nullptr, D, s);
-    // Threshold bound from the tile ranges, the first-pass radii and the
-    // nearest cell of each query.
+    // Threshold bound from the tile ranges, the first-pass radii and their
+    // cell counts, and the nearest cell of each query.
     hipLaunchKernelGGL(cell_threshold_kernel, dim3((unsigned)Q), dim3(256), 0, s, D, Q, P, Pp, c.csize.get(),
-                       c.ctile.get(), c.tinfo.get(), bits, K, v.thr, R, nearest);
+                       c.ctile.get(), c.tinfo.get(), bits, K, v.thr, R, nearest, cnt1);
     check_launch("cell_threshold_kernel");
     if (qorder) {
         // Queries grouped by nearest cell (a counting sort): neighbouring
         // queries then share most candidates, which the rerank can reuse.
-        launch_bucket_build(nearest, Q, 1, (uint32_t)P, meta + 3 * (P + 1), meta + 4 * (P + 1),
-                            reinterpret_cast<uint32_t*>(qorder), s);
+        launch_bucket_build(nearest, Q, 1, (uint32_t)P, sort_off, sort_cur, reinterpret_cast<uint32_t*>(qorder), s);
     }
-    // First pass: exact distances over the nearest cells, K-th by histogram.
-    LSH_HIP(hipMemsetAsync(hist, 0, sizeof(uint32_t) * hn, s));
-    const Lists l1 = count_lists(c, D, Q, Pp, nullptr, R, cnt, coff, choff, s);
+    // First pass: exact distances over the nearest cells, K-th by histogram,
+    // then the second pass's cell counts under the final threshold.
+    const Lists l1 = list_offsets(c, cnt1, coff, choff, s);
     int32_t* list = sc.get<int32_t>(depth, 2, (size_t)l1.entries + l1.chunks + 1);
     if (l1.chunks) {
-        place_lists(c, D, Q, Pp, nullptr, R, cnt, coff, list, s);
+        place_lists(c, D, Q, Pp, nullptr, R, cnt1, coff, list, s);
         cell_scan(c, l1.chunks, coff, choff, reinterpret_cast<uint32_t*>(list + l1.entries), Pp, list, D, qs, v,
                   hist, s);
-        hipLaunchKernelGGL(hist_threshold_kernel, dim3((unsigned)((Q + 3) / 4)), dim3(256), 0, s, hist, Q, bits, K,
-                           v.thr);
-        check_launch("hist_threshold_kernel");
     }
+    hipLaunchKernelGGL(hist_threshold_kernel, dim3((unsigned)((Q + 3) / 4)), dim3(256), 0, s, hist, Q, bits, K, v.thr,
+                       D, P, Pp, c.cmin.get(), c.cmax.get(), cnt2);
+    check_launch("hist_threshold_kernel");
     // Second pass: the tiles the bounds leave.
-    const Lists l2 = count_lists(c, D, Q, Pp, v.thr, nullptr, cnt, coff, choff, s);
+    const Lists l2 = list_offsets(c, cnt2, coff, choff, s);
     if (debug_cells()) {
         std::vector<uint64_t> ht((size_t)Q);
         LSH_HIP(hipMemcpy(ht.data(), v.thr, sizeof(uint64_t) * Q, hipMemcpyDeviceToHost));
@@ -820,15 +867,14 @@ bool cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K, const Top
                      "entries %u chunks\n",
                      P, (long long)c.ntiles, Q, K, mt / Q, l1.entries, l1.chunks, l2.entries, l2.chunks);
     }
-    // Work in tiles (chunks x mean tiles per cell) against the full pass
-    // (query tiles x point tiles): too dense, leave it to the full pass.
-    const double pruned = (double)l2.chunks * mean_tiles;
+    // Work in tile products (32-query units) against the full pass (query
+    // tiles x point tiles): too dense, leave it to the full pass.
     const double full = std::ceil(Q / 32.0) * (double)c.ntiles;
-    if (pruned > 0.5 * full) return false;
-    if (tiles_out) *tiles_out += (int64_t)((l1.chunks + (double)l2.chunks) * mean_tiles);
+    if ((double)l2.work > 0.5 * full) return false;
+    if (tiles_out) *tiles_out += (int64_t)l1.work + (int64_t)l2.work;
     if (l2.chunks == 0) return true;
     list = sc.get<int32_t>(depth, 2, (size_t)l2.entries + l2.chunks);
-    place_lists(c, D, Q, Pp, v.thr, nullptr, cnt, coff, list, s);
+    place_lists(c, D, Q, Pp, v.thr, nullptr, cnt2, coff, list, s);
     cell_scan(c, l2.chunks, coff, choff, reinterpret_cast<uint32_t*>(list + l2.entries), Pp, list, D, qs, v,
               nullptr, s);
     return true;

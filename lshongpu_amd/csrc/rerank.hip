@@ -102,9 +102,47 @@ __device__ int select_smallest(const uint64_t* keys, int n, int kk, uint64_t* ou
     return m;
 }
 
+// Sum over the 16 lanes of a DPP row, in every lane of it: rotations by 8,
+// 4, 2 and 1 within the row (row_ror), one VALU add each.
+__device__ inline float row_sum16(float x) {
+    x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x128, 0xF, 0xF, false));
+    x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x124, 0xF, 0xF, false));
+    x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x122, 0xF, 0xF, false));
+    x += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x121, 0xF, 0xF, false));
+    return x;
+}
+
+// group_accumulate / group_distance with the distance kind fixed at compile
+// time (the same operations).
+template <int KIND>
+__device__ inline void accumulate_k(float x, float y, float& a, float& b) {
+    if (KIND == kNegCos || KIND == kAngular) {
+        a = __builtin_fmaf(x, y, a);
+        b = __builtin_fmaf(x, x, b);
+    } else if (KIND == kGenJaccard) {
+        a += fminf(x, y);
+        b += fmaxf(x, y);
+    } else {
+        const float t = y - x;
+        a = __builtin_fmaf(t, t, a);
+    }
+}
+
+template <int KIND>
+__device__ inline float distance_k(float a, float b, float q_norm) {
+    if (KIND == kNegCos || KIND == kAngular) {
+        const float c = a / (q_norm * __builtin_sqrtf(b));
+        return KIND == kNegCos ? -c : 1.0f - c;
+    } else if (KIND == kGenJaccard) {
+        return 1.0f - a / b;
+    }
+    return a;
+}
+
+template <int KIND>
 __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ data,
                                                      const float* __restrict__ queries, int d,
-                                                     int kind, const uint64_t* __restrict__ keys,
+                                                     const uint64_t* __restrict__ keys,
                                                      int key_stride, const int32_t* __restrict__ cand,
                                                      int C, int Cp, int k, int kprime, double eps,
                                                      lsh_point_t* __restrict__ out,
@@ -130,7 +168,7 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
 
     float qn = 0.0f;
     double nq64 = 0.0;
-    if (kind == kNegCos || kind == kAngular) {
+    if (KIND == kNegCos || KIND == kAngular) {
         for (int j = 0; j < d; ++j) qn = __builtin_fmaf(qv[j], qv[j], qn);
         qn = __builtin_sqrtf(qn);
         nq64 = norm2_64(qv, d);
@@ -152,20 +190,24 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
     }
     __syncthreads();
     // FP32 distances: a group of 16 lanes reads one candidate row with
-    // coalesced float4 loads and reduces its partial sums with shuffles.
+    // coalesced float4 loads; the group's partial sums are added with DPP
+    // rotations inside the 16-lane row, and lane u of the group keeps row u's
+    // sums in LDS (k64 serves as scratch: two floats per candidate).  The
+    // distances themselves are then taken one candidate per lane.
     const int gl = threadIdx.x & 15;
     const int ngroups = blockDim.x >> 4;
     const bool vec4 = (d & 3) == 0;
+    float2* sums = reinterpret_cast<float2*>(k64);
     // Each group takes U rows per step so that U row reads are in flight.
-    constexpr int U = 4;
+    constexpr int U = 6;
     for (int t0 = threadIdx.x >> 4; t0 < Cp; t0 += ngroups * U) {
-        int32_t id[U], row[U];
+        const float* rp[U];
         float a[U], b[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int t = t0 + u * ngroups;
-            id[u] = t < Cp ? ids[t] : -1;
-            row[u] = t < Cp ? rows[t] : -1;
+            const int r = t < Cp ? rows[t] : -1;
+            rp[u] = data + (size_t)(r < 0 ? 0 : r) * d;  // row 0 stands in; its sums are not used
             a[u] = 0.0f;
             b[u] = 0.0f;
         }
@@ -174,15 +216,13 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
                 const float4 yv = *reinterpret_cast<const float4*>(qv + j);
                 float4 xv[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u)
-                    xv[u] = id[u] >= 0 ? *reinterpret_cast<const float4*>(data + (size_t)row[u] * d + j)
-                                       : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                for (int u = 0; u < U; ++u) xv[u] = *reinterpret_cast<const float4*>(rp[u] + j);
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    group_accumulate(kind, xv[u].x, yv.x, a[u], b[u]);
-                    group_accumulate(kind, xv[u].y, yv.y, a[u], b[u]);
-                    group_accumulate(kind, xv[u].z, yv.z, a[u], b[u]);
-                    group_accumulate(kind, xv[u].w, yv.w, a[u], b[u]);
+                    accumulate_k<KIND>(xv[u].x, yv.x, a[u], b[u]);
+                    accumulate_k<KIND>(xv[u].y, yv.y, a[u], b[u]);
+                    accumulate_k<KIND>(xv[u].z, yv.z, a[u], b[u]);
+                    accumulate_k<KIND>(xv[u].w, yv.w, a[u], b[u]);
                 }
             }
         } else {
@@ -190,25 +230,33 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
                 const float y = qv[j];
                 float xs[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u) xs[u] = id[u] >= 0 ? data[(size_t)row[u] * d + j] : 0.0f;
+                for (int u = 0; u < U; ++u) xs[u] = rp[u][j];
 #pragma unroll
-                for (int u = 0; u < U; ++u) group_accumulate(kind, xs[u], y, a[u], b[u]);
+                for (int u = 0; u < U; ++u) accumulate_k<KIND>(xs[u], y, a[u], b[u]);
             }
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-#pragma unroll
-            for (int o = 8; o > 0; o >>= 1) {
-                a[u] += __shfl_xor(a[u], o, 16);
-                b[u] += __shfl_xor(b[u], o, 16);
-            }
-            const int t = t0 + u * ngroups;
-            if (gl == 0 && t < Cp) {
-                const float dist = group_distance(kind, a[u], b[u], qn);
-                k32[t] = id[u] < 0 ? kEmptyKey : make_key(f32_key(dist), (uint32_t)id[u]);
-            }
+            a[u] = row_sum16(a[u]);
+            b[u] = row_sum16(b[u]);
         }
+        float as = a[0], bs = b[0];
+#pragma unroll
+        for (int u = 1; u < U; ++u)
+            if (gl == u) {
+                as = a[u];
+                bs = b[u];
+            }
+        const int t = t0 + gl * ngroups;
+        if (gl < U && t < Cp) sums[t] = make_float2(as, bs);
     }
+    __syncthreads();
+    for (int t = threadIdx.x; t < Cp; t += blockDim.x) {
+        const int32_t idt = ids[t];
+        const float2 ab = sums[t];
+        k32[t] = idt < 0 ? kEmptyKey : make_key(f32_key(distance_k<KIND>(ab.x, ab.y, qn)), (uint32_t)idt);
+    }
+    __syncthreads();
     // The kp best by FP32 distance, sorted, in k64 (k32 keeps every key).
     const int kp = min(kprime, Cp);
     select_smallest(k32, Cp, kp, k64, rs_hist, rs_sv);
@@ -219,7 +267,7 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
             if (kK != kEmptyKey && kk != kEmptyKey) {
                 const double vk = key_f32((uint32_t)(kk >> 32));
                 const double vK = key_f32((uint32_t)(kK >> 32));
-                if (kind == kEuclid)
+                if (KIND == kEuclid)
                     ok = vK * (1.0 - eps) > vk * (1.0 + eps);
                 else
                     ok = vK - vk > 2.0 * eps;
@@ -237,7 +285,7 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
                 ids[t] = 0x7FFFFFFF;
             } else {
                 const int32_t id = (int32_t)(uint32_t)key;
-                k64[t] = f64_key(dist64_row(kind, qv, data + (size_t)(rowmap ? rowmap[id] : id) * d, d, nq64));
+                k64[t] = f64_key(dist64_row(KIND, qv, data + (size_t)(rowmap ? rowmap[id] : id) * d, d, nq64));
                 ids[t] = id;
             }
         }
@@ -250,14 +298,14 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
                 ids[t] = 0x7FFFFFFF;
             } else {
                 const int32_t id = (int32_t)(uint32_t)key;
-                k64[t] = f64_key(dist64_row(kind, qv, data + (size_t)(rowmap ? rowmap[id] : id) * d, d, nq64));
+                k64[t] = f64_key(dist64_row(KIND, qv, data + (size_t)(rowmap ? rowmap[id] : id) * d, d, nq64));
                 ids[t] = id;
             }
         }
         n = Cp;
     }
     bitonic_sort_pairs(k64, ids, n);
-    const float pad = kind == kNegCos ? 2.0f : kPadDistance;
+    const float pad = KIND == kNegCos ? 2.0f : kPadDistance;
     for (int t = threadIdx.x; t < k; t += blockDim.x) {
         lsh_point_t p;
         if (t < n && ids[t] != 0x7FFFFFFF) {
@@ -281,10 +329,19 @@ void launch_rerank(const float* data, const float* queries, int d, int kind, con
     LSH_REQUIRE(Cp <= kMaxTopK, LSH_ERR_INVALID, "candidate set larger than 4096");
     const int kp = pow2_ceil(std::max(kprime, k));
     const size_t lds = (size_t)Cp * (2 * sizeof(uint64_t) + 2 * sizeof(int32_t));
-    set_dynamic_lds((const void*)rerank_kernel, lds);
     const int per = (Q + 7) / 8;
-    hipLaunchKernelGGL(rerank_kernel, dim3(qorder ? 8 * per : Q), dim3(256), lds, s, data, queries, d, kind, keys,
-                       key_stride, cand, C, Cp, k, kp, eps, out, qorder, Q, per, rowmap);
+    auto go = [&](auto kernel) {
+        set_dynamic_lds((const void*)kernel, lds);
+        hipLaunchKernelGGL(kernel, dim3(qorder ? 8 * per : Q), dim3(256), lds, s, data, queries, d, keys, key_stride,
+                           cand, C, Cp, k, kp, eps, out, qorder, Q, per, rowmap);
+    };
+    switch (kind) {
+        case kNegCos: go(rerank_kernel<kNegCos>); break;
+        case kAngular: go(rerank_kernel<kAngular>); break;
+        case kGenJaccard: go(rerank_kernel<kGenJaccard>); break;
+        case kEuclid: go(rerank_kernel<kEuclid>); break;
+        default: fail(LSH_ERR_INTERNAL, "rerank: unknown distance kind");
+    }
     check_launch("rerank_kernel");
 }
 

@@ -155,6 +155,13 @@ __global__ __launch_bounds__(kSelThreads) void sort_rows_kernel(uint64_t* keys, 
     for (int i = threadIdx.x; i < K; i += blockDim.x) row[i] = sm[i];
 }
 
+// The first `cols` entries of each of `rows` rows of length `stride`.
+__global__ void fill_row_heads_u64(uint64_t* p, int rows, int cols, int stride, uint64_t val) {
+    const size_t total = (size_t)rows * cols;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x)
+        p[(i / cols) * stride + i % cols] = val;
+}
+
 __global__ void fill_u64(uint64_t* p, size_t n, uint64_t val) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t st = (size_t)gridDim.x * blockDim.x;
@@ -212,8 +219,10 @@ void TopKState::reset(int Q, int K, int newcap, hipStream_t s, bool int_keys) {
     ovf_.ensure((size_t)Q);
     v_ = TopKView{keys_.get(), cnt_.get(), thr_.get(), ovf_.get(), K, newcap, stride, Q, int_keys ? 1 : 0};
     // Fill kernels rather than memsets keep the sequence capturable in a graph.
-    size_t nk = (size_t)Q * stride;
-    hipLaunchKernelGGL(fill_u64, dim3(grid_for(nk)), dim3(256), 0, s, keys_.get(), nk, kEmptyKey);
+    // Only the best lists need clearing: the selects read the appended slots
+    // up to each query's count.
+    hipLaunchKernelGGL(fill_row_heads_u64, dim3(grid_for((size_t)Q * K)), dim3(256), 0, s, keys_.get(), Q, K, stride,
+                       kEmptyKey);
     hipLaunchKernelGGL(fill_u64, dim3(grid_for(Q)), dim3(256), 0, s, thr_.get(), (size_t)Q, kEmptyKey);
     hipLaunchKernelGGL(fill_u32, dim3(grid_for(Q)), dim3(256), 0, s, cnt_.get(), (size_t)Q, 0u);
     hipLaunchKernelGGL(fill_u32, dim3(grid_for(Q)), dim3(256), 0, s, ovf_.get(), (size_t)Q, 0u);
