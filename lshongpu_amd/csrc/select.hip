@@ -55,30 +55,73 @@ __global__ __launch_bounds__(kSelThreads) void select_merge_kernel(TopKView v) {
 // Integer-key variant (sketch distances are small integers): the best list
 // is kept as an unordered set of exactly K keys (K need not be a power of
 // two).  A histogram of the distances of (best list + new keys) in LDS gives
-// the distance T of the K-th key; every key below T is kept, and the keys at
-// exactly T are sorted by ID to keep the lowest.  thr becomes the largest
-// kept key.  Distances at or above kSelBins-1 share the last bin, which the
-// tie step orders exactly.
-constexpr int kSelBins = 4096;
+// the bin T that holds the K-th key; a radix select over the keys of that
+// bin (8-bit digits of the ID, or of the whole key in the last bin, which
+// also holds every larger distance) finds the K-th key itself, and every key
+// at or below it is kept.  thr becomes that key.
+constexpr int kSelBins = 2048;
 
-__global__ __launch_bounds__(kSelThreads) void select_hist_kernel(TopKView v, int tie_cap) {
-    extern __shared__ uint64_t sm[];
+// The rank-th smallest (1-based) key among the non-empty keys of row[0, n)
+// whose bin is T and whose bits above `shift` + 8 match prefix; shifts run
+// from `top` down to 0.  Uses hist[0, 256).
+__device__ uint64_t radix_kth_in_bin(const uint64_t* row, int n, uint32_t T, uint32_t rank, int top, uint64_t prefix,
+                                     uint32_t* hist, uint32_t* s_dig, uint32_t* s_rank) {
+    uint64_t mask = top == 24 ? 0xFFFFFFFF00000000ull : 0ull;
+    for (int shift = top; shift >= 0; shift -= 8) {
+        for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0;
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            const uint64_t key = row[i];
+            if (key == kEmptyKey || min((uint32_t)(key >> 32), (uint32_t)(kSelBins - 1)) != T) continue;
+            if ((key & mask) == prefix) atomicAdd(&hist[(uint32_t)(key >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            const int lane = threadIdx.x;
+            uint32_t c[4], sum = 0;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                c[e] = hist[4 * lane + e];
+                sum += c[e];
+            }
+            uint32_t inc = sum;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t x = __shfl_up(inc, o);
+                if (lane >= o) inc += x;
+            }
+            uint32_t run = inc - sum;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                if (run < rank && run + c[e] >= rank) {
+                    *s_dig = 4u * lane + e;
+                    *s_rank = rank - run;
+                }
+                run += c[e];
+            }
+        }
+        __syncthreads();
+        prefix |= (uint64_t)*s_dig << shift;
+        mask |= 0xFFull << shift;
+        rank = *s_rank;
+        __syncthreads();
+    }
+    return prefix;
+}
+
+__global__ __launch_bounds__(kSelThreads) void select_hist_kernel(TopKView v) {
+    extern __shared__ uint64_t kept[];  // K
     __shared__ uint32_t hist[kSelBins];
-    __shared__ uint32_t s_T, s_below, s_nkeep, s_ntie;
+    __shared__ uint32_t s_T, s_below, s_nkeep, s_dig, s_rank;
     const int q = blockIdx.x;
     const uint32_t n_raw = v.cnt[q];
     if (n_raw == 0) return;
     const int K = v.K;
-    uint64_t* kept = sm;       // K
-    uint64_t* ties = sm + K;   // tie_cap (power of two >= K + newcap)
     const int n = (int)min(n_raw, (uint32_t)v.newcap);
     const int total = K + n;
     uint64_t* row = v.keys + (size_t)q * v.stride;
     for (int i = threadIdx.x; i < kSelBins; i += blockDim.x) hist[i] = 0;
-    if (threadIdx.x == 0) {
-        s_nkeep = 0;
-        s_ntie = 0;
-    }
+    if (threadIdx.x == 0) s_nkeep = 0;
     __syncthreads();
     for (int i = threadIdx.x; i < total; i += blockDim.x) {
         const uint64_t key = row[i];
@@ -118,32 +161,28 @@ __global__ __launch_bounds__(kSelThreads) void select_hist_kernel(TopKView v, in
     }
     __syncthreads();
     const uint32_t T = s_T;
+    uint64_t kth = kEmptyKey;  // the K-th smallest key (none: keep every key)
+    if (T < (uint32_t)kSelBins) {
+        const bool last = T == (uint32_t)(kSelBins - 1);
+        kth = radix_kth_in_bin(row, total, T, (uint32_t)K - s_below, last ? 56 : 24, last ? 0ull : (uint64_t)T << 32,
+                               hist, &s_dig, &s_rank);
+    }
     for (int i = threadIdx.x; i < total; i += blockDim.x) {
         const uint64_t key = row[i];
-        if (key == kEmptyKey) continue;
-        const uint32_t b = min((uint32_t)(key >> 32), (uint32_t)(kSelBins - 1));
-        if (b < T)
-            kept[atomicAdd(&s_nkeep, 1u)] = key;
-        else if (b == T)
-            ties[atomicAdd(&s_ntie, 1u)] = key;
+        if (key != kEmptyKey && key <= kth) {
+            const uint32_t at = atomicAdd(&s_nkeep, 1u);
+            if (at < (uint32_t)K) kept[at] = key;
+        }
     }
     __syncthreads();
-    const uint32_t below = s_below;  // == s_nkeep
-    const uint32_t ntie = s_ntie;
-    int P = 1;
-    while (P < (int)ntie) P <<= 1;
-    for (int i = (int)ntie + threadIdx.x; i < P; i += blockDim.x) ties[i] = kEmptyKey;
-    bitonic_sort_lds(ties, P);
-    const uint32_t take = T == kSelBins ? 0u : min((uint32_t)K - below, ntie);
-    for (uint32_t i = threadIdx.x; i < below; i += blockDim.x) row[i] = kept[i];
-    for (uint32_t i = threadIdx.x; i < take; i += blockDim.x) row[below + i] = ties[i];
-    for (uint32_t i = below + take + threadIdx.x; i < (uint32_t)K; i += blockDim.x) row[i] = kEmptyKey;
+    const uint32_t nkeep = min(s_nkeep, (uint32_t)K);
+    for (uint32_t i = threadIdx.x; i < nkeep; i += blockDim.x) row[i] = kept[i];
+    for (uint32_t i = nkeep + threadIdx.x; i < (uint32_t)K; i += blockDim.x) row[i] = kEmptyKey;
     if (threadIdx.x == 0) {
-        v.thr[q] = (below + take == (uint32_t)K) ? ties[take - 1] : kEmptyKey;
+        v.thr[q] = nkeep == (uint32_t)K ? kth : kEmptyKey;
         v.cnt[q] = 0;
         if (n_raw > (uint32_t)v.newcap) v.ovf[q] = 1;
     }
-    (void)tie_cap;
 }
 
 // Sorts the first K keys of every row ascending (K <= 4096).
@@ -231,10 +270,9 @@ void TopKState::reset(int Q, int K, int newcap, hipStream_t s, bool int_keys) {
 
 void launch_select_merge(const TopKView& v, hipStream_t s) {
     if (v.int_keys) {
-        const int tie_cap = pow2_ceil(v.K + v.newcap);
-        const size_t lds = (size_t)(v.K + tie_cap) * sizeof(uint64_t);
-        set_dynamic_lds((const void*)select_hist_kernel, lds + sizeof(uint32_t) * (kSelBins + 4));
-        hipLaunchKernelGGL(select_hist_kernel, dim3(v.Q), dim3(kSelThreads), lds, s, v, tie_cap);
+        const size_t lds = (size_t)v.K * sizeof(uint64_t);
+        set_dynamic_lds((const void*)select_hist_kernel, lds + sizeof(uint32_t) * (kSelBins + 8));
+        hipLaunchKernelGGL(select_hist_kernel, dim3(v.Q), dim3(kSelThreads), lds, s, v);
         check_launch("select_hist_kernel");
         return;
     }
