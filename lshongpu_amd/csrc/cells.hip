@@ -335,15 +335,15 @@ __global__ __launch_bounds__(256) void cell_threshold_kernel(const uint16_t* __r
 // Per-cell query lists.  R == nullptr: the queries whose bound on the cell
 // is below their threshold distance; otherwise those whose distance to the
 // cell's pivot is at most their radius (the first pass).  list == nullptr:
-// count into cnt; otherwise place them.  Grid: x over cells, y (strided)
-// over queries.
+// count into cnt; otherwise place them.  A set skip flag leaves the lists
+// empty.  Grid: x over cells, y (strided) over queries.
 __global__ void cell_need_kernel(const uint16_t* __restrict__ D, int Q, int P, int Pp,
                                  const uint16_t* __restrict__ cmin, const uint16_t* __restrict__ cmax,
                                  const uint64_t* __restrict__ thr, const uint16_t* __restrict__ R,
                                  uint32_t* __restrict__ cnt, const uint32_t* __restrict__ coff,
-                                 int32_t* __restrict__ list) {
+                                 int32_t* __restrict__ list, const uint32_t* __restrict__ skip) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= P) return;
+    if (j >= P || (skip && *skip)) return;
     const uint32_t lo = cmin[j], hi = cmax[j];
     if (lo > hi) return;
     for (int q = blockIdx.y; q < Q; q += gridDim.y) {
@@ -363,12 +363,15 @@ __device__ inline uint32_t cell_parts(const uint32_t* ctile, int j) {
 
 // coff = exclusive prefix of cnt, choff = exclusive prefix of the cells'
 // chunks (ceil(cnt/32) query groups x cell_parts tile ranges), both with a
-// final total entry; choff[P+1] = the tile products those chunks can make
-// (query groups x tiles, summed); cursor (= cnt for the placing pass) zeroed.
+// final total entry; choff[P+1] and *work_out = the tile products those
+// chunks can make (query groups x tiles, summed); *dense (may be null) = 1
+// when that exceeds dense_limit; cursor (= cnt for the placing pass) zeroed.
 __global__ __launch_bounds__(1024) void cell_offsets_kernel(uint32_t* __restrict__ cnt, int P,
                                                             const uint32_t* __restrict__ ctile,
                                                             uint32_t* __restrict__ coff,
-                                                            uint32_t* __restrict__ choff) {
+                                                            uint32_t* __restrict__ choff,
+                                                            uint32_t* __restrict__ work_out,
+                                                            uint32_t* __restrict__ dense, uint32_t dense_limit) {
     __shared__ uint32_t ws[3][16];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint32_t carry_a = 0, carry_b = 0, carry_w = 0;
@@ -418,6 +421,8 @@ __global__ __launch_bounds__(1024) void cell_offsets_kernel(uint32_t* __restrict
         coff[P] = carry_a;
         choff[P] = carry_b;
         choff[P + 1] = carry_w;
+        *work_out = carry_w;
+        if (dense) *dense = carry_w > dense_limit ? 1u : 0u;
     }
 }
 
@@ -443,11 +448,12 @@ __global__ __launch_bounds__(256) void cell_scan_kernel(const uint32_t* __restri
                                                         const uint32_t* __restrict__ ctile,
                                                         const uint32_t* __restrict__ coff,
                                                         const uint32_t* __restrict__ choff,
-                                                        const uint32_t* __restrict__ cmap, uint32_t nchunks, int Pp,
+                                                        const uint32_t* __restrict__ cmap, int P, int Pp,
                                                         const int32_t* __restrict__ list,
                                                         const uint16_t* __restrict__ D,
                                                         const uint32_t* __restrict__ qs, TopKView v,
-                                                        uint32_t* __restrict__ hist) {
+                                                        uint32_t* __restrict__ hist,
+                                                        const uint32_t* __restrict__ skip) {
     constexpr int W = 2 * KS;
     constexpr int BITS = 32 * W;
     constexpr int HW = kHistBins / 2;  // LDS words per query
@@ -455,129 +461,132 @@ __global__ __launch_bounds__(256) void cell_scan_kernel(const uint32_t* __restri
     constexpr int NV = KS / 4;         // uint4 loads per lane per tile
     __shared__ uint32_t lhist[LH];
     const int lane = threadIdx.x & 63;
-    const uint32_t c = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (c >= nchunks) return;
-    const uint32_t j = cmap[c];
-    const uint32_t parts = cell_parts(ctile, j);
-    const uint32_t local = c - choff[j];
-    const uint32_t q0 = coff[j] + 32 * (local / parts);
-    const uint32_t m = min(32u, coff[j + 1] - q0);
-    const int col = lane & 31;
-    const int q = (uint32_t)col < m ? list[q0 + col] : -1;
-    uint32_t h = 0;
-    float tf = __builtin_inff();  // idle columns never pass
-    uint32_t dq = 0;
-    if (q >= 0) {
-        h = kHist ? 0xFFFFFFFFu : (uint32_t)(v.thr[q] >> 32);
-        tf = h == 0xFFFFFFFFu ? -__builtin_inff() : (float)BITS - 2.0f * (float)h;
-        dq = D[(size_t)q * Pp + j];
-    }
-    uint32_t* lh = lhist + (kHist ? (threadIdx.x >> 6) * 32 * HW : 0);
-    auto flush = [&]() {
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        for (int e = lane; e < 32 * HW; e += 64) {
-            const uint32_t x = lh[e];
-            const int cq = e / HW;
-            if (x && (uint32_t)cq < m) {
-                uint32_t* row = hist + (size_t)list[q0 + cq] * kHistBins + 2 * (e % HW);
-                if (x & 0xFFFFu) atomicAdd(&row[0], x & 0xFFFFu);
-                if (x >> 16) atomicAdd(&row[1], x >> 16);
-            }
-            lh[e] = 0;
+    // Chunks are taken in a grid-stride loop (their count is on the device);
+    // a set skip flag (the second pass found too dense) leaves them all.
+    const uint32_t nchunks = (skip && *skip) ? 0u : choff[P];
+    for (uint32_t c = blockIdx.x * 4 + (threadIdx.x >> 6); c < nchunks; c += gridDim.x * 4) {
+        const uint32_t j = cmap[c];
+        const uint32_t parts = cell_parts(ctile, j);
+        const uint32_t local = c - choff[j];
+        const uint32_t q0 = coff[j] + 32 * (local / parts);
+        const uint32_t m = min(32u, coff[j + 1] - q0);
+        const int col = lane & 31;
+        const int q = (uint32_t)col < m ? list[q0 + col] : -1;
+        uint32_t h = 0;
+        float tf = __builtin_inff();  // idle columns never pass
+        uint32_t dq = 0;
+        if (q >= 0) {
+            h = kHist ? 0xFFFFFFFFu : (uint32_t)(v.thr[q] >> 32);
+            tf = h == 0xFFFFFFFFu ? -__builtin_inff() : (float)BITS - 2.0f * (float)h;
+            dq = D[(size_t)q * Pp + j];
         }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    };
-    if (kHist) {
-        for (int e = lane; e < 32 * HW; e += 64) lh[e] = 0;
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    }
-    v4i b[KS];
-#pragma unroll
-    for (int s = 0; s < KS; ++s) b[s] = expand_word(q >= 0 ? qs[(size_t)q * W + 2 * s + (lane >> 5)] : 0u);
-    const uint32_t t0 = ctile[j] + (local % parts) * kChunkTiles;
-    const uint32_t nt = min((uint32_t)kChunkTiles, ctile[j + 1] - t0);
-    // Lane l's half of point row l%32 of tile t: W/2 words.
-    auto load = [&](uint32_t t, uint4* x) {
-        const uint4* src = reinterpret_cast<const uint4*>(sk + ((size_t)t * 32 + col) * W + (lane >> 5) * KS);
-#pragma unroll
-        for (int i = 0; i < NV; ++i) x[i] = src[i];
-    };
-    int since_flush = 0;
-    for (uint32_t i0 = 0; i0 < nt; i0 += 64) {
-        const uint32_t n64 = min(64u, nt - i0);
-        const uint32_t myinfo = (uint32_t)lane < n64 ? tinfo[t0 + i0 + lane] : 0u;
-        uint64_t needm = n64 == 64 ? ~0ull : ((1ull << n64) - 1ull);
-        if (!kHist) {
-            uint64_t nm = 0;
-            for (uint32_t tt = 0; tt < n64; ++tt) {
-                const uint32_t info = (uint32_t)__shfl((int)myinfo, (int)tt);
-                const uint32_t tlo = info & kDistMask, thi = (info >> kKeyShift) & kDistMask;
-                if (__ballot(q >= 0 && lower_bound_to(dq, tlo, thi) < h)) nm |= 1ull << tt;
+        uint32_t* lh = lhist + (kHist ? (threadIdx.x >> 6) * 32 * HW : 0);
+        auto flush = [&]() {
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            for (int e = lane; e < 32 * HW; e += 64) {
+                const uint32_t x = lh[e];
+                const int cq = e / HW;
+                if (x && (uint32_t)cq < m) {
+                    uint32_t* row = hist + (size_t)list[q0 + cq] * kHistBins + 2 * (e % HW);
+                    if (x & 0xFFFFu) atomicAdd(&row[0], x & 0xFFFFu);
+                    if (x >> 16) atomicAdd(&row[1], x >> 16);
+                }
+                lh[e] = 0;
             }
-            needm = nm;
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        };
+        if (kHist) {
+            for (int e = lane; e < 32 * HW; e += 64) lh[e] = 0;
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         }
-        if (!needm) continue;
-        uint4 nx[NV];
-        int nt_next = __builtin_ctzll(needm);
-        load(t0 + i0 + nt_next, nx);
-        while (true) {
-            const int tt = nt_next;
-            uint4 cur[NV];
-#pragma unroll
-            for (int i = 0; i < NV; ++i) cur[i] = nx[i];
-            needm &= needm - 1ull;
-            if (needm) {
-                nt_next = __builtin_ctzll(needm);
-                load(t0 + i0 + nt_next, nx);
+        v4i b[KS];
+    #pragma unroll
+        for (int s = 0; s < KS; ++s) b[s] = expand_word(q >= 0 ? qs[(size_t)q * W + 2 * s + (lane >> 5)] : 0u);
+        const uint32_t t0 = ctile[j] + (local % parts) * kChunkTiles;
+        const uint32_t nt = min((uint32_t)kChunkTiles, ctile[j + 1] - t0);
+        // Lane l's half of point row l%32 of tile t: W/2 words.
+        auto load = [&](uint32_t t, uint4* x) {
+            const uint4* src = reinterpret_cast<const uint4*>(sk + ((size_t)t * 32 + col) * W + (lane >> 5) * KS);
+    #pragma unroll
+            for (int i = 0; i < NV; ++i) x[i] = src[i];
+        };
+        int since_flush = 0;
+        for (uint32_t i0 = 0; i0 < nt; i0 += 64) {
+            const uint32_t n64 = min(64u, nt - i0);
+            const uint32_t myinfo = (uint32_t)lane < n64 ? tinfo[t0 + i0 + lane] : 0u;
+            uint64_t needm = n64 == 64 ? ~0ull : ((1ull << n64) - 1ull);
+            if (!kHist) {
+                uint64_t nm = 0;
+                for (uint32_t tt = 0; tt < n64; ++tt) {
+                    const uint32_t info = (uint32_t)__shfl((int)myinfo, (int)tt);
+                    const uint32_t tlo = info & kDistMask, thi = (info >> kKeyShift) & kDistMask;
+                    if (__ballot(q >= 0 && lower_bound_to(dq, tlo, thi) < h)) nm |= 1ull << tt;
+                }
+                needm = nm;
             }
-            const uint32_t t = t0 + i0 + (uint32_t)tt;
-            const uint32_t nv = (uint32_t)__shfl((int)myinfo, tt) >> 22;
-            const uint32_t* cw = reinterpret_cast<const uint32_t*>(cur);
-            v16f acc = {};
-#pragma unroll
-            for (int s = 0; s < KS; ++s) acc = fp4::mma(expand_word(cw[s]), b[s], acc);
-            uint32_t pass = 0;
-#pragma unroll
-            for (int r = 0; r < 16; ++r)
-                pass |= (acc[r] > tf && (uint32_t)fp4::acc_row(r, lane) < nv ? 1u : 0u) << r;
-            if (kHist) {
-#pragma unroll
+            if (!needm) continue;
+            uint4 nx[NV];
+            int nt_next = __builtin_ctzll(needm);
+            load(t0 + i0 + nt_next, nx);
+            while (true) {
+                const int tt = nt_next;
+                uint4 cur[NV];
+    #pragma unroll
+                for (int i = 0; i < NV; ++i) cur[i] = nx[i];
+                needm &= needm - 1ull;
+                if (needm) {
+                    nt_next = __builtin_ctzll(needm);
+                    load(t0 + i0 + nt_next, nx);
+                }
+                const uint32_t t = t0 + i0 + (uint32_t)tt;
+                const uint32_t nv = (uint32_t)__shfl((int)myinfo, tt) >> 22;
+                const uint32_t* cw = reinterpret_cast<const uint32_t*>(cur);
+                v16f acc = {};
+    #pragma unroll
+                for (int s = 0; s < KS; ++s) acc = fp4::mma(expand_word(cw[s]), b[s], acc);
+                uint32_t pass = 0;
+    #pragma unroll
                 for (int r = 0; r < 16; ++r)
-                    if ((pass >> r) & 1u) {
-                        const int bin = hist_bin((BITS - (int)acc[r]) >> 1);
-                        atomicAdd(&lh[col * HW + (bin >> 1)], 1u << (16 * (bin & 1)));
-                    }
-                if (++since_flush == kFlushTiles) {
-                    flush();
-                    since_flush = 0;
-                }
-            } else {
-                const uint32_t mine = (uint32_t)__popc(pass);
-                const uint32_t other = (uint32_t)__shfl_xor((int)mine, 32);
-                uint32_t base = 0;
-                if (lane < 32 && mine + other > 0) base = atomicAdd(&v.cnt[q], mine + other);
-                base = (uint32_t)__shfl((int)base, col);
-                if (lane >= 32) base += other;
-                if (pass) {
-                    const size_t row0 = (size_t)q * v.stride + v.K;
-#pragma unroll
-                    for (int r = 0; r < 16; ++r) {
-                        if (!((pass >> r) & 1u)) continue;
-                        if (base < (uint32_t)v.newcap) {
-                            const int32_t id = ids[(size_t)t * 32 + fp4::acc_row(r, lane)];
-                            v.keys[row0 + base] = make_key((uint32_t)((BITS - (int)acc[r]) >> 1), (uint32_t)id);
+                    pass |= (acc[r] > tf && (uint32_t)fp4::acc_row(r, lane) < nv ? 1u : 0u) << r;
+                if (kHist) {
+    #pragma unroll
+                    for (int r = 0; r < 16; ++r)
+                        if ((pass >> r) & 1u) {
+                            const int bin = hist_bin((BITS - (int)acc[r]) >> 1);
+                            atomicAdd(&lh[col * HW + (bin >> 1)], 1u << (16 * (bin & 1)));
                         }
-                        ++base;
+                    if (++since_flush == kFlushTiles) {
+                        flush();
+                        since_flush = 0;
+                    }
+                } else {
+                    const uint32_t mine = (uint32_t)__popc(pass);
+                    const uint32_t other = (uint32_t)__shfl_xor((int)mine, 32);
+                    uint32_t base = 0;
+                    if (lane < 32 && mine + other > 0) base = atomicAdd(&v.cnt[q], mine + other);
+                    base = (uint32_t)__shfl((int)base, col);
+                    if (lane >= 32) base += other;
+                    if (pass) {
+                        const size_t row0 = (size_t)q * v.stride + v.K;
+    #pragma unroll
+                        for (int r = 0; r < 16; ++r) {
+                            if (!((pass >> r) & 1u)) continue;
+                            if (base < (uint32_t)v.newcap) {
+                                const int32_t id = ids[(size_t)t * 32 + fp4::acc_row(r, lane)];
+                                v.keys[row0 + base] = make_key((uint32_t)((BITS - (int)acc[r]) >> 1), (uint32_t)id);
+                            }
+                            ++base;
+                        }
                     }
                 }
+                if (!needm) break;
             }
-            if (!needm) break;
         }
+        if (kHist && since_flush) flush();
     }
-    if (kHist && since_flush) flush();
 }
 
 // thr[q] = min(thr[q], (T + 1, 0)) with T the upper edge of the bin at which
@@ -642,56 +651,45 @@ dim3 need_grid(int P, int Q) {
 }
 
 template <int KS>
-void launch_cell_scan_ks(const CellIndex& c, unsigned chunks, const uint32_t* coff, const uint32_t* choff,
+void launch_cell_scan_ks(const CellIndex& c, unsigned grid, const uint32_t* coff, const uint32_t* choff,
                          const uint32_t* cmap, int Pp, const int32_t* list, const uint16_t* D, const uint32_t* qs,
-                         const TopKView& v, uint32_t* hist, hipStream_t s) {
+                         const TopKView& v, uint32_t* hist, const uint32_t* skip, hipStream_t s) {
     if (hist)
-        hipLaunchKernelGGL((cell_scan_kernel<KS, true>), dim3(blocks_for(chunks, 4)), dim3(256), 0, s, c.sk.get(),
-                           c.ids.get(), c.tinfo.get(), c.ctile.get(), coff, choff, cmap, chunks, Pp, list, D, qs, v,
-                           hist);
+        hipLaunchKernelGGL((cell_scan_kernel<KS, true>), dim3(grid), dim3(256), 0, s, c.sk.get(), c.ids.get(),
+                           c.tinfo.get(), c.ctile.get(), coff, choff, cmap, c.P, Pp, list, D, qs, v, hist, skip);
     else
-        hipLaunchKernelGGL((cell_scan_kernel<KS, false>), dim3(blocks_for(chunks, 4)), dim3(256), 0, s, c.sk.get(),
-                           c.ids.get(), c.tinfo.get(), c.ctile.get(), coff, choff, cmap, chunks, Pp, list, D, qs, v,
-                           hist);
+        hipLaunchKernelGGL((cell_scan_kernel<KS, false>), dim3(grid), dim3(256), 0, s, c.sk.get(), c.ids.get(),
+                           c.tinfo.get(), c.ctile.get(), coff, choff, cmap, c.P, Pp, list, D, qs, v, hist, skip);
     check_launch("cell_scan_kernel");
 }
 
-// The chunk -> cell map (after the lists' offsets), then the scan.
-void cell_scan(const CellIndex& c, unsigned chunks, const uint32_t* coff, const uint32_t* choff, uint32_t* cmap,
+// The chunk -> cell map (after the lists' offsets), then the scan over a
+// grid of `grid` workgroups that loops over the chunks.
+void cell_scan(const CellIndex& c, unsigned grid, const uint32_t* coff, const uint32_t* choff, uint32_t* cmap,
                int Pp, const int32_t* list, const uint16_t* D, const uint32_t* qs, const TopKView& v, uint32_t* hist,
-               hipStream_t s) {
+               const uint32_t* skip, hipStream_t s) {
     hipLaunchKernelGGL(chunk_map_kernel, dim3(blocks_for((size_t)c.P, 256)), dim3(256), 0, s, choff, c.P, cmap);
     check_launch("chunk_map_kernel");
     switch (c.words) {
-        case 8: launch_cell_scan_ks<4>(c, chunks, coff, choff, cmap, Pp, list, D, qs, v, hist, s); break;
-        case 16: launch_cell_scan_ks<8>(c, chunks, coff, choff, cmap, Pp, list, D, qs, v, hist, s); break;
-        case 24: launch_cell_scan_ks<12>(c, chunks, coff, choff, cmap, Pp, list, D, qs, v, hist, s); break;
-        case 32: launch_cell_scan_ks<16>(c, chunks, coff, choff, cmap, Pp, list, D, qs, v, hist, s); break;
+        case 8: launch_cell_scan_ks<4>(c, grid, coff, choff, cmap, Pp, list, D, qs, v, hist, skip, s); break;
+        case 16: launch_cell_scan_ks<8>(c, grid, coff, choff, cmap, Pp, list, D, qs, v, hist, skip, s); break;
+        case 24: launch_cell_scan_ks<12>(c, grid, coff, choff, cmap, Pp, list, D, qs, v, hist, skip, s); break;
+        case 32: launch_cell_scan_ks<16>(c, grid, coff, choff, cmap, Pp, list, D, qs, v, hist, skip, s); break;
         default: fail(LSH_ERR_INTERNAL, "pivot cells need 8, 16, 24 or 32 sketch words");
     }
 }
 
-struct Lists {
-    uint32_t entries = 0, chunks = 0, work = 0;  // work: tile products (32-query units) at most
-};
-
-// Offsets and chunking of per-cell query lists already counted in cnt;
-// one host synchronisation for their sizes.
-Lists list_offsets(const CellIndex& c, uint32_t* cnt, uint32_t* coff, uint32_t* choff, hipStream_t s) {
-    const int P = c.P;
-    hipLaunchKernelGGL(cell_offsets_kernel, dim3(1), dim3(1024), 0, s, cnt, P, c.ctile.get(), coff, choff);
+void list_offsets(const CellIndex& c, uint32_t* cnt, uint32_t* coff, uint32_t* choff, uint32_t* work_out,
+                  uint32_t* dense, uint32_t dense_limit, hipStream_t s) {
+    hipLaunchKernelGGL(cell_offsets_kernel, dim3(1), dim3(1024), 0, s, cnt, c.P, c.ctile.get(), coff, choff, work_out,
+                       dense, dense_limit);
     check_launch("cell_offsets_kernel");
-    uint32_t tot[3] = {0, 0, 0};
-    LSH_HIP(hipMemcpyAsync(&tot[0], coff + P, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    LSH_HIP(hipMemcpyAsync(&tot[1], choff + P, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    LSH_HIP(hipStreamSynchronize(s));
-    return Lists{tot[0], tot[1], tot[2]};
 }
 
 void place_lists(const CellIndex& c, const uint16_t* D, int Q, int Pp, const uint64_t* thr, const uint16_t* R,
-                 uint32_t* cnt, const uint32_t* coff, int32_t* list, hipStream_t s) {
+                 uint32_t* cnt, const uint32_t* coff, int32_t* list, const uint32_t* skip, hipStream_t s) {
     hipLaunchKernelGGL(cell_need_kernel, need_grid(c.P, Q), dim3(256), 0, s, D, Q, c.P, Pp, c.cmin.get(),
-                       c.cmax.get(), thr, R, cnt, coff, list);
+                       c.cmax.get(), thr, R, cnt, coff, list, skip);
     check_launch("cell_need_kernel");
 }
 
@@ -763,6 +761,8 @@ void CellIndex::build(const uint32_t* ds, int64_t N, int wpad, hipStream_t s) {
     ht[0] = 0;
     for (int j = 0; j < np; ++j) ht[j + 1] = ht[j] + (hs[j + 1] - hs[j] + 31) / 32;
     ntiles = ht[np];
+    nparts = 0;
+    for (int j = 0; j < np; ++j) nparts += (ht[j + 1] - ht[j] + 7) / 8;
     ctile.alloc((size_t)np + 1);
     LSH_HIP(hipMemcpyAsync(ctile.get(), ht.data(), sizeof(uint32_t) * ht.size(), hipMemcpyHostToDevice, s));
     std::vector<uint32_t> hz((size_t)np);
@@ -810,16 +810,17 @@ size_t CellIndex::bytes() const {
            cmax.bytes();
 }
 
-bool cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K, const TopKView& v, Scratch& sc, int depth,
-                 hipStream_t s, int64_t* tiles_out, int32_t* qorder) {
-    if (!c.built() || Q <= 0) return false;
+const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K, const TopKView& v, Scratch& sc,
+                            int depth, hipStream_t s, int32_t* qorder) {
+    if (!c.built() || Q <= 0) return nullptr;
     const int P = c.P;
     const int Pp = (P + 31) / 32 * 32;
     const int bits = 32 * c.words;
     uint16_t* D = sc.get<uint16_t>(depth, 0, (size_t)Q * Pp);
     // meta: cnt1, cnt2 [P+1 each], coff [P+1], choff [P+2], the query sort's
-    // offsets and cursor [P+1 each], nearest cell [Q].
-    uint32_t* meta = sc.get<uint32_t>(depth, 1, 6 * ((size_t)P + 1) + 1 + Q);
+    // offsets and cursor [P+1 each], nearest cell [Q], stat [4] = {second pass
+    // too dense, first-pass tile products, second-pass tile products}.
+    uint32_t* meta = sc.get<uint32_t>(depth, 1, 6 * ((size_t)P + 1) + 1 + Q + 4);
     uint32_t* cnt1 = meta;
     uint32_t* cnt2 = meta + (P + 1);
     uint32_t* coff = meta + 2 * (P + 1);
@@ -827,10 +828,24 @@ bool cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K, const Top
     uint32_t* sort_off = meta + 4 * (P + 1) + 1;
     uint32_t* sort_cur = meta + 5 * (P + 1) + 1;
     uint32_t* nearest = meta + 6 * (P + 1) + 1;
+    uint32_t* stat = nearest + Q;
     const size_t hn = (size_t)Q * kHistBins;
     uint32_t* hist = sc.get<uint32_t>(depth, 3, hn + ((size_t)Q + 1) / 2);
     uint16_t* R = reinterpret_cast<uint16_t*>(hist + hn);
+    // Lists of at most every (query, cell) pair, and chunk maps of at most
+    // every (query group, tile range) pair: no count has to come back to the
+    // host before the scans.
+    const size_t qgroups = ((size_t)Q + 31) / 32;
+    const size_t cap_list = (size_t)Q * P, cap_chunks = qgroups * (size_t)c.nparts;
+    int32_t* list = sc.get<int32_t>(depth, 2, cap_list + cap_chunks);
+    uint32_t* cmap = reinterpret_cast<uint32_t*>(list + cap_list);
+    const unsigned grid = (unsigned)std::max<size_t>(1, std::min<size_t>(4096, (cap_chunks + 3) / 4));
+    // The second pass is "too dense" when it would multiply more than half
+    // of the tiles of a full pass (query tiles x point tiles).
+    const uint32_t dense_limit = (uint32_t)std::min<double>(4e9, 0.5 * (double)qgroups * (double)c.ntiles);
+
     LSH_HIP(hipMemsetAsync(cnt1, 0, sizeof(uint32_t) * 2 * (P + 1), s));
+    LSH_HIP(hipMemsetAsync(stat, 0, sizeof(uint32_t) * 4, s));
     LSH_HIP(hipMemsetAsync(hist, 0, sizeof(uint32_t) * hn, s));
     pivots(c.words, qs, Q, c.pfp4.get(), P, Pp, nullptr, D, s);
     // Threshold bound from the tile ranges, the first-pass radii and their
@@ -845,39 +860,28 @@ bool cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K, const Top
     }
     // First pass: exact distances over the nearest cells, K-th by histogram,
     // then the second pass's cell counts under the final threshold.
-    const Lists l1 = list_offsets(c, cnt1, coff, choff, s);
-    int32_t* list = sc.get<int32_t>(depth, 2, (size_t)l1.entries + l1.chunks + 1);
-    if (l1.chunks) {
-        place_lists(c, D, Q, Pp, nullptr, R, cnt1, coff, list, s);
-        cell_scan(c, l1.chunks, coff, choff, reinterpret_cast<uint32_t*>(list + l1.entries), Pp, list, D, qs, v,
-                  hist, s);
-    }
+    list_offsets(c, cnt1, coff, choff, stat + 1, nullptr, 0, s);
+    place_lists(c, D, Q, Pp, nullptr, R, cnt1, coff, list, nullptr, s);
+    cell_scan(c, grid, coff, choff, cmap, Pp, list, D, qs, v, hist, nullptr, s);
     hipLaunchKernelGGL(hist_threshold_kernel, dim3((unsigned)((Q + 3) / 4)), dim3(256), 0, s, hist, Q, bits, K, v.thr,
                        D, P, Pp, c.cmin.get(), c.cmax.get(), cnt2);
     check_launch("hist_threshold_kernel");
-    // Second pass: the tiles the bounds leave.
-    const Lists l2 = list_offsets(c, cnt2, coff, choff, s);
+    // Second pass: the tiles the bounds leave (skipped when too dense).
+    list_offsets(c, cnt2, coff, choff, stat + 2, stat, dense_limit, s);
+    place_lists(c, D, Q, Pp, v.thr, nullptr, cnt2, coff, list, stat, s);
+    cell_scan(c, grid, coff, choff, cmap, Pp, list, D, qs, v, nullptr, stat, s);
     if (debug_cells()) {
         std::vector<uint64_t> ht((size_t)Q);
-        LSH_HIP(hipMemcpy(ht.data(), v.thr, sizeof(uint64_t) * Q, hipMemcpyDeviceToHost));
+        uint32_t hs[3];
+        LSH_HIP(hipMemcpyAsync(ht.data(), v.thr, sizeof(uint64_t) * Q, hipMemcpyDeviceToHost, s));
+        LSH_HIP(hipMemcpyAsync(hs, stat, sizeof(hs), hipMemcpyDeviceToHost, s));
+        LSH_HIP(hipStreamSynchronize(s));
         double mt = 0;
         for (int q = 0; q < Q; ++q) mt += ht[q] == kEmptyKey ? 2.0 * bits : (double)(ht[q] >> 32);
-        std::fprintf(stderr,
-                     "cells: P=%d tiles=%lld Q=%d K=%d mean T=%.1f first pass %u entries %u chunks, second %u "
-                     "entries %u chunks\n",
-                     P, (long long)c.ntiles, Q, K, mt / Q, l1.entries, l1.chunks, l2.entries, l2.chunks);
+        std::fprintf(stderr, "cells: P=%d tiles=%lld Q=%d K=%d mean T=%.1f tile products %u + %u%s\n", P,
+                     (long long)c.ntiles, Q, K, mt / Q, hs[1], hs[2], hs[0] ? " (too dense)" : "");
     }
-    // Work in tile products (32-query units) against the full pass (query
-    // tiles x point tiles): too dense, leave it to the full pass.
-    const double full = std::ceil(Q / 32.0) * (double)c.ntiles;
-    if ((double)l2.work > 0.5 * full) return false;
-    if (tiles_out) *tiles_out += (int64_t)l1.work + (int64_t)l2.work;
-    if (l2.chunks == 0) return true;
-    list = sc.get<int32_t>(depth, 2, (size_t)l2.entries + l2.chunks);
-    place_lists(c, D, Q, Pp, v.thr, nullptr, cnt2, coff, list, s);
-    cell_scan(c, l2.chunks, coff, choff, reinterpret_cast<uint32_t*>(list + l2.entries), Pp, list, D, qs, v,
-              nullptr, s);
-    return true;
+    return stat;
 }
 
 }  // namespace lshgpu

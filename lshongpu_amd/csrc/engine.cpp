@@ -52,10 +52,13 @@ bool force_full_pass() {
     return on;
 }
 
-// Indices of the nonzero entries of a device flag array.
-std::vector<int32_t> flagged(const uint32_t* dflags, int Q, hipStream_t s) {
+// Indices of the nonzero entries of a device flag array (and, with extra,
+// n_extra more device words copied in the same synchronisation).
+std::vector<int32_t> flagged(const uint32_t* dflags, int Q, hipStream_t s, const uint32_t* extra = nullptr,
+                             uint32_t* h_extra = nullptr, int n_extra = 0) {
     std::vector<uint32_t> h((size_t)Q);
     LSH_HIP(hipMemcpyAsync(h.data(), dflags, sizeof(uint32_t) * (size_t)Q, hipMemcpyDeviceToHost, s));
+    if (extra) LSH_HIP(hipMemcpyAsync(h_extra, extra, sizeof(uint32_t) * n_extra, hipMemcpyDeviceToHost, s));
     LSH_HIP(hipStreamSynchronize(s));
     std::vector<int32_t> idx;
     for (int i = 0; i < Q; ++i)
@@ -192,21 +195,8 @@ void sketch_topk(const SketchSet& d, const SketchSet* samples, int nsamples, int
         launch_scatter_rows_u64(sv.keys, sv.stride, K, idx, m, v.keys, v.stride, s);
     };
     if (mfma && mode == kNormal && nsamples > 0 && samples[0].N > 0 && d.fp4 && int_keys) {
-        st.reset(Q, K, 2 * K + 512, s, true);
-        int64_t tiles = 0;
-        bool pruned = false;
-        if (d.cells && !force_full_pass()) {
-            // Threshold from the nearest pivot cells, then the cells the
-            // triangle inequality leaves.
-            pt.begin(PH_SCAN, s);
-            pt.begin(PH_MFMA, s);
-            pruned = cell_search(*d.cells, qs, Q, K, st.view(), sc, depth, s, &tiles, d.qorder);
-            if (d.ordered) *d.ordered = d.qorder != nullptr;
-            pt.end(PH_MFMA, s);
-            pt.end(PH_SCAN, s);
-        }
-        if (!pruned) {
-            // Threshold from the sample, then one pass over all points.
+        // Threshold from the sample, then one pass over all points.
+        auto sampled_pass = [&]() {
             const int Ks = sample_k(K, sample_step(d.N, samples[0].N));
             TopKState& sa = sc.tk[depth];
             sketch_topk(samples[0], samples + 1, nsamples - 1, wpad, kind, qs, Q, Ks, kNormal, sa, sc, depth + 1, s,
@@ -217,20 +207,49 @@ void sketch_topk(const SketchSet& d, const SketchSet* samples, int nsamples, int
             launch_hamming_mfma(d.fp4, qs, wpad, Q, 0, d.N, st.view(), s);
             pt.end(PH_MFMA, s);
             pt.end(PH_SCAN, s);
+            ++launches;
+            if (mfma_launches) ++*mfma_launches;
+            if (mfma_points) *mfma_points += d.N;
+        };
+        auto select_and_flag = [&](const uint32_t* extra, uint32_t* h_extra, int n_extra) {
+            pt.begin(PH_SELECT, s);
+            launch_select_merge(st.view(), s);
+            pt.end(PH_SELECT, s);
+            uint32_t* flags = sc.get<uint32_t>(depth, 0, (size_t)Q);
+            launch_incomplete_flags(st.view(), flags, s);
+            return flagged(flags, Q, s, extra, h_extra, n_extra);
+        };
+        st.reset(Q, K, 2 * K + 512, s, true);
+        const uint32_t* cstat = nullptr;
+        if (d.cells && !force_full_pass()) {
+            // Threshold from the nearest pivot cells, then the cells the
+            // triangle inequality leaves.
+            pt.begin(PH_SCAN, s);
+            pt.begin(PH_MFMA, s);
+            cstat = cell_search(*d.cells, qs, Q, K, st.view(), sc, depth, s, d.qorder);
+            pt.end(PH_MFMA, s);
+            pt.end(PH_SCAN, s);
+            if (cstat && d.ordered) *d.ordered = d.qorder != nullptr;
         }
-        ++launches;
-        if (mfma_launches) ++*mfma_launches;
-        // Pairs multiplied per query: the cells' tile products cover 32 x 32 pairs.
-        if (mfma_points) *mfma_points += pruned ? tiles * 1024 / std::max(Q, 1) : d.N;
-        pt.begin(PH_SELECT, s);
-        launch_select_merge(st.view(), s);
-        pt.end(PH_SELECT, s);
-        uint32_t* flags = sc.get<uint32_t>(depth, 0, (size_t)Q);
-        launch_incomplete_flags(st.view(), flags, s);
-        const std::vector<int32_t> bad = flagged(flags, Q, s);
+        if (!cstat) sampled_pass();
+        uint32_t hstat[3] = {0, 0, 0};
+        std::vector<int32_t> bad = select_and_flag(cstat, hstat, cstat ? 3 : 0);
+        if (cstat) {
+            if (hstat[0]) {
+                // Too dense for the cells: nothing was appended.
+                st.reset(Q, K, 2 * K + 512, s, true);
+                sampled_pass();
+                bad = select_and_flag(nullptr, nullptr, 0);
+            } else {
+                ++launches;
+                if (mfma_launches) ++*mfma_launches;
+                // Pairs multiplied per query: the cells' tile products cover 32 x 32 pairs.
+                if (mfma_points) *mfma_points += ((int64_t)hstat[1] + hstat[2]) * 1024 / std::max(Q, 1);
+            }
+        }
         if (std::getenv("LSHGPU_DEBUG_CELLS"))
-            std::fprintf(stderr, "threshold pass (%s): %zu of %d queries redone\n", pruned ? "cells" : "full",
-                         bad.size(), Q);
+            std::fprintf(stderr, "threshold pass (%s): %zu of %d queries redone\n",
+                         cstat && !hstat[0] ? "cells" : "full", bad.size(), Q);
         if (!bad.empty()) redo(bad, kCautious);
         return;
     }

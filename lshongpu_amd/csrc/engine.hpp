@@ -82,6 +82,7 @@ struct CellIndex {
     int P = 0;       // pivots; 0 when no cells were built
     int words = 0;   // sketch words per row (8, 16, 24 or 32)
     int64_t ntiles = 0;
+    int64_t nparts = 0;            // sum over cells of ceil(tiles / 8): the tile ranges of a chunk list
     DevBuf<uint32_t> pfp4;         // FP4 image of the pivot sketches
     DevBuf<uint32_t> sk;           // [ntiles*32][W] packed sketches in cell order, even words then odd
     DevBuf<int32_t> ids;           // [ntiles*32] data row of each position, -1 for padding
@@ -101,14 +102,15 @@ void cell_rows(const CellIndex& c, const float* data, int64_t N, int d, DevBuf<f
 
 // Threshold search over the cells (Q query rows of qs, top-K): sets v.thr
 // to a valid threshold from each query's nearest cells, then appends every
-// pair below it to v, as launch_hamming_mfma over all points would.  Returns
-// false, having appended nothing (v.thr stays valid), when the thresholds
-// leave too many tiles for pruning to pay.  Uses scratch slots 0-3 of
-// `depth`.  tiles_out (may be null) accumulates the tiles multiplied, in
-// 32-query units.  qorder (may be null; Q entries) receives the queries
-// grouped by their nearest cell.
-bool cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K, const TopKView& v, Scratch& sc, int depth,
-                 hipStream_t s, int64_t* tiles_out, int32_t* qorder);
+// pair below it to v, as launch_hamming_mfma over all points would.  Nothing
+// waits on the host.  Returns a device array stat[3]: stat[0] = 1 when the
+// thresholds left too many tiles for pruning to pay and nothing was appended
+// (v.thr is still valid; the caller then runs the full pass); stat[1] and
+// stat[2] = the tile products of the two passes (32-query units), an upper
+// bound.  Uses scratch slots 0-3 of `depth`.  qorder (may be null; Q
+// entries) receives the queries grouped by their nearest cell.
+const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K, const TopKView& v, Scratch& sc,
+                            int depth, hipStream_t s, int32_t* qorder);
 
 }  // namespace lshgpu
 

@@ -19,8 +19,61 @@ namespace {
 
 constexpr int TPB = 256;
 
+// SimHash of 64 rows per workgroup: the rows are staged in LDS with
+// coalesced reads (row stride d + 1, so that the lanes' reads of one
+// dimension spread over the banks), and each of the 4 waves computes one
+// word (blockIdx.y * 4 + wave) of all 64 rows, one row per lane.  The
+// hyperplane values are wave-uniform (scalar operands).
+constexpr int kStageRows = 64;
+__global__ __launch_bounds__(256) void simhash_staged_kernel(const float* __restrict__ x, int64_t n, int d,
+                                                             const float* __restrict__ pT, int words, int hb,
+                                                             uint32_t* __restrict__ out, int ostride) {
+    extern __shared__ float xs[];  // [kStageRows][d + 1]
+    const int64_t i0 = (int64_t)blockIdx.x * kStageRows;
+    const int rows = (int)min<int64_t>(kStageRows, n - i0);
+    const int ld = d + 1;
+    for (int e = threadIdx.x; e < rows * d; e += 256) {
+        const int r = e / d, j = e - r * d;
+        xs[r * ld + j] = x[(size_t)i0 * d + e];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    // The wave's word, made provably uniform so the hyperplane reads are scalar.
+    const int w = __builtin_amdgcn_readfirstlane(blockIdx.y * 4 + (threadIdx.x >> 6));
+    if (w >= words || lane >= rows) return;
+    const float* __restrict__ xr = xs + lane * ld;
+    float acc[32];
+#pragma unroll
+    for (int b = 0; b < 32; ++b) acc[b] = 0.0f;
+    const float* __restrict__ p = pT + (size_t)w * d * hb;
+    if (hb == 32) {
+        // Full words: the 32 hyperplane values of a dimension are one
+        // contiguous uniform block (wide scalar loads, no per-bit tests).
+        for (int j = 0; j < d; ++j) {
+            const float xj = xr[j];
+            const float* __restrict__ pj = p + (size_t)j * 32;
+#pragma unroll
+            for (int b = 0; b < 32; ++b) acc[b] = __builtin_fmaf(xj, pj[b], acc[b]);
+        }
+    } else {
+        for (int j = 0; j < d; ++j) {
+            const float xj = xr[j];
+            const float* __restrict__ pj = p + (size_t)j * hb;
+#pragma unroll
+            for (int b = 0; b < 32; ++b)
+                if (b < hb) acc[b] = __builtin_fmaf(xj, pj[b], acc[b]);
+        }
+    }
+    uint32_t sk = 0;
+#pragma unroll
+    for (int b = 0; b < 32; ++b)
+        if (b < hb) sk |= (acc[b] >= 0.0f ? 1u : 0u) << b;
+    out[(size_t)(i0 + lane) * ostride + w] = sk;
+}
+
 // One lane per (point, word); blockIdx.y is the word, so the hyperplane
-// values a wave reads are uniform across it.
+// values a wave reads are uniform across it.  Used when a staged block of
+// rows does not fit the LDS.
 __global__ __launch_bounds__(TPB) void simhash_kernel(const float* __restrict__ x, int64_t n, int d,
                                                       const float* __restrict__ pT, int words, int hb,
                                                       uint32_t* __restrict__ out, int ostride) {
@@ -157,6 +210,13 @@ void launch_simhash(const float* x, int64_t n, int d, const float* planesT, int 
     if (n <= 0) return;
     LSH_REQUIRE(hashBits >= 1 && hashBits <= 32, LSH_ERR_INVALID, "hashBits must be in 1..32");
     LSH_REQUIRE(words <= 65535, LSH_ERR_UNSUPPORTED, "at most 65535 sketch words");
+    const size_t lds = sizeof(float) * kStageRows * (size_t)(d + 1);
+    if (lds <= 64 * 1024) {
+        hipLaunchKernelGGL(simhash_staged_kernel, dim3(blocks_for(n, kStageRows), (words + 3) / 4), dim3(256), lds, s,
+                           x, n, d, planesT, words, hashBits, out, out_stride);
+        check_launch("simhash_staged_kernel");
+        return;
+    }
     hipLaunchKernelGGL(simhash_kernel, dim3(blocks_for(n, TPB), words), dim3(TPB), 0, s, x, n, d, planesT,
                        words, hashBits, out, out_stride);
     check_launch("simhash_kernel");
