@@ -270,7 +270,9 @@ __device__ int kth_bin(const uint32_t* hist, int nbins, uint32_t K) {
 //    are counted into cnt1 for the first pass;
 //  - the K-th smallest upper bound D + hi_t (weighted by the tile's rows)
 //    over the tiles of the cells within R + kRadiusMargin: thr = (bound + 1,
-//    0).
+//    0);
+//  - per query, the lists of its first-pass cells (qcells1, n1) and of the
+//    cells whose bound is below that threshold (qcells2, n2; rows of P).
 constexpr int kRadiusMargin = 16;
 constexpr int kWindow = 256;
 __global__ __launch_bounds__(256) void cell_threshold_kernel(const uint16_t* __restrict__ D, int Q, int P, int Pp,
@@ -279,18 +281,27 @@ __global__ __launch_bounds__(256) void cell_threshold_kernel(const uint16_t* __r
                                                              const uint32_t* __restrict__ tinfo, int bits, int K,
                                                              uint64_t* __restrict__ thr, uint16_t* __restrict__ R,
                                                              uint32_t* __restrict__ nearest,
-                                                             uint32_t* __restrict__ cnt1) {
+                                                             uint32_t* __restrict__ cnt1,
+                                                             const uint16_t* __restrict__ cmin,
+                                                             const uint16_t* __restrict__ cmax,
+                                                             int32_t* __restrict__ qcells1, uint32_t* __restrict__ n1,
+                                                             int32_t* __restrict__ qcells2,
+                                                             uint32_t* __restrict__ n2) {
     __shared__ uint32_t h1[kWindow];
     __shared__ uint32_t h2[2049];
     __shared__ int rad;
-    __shared__ uint32_t best;
+    __shared__ uint32_t best, s_n1, s_n2, s_h;
     const int q = blockIdx.x;
     const uint16_t* __restrict__ row = D + (size_t)q * Pp;
     for (int b = threadIdx.x; b < 2049; b += 256) {
         if (b < kWindow) h1[b] = 0;
         h2[b] = 0;
     }
-    if (threadIdx.x == 0) best = 0xFFFFFFFFu;
+    if (threadIdx.x == 0) {
+        best = 0xFFFFFFFFu;
+        s_n1 = 0;
+        s_n2 = 0;
+    }
     __syncthreads();
     uint32_t mine = 0xFFFFFFFFu;  // (distance, pivot) of the nearest non-empty cell
     for (int j = threadIdx.x; j < P; j += 256)
@@ -319,7 +330,10 @@ __global__ __launch_bounds__(256) void cell_threshold_kernel(const uint16_t* __r
     for (int j = threadIdx.x; j < P; j += 256) {
         const int dq = row[j];
         if (dq > rm || csize[j] == 0) continue;
-        if (dq <= r0) atomicAdd(&cnt1[j], 1u);
+        if (dq <= r0) {
+            atomicAdd(&cnt1[j], 1u);
+            qcells1[(size_t)q * P + atomicAdd(&s_n1, 1u)] = j;
+        }
         for (uint32_t t = ctile[j]; t < ctile[j + 1]; ++t) {
             const uint32_t info = tinfo[t];
             atomicAdd(&h2[dq + ((info >> kKeyShift) & kDistMask)], info >> 22);
@@ -328,29 +342,23 @@ __global__ __launch_bounds__(256) void cell_threshold_kernel(const uint16_t* __r
     __syncthreads();
     if (threadIdx.x < 64) {
         const int ub = kth_bin(h2, 2 * bits + 1, (uint32_t)K);
-        if (threadIdx.x == 0) thr[q] = ub > 2 * bits ? kEmptyKey : make_key((uint32_t)ub + 1u, 0u);
+        if (threadIdx.x == 0) {
+            thr[q] = ub > 2 * bits ? kEmptyKey : make_key((uint32_t)ub + 1u, 0u);
+            s_h = ub > 2 * bits ? 0xFFFFFFFFu : (uint32_t)ub + 1u;
+        }
     }
-}
-
-// Per-cell query lists.  R == nullptr: the queries whose bound on the cell
-// is below their threshold distance; otherwise those whose distance to the
-// cell's pivot is at most their radius (the first pass).  list == nullptr:
-// count into cnt; otherwise place them.  A set skip flag leaves the lists
-// empty.  Grid: x over cells, y (strided) over queries.
-__global__ void cell_need_kernel(const uint16_t* __restrict__ D, int Q, int P, int Pp,
-                                 const uint16_t* __restrict__ cmin, const uint16_t* __restrict__ cmax,
-                                 const uint64_t* __restrict__ thr, const uint16_t* __restrict__ R,
-                                 uint32_t* __restrict__ cnt, const uint32_t* __restrict__ coff,
-                                 int32_t* __restrict__ list, const uint32_t* __restrict__ skip) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= P || (skip && *skip)) return;
-    const uint32_t lo = cmin[j], hi = cmax[j];
-    if (lo > hi) return;
-    for (int q = blockIdx.y; q < Q; q += gridDim.y) {
-        const uint32_t dq = D[(size_t)q * Pp + j];
-        if (R ? dq > R[q] : lower_bound_to(dq, lo, hi) >= (uint32_t)(thr[q] >> 32)) continue;
-        const uint32_t slot = atomicAdd(&cnt[j], 1u);
-        if (list) list[coff[j] + slot] = q;
+    __syncthreads();
+    // The cells the second pass can need: the final threshold is at most this
+    // one, so the cells whose bound is below it are a superset.
+    const uint32_t h = s_h;
+    for (int j = threadIdx.x; j < P; j += 256) {
+        const uint32_t lo = cmin[j], hi = cmax[j];
+        if (lo <= hi && lower_bound_to(row[j], lo, hi) < h) qcells2[(size_t)q * P + atomicAdd(&s_n2, 1u)] = j;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        n1[q] = s_n1;
+        n2[q] = s_n2;
     }
 }
 
@@ -591,14 +599,17 @@ __global__ __launch_bounds__(256) void cell_scan_kernel(const uint32_t* __restri
 
 // thr[q] = min(thr[q], (T + 1, 0)) with T the upper edge of the bin at which
 // the query's first-pass counts reach K (the last bin's edge: bits); then the
-// cells whose bound is below the final threshold are counted into cnt2 for
-// the second pass.  One wave per query.
+// query's candidate cells (qcells2) whose bound is below the final threshold
+// are kept (compacted in place, n2 updated) and counted into cnt2 for the
+// second pass.  One wave per query.
 __global__ __launch_bounds__(256) void hist_threshold_kernel(const uint32_t* __restrict__ hist, int Q, int bits,
                                                              int K, uint64_t* __restrict__ thr,
                                                              const uint16_t* __restrict__ D, int P, int Pp,
                                                              const uint16_t* __restrict__ cmin,
                                                              const uint16_t* __restrict__ cmax,
-                                                             uint32_t* __restrict__ cnt2) {
+                                                             uint32_t* __restrict__ cnt2,
+                                                             int32_t* __restrict__ qcells2,
+                                                             uint32_t* __restrict__ n2) {
     const int q = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (q >= Q) return;
     const int lane = threadIdx.x & 63;
@@ -612,9 +623,39 @@ __global__ __launch_bounds__(256) void hist_threshold_kernel(const uint32_t* __r
     if (lane == 0) thr[q] = t;
     const uint32_t h = (uint32_t)(t >> 32);
     const uint16_t* __restrict__ row = D + (size_t)q * Pp;
-    for (int j = lane; j < P; j += 64) {
-        const uint32_t lo = cmin[j], hi = cmax[j];
-        if (lo <= hi && lower_bound_to(row[j], lo, hi) < h) atomicAdd(&cnt2[j], 1u);
+    int32_t* __restrict__ cells = qcells2 + (size_t)q * P;
+    const uint32_t n = n2[q];
+    uint32_t kept = 0;
+    for (uint32_t i0 = 0; i0 < n; i0 += 64) {
+        const uint32_t i = i0 + lane;
+        const int32_t j = i < n ? cells[i] : -1;
+        const bool keep = j >= 0 && lower_bound_to(row[j], cmin[j], cmax[j]) < h;
+        const uint64_t m = __ballot(keep);
+        if (keep) {
+            atomicAdd(&cnt2[j], 1u);
+            cells[kept + __popcll(m & ((1ull << lane) - 1ull))] = j;
+        }
+        kept += (uint32_t)__popcll(m);
+    }
+    if (lane == 0) n2[q] = kept;
+}
+
+// Per-cell query lists from the per-query cell lists: query q goes to slot
+// cursor[j]++ of cell j's list for each of its cells.  One wave per query;
+// a set skip flag leaves the lists empty.
+__global__ __launch_bounds__(256) void place_from_cells_kernel(const int32_t* __restrict__ qcells,
+                                                               const uint32_t* __restrict__ nq, int Q, int P,
+                                                               uint32_t* __restrict__ cursor,
+                                                               const uint32_t* __restrict__ coff,
+                                                               int32_t* __restrict__ list,
+                                                               const uint32_t* __restrict__ skip) {
+    const int q = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= Q || (skip && *skip)) return;
+    const int lane = threadIdx.x & 63;
+    const uint32_t n = nq[q];
+    for (uint32_t i = lane; i < n; i += 64) {
+        const int32_t j = qcells[(size_t)q * P + i];
+        list[coff[j] + atomicAdd(&cursor[j], 1u)] = q;
     }
 }
 
@@ -646,9 +687,6 @@ void pivots(int words, const uint32_t* rows, int64_t n, const uint32_t* pfrag, i
     }
 }
 
-dim3 need_grid(int P, int Q) {
-    return dim3((unsigned)((P + 255) / 256), (unsigned)std::max(1, std::min(Q, 4096)));
-}
 
 template <int KS>
 void launch_cell_scan_ks(const CellIndex& c, unsigned grid, const uint32_t* coff, const uint32_t* choff,
@@ -686,12 +724,6 @@ void list_offsets(const CellIndex& c, uint32_t* cnt, uint32_t* coff, uint32_t* c
     check_launch("cell_offsets_kernel");
 }
 
-void place_lists(const CellIndex& c, const uint16_t* D, int Q, int Pp, const uint64_t* thr, const uint16_t* R,
-                 uint32_t* cnt, const uint32_t* coff, int32_t* list, const uint32_t* skip, hipStream_t s) {
-    hipLaunchKernelGGL(cell_need_kernel, need_grid(c.P, Q), dim3(256), 0, s, D, Q, c.P, Pp, c.cmin.get(),
-                       c.cmax.get(), thr, R, cnt, coff, list, skip);
-    check_launch("cell_need_kernel");
-}
 
 }  // namespace
 
@@ -710,10 +742,14 @@ void CellIndex::build(const uint32_t* ds, int64_t N, int wpad, hipStream_t s) {
     words = wpad;
     // Pivots: every (N/np)-th row.  A group of points that drew no pivot of
     // its own ends up far from the pivot it joins and widens that cell's
-    // range; so points farther than twice the median distance from their
-    // pivot seed more pivots (a stride over them, up to a quarter of the
-    // pivots a round), and the points are assigned again, until at most one
-    // point in a thousand is far (or six rounds, or kMaxPivots).
+    // range.  So a point is "far" beyond the smaller of twice the median and
+    // three times the 10th percentile of the distances to the pivots (and
+    // beyond bits/16); far points seed more pivots (a stride over them, up to
+    // a quarter of the pivots a round), and the points are assigned again,
+    // until at most one point in a thousand is far (or eight rounds, or
+    // kMaxPivots).  The median keeps well-seeded data from drawing pivots for
+    // the tails of its groups; the low percentile keeps the rule working when
+    // most points start without a near pivot (the median is then far too).
     const int bits = 32 * wpad;
     std::vector<int32_t> prow((size_t)np);
     const int64_t step = N / np;
@@ -729,21 +765,26 @@ void CellIndex::build(const uint32_t* ds, int64_t N, int wpad, hipStream_t s) {
         launch_expand_fp4(psk.get(), n, wpad, pfp4.get(), s);
         // Nearest pivot of every point.
         pivots(wpad, ds, N, pfp4.get(), n, 0, ckey.get(), nullptr, s);
-        if (round == 6 || n >= kMaxPivots) break;
+        if (round == 8 || n >= kMaxPivots) break;
         std::vector<uint32_t> hk((size_t)N);
         LSH_HIP(hipMemcpyAsync(hk.data(), ckey.get(), sizeof(uint32_t) * N, hipMemcpyDeviceToHost, s));
         LSH_HIP(hipStreamSynchronize(s));
         std::vector<int64_t> dh((size_t)bits + 1, 0);
         for (uint32_t k : hk) ++dh[k & kDistMask];
         int64_t acc = 0;
-        int median = 0;
-        while (median < bits && (acc += dh[median]) * 2 < N) ++median;
-        std::vector<int32_t> far;
+        int p10 = -1, med = -1;
+        for (int b = 0; b <= bits && med < 0; ++b) {
+            acc += dh[b];
+            if (p10 < 0 && acc * 10 >= N) p10 = b;
+            if (acc * 2 >= N) med = b;
+        }
+        const int far_at = std::max(std::min(2 * med, 3 * p10), bits / 16);
+        std::vector<int32_t> add;
         for (int64_t i = 0; i < N; ++i)
-            if ((int)(hk[i] & kDistMask) > 2 * median) far.push_back((int32_t)i);
-        if ((int64_t)far.size() * 1000 <= N) break;
-        const size_t add = std::min<size_t>({far.size(), (size_t)(kMaxPivots - n), (size_t)std::max(1, n / 4)});
-        for (size_t i = 0; i < add; ++i) prow.push_back(far[i * far.size() / add]);
+            if ((int)(hk[i] & kDistMask) > far_at) add.push_back((int32_t)i);
+        if ((int64_t)add.size() * 1000 <= N) break;
+        const size_t take = std::min<size_t>({add.size(), (size_t)(kMaxPivots - n), (size_t)std::max(1, n / 4)});
+        for (size_t i = 0; i < take; ++i) prow.push_back(add[i * add.size() / take]);
     }
     np = (int)prow.size();
 
@@ -829,9 +870,15 @@ const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K
     uint32_t* sort_cur = meta + 5 * (P + 1) + 1;
     uint32_t* nearest = meta + 6 * (P + 1) + 1;
     uint32_t* stat = nearest + Q;
-    const size_t hn = (size_t)Q * kHistBins;
-    uint32_t* hist = sc.get<uint32_t>(depth, 3, hn + ((size_t)Q + 1) / 2);
-    uint16_t* R = reinterpret_cast<uint16_t*>(hist + hn);
+    // hist [Q][kHistBins], the per-query cell lists [Q][P] x 2 and their
+    // lengths [Q] x 2, the radii [Q] (u16).
+    const size_t hn = (size_t)Q * kHistBins, qp = (size_t)Q * P;
+    uint32_t* hist = sc.get<uint32_t>(depth, 3, hn + 2 * qp + 2 * (size_t)Q + ((size_t)Q + 1) / 2);
+    int32_t* qcells1 = reinterpret_cast<int32_t*>(hist + hn);
+    int32_t* qcells2 = qcells1 + qp;
+    uint32_t* n1 = reinterpret_cast<uint32_t*>(qcells2 + qp);
+    uint32_t* n2 = n1 + Q;
+    uint16_t* R = reinterpret_cast<uint16_t*>(n2 + Q);
     // Lists of at most every (query, cell) pair, and chunk maps of at most
     // every (query group, tile range) pair: no count has to come back to the
     // host before the scans.
@@ -851,7 +898,8 @@ const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K
     // Threshold bound from the tile ranges, the first-pass radii and their
     // cell counts, and the nearest cell of each query.
     hipLaunchKernelGGL(cell_threshold_kernel, dim3((unsigned)Q), dim3(256), 0, s, D, Q, P, Pp, c.csize.get(),
-                       c.ctile.get(), c.tinfo.get(), bits, K, v.thr, R, nearest, cnt1);
+                       c.ctile.get(), c.tinfo.get(), bits, K, v.thr, R, nearest, cnt1, c.cmin.get(), c.cmax.get(),
+                       qcells1, n1, qcells2, n2);
     check_launch("cell_threshold_kernel");
     if (qorder) {
         // Queries grouped by nearest cell (a counting sort): neighbouring
@@ -860,15 +908,20 @@ const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K
     }
     // First pass: exact distances over the nearest cells, K-th by histogram,
     // then the second pass's cell counts under the final threshold.
+    const unsigned qwaves = (unsigned)((Q + 3) / 4);
     list_offsets(c, cnt1, coff, choff, stat + 1, nullptr, 0, s);
-    place_lists(c, D, Q, Pp, nullptr, R, cnt1, coff, list, nullptr, s);
+    hipLaunchKernelGGL(place_from_cells_kernel, dim3(qwaves), dim3(256), 0, s, qcells1, n1, Q, P, cnt1, coff, list,
+                       nullptr);
+    check_launch("place_from_cells_kernel");
     cell_scan(c, grid, coff, choff, cmap, Pp, list, D, qs, v, hist, nullptr, s);
-    hipLaunchKernelGGL(hist_threshold_kernel, dim3((unsigned)((Q + 3) / 4)), dim3(256), 0, s, hist, Q, bits, K, v.thr,
-                       D, P, Pp, c.cmin.get(), c.cmax.get(), cnt2);
+    hipLaunchKernelGGL(hist_threshold_kernel, dim3(qwaves), dim3(256), 0, s, hist, Q, bits, K, v.thr, D, P, Pp,
+                       c.cmin.get(), c.cmax.get(), cnt2, qcells2, n2);
     check_launch("hist_threshold_kernel");
     // Second pass: the tiles the bounds leave (skipped when too dense).
     list_offsets(c, cnt2, coff, choff, stat + 2, stat, dense_limit, s);
-    place_lists(c, D, Q, Pp, v.thr, nullptr, cnt2, coff, list, stat, s);
+    hipLaunchKernelGGL(place_from_cells_kernel, dim3(qwaves), dim3(256), 0, s, qcells2, n2, Q, P, cnt2, coff, list,
+                       stat);
+    check_launch("place_from_cells_kernel");
     cell_scan(c, grid, coff, choff, cmap, Pp, list, D, qs, v, nullptr, stat, s);
     if (debug_cells()) {
         std::vector<uint64_t> ht((size_t)Q);

@@ -7,6 +7,8 @@
 // geometrically so that each round is expected to append about half of the
 // append capacity per query; a query whose appends overflow is re-run on its
 // own with ranges no larger than the capacity, which cannot overflow.
+#include <functional>
+
 #include "engine.hpp"
 
 #include <algorithm>
@@ -141,6 +143,16 @@ struct SketchSet {
     const CellIndex* cells = nullptr;  // pivot cells of ds, for the threshold pass
     int32_t* qorder = nullptr;         // out: queries grouped by nearest cell, when the cells ran
     bool* ordered = nullptr;           // out: qorder was written
+    // Work to enqueue once the threshold pass and its select are enqueued,
+    // before the host waits for the incomplete-query flags (the rerank, so
+    // that it does not wait on the host).  `ran` tells whether it was
+    // enqueued over the final candidates; `redone` lists the queries whose
+    // candidates changed afterwards.
+    struct Hook {
+        std::function<void()> fn;
+        bool ran = false;
+        std::vector<int32_t> redone;
+    }* hook = nullptr;
 };
 
 // Rows of a level per row of its sample.
@@ -217,6 +229,10 @@ void sketch_topk(const SketchSet& d, const SketchSet* samples, int nsamples, int
             pt.end(PH_SELECT, s);
             uint32_t* flags = sc.get<uint32_t>(depth, 0, (size_t)Q);
             launch_incomplete_flags(st.view(), flags, s);
+            if (d.hook && d.hook->fn) {
+                d.hook->fn();
+                d.hook->ran = true;
+            }
             return flagged(flags, Q, s, extra, h_extra, n_extra);
         };
         st.reset(Q, K, 2 * K + 512, s, true);
@@ -239,7 +255,7 @@ void sketch_topk(const SketchSet& d, const SketchSet* samples, int nsamples, int
                 // Too dense for the cells: nothing was appended.
                 st.reset(Q, K, 2 * K + 512, s, true);
                 sampled_pass();
-                bad = select_and_flag(nullptr, nullptr, 0);
+                bad = select_and_flag(nullptr, nullptr, 0);  // runs the hook again over the new candidates
             } else {
                 ++launches;
                 if (mfma_launches) ++*mfma_launches;
@@ -250,7 +266,10 @@ void sketch_topk(const SketchSet& d, const SketchSet* samples, int nsamples, int
         if (std::getenv("LSHGPU_DEBUG_CELLS"))
             std::fprintf(stderr, "threshold pass (%s): %zu of %d queries redone\n",
                          cstat && !hstat[0] ? "cells" : "full", bad.size(), Q);
-        if (!bad.empty()) redo(bad, kCautious);
+        if (!bad.empty()) {
+            redo(bad, kCautious);
+            if (d.hook) d.hook->redone = bad;
+        }
         return;
     }
     auto scan = [&](int64_t p0, int64_t p1, const TopKView& v) {
@@ -727,21 +746,37 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
         LSH_REQUIRE(K <= kMaxTopK, LSH_ERR_UNSUPPORTED, "candidate sets larger than 4096 are not supported");
         bool ordered = false;
         if (cells.built()) qorder.ensure((size_t)Q);
-        const SketchSet full{dsk.get(), dfp4.get(), N, &cells, cells.built() ? qorder.get() : nullptr, &ordered};
+        const int kind = impl == 3 ? kAngular : kGenJaccard;
+        // The rerank of queries `order` (all queries when null; m of them).
+        auto rerank = [&](const int32_t* order, int m) {
+            pt.begin(PH_FINAL, s);
+            if (cells.built())
+                launch_rerank(cdata.get(), q.get(), d, kind, st.view().keys, st.view().stride, nullptr, Ceff, m, kq,
+                              kq + margin, fp32_error_bound(kind, d), out.get(), s, order, cpos.get());
+            else
+                launch_rerank(data.get(), q.get(), d, kind, st.view().keys, st.view().stride, nullptr, Ceff, m, kq,
+                              kq + margin, fp32_error_bound(kind, d), out.get(), s, order);
+            pt.end(PH_FINAL, s);
+        };
+        SketchSet::Hook hook;
+        hook.fn = [&] { rerank(ordered ? qorder.get() : nullptr, Q); };
+        const SketchSet full{dsk.get(), dfp4.get(), N, &cells, cells.built() ? qorder.get() : nullptr, &ordered,
+                             &hook};
         const std::vector<SketchSet> sv = sample_views(samples);
         sketch_topk(full, sv.data(), (int)sv.size(), wpad, impl == 4 ? 1 : 0, qsk.get(), Q, K, kNormal, st,
                     scratch, 0, s, pt, scan_launches,
                     &mfma_launches, &mfma_points);
-        const int kind = impl == 3 ? kAngular : kGenJaccard;
-        pt.begin(PH_FINAL, s);
-        if (cells.built())
-            launch_rerank(cdata.get(), q.get(), d, kind, st.view().keys, st.view().stride, nullptr, Ceff, Q, kq,
-                          kq + margin, fp32_error_bound(kind, d), out.get(), s, ordered ? qorder.get() : nullptr,
-                          cpos.get());
-        else
-            launch_rerank(data.get(), q.get(), d, kind, st.view().keys, st.view().stride, nullptr, Ceff, Q, kq,
-                          kq + margin, fp32_error_bound(kind, d), out.get(), s);
-        pt.end(PH_FINAL, s);
+        if (!hook.ran) {
+            rerank(ordered ? qorder.get() : nullptr, Q);
+        } else if (!hook.redone.empty()) {
+            // Queries whose candidates were redone after the rerank was
+            // enqueued: rerank them again.
+            const int m = (int)hook.redone.size();
+            flags.ensure((size_t)m);
+            int32_t* idx = reinterpret_cast<int32_t*>(flags.get());
+            LSH_HIP(hipMemcpyAsync(idx, hook.redone.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, s));
+            rerank(idx, m);
+        }
     } else {
         qjl.ensure((size_t)Q * sdim);
         pt.begin(PH_PREP, s);

@@ -276,6 +276,47 @@ def test_simhash_index_with_pivot_cells_matches_oracle(lsh, orc):
     _close(res["distance"], odist)
 
 
+def _index_vs_oracle(lsh, orc, data, queries, words, C, k):
+    d = data.shape[1]
+    with lsh.Index(data, implementation=3, sketchDim=words, distanceFunc=1, candidates=C) as ix:
+        res = ix.search(queries, k)
+    planes = orc.normal_stream(words * 32 * d)
+    cand, _ = orc.sketch_topc(0, orc.simhash(data, planes, words), orc.simhash(queries, planes, words), C)
+    oids, odist = orc.rerank(data, queries, cand, k, 1)
+    np.testing.assert_array_equal(res["ID"], oids)
+    _close(res["distance"], odist)
+    return res
+
+
+@pytest.mark.gpu
+def test_simhash_index_redo_after_early_rerank(lsh, orc):
+    # 1,500 copies of one point overflow its query's pass (more than 2C + 512
+    # pairs at distance 0): the query is redone after its rerank was already
+    # enqueued, and reranked again.
+    rng = np.random.default_rng(83)
+    N, Q, d, words, C, k = 120000, 640, 48, 16, 256, 32
+    centres = rng.standard_normal((400, d)).astype(np.float32)
+    data = centres[rng.integers(0, 400, N)] + 0.3 * rng.standard_normal((N, d)).astype(np.float32)
+    data /= np.linalg.norm(data, axis=1, keepdims=True)
+    copies = rng.choice(N, 1500, replace=False)
+    data[copies] = data[copies[0]]
+    queries = data[rng.integers(0, N, Q)] + 0.01 * rng.standard_normal((Q, d)).astype(np.float32)
+    queries[0] = data[copies[0]]
+    res = _index_vs_oracle(lsh, orc, data, queries, words, C, k)
+    assert (res["ID"][0] == np.sort(copies)[:k]).all()
+
+
+@pytest.mark.gpu
+def test_simhash_index_uniform_data_falls_back(lsh, orc):
+    # Unclustered data: the pivot cells cannot prune, the search falls back to
+    # the sampled threshold and the full pass after the cells' rerank was
+    # enqueued, and reranks again.
+    rng = np.random.default_rng(84)
+    data = _unit(rng, 100000, 32)
+    queries = _unit(rng, 80, 32)
+    _index_vs_oracle(lsh, orc, data, queries, 8, 256, 32)
+
+
 # ------------------------------------------------------------ full pipelines
 
 def test_simhash_pipeline_matches_oracle(lsh, orc):
