@@ -232,6 +232,17 @@ __global__ void cell_rows_kernel(const float* __restrict__ src, int d, const int
     }
 }
 
+// Zeroes the list counts, the stat words and the first-pass histograms of
+// a search in one launch.
+__global__ void cell_clear_kernel(uint32_t* __restrict__ cnt, int ncnt, uint32_t* __restrict__ stat,
+                                  uint32_t* __restrict__ hist, size_t nhist) {
+    const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (size_t i = i0; i < nhist; i += (size_t)gridDim.x * blockDim.x) hist[i] = 0u;
+    if (i0 < (size_t)ncnt) cnt[i0] = 0u;
+    if (i0 < 4) stat[i0] = 0u;
+    for (size_t i = i0 + (size_t)gridDim.x * blockDim.x; i < (size_t)ncnt; i += (size_t)gridDim.x * blockDim.x) cnt[i] = 0u;
+}
+
 // cmap[c] = the cell of chunk c (choff: exclusive prefix of chunks per cell).
 __global__ void chunk_map_kernel(const uint32_t* __restrict__ choff, int P, uint32_t* __restrict__ cmap) {
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < P; j += gridDim.x * blockDim.x)
@@ -891,9 +902,9 @@ const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K
     // of the tiles of a full pass (query tiles x point tiles).
     const uint32_t dense_limit = (uint32_t)std::min<double>(4e9, 0.5 * (double)qgroups * (double)c.ntiles);
 
-    LSH_HIP(hipMemsetAsync(cnt1, 0, sizeof(uint32_t) * 2 * (P + 1), s));
-    LSH_HIP(hipMemsetAsync(stat, 0, sizeof(uint32_t) * 4, s));
-    LSH_HIP(hipMemsetAsync(hist, 0, sizeof(uint32_t) * hn, s));
+    hipLaunchKernelGGL(cell_clear_kernel, dim3((unsigned)std::min<size_t>(4096, (hn + 255) / 256)), dim3(256), 0, s,
+                       cnt1, 2 * (P + 1), stat, hist, hn);
+    check_launch("cell_clear_kernel");
     pivots(c.words, qs, Q, c.pfp4.get(), P, Pp, nullptr, D, s);
     // Threshold bound from the tile ranges, the first-pass radii and their
     // cell counts, and the nearest cell of each query.

@@ -229,11 +229,23 @@ void sketch_topk(const SketchSet& d, const SketchSet* samples, int nsamples, int
             pt.end(PH_SELECT, s);
             uint32_t* flags = sc.get<uint32_t>(depth, 0, (size_t)Q);
             launch_incomplete_flags(st.view(), flags, s);
+            // The flags (and extra words) go to pinned memory ahead of the
+            // hook's work, so the host reads them while that work runs.
+            uint32_t* hb = sc.pinned((size_t)Q + n_extra);
+            LSH_HIP(hipMemcpyAsync(hb, flags, sizeof(uint32_t) * (size_t)Q, hipMemcpyDeviceToHost, s));
+            if (extra) LSH_HIP(hipMemcpyAsync(hb + Q, extra, sizeof(uint32_t) * n_extra, hipMemcpyDeviceToHost, s));
+            const hipEvent_t ev = sc.event();
+            LSH_HIP(hipEventRecord(ev, s));
             if (d.hook && d.hook->fn) {
                 d.hook->fn();
                 d.hook->ran = true;
             }
-            return flagged(flags, Q, s, extra, h_extra, n_extra);
+            LSH_HIP(hipEventSynchronize(ev));
+            std::vector<int32_t> bad;
+            for (int i = 0; i < Q; ++i)
+                if (hb[i]) bad.push_back(i);
+            for (int i = 0; i < n_extra; ++i) h_extra[i] = hb[Q + i];
+            return bad;
         };
         st.reset(Q, K, 2 * K + 512, s, true);
         const uint32_t* cstat = nullptr;

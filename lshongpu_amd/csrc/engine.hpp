@@ -56,6 +56,37 @@ struct Scratch {
     static constexpr int kDepth = 8;
     DevBuf<uint8_t> raw[4 * kDepth];
     TopKState tk[kDepth];
+    // Pinned host words (grow-only) and an event, for readbacks that the
+    // host waits on while later work is already queued behind them.
+    uint32_t* host = nullptr;
+    size_t host_n = 0;
+    hipEvent_t ev = nullptr;
+    Scratch() = default;
+    Scratch(const Scratch&) = delete;
+    Scratch& operator=(const Scratch&) = delete;
+    ~Scratch() {
+        if (host) (void)hipHostFree(host);
+        if (ev) (void)hipEventDestroy(ev);
+    }
+    uint32_t* pinned(size_t n) {
+        if (n > host_n) {
+            if (host) {
+                (void)hipDeviceSynchronize();  // no copy into the old buffer is still queued
+                (void)hipHostFree(host);
+                host = nullptr;
+                host_n = 0;
+            }
+            void* p = nullptr;
+            LSH_HIP(hipHostMalloc(&p, n * sizeof(uint32_t), hipHostMallocDefault));
+            host = static_cast<uint32_t*>(p);
+            host_n = n;
+        }
+        return host;
+    }
+    hipEvent_t event() {
+        if (!ev) LSH_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        return ev;
+    }
     template <class T>
     T* get(int depth, int slot, size_t n) {
         LSH_REQUIRE(depth < kDepth, LSH_ERR_INTERNAL, "scratch depth exceeded");

@@ -194,11 +194,19 @@ __global__ __launch_bounds__(kSelThreads) void sort_rows_kernel(uint64_t* keys, 
     for (int i = threadIdx.x; i < K; i += blockDim.x) row[i] = sm[i];
 }
 
-// The first `cols` entries of each of `rows` rows of length `stride`.
-__global__ void fill_row_heads_u64(uint64_t* p, int rows, int cols, int stride, uint64_t val) {
-    const size_t total = (size_t)rows * cols;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x)
-        p[(i / cols) * stride + i % cols] = val;
+// Reset of a top-K state: the first K keys of every row (the best lists)
+// and the threshold empty, the counts and overflow flags zero.
+__global__ void reset_state_kernel(TopKView v) {
+    const size_t total = (size_t)v.Q * v.K;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t q = i / v.K;
+        v.keys[q * v.stride + (i - q * v.K)] = kEmptyKey;
+        if (i < (size_t)v.Q) {
+            v.thr[i] = kEmptyKey;
+            v.cnt[i] = 0u;
+            v.ovf[i] = 0u;
+        }
+    }
 }
 
 __global__ void fill_u64(uint64_t* p, size_t n, uint64_t val) {
@@ -257,14 +265,10 @@ void TopKState::reset(int Q, int K, int newcap, hipStream_t s, bool int_keys) {
     thr_.ensure((size_t)Q);
     ovf_.ensure((size_t)Q);
     v_ = TopKView{keys_.get(), cnt_.get(), thr_.get(), ovf_.get(), K, newcap, stride, Q, int_keys ? 1 : 0};
-    // Fill kernels rather than memsets keep the sequence capturable in a graph.
-    // Only the best lists need clearing: the selects read the appended slots
-    // up to each query's count.
-    hipLaunchKernelGGL(fill_row_heads_u64, dim3(grid_for((size_t)Q * K)), dim3(256), 0, s, keys_.get(), Q, K, stride,
-                       kEmptyKey);
-    hipLaunchKernelGGL(fill_u64, dim3(grid_for(Q)), dim3(256), 0, s, thr_.get(), (size_t)Q, kEmptyKey);
-    hipLaunchKernelGGL(fill_u32, dim3(grid_for(Q)), dim3(256), 0, s, cnt_.get(), (size_t)Q, 0u);
-    hipLaunchKernelGGL(fill_u32, dim3(grid_for(Q)), dim3(256), 0, s, ovf_.get(), (size_t)Q, 0u);
+    // A fill kernel rather than memsets keeps the sequence capturable in a
+    // graph.  Only the best lists need clearing: the selects read the
+    // appended slots up to each query's count.
+    hipLaunchKernelGGL(reset_state_kernel, dim3(grid_for((size_t)Q * K)), dim3(256), 0, s, v_);
     check_launch("top-K state reset");
 }
 
