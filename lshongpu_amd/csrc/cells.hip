@@ -300,10 +300,11 @@ __global__ __launch_bounds__(256) void cell_threshold_kernel(const uint16_t* __r
                                                              uint32_t* __restrict__ n2) {
     __shared__ uint32_t h1[kWindow];
     __shared__ uint32_t h2[2049];
+    __shared__ uint16_t row[kMaxPivots];  // the query's distances to the pivots
     __shared__ int rad;
     __shared__ uint32_t best, s_n1, s_n2, s_h;
     const int q = blockIdx.x;
-    const uint16_t* __restrict__ row = D + (size_t)q * Pp;
+    const uint16_t* __restrict__ drow = D + (size_t)q * Pp;
     for (int b = threadIdx.x; b < 2049; b += 256) {
         if (b < kWindow) h1[b] = 0;
         h2[b] = 0;
@@ -315,8 +316,11 @@ __global__ __launch_bounds__(256) void cell_threshold_kernel(const uint16_t* __r
     }
     __syncthreads();
     uint32_t mine = 0xFFFFFFFFu;  // (distance, pivot) of the nearest non-empty cell
-    for (int j = threadIdx.x; j < P; j += 256)
-        if (csize[j]) mine = min(mine, ((uint32_t)row[j] << 13) | (uint32_t)j);
+    for (int j = threadIdx.x; j < P; j += 256) {
+        const uint16_t dj = drow[j];
+        row[j] = dj;
+        if (csize[j]) mine = min(mine, ((uint32_t)dj << 13) | (uint32_t)j);
+    }
     atomicMin(&best, mine);
     __syncthreads();
     const uint32_t dmin = best == 0xFFFFFFFFu ? 0u : best >> 13;
@@ -814,7 +818,7 @@ void CellIndex::build(const uint32_t* ds, int64_t N, int wpad, hipStream_t s) {
     for (int j = 0; j < np; ++j) ht[j + 1] = ht[j] + (hs[j + 1] - hs[j] + 31) / 32;
     ntiles = ht[np];
     nparts = 0;
-    for (int j = 0; j < np; ++j) nparts += (ht[j + 1] - ht[j] + 7) / 8;
+    for (int j = 0; j < np; ++j) nparts += (ht[j + 1] - ht[j] + kChunkTiles - 1) / kChunkTiles;
     ctile.alloc((size_t)np + 1);
     LSH_HIP(hipMemcpyAsync(ctile.get(), ht.data(), sizeof(uint32_t) * ht.size(), hipMemcpyHostToDevice, s));
     std::vector<uint32_t> hz((size_t)np);

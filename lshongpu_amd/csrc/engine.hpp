@@ -113,7 +113,7 @@ struct CellIndex {
     int P = 0;       // pivots; 0 when no cells were built
     int words = 0;   // sketch words per row (8, 16, 24 or 32)
     int64_t ntiles = 0;
-    int64_t nparts = 0;            // sum over cells of ceil(tiles / 8): the tile ranges of a chunk list
+    int64_t nparts = 0;            // sum over cells of their tile ranges (chunks of one query group)
     DevBuf<uint32_t> pfp4;         // FP4 image of the pivot sketches
     DevBuf<uint32_t> sk;           // [ntiles*32][W] packed sketches in cell order, even words then odd
     DevBuf<int32_t> ids;           // [ntiles*32] data row of each position, -1 for padding

@@ -109,10 +109,13 @@ __device__ uint64_t radix_kth_in_bin(const uint64_t* row, int n, uint32_t T, uin
     return prefix;
 }
 
+constexpr int kTieCap = 512;  // keys of the boundary bin gathered in LDS (more: selected in place)
+
 __global__ __launch_bounds__(kSelThreads) void select_hist_kernel(TopKView v) {
     extern __shared__ uint64_t kept[];  // K
     __shared__ uint32_t hist[kSelBins];
-    __shared__ uint32_t s_T, s_below, s_nkeep, s_dig, s_rank;
+    __shared__ uint64_t ties[kTieCap];
+    __shared__ uint32_t s_T, s_below, s_nkeep, s_dig, s_rank, s_nties;
     const int q = blockIdx.x;
     const uint32_t n_raw = v.cnt[q];
     if (n_raw == 0) return;
@@ -121,7 +124,10 @@ __global__ __launch_bounds__(kSelThreads) void select_hist_kernel(TopKView v) {
     const int total = K + n;
     uint64_t* row = v.keys + (size_t)q * v.stride;
     for (int i = threadIdx.x; i < kSelBins; i += blockDim.x) hist[i] = 0;
-    if (threadIdx.x == 0) s_nkeep = 0;
+    if (threadIdx.x == 0) {
+        s_nkeep = 0;
+        s_nties = 0;
+    }
     __syncthreads();
     for (int i = threadIdx.x; i < total; i += blockDim.x) {
         const uint64_t key = row[i];
@@ -164,7 +170,22 @@ __global__ __launch_bounds__(kSelThreads) void select_hist_kernel(TopKView v) {
     uint64_t kth = kEmptyKey;  // the K-th smallest key (none: keep every key)
     if (T < (uint32_t)kSelBins) {
         const bool last = T == (uint32_t)(kSelBins - 1);
-        kth = radix_kth_in_bin(row, total, T, (uint32_t)K - s_below, last ? 56 : 24, last ? 0ull : (uint64_t)T << 32,
+        const uint32_t nt = hist[T];  // keys in the boundary bin
+        __syncthreads();
+        const uint64_t* src = row;
+        int nsrc = total;
+        if (nt <= (uint32_t)kTieCap) {
+            // Gather them once; the radix passes then run over LDS.
+            for (int i = threadIdx.x; i < total; i += blockDim.x) {
+                const uint64_t key = row[i];
+                if (key != kEmptyKey && min((uint32_t)(key >> 32), (uint32_t)(kSelBins - 1)) == T)
+                    ties[atomicAdd(&s_nties, 1u)] = key;
+            }
+            __syncthreads();
+            src = ties;
+            nsrc = (int)nt;
+        }
+        kth = radix_kth_in_bin(src, nsrc, T, (uint32_t)K - s_below, last ? 56 : 24, last ? 0ull : (uint64_t)T << 32,
                                hist, &s_dig, &s_rank);
     }
     for (int i = threadIdx.x; i < total; i += blockDim.x) {
