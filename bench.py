@@ -130,13 +130,18 @@ def main() -> int:
     res = out.cpu().numpy().view(lsh.POINT_DTYPE).reshape(Q, kq)
 
     # ---- dominant kernel: the rerank (one launch per step), a gather of the
-    # candidate rows bound by HBM.  Algorithmic bytes per launch: every
-    # candidate row once (4*d*C per query), the candidate keys (8*C), the
-    # query rows and the results.
+    # candidate rows bound by HBM.  Algorithmic bytes per launch, per query:
+    # every candidate row once in the filter's format (FP16 unit rows of
+    # hs = d rounded up to 8 halves, or FP32 rows), the candidate keys
+    # (8*C), the query row, the k FP32 rows that the FP64 recheck needs at
+    # the least, and the results.
     steps = args.steps
     C = min(args.candidates, args.n)
     avg_launch_ms = phases["final_ms"] / steps
-    bytes_launch = float(Q) * (4.0 * d * C + 8.0 * C + 4.0 * d + 8.0 * kq)
+    half_filter = os.environ.get("LSHGPU_RERANK_F32", "0") != "1"
+    row_bytes = 2.0 * ((d + 7) // 8 * 8) if half_filter else 4.0 * d
+    bytes_launch = float(Q) * (row_bytes * C + 8.0 * C + 4.0 * d + (4.0 * d * kq if half_filter else 0.0)
+                               + 8.0 * kq)
     achieved_gbs = bytes_launch / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else 0.0
     # The threshold pass over the pivot cells: matrix-core tile products
     # (32 queries x 32 points x all sketch bits) against a pass over all points.
@@ -193,7 +198,9 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u32 sketch distances, f32 rerank with f64 recheck",
+        "dtype": "u32 Hamming distances; angular rerank filter in f32 arithmetic over f16 unit rows with a proven "
+                 "error band, f64 recheck of the band" if half_filter else
+                 "u32 Hamming distances; f32 rerank filter, f64 recheck of the error band",
         "data": "synthetic (seeded Gaussian mixture), generated in-process",
         "config": {"workload": f"SimHash {32 * args.words}-bit sketch scan, top-{args.candidates} by Hamming, "
                                f"exact angular rerank to k={k}",
@@ -204,7 +211,7 @@ def main() -> int:
         "index_build_ms": build_ms,
         "phase_ms_per_step": {key: phases[key] / args.steps for key in phases},
         "roofline": {
-            "kernel": "rerank_kernel (gather of the candidate rows, FP32 angular distance, FP64 recheck)",
+            "kernel": "rerank_kernel (gather of the candidate rows, approximate angular distance, FP64 recheck)",
             "bound": "hbm",
             "achieved": achieved_gbs,
             "peak": HBM_PEAK_GBS,

@@ -497,7 +497,7 @@ size_t lsh_index::device_bytes() const {
                qkeys.bytes() + qlo.bytes() + qpre.bytes() + qmax.bytes() + q.bytes() + qT.bytes() +
                q_inv.bytes() + qjl.bytes() + qjlT.bytes() + qsk.bytes() + out.bytes() + flags.bytes() + st.bytes();
     for (const SampleLevel& L : samples) b += L.ds.bytes() + L.fp4.bytes();
-    b += cells.bytes() + qorder.bytes() + cdata.bytes() + cpos.bytes();
+    b += cells.bytes() + qorder.bytes() + cdata.bytes() + cpos.bytes() + chalf.bytes();
     for (const Transform& t : key_steps) b += t.mins.bytes();
     for (const Transform& t : sketch_steps) b += t.mins.bytes();
     return b;
@@ -581,7 +581,15 @@ void lsh_index::build_fp4(hipStream_t s) {
     build_sample_levels(dsk.get(), N, wpad, candidates, samples, s);
     if (!samples.empty() && framework == 0) {
         cells.build(dsk.get(), N, wpad, s);
-        if (cells.built()) cell_rows(cells, data.get(), N, d, cdata, cpos, s);
+        if (cells.built()) {
+            cell_rows(cells, data.get(), N, d, cdata, cpos, s);
+            // The angular rerank (SimHash) filters on an FP16 image of the
+            // unit rows; data with a zero or non-finite row keeps FP32.
+            // LSHGPU_RERANK_F32=1 keeps the FP32 filter (for comparisons).
+            const char* f32 = std::getenv("LSHGPU_RERANK_F32");
+            if (impl == 3 && !(f32 && f32[0] == '1'))
+                build_unit_rows_half(cdata.get(), cells.ids.get(), cells.ntiles * 32, d, chalf, s);
+        }
     }
 }
 
@@ -764,10 +772,11 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
             pt.begin(PH_FINAL, s);
             if (cells.built())
                 launch_rerank(cdata.get(), q.get(), d, kind, st.view().keys, st.view().stride, nullptr, Ceff, m, kq,
-                              kq + margin, fp32_error_bound(kind, d), out.get(), s, order, cpos.get());
+                              chalf.get() ? half_error_bound(d) : fp32_error_bound(kind, d), out.get(), s, order,
+                              cpos.get(), chalf.get());
             else
                 launch_rerank(data.get(), q.get(), d, kind, st.view().keys, st.view().stride, nullptr, Ceff, m, kq,
-                              kq + margin, fp32_error_bound(kind, d), out.get(), s, order);
+                              fp32_error_bound(kind, d), out.get(), s, order);
             pt.end(PH_FINAL, s);
         };
         SketchSet::Hook hook;
@@ -801,7 +810,7 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
                    pt, scan_launches);
         pt.begin(PH_FINAL, s);
         launch_rerank(data.get(), q.get(), d, kAngular, st.view().keys, st.view().stride, nullptr, Ceff, Q, kq,
-                      kq + margin, fp32_error_bound(kAngular, d), out.get(), s);
+                      fp32_error_bound(kAngular, d), out.get(), s);
         pt.end(PH_FINAL, s);
     }
 
@@ -876,8 +885,8 @@ void lsh_index::search_lsh(int Q, int kq, hipStream_t s, PhaseTimer& pt) {
         LSH_REQUIRE(K <= kMaxTopK, LSH_ERR_UNSUPPORTED, "candidate sets larger than 4096 are not supported");
         lsh_topk(a, none, maxS, K, kNormal, int_keys && K <= kMaxIntTopK, st, scratch, 0, s, pt, scan_launches);
         pt.begin(PH_FINAL, s);
-        launch_rerank(data.get(), q.get(), d, kind, st.view().keys, st.view().stride, nullptr, Ceff, Q, kq,
-                      kq + margin, eps, out.get(), s);
+        launch_rerank(data.get(), q.get(), d, kind, st.view().keys, st.view().stride, nullptr, Ceff, Q, kq, eps,
+                      out.get(), s);
         pt.end(PH_FINAL, s);
         return;
     }

@@ -178,6 +178,7 @@ struct lsh_index {
     lshgpu::CellIndex cells;         // pivot cells of dsk (pruned threshold pass)
     lshgpu::DevBuf<float> cdata;     // data rows in cell order (rerank reads)
     lshgpu::DevBuf<int32_t> cpos;    // position of each data row in cdata
+    lshgpu::DevBuf<_Float16> chalf;  // FP16 image of the unit rows of cdata (angular rerank filter)
     lshgpu::DevBuf<float> planesT;   // SimHash [words][d][32], JL [groups][d][32]
     int sdim = 0;                    // JL sketch length
     lshgpu::DevBuf<float> djl;       // [N][sdim]

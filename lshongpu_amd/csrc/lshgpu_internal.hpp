@@ -242,15 +242,28 @@ void launch_exact_finalize(const float* data, const float* queries, int d, int k
                            uint32_t* flags, hipStream_t s);
 
 // rerank.hip: candidates = low 32 bits of the first C keys of each query
-// (or `cand` ids when non-null, [Q][C]); exact FP32 distances, best K' kept,
-// FP64 recheck (all C in FP64 when the margin fails), k written.
+// (or `cand` ids when non-null, [Q][C]).  Approximate distances with error
+// at most eps (FP32 over data, or FP32 over the FP16 image hdata of the unit
+// rows, angular kinds only); the candidates within the band that eps leaves
+// above the k-th approximate distance are recomputed in FP64, and the k
+// best by (FP64 distance, ID) are written.
 // qorder (may be null): a permutation of the queries, taken in that order
 // per XCD so that queries with shared candidates run close together.
-// rowmap (may be null): candidate ID i is row rowmap[i] of data.
+// rowmap (may be null): candidate ID i is row rowmap[i] of data and hdata.
 void launch_rerank(const float* data, const float* queries, int d, int kind, const uint64_t* keys,
-                   int key_stride, const int32_t* cand, int C, int Q, int k, int kprime, double eps,
+                   int key_stride, const int32_t* cand, int C, int Q, int k, double eps,
                    lsh_point_t* out, hipStream_t s, const int32_t* qorder = nullptr,
-                   const int32_t* rowmap = nullptr);
+                   const int32_t* rowmap = nullptr, const _Float16* hdata = nullptr);
+
+// FP16 image of npos rows for the angular rerank: row i is x_i * 2^14 /
+// ||x_i|| (FP64 norm), rounded to FP16 and zero padded to
+// half_row_stride(d) halves; rows with ids[i] < 0 are zero.  Returns false
+// (and releases out) when a real row's norm is zero or not finite.
+bool build_unit_rows_half(const float* rows, const int32_t* ids, int64_t npos, int d, DevBuf<_Float16>& out,
+                          hipStream_t s);
+int half_row_stride(int d);
+// Error bound of the angular distance computed over that image.
+double half_error_bound(int d);
 
 // sketch.hip (out rows have `out_stride` words; planesT is [words][d][hashBits])
 void launch_simhash(const float* x, int64_t n, int d, const float* planesT, int words,

@@ -3,17 +3,27 @@
 // candidateSetScanner.cuh:10-32 recomputes the exact distance of every slot
 // of a lane's queue and the final startSort/write of
 // sketchedDistanceScanners.cuh:87-99 keeps the k best.  Here one workgroup
-// owns one query: the C candidates get their FP32 distance (gathered rows),
-// a bitonic sort in LDS orders them by (FP32 distance, ID), the best K' are
-// recomputed in FP64 and re-sorted by (FP64 distance, ID).  When the FP32
-// gap between the k-th and the K'-th candidate is not larger than twice the
-// FP32 error bound, every candidate is recomputed in FP64 instead, so the
-// returned IDs are always the FP64 order of the candidate set.
+// owns one query.  The C candidates get an approximate distance with a
+// proven error bound eps: FP32 over the FP32 rows, or, for the angular
+// kinds, FP32 arithmetic over an FP16 image of the unit rows.  A radix
+// select finds the k-th smallest approximate distance v_k; every candidate
+// whose approximate distance is within the band that eps leaves above v_k
+// (v_k + 2 eps, or the relative band for the Euclidean kind) is recomputed
+// in FP64 and the set is sorted by (FP64 distance, ID).  A candidate outside
+// the band is farther, in exact arithmetic, than each of the k approximate
+// best, so the k written are always the FP64 order of the whole candidate
+// set.
 #include "device_common.hpp"
 
 namespace lshgpu {
 
 namespace {
+
+// Rows of the FP16 image hold x / ||x|| scaled by 2^14, so that components
+// down to 2^-28 of the row norm stay out of the FP16 subnormal range.
+constexpr float kHalfScale = 16384.0f;
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 
 __device__ inline uint64_t shfl_xor_u64(uint64_t x, int m) {
     const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)x, m);
@@ -21,25 +31,26 @@ __device__ inline uint64_t shfl_xor_u64(uint64_t x, int m) {
     return ((uint64_t)hi << 32) | lo;
 }
 
-// Ascending bitonic sort of one value per lane across a wave.
-__device__ inline uint64_t wave_sort_u64(uint64_t x) {
+// Ascending sort of one (key, id) pair per lane across a wave.
+__device__ inline void wave_sort_pairs(uint64_t& k, int32_t& id) {
     const int lane = threadIdx.x & 63;
     for (int size = 2; size <= 64; size <<= 1)
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            const uint64_t y = shfl_xor_u64(x, stride);
+            const uint64_t ko = shfl_xor_u64(k, stride);
+            const int32_t io = __shfl_xor(id, stride);
             const bool up = (lane & size) == 0, lower = (lane & stride) == 0;
-            x = (lower == up) ? (x < y ? x : y) : (x < y ? y : x);
+            const bool mine_smaller = k < ko || (k == ko && id < io);
+            // The lower lane of an ascending pair keeps the smaller pair.
+            if (mine_smaller != (lower == up)) {
+                k = ko;
+                id = io;
+            }
         }
-    return x;
 }
 
-// The kk smallest of the n (a power of two, distinct or kEmptyKey) keys, in
-// ascending order, into out[0..kk): a radix select of the kk-th smallest
-// distance key (four 8-bit digits of the high word), every key at or below
-// it collected, and those sorted (in registers when they fit a wave).
-// Returns the number collected (>= kk).  out has room for n keys.
-__device__ int select_smallest(const uint64_t* keys, int n, int kk, uint64_t* out, uint32_t* hist,
-                               uint32_t* sv) {
+// High word of the kk-th smallest (1-based) of the n keys: a radix select
+// over four 8-bit digits of the high word.
+__device__ uint32_t kth_high_word(const uint64_t* keys, int n, int kk, uint32_t* hist, uint32_t* sv) {
     const int tid = threadIdx.x, bs = blockDim.x;
     uint32_t prefix = 0, mask = 0, rank = (uint32_t)kk;
     for (int shift = 24; shift >= 0; shift -= 8) {
@@ -79,27 +90,7 @@ __device__ int select_smallest(const uint64_t* keys, int n, int kk, uint64_t* ou
         rank = sv[1];
         __syncthreads();
     }
-    if (tid == 0) sv[2] = 0;
-    __syncthreads();
-    for (int t = tid; t < n; t += bs) {
-        const uint64_t key = keys[t];
-        if ((uint32_t)(key >> 32) <= prefix) out[atomicAdd(&sv[2], 1u)] = key;
-    }
-    __syncthreads();
-    const int m = (int)sv[2];
-    if (m <= 64) {
-        if (tid < 64) {
-            const uint64_t x = wave_sort_u64(tid < m ? out[tid] : kEmptyKey);
-            if (tid < n) out[tid] = x;
-        }
-    } else {
-        int m2 = 1;
-        while (m2 < m) m2 <<= 1;
-        for (int t = m + tid; t < m2; t += bs) out[t] = kEmptyKey;
-        bitonic_sort_lds(out, m2);
-    }
-    __syncthreads();
-    return m;
+    return prefix;
 }
 
 // Sum over the 16 lanes of a DPP row, in every lane of it: rotations by 8,
@@ -139,67 +130,16 @@ __device__ inline float distance_k(float a, float b, float q_norm) {
     return a;
 }
 
-template <int KIND>
-__global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ data,
-                                                     const float* __restrict__ queries, int d,
-                                                     const uint64_t* __restrict__ keys,
-                                                     int key_stride, const int32_t* __restrict__ cand,
-                                                     int C, int Cp, int k, int kprime, double eps,
-                                                     lsh_point_t* __restrict__ out,
-                                                     const int32_t* __restrict__ qorder, int Q, int per,
-                                                     const int32_t* __restrict__ rowmap) {
-    extern __shared__ uint64_t sm[];
-    uint64_t* k32 = sm;                                        // Cp
-    uint64_t* k64 = sm + Cp;                                   // Cp
-    int32_t* ids = reinterpret_cast<int32_t*>(sm + 2 * Cp);    // Cp
-    int32_t* rows = ids + Cp;                                  // Cp: data row of each candidate
-    __shared__ int s_ok;
-    __shared__ uint32_t rs_hist[256];
-    __shared__ uint32_t rs_sv[4];
-    // With an order, XCD x (workgroups x, x + 8, ...) takes the x-th eighth
-    // of it in sequence.
-    int q = blockIdx.x;
-    if (qorder) {
-        const int pos = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
-        if (pos >= Q) return;
-        q = qorder[pos];
-    }
-    const float* __restrict__ qv = queries + (size_t)q * d;
-
-    float qn = 0.0f;
-    double nq64 = 0.0;
-    if (KIND == kNegCos || KIND == kAngular) {
-        for (int j = 0; j < d; ++j) qn = __builtin_fmaf(qv[j], qv[j], qn);
-        qn = __builtin_sqrtf(qn);
-        nq64 = norm2_64(qv, d);
-    }
-    // Candidate IDs first (to LDS), so the row reads below do not wait on
-    // the key reads.
-    for (int t = threadIdx.x; t < Cp; t += blockDim.x) {
-        int32_t id = -1;
-        if (t < C) {
-            if (cand) {
-                id = cand[(size_t)q * C + t];
-            } else {
-                const uint64_t key = keys[(size_t)q * key_stride + t];
-                id = key == kEmptyKey ? -1 : (int32_t)(uint32_t)key;
-            }
-        }
-        ids[t] = id;
-        rows[t] = id < 0 ? -1 : (rowmap ? rowmap[id] : id);
-    }
-    __syncthreads();
-    // FP32 distances: a group of 16 lanes reads one candidate row with
-    // coalesced float4 loads; the group's partial sums are added with DPP
-    // rotations inside the 16-lane row, and lane u of the group keeps row u's
-    // sums in LDS (k64 serves as scratch: two floats per candidate).  The
-    // distances themselves are then taken one candidate per lane.
+// Approximate distances of candidates [0, Cp) into sums[] (two floats per
+// candidate), FP32 rows: a group of 16 lanes reads one candidate row with
+// coalesced float4 loads, U rows per group in flight; the partial sums are
+// added with DPP rotations and lane u of the group keeps row u's sums.
+template <int KIND, int U>
+__device__ inline void sums_f32(const float* __restrict__ data, const float* __restrict__ qv, int d,
+                                const int32_t* rows, int Cp, float2* sums) {
     const int gl = threadIdx.x & 15;
     const int ngroups = blockDim.x >> 4;
     const bool vec4 = (d & 3) == 0;
-    float2* sums = reinterpret_cast<float2*>(k64);
-    // Each group takes U rows per step so that U row reads are in flight.
-    constexpr int U = 6;
     for (int t0 = threadIdx.x >> 4; t0 < Cp; t0 += ngroups * U) {
         const float* rp[U];
         float a[U], b[U];
@@ -250,65 +190,185 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
         const int t = t0 + gl * ngroups;
         if (gl < U && t < Cp) sums[t] = make_float2(as, bs);
     }
+}
+
+// The same over the FP16 image (rows of hs halves, hs a multiple of 8, zero
+// padded): one 16-byte load per lane covers 8 components, the query comes
+// from LDS (qs, zero padded to hs), and only the dot product is summed.
+template <int U>
+__device__ inline void sums_f16(const _Float16* __restrict__ hdata, int hs, const float* qs, const int32_t* rows,
+                                int Cp, float2* sums) {
+    const int gl = threadIdx.x & 15;
+    const int ngroups = blockDim.x >> 4;
+    for (int t0 = threadIdx.x >> 4; t0 < Cp; t0 += ngroups * U) {
+        const _Float16* rp[U];
+        float a[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int t = t0 + u * ngroups;
+            const int r = t < Cp ? rows[t] : -1;
+            rp[u] = hdata + (size_t)(r < 0 ? 0 : r) * hs;
+            a[u] = 0.0f;
+        }
+        for (int j = gl * 8; j < hs; j += 128) {
+            const float4 y0 = *reinterpret_cast<const float4*>(qs + j);
+            const float4 y1 = *reinterpret_cast<const float4*>(qs + j + 4);
+            half8 xv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) xv[u] = *reinterpret_cast<const half8*>(rp[u] + j);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                a[u] = __builtin_fmaf((float)xv[u][0], y0.x, a[u]);
+                a[u] = __builtin_fmaf((float)xv[u][1], y0.y, a[u]);
+                a[u] = __builtin_fmaf((float)xv[u][2], y0.z, a[u]);
+                a[u] = __builtin_fmaf((float)xv[u][3], y0.w, a[u]);
+                a[u] = __builtin_fmaf((float)xv[u][4], y1.x, a[u]);
+                a[u] = __builtin_fmaf((float)xv[u][5], y1.y, a[u]);
+                a[u] = __builtin_fmaf((float)xv[u][6], y1.z, a[u]);
+                a[u] = __builtin_fmaf((float)xv[u][7], y1.w, a[u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) a[u] = row_sum16(a[u]);
+        float as = a[0];
+#pragma unroll
+        for (int u = 1; u < U; ++u)
+            if (gl == u) as = a[u];
+        const int t = t0 + gl * ngroups;
+        if (gl < U && t < Cp) sums[t] = make_float2(as, 0.0f);
+    }
+}
+
+// HALF: the approximate distances come from hdata (angular kinds only).
+template <int KIND, bool HALF>
+__global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ data,
+                                                     const _Float16* __restrict__ hdata, int hs,
+                                                     const float* __restrict__ queries, int d,
+                                                     const uint64_t* __restrict__ keys,
+                                                     int key_stride, const int32_t* __restrict__ cand,
+                                                     int C, int Cp, int k, double eps,
+                                                     lsh_point_t* __restrict__ out,
+                                                     const int32_t* __restrict__ qorder, int Q, int per,
+                                                     const int32_t* __restrict__ rowmap) {
+    extern __shared__ uint64_t sm[];
+    uint64_t* k32 = sm;                                        // Cp
+    uint64_t* k64 = sm + Cp;                                   // Cp
+    int32_t* ids = reinterpret_cast<int32_t*>(sm + 2 * Cp);    // Cp
+    int32_t* rows = ids + Cp;                                  // Cp: data row of each candidate
+    float* qs = reinterpret_cast<float*>(rows + Cp);           // hs (HALF only)
+    __shared__ uint32_t rs_hist[256];
+    __shared__ uint32_t rs_sv[4];
+    // With an order, XCD x (workgroups x, x + 8, ...) takes the x-th eighth
+    // of it in sequence.
+    int q = blockIdx.x;
+    if (qorder) {
+        const int pos = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+        if (pos >= Q) return;
+        q = qorder[pos];
+    }
+    const float* __restrict__ qv = queries + (size_t)q * d;
+
+    float qn = 0.0f;
+    double nq64 = 0.0;
+    if (KIND == kNegCos || KIND == kAngular) {
+        for (int j = 0; j < d; ++j) qn = __builtin_fmaf(qv[j], qv[j], qn);
+        qn = __builtin_sqrtf(qn);
+        nq64 = norm2_64(qv, d);
+    }
+    // Candidate IDs first (to LDS), so the row reads below do not wait on
+    // the key reads.
+    for (int t = threadIdx.x; t < Cp; t += blockDim.x) {
+        int32_t id = -1;
+        if (t < C) {
+            if (cand) {
+                id = cand[(size_t)q * C + t];
+            } else {
+                const uint64_t key = keys[(size_t)q * key_stride + t];
+                id = key == kEmptyKey ? -1 : (int32_t)(uint32_t)key;
+            }
+        }
+        ids[t] = id;
+        rows[t] = id < 0 ? -1 : (rowmap ? rowmap[id] : id);
+    }
+    if (HALF)
+        for (int j = threadIdx.x; j < hs; j += blockDim.x) qs[j] = j < d ? qv[j] : 0.0f;
     __syncthreads();
+    // Approximate distances (k64 serves as scratch for the sums).
+    float2* sums = reinterpret_cast<float2*>(k64);
+    if (HALF)
+        sums_f16<8>(hdata, hs, qs, rows, Cp, sums);
+    else
+        sums_f32<KIND, 6>(data, qv, d, rows, Cp, sums);
+    __syncthreads();
+    // Queries of extreme norm leave the FP16 bound's range assumptions;
+    // every candidate is then rechecked.
+    const bool all = HALF && !(qn >= 0x1p-60f && qn <= 0x1p60f);
     for (int t = threadIdx.x; t < Cp; t += blockDim.x) {
         const int32_t idt = ids[t];
         const float2 ab = sums[t];
-        k32[t] = idt < 0 ? kEmptyKey : make_key(f32_key(distance_k<KIND>(ab.x, ab.y, qn)), (uint32_t)idt);
+        float dist;
+        if (HALF) {
+            const float c = (ab.x * (1.0f / kHalfScale)) / qn;
+            dist = KIND == kNegCos ? -c : 1.0f - c;
+        } else {
+            dist = distance_k<KIND>(ab.x, ab.y, qn);
+        }
+        k32[t] = idt < 0 ? kEmptyKey : make_key(f32_key(dist), (uint32_t)idt);
     }
     __syncthreads();
-    // The kp best by FP32 distance, sorted, in k64 (k32 keeps every key).
-    const int kp = min(kprime, Cp);
-    select_smallest(k32, Cp, kp, k64, rs_hist, rs_sv);
-    if (threadIdx.x == 0) {
-        bool ok = true;
-        if (kp < Cp) {
-            const uint64_t kk = k64[k - 1], kK = k64[kp - 1];
-            if (kK != kEmptyKey && kk != kEmptyKey) {
-                const double vk = key_f32((uint32_t)(kk >> 32));
-                const double vK = key_f32((uint32_t)(kK >> 32));
-                if (KIND == kEuclid)
-                    ok = vK * (1.0 - eps) > vk * (1.0 + eps);
-                else
-                    ok = vK - vk > 2.0 * eps;
-            }
+    // The band: approximate distance at most ub (as a key's high word).
+    uint32_t ub = 0xFFFFFFFFu;
+    if (k < Cp && !all) {
+        const uint32_t kth = kth_high_word(k32, Cp, k, rs_hist, rs_sv);
+        const double vk = (double)key_f32(kth);
+        const double t = KIND == kEuclid ? vk * (1.0 + eps) / (1.0 - eps) : vk + 2.0 * eps;
+        if (kth != 0xFFFFFFFFu && __builtin_isfinite(t) && t < 3.0e38) {
+            // Rounded up: consecutive floats have consecutive keys.
+            const float tf = (float)t;
+            ub = f32_key(tf) + ((double)tf < t ? 1u : 0u);
         }
-        s_ok = ok;
+    }
+    // Collect the band: ids[slot] = candidate index (the candidate IDs live
+    // on in k32's low words).
+    if (threadIdx.x == 0) rs_sv[2] = 0;
+    __syncthreads();
+    for (int t = threadIdx.x; t < Cp; t += blockDim.x) {
+        const uint64_t key = k32[t];
+        if (key != kEmptyKey && (uint32_t)(key >> 32) <= ub) ids[atomicAdd(&rs_sv[2], 1u)] = t;
     }
     __syncthreads();
-    int n = kp;
-    if (s_ok) {
-        // FP64 for the kp best, in place.
-        for (int t = threadIdx.x; t < kp; t += blockDim.x) {
-            const uint64_t key = k64[t];
-            if (key == kEmptyKey) {
-                ids[t] = 0x7FFFFFFF;
-            } else {
-                const int32_t id = (int32_t)(uint32_t)key;
-                k64[t] = f64_key(dist64_row(KIND, qv, data + (size_t)(rowmap ? rowmap[id] : id) * d, d, nq64));
-                ids[t] = id;
-            }
+    const int m = (int)rs_sv[2];
+    // FP64 for the band.  FP64 keys carry no ID: the sort compares IDs on
+    // equal keys.
+    for (int s = threadIdx.x; s < m; s += blockDim.x) {
+        const int t = ids[s];
+        const int32_t id = (int32_t)(uint32_t)k32[t];
+        k64[s] = f64_key(dist64_row(KIND, qv, data + (size_t)rows[t] * d, d, nq64));
+        ids[s] = id;
+    }
+    __syncthreads();
+    int n = 1;
+    while (n < m) n <<= 1;
+    if (n <= 64) {
+        if (threadIdx.x < 64) {
+            uint64_t kk = threadIdx.x < m ? k64[threadIdx.x] : kEmptyKey;
+            int32_t ii = threadIdx.x < m ? ids[threadIdx.x] : 0x7FFFFFFF;
+            wave_sort_pairs(kk, ii);
+            k64[threadIdx.x] = kk;
+            ids[threadIdx.x] = ii;
         }
+        __syncthreads();
     } else {
-        // FP64 for every candidate.
-        for (int t = threadIdx.x; t < Cp; t += blockDim.x) {
-            const uint64_t key = k32[t];
-            if (key == kEmptyKey) {
-                k64[t] = kEmptyKey;
-                ids[t] = 0x7FFFFFFF;
-            } else {
-                const int32_t id = (int32_t)(uint32_t)key;
-                k64[t] = f64_key(dist64_row(KIND, qv, data + (size_t)(rowmap ? rowmap[id] : id) * d, d, nq64));
-                ids[t] = id;
-            }
+        for (int t = m + threadIdx.x; t < n; t += blockDim.x) {
+            k64[t] = kEmptyKey;
+            ids[t] = 0x7FFFFFFF;
         }
-        n = Cp;
+        bitonic_sort_pairs(k64, ids, n);
     }
-    bitonic_sort_pairs(k64, ids, n);
     const float pad = KIND == kNegCos ? 2.0f : kPadDistance;
     for (int t = threadIdx.x; t < k; t += blockDim.x) {
         lsh_point_t p;
-        if (t < n && ids[t] != 0x7FFFFFFF) {
+        if (t < m) {
             p.ID = ids[t];
             p.distance = (float)key_f64(k64[t]);
         } else {
@@ -319,28 +379,90 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
     }
 }
 
+// One 16-lane group per row: out row = (float)(x * 2^14 / ||x||) rounded to
+// FP16, zero padded to hs; padding positions (ids < 0) are zero rows.  A
+// real row whose FP64 norm is zero or not finite sets *bad.
+__global__ void unit_rows_half_kernel(const float* __restrict__ rows, const int32_t* __restrict__ ids,
+                                      int64_t npos, int d, int hs, _Float16* __restrict__ out,
+                                      uint32_t* __restrict__ bad) {
+    const int gl = threadIdx.x & 15;
+    const int64_t ngroups = (int64_t)gridDim.x * (blockDim.x >> 4);
+    for (int64_t r = (int64_t)blockIdx.x * (blockDim.x >> 4) + (threadIdx.x >> 4); r < npos; r += ngroups) {
+        const float* x = rows + (size_t)r * d;
+        const bool real = ids[r] >= 0;
+        double s = 0.0;
+        for (int j = gl; j < d; j += 16) s = __builtin_fma((double)x[j], (double)x[j], s);
+        for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 16);
+        const bool ok = real && s > 0.0 && s <= 1.0e300;
+        if (gl == 0 && real && !ok) atomicOr(bad, 1u);
+        const double scale = ok ? (double)kHalfScale / __builtin_sqrt(s) : 0.0;
+        for (int j = gl; j < hs; j += 16)
+            out[(size_t)r * hs + j] = (_Float16)(j < d ? (float)((double)x[j] * scale) : 0.0f);
+    }
+}
+
 }  // namespace
 
+double half_error_bound(int d) {
+    // Per component, the FP16 image differs from 2^14 x / ||x|| by at most
+    // (2^-11 + 2^-23) of its magnitude (two roundings, FP64 -> FP32 -> FP16)
+    // plus 2^-14 (a subnormal flushed), i.e. 2^-28 of the unit row.  Over
+    // the dot product with q that is (2^-11 + 2^-23) + sqrt(d) 2^-28 of
+    // ||q||; the FP32 sums, norm and division add at most the FP32 bound.
+    const double u = 1.0 / 16777216.0;
+    return 1.01 * (1.0 / 2048.0 + 1.0 / 8388608.0) + std::sqrt((double)d) * (1.0 / 268435456.0) +
+           (2.0 * d + 16.0) * u;
+}
+
+int half_row_stride(int d) { return (d + 7) / 8 * 8; }
+
+bool build_unit_rows_half(const float* rows, const int32_t* ids, int64_t npos, int d, DevBuf<_Float16>& out,
+                          hipStream_t s) {
+    const int hs = half_row_stride(d);
+    out.alloc((size_t)npos * hs);
+    DevBuf<uint32_t> bad(1);
+    LSH_HIP(hipMemsetAsync(bad.get(), 0, sizeof(uint32_t), s));
+    if (npos > 0) {
+        const unsigned blocks = (unsigned)std::min<int64_t>(65535, (npos + 15) / 16);
+        hipLaunchKernelGGL(unit_rows_half_kernel, dim3(blocks), dim3(256), 0, s, rows, ids, npos, d, hs, out.get(),
+                           bad.get());
+        check_launch("unit_rows_half_kernel");
+    }
+    uint32_t h = 0;
+    LSH_HIP(hipMemcpyAsync(&h, bad.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    LSH_HIP(hipStreamSynchronize(s));
+    if (h) out.release();
+    return h == 0;
+}
+
 void launch_rerank(const float* data, const float* queries, int d, int kind, const uint64_t* keys,
-                   int key_stride, const int32_t* cand, int C, int Q, int k, int kprime, double eps,
-                   lsh_point_t* out, hipStream_t s, const int32_t* qorder, const int32_t* rowmap) {
+                   int key_stride, const int32_t* cand, int C, int Q, int k, double eps,
+                   lsh_point_t* out, hipStream_t s, const int32_t* qorder, const int32_t* rowmap,
+                   const _Float16* hdata) {
     if (Q <= 0) return;
     const int Cp = pow2_ceil(std::max(C, 2));
     LSH_REQUIRE(Cp <= kMaxTopK, LSH_ERR_INVALID, "candidate set larger than 4096");
-    const int kp = pow2_ceil(std::max(kprime, k));
-    const size_t lds = (size_t)Cp * (2 * sizeof(uint64_t) + 2 * sizeof(int32_t));
+    LSH_REQUIRE(!hdata || kind == kAngular || kind == kNegCos, LSH_ERR_INTERNAL,
+                "rerank: the FP16 image serves the angular kinds only");
+    const int hs = hdata ? half_row_stride(d) : 0;
+    const size_t lds = (size_t)Cp * (2 * sizeof(uint64_t) + 2 * sizeof(int32_t)) + (size_t)hs * sizeof(float);
     const int per = (Q + 7) / 8;
     auto go = [&](auto kernel) {
         set_dynamic_lds((const void*)kernel, lds);
-        hipLaunchKernelGGL(kernel, dim3(qorder ? 8 * per : Q), dim3(256), lds, s, data, queries, d, keys, key_stride,
-                           cand, C, Cp, k, kp, eps, out, qorder, Q, per, rowmap);
+        hipLaunchKernelGGL(kernel, dim3(qorder ? 8 * per : Q), dim3(256), lds, s, data, hdata, hs, queries, d, keys,
+                           key_stride, cand, C, Cp, k, eps, out, qorder, Q, per, rowmap);
     };
-    switch (kind) {
-        case kNegCos: go(rerank_kernel<kNegCos>); break;
-        case kAngular: go(rerank_kernel<kAngular>); break;
-        case kGenJaccard: go(rerank_kernel<kGenJaccard>); break;
-        case kEuclid: go(rerank_kernel<kEuclid>); break;
-        default: fail(LSH_ERR_INTERNAL, "rerank: unknown distance kind");
+    if (hdata) {
+        if (kind == kAngular) go(rerank_kernel<kAngular, true>);
+        else go(rerank_kernel<kNegCos, true>);
+    } else {
+        switch (kind) {
+            case kNegCos: go(rerank_kernel<kNegCos, false>); break;
+            case kAngular: go(rerank_kernel<kAngular, false>); break;
+            case kGenJaccard: go(rerank_kernel<kGenJaccard, false>); break;
+            case kEuclid: go(rerank_kernel<kEuclid, false>); break;
+            default: fail(LSH_ERR_INTERNAL, "rerank: unknown distance kind");
+        }
     }
     check_launch("rerank_kernel");
 }

@@ -317,6 +317,54 @@ def test_simhash_index_uniform_data_falls_back(lsh, orc):
     _index_vs_oracle(lsh, orc, data, queries, 8, 256, 32)
 
 
+def _rescaled_clusters(rng, N, Q, d, nc=400):
+    """Clustered rows with norms spread over 1e-3..1e3 (the angular distance
+    does not see the scale; the rerank's FP16 image stores unit rows)."""
+    centres = rng.standard_normal((nc, d)).astype(np.float32)
+    data = centres[rng.integers(0, nc, N)] + 0.3 * rng.standard_normal((N, d)).astype(np.float32)
+    queries = centres[rng.integers(0, nc, Q)] + 0.3 * rng.standard_normal((Q, d)).astype(np.float32)
+    data *= (10.0 ** rng.uniform(-3, 3, (N, 1))).astype(np.float32)
+    queries *= (10.0 ** rng.uniform(-3, 3, (Q, 1))).astype(np.float32)
+    return data, queries
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("f32", ["0", "1"])
+def test_simhash_index_rerank_filters_match_oracle(lsh, orc, monkeypatch, f32):
+    # The cell index reranks with an FP16 image of the unit rows ("0") or the
+    # FP32 rows ("1"); both recheck in FP64 every candidate within the error
+    # band of the k-th and must give the oracle's result.  Odd d pads the
+    # FP16 rows; near-duplicate groups (offsets of 1e-4) put hundreds of
+    # candidates inside the FP16 band, exact copies make ties broken by ID.
+    monkeypatch.setenv("LSHGPU_RERANK_F32", f32)
+    rng = np.random.default_rng(85)
+    N, Q, d, words, C, k = 110000, 160, 37, 16, 512, 32
+    data, queries = _rescaled_clusters(rng, N, Q, d)
+    base = rng.integers(0, N, 8)
+    for j, b in enumerate(base):
+        grp = rng.choice(N, 300, replace=False)
+        data[grp] = data[b] * (1 + 1e-4 * rng.standard_normal((300, d)).astype(np.float32))
+        data[grp[:40]] = data[b]
+        queries[j] = data[b] * 0.5
+    _index_vs_oracle(lsh, orc, data, queries, words, C, k)
+
+
+@pytest.mark.gpu
+def test_simhash_index_zero_row_keeps_fp32_filter(lsh, orc, monkeypatch):
+    # A zero row has no unit direction: the index keeps the FP32 filter (no
+    # FP16 image, so fewer device bytes) and still matches the oracle.
+    monkeypatch.delenv("LSHGPU_RERANK_F32", raising=False)
+    rng = np.random.default_rng(86)
+    N, Q, d, words, C, k = 100000, 64, 40, 16, 256, 32
+    data, queries = _rescaled_clusters(rng, N, Q, d)
+    with lsh.Index(data, implementation=3, sketchDim=words, distanceFunc=1, candidates=C) as ix:
+        with_half = ix.device_bytes()
+    data[7] = 0.0
+    with lsh.Index(data, implementation=3, sketchDim=words, distanceFunc=1, candidates=C) as ix:
+        assert with_half - ix.device_bytes() >= 0.9 * N * 40 * 2
+    _index_vs_oracle(lsh, orc, data, queries, words, C, k)
+
+
 # ------------------------------------------------------------ full pipelines
 
 def test_simhash_pipeline_matches_oracle(lsh, orc):
