@@ -195,11 +195,49 @@ __device__ inline void sums_f32(const float* __restrict__ data, const float* __r
 // The same over the FP16 image (rows of hs halves, hs a multiple of 8, zero
 // padded): one 16-byte load per lane covers 8 components, the query comes
 // from LDS (qs, zero padded to hs), and only the dot product is summed.
+// With hs <= 128 a row is one load per lane and the query stays in
+// registers; the U loads of a step are issued before any is used.
 template <int U>
 __device__ inline void sums_f16(const _Float16* __restrict__ hdata, int hs, const float* qs, const int32_t* rows,
                                 int Cp, float2* sums) {
     const int gl = threadIdx.x & 15;
     const int ngroups = blockDim.x >> 4;
+    if (hs <= 128) {
+        const bool act = gl * 8 < hs;
+        const int j = act ? gl * 8 : 0;
+        float4 y0 = *reinterpret_cast<const float4*>(qs + j);
+        float4 y1 = *reinterpret_cast<const float4*>(qs + j + 4);
+        if (!act) y0 = y1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        for (int t0 = threadIdx.x >> 4; t0 < Cp; t0 += ngroups * U) {
+            half8 xv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int t = t0 + u * ngroups;
+                const int r = t < Cp ? rows[t] : -1;
+                xv[u] = *reinterpret_cast<const half8*>(hdata + (size_t)(r < 0 ? 0 : r) * hs + j);
+            }
+            float a[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                a[u] = (float)xv[u][0] * y0.x;
+                a[u] = __builtin_fmaf((float)xv[u][1], y0.y, a[u]);
+                a[u] = __builtin_fmaf((float)xv[u][2], y0.z, a[u]);
+                a[u] = __builtin_fmaf((float)xv[u][3], y0.w, a[u]);
+                a[u] = __builtin_fmaf((float)xv[u][4], y1.x, a[u]);
+                a[u] = __builtin_fmaf((float)xv[u][5], y1.y, a[u]);
+                a[u] = __builtin_fmaf((float)xv[u][6], y1.z, a[u]);
+                a[u] = __builtin_fmaf((float)xv[u][7], y1.w, a[u]);
+                a[u] = row_sum16(a[u]);
+            }
+            float as = a[0];
+#pragma unroll
+            for (int u = 1; u < U; ++u)
+                if (gl == u) as = a[u];
+            const int t = t0 + gl * ngroups;
+            if (gl < U && t < Cp) sums[t] = make_float2(as, 0.0f);
+        }
+        return;
+    }
     for (int t0 = threadIdx.x >> 4; t0 < Cp; t0 += ngroups * U) {
         const _Float16* rp[U];
         float a[U];
@@ -251,11 +289,15 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
                                                      const int32_t* __restrict__ qorder, int Q, int per,
                                                      const int32_t* __restrict__ rowmap) {
     extern __shared__ uint64_t sm[];
-    uint64_t* k32 = sm;                                        // Cp
-    uint64_t* k64 = sm + Cp;                                   // Cp
-    int32_t* ids = reinterpret_cast<int32_t*>(sm + 2 * Cp);    // Cp
-    int32_t* rows = ids + Cp;                                  // Cp: data row of each candidate
-    float* qs = reinterpret_cast<float*>(rows + Cp);           // hs (HALF only)
+    // LDS, in bytes from sm: k64 [0, 8Cp), ids [8Cp, 12Cp), k32 [12Cp,
+    // 20Cp), rows [20Cp, 24Cp), qs [24Cp, 24Cp + 4hs).  k32 and rows are
+    // adjacent: once the band is known they serve as one 12Cp-byte staging
+    // buffer for its rows.
+    uint64_t* k64 = sm;                                              // Cp
+    int32_t* ids = reinterpret_cast<int32_t*>(sm + Cp);              // Cp
+    uint64_t* k32 = reinterpret_cast<uint64_t*>(ids + Cp);           // Cp
+    int32_t* rows = reinterpret_cast<int32_t*>(k32 + Cp);            // Cp: data row of each candidate
+    float* qs = reinterpret_cast<float*>(rows + Cp);                 // hs (HALF only)
     __shared__ uint32_t rs_hist[256];
     __shared__ uint32_t rs_sv[4];
     // With an order, XCD x (workgroups x, x + 8, ...) takes the x-th eighth
@@ -296,7 +338,7 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
     // Approximate distances (k64 serves as scratch for the sums).
     float2* sums = reinterpret_cast<float2*>(k64);
     if (HALF)
-        sums_f16<8>(hdata, hs, qs, rows, Cp, sums);
+        sums_f16<4>(hdata, hs, qs, rows, Cp, sums);
     else
         sums_f32<KIND, 6>(data, qv, d, rows, Cp, sums);
     __syncthreads();
@@ -339,14 +381,43 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
     __syncthreads();
     const int m = (int)rs_sv[2];
     // FP64 for the band.  FP64 keys carry no ID: the sort compares IDs on
-    // equal keys.
+    // equal keys.  Each slot's (row, ID) is parked in k64 first; k32 and
+    // rows are then free and hold chunks of the band's rows, read with
+    // coalesced loads, so that each FP64 sum reads its row from LDS.
     for (int s = threadIdx.x; s < m; s += blockDim.x) {
         const int t = ids[s];
-        const int32_t id = (int32_t)(uint32_t)k32[t];
-        k64[s] = f64_key(dist64_row(KIND, qv, data + (size_t)rows[t] * d, d, nq64));
-        ids[s] = id;
+        k64[s] = ((uint64_t)(uint32_t)rows[t] << 32) | (uint32_t)k32[t];
     }
     __syncthreads();
+    // Staging: k32 and rows, 12 Cp bytes from byte 12 Cp (16-byte aligned
+    // for Cp >= 4); R rows of 4d bytes per chunk, R * 4d <= 12 Cp.
+    float* stage = reinterpret_cast<float*>(k32);
+    const int R = ((d & 3) == 0 && Cp >= 4) ? (3 * Cp) / d : 0;
+    if (R > 0) {
+        const int d4 = d >> 2;
+        for (int c0 = 0; c0 < m; c0 += R) {
+            const int nr = min(R, m - c0);
+            for (int i = threadIdx.x; i < nr * d4; i += blockDim.x) {
+                const int rr = i / d4, j = i - rr * d4;
+                const uint32_t row = (uint32_t)(k64[c0 + rr] >> 32);
+                reinterpret_cast<float4*>(stage)[i] = reinterpret_cast<const float4*>(data + (size_t)row * d)[j];
+            }
+            __syncthreads();
+            for (int s = c0 + threadIdx.x; s < c0 + nr; s += blockDim.x) {
+                const int32_t id = (int32_t)(uint32_t)k64[s];
+                k64[s] = f64_key(dist64_row(KIND, qv, stage + (size_t)(s - c0) * d, d, nq64));
+                ids[s] = id;
+            }
+            __syncthreads();
+        }
+    } else {
+        for (int s = threadIdx.x; s < m; s += blockDim.x) {
+            const uint64_t v = k64[s];
+            k64[s] = f64_key(dist64_row(KIND, qv, data + (size_t)(uint32_t)(v >> 32) * d, d, nq64));
+            ids[s] = (int32_t)(uint32_t)v;
+        }
+        __syncthreads();
+    }
     int n = 1;
     while (n < m) n <<= 1;
     if (n <= 64) {
@@ -354,8 +425,11 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
             uint64_t kk = threadIdx.x < m ? k64[threadIdx.x] : kEmptyKey;
             int32_t ii = threadIdx.x < m ? ids[threadIdx.x] : 0x7FFFFFFF;
             wave_sort_pairs(kk, ii);
-            k64[threadIdx.x] = kk;
-            ids[threadIdx.x] = ii;
+            // n <= Cp: the first n lanes hold the m entries and padding.
+            if ((int)threadIdx.x < n) {
+                k64[threadIdx.x] = kk;
+                ids[threadIdx.x] = ii;
+            }
         }
         __syncthreads();
     } else {

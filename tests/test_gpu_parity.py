@@ -330,15 +330,19 @@ def _rescaled_clusters(rng, N, Q, d, nc=400):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("f32", ["0", "1"])
-def test_simhash_index_rerank_filters_match_oracle(lsh, orc, monkeypatch, f32):
+@pytest.mark.parametrize("d,C", [(37, 512), (64, 512), (40, 24)])
+def test_simhash_index_rerank_filters_match_oracle(lsh, orc, monkeypatch, f32, d, C):
     # The cell index reranks with an FP16 image of the unit rows ("0") or the
     # FP32 rows ("1"); both recheck in FP64 every candidate within the error
     # band of the k-th and must give the oracle's result.  Odd d pads the
     # FP16 rows; near-duplicate groups (offsets of 1e-4) put hundreds of
     # candidates inside the FP16 band, exact copies make ties broken by ID.
+    # With d % 4 == 0 the band's rows are staged through LDS in chunks (many
+    # chunks at C = 512; two rows a chunk at C = 24, where fewer candidates
+    # than k leave padded results).
     monkeypatch.setenv("LSHGPU_RERANK_F32", f32)
-    rng = np.random.default_rng(85)
-    N, Q, d, words, C, k = 110000, 160, 37, 16, 512, 32
+    rng = np.random.default_rng(85 + d + C)
+    N, Q, words, k = 110000, 160, 16, 32
     data, queries = _rescaled_clusters(rng, N, Q, d)
     base = rng.integers(0, N, 8)
     for j, b in enumerate(base):
