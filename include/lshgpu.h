@@ -67,8 +67,10 @@ typedef struct lsh_options {
                            THREAD_QUEUE_SIZE 128, the width of the reference's
                            rerank queue (constants.cuh:6, candidateSetScanner.cuh:21). */
     int device;         /* HIP device ordinal, default 0 (cudaHelpers.cu:16-18) */
-    int rescan_margin;  /* extra FP32 candidates kept past k for the FP64 recheck
-                           (default 32) */
+    int rescan_margin;  /* exact scans: extra FP32 candidates kept past k for the
+                           FP64 recheck (default 32).  The sketch paths' rerank
+                           rechecks in FP64 every candidate within the error
+                           band above the k-th approximate distance instead. */
     int reserved[8];
 } lsh_options_t;
 
