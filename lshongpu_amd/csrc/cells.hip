@@ -178,19 +178,21 @@ __global__ void cell_tile_info_kernel(const int32_t* __restrict__ ids, const uin
 }
 
 // Per cell: distance range of its points to the pivot (first and last point
-// of the cell in layout order); an empty cell gets min > max.
+// of the cell in layout order); an empty cell gets min > max.  crec packs
+// (size, min | max << 16) for the threshold kernel's one-load reads.
 __global__ void cell_range_kernel(const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ ctile,
                                   const uint16_t* __restrict__ pdist, int P, uint16_t* __restrict__ cmin,
-                                  uint16_t* __restrict__ cmax) {
+                                  uint16_t* __restrict__ cmax, uint2* __restrict__ crec) {
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < P; j += gridDim.x * blockDim.x) {
         const uint32_t n = cstart[j + 1] - cstart[j];
-        if (n == 0) {
-            cmin[j] = 1;
-            cmax[j] = 0;
-        } else {
-            cmin[j] = pdist[(size_t)ctile[j] * 32];
-            cmax[j] = pdist[(size_t)ctile[j] * 32 + n - 1];
+        uint32_t lo = 1, hi = 0;
+        if (n != 0) {
+            lo = pdist[(size_t)ctile[j] * 32];
+            hi = pdist[(size_t)ctile[j] * 32 + n - 1];
         }
+        cmin[j] = (uint16_t)lo;
+        cmax[j] = (uint16_t)hi;
+        crec[j] = make_uint2(n, lo | (hi << 16));
     }
 }
 
@@ -273,6 +275,70 @@ __device__ int kth_bin(const uint32_t* hist, int nbins, uint32_t K) {
     return nbins;
 }
 
+// Slot of this lane among the wave's lanes with `pass`, counted from a
+// shared counter (one atomic per wave).  Every lane of the wave must call it.
+__device__ inline uint32_t wave_slot(bool pass, uint32_t* counter) {
+    const unsigned long long m = __ballot(pass);
+    if (m == 0) return 0;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+// Exclusive prefix of v over the NT threads of the workgroup; total = the
+// sum.  ws holds NT / 64 words.  Every thread must call it.
+template <int NT>
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* ws, uint32_t& total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t x = __shfl_up(inc, o);
+        if (lane >= o) inc += x;
+    }
+    if (lane == 63) ws[w] = inc;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NT / 64; ++i) {
+        const uint32_t x = ws[i];
+        pre += i < w ? x : 0u;
+        tot += x;
+    }
+    __syncthreads();
+    total = tot;
+    return pre + inc - v;
+}
+
+// kth_bin by the whole workgroup: each thread sums a run of bins, a block
+// scan places the K-th count in one run, whose thread walks it.  *out
+// (shared) receives the bin; every thread must call it.
+template <int NT>
+__device__ inline int block_kth_bin(const uint32_t* hist, int nbins, uint32_t K, uint32_t* ws, int* out) {
+    const int per = (nbins + NT - 1) / NT;
+    const int b0 = (int)threadIdx.x * per, b1 = min(b0 + per, nbins);
+    uint32_t sum = 0;
+    for (int b = b0; b < b1; ++b) sum += hist[b];
+    uint32_t total;
+    const uint32_t pre = block_exclusive_scan<NT>(sum, ws, total);
+    if (pre < K && pre + sum >= K) {
+        uint32_t run = pre;
+        for (int b = b0; b < b1; ++b) {
+            run += hist[b];
+            if (run >= K) {
+                *out = b;
+                break;
+            }
+        }
+    }
+    if (threadIdx.x == 0 && total < K) *out = nbins;
+    __syncthreads();
+    return *out;
+}
+
 // Threshold of each query (one workgroup per query):
 //  - its nearest non-empty cell (kept for the query order);
 //  - the radius R at which the nearest cells hold K points, from a histogram
@@ -284,28 +350,47 @@ __device__ int kth_bin(const uint32_t* hist, int nbins, uint32_t K) {
 //    0);
 //  - per query, the lists of its first-pass cells (qcells1, n1) and of the
 //    cells whose bound is below that threshold (qcells2, n2; rows of P).
+// Thread i of the NT-thread workgroup owns cells i, i + NT, ... (CPT of
+// them, P <= NT CPT): their distance and packed record are loaded once, in
+// one batch, and every later step reads them from registers.  The tiles of
+// the cells within R + kRadiusMargin (up to NT cells) are spread over all
+// threads, so that one large cell does not hold up the workgroup; cells past
+// the first NT are walked by their owners.
 constexpr int kRadiusMargin = 16;
 constexpr int kWindow = 256;
-__global__ __launch_bounds__(256) void cell_threshold_kernel(const uint16_t* __restrict__ D, int Q, int P, int Pp,
-                                                             const uint32_t* __restrict__ csize,
-                                                             const uint32_t* __restrict__ ctile,
-                                                             const uint32_t* __restrict__ tinfo, int bits, int K,
-                                                             uint64_t* __restrict__ thr, uint16_t* __restrict__ R,
-                                                             uint32_t* __restrict__ nearest,
-                                                             uint32_t* __restrict__ cnt1,
-                                                             const uint16_t* __restrict__ cmin,
-                                                             const uint16_t* __restrict__ cmax,
-                                                             int32_t* __restrict__ qcells1, uint32_t* __restrict__ n1,
-                                                             int32_t* __restrict__ qcells2,
-                                                             uint32_t* __restrict__ n2) {
+template <int CPT, int NT>
+__global__ __launch_bounds__(NT) void cell_threshold_kernel(const uint16_t* __restrict__ D, int Q, int P, int Pp,
+                                                            const uint2* __restrict__ crec,
+                                                            const uint32_t* __restrict__ ctile,
+                                                            const uint32_t* __restrict__ tinfo, int bits, int K,
+                                                            uint64_t* __restrict__ thr, uint16_t* __restrict__ R,
+                                                            uint32_t* __restrict__ nearest,
+                                                            uint32_t* __restrict__ cnt1,
+                                                            int32_t* __restrict__ qcells1, uint32_t* __restrict__ n1,
+                                                            int32_t* __restrict__ qcells2,
+                                                            uint32_t* __restrict__ n2) {
     __shared__ uint32_t h1[kWindow];
     __shared__ uint32_t h2[2049];
-    __shared__ uint16_t row[kMaxPivots];  // the query's distances to the pivots
-    __shared__ int rad;
-    __shared__ uint32_t best, s_n1, s_n2, s_h;
+    __shared__ uint32_t l_t0[NT], l_pre[NT + 1];  // listed cells: first tile, prefix of tile counts
+    __shared__ uint16_t l_dq[NT];                 // listed cells: distance to the query
+    __shared__ uint32_t ws[NT / 64];
+    __shared__ int s_bin;
+    __shared__ uint32_t best, s_n1, s_n2, s_nl;
     const int q = blockIdx.x;
     const uint16_t* __restrict__ drow = D + (size_t)q * Pp;
-    for (int b = threadIdx.x; b < 2049; b += 256) {
+    uint32_t dq[CPT], sz[CPT], rg[CPT];
+#pragma unroll
+    for (int i = 0; i < CPT; ++i) {
+        const int j = (int)threadIdx.x + NT * i;
+        const bool in = j < P;
+        const int jj = in ? j : 0;  // every load in bounds (P >= 1); values masked
+        const uint32_t a = drow[jj];
+        const uint2 r = crec[jj];
+        dq[i] = in ? a : 0u;
+        sz[i] = in ? r.x : 0u;
+        rg[i] = in ? r.y : 1u;  // min 1 > max 0: empty
+    }
+    for (int b = threadIdx.x; b < 2049; b += NT) {
         if (b < kWindow) h1[b] = 0;
         h2[b] = 0;
     }
@@ -313,62 +398,106 @@ __global__ __launch_bounds__(256) void cell_threshold_kernel(const uint16_t* __r
         best = 0xFFFFFFFFu;
         s_n1 = 0;
         s_n2 = 0;
+        s_nl = 0;
     }
     __syncthreads();
     uint32_t mine = 0xFFFFFFFFu;  // (distance, pivot) of the nearest non-empty cell
-    for (int j = threadIdx.x; j < P; j += 256) {
-        const uint16_t dj = drow[j];
-        row[j] = dj;
-        if (csize[j]) mine = min(mine, ((uint32_t)dj << 13) | (uint32_t)j);
-    }
+#pragma unroll
+    for (int i = 0; i < CPT; ++i)
+        if (sz[i]) mine = min(mine, (dq[i] << 13) | (uint32_t)(threadIdx.x + NT * i));
     atomicMin(&best, mine);
     __syncthreads();
     const uint32_t dmin = best == 0xFFFFFFFFu ? 0u : best >> 13;
     if (threadIdx.x == 0) nearest[q] = best == 0xFFFFFFFFu ? 0u : (best & 0x1FFFu);
-    for (int j = threadIdx.x; j < P; j += 256) {
-        const uint32_t n = csize[j];
-        const uint32_t dd = (uint32_t)row[j] - dmin;
-        if (n && dd < (uint32_t)kWindow) atomicAdd(&h1[dd], n);
+#pragma unroll
+    for (int i = 0; i < CPT; ++i) {
+        const uint32_t dd = dq[i] - dmin;
+        if (sz[i] && dd < (uint32_t)kWindow) atomicAdd(&h1[dd], sz[i]);
     }
     __syncthreads();
-    if (threadIdx.x < 64) {
-        const int r = kth_bin(h1, kWindow, (uint32_t)K);
-        if (threadIdx.x == 0) {
-            const int rr = r < kWindow ? (int)dmin + r : bits;
-            rad = rr + kRadiusMargin;
-            R[q] = (uint16_t)min(rr, bits);
-        }
-    }
-    __syncthreads();
-    const int rm = rad;
-    const int r0 = rm - kRadiusMargin;
-    for (int j = threadIdx.x; j < P; j += 256) {
-        const int dq = row[j];
-        if (dq > rm || csize[j] == 0) continue;
-        if (dq <= r0) {
+    const int r1 = block_kth_bin<NT>(h1, kWindow, (uint32_t)K, ws, &s_bin);
+    const int rr = r1 < kWindow ? (int)dmin + r1 : bits;
+    if (threadIdx.x == 0) R[q] = (uint16_t)min(rr, bits);
+    const int rm = rr + kRadiusMargin;
+    const int r0 = rr;
+    int32_t* __restrict__ q1 = qcells1 + (size_t)q * P;
+#pragma unroll
+    for (int i = 0; i < CPT; ++i) {
+        const int j = (int)threadIdx.x + NT * i;
+        const bool pass = sz[i] != 0 && (int)dq[i] <= rm;
+        const bool first = pass && (int)dq[i] <= r0;
+        const uint32_t s1 = wave_slot(first, &s_n1);
+        if (first) {
             atomicAdd(&cnt1[j], 1u);
-            qcells1[(size_t)q * P + atomicAdd(&s_n1, 1u)] = j;
+            q1[s1] = j;
         }
-        for (uint32_t t = ctile[j]; t < ctile[j + 1]; ++t) {
-            const uint32_t info = tinfo[t];
-            atomicAdd(&h2[dq + ((info >> kKeyShift) & kDistMask)], info >> 22);
+        const uint32_t sl = wave_slot(pass, &s_nl);
+        if (pass) {
+            const uint32_t t0 = ctile[j], t1 = ctile[j + 1];
+            if (sl < (uint32_t)NT) {
+                l_t0[sl] = t0;
+                l_pre[sl] = t1 - t0;
+                l_dq[sl] = (uint16_t)dq[i];
+            } else {
+                for (uint32_t t = t0; t < t1; t += 8) {
+                    uint32_t info[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) info[u] = tinfo[min(t + u, t1 - 1u)];  // t < t1: in bounds
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+                        if (t + u < t1) atomicAdd(&h2[dq[i] + ((info[u] >> kKeyShift) & kDistMask)], info[u] >> 22);
+                }
+            }
         }
     }
     __syncthreads();
-    if (threadIdx.x < 64) {
-        const int ub = kth_bin(h2, 2 * bits + 1, (uint32_t)K);
-        if (threadIdx.x == 0) {
-            thr[q] = ub > 2 * bits ? kEmptyKey : make_key((uint32_t)ub + 1u, 0u);
-            s_h = ub > 2 * bits ? 0xFFFFFFFFu : (uint32_t)ub + 1u;
+    // The listed cells' tiles, flattened: tile f of the list belongs to the
+    // cell c with l_pre[c] <= f < l_pre[c + 1].
+    const int nl = (int)min(s_nl, (uint32_t)NT);
+    {
+        const uint32_t cntv = (int)threadIdx.x < nl ? l_pre[threadIdx.x] : 0u;
+        uint32_t total;
+        const uint32_t pre = block_exclusive_scan<NT>(cntv, ws, total);
+        if ((int)threadIdx.x < nl) l_pre[threadIdx.x] = pre;
+        if (threadIdx.x == 0) l_pre[nl] = total;
+        __syncthreads();
+        for (uint32_t f0 = 0; f0 < total; f0 += 4u * NT) {
+            uint32_t info[4], dv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t f = f0 + (uint32_t)u * NT + threadIdx.x;
+                int lo = 0, hi = nl;
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (l_pre[mid] <= f) lo = mid;
+                    else hi = mid;
+                }
+                const bool ok = f < total;
+                // In bounds either way: a listed cell's first tile plus an
+                // offset below its count, or tile 0.
+                const uint32_t t = ok ? l_t0[lo] + (f - l_pre[lo]) : 0u;
+                info[u] = tinfo[t];
+                dv[u] = ok ? l_dq[lo] : 0xFFFFFFFFu;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (dv[u] != 0xFFFFFFFFu) atomicAdd(&h2[dv[u] + ((info[u] >> kKeyShift) & kDistMask)], info[u] >> 22);
         }
     }
     __syncthreads();
+    const int ub = block_kth_bin<NT>(h2, 2 * bits + 1, (uint32_t)K, ws, &s_bin);
+    if (threadIdx.x == 0) thr[q] = ub > 2 * bits ? kEmptyKey : make_key((uint32_t)ub + 1u, 0u);
     // The cells the second pass can need: the final threshold is at most this
     // one, so the cells whose bound is below it are a superset.
-    const uint32_t h = s_h;
-    for (int j = threadIdx.x; j < P; j += 256) {
-        const uint32_t lo = cmin[j], hi = cmax[j];
-        if (lo <= hi && lower_bound_to(row[j], lo, hi) < h) qcells2[(size_t)q * P + atomicAdd(&s_n2, 1u)] = j;
+    const uint32_t h = ub > 2 * bits ? 0xFFFFFFFFu : (uint32_t)ub + 1u;
+    int32_t* __restrict__ q2 = qcells2 + (size_t)q * P;
+#pragma unroll
+    for (int i = 0; i < CPT; ++i) {
+        const int j = (int)threadIdx.x + NT * i;
+        const uint32_t lo = rg[i] & 0xFFFFu, hi = rg[i] >> 16;
+        const bool keep = lo <= hi && lower_bound_to(dq[i], lo, hi) < h;
+        const uint32_t s2 = wave_slot(keep, &s_n2);
+        if (keep) q2[s2] = j;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -840,8 +969,9 @@ void CellIndex::build(const uint32_t* ds, int64_t N, int wpad, hipStream_t s) {
     check_launch("cell_tile_info_kernel");
     cmin.alloc((size_t)np);
     cmax.alloc((size_t)np);
+    crec.alloc((size_t)np);
     hipLaunchKernelGGL(cell_range_kernel, dim3(blocks_for((size_t)np, 256)), dim3(256), 0, s, cstart.get(),
-                       ctile.get(), pdist.get(), np, cmin.get(), cmax.get());
+                       ctile.get(), pdist.get(), np, cmin.get(), cmax.get(), crec.get());
     check_launch("cell_range_kernel");
     sk.alloc((size_t)npos * wpad);
     hipLaunchKernelGGL(pack_cells_kernel, dim3(blocks_for((size_t)npos * wpad, 256)), dim3(256), 0, s, ds, wpad,
@@ -863,7 +993,7 @@ void cell_rows(const CellIndex& c, const float* data, int64_t N, int d, DevBuf<f
 
 size_t CellIndex::bytes() const {
     return pfp4.bytes() + sk.bytes() + ids.bytes() + tinfo.bytes() + ctile.bytes() + csize.bytes() + cmin.bytes() +
-           cmax.bytes();
+           cmax.bytes() + crec.bytes();
 }
 
 const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K, const TopKView& v, Scratch& sc,
@@ -912,9 +1042,15 @@ const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K
     pivots(c.words, qs, Q, c.pfp4.get(), P, Pp, nullptr, D, s);
     // Threshold bound from the tile ranges, the first-pass radii and their
     // cell counts, and the nearest cell of each query.
-    hipLaunchKernelGGL(cell_threshold_kernel, dim3((unsigned)Q), dim3(256), 0, s, D, Q, P, Pp, c.csize.get(),
-                       c.ctile.get(), c.tinfo.get(), bits, K, v.thr, R, nearest, cnt1, c.cmin.get(), c.cmax.get(),
-                       qcells1, n1, qcells2, n2);
+    auto threshold = [&](auto kernel, int nt) {
+        hipLaunchKernelGGL(kernel, dim3((unsigned)Q), dim3(nt), 0, s, D, Q, P, Pp, c.crec.get(), c.ctile.get(),
+                           c.tinfo.get(), bits, K, v.thr, R, nearest, cnt1, qcells1, n1, qcells2, n2);
+    };
+    LSH_REQUIRE(P >= 1 && P <= 256 * 32, LSH_ERR_INTERNAL, "cell_search: pivot count outside the threshold kernel's range");
+    if (P <= 256 * 4) threshold(cell_threshold_kernel<4, 256>, 256);
+    else if (P <= 256 * 8) threshold(cell_threshold_kernel<8, 256>, 256);
+    else if (P <= 256 * 16) threshold(cell_threshold_kernel<16, 256>, 256);
+    else threshold(cell_threshold_kernel<32, 256>, 256);
     check_launch("cell_threshold_kernel");
     if (qorder) {
         // Queries grouped by nearest cell (a counting sort): neighbouring

@@ -121,6 +121,7 @@ struct CellIndex {
     DevBuf<uint32_t> ctile;        // [P+1] first tile of each cell
     DevBuf<uint32_t> csize;        // [P] points of each cell
     DevBuf<uint16_t> cmin, cmax;   // per cell: distance range to the pivot (min > max: empty)
+    DevBuf<uint2> crec;            // per cell: (size, min | max << 16)
     void build(const uint32_t* ds, int64_t N, int wpad, hipStream_t s);
     size_t bytes() const;
     bool built() const { return P > 0; }
