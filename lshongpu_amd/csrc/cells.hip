@@ -245,12 +245,6 @@ __global__ void cell_clear_kernel(uint32_t* __restrict__ cnt, int ncnt, uint32_t
     for (size_t i = i0 + (size_t)gridDim.x * blockDim.x; i < (size_t)ncnt; i += (size_t)gridDim.x * blockDim.x) cnt[i] = 0u;
 }
 
-// cmap[c] = the cell of chunk c (choff: exclusive prefix of chunks per cell).
-__global__ void chunk_map_kernel(const uint32_t* __restrict__ choff, int P, uint32_t* __restrict__ cmap) {
-    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < P; j += gridDim.x * blockDim.x)
-        for (uint32_t c = choff[j]; c < choff[j + 1]; ++c) cmap[c] = (uint32_t)j;
-}
-
 __device__ inline uint32_t lower_bound_to(uint32_t dq, uint32_t lo, uint32_t hi) {
     return dq > hi ? dq - hi : (lo > dq ? lo - dq : 0u);
 }
@@ -506,6 +500,44 @@ __global__ __launch_bounds__(NT) void cell_threshold_kernel(const uint16_t* __re
     }
 }
 
+// The queries grouped by nearest cell (a counting sort by one workgroup,
+// counts in LDS): order[] lists the queries of cell 0, then of cell 1, ...;
+// the order inside a cell is not fixed.
+__global__ __launch_bounds__(1024) void order_by_cell_kernel(const uint32_t* __restrict__ nearest, int Q, int P,
+                                                             int32_t* __restrict__ order) {
+    extern __shared__ uint32_t cnt[];  // P
+    __shared__ uint32_t ws[16];
+    for (int j = threadIdx.x; j < P; j += 1024) cnt[j] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < Q; i += 1024) atomicAdd(&cnt[min(nearest[i], (uint32_t)(P - 1))], 1u);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t carry = 0;
+    for (int c0 = 0; c0 < P; c0 += 1024) {
+        const int j = c0 + (int)threadIdx.x;
+        const uint32_t a = j < P ? cnt[j] : 0u;
+        uint32_t inc = a;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t x = __shfl_up(inc, o);
+            if (lane >= o) inc += x;
+        }
+        if (lane == 63) ws[w] = inc;
+        __syncthreads();
+        uint32_t pre = 0, tot = 0;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            pre += i < w ? ws[i] : 0u;
+            tot += ws[i];
+        }
+        if (j < P) cnt[j] = carry + pre + inc - a;
+        carry += tot;
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < Q; i += 1024)
+        order[atomicAdd(&cnt[min(nearest[i], (uint32_t)(P - 1))], 1u)] = i;
+}
+
 // A chunk is up to 32 queries of one cell over up to kChunkTiles of its
 // tiles, so that no wave walks a long cell alone.
 constexpr int kChunkTiles = 8;
@@ -517,11 +549,13 @@ __device__ inline uint32_t cell_parts(const uint32_t* ctile, int j) {
 // chunks (ceil(cnt/32) query groups x cell_parts tile ranges), both with a
 // final total entry; choff[P+1] and *work_out = the tile products those
 // chunks can make (query groups x tiles, summed); *dense (may be null) = 1
-// when that exceeds dense_limit; cursor (= cnt for the placing pass) zeroed.
+// when that exceeds dense_limit; cursor (= cnt for the placing pass) zeroed;
+// cmap[c] = the cell of chunk c.
 __global__ __launch_bounds__(1024) void cell_offsets_kernel(uint32_t* __restrict__ cnt, int P,
                                                             const uint32_t* __restrict__ ctile,
                                                             uint32_t* __restrict__ coff,
                                                             uint32_t* __restrict__ choff,
+                                                            uint32_t* __restrict__ cmap,
                                                             uint32_t* __restrict__ work_out,
                                                             uint32_t* __restrict__ dense, uint32_t dense_limit) {
     __shared__ uint32_t ws[3][16];
@@ -560,9 +594,11 @@ __global__ __launch_bounds__(1024) void cell_offsets_kernel(uint32_t* __restrict
             tw += ws[2][q];
         }
         if (j < P) {
+            const uint32_t c0 = carry_b + pb + ib - b;
             coff[j] = carry_a + pa + ia - a;
-            choff[j] = carry_b + pb + ib - b;
+            choff[j] = c0;
             cnt[j] = 0;
+            for (uint32_t c = 0; c < b; ++c) cmap[c0 + c] = (uint32_t)j;
         }
         carry_a += ta;
         carry_b += tb;
@@ -845,13 +881,10 @@ void launch_cell_scan_ks(const CellIndex& c, unsigned grid, const uint32_t* coff
     check_launch("cell_scan_kernel");
 }
 
-// The chunk -> cell map (after the lists' offsets), then the scan over a
-// grid of `grid` workgroups that loops over the chunks.
+// The scan over a grid of `grid` workgroups that loops over the chunks.
 void cell_scan(const CellIndex& c, unsigned grid, const uint32_t* coff, const uint32_t* choff, uint32_t* cmap,
                int Pp, const int32_t* list, const uint16_t* D, const uint32_t* qs, const TopKView& v, uint32_t* hist,
                const uint32_t* skip, hipStream_t s) {
-    hipLaunchKernelGGL(chunk_map_kernel, dim3(blocks_for((size_t)c.P, 256)), dim3(256), 0, s, choff, c.P, cmap);
-    check_launch("chunk_map_kernel");
     switch (c.words) {
         case 8: launch_cell_scan_ks<4>(c, grid, coff, choff, cmap, Pp, list, D, qs, v, hist, skip, s); break;
         case 16: launch_cell_scan_ks<8>(c, grid, coff, choff, cmap, Pp, list, D, qs, v, hist, skip, s); break;
@@ -861,10 +894,10 @@ void cell_scan(const CellIndex& c, unsigned grid, const uint32_t* coff, const ui
     }
 }
 
-void list_offsets(const CellIndex& c, uint32_t* cnt, uint32_t* coff, uint32_t* choff, uint32_t* work_out,
-                  uint32_t* dense, uint32_t dense_limit, hipStream_t s) {
-    hipLaunchKernelGGL(cell_offsets_kernel, dim3(1), dim3(1024), 0, s, cnt, c.P, c.ctile.get(), coff, choff, work_out,
-                       dense, dense_limit);
+void list_offsets(const CellIndex& c, uint32_t* cnt, uint32_t* coff, uint32_t* choff, uint32_t* cmap,
+                  uint32_t* work_out, uint32_t* dense, uint32_t dense_limit, hipStream_t s) {
+    hipLaunchKernelGGL(cell_offsets_kernel, dim3(1), dim3(1024), 0, s, cnt, c.P, c.ctile.get(), coff, choff, cmap,
+                       work_out, dense, dense_limit);
     check_launch("cell_offsets_kernel");
 }
 
@@ -1003,17 +1036,15 @@ const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K
     const int Pp = (P + 31) / 32 * 32;
     const int bits = 32 * c.words;
     uint16_t* D = sc.get<uint16_t>(depth, 0, (size_t)Q * Pp);
-    // meta: cnt1, cnt2 [P+1 each], coff [P+1], choff [P+2], the query sort's
-    // offsets and cursor [P+1 each], nearest cell [Q], stat [4] = {second pass
-    // too dense, first-pass tile products, second-pass tile products}.
-    uint32_t* meta = sc.get<uint32_t>(depth, 1, 6 * ((size_t)P + 1) + 1 + Q + 4);
+    // meta: cnt1, cnt2 [P+1 each], coff [P+1], choff [P+2], nearest cell [Q],
+    // stat [4] = {second pass too dense, first-pass tile products, second-pass
+    // tile products}.
+    uint32_t* meta = sc.get<uint32_t>(depth, 1, 4 * ((size_t)P + 1) + 1 + Q + 4);
     uint32_t* cnt1 = meta;
     uint32_t* cnt2 = meta + (P + 1);
     uint32_t* coff = meta + 2 * (P + 1);
     uint32_t* choff = meta + 3 * (P + 1);
-    uint32_t* sort_off = meta + 4 * (P + 1) + 1;
-    uint32_t* sort_cur = meta + 5 * (P + 1) + 1;
-    uint32_t* nearest = meta + 6 * (P + 1) + 1;
+    uint32_t* nearest = meta + 4 * (P + 1) + 1;
     uint32_t* stat = nearest + Q;
     // hist [Q][kHistBins], the per-query cell lists [Q][P] x 2 and their
     // lengths [Q] x 2, the radii [Q] (u16).
@@ -1055,12 +1086,14 @@ const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K
     if (qorder) {
         // Queries grouped by nearest cell (a counting sort): neighbouring
         // queries then share most candidates, which the rerank can reuse.
-        launch_bucket_build(nearest, Q, 1, (uint32_t)P, sort_off, sort_cur, reinterpret_cast<uint32_t*>(qorder), s);
+        hipLaunchKernelGGL(order_by_cell_kernel, dim3(1), dim3(1024), sizeof(uint32_t) * (size_t)P, s, nearest, Q,
+                           P, qorder);
+        check_launch("order_by_cell_kernel");
     }
     // First pass: exact distances over the nearest cells, K-th by histogram,
     // then the second pass's cell counts under the final threshold.
     const unsigned qwaves = (unsigned)((Q + 3) / 4);
-    list_offsets(c, cnt1, coff, choff, stat + 1, nullptr, 0, s);
+    list_offsets(c, cnt1, coff, choff, cmap, stat + 1, nullptr, 0, s);
     hipLaunchKernelGGL(place_from_cells_kernel, dim3(qwaves), dim3(256), 0, s, qcells1, n1, Q, P, cnt1, coff, list,
                        nullptr);
     check_launch("place_from_cells_kernel");
@@ -1069,7 +1102,7 @@ const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K
                        c.cmin.get(), c.cmax.get(), cnt2, qcells2, n2);
     check_launch("hist_threshold_kernel");
     // Second pass: the tiles the bounds leave (skipped when too dense).
-    list_offsets(c, cnt2, coff, choff, stat + 2, stat, dense_limit, s);
+    list_offsets(c, cnt2, coff, choff, cmap, stat + 2, stat, dense_limit, s);
     hipLaunchKernelGGL(place_from_cells_kernel, dim3(qwaves), dim3(256), 0, s, qcells2, n2, Q, P, cnt2, coff, list,
                        stat);
     check_launch("place_from_cells_kernel");

@@ -227,13 +227,10 @@ void sketch_topk(const SketchSet& d, const SketchSet* samples, int nsamples, int
             pt.begin(PH_SELECT, s);
             launch_select_merge(st.view(), s);
             pt.end(PH_SELECT, s);
-            uint32_t* flags = sc.get<uint32_t>(depth, 0, (size_t)Q);
-            launch_incomplete_flags(st.view(), flags, s);
-            // The flags (and extra words) go to pinned memory ahead of the
-            // hook's work, so the host reads them while that work runs.
+            // The flags (and extra words) are written to pinned memory ahead
+            // of the hook's work, so the host reads them while that work runs.
             uint32_t* hb = sc.pinned((size_t)Q + n_extra);
-            LSH_HIP(hipMemcpyAsync(hb, flags, sizeof(uint32_t) * (size_t)Q, hipMemcpyDeviceToHost, s));
-            if (extra) LSH_HIP(hipMemcpyAsync(hb + Q, extra, sizeof(uint32_t) * n_extra, hipMemcpyDeviceToHost, s));
+            launch_incomplete_flags(st.view(), hb, s, extra, extra ? n_extra : 0);
             const hipEvent_t ev = sc.event();
             LSH_HIP(hipEventRecord(ev, s));
             if (d.hook && d.hook->fn) {

@@ -216,7 +216,8 @@ void launch_count_overflow(const TopKView& v, uint32_t* out_count, hipStream_t s
 // appends exactly the pairs at or below that distance.
 void launch_threshold_above(const TopKView& src, const TopKView& dst, hipStream_t s);
 // flags[q] = 1 when query q overflowed or holds fewer than K keys.
-void launch_incomplete_flags(const TopKView& v, uint32_t* flags, hipStream_t s);
+void launch_incomplete_flags(const TopKView& v, uint32_t* flags, hipStream_t s, const uint32_t* extra = nullptr,
+                             int n_extra = 0);
 
 // scan_exact.hip: distances of queries [0,Q) to points [p0,p1) of data.
 struct ExactScanArgs {

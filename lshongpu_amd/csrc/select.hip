@@ -256,10 +256,12 @@ __global__ void threshold_above_kernel(TopKView src, TopKView dst) {
     dst.thr[q] = t == kEmptyKey ? kEmptyKey : make_key((uint32_t)(t >> 32) + 1u, 0u);
 }
 
-__global__ void incomplete_flags_kernel(TopKView v, uint32_t* flags) {
+// flags[q] = 1 when query q's list is incomplete; flags[Q + i] = extra[i].
+// flags may be pinned host memory (the host reads it after an event).
+__global__ void incomplete_flags_kernel(TopKView v, uint32_t* flags, const uint32_t* extra, int n_extra) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= v.Q) return;
-    flags[q] = (v.ovf[q] != 0u || v.thr[q] == kEmptyKey) ? 1u : 0u;
+    if (q < v.Q) flags[q] = (v.ovf[q] != 0u || v.thr[q] == kEmptyKey) ? 1u : 0u;
+    if (q < n_extra) flags[v.Q + q] = extra[q];
 }
 
 unsigned grid_for(size_t n) {
@@ -327,8 +329,10 @@ void launch_threshold_above(const TopKView& src, const TopKView& dst, hipStream_
     check_launch("threshold_above_kernel");
 }
 
-void launch_incomplete_flags(const TopKView& v, uint32_t* flags, hipStream_t s) {
-    hipLaunchKernelGGL(incomplete_flags_kernel, dim3((v.Q + 255) / 256), dim3(256), 0, s, v, flags);
+void launch_incomplete_flags(const TopKView& v, uint32_t* flags, hipStream_t s, const uint32_t* extra, int n_extra) {
+    const int n = std::max(v.Q, n_extra);
+    if (n <= 0) return;
+    hipLaunchKernelGGL(incomplete_flags_kernel, dim3((n + 255) / 256), dim3(256), 0, s, v, flags, extra, n_extra);
     check_launch("incomplete_flags_kernel");
 }
 
