@@ -226,6 +226,7 @@ void sketch_topk(const SketchSet& d, const SketchSet* samples, int nsamples, int
         auto select_and_flag = [&](const uint32_t* extra, uint32_t* h_extra, int n_extra) {
             pt.begin(PH_SELECT, s);
             launch_select_merge(st.view(), s);
+            st.selected();
             pt.end(PH_SELECT, s);
             // The flags (and extra words) are written to pinned memory ahead
             // of the hook's work, so the host reads them while that work runs.
@@ -244,7 +245,9 @@ void sketch_topk(const SketchSet& d, const SketchSet* samples, int nsamples, int
             for (int i = 0; i < n_extra; ++i) h_extra[i] = hb[Q + i];
             return bad;
         };
-        st.reset(Q, K, 2 * K + 512, s, true);
+        // One select follows each reset here, so the best lists need no
+        // clearing (lazy reset).
+        st.reset(Q, K, 2 * K + 512, s, true, true);
         const uint32_t* cstat = nullptr;
         if (d.cells && !force_full_pass()) {
             // Threshold from the nearest pivot cells, then the cells the
@@ -262,7 +265,7 @@ void sketch_topk(const SketchSet& d, const SketchSet* samples, int nsamples, int
         if (cstat) {
             if (hstat[0]) {
                 // Too dense for the cells: nothing was appended.
-                st.reset(Q, K, 2 * K + 512, s, true);
+                st.reset(Q, K, 2 * K + 512, s, true, true);
                 sampled_pass();
                 bad = select_and_flag(nullptr, nullptr, 0);  // runs the hook again over the new candidates
             } else {

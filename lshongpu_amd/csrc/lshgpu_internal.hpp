@@ -178,14 +178,19 @@ struct TopKView {
     int stride;  // K + newcap
     int Q;
     int int_keys;  // 1: integer distances, best list unordered (select_hist_kernel)
+    int fresh;     // 1: the best lists are empty and not stored (lazy reset; the next select reads only appends)
 };
 
 class TopKState {
    public:
     // int_keys: the distances are small integers; the best list is then an
     // unordered set of exactly K keys and K need not be a power of two.
-    void reset(int Q, int K, int newcap, hipStream_t s, bool int_keys = false);
+    // lazy (integer keys only): the best lists are not written; the next
+    // select treats them as empty and must be followed by selected().
+    void reset(int Q, int K, int newcap, hipStream_t s, bool int_keys = false, bool lazy = false);
     TopKView view() const { return v_; }
+    // After the select that follows a lazy reset: the best lists are stored.
+    void selected() { v_.fresh = 0; }
     size_t bytes() const { return keys_.bytes() + cnt_.bytes() + thr_.bytes() + ovf_.bytes(); }
 
    private:

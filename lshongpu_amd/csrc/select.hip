@@ -118,11 +118,14 @@ __global__ __launch_bounds__(kSelThreads) void select_hist_kernel(TopKView v) {
     __shared__ uint32_t s_T, s_below, s_nkeep, s_dig, s_rank, s_nties;
     const int q = blockIdx.x;
     const uint32_t n_raw = v.cnt[q];
-    if (n_raw == 0) return;
+    if (n_raw == 0 && !v.fresh) return;
     const int K = v.K;
     const int n = (int)min(n_raw, (uint32_t)v.newcap);
-    const int total = K + n;
     uint64_t* row = v.keys + (size_t)q * v.stride;
+    // A fresh view has no stored best list: only the appends are read (the
+    // K written below are disjoint from them).
+    const uint64_t* in = v.fresh ? row + K : row;
+    const int total = v.fresh ? n : K + n;
     for (int i = threadIdx.x; i < kSelBins; i += blockDim.x) hist[i] = 0;
     if (threadIdx.x == 0) {
         s_nkeep = 0;
@@ -130,7 +133,7 @@ __global__ __launch_bounds__(kSelThreads) void select_hist_kernel(TopKView v) {
     }
     __syncthreads();
     for (int i = threadIdx.x; i < total; i += blockDim.x) {
-        const uint64_t key = row[i];
+        const uint64_t key = in[i];
         if (key != kEmptyKey) atomicAdd(&hist[min((uint32_t)(key >> 32), (uint32_t)(kSelBins - 1))], 1u);
     }
     __syncthreads();
@@ -172,12 +175,12 @@ __global__ __launch_bounds__(kSelThreads) void select_hist_kernel(TopKView v) {
         const bool last = T == (uint32_t)(kSelBins - 1);
         const uint32_t nt = hist[T];  // keys in the boundary bin
         __syncthreads();
-        const uint64_t* src = row;
+        const uint64_t* src = in;
         int nsrc = total;
         if (nt <= (uint32_t)kTieCap) {
             // Gather them once; the radix passes then run over LDS.
             for (int i = threadIdx.x; i < total; i += blockDim.x) {
-                const uint64_t key = row[i];
+                const uint64_t key = in[i];
                 if (key != kEmptyKey && min((uint32_t)(key >> 32), (uint32_t)(kSelBins - 1)) == T)
                     ties[atomicAdd(&s_nties, 1u)] = key;
             }
@@ -189,7 +192,7 @@ __global__ __launch_bounds__(kSelThreads) void select_hist_kernel(TopKView v) {
                                hist, &s_dig, &s_rank);
     }
     for (int i = threadIdx.x; i < total; i += blockDim.x) {
-        const uint64_t key = row[i];
+        const uint64_t key = in[i];
         if (key != kEmptyKey && key <= kth) {
             const uint32_t at = atomicAdd(&s_nkeep, 1u);
             if (at < (uint32_t)K) kept[at] = key;
@@ -215,13 +218,16 @@ __global__ __launch_bounds__(kSelThreads) void sort_rows_kernel(uint64_t* keys, 
     for (int i = threadIdx.x; i < K; i += blockDim.x) row[i] = sm[i];
 }
 
-// Reset of a top-K state: the first K keys of every row (the best lists)
-// and the threshold empty, the counts and overflow flags zero.
+// Reset of a top-K state: the first K keys of every row (the best lists,
+// unless the view is fresh) and the threshold empty, the counts and overflow
+// flags zero.
 __global__ void reset_state_kernel(TopKView v) {
-    const size_t total = (size_t)v.Q * v.K;
+    const size_t total = v.fresh ? (size_t)v.Q : (size_t)v.Q * v.K;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
-        const size_t q = i / v.K;
-        v.keys[q * v.stride + (i - q * v.K)] = kEmptyKey;
+        if (!v.fresh) {
+            const size_t q = i / v.K;
+            v.keys[q * v.stride + (i - q * v.K)] = kEmptyKey;
+        }
         if (i < (size_t)v.Q) {
             v.thr[i] = kEmptyKey;
             v.cnt[i] = 0u;
@@ -279,15 +285,17 @@ int pow2_ceil(int x) {
 
 int newcap_for(int K) { return K <= 1024 ? 2 * K + 256 : K + 512; }
 
-void TopKState::reset(int Q, int K, int newcap, hipStream_t s, bool int_keys) {
+void TopKState::reset(int Q, int K, int newcap, hipStream_t s, bool int_keys, bool lazy) {
     LSH_REQUIRE(K >= 1 && K <= kMaxTopK && (int_keys || (K & (K - 1)) == 0), LSH_ERR_INVALID,
                 "top-K size must be a power of two <= 4096");
+    LSH_REQUIRE(!lazy || (int_keys && K <= kMaxIntTopK), LSH_ERR_INTERNAL, "lazy reset needs the integer-key select");
     const int stride = K + newcap;
     keys_.ensure((size_t)Q * stride);
     cnt_.ensure((size_t)Q);
     thr_.ensure((size_t)Q);
     ovf_.ensure((size_t)Q);
-    v_ = TopKView{keys_.get(), cnt_.get(), thr_.get(), ovf_.get(), K, newcap, stride, Q, int_keys ? 1 : 0};
+    v_ = TopKView{keys_.get(), cnt_.get(), thr_.get(), ovf_.get(), K, newcap, stride, Q, int_keys ? 1 : 0,
+                  lazy ? 1 : 0};
     // A fill kernel rather than memsets keeps the sequence capturable in a
     // graph.  Only the best lists need clearing: the selects read the
     // appended slots up to each query's count.
