@@ -695,12 +695,38 @@ __global__ __launch_bounds__(256) void cell_scan_kernel(const uint32_t* __restri
         for (int s = 0; s < KS; ++s) b[s] = expand_word(q >= 0 ? qs[(size_t)q * W + 2 * s + (lane >> 5)] : 0u);
         const uint32_t t0 = ctile[j] + (local % parts) * kChunkTiles;
         const uint32_t nt = min((uint32_t)kChunkTiles, ctile[j + 1] - t0);
-        // Lane l's half of point row l%32 of tile t: W/2 words.
-        auto load = [&](uint32_t t, uint4* x) {
+        // Lane l's half of point row l%32 of tile t: W/2 words (and, when
+        // appending, the ID of that point).
+        auto load = [&](uint32_t t, uint4* x, int32_t& id) {
             const uint4* src = reinterpret_cast<const uint4*>(sk + ((size_t)t * 32 + col) * W + (lane >> 5) * KS);
     #pragma unroll
             for (int i = 0; i < NV; ++i) x[i] = src[i];
+            if (!kHist) id = ids[(size_t)t * 32 + col];
         };
+        // Appends of one tile: slots from the count atomic's old value `ret`
+        // (lanes < 32; `other` = the upper half's hits), IDs from the tile's
+        // ID lanes.  Called one tile late, so the atomic's round trip
+        // overlaps the next tile's products.
+        // dist: the 16 distances, two 16-bit halves per word.
+        auto emit = [&](const uint32_t* dist, uint32_t pass, uint32_t ret, uint32_t other, int32_t tid) {
+            uint32_t base = (uint32_t)__shfl((int)ret, col);
+            if (lane >= 32) base += other;
+            const size_t row0 = (size_t)q * v.stride + v.K;
+    #pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                if (!__ballot((pass >> r) & 1u)) continue;
+                const int32_t id = __shfl(tid, fp4::acc_row(r, lane));
+                if ((pass >> r) & 1u) {
+                    if (base < (uint32_t)v.newcap)
+                        v.keys[row0 + base] = make_key((dist[r >> 1] >> (16 * (r & 1))) & 0xFFFFu, (uint32_t)id);
+                    ++base;
+                }
+            }
+        };
+        bool dval = false;  // a tile's appends are pending
+        uint32_t ddist[8] = {};
+        uint32_t dpass = 0, dret = 0, dother = 0;
+        int32_t did = 0;
         int since_flush = 0;
         for (uint32_t i0 = 0; i0 < nt; i0 += 64) {
             const uint32_t n64 = min(64u, nt - i0);
@@ -717,19 +743,20 @@ __global__ __launch_bounds__(256) void cell_scan_kernel(const uint32_t* __restri
             }
             if (!needm) continue;
             uint4 nx[NV];
+            int32_t nid = 0;
             int nt_next = __builtin_ctzll(needm);
-            load(t0 + i0 + nt_next, nx);
+            load(t0 + i0 + nt_next, nx, nid);
             while (true) {
                 const int tt = nt_next;
                 uint4 cur[NV];
     #pragma unroll
                 for (int i = 0; i < NV; ++i) cur[i] = nx[i];
+                const int32_t cid = nid;
                 needm &= needm - 1ull;
                 if (needm) {
                     nt_next = __builtin_ctzll(needm);
-                    load(t0 + i0 + nt_next, nx);
+                    load(t0 + i0 + nt_next, nx, nid);
                 }
-                const uint32_t t = t0 + i0 + (uint32_t)tt;
                 const uint32_t nv = (uint32_t)__shfl((int)myinfo, tt) >> 22;
                 const uint32_t* cw = reinterpret_cast<const uint32_t*>(cur);
                 v16f acc = {};
@@ -753,26 +780,23 @@ __global__ __launch_bounds__(256) void cell_scan_kernel(const uint32_t* __restri
                 } else {
                     const uint32_t mine = (uint32_t)__popc(pass);
                     const uint32_t other = (uint32_t)__shfl_xor((int)mine, 32);
-                    uint32_t base = 0;
-                    if (lane < 32 && mine + other > 0) base = atomicAdd(&v.cnt[q], mine + other);
-                    base = (uint32_t)__shfl((int)base, col);
-                    if (lane >= 32) base += other;
-                    if (pass) {
-                        const size_t row0 = (size_t)q * v.stride + v.K;
+                    uint32_t ret = 0;
+                    if (lane < 32 && mine + other > 0) ret = atomicAdd(&v.cnt[q], mine + other);
+                    if (dval) emit(ddist, dpass, dret, dother, did);
+                    dval = true;
     #pragma unroll
-                        for (int r = 0; r < 16; ++r) {
-                            if (!((pass >> r) & 1u)) continue;
-                            if (base < (uint32_t)v.newcap) {
-                                const int32_t id = ids[(size_t)t * 32 + fp4::acc_row(r, lane)];
-                                v.keys[row0 + base] = make_key((uint32_t)((BITS - (int)acc[r]) >> 1), (uint32_t)id);
-                            }
-                            ++base;
-                        }
-                    }
+                    for (int r = 0; r < 16; r += 2)
+                        ddist[r >> 1] = (uint32_t)((BITS - (int)acc[r]) >> 1) |
+                                        ((uint32_t)((BITS - (int)acc[r + 1]) >> 1) << 16);
+                    dpass = pass;
+                    dret = ret;
+                    dother = other;
+                    did = cid;
                 }
                 if (!needm) break;
             }
         }
+        if (!kHist && dval) emit(ddist, dpass, dret, dother, did);
         if (kHist && since_flush) flush();
     }
 }
