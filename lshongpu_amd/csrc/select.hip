@@ -10,6 +10,7 @@
 // with exact (distance, ID) keys instead of its EPSILON comparator
 // (pointExtensions.cuh:8-33).
 #include "device_common.hpp"
+#include "select_int.hpp"
 
 namespace lshgpu {
 
@@ -59,63 +60,12 @@ __global__ __launch_bounds__(kSelThreads) void select_merge_kernel(TopKView v) {
 // bin (8-bit digits of the ID, or of the whole key in the last bin, which
 // also holds every larger distance) finds the K-th key itself, and every key
 // at or below it is kept.  thr becomes that key.
-constexpr int kSelBins = 2048;
-
-// The rank-th smallest (1-based) key among the non-empty keys of row[0, n)
-// whose bin is T and whose bits above `shift` + 8 match prefix; shifts run
-// from `top` down to 0.  Uses hist[0, 256).
-__device__ uint64_t radix_kth_in_bin(const uint64_t* row, int n, uint32_t T, uint32_t rank, int top, uint64_t prefix,
-                                     uint32_t* hist, uint32_t* s_dig, uint32_t* s_rank) {
-    uint64_t mask = top == 24 ? 0xFFFFFFFF00000000ull : 0ull;
-    for (int shift = top; shift >= 0; shift -= 8) {
-        for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0;
-        __syncthreads();
-        for (int i = threadIdx.x; i < n; i += blockDim.x) {
-            const uint64_t key = row[i];
-            if (key == kEmptyKey || min((uint32_t)(key >> 32), (uint32_t)(kSelBins - 1)) != T) continue;
-            if ((key & mask) == prefix) atomicAdd(&hist[(uint32_t)(key >> shift) & 255u], 1u);
-        }
-        __syncthreads();
-        if (threadIdx.x < 64) {
-            const int lane = threadIdx.x;
-            uint32_t c[4], sum = 0;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                c[e] = hist[4 * lane + e];
-                sum += c[e];
-            }
-            uint32_t inc = sum;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t x = __shfl_up(inc, o);
-                if (lane >= o) inc += x;
-            }
-            uint32_t run = inc - sum;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                if (run < rank && run + c[e] >= rank) {
-                    *s_dig = 4u * lane + e;
-                    *s_rank = rank - run;
-                }
-                run += c[e];
-            }
-        }
-        __syncthreads();
-        prefix |= (uint64_t)*s_dig << shift;
-        mask |= 0xFFull << shift;
-        rank = *s_rank;
-        __syncthreads();
-    }
-    return prefix;
-}
-
-constexpr int kTieCap = 512;  // keys of the boundary bin gathered in LDS (more: selected in place)
-
 __global__ __launch_bounds__(kSelThreads) void select_hist_kernel(TopKView v) {
     extern __shared__ uint64_t kept[];  // K
     __shared__ uint32_t hist[kSelBins];
     __shared__ uint64_t ties[kTieCap];
-    __shared__ uint32_t s_T, s_below, s_nkeep, s_dig, s_rank, s_nties;
+    __shared__ uint32_t sv[kSelScratch];
+    __shared__ uint32_t s_nkeep;
     const int q = blockIdx.x;
     const uint32_t n_raw = v.cnt[q];
     if (n_raw == 0 && !v.fresh) return;
@@ -126,71 +76,8 @@ __global__ __launch_bounds__(kSelThreads) void select_hist_kernel(TopKView v) {
     // K written below are disjoint from them).
     const uint64_t* in = v.fresh ? row + K : row;
     const int total = v.fresh ? n : K + n;
-    for (int i = threadIdx.x; i < kSelBins; i += blockDim.x) hist[i] = 0;
-    if (threadIdx.x == 0) {
-        s_nkeep = 0;
-        s_nties = 0;
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < total; i += blockDim.x) {
-        const uint64_t key = in[i];
-        if (key != kEmptyKey) atomicAdd(&hist[min((uint32_t)(key >> 32), (uint32_t)(kSelBins - 1))], 1u);
-    }
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        // Wave 0 finds the first bin at which the running count reaches K.
-        const int lane = threadIdx.x;
-        constexpr int per = kSelBins / 64;
-        uint32_t local = 0;
-        for (int b = 0; b < per; ++b) local += hist[lane * per + b];
-        uint32_t incl = local;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t t = __shfl_up(incl, o);
-            if (lane >= o) incl += t;
-        }
-        const uint32_t excl = incl - local;
-        const uint32_t all = __shfl(incl, 63);
-        const unsigned long long has = __ballot(incl >= (uint32_t)K && excl < (uint32_t)K);
-        if (all < (uint32_t)K) {
-            if (lane == 0) {
-                s_T = kSelBins;  // fewer than K keys in total: keep them all
-                s_below = all;
-            }
-        } else {
-            const int owner = __ffsll((long long)has) - 1;
-            if (lane == owner) {
-                uint32_t c = excl;
-                int b = lane * per;
-                while (c + hist[b] < (uint32_t)K) c += hist[b++];
-                s_T = (uint32_t)b;
-                s_below = c;
-            }
-        }
-    }
-    __syncthreads();
-    const uint32_t T = s_T;
-    uint64_t kth = kEmptyKey;  // the K-th smallest key (none: keep every key)
-    if (T < (uint32_t)kSelBins) {
-        const bool last = T == (uint32_t)(kSelBins - 1);
-        const uint32_t nt = hist[T];  // keys in the boundary bin
-        __syncthreads();
-        const uint64_t* src = in;
-        int nsrc = total;
-        if (nt <= (uint32_t)kTieCap) {
-            // Gather them once; the radix passes then run over LDS.
-            for (int i = threadIdx.x; i < total; i += blockDim.x) {
-                const uint64_t key = in[i];
-                if (key != kEmptyKey && min((uint32_t)(key >> 32), (uint32_t)(kSelBins - 1)) == T)
-                    ties[atomicAdd(&s_nties, 1u)] = key;
-            }
-            __syncthreads();
-            src = ties;
-            nsrc = (int)nt;
-        }
-        kth = radix_kth_in_bin(src, nsrc, T, (uint32_t)K - s_below, last ? 56 : 24, last ? 0ull : (uint64_t)T << 32,
-                               hist, &s_dig, &s_rank);
-    }
+    if (threadIdx.x == 0) s_nkeep = 0;
+    const uint64_t kth = int_kth_key(in, total, (uint32_t)K, hist, ties, sv);  // the K-th key (none: keep all)
     for (int i = threadIdx.x; i < total; i += blockDim.x) {
         const uint64_t key = in[i];
         if (key != kEmptyKey && key <= kth) {

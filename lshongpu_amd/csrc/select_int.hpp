@@ -1,0 +1,157 @@
+// select_int.hpp -- the K-th smallest of a set of integer-distance keys, by
+// one workgroup (header-only device code of the integer-key top-K select).
+//
+// A histogram of the distances (the key's high word; the last bin also holds
+// every larger distance) in LDS gives the bin T that holds the K-th key.  The
+// keys of that bin, when at most kTieCap, are gathered in LDS and the K-th is
+// the one whose rank among them (keys are distinct) is right; a larger bin
+// is resolved by a radix select over its keys in place (8-bit digits of the
+// ID, or of the whole key in the last bin).
+#pragma once
+
+#include "lshgpu_internal.hpp"
+
+namespace lshgpu {
+
+constexpr int kSelBins = 2048;
+constexpr int kTieCap = 512;      // keys of the boundary bin gathered in LDS (more: selected in place)
+constexpr int kSelScratch = 8;    // LDS words int_kth_key uses for its shared scalars
+
+// The rank-th smallest (1-based) key among the non-empty keys of row[0, n)
+// whose bin is T and whose bits above `shift` + 8 match prefix; shifts run
+// from `top` down to 0.  Uses hist[0, 256).
+__device__ inline uint64_t radix_kth_in_bin(const uint64_t* row, int n, uint32_t T, uint32_t rank, int top,
+                                            uint64_t prefix, uint32_t* hist, uint32_t* s_dig, uint32_t* s_rank) {
+    uint64_t mask = top == 24 ? 0xFFFFFFFF00000000ull : 0ull;
+    for (int shift = top; shift >= 0; shift -= 8) {
+        for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0;
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            const uint64_t key = row[i];
+            if (key == kEmptyKey || min((uint32_t)(key >> 32), (uint32_t)(kSelBins - 1)) != T) continue;
+            if ((key & mask) == prefix) atomicAdd(&hist[(uint32_t)(key >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            const int lane = threadIdx.x;
+            uint32_t c[4], sum = 0;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                c[e] = hist[4 * lane + e];
+                sum += c[e];
+            }
+            uint32_t inc = sum;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t x = __shfl_up(inc, o);
+                if (lane >= o) inc += x;
+            }
+            uint32_t run = inc - sum;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                if (run < rank && run + c[e] >= rank) {
+                    *s_dig = 4u * lane + e;
+                    *s_rank = rank - run;
+                }
+                run += c[e];
+            }
+        }
+        __syncthreads();
+        prefix |= (uint64_t)*s_dig << shift;
+        mask |= 0xFFull << shift;
+        rank = *s_rank;
+        __syncthreads();
+    }
+    return prefix;
+}
+
+// The K-th smallest non-empty key of in[0, total) (K >= 1), or kEmptyKey when
+// there are fewer than K.  LDS: hist[kSelBins], ties[kTieCap],
+// sv[kSelScratch].  Every thread of the workgroup must call it; it ends with
+// a barrier, so the caller may reuse that LDS at once.
+__device__ inline uint64_t int_kth_key(const uint64_t* in, int total, uint32_t K, uint32_t* hist, uint64_t* ties,
+                                       uint32_t* sv) {
+    uint32_t& s_T = sv[0];
+    uint32_t& s_below = sv[1];
+    uint32_t& s_nties = sv[4];
+    for (int i = threadIdx.x; i < kSelBins; i += blockDim.x) hist[i] = 0;
+    if (threadIdx.x == 0) s_nties = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < total; i += blockDim.x) {
+        const uint64_t key = in[i];
+        if (key != kEmptyKey) atomicAdd(&hist[min((uint32_t)(key >> 32), (uint32_t)(kSelBins - 1))], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        // Wave 0 finds the first bin at which the running count reaches K.
+        const int lane = threadIdx.x;
+        constexpr int per = kSelBins / 64;
+        uint32_t local = 0;
+        for (int b = 0; b < per; ++b) local += hist[lane * per + b];
+        uint32_t incl = local;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(incl, o);
+            if (lane >= o) incl += t;
+        }
+        const uint32_t excl = incl - local;
+        const uint32_t all = __shfl(incl, 63);
+        const unsigned long long has = __ballot(incl >= K && excl < K);
+        if (all < K) {
+            if (lane == 0) {
+                s_T = kSelBins;  // fewer than K keys in total: keep them all
+                s_below = all;
+            }
+        } else {
+            const int owner = __ffsll((long long)has) - 1;
+            if (lane == owner) {
+                uint32_t c = excl;
+                int b = lane * per;
+                while (c + hist[b] < K) c += hist[b++];
+                s_T = (uint32_t)b;
+                s_below = c;
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t T = s_T;
+    uint64_t kth = kEmptyKey;
+    if (T < (uint32_t)kSelBins) {
+        const bool last = T == (uint32_t)(kSelBins - 1);
+        const uint32_t nt = hist[T];  // keys in the boundary bin
+        const uint32_t below = s_below;
+        __syncthreads();
+        if (nt <= (uint32_t)kTieCap) {
+            // Gather them; the K-th is the key with at most want = K - below
+            // - 1 smaller keys and more than want smaller or equal (0 <=
+            // want < nt, so exactly one key value qualifies).
+            for (int i = threadIdx.x; i < total; i += blockDim.x) {
+                const uint64_t key = in[i];
+                if (key != kEmptyKey && min((uint32_t)(key >> 32), (uint32_t)(kSelBins - 1)) == T)
+                    ties[atomicAdd(&s_nties, 1u)] = key;
+            }
+            __syncthreads();
+            uint64_t* s_kth = reinterpret_cast<uint64_t*>(hist);  // hist is no longer needed
+            const uint32_t want = K - below - 1u;
+            for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) {
+                const uint64_t key = ties[t];
+                uint32_t lt = 0, le = 0;
+                for (uint32_t u = 0; u < nt; ++u) {
+                    const uint64_t x = ties[u];
+                    lt += x < key ? 1u : 0u;
+                    le += x <= key ? 1u : 0u;
+                }
+                if (lt <= want && want < le) *s_kth = key;
+            }
+            __syncthreads();
+            kth = *s_kth;
+        } else {
+            kth = radix_kth_in_bin(in, total, T, K - below, last ? 56 : 24, last ? 0ull : (uint64_t)T << 32, hist,
+                                   &sv[2], &sv[3]);
+        }
+    }
+    __syncthreads();
+    return kth;
+}
+
+}  // namespace lshgpu
