@@ -339,24 +339,20 @@ __device__ inline int block_kth_bin(const uint32_t* hist, int nbins, uint32_t K,
 //    of cell sizes over the 256 distances above the nearest cell's (all
 //    cells when that window holds fewer than K points); the cells within R
 //    are counted into cnt1 for the first pass;
-//  - the K-th smallest upper bound D + hi_t (weighted by the tile's rows)
-//    over the tiles of the cells within R + kRadiusMargin: thr = (bound + 1,
-//    0);
+//  - an upper bound on the K-th distance: the K-th smallest D + max (each
+//    cell's points lie within its max of the pivot, so within D + max of
+//    the query), weighted by cell size, over the cells within R +
+//    kRadiusMargin: thr = (bound + 1, 0);
 //  - per query, the lists of its first-pass cells (qcells1, n1) and of the
 //    cells whose bound is below that threshold (qcells2, n2; rows of P).
 // Thread i of the NT-thread workgroup owns cells i, i + NT, ... (CPT of
 // them, P <= NT CPT): their distance and packed record are loaded once, in
-// one batch, and every later step reads them from registers.  The tiles of
-// the cells within R + kRadiusMargin (up to NT cells) are spread over all
-// threads, so that one large cell does not hold up the workgroup; cells past
-// the first NT are walked by their owners.
+// one batch, and every step reads them from registers.
 constexpr int kRadiusMargin = 16;
 constexpr int kWindow = 256;
 template <int CPT, int NT>
 __global__ __launch_bounds__(NT) void cell_threshold_kernel(const uint16_t* __restrict__ D, int Q, int P, int Pp,
-                                                            const uint2* __restrict__ crec,
-                                                            const uint32_t* __restrict__ ctile,
-                                                            const uint32_t* __restrict__ tinfo, int bits, int K,
+                                                            const uint2* __restrict__ crec, int bits, int K,
                                                             uint64_t* __restrict__ thr, uint16_t* __restrict__ R,
                                                             uint32_t* __restrict__ nearest,
                                                             uint32_t* __restrict__ cnt1,
@@ -365,11 +361,9 @@ __global__ __launch_bounds__(NT) void cell_threshold_kernel(const uint16_t* __re
                                                             uint32_t* __restrict__ n2) {
     __shared__ uint32_t h1[kWindow];
     __shared__ uint32_t h2[2049];
-    __shared__ uint32_t l_t0[NT], l_pre[NT + 1];  // listed cells: first tile, prefix of tile counts
-    __shared__ uint16_t l_dq[NT];                 // listed cells: distance to the query
     __shared__ uint32_t ws[NT / 64];
     __shared__ int s_bin;
-    __shared__ uint32_t best, s_n1, s_n2, s_nl;
+    __shared__ uint32_t best, s_n1, s_n2;
     const int q = blockIdx.x;
     const uint16_t* __restrict__ drow = D + (size_t)q * Pp;
     uint32_t dq[CPT], sz[CPT], rg[CPT];
@@ -392,7 +386,6 @@ __global__ __launch_bounds__(NT) void cell_threshold_kernel(const uint16_t* __re
         best = 0xFFFFFFFFu;
         s_n1 = 0;
         s_n2 = 0;
-        s_nl = 0;
     }
     __syncthreads();
     uint32_t mine = 0xFFFFFFFFu;  // (distance, pivot) of the nearest non-empty cell
@@ -425,58 +418,7 @@ __global__ __launch_bounds__(NT) void cell_threshold_kernel(const uint16_t* __re
             atomicAdd(&cnt1[j], 1u);
             q1[s1] = j;
         }
-        const uint32_t sl = wave_slot(pass, &s_nl);
-        if (pass) {
-            const uint32_t t0 = ctile[j], t1 = ctile[j + 1];
-            if (sl < (uint32_t)NT) {
-                l_t0[sl] = t0;
-                l_pre[sl] = t1 - t0;
-                l_dq[sl] = (uint16_t)dq[i];
-            } else {
-                for (uint32_t t = t0; t < t1; t += 8) {
-                    uint32_t info[8];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) info[u] = tinfo[min(t + u, t1 - 1u)];  // t < t1: in bounds
-#pragma unroll
-                    for (int u = 0; u < 8; ++u)
-                        if (t + u < t1) atomicAdd(&h2[dq[i] + ((info[u] >> kKeyShift) & kDistMask)], info[u] >> 22);
-                }
-            }
-        }
-    }
-    __syncthreads();
-    // The listed cells' tiles, flattened: tile f of the list belongs to the
-    // cell c with l_pre[c] <= f < l_pre[c + 1].
-    const int nl = (int)min(s_nl, (uint32_t)NT);
-    {
-        const uint32_t cntv = (int)threadIdx.x < nl ? l_pre[threadIdx.x] : 0u;
-        uint32_t total;
-        const uint32_t pre = block_exclusive_scan<NT>(cntv, ws, total);
-        if ((int)threadIdx.x < nl) l_pre[threadIdx.x] = pre;
-        if (threadIdx.x == 0) l_pre[nl] = total;
-        __syncthreads();
-        for (uint32_t f0 = 0; f0 < total; f0 += 4u * NT) {
-            uint32_t info[4], dv[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const uint32_t f = f0 + (uint32_t)u * NT + threadIdx.x;
-                int lo = 0, hi = nl;
-                while (hi - lo > 1) {
-                    const int mid = (lo + hi) >> 1;
-                    if (l_pre[mid] <= f) lo = mid;
-                    else hi = mid;
-                }
-                const bool ok = f < total;
-                // In bounds either way: a listed cell's first tile plus an
-                // offset below its count, or tile 0.
-                const uint32_t t = ok ? l_t0[lo] + (f - l_pre[lo]) : 0u;
-                info[u] = tinfo[t];
-                dv[u] = ok ? l_dq[lo] : 0xFFFFFFFFu;
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (dv[u] != 0xFFFFFFFFu) atomicAdd(&h2[dv[u] + ((info[u] >> kKeyShift) & kDistMask)], info[u] >> 22);
-        }
+        if (pass) atomicAdd(&h2[dq[i] + (rg[i] >> 16)], sz[i]);
     }
     __syncthreads();
     const int ub = block_kth_bin<NT>(h2, 2 * bits + 1, (uint32_t)K, ws, &s_bin);
@@ -1098,8 +1040,8 @@ const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K
     // Threshold bound from the tile ranges, the first-pass radii and their
     // cell counts, and the nearest cell of each query.
     auto threshold = [&](auto kernel, int nt) {
-        hipLaunchKernelGGL(kernel, dim3((unsigned)Q), dim3(nt), 0, s, D, Q, P, Pp, c.crec.get(), c.ctile.get(),
-                           c.tinfo.get(), bits, K, v.thr, R, nearest, cnt1, qcells1, n1, qcells2, n2);
+        hipLaunchKernelGGL(kernel, dim3((unsigned)Q), dim3(nt), 0, s, D, Q, P, Pp, c.crec.get(), bits, K, v.thr, R,
+                           nearest, cnt1, qcells1, n1, qcells2, n2);
     };
     LSH_REQUIRE(P >= 1 && P <= 256 * 32, LSH_ERR_INTERNAL, "cell_search: pivot count outside the threshold kernel's range");
     if (P <= 256 * 4) threshold(cell_threshold_kernel<4, 256>, 256);
