@@ -306,20 +306,25 @@ int lsh_index_create(const lsh_index_config_t* cfg, const float* data,
 
 /* Search N_query queries for their k nearest (k rounded up to a multiple of
  * 32, as lsh_calculate_k).  results is [N_query][lsh_calculate_k(k)].
- * stream: a hipStream_t (NULL = the index's own stream); the call returns
+ * stream: a hipStream_t (NULL = the index's own stream).  With device results
+ * on a given stream the results are complete in that stream's order (the
+ * call may return while the last kernels run); otherwise the call returns
  * when the results are complete. */
 int lsh_index_search(lsh_index_t* index, const float* queries, int N_query,
                      int k, int queries_on_device, lsh_point_t* results,
                      int results_on_device, void* stream);
 
-/* Per-phase device times (ms) of the last search: [0] query preparation and
+/* Per-phase device times (ms), summed over the searches since profiling was
+ * turned on or since the previous lsh_index_phase_times call (which waits
+ * for those searches and starts a new sum): [0] query preparation and
  * sketch, [1] candidate scans (all rounds), [2] candidate selection, [3]
  * rerank / finalisation, [4] total, [5] the matrix-core Hamming scan
  * launches among [1]; [6] the number of those launches and [7] the (query,
- * point) pairs they multiplied per query (summed over launches).  Filled only while profiling
- * is on. */
+ * point) pairs they multiplied per query (summed over launches).  Recorded
+ * only while profiling is on: on = 1 times every phase, on = 2 only [3] (two
+ * events per search), 0 turns it off. */
 int lsh_index_set_profiling(lsh_index_t* index, int on);
-int lsh_index_phase_times(const lsh_index_t* index, double* out8);
+int lsh_index_phase_times(lsh_index_t* index, double* out8);
 
 /* Bucket occupancy of table `table` (0 <= table < tables) of an index built
  * with framework 1: the figures that guide the choice of bucketKeyBits and

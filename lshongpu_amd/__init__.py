@@ -470,10 +470,15 @@ class Index:
         _check(lib.lsh_index_search(self._h, qp, nq, int(k), int(q_dev), rp, int(r_dev), sp))
         return res
 
-    def set_profiling(self, on: bool = True) -> None:
-        _check(load().lsh_index_set_profiling(self._h, int(bool(on))))
+    def set_profiling(self, on=True) -> None:
+        """True / 1: time every phase; 2: only the rerank / finalisation phase;
+        False / 0: off."""
+        _check(load().lsh_index_set_profiling(self._h, int(on)))
 
     def phase_times(self) -> dict:
+        """Device time per phase (ms) and matrix-core scan counters, summed over
+        the searches since profiling was turned on or since the last call
+        (which waits for those searches)."""
         arr = (C.c_double * 8)()
         _check(load().lsh_index_phase_times(self._h, arr))
         v = list(arr)

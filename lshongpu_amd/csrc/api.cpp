@@ -339,14 +339,22 @@ int lsh_index_search(lsh_index_t* index, const float* queries, int N_query, int 
 int lsh_index_set_profiling(lsh_index_t* index, int on) {
     return guarded([&] {
         LSH_REQUIRE(index, LSH_ERR_INVALID, "index is null");
-        index->profiling = on != 0;
+        LSH_REQUIRE(on >= 0 && on <= 2, LSH_ERR_INVALID, "profiling mode must be 0, 1 or 2");
+        // Mode 2 times only the rerank / finalisation phase (index 3).
+        index->timer.enable(on == 0 ? 0u : on == 1 ? ~0u : (1u << 3));
+        index->prof_mfma_launches = 0;
+        index->prof_mfma_points = 0;
     });
 }
 
-int lsh_index_phase_times(const lsh_index_t* index, double* out8) {
+int lsh_index_phase_times(lsh_index_t* index, double* out8) {
     return guarded([&] {
         LSH_REQUIRE(index && out8, LSH_ERR_INVALID, "null pointer");
-        for (int i = 0; i < 8; ++i) out8[i] = index->phase_ms[i];
+        index->timer.collect(out8, 6);
+        out8[6] = (double)index->prof_mfma_launches;
+        out8[7] = (double)index->prof_mfma_points;
+        index->prof_mfma_launches = 0;
+        index->prof_mfma_points = 0;
     });
 }
 

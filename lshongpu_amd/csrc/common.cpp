@@ -61,22 +61,43 @@ PhaseTimer::~PhaseTimer() {
         (void)hipEventDestroy(r.b);
     }
     for (auto& o : open_) (void)hipEventDestroy(o.second);
+    for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+}
+
+hipEvent_t PhaseTimer::take() {
+    if (!pool_.empty()) {
+        const hipEvent_t e = pool_.back();
+        pool_.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    LSH_HIP(hipEventCreate(&e));
+    return e;
+}
+
+void PhaseTimer::enable(uint32_t mask) {
+    for (auto& r : recs_) {
+        pool_.push_back(r.a);
+        pool_.push_back(r.b);
+    }
+    recs_.clear();
+    for (auto& o : open_) pool_.push_back(o.second);
+    open_.clear();
+    mask_ = mask;
 }
 
 void PhaseTimer::begin(int phase, hipStream_t s) {
-    if (!on_) return;
-    hipEvent_t e;
-    LSH_HIP(hipEventCreate(&e));
+    if (phase < 0 || phase >= 32 || !((mask_ >> phase) & 1u)) return;
+    const hipEvent_t e = take();
     LSH_HIP(hipEventRecord(e, s));
     open_.push_back({phase, e});
 }
 
 void PhaseTimer::end(int phase, hipStream_t s) {
-    if (!on_) return;
+    if (phase < 0 || phase >= 32 || !((mask_ >> phase) & 1u)) return;
     for (size_t i = open_.size(); i-- > 0;) {
         if (open_[i].first == phase) {
-            hipEvent_t e;
-            LSH_HIP(hipEventCreate(&e));
+            const hipEvent_t e = take();
             LSH_HIP(hipEventRecord(e, s));
             recs_.push_back({phase, open_[i].second, e});
             open_.erase(open_.begin() + (long)i);
@@ -87,13 +108,15 @@ void PhaseTimer::end(int phase, hipStream_t s) {
 
 void PhaseTimer::collect(double* ms, int nphase) {
     for (int p = 0; p < nphase; ++p) ms[p] = 0.0;
-    if (!on_) return;
     for (auto& r : recs_) {
         LSH_HIP(hipEventSynchronize(r.b));
         float t = 0.0f;
         LSH_HIP(hipEventElapsedTime(&t, r.a, r.b));
         if (r.phase >= 0 && r.phase < nphase) ms[r.phase] += t;
+        pool_.push_back(r.a);
+        pool_.push_back(r.b);
     }
+    recs_.clear();
 }
 
 }  // namespace lshgpu

@@ -703,7 +703,7 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
     LSH_REQUIRE(k >= 1, LSH_ERR_INVALID, "k must be >= 1");
     const int kq = calculate_k(k);
     hipStream_t s = user_stream ? user_stream : stream;
-    PhaseTimer pt(profiling);
+    PhaseTimer& pt = timer;
     scan_launches = 0;
     mfma_launches = 0;
     mfma_points = 0;
@@ -817,10 +817,13 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
     LSH_HIP(hipMemcpyAsync(results, out.get(), sizeof(lsh_point_t) * (size_t)Q * kq,
                            r_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
     pt.end(PH_TOTAL, s);
-    LSH_HIP(hipStreamSynchronize(s));
-    pt.collect(phase_ms, 6);
-    phase_ms[6] = (double)mfma_launches;
-    phase_ms[7] = (double)mfma_points;
+    // Device results on the caller's stream are ordered on it; otherwise the
+    // call returns with the results complete.
+    if (!r_on_device || !user_stream) LSH_HIP(hipStreamSynchronize(s));
+    if (pt.on()) {
+        prof_mfma_launches += mfma_launches;
+        prof_mfma_points += mfma_points;
+    }
 }
 
 // lshFramework.cuh:142-280 scan + removeDuplicates over the unique candidate

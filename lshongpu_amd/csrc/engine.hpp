@@ -163,8 +163,11 @@ struct lsh_index {
     int device = 0;
 
     hipStream_t stream = nullptr;
-    bool profiling = false;
-    double phase_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // Phase timing while profiling is on: intervals and counters summed over
+    // the searches since the last read.
+    lshgpu::PhaseTimer timer{false};
+    int64_t prof_mfma_launches = 0;
+    int64_t prof_mfma_points = 0;
 
     // Working copy of the data after the path's preprocessing.
     lshgpu::DevBuf<float> data;

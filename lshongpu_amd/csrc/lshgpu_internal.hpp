@@ -380,21 +380,30 @@ std::vector<uint8_t> bit_stream(int64_t n);
 // Phase timing with HIP events on one stream.
 class PhaseTimer {
    public:
-    explicit PhaseTimer(bool on) : on_(on) {}
+    explicit PhaseTimer(bool on) : mask_(on ? ~0u : 0u) {}
     ~PhaseTimer();
+    PhaseTimer(const PhaseTimer&) = delete;
+    PhaseTimer& operator=(const PhaseTimer&) = delete;
+    bool on() const { return mask_ != 0; }
+    // Records the phases whose bit is set in mask (0: none) and drops what
+    // was recorded.
+    void enable(uint32_t mask);
     void begin(int phase, hipStream_t s);
     void end(int phase, hipStream_t s);
-    // Sum of the recorded intervals per phase (ms); synchronises the events.
+    // Sum per phase (ms) of the intervals recorded since the last collect;
+    // synchronises their events, then recycles them.
     void collect(double* ms, int nphase);
 
    private:
-    bool on_;
+    hipEvent_t take();
+    uint32_t mask_;
     struct Rec {
         int phase;
         hipEvent_t a, b;
     };
     std::vector<Rec> recs_;
     std::vector<std::pair<int, hipEvent_t>> open_;
+    std::vector<hipEvent_t> pool_;
 };
 
 }  // namespace lshgpu
