@@ -315,16 +315,20 @@ int lsh_index_search(lsh_index_t* index, const float* queries, int N_query,
                      int results_on_device, void* stream);
 
 /* Per-phase device times (ms), summed over the searches since profiling was
- * turned on or since the previous lsh_index_phase_times call (which waits
+ * turned on or since the previous lsh_index_phase_times(_n) call (which waits
  * for those searches and starts a new sum): [0] query preparation and
  * sketch, [1] candidate scans (all rounds), [2] candidate selection, [3]
- * rerank / finalisation, [4] total, [5] the matrix-core Hamming scan
- * launches among [1]; [6] the number of those launches and [7] the (query,
- * point) pairs they multiplied per query (summed over launches).  Recorded
- * only while profiling is on: on = 1 times every phase, on = 2 only [3] (two
- * events per search), 0 turns it off. */
+ * rerank / finalisation (the rerank's filter kernel alone when the FP64
+ * recheck runs as its own kernel), [4] total, [5] the matrix-core Hamming
+ * scan launches among [1]; [6] the number of those launches and [7] the
+ * (query, point) pairs they multiplied per query (summed over launches); [8]
+ * the rerank's separate FP64 recheck.  Recorded only while profiling is on:
+ * on = 1 times every phase, on = 2 only [3] (two events per search), 0 turns
+ * it off.  lsh_index_phase_times fills [0, 8), lsh_index_phase_times_n the
+ * first n (entries past [8] read 0). */
 int lsh_index_set_profiling(lsh_index_t* index, int on);
 int lsh_index_phase_times(lsh_index_t* index, double* out8);
+int lsh_index_phase_times_n(lsh_index_t* index, double* out, int n);
 
 /* Bucket occupancy of table `table` (0 <= table < tables) of an index built
  * with framework 1: the figures that guide the choice of bucketKeyBits and

@@ -145,6 +145,7 @@ _SIGS = {
                                    C.c_void_p]),
     "lsh_index_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "lsh_index_phase_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "lsh_index_phase_times_n": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_int]),
     "lsh_index_bucket_stats": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(_BucketStats), C.POINTER(C.c_int64),
                                          C.c_int]),
     "lsh_index_device_bytes": (C.c_int64, [C.c_void_p]),
@@ -479,11 +480,11 @@ class Index:
         """Device time per phase (ms) and matrix-core scan counters, summed over
         the searches since profiling was turned on or since the last call
         (which waits for those searches)."""
-        arr = (C.c_double * 8)()
-        _check(load().lsh_index_phase_times(self._h, arr))
+        arr = (C.c_double * 9)()
+        _check(load().lsh_index_phase_times_n(self._h, arr, 9))
         v = list(arr)
         return {"prep_ms": v[0], "scan_ms": v[1], "select_ms": v[2], "final_ms": v[3], "total_ms": v[4],
-                "mfma_scan_ms": v[5], "mfma_launches": int(v[6]), "mfma_points": int(v[7])}
+                "mfma_scan_ms": v[5], "mfma_launches": int(v[6]), "mfma_points": int(v[7]), "recheck_ms": v[8]}
 
     def bucket_stats(self, table: int, hist_len: int = 0):
         """Occupancy of one (K,L) table: ({buckets, nonempty, points, max_size},

@@ -347,16 +347,20 @@ int lsh_index_set_profiling(lsh_index_t* index, int on) {
     });
 }
 
-int lsh_index_phase_times(lsh_index_t* index, double* out8) {
+int lsh_index_phase_times_n(lsh_index_t* index, double* out, int n) {
     return guarded([&] {
-        LSH_REQUIRE(index && out8, LSH_ERR_INVALID, "null pointer");
-        index->timer.collect(out8, 6);
-        out8[6] = (double)index->prof_mfma_launches;
-        out8[7] = (double)index->prof_mfma_points;
+        LSH_REQUIRE(index && out && n >= 0, LSH_ERR_INVALID, "null pointer or negative count");
+        double ms[7];
+        index->timer.collect(ms, 7);
+        const double all[9] = {ms[0], ms[1], ms[2], ms[3], ms[4], ms[5], (double)index->prof_mfma_launches,
+                               (double)index->prof_mfma_points, ms[6]};
+        for (int i = 0; i < n; ++i) out[i] = i < 9 ? all[i] : 0.0;
         index->prof_mfma_launches = 0;
         index->prof_mfma_points = 0;
     });
 }
+
+int lsh_index_phase_times(lsh_index_t* index, double* out8) { return lsh_index_phase_times_n(index, out8, 8); }
 
 int lsh_index_bucket_stats(lsh_index_t* index, int table, lsh_bucket_stats_t* out, int64_t* hist, int hist_len) {
     return guarded([&] {
@@ -382,6 +386,11 @@ int lsh_index_destroy(lsh_index_t* index) {
         (void)hipSetDevice(index->device);
         hipStream_t s = index->stream;
         if (s) (void)hipStreamSynchronize(s);
+        // A search on a caller's stream may still be running.
+        if (index->done_ev) {
+            (void)hipEventSynchronize(index->done_ev);
+            (void)hipEventDestroy(index->done_ev);
+        }
         delete index;
         if (s) (void)hipStreamDestroy(s);
         (void)hipSetDevice(prev);

@@ -19,7 +19,7 @@ using namespace lshgpu;
 
 namespace {
 
-enum Phase { PH_PREP = 0, PH_SCAN = 1, PH_SELECT = 2, PH_FINAL = 3, PH_TOTAL = 4, PH_MFMA = 5 };
+enum Phase { PH_PREP = 0, PH_SCAN = 1, PH_SELECT = 2, PH_FINAL = 3, PH_TOTAL = 4, PH_MFMA = 5, PH_RECHECK = 6 };
 
 struct DeviceGuard {
     int prev = 0;
@@ -497,7 +497,8 @@ size_t lsh_index::device_bytes() const {
                qkeys.bytes() + qlo.bytes() + qpre.bytes() + qmax.bytes() + q.bytes() + qT.bytes() +
                q_inv.bytes() + qjl.bytes() + qjlT.bytes() + qsk.bytes() + out.bytes() + flags.bytes() + st.bytes();
     for (const SampleLevel& L : samples) b += L.ds.bytes() + L.fp4.bytes();
-    b += cells.bytes() + qorder.bytes() + cdata.bytes() + cpos.bytes() + chalf.bytes();
+    b += cells.bytes() + qorder.bytes() + cdata.bytes() + cpos.bytes() + chalf.bytes() + rband.band.bytes() +
+         rband.cnt.bytes();
     for (const Transform& t : key_steps) b += t.mins.bytes();
     for (const Transform& t : sketch_steps) b += t.mins.bytes();
     return b;
@@ -703,6 +704,7 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
     LSH_REQUIRE(k >= 1, LSH_ERR_INVALID, "k must be >= 1");
     const int kq = calculate_k(k);
     hipStream_t s = user_stream ? user_stream : stream;
+    if (done_rec) LSH_HIP(hipStreamWaitEvent(s, done_ev, 0));
     PhaseTimer& pt = timer;
     scan_launches = 0;
     mfma_launches = 0;
@@ -770,14 +772,23 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
         // The rerank of queries `order` (all queries when null; m of them).
         auto rerank = [&](const int32_t* order, int m) {
             pt.begin(PH_FINAL, s);
-            if (cells.built())
-                launch_rerank(cdata.get(), q.get(), d, kind, st.view().keys, st.view().stride, nullptr, Ceff, m, kq,
-                              chalf.get() ? half_error_bound(d) : fp32_error_bound(kind, d), out.get(), s, order,
-                              cpos.get(), chalf.get());
-            else
-                launch_rerank(data.get(), q.get(), d, kind, st.view().keys, st.view().stride, nullptr, Ceff, m, kq,
-                              fp32_error_bound(kind, d), out.get(), s, order);
+            // The rerank's filter (timed as the rerank phase), then its FP64
+            // recheck when it splits.
+            const bool split =
+                cells.built()
+                    ? launch_rerank(cdata.get(), q.get(), d, kind, st.view().keys, st.view().stride, nullptr, Ceff, m,
+                                    kq, chalf.get() ? half_error_bound(d) : fp32_error_bound(kind, d), out.get(), s,
+                                    order, cpos.get(), chalf.get(), &rband, false)
+                    : launch_rerank(data.get(), q.get(), d, kind, st.view().keys, st.view().stride, nullptr, Ceff, m,
+                                    kq, fp32_error_bound(kind, d), out.get(), s, order, nullptr, nullptr, &rband,
+                                    false);
             pt.end(PH_FINAL, s);
+            if (split) {
+                pt.begin(PH_RECHECK, s);
+                launch_rerank_recheck(cells.built() ? cdata.get() : data.get(), q.get(), d, kind, Ceff, m, kq,
+                                      out.get(), s, order, rband);
+                pt.end(PH_RECHECK, s);
+            }
         };
         SketchSet::Hook hook;
         hook.fn = [&] { rerank(ordered ? qorder.get() : nullptr, Q); };
@@ -810,7 +821,7 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
                    pt, scan_launches);
         pt.begin(PH_FINAL, s);
         launch_rerank(data.get(), q.get(), d, kAngular, st.view().keys, st.view().stride, nullptr, Ceff, Q, kq,
-                      fp32_error_bound(kAngular, d), out.get(), s);
+                      fp32_error_bound(kAngular, d), out.get(), s, nullptr, nullptr, nullptr, &rband);
         pt.end(PH_FINAL, s);
     }
 
@@ -819,6 +830,9 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
     pt.end(PH_TOTAL, s);
     // Device results on the caller's stream are ordered on it; otherwise the
     // call returns with the results complete.
+    if (!done_ev) LSH_HIP(hipEventCreateWithFlags(&done_ev, hipEventDisableTiming));
+    LSH_HIP(hipEventRecord(done_ev, s));
+    done_rec = true;
     if (!r_on_device || !user_stream) LSH_HIP(hipStreamSynchronize(s));
     if (pt.on()) {
         prof_mfma_launches += mfma_launches;
@@ -889,7 +903,7 @@ void lsh_index::search_lsh(int Q, int kq, hipStream_t s, PhaseTimer& pt) {
         lsh_topk(a, none, maxS, K, kNormal, int_keys && K <= kMaxIntTopK, st, scratch, 0, s, pt, scan_launches);
         pt.begin(PH_FINAL, s);
         launch_rerank(data.get(), q.get(), d, kind, st.view().keys, st.view().stride, nullptr, Ceff, Q, kq, eps,
-                      out.get(), s);
+                      out.get(), s, nullptr, nullptr, nullptr, &rband);
         pt.end(PH_FINAL, s);
         return;
     }

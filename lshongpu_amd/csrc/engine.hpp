@@ -183,6 +183,11 @@ struct lsh_index {
     lshgpu::DevBuf<float> cdata;     // data rows in cell order (rerank reads)
     lshgpu::DevBuf<int32_t> cpos;    // position of each data row in cdata
     lshgpu::DevBuf<_Float16> chalf;  // FP16 image of the unit rows of cdata (angular rerank filter)
+    lshgpu::RerankBand rband;        // the rerank's band workspace (grow-only)
+    // Recorded at the end of each search: the next search's stream waits on it,
+    // since a search may return while its last kernels still run.
+    hipEvent_t done_ev = nullptr;
+    bool done_rec = false;
     lshgpu::DevBuf<float> planesT;   // SimHash [words][d][32], JL [groups][d][32]
     int sdim = 0;                    // JL sketch length
     lshgpu::DevBuf<float> djl;       // [N][sdim]

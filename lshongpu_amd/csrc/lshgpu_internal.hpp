@@ -256,10 +256,22 @@ void launch_exact_finalize(const float* data, const float* queries, int d, int k
 // qorder (may be null): a permutation of the queries, taken in that order
 // per XCD so that queries with shared candidates run close together.
 // rowmap (may be null): candidate ID i is row rowmap[i] of data and hdata.
-void launch_rerank(const float* data, const float* queries, int d, int kind, const uint64_t* keys,
+// rb (may be null): workspace for the band; with it and k <= 64 the rerank
+// runs as two kernels, the filter (which writes each query's band) and a
+// one-wave-per-query FP64 recheck, the filter then needing less LDS.
+// Returns whether it split; with recheck_too false the caller launches the
+// recheck itself (launch_rerank_recheck, same queries and order).
+struct RerankBand {
+    DevBuf<uint64_t> band;  // [queries][Cp]: (row << 32 | ID) of the band's members
+    DevBuf<uint32_t> cnt;   // [queries]: members
+};
+bool launch_rerank(const float* data, const float* queries, int d, int kind, const uint64_t* keys,
                    int key_stride, const int32_t* cand, int C, int Q, int k, double eps,
                    lsh_point_t* out, hipStream_t s, const int32_t* qorder = nullptr,
-                   const int32_t* rowmap = nullptr, const _Float16* hdata = nullptr);
+                   const int32_t* rowmap = nullptr, const _Float16* hdata = nullptr, RerankBand* rb = nullptr,
+                   bool recheck_too = true);
+void launch_rerank_recheck(const float* data, const float* queries, int d, int kind, int C, int Q, int k,
+                           lsh_point_t* out, hipStream_t s, const int32_t* qorder, RerankBand& rb);
 
 // FP16 image of npos rows for the angular rerank: row i is x_i * 2^14 /
 // ||x_i|| (FP64 norm), rounded to FP16 and zero padded to

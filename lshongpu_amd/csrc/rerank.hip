@@ -15,6 +15,8 @@
 // set.
 #include "device_common.hpp"
 
+#include <type_traits>
+
 namespace lshgpu {
 
 namespace {
@@ -278,7 +280,11 @@ __device__ inline void sums_f16(const _Float16* __restrict__ hdata, int hs, cons
 }
 
 // HALF: the approximate distances come from hdata (angular kinds only).
-template <int KIND, bool HALF>
+// BAND: the kernel stops once the band is known and writes it, (row << 32 |
+// ID) per member, to band[pos][0, m) with bcnt[pos] = m, pos being the
+// query's place in qorder (its index without one); rerank_recheck_kernel
+// finishes the query.
+template <int KIND, bool HALF, bool BAND>
 __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ data,
                                                      const _Float16* __restrict__ hdata, int hs,
                                                      const float* __restrict__ queries, int d,
@@ -287,24 +293,28 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
                                                      int C, int Cp, int k, double eps,
                                                      lsh_point_t* __restrict__ out,
                                                      const int32_t* __restrict__ qorder, int Q, int per,
-                                                     const int32_t* __restrict__ rowmap) {
+                                                     const int32_t* __restrict__ rowmap,
+                                                     uint64_t* __restrict__ band, uint32_t* __restrict__ bcnt) {
     extern __shared__ uint64_t sm[];
-    // LDS, in bytes from sm: k64 [0, 8Cp), ids [8Cp, 12Cp), k32 [12Cp,
-    // 20Cp), rows [20Cp, 24Cp), qs [24Cp, 24Cp + 4hs).  k32 and rows are
+    // LDS, in bytes from sm: k64 [0, 8Cp) (absent with BAND: it aliases
+    // k32), ids [8Cp, 12Cp), k32 [12Cp, 20Cp), rows [20Cp, 24Cp), qs [24Cp,
+    // 24Cp + 4hs), each shifted down by 8Cp with BAND.  k32 and rows are
     // adjacent: once the band is known they serve as one 12Cp-byte staging
-    // buffer for its rows.
+    // buffer for its rows.  The approximate sums (float2) alias k64, or k32
+    // with BAND (each key replaces its own candidate's sums).
     uint64_t* k64 = sm;                                              // Cp
-    int32_t* ids = reinterpret_cast<int32_t*>(sm + Cp);              // Cp
+    int32_t* ids = reinterpret_cast<int32_t*>(sm + (BAND ? 0 : Cp)); // Cp
     uint64_t* k32 = reinterpret_cast<uint64_t*>(ids + Cp);           // Cp
+    if (BAND) k64 = k32;
     int32_t* rows = reinterpret_cast<int32_t*>(k32 + Cp);            // Cp: data row of each candidate
     float* qs = reinterpret_cast<float*>(rows + Cp);                 // hs (HALF only)
     __shared__ uint32_t rs_hist[256];
     __shared__ uint32_t rs_sv[4];
     // With an order, XCD x (workgroups x, x + 8, ...) takes the x-th eighth
     // of it in sequence.
-    int q = blockIdx.x;
+    int pos = blockIdx.x, q = pos;  // pos: place in the list of queries (band row)
     if (qorder) {
-        const int pos = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+        pos = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
         if (pos >= Q) return;
         q = qorder[pos];
     }
@@ -380,6 +390,15 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
     }
     __syncthreads();
     const int m = (int)rs_sv[2];
+    if (BAND) {
+        uint64_t* __restrict__ bq = band + (size_t)pos * Cp;
+        for (int s = threadIdx.x; s < m; s += blockDim.x) {
+            const int t = ids[s];
+            bq[s] = ((uint64_t)(uint32_t)rows[t] << 32) | (uint32_t)k32[t];
+        }
+        if (threadIdx.x == 0) bcnt[pos] = (uint32_t)m;
+        return;
+    }
     // FP64 for the band.  FP64 keys carry no ID: the sort compares IDs on
     // equal keys.  Each slot's (row, ID) is parked in k64 first; k32 and
     // rows are then free and hold chunks of the band's rows, read with
@@ -453,6 +472,80 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
     }
 }
 
+// Smaller of two (key, ID) pairs.
+__device__ inline bool pair_less(uint64_t ka, int32_t ia, uint64_t kb, int32_t ib) {
+    return ka < kb || (ka == kb && ia < ib);
+}
+
+// Ascending bitonic merge of a bitonic sequence held one pair per lane.
+__device__ inline void wave_bitonic_clean(uint64_t& k, int32_t& id) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int stride = 32; stride > 0; stride >>= 1) {
+        const uint64_t ko = shfl_xor_u64(k, stride);
+        const int32_t io = __shfl_xor(id, stride);
+        const bool lower = (lane & stride) == 0;
+        const bool other_less = pair_less(ko, io, k, id);
+        if (lower == other_less) {
+            k = ko;
+            id = io;
+        }
+    }
+}
+
+// The band's FP64 recheck, one wave per query (k <= 64): 64 members at a
+// time, each lane computes one member's FP64 distance (the oracle's
+// operation order), the wave sorts them and merges them into the running 64
+// best held one per lane (min against the reversed chunk, then a bitonic
+// clean); the first k are written.
+template <int KIND>
+__global__ __launch_bounds__(64) void rerank_recheck_kernel(const float* __restrict__ data,
+                                                            const float* __restrict__ queries, int d,
+                                                            const uint64_t* __restrict__ band,
+                                                            const uint32_t* __restrict__ bcnt, int Cp, int k,
+                                                            lsh_point_t* __restrict__ out,
+                                                            const int32_t* __restrict__ qlist) {
+    const int lane = threadIdx.x;
+    const int pos = blockIdx.x;
+    const int q = qlist ? qlist[pos] : pos;
+    const float* __restrict__ qv = queries + (size_t)q * d;
+    const double nq64 = (KIND == kNegCos || KIND == kAngular) ? norm2_64(qv, d) : 0.0;
+    const int m = (int)bcnt[pos];
+    const uint64_t* __restrict__ bq = band + (size_t)pos * Cp;
+    uint64_t bk = kEmptyKey;  // running best, ascending across lanes
+    int32_t bi = 0x7FFFFFFF;
+    for (int c0 = 0; c0 < m; c0 += 64) {
+        uint64_t ck = kEmptyKey;
+        int32_t ci = 0x7FFFFFFF;
+        if (c0 + lane < m) {
+            const uint64_t v = bq[c0 + lane];
+            ci = (int32_t)(uint32_t)v;
+            ck = f64_key(dist64_row(KIND, qv, data + (size_t)(uint32_t)(v >> 32) * d, d, nq64));
+        }
+        wave_sort_pairs(ck, ci);
+        // The 64 smallest of both: min(best[i], chunk[63 - i]) is bitonic.
+        const uint64_t rk = shfl_xor_u64(ck, 63);
+        const int32_t ri = __shfl_xor(ci, 63);
+        if (pair_less(rk, ri, bk, bi)) {
+            bk = rk;
+            bi = ri;
+        }
+        wave_bitonic_clean(bk, bi);
+    }
+    const float pad = KIND == kNegCos ? 2.0f : kPadDistance;
+    if (lane < k) {
+        lsh_point_t p;
+        if (bi != 0x7FFFFFFF) {
+            p.ID = bi;
+            p.distance = (float)key_f64(bk);
+        } else {
+            p.ID = -1;
+            p.distance = pad;
+        }
+        out[(size_t)q * k + lane] = p;
+    }
+}
+
 // One 16-lane group per row: out row = (float)(x * 2^14 / ||x||) rounded to
 // FP16, zero padded to hs; padding positions (ids < 0) are zero rows.  A
 // real row whose FP64 norm is zero or not finite sets *bad.
@@ -509,36 +602,75 @@ bool build_unit_rows_half(const float* rows, const int32_t* ids, int64_t npos, i
     return h == 0;
 }
 
-void launch_rerank(const float* data, const float* queries, int d, int kind, const uint64_t* keys,
+bool launch_rerank(const float* data, const float* queries, int d, int kind, const uint64_t* keys,
                    int key_stride, const int32_t* cand, int C, int Q, int k, double eps,
                    lsh_point_t* out, hipStream_t s, const int32_t* qorder, const int32_t* rowmap,
-                   const _Float16* hdata) {
-    if (Q <= 0) return;
+                   const _Float16* hdata, RerankBand* rb, bool recheck_too) {
+    if (Q <= 0) return false;
     const int Cp = pow2_ceil(std::max(C, 2));
     LSH_REQUIRE(Cp <= kMaxTopK, LSH_ERR_INVALID, "candidate set larger than 4096");
     LSH_REQUIRE(!hdata || kind == kAngular || kind == kNegCos, LSH_ERR_INTERNAL,
                 "rerank: the FP16 image serves the angular kinds only");
     const int hs = hdata ? half_row_stride(d) : 0;
-    const size_t lds = (size_t)Cp * (2 * sizeof(uint64_t) + 2 * sizeof(int32_t)) + (size_t)hs * sizeof(float);
+    // Split into the filter and the recheck when a band workspace is given
+    // and k fits one wave.
+    const bool split = rb != nullptr && k <= 64;
+    uint64_t* band = nullptr;
+    uint32_t* bcnt = nullptr;
+    if (split) {
+        rb->band.ensure((size_t)Q * Cp);
+        rb->cnt.ensure((size_t)Q);
+        band = rb->band.get();
+        bcnt = rb->cnt.get();
+    }
+    const size_t lds = (size_t)Cp * ((split ? 1 : 2) * sizeof(uint64_t) + 2 * sizeof(int32_t)) +
+                       (size_t)hs * sizeof(float);
     const int per = (Q + 7) / 8;
     auto go = [&](auto kernel) {
         set_dynamic_lds((const void*)kernel, lds);
         hipLaunchKernelGGL(kernel, dim3(qorder ? 8 * per : Q), dim3(256), lds, s, data, hdata, hs, queries, d, keys,
-                           key_stride, cand, C, Cp, k, eps, out, qorder, Q, per, rowmap);
+                           key_stride, cand, C, Cp, k, eps, out, qorder, Q, per, rowmap, band, bcnt);
     };
-    if (hdata) {
-        if (kind == kAngular) go(rerank_kernel<kAngular, true>);
-        else go(rerank_kernel<kNegCos, true>);
-    } else {
-        switch (kind) {
-            case kNegCos: go(rerank_kernel<kNegCos, false>); break;
-            case kAngular: go(rerank_kernel<kAngular, false>); break;
-            case kGenJaccard: go(rerank_kernel<kGenJaccard, false>); break;
-            case kEuclid: go(rerank_kernel<kEuclid, false>); break;
-            default: fail(LSH_ERR_INTERNAL, "rerank: unknown distance kind");
+    auto pick = [&](auto band_tag) {
+        constexpr bool B = decltype(band_tag)::value;
+        if (hdata) {
+            if (kind == kAngular) go(rerank_kernel<kAngular, true, B>);
+            else go(rerank_kernel<kNegCos, true, B>);
+        } else {
+            switch (kind) {
+                case kNegCos: go(rerank_kernel<kNegCos, false, B>); break;
+                case kAngular: go(rerank_kernel<kAngular, false, B>); break;
+                case kGenJaccard: go(rerank_kernel<kGenJaccard, false, B>); break;
+                case kEuclid: go(rerank_kernel<kEuclid, false, B>); break;
+                default: fail(LSH_ERR_INTERNAL, "rerank: unknown distance kind");
+            }
         }
-    }
+    };
+    if (split) pick(std::true_type{});
+    else pick(std::false_type{});
     check_launch("rerank_kernel");
+    if (split && recheck_too) launch_rerank_recheck(data, queries, d, kind, C, Q, k, out, s, qorder, *rb);
+    return split;
+}
+
+void launch_rerank_recheck(const float* data, const float* queries, int d, int kind, int C, int Q, int k,
+                           lsh_point_t* out, hipStream_t s, const int32_t* qorder, RerankBand& rb) {
+    if (Q <= 0) return;
+    const int Cp = pow2_ceil(std::max(C, 2));
+    LSH_REQUIRE(k <= 64 && rb.band.size() >= (size_t)Q * Cp && rb.cnt.size() >= (size_t)Q, LSH_ERR_INTERNAL,
+                "rerank recheck: no band for these queries");
+    auto recheck = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(Q), dim3(64), 0, s, data, queries, d, rb.band.get(), rb.cnt.get(), Cp, k,
+                           out, qorder);
+    };
+    switch (kind) {
+        case kNegCos: recheck(rerank_recheck_kernel<kNegCos>); break;
+        case kAngular: recheck(rerank_recheck_kernel<kAngular>); break;
+        case kGenJaccard: recheck(rerank_recheck_kernel<kGenJaccard>); break;
+        case kEuclid: recheck(rerank_recheck_kernel<kEuclid>); break;
+        default: fail(LSH_ERR_INTERNAL, "rerank: unknown distance kind");
+    }
+    check_launch("rerank_recheck_kernel");
 }
 
 }  // namespace lshgpu
