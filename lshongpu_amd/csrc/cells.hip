@@ -107,10 +107,11 @@ __global__ __launch_bounds__(256) void pivot_assign_kernel(const uint32_t* __res
 constexpr int kGroup = 8;
 template <int KS>
 __global__ __launch_bounds__(256) void pivot_matrix_kernel(const uint32_t* __restrict__ qs, int Q,
-                                                           const v4i* __restrict__ pfrag, int P, int Pp,
+                                                           const uint32_t* __restrict__ pk, int P, int Pp,
                                                            uint16_t* __restrict__ D) {
     constexpr int W = 2 * KS;
     constexpr int BITS = 32 * W;
+    constexpr int NV = KS / 4;  // uint4 loads per lane per pivot tile
     __shared__ uint32_t tb[4][32][33];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int q0 = (blockIdx.x * 4 + wave) * 32;
@@ -120,12 +121,26 @@ __global__ __launch_bounds__(256) void pivot_matrix_kernel(const uint32_t* __res
 #pragma unroll
     for (int s = 0; s < KS; ++s) b[s] = expand_word(q < Q ? qs[(size_t)q * W + 2 * s + (lane >> 5)] : 0u);
     const int ptiles = (P + 31) / 32;
-    const int pt1 = min(ptiles, (int)(blockIdx.y + 1) * kGroup);
+    const int pt0 = blockIdx.y * kGroup;
+    const int pt1 = min(ptiles, pt0 + kGroup);
     uint32_t(*t)[33] = tb[wave];
-    for (int pt = blockIdx.y * kGroup; pt < pt1; ++pt) {
+    // Lane l's half of pivot row l%32 of tile pt: W/2 packed words.
+    auto load = [&](int pt, uint4* x) {
+        const uint4* src = reinterpret_cast<const uint4*>(pk + ((size_t)pt * 32 + (lane & 31)) * W + (lane >> 5) * KS);
+#pragma unroll
+        for (int i = 0; i < NV; ++i) x[i] = src[i];
+    };
+    uint4 nx[NV];
+    if (pt0 < pt1) load(pt0, nx);
+    for (int pt = pt0; pt < pt1; ++pt) {
+        uint4 cur[NV];
+#pragma unroll
+        for (int i = 0; i < NV; ++i) cur[i] = nx[i];
+        if (pt + 1 < pt1) load(pt + 1, nx);
+        const uint32_t* cw = reinterpret_cast<const uint32_t*>(cur);
         v16f acc = {};
 #pragma unroll
-        for (int s = 0; s < KS; ++s) acc = fp4::mma(pfrag[((size_t)pt * KS + s) * 64 + lane], b[s], acc);
+        for (int s = 0; s < KS; ++s) acc = fp4::mma(expand_word(cw[s]), b[s], acc);
 #pragma unroll
         for (int r = 0; r < 16; ++r) t[lane & 31][fp4::acc_row(r, lane)] = (uint32_t)((BITS - (int)acc[r]) >> 1);
         __builtin_amdgcn_wave_barrier();
@@ -806,33 +821,33 @@ __global__ __launch_bounds__(256) void place_from_cells_kernel(const int32_t* __
 }
 
 template <int KS>
-void launch_pivots_ks(const uint32_t* rows, int64_t n, const uint32_t* pfrag, int P, int Pp, uint32_t* best,
-                      uint16_t* D, hipStream_t s) {
-    const v4i* f = reinterpret_cast<const v4i*>(pfrag);
+void launch_pivots_ks(const uint32_t* rows, int64_t n, const uint32_t* pfrag, const uint32_t* pk, int P, int Pp,
+                      uint32_t* best, uint16_t* D, hipStream_t s) {
     if (best) {
+        const v4i* f = reinterpret_cast<const v4i*>(pfrag);
         hipLaunchKernelGGL((pivot_assign_kernel<KS>), dim3(blocks_for((size_t)((n + 31) / 32), 4)), dim3(256), 0, s,
                            rows, n, f, P, best);
         check_launch("pivot_assign_kernel");
     } else {
         const unsigned gy = (unsigned)(((P + 31) / 32 + kGroup - 1) / kGroup);
         hipLaunchKernelGGL((pivot_matrix_kernel<KS>), dim3(blocks_for((size_t)((n + 31) / 32), 4), gy), dim3(256), 0,
-                           s, rows, (int)n, f, P, Pp, D);
+                           s, rows, (int)n, pk, P, Pp, D);
         check_launch("pivot_matrix_kernel");
     }
 }
 
-// best != nullptr: nearest pivot of each row; otherwise the distance matrix D.
-void pivots(int words, const uint32_t* rows, int64_t n, const uint32_t* pfrag, int P, int Pp, uint32_t* best,
-            uint16_t* D, hipStream_t s) {
+// best != nullptr: nearest pivot of each row (from the FP4 image pfrag);
+// otherwise the distance matrix D (from the packed pivot sketches pk).
+void pivots(int words, const uint32_t* rows, int64_t n, const uint32_t* pfrag, const uint32_t* pk, int P, int Pp,
+            uint32_t* best, uint16_t* D, hipStream_t s) {
     switch (words) {
-        case 8: launch_pivots_ks<4>(rows, n, pfrag, P, Pp, best, D, s); break;
-        case 16: launch_pivots_ks<8>(rows, n, pfrag, P, Pp, best, D, s); break;
-        case 24: launch_pivots_ks<12>(rows, n, pfrag, P, Pp, best, D, s); break;
-        case 32: launch_pivots_ks<16>(rows, n, pfrag, P, Pp, best, D, s); break;
+        case 8: launch_pivots_ks<4>(rows, n, pfrag, pk, P, Pp, best, D, s); break;
+        case 16: launch_pivots_ks<8>(rows, n, pfrag, pk, P, Pp, best, D, s); break;
+        case 24: launch_pivots_ks<12>(rows, n, pfrag, pk, P, Pp, best, D, s); break;
+        case 32: launch_pivots_ks<16>(rows, n, pfrag, pk, P, Pp, best, D, s); break;
         default: fail(LSH_ERR_INTERNAL, "pivot cells need 8, 16, 24 or 32 sketch words");
     }
 }
-
 
 template <int KS>
 void launch_cell_scan_ks(const CellIndex& c, unsigned grid, const uint32_t* coff, const uint32_t* choff,
@@ -907,7 +922,7 @@ void CellIndex::build(const uint32_t* ds, int64_t N, int wpad, hipStream_t s) {
         pfp4.alloc(hamming_fp4_words(n, wpad));
         launch_expand_fp4(psk.get(), n, wpad, pfp4.get(), s);
         // Nearest pivot of every point.
-        pivots(wpad, ds, N, pfp4.get(), n, 0, ckey.get(), nullptr, s);
+        pivots(wpad, ds, N, pfp4.get(), nullptr, n, 0, ckey.get(), nullptr, s);
         if (round == 8 || n >= kMaxPivots) break;
         std::vector<uint32_t> hk((size_t)N);
         LSH_HIP(hipMemcpyAsync(hk.data(), ckey.get(), sizeof(uint32_t) * N, hipMemcpyDeviceToHost, s));
@@ -930,6 +945,20 @@ void CellIndex::build(const uint32_t* ds, int64_t N, int wpad, hipStream_t s) {
         for (size_t i = 0; i < take; ++i) prow.push_back(add[i * add.size() / take]);
     }
     np = (int)prow.size();
+    {
+        // The final pivots' sketches, packed as the cells' tiles are (even
+        // words, then odd), zero rows up to a whole tile.
+        const int Ppad = (np + 31) / 32 * 32;
+        std::vector<int32_t> pr(prow);
+        pr.resize((size_t)Ppad, -1);
+        DevBuf<int32_t> pidx((size_t)Ppad);
+        LSH_HIP(hipMemcpyAsync(pidx.get(), pr.data(), sizeof(int32_t) * Ppad, hipMemcpyHostToDevice, s));
+        pk.alloc((size_t)Ppad * wpad);
+        hipLaunchKernelGGL(pack_cells_kernel, dim3(blocks_for((size_t)Ppad * wpad, 256)), dim3(256), 0, s, ds, wpad,
+                           pidx.get(), (int64_t)Ppad, pk.get());
+        check_launch("pack_cells_kernel");
+        LSH_HIP(hipStreamSynchronize(s));
+    }
 
     // Counting sort by (pivot, distance).
     const uint32_t T = (uint32_t)np << kKeyShift;
@@ -991,8 +1020,8 @@ void cell_rows(const CellIndex& c, const float* data, int64_t N, int d, DevBuf<f
 }
 
 size_t CellIndex::bytes() const {
-    return pfp4.bytes() + sk.bytes() + ids.bytes() + tinfo.bytes() + ctile.bytes() + csize.bytes() + cmin.bytes() +
-           cmax.bytes() + crec.bytes();
+    return pfp4.bytes() + pk.bytes() + sk.bytes() + ids.bytes() + tinfo.bytes() + ctile.bytes() + csize.bytes() +
+           cmin.bytes() + cmax.bytes() + crec.bytes();
 }
 
 const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K, const TopKView& v, Scratch& sc,
@@ -1036,7 +1065,7 @@ const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K
     hipLaunchKernelGGL(cell_clear_kernel, dim3((unsigned)std::min<size_t>(4096, (hn + 255) / 256)), dim3(256), 0, s,
                        cnt1, 2 * (P + 1), stat, hist, hn);
     check_launch("cell_clear_kernel");
-    pivots(c.words, qs, Q, c.pfp4.get(), P, Pp, nullptr, D, s);
+    pivots(c.words, qs, Q, c.pfp4.get(), c.pk.get(), P, Pp, nullptr, D, s);
     // Threshold bound from the tile ranges, the first-pass radii and their
     // cell counts, and the nearest cell of each query.
     auto threshold = [&](auto kernel, int nt) {
