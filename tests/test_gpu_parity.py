@@ -428,6 +428,40 @@ def test_index_device_pointers(lsh):
     np.testing.assert_array_equal(got, host)
 
 
+def test_index_searches_on_streams_are_ordered(lsh):
+    # Device results on a caller's stream: the search may return before its
+    # rerank has run.  Back-to-back searches on two streams into two outputs
+    # are ordered by the index, and each matches the synchronous search; the
+    # rerank's split filter and FP64 recheck (profiling mode 2 times the
+    # filter only) run on the pivot-cell index.
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(25)
+    N, Q, d = 120000, 300, 48
+    centres = rng.standard_normal((400, d)).astype(np.float32)
+    data = centres[rng.integers(0, 400, N)] + 0.3 * rng.standard_normal((N, d)).astype(np.float32)
+    queries = centres[rng.integers(0, 400, Q)] + 0.3 * rng.standard_normal((Q, d)).astype(np.float32)
+    dd, dq = torch.from_numpy(data).cuda(), torch.from_numpy(queries).cuda()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    with lsh.Index(dd, implementation=3, sketchDim=16, distanceFunc=1, candidates=256) as ix:
+        want = ix.search(queries, 32)
+        ix.set_profiling(2)
+        outs = []
+        for rep in range(3):
+            o1 = torch.full((Q, 64), -7, dtype=torch.int32, device="cuda")
+            o2 = torch.full((Q, 64), -7, dtype=torch.int32, device="cuda")
+            torch.cuda.synchronize()  # the fills ran on torch's current stream
+            ix.search(dq, 32, out=o1, stream=s1.cuda_stream)
+            ix.search(dq, 32, out=o2, stream=s2.cuda_stream)
+            outs += [(o1, s1), (o2, s2)]
+        pt = ix.phase_times()
+        ix.set_profiling(False)
+    for o, st in outs:
+        st.synchronize()
+        got = o.cpu().numpy().view(lsh.POINT_DTYPE).reshape(Q, 32)
+        np.testing.assert_array_equal(got, want)
+    assert pt["final_ms"] > 0 and pt["scan_ms"] == 0 and pt["recheck_ms"] == 0
+
+
 # ------------------------------------------------------------ weighted MinHash
 
 def _wmh_rows(rng, n, d, density):
