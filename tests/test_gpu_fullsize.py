@@ -1,0 +1,74 @@
+"""Parity at the benchmark configurations' full sizes.
+
+The oracle cannot sketch millions of rows in test time, so each test pins the
+GPU's own sketches of every row on a random sample of rows (bit for bit
+against the oracle), then checks, for a random sample of queries, that the
+resident index returns exactly what the oracle's top-C and FP64 rerank give
+over those sketches: the index's threshold pass (pivot cells or the sampled
+full pass), selection and rerank are then checked at the real shapes.
+"""
+import numpy as np
+import pytest
+
+from lshongpu_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-5
+
+
+def _close(a, b):
+    np.testing.assert_allclose(a, b, rtol=RTOL, atol=1e-6)
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("cfg,words,C,nsample", [(1, 32, 1024, 64), (4, 16, 1024, 32)])
+def test_simhash_index_full_size(lsh, orc, cfg, words, C, nsample):
+    # cfg 1: 1.2M points, 10k queries, 1024-bit sketches (the benchmark);
+    # cfg 4: 10M points, 2,048 queries, 512-bit sketches.
+    data, queries = synth.config(cfg)
+    N, d = data.shape
+    Q = queries.shape[0]
+    rng = np.random.default_rng(1000 + cfg)
+    planes = orc.normal_stream(words * 32 * d)
+    ds = lsh.simhash_sketch(data, words)
+    rows = rng.choice(N, 256, replace=False)
+    np.testing.assert_array_equal(ds[rows], orc.simhash(data[rows], planes, words))
+    sample = np.sort(rng.choice(Q, nsample, replace=False))
+    qs = orc.simhash(queries[sample], planes, words)
+    np.testing.assert_array_equal(lsh.simhash_sketch(queries[sample], words), qs)
+    with lsh.Index(data, implementation=3, sketchDim=words, distanceFunc=1, candidates=C) as ix:
+        res = ix.search(queries, 32)
+    cand, _ = orc.sketch_topc(0, ds, qs, C)
+    ids, dist = orc.rerank(data, queries[sample], cand, 32, 1)
+    np.testing.assert_array_equal(res["ID"][sample], ids)
+    _close(res["distance"][sample], dist)
+
+
+@pytest.mark.timeout(600)
+def test_weighted_minhash_index_full_size(lsh, orc):
+    # cfg 2: 1M sparse non-negative points, d = 128, 256 weighted-MinHash
+    # values per point, top-32 by matching values, then the
+    # generalized-Jaccard rerank (32 of its 5,000 queries).
+    data, queries = synth.config(2)
+    N, d = data.shape
+    H, C, nsample = 256, 32, 32
+    rng = np.random.default_rng(1002)
+    # The data are non-negative, so the shift is zero whether or not the
+    # queries take part; the index bounds the draws over its data alone.
+    td, tq = orc.gj_transform(data, queries)
+    ds = lsh.wmh_sketch(td, H)
+    bounds, start, M = orc.wmh_bounds(td)
+    bm = orc.bit_stream(4096)
+    rows = rng.choice(N, 128, replace=False)
+    np.testing.assert_array_equal(ds[rows], orc.wmh_sketch(td[rows], bounds, start, M, H, 0, bm))
+    sample = np.sort(rng.choice(queries.shape[0], nsample, replace=False))
+    qs = orc.wmh_sketch(tq[sample], bounds, start, M, H, 0, bm)
+    # The match-count scan has no matrix-core form: the sampled queries are
+    # searched (against all 1M points).
+    with lsh.Index(data, implementation=4, sketchDim=H, distanceFunc=2, candidates=C) as ix:
+        res = ix.search(queries[sample], 32)
+    cand, _ = orc.sketch_topc(1, ds, qs, C)
+    ids, dist = orc.rerank(td, tq[sample], cand, 32, 2)
+    np.testing.assert_array_equal(res["ID"], ids)
+    _close(res["distance"], dist)
