@@ -72,3 +72,32 @@ def test_weighted_minhash_index_full_size(lsh, orc):
     ids, dist = orc.rerank(td, tq[sample], cand, 32, 2)
     np.testing.assert_array_equal(res["ID"], ids)
     _close(res["distance"], dist)
+
+
+@pytest.mark.timeout(600)
+def test_lsh_framework_full_size(lsh, orc):
+    # cfg 3: 4M clustered points, d = 128, (K,L) framework with one
+    # cross-polytope key per table (2P = 256 buckets), L = 32 tables, exact
+    # angular rescan of each query's bucket union (16 of its 50,000 queries).
+    data, queries = synth.config(3)
+    N, d = data.shape
+    tables, bits, k, nsample = 32, 8, 32, 16
+    rng = np.random.default_rng(1003)
+    sample = np.sort(rng.choice(queries.shape[0], nsample, replace=False))
+    qs = queries[sample]
+    P = 128
+    signs = orc.sign_stream(tables * 3 * P)
+    xn, qn = orc.normalize_rows(data), orc.normalize_rows(qs)
+    dk = lsh.crosspoly_keys(data, tables)
+    rows = rng.choice(N, 256, replace=False)
+    np.testing.assert_array_equal(dk[rows], orc.crosspoly(xn[rows], tables, signs))
+    qk = orc.crosspoly(qn, tables, signs)
+    np.testing.assert_array_equal(lsh.crosspoly_keys(qs, tables), qk)
+    res = lsh.LSH(2, 7, k, d, nsample, N, data, qs, 1, 1, bits, tables, False)
+    dk32, qk32 = dk.astype(np.uint32), qk.astype(np.uint32)
+    for i in range(nsample):
+        cand = orc.lsh_candidates(dk32, qk32, i)
+        assert cand.size >= k
+        ids, dist = orc.rerank(xn, qn[i:i + 1], cand[None, :], k, 1)
+        np.testing.assert_array_equal(res.ids[i], ids[0])
+        _close(res.distances[i], dist[0])
