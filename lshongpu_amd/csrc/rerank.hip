@@ -50,17 +50,52 @@ __device__ inline void wave_sort_pairs(uint64_t& k, int32_t& id) {
         }
 }
 
-// High word of the kk-th smallest (1-based) of the n keys: a radix select
-// over four 8-bit digits of the high word.
-__device__ uint32_t kth_high_word(const uint64_t* keys, int n, int kk, uint32_t* hist, uint32_t* sv) {
-    const int tid = threadIdx.x, bs = blockDim.x;
-    uint32_t prefix = 0, mask = 0, rank = (uint32_t)kk;
-    for (int shift = 24; shift >= 0; shift -= 8) {
+// High word of the kk-th smallest (1-based) of the n keys (kk <= n).  The
+// high words' minimum and maximum fix the bits they share; 8-bit digits are
+// then histogrammed from the first bit that differs down, until the bin
+// holding the kk-th has at most 256 keys, whose high words are gathered (in
+// ties) and ranked against each other.  sv: 8 words, hist: 256 words, ties:
+// min(n, 256) words, all LDS; every thread must call it.
+__device__ uint32_t kth_high_word(const uint64_t* keys, int n, int kk, uint32_t* hist, uint32_t* sv,
+                                  uint32_t* ties) {
+    const int tid = threadIdx.x, bs = blockDim.x, lane = tid & 63;
+    if (tid == 0) {
+        sv[0] = 0xFFFFFFFFu;
+        sv[1] = 0u;
+        sv[4] = 0u;
+    }
+    __syncthreads();
+    uint32_t mn = 0xFFFFFFFFu, mx = 0u;
+    for (int t = tid; t < n; t += bs) {
+        const uint32_t h = (uint32_t)(keys[t] >> 32);
+        mn = min(mn, h);
+        mx = max(mx, h);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        mn = min(mn, (uint32_t)__shfl_xor((int)mn, o));
+        mx = max(mx, (uint32_t)__shfl_xor((int)mx, o));
+    }
+    if (lane == 0) {
+        atomicMin(&sv[0], mn);
+        atomicMax(&sv[1], mx);
+    }
+    __syncthreads();
+    const uint32_t lo = sv[0], hi = sv[1];
+    if (lo == hi) return lo;
+    // The first differing bit is 31 - clz(lo ^ hi); the digit [shift, shift
+    // + 8) holds it, and every key shares lo's bits above the digit.
+    int shift = max(0, 24 - (int)__builtin_clz(lo ^ hi));
+    uint32_t mask = shift + 8 >= 32 ? 0u : ~0u << (shift + 8);
+    uint32_t prefix = lo & mask;
+    uint32_t rank = (uint32_t)kk;
+    uint32_t nt;
+    while (true) {
         for (int i = tid; i < 256; i += bs) hist[i] = 0;
         __syncthreads();
         for (int t = tid; t < n; t += bs) {
-            const uint32_t dk = (uint32_t)(keys[t] >> 32);
-            if ((dk & mask) == prefix) atomicAdd(&hist[(dk >> shift) & 255u], 1u);
+            const uint32_t h = (uint32_t)(keys[t] >> 32);
+            if ((h & mask) == prefix) atomicAdd(&hist[(h >> shift) & 255u], 1u);
         }
         __syncthreads();
         if (tid < 64) {
@@ -80,19 +115,44 @@ __device__ uint32_t kth_high_word(const uint64_t* keys, int n, int kk, uint32_t*
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 if (run < rank && run + c[e] >= rank) {
-                    sv[0] = 4u * tid + e;
-                    sv[1] = rank - run;
+                    sv[2] = 4u * tid + e;
+                    sv[3] = rank - run;
+                    sv[5] = c[e];
                 }
                 run += c[e];
             }
         }
         __syncthreads();
-        prefix |= sv[0] << shift;
-        mask |= 0xFFu << shift;
-        rank = sv[1];
+        const uint32_t dig = sv[2];
+        rank = sv[3];
+        nt = sv[5];
+        prefix |= dig << shift;
+        mask |= 255u << shift;
         __syncthreads();
+        if (shift == 0) return prefix;  // every bit fixed
+        if (nt <= 256u) break;
+        shift = max(0, shift - 8);
     }
-    return prefix;
+    // The nt keys of the bin: the one with at most rank - 1 smaller high
+    // words and more than rank - 1 smaller or equal ones.
+    for (int t = tid; t < n; t += bs) {
+        const uint32_t h = (uint32_t)(keys[t] >> 32);
+        if ((h & mask) == prefix) ties[atomicAdd(&sv[4], 1u)] = h;
+    }
+    __syncthreads();
+    const uint32_t want = rank - 1u;
+    for (uint32_t i = tid; i < nt; i += bs) {
+        const uint32_t x = ties[i];
+        uint32_t lt = 0, le = 0;
+        for (uint32_t j = 0; j < nt; ++j) {
+            const uint32_t y = ties[j];
+            lt += y < x ? 1u : 0u;
+            le += y <= x ? 1u : 0u;
+        }
+        if (lt <= want && want < le) sv[6] = x;
+    }
+    __syncthreads();
+    return sv[6];
 }
 
 // Sum over the 16 lanes of a DPP row, in every lane of it: rotations by 8,
@@ -309,7 +369,7 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
     int32_t* rows = reinterpret_cast<int32_t*>(k32 + Cp);            // Cp: data row of each candidate
     float* qs = reinterpret_cast<float*>(rows + Cp);                 // hs (HALF only)
     __shared__ uint32_t rs_hist[256];
-    __shared__ uint32_t rs_sv[4];
+    __shared__ uint32_t rs_sv[8];
     // With an order, XCD x (workgroups x, x + 8, ...) takes the x-th eighth
     // of it in sequence.
     int pos = blockIdx.x, q = pos;  // pos: place in the list of queries (band row)
@@ -371,7 +431,8 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
     // The band: approximate distance at most ub (as a key's high word).
     uint32_t ub = 0xFFFFFFFFu;
     if (k < Cp && !all) {
-        const uint32_t kth = kth_high_word(k32, Cp, k, rs_hist, rs_sv);
+        // ids is free until the band is collected: it holds the ties.
+        const uint32_t kth = kth_high_word(k32, Cp, k, rs_hist, rs_sv, reinterpret_cast<uint32_t*>(ids));
         const double vk = (double)key_f32(kth);
         const double t = KIND == kEuclid ? vk * (1.0 + eps) / (1.0 - eps) : vk + 2.0 * eps;
         if (kth != 0xFFFFFFFFu && __builtin_isfinite(t) && t < 3.0e38) {
