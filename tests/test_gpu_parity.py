@@ -120,11 +120,16 @@ def test_exact_scan_near_ties_escalate(lsh, orc):
 
 # ------------------------------------------------------------ sketch stages
 
-@pytest.mark.parametrize("words,bits", [(32, 32), (3, 32), (12, 10)])
-def test_simhash_bit_exact(lsh, orc, words, bits):
-    rng = np.random.default_rng(words * 100 + bits)
-    x = rng.standard_normal((3001, 100)).astype(np.float32)
-    planes = orc.normal_stream(words * bits * 100)
+# Rows staged in LDS (d <= 255; full and partial words, word counts not a
+# multiple of the 4 words a workgroup computes, rows not a multiple of its 64)
+# and the unstaged kernel (d = 300).
+@pytest.mark.parametrize("words,bits,n,d", [(32, 32, 3001, 100), (3, 32, 3001, 100), (12, 10, 3001, 100),
+                                            (5, 32, 1027, 37), (4, 32, 130, 64), (1, 32, 2, 1),
+                                            (7, 32, 515, 129), (2, 17, 70, 300)])
+def test_simhash_bit_exact(lsh, orc, words, bits, n, d):
+    rng = np.random.default_rng(words * 100 + bits + d)
+    x = rng.standard_normal((n, d)).astype(np.float32)
+    planes = orc.normal_stream(words * bits * d)
     np.testing.assert_array_equal(lsh.simhash_sketch(x, words, bits), orc.simhash(x, planes, words, bits))
 
 
