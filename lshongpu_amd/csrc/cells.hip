@@ -249,15 +249,12 @@ __global__ void cell_rows_kernel(const float* __restrict__ src, int d, const int
     }
 }
 
-// Zeroes the list counts, the stat words and the first-pass histograms of
-// a search in one launch.
-__global__ void cell_clear_kernel(uint32_t* __restrict__ cnt, int ncnt, uint32_t* __restrict__ stat,
-                                  uint32_t* __restrict__ hist, size_t nhist) {
-    const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (size_t i = i0; i < nhist; i += (size_t)gridDim.x * blockDim.x) hist[i] = 0u;
-    if (i0 < (size_t)ncnt) cnt[i0] = 0u;
+// Zeroes the list counts and the stat words of a search in one launch (the
+// first-pass histograms are zeroed by the threshold kernel).
+__global__ void cell_clear_kernel(uint32_t* __restrict__ cnt, int ncnt, uint32_t* __restrict__ stat) {
+    const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
     if (i0 < 4) stat[i0] = 0u;
-    for (size_t i = i0 + (size_t)gridDim.x * blockDim.x; i < (size_t)ncnt; i += (size_t)gridDim.x * blockDim.x) cnt[i] = 0u;
+    for (int i = i0; i < ncnt; i += gridDim.x * blockDim.x) cnt[i] = 0u;
 }
 
 __device__ inline uint32_t lower_bound_to(uint32_t dq, uint32_t lo, uint32_t hi) {
@@ -365,6 +362,7 @@ __device__ inline int block_kth_bin(const uint32_t* hist, int nbins, uint32_t K,
 // one batch, and every step reads them from registers.
 constexpr int kRadiusMargin = 16;
 constexpr int kWindow = 256;
+constexpr int kHistBins = 128;  // first-pass histogram bins per query (8 bits each)
 template <int CPT, int NT>
 __global__ __launch_bounds__(NT) void cell_threshold_kernel(const uint16_t* __restrict__ D, int Q, int P, int Pp,
                                                             const uint2* __restrict__ crec, int bits, int K,
@@ -373,7 +371,8 @@ __global__ __launch_bounds__(NT) void cell_threshold_kernel(const uint16_t* __re
                                                             uint32_t* __restrict__ cnt1,
                                                             int32_t* __restrict__ qcells1, uint32_t* __restrict__ n1,
                                                             int32_t* __restrict__ qcells2,
-                                                            uint32_t* __restrict__ n2) {
+                                                            uint32_t* __restrict__ n2,
+                                                            uint32_t* __restrict__ hist) {
     __shared__ uint32_t h1[kWindow];
     __shared__ uint32_t h2[2049];
     __shared__ uint32_t ws[NT / 64];
@@ -397,6 +396,9 @@ __global__ __launch_bounds__(NT) void cell_threshold_kernel(const uint16_t* __re
         if (b < kWindow) h1[b] = 0;
         h2[b] = 0;
     }
+    // The query's first-pass histogram starts at zero (the first pass runs
+    // after this kernel).
+    for (int b = threadIdx.x; b < kHistBins; b += NT) hist[(size_t)q * kHistBins + b] = 0u;
     if (threadIdx.x == 0) {
         best = 0xFFFFFFFFu;
         s_n1 = 0;
@@ -576,7 +578,6 @@ __global__ __launch_bounds__(1024) void cell_offsets_kernel(uint32_t* __restrict
 // counters share a word; a wave flushes them to the global histogram at the
 // end of its chunk and after every kFlushTiles tiles (32 pairs per query
 // per tile, so no counter reaches 2^16).
-constexpr int kHistBins = 128;
 constexpr int kFlushTiles = 2047;
 __device__ inline int hist_bin(int d) { return min(d >> 3, kHistBins - 1); }
 
@@ -1062,15 +1063,15 @@ const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K
     // of the tiles of a full pass (query tiles x point tiles).
     const uint32_t dense_limit = (uint32_t)std::min<double>(4e9, 0.5 * (double)qgroups * (double)c.ntiles);
 
-    hipLaunchKernelGGL(cell_clear_kernel, dim3((unsigned)std::min<size_t>(4096, (hn + 255) / 256)), dim3(256), 0, s,
-                       cnt1, 2 * (P + 1), stat, hist, hn);
+    hipLaunchKernelGGL(cell_clear_kernel, dim3((unsigned)((2 * (P + 1) + 255) / 256)), dim3(256), 0, s, cnt1,
+                       2 * (P + 1), stat);
     check_launch("cell_clear_kernel");
     pivots(c.words, qs, Q, c.pfp4.get(), c.pk.get(), P, Pp, nullptr, D, s);
     // Threshold bound from the tile ranges, the first-pass radii and their
     // cell counts, and the nearest cell of each query.
     auto threshold = [&](auto kernel, int nt) {
         hipLaunchKernelGGL(kernel, dim3((unsigned)Q), dim3(nt), 0, s, D, Q, P, Pp, c.crec.get(), bits, K, v.thr, R,
-                           nearest, cnt1, qcells1, n1, qcells2, n2);
+                           nearest, cnt1, qcells1, n1, qcells2, n2, hist);
     };
     LSH_REQUIRE(P >= 1 && P <= 256 * 32, LSH_ERR_INTERNAL, "cell_search: pivot count outside the threshold kernel's range");
     if (P <= 256 * 4) threshold(cell_threshold_kernel<4, 256>, 256);

@@ -186,7 +186,7 @@ void TopKState::reset(int Q, int K, int newcap, hipStream_t s, bool int_keys, bo
     // A fill kernel rather than memsets keeps the sequence capturable in a
     // graph.  Only the best lists need clearing: the selects read the
     // appended slots up to each query's count.
-    hipLaunchKernelGGL(reset_state_kernel, dim3(grid_for((size_t)Q * K)), dim3(256), 0, s, v_);
+    hipLaunchKernelGGL(reset_state_kernel, dim3(grid_for(lazy ? (size_t)Q : (size_t)Q * K)), dim3(256), 0, s, v_);
     check_launch("top-K state reset");
 }
 
