@@ -108,7 +108,11 @@ constexpr int kGroup = 8;
 template <int KS>
 __global__ __launch_bounds__(256) void pivot_matrix_kernel(const uint32_t* __restrict__ qs, int Q,
                                                            const uint32_t* __restrict__ pk, int P, int Pp,
-                                                           uint16_t* __restrict__ D) {
+                                                           uint16_t* __restrict__ D, uint32_t* __restrict__ zero,
+                                                           int nzero) {
+    // The first workgroup also zeroes zero[0, nzero) (a search's counters).
+    if (blockIdx.x == 0 && blockIdx.y == 0)
+        for (int i = threadIdx.x; i < nzero; i += blockDim.x) zero[i] = 0u;
     constexpr int W = 2 * KS;
     constexpr int BITS = 32 * W;
     constexpr int NV = KS / 4;  // uint4 loads per lane per pivot tile
@@ -247,14 +251,6 @@ __global__ void cell_rows_kernel(const float* __restrict__ src, int d, const int
         dst[i] = row >= 0 ? src[(size_t)row * d + c] : 0.0f;
         if (c == 0 && row >= 0) pos_of[row] = (int32_t)pos;
     }
-}
-
-// Zeroes the list counts and the stat words of a search in one launch (the
-// first-pass histograms are zeroed by the threshold kernel).
-__global__ void cell_clear_kernel(uint32_t* __restrict__ cnt, int ncnt, uint32_t* __restrict__ stat) {
-    const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i0 < 4) stat[i0] = 0u;
-    for (int i = i0; i < ncnt; i += gridDim.x * blockDim.x) cnt[i] = 0u;
 }
 
 __device__ inline uint32_t lower_bound_to(uint32_t dq, uint32_t lo, uint32_t hi) {
@@ -823,7 +819,7 @@ __global__ __launch_bounds__(256) void place_from_cells_kernel(const int32_t* __
 
 template <int KS>
 void launch_pivots_ks(const uint32_t* rows, int64_t n, const uint32_t* pfrag, const uint32_t* pk, int P, int Pp,
-                      uint32_t* best, uint16_t* D, hipStream_t s) {
+                      uint32_t* best, uint16_t* D, uint32_t* zero, int nzero, hipStream_t s) {
     if (best) {
         const v4i* f = reinterpret_cast<const v4i*>(pfrag);
         hipLaunchKernelGGL((pivot_assign_kernel<KS>), dim3(blocks_for((size_t)((n + 31) / 32), 4)), dim3(256), 0, s,
@@ -832,20 +828,21 @@ void launch_pivots_ks(const uint32_t* rows, int64_t n, const uint32_t* pfrag, co
     } else {
         const unsigned gy = (unsigned)(((P + 31) / 32 + kGroup - 1) / kGroup);
         hipLaunchKernelGGL((pivot_matrix_kernel<KS>), dim3(blocks_for((size_t)((n + 31) / 32), 4), gy), dim3(256), 0,
-                           s, rows, (int)n, pk, P, Pp, D);
+                           s, rows, (int)n, pk, P, Pp, D, zero, nzero);
         check_launch("pivot_matrix_kernel");
     }
 }
 
 // best != nullptr: nearest pivot of each row (from the FP4 image pfrag);
-// otherwise the distance matrix D (from the packed pivot sketches pk).
+// otherwise the distance matrix D (from the packed pivot sketches pk), whose
+// launch also zeroes zero[0, nzero).
 void pivots(int words, const uint32_t* rows, int64_t n, const uint32_t* pfrag, const uint32_t* pk, int P, int Pp,
-            uint32_t* best, uint16_t* D, hipStream_t s) {
+            uint32_t* best, uint16_t* D, hipStream_t s, uint32_t* zero = nullptr, int nzero = 0) {
     switch (words) {
-        case 8: launch_pivots_ks<4>(rows, n, pfrag, pk, P, Pp, best, D, s); break;
-        case 16: launch_pivots_ks<8>(rows, n, pfrag, pk, P, Pp, best, D, s); break;
-        case 24: launch_pivots_ks<12>(rows, n, pfrag, pk, P, Pp, best, D, s); break;
-        case 32: launch_pivots_ks<16>(rows, n, pfrag, pk, P, Pp, best, D, s); break;
+        case 8: launch_pivots_ks<4>(rows, n, pfrag, pk, P, Pp, best, D, zero, nzero, s); break;
+        case 16: launch_pivots_ks<8>(rows, n, pfrag, pk, P, Pp, best, D, zero, nzero, s); break;
+        case 24: launch_pivots_ks<12>(rows, n, pfrag, pk, P, Pp, best, D, zero, nzero, s); break;
+        case 32: launch_pivots_ks<16>(rows, n, pfrag, pk, P, Pp, best, D, zero, nzero, s); break;
         default: fail(LSH_ERR_INTERNAL, "pivot cells need 8, 16, 24 or 32 sketch words");
     }
 }
@@ -1032,16 +1029,18 @@ const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K
     const int Pp = (P + 31) / 32 * 32;
     const int bits = 32 * c.words;
     uint16_t* D = sc.get<uint16_t>(depth, 0, (size_t)Q * Pp);
-    // meta: cnt1, cnt2 [P+1 each], coff [P+1], choff [P+2], nearest cell [Q],
-    // stat [4] = {second pass too dense, first-pass tile products, second-pass
-    // tile products}.
+    // meta: cnt1, cnt2 [P+1 each], stat [4] = {second pass too dense,
+    // first-pass tile products, second-pass tile products}, coff [P+1],
+    // choff [P+2], nearest cell [Q].  The first 2 (P+1) + 4 words start at
+    // zero (cleared by the pivot-matrix launch).
     uint32_t* meta = sc.get<uint32_t>(depth, 1, 4 * ((size_t)P + 1) + 1 + Q + 4);
     uint32_t* cnt1 = meta;
     uint32_t* cnt2 = meta + (P + 1);
-    uint32_t* coff = meta + 2 * (P + 1);
-    uint32_t* choff = meta + 3 * (P + 1);
-    uint32_t* nearest = meta + 4 * (P + 1) + 1;
-    uint32_t* stat = nearest + Q;
+    uint32_t* stat = meta + 2 * (P + 1);
+    uint32_t* coff = stat + 4;
+    uint32_t* choff = coff + (P + 1);
+    uint32_t* nearest = choff + (P + 2);
+    const int nzero = 2 * (P + 1) + 4;
     // hist [Q][kHistBins], the per-query cell lists [Q][P] x 2 and their
     // lengths [Q] x 2, the radii [Q] (u16).
     const size_t hn = (size_t)Q * kHistBins, qp = (size_t)Q * P;
@@ -1063,10 +1062,7 @@ const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K
     // of the tiles of a full pass (query tiles x point tiles).
     const uint32_t dense_limit = (uint32_t)std::min<double>(4e9, 0.5 * (double)qgroups * (double)c.ntiles);
 
-    hipLaunchKernelGGL(cell_clear_kernel, dim3((unsigned)((2 * (P + 1) + 255) / 256)), dim3(256), 0, s, cnt1,
-                       2 * (P + 1), stat);
-    check_launch("cell_clear_kernel");
-    pivots(c.words, qs, Q, c.pfp4.get(), c.pk.get(), P, Pp, nullptr, D, s);
+    pivots(c.words, qs, Q, c.pfp4.get(), c.pk.get(), P, Pp, nullptr, D, s, meta, nzero);
     // Threshold bound from the tile ranges, the first-pass radii and their
     // cell counts, and the nearest cell of each query.
     auto threshold = [&](auto kernel, int nt) {

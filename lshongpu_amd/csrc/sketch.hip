@@ -49,11 +49,39 @@ __global__ __launch_bounds__(256) void simhash_staged_kernel(const float* __rest
     if (hb == 32) {
         // Full words: the 32 hyperplane values of a dimension are one
         // contiguous uniform block (wide scalar loads, no per-bit tests).
-        for (int j = 0; j < d; ++j) {
-            const float xj = xr[j];
-            const float* __restrict__ pj = p + (size_t)j * 32;
+        // Two register sets alternate: the block and row value of dimension
+        // j + 1 load while dimension j's multiply-adds run.  Scalar loads
+        // complete out of order, so any use waits for every load in flight:
+        // each set's first use (which takes that wait) is fenced before the
+        // other set's loads are issued, and its other 31 multiply-adds then
+        // cover their latency.
+        float pa[32], pb[32];
 #pragma unroll
-            for (int b = 0; b < 32; ++b) acc[b] = __builtin_fmaf(xj, pj[b], acc[b]);
+        for (int b = 0; b < 32; ++b) pa[b] = p[b];
+        float xa = xr[0], xb;
+        int j = 0;
+        for (; j + 1 < d; j += 2) {
+            acc[0] = __builtin_fmaf(xa, pa[0], acc[0]);
+            __builtin_amdgcn_sched_barrier(0);
+            xb = xr[j + 1];
+#pragma unroll
+            for (int b = 0; b < 32; ++b) pb[b] = p[(size_t)(j + 1) * 32 + b];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int b = 1; b < 32; ++b) acc[b] = __builtin_fmaf(xa, pa[b], acc[b]);
+            acc[0] = __builtin_fmaf(xb, pb[0], acc[0]);
+            __builtin_amdgcn_sched_barrier(0);
+            const int jn = min(j + 2, d - 1);  // past d - 1 the set is loaded but unused
+            xa = xr[jn];
+#pragma unroll
+            for (int b = 0; b < 32; ++b) pa[b] = p[(size_t)jn * 32 + b];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int b = 1; b < 32; ++b) acc[b] = __builtin_fmaf(xb, pb[b], acc[b]);
+        }
+        if (j < d) {  // odd d: the last dimension is in the first set
+#pragma unroll
+            for (int b = 0; b < 32; ++b) acc[b] = __builtin_fmaf(xa, pa[b], acc[b]);
         }
     } else {
         for (int j = 0; j < d; ++j) {
