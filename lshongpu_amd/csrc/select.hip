@@ -60,6 +60,29 @@ __global__ __launch_bounds__(kSelThreads) void select_merge_kernel(TopKView v) {
 // bin (8-bit digits of the ID, or of the whole key in the last bin, which
 // also holds every larger distance) finds the K-th key itself, and every key
 // at or below it is kept.  thr becomes that key.
+//
+// RK > 0: every thread loads its keys (at most RK, i = thread + 256 r) into
+// registers once, and every pass reads them there; the host picks it when
+// the view can hold at most 256 RK keys per query.
+template <int RK>
+struct KeysInRegisters {
+    uint64_t key[RK];
+    __device__ KeysInRegisters(const uint64_t* in, int n) {
+#pragma unroll
+        for (int r = 0; r < RK; ++r) {
+            const int i = (int)threadIdx.x + kSelThreads * r;
+            key[r] = i < n ? in[i] : kEmptyKey;
+        }
+    }
+    template <class F>
+    __device__ void operator()(F&& f) const {
+#pragma unroll
+        for (int r = 0; r < RK; ++r)
+            if (key[r] != kEmptyKey) f(key[r]);
+    }
+};
+
+template <int RK>
 __global__ __launch_bounds__(kSelThreads) void select_hist_kernel(TopKView v) {
     extern __shared__ uint64_t kept[];  // K
     __shared__ uint32_t hist[kSelBins];
@@ -77,14 +100,19 @@ __global__ __launch_bounds__(kSelThreads) void select_hist_kernel(TopKView v) {
     const uint64_t* in = v.fresh ? row + K : row;
     const int total = v.fresh ? n : K + n;
     if (threadIdx.x == 0) s_nkeep = 0;
-    const uint64_t kth = int_kth_key(in, total, (uint32_t)K, hist, ties, sv);  // the K-th key (none: keep all)
-    for (int i = threadIdx.x; i < total; i += blockDim.x) {
-        const uint64_t key = in[i];
-        if (key != kEmptyKey && key <= kth) {
-            const uint32_t at = atomicAdd(&s_nkeep, 1u);
-            if (at < (uint32_t)K) kept[at] = key;
-        }
-    }
+    auto run = [&](const auto& each) {
+        const uint64_t kth = int_kth_key(each, (uint32_t)K, hist, ties, sv);  // the K-th key (none: keep all)
+        each([&](uint64_t key) {
+            if (key != kEmptyKey && key <= kth) {
+                const uint32_t at = atomicAdd(&s_nkeep, 1u);
+                if (at < (uint32_t)K) kept[at] = key;
+            }
+        });
+        return kth;
+    };
+    uint64_t kth;
+    if constexpr (RK > 0) kth = run(KeysInRegisters<RK>(in, total));
+    else kth = run(KeysInMemory{in, total});
     __syncthreads();
     const uint32_t nkeep = min(s_nkeep, (uint32_t)K);
     for (uint32_t i = threadIdx.x; i < nkeep; i += blockDim.x) row[i] = kept[i];
@@ -193,8 +221,15 @@ void TopKState::reset(int Q, int K, int newcap, hipStream_t s, bool int_keys, bo
 void launch_select_merge(const TopKView& v, hipStream_t s) {
     if (v.int_keys) {
         const size_t lds = (size_t)v.K * sizeof(uint64_t);
-        set_dynamic_lds((const void*)select_hist_kernel, lds + sizeof(uint32_t) * (kSelBins + 8));
-        hipLaunchKernelGGL(select_hist_kernel, dim3(v.Q), dim3(kSelThreads), lds, s, v);
+        const int most = v.fresh ? v.newcap : v.K + v.newcap;  // keys a query can hold
+        constexpr int RK = 12;
+        if (most <= kSelThreads * RK) {
+            set_dynamic_lds((const void*)select_hist_kernel<RK>, lds + sizeof(uint32_t) * (kSelBins + 8));
+            hipLaunchKernelGGL(select_hist_kernel<RK>, dim3(v.Q), dim3(kSelThreads), lds, s, v);
+        } else {
+            set_dynamic_lds((const void*)select_hist_kernel<0>, lds + sizeof(uint32_t) * (kSelBins + 8));
+            hipLaunchKernelGGL(select_hist_kernel<0>, dim3(v.Q), dim3(kSelThreads), lds, s, v);
+        }
         check_launch("select_hist_kernel");
         return;
     }

@@ -17,20 +17,32 @@ constexpr int kSelBins = 2048;
 constexpr int kTieCap = 512;      // keys of the boundary bin gathered in LDS (more: selected in place)
 constexpr int kSelScratch = 8;    // LDS words int_kth_key uses for its shared scalars
 
-// The rank-th smallest (1-based) key among the non-empty keys of row[0, n)
-// whose bin is T and whose bits above `shift` + 8 match prefix; shifts run
-// from `top` down to 0.  Uses hist[0, 256).
-__device__ inline uint64_t radix_kth_in_bin(const uint64_t* row, int n, uint32_t T, uint32_t rank, int top,
-                                            uint64_t prefix, uint32_t* hist, uint32_t* s_dig, uint32_t* s_rank) {
+// The keys are visited through `each(f)`, which calls f(key) for every key
+// the calling thread owns (every thread of the workgroup together covers the
+// set once): from memory, or from registers loaded once.
+struct KeysInMemory {
+    const uint64_t* in;
+    int n;
+    template <class F>
+    __device__ void operator()(F&& f) const {
+        for (int i = threadIdx.x; i < n; i += blockDim.x) f(in[i]);
+    }
+};
+
+// The rank-th smallest (1-based) key among the non-empty keys whose bin is T
+// and whose bits above `shift` + 8 match prefix; shifts run from `top` down
+// to 0.  Uses hist[0, 256).
+template <class Each>
+__device__ inline uint64_t radix_kth_in_bin(const Each& each, uint32_t T, uint32_t rank, int top, uint64_t prefix,
+                                            uint32_t* hist, uint32_t* s_dig, uint32_t* s_rank) {
     uint64_t mask = top == 24 ? 0xFFFFFFFF00000000ull : 0ull;
     for (int shift = top; shift >= 0; shift -= 8) {
         for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0;
         __syncthreads();
-        for (int i = threadIdx.x; i < n; i += blockDim.x) {
-            const uint64_t key = row[i];
-            if (key == kEmptyKey || min((uint32_t)(key >> 32), (uint32_t)(kSelBins - 1)) != T) continue;
+        each([&](uint64_t key) {
+            if (key == kEmptyKey || min((uint32_t)(key >> 32), (uint32_t)(kSelBins - 1)) != T) return;
             if ((key & mask) == prefix) atomicAdd(&hist[(uint32_t)(key >> shift) & 255u], 1u);
-        }
+        });
         __syncthreads();
         if (threadIdx.x < 64) {
             const int lane = threadIdx.x;
@@ -65,22 +77,21 @@ __device__ inline uint64_t radix_kth_in_bin(const uint64_t* row, int n, uint32_t
     return prefix;
 }
 
-// The K-th smallest non-empty key of in[0, total) (K >= 1), or kEmptyKey when
-// there are fewer than K.  LDS: hist[kSelBins], ties[kTieCap],
-// sv[kSelScratch].  Every thread of the workgroup must call it; it ends with
-// a barrier, so the caller may reuse that LDS at once.
-__device__ inline uint64_t int_kth_key(const uint64_t* in, int total, uint32_t K, uint32_t* hist, uint64_t* ties,
-                                       uint32_t* sv) {
+// The K-th smallest non-empty key of the set `each` visits (K >= 1), or
+// kEmptyKey when there are fewer than K.  LDS: hist[kSelBins],
+// ties[kTieCap], sv[kSelScratch].  Every thread of the workgroup must call
+// it; it ends with a barrier, so the caller may reuse that LDS at once.
+template <class Each>
+__device__ inline uint64_t int_kth_key(const Each& each, uint32_t K, uint32_t* hist, uint64_t* ties, uint32_t* sv) {
     uint32_t& s_T = sv[0];
     uint32_t& s_below = sv[1];
     uint32_t& s_nties = sv[4];
     for (int i = threadIdx.x; i < kSelBins; i += blockDim.x) hist[i] = 0;
     if (threadIdx.x == 0) s_nties = 0;
     __syncthreads();
-    for (int i = threadIdx.x; i < total; i += blockDim.x) {
-        const uint64_t key = in[i];
+    each([&](uint64_t key) {
         if (key != kEmptyKey) atomicAdd(&hist[min((uint32_t)(key >> 32), (uint32_t)(kSelBins - 1))], 1u);
-    }
+    });
     __syncthreads();
     if (threadIdx.x < 64) {
         // Wave 0 finds the first bin at which the running count reaches K.
@@ -125,11 +136,10 @@ __device__ inline uint64_t int_kth_key(const uint64_t* in, int total, uint32_t K
             // Gather them; the K-th is the key with at most want = K - below
             // - 1 smaller keys and more than want smaller or equal (0 <=
             // want < nt, so exactly one key value qualifies).
-            for (int i = threadIdx.x; i < total; i += blockDim.x) {
-                const uint64_t key = in[i];
+            each([&](uint64_t key) {
                 if (key != kEmptyKey && min((uint32_t)(key >> 32), (uint32_t)(kSelBins - 1)) == T)
                     ties[atomicAdd(&s_nties, 1u)] = key;
-            }
+            });
             __syncthreads();
             uint64_t* s_kth = reinterpret_cast<uint64_t*>(hist);  // hist is no longer needed
             const uint32_t want = K - below - 1u;
@@ -146,8 +156,8 @@ __device__ inline uint64_t int_kth_key(const uint64_t* in, int total, uint32_t K
             __syncthreads();
             kth = *s_kth;
         } else {
-            kth = radix_kth_in_bin(in, total, T, K - below, last ? 56 : 24, last ? 0ull : (uint64_t)T << 32, hist,
-                                   &sv[2], &sv[3]);
+            kth = radix_kth_in_bin(each, T, K - below, last ? 56 : 24, last ? 0ull : (uint64_t)T << 32, hist, &sv[2],
+                                   &sv[3]);
         }
     }
     __syncthreads();
