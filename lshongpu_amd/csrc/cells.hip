@@ -783,17 +783,36 @@ __global__ __launch_bounds__(256) void hist_threshold_kernel(const uint32_t* __r
     const uint16_t* __restrict__ row = D + (size_t)q * Pp;
     int32_t* __restrict__ cells = qcells2 + (size_t)q * P;
     const uint32_t n = n2[q];
+    // U blocks of 64 cells at a time: every list load, then every gather, is
+    // issued before the first use.  The list is compacted in place, in
+    // order: a write never lands past an entry already read.
+    constexpr int U = 4;
     uint32_t kept = 0;
-    for (uint32_t i0 = 0; i0 < n; i0 += 64) {
-        const uint32_t i = i0 + lane;
-        const int32_t j = i < n ? cells[i] : -1;
-        const bool keep = j >= 0 && lower_bound_to(row[j], cmin[j], cmax[j]) < h;
-        const uint64_t m = __ballot(keep);
-        if (keep) {
-            atomicAdd(&cnt2[j], 1u);
-            cells[kept + __popcll(m & ((1ull << lane) - 1ull))] = j;
+    for (uint32_t i0 = 0; i0 < n; i0 += 64 * U) {
+        int32_t j[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t i = i0 + 64 * u + lane;
+            j[u] = i < n ? cells[i] : -1;
         }
-        kept += (uint32_t)__popcll(m);
+        uint32_t dv[U], lo[U], hi[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int jj = j[u] >= 0 ? j[u] : 0;
+            dv[u] = row[jj];
+            lo[u] = cmin[jj];
+            hi[u] = cmax[jj];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const bool keep = j[u] >= 0 && lower_bound_to(dv[u], lo[u], hi[u]) < h;
+            const uint64_t m = __ballot(keep);
+            if (keep) {
+                atomicAdd(&cnt2[j[u]], 1u);
+                cells[kept + __popcll(m & ((1ull << lane) - 1ull))] = j[u];
+            }
+            kept += (uint32_t)__popcll(m);
+        }
     }
     if (lane == 0) n2[q] = kept;
 }
@@ -1072,6 +1091,7 @@ const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K
     LSH_REQUIRE(P >= 1 && P <= 256 * 32, LSH_ERR_INTERNAL, "cell_search: pivot count outside the threshold kernel's range");
     if (P <= 256 * 4) threshold(cell_threshold_kernel<4, 256>, 256);
     else if (P <= 256 * 8) threshold(cell_threshold_kernel<8, 256>, 256);
+    else if (P <= 256 * 12) threshold(cell_threshold_kernel<12, 256>, 256);
     else if (P <= 256 * 16) threshold(cell_threshold_kernel<16, 256>, 256);
     else threshold(cell_threshold_kernel<32, 256>, 256);
     check_launch("cell_threshold_kernel");
