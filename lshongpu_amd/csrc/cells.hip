@@ -458,9 +458,9 @@ __global__ __launch_bounds__(NT) void cell_threshold_kernel(const uint16_t* __re
 // The queries grouped by nearest cell (a counting sort by one workgroup,
 // counts in LDS): order[] lists the queries of cell 0, then of cell 1, ...;
 // the order inside a cell is not fixed.
-__global__ __launch_bounds__(1024) void order_by_cell_kernel(const uint32_t* __restrict__ nearest, int Q, int P,
-                                                             int32_t* __restrict__ order) {
-    extern __shared__ uint32_t cnt[];  // P
+// One 1024-thread workgroup; cnt is LDS of P words.
+__device__ void order_by_cell(const uint32_t* __restrict__ nearest, int Q, int P, int32_t* __restrict__ order,
+                              uint32_t* cnt) {
     __shared__ uint32_t ws[16];
     for (int j = threadIdx.x; j < P; j += 1024) cnt[j] = 0;
     __syncthreads();
@@ -505,14 +505,10 @@ __device__ inline uint32_t cell_parts(const uint32_t* ctile, int j) {
 // final total entry; choff[P+1] and *work_out = the tile products those
 // chunks can make (query groups x tiles, summed); *dense (may be null) = 1
 // when that exceeds dense_limit; cursor (= cnt for the placing pass) zeroed;
-// cmap[c] = the cell of chunk c.
-__global__ __launch_bounds__(1024) void cell_offsets_kernel(uint32_t* __restrict__ cnt, int P,
-                                                            const uint32_t* __restrict__ ctile,
-                                                            uint32_t* __restrict__ coff,
-                                                            uint32_t* __restrict__ choff,
-                                                            uint32_t* __restrict__ cmap,
-                                                            uint32_t* __restrict__ work_out,
-                                                            uint32_t* __restrict__ dense, uint32_t dense_limit) {
+// cmap[c] = the cell of chunk c.  By one 1024-thread workgroup.
+__device__ void cell_offsets(uint32_t* __restrict__ cnt, int P, const uint32_t* __restrict__ ctile,
+                             uint32_t* __restrict__ coff, uint32_t* __restrict__ choff, uint32_t* __restrict__ cmap,
+                             uint32_t* __restrict__ work_out, uint32_t* __restrict__ dense, uint32_t dense_limit) {
     __shared__ uint32_t ws[3][16];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint32_t carry_a = 0, carry_b = 0, carry_w = 0;
@@ -567,6 +563,32 @@ __global__ __launch_bounds__(1024) void cell_offsets_kernel(uint32_t* __restrict
         *work_out = carry_w;
         if (dense) *dense = carry_w > dense_limit ? 1u : 0u;
     }
+}
+
+__global__ __launch_bounds__(1024) void cell_offsets_kernel(uint32_t* __restrict__ cnt, int P,
+                                                            const uint32_t* __restrict__ ctile,
+                                                            uint32_t* __restrict__ coff,
+                                                            uint32_t* __restrict__ choff,
+                                                            uint32_t* __restrict__ cmap,
+                                                            uint32_t* __restrict__ work_out,
+                                                            uint32_t* __restrict__ dense, uint32_t dense_limit) {
+    cell_offsets(cnt, P, ctile, coff, choff, cmap, work_out, dense, dense_limit);
+}
+
+// The first pass's offsets (workgroup 0) and, in the same launch, the query
+// order by nearest cell (workgroup 1, when launched): both need only the
+// threshold kernel's output.
+__global__ __launch_bounds__(1024) void offsets_and_order_kernel(uint32_t* __restrict__ cnt, int P,
+                                                                 const uint32_t* __restrict__ ctile,
+                                                                 uint32_t* __restrict__ coff,
+                                                                 uint32_t* __restrict__ choff,
+                                                                 uint32_t* __restrict__ cmap,
+                                                                 uint32_t* __restrict__ work_out,
+                                                                 const uint32_t* __restrict__ nearest, int Q,
+                                                                 int32_t* __restrict__ order) {
+    extern __shared__ uint32_t ocnt[];  // P, for the order
+    if (blockIdx.x == 0) cell_offsets(cnt, P, ctile, coff, choff, cmap, work_out, nullptr, 0u);
+    else order_by_cell(nearest, Q, P, order, ocnt);
 }
 
 // Coarse distance histogram of the exact first pass: bin = distance / 8,
@@ -760,6 +782,9 @@ __global__ __launch_bounds__(256) void cell_scan_kernel(const uint32_t* __restri
 // query's candidate cells (qcells2) whose bound is below the final threshold
 // are kept (compacted in place, n2 updated) and counted into cnt2 for the
 // second pass.  One wave per query.
+// One wave per query: the threshold from the first pass's histogram, then
+// the query's second-pass cell list compacted to the cells whose bound is
+// below it, counted into cnt2.
 __global__ __launch_bounds__(256) void hist_threshold_kernel(const uint32_t* __restrict__ hist, int Q, int bits,
                                                              int K, uint64_t* __restrict__ thr,
                                                              const uint16_t* __restrict__ D, int P, int Pp,
@@ -892,12 +917,6 @@ void cell_scan(const CellIndex& c, unsigned grid, const uint32_t* coff, const ui
     }
 }
 
-void list_offsets(const CellIndex& c, uint32_t* cnt, uint32_t* coff, uint32_t* choff, uint32_t* cmap,
-                  uint32_t* work_out, uint32_t* dense, uint32_t dense_limit, hipStream_t s) {
-    hipLaunchKernelGGL(cell_offsets_kernel, dim3(1), dim3(1024), 0, s, cnt, c.P, c.ctile.get(), coff, choff, cmap,
-                       work_out, dense, dense_limit);
-    check_launch("cell_offsets_kernel");
-}
 
 
 }  // namespace
@@ -1091,21 +1110,21 @@ const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K
     LSH_REQUIRE(P >= 1 && P <= 256 * 32, LSH_ERR_INTERNAL, "cell_search: pivot count outside the threshold kernel's range");
     if (P <= 256 * 4) threshold(cell_threshold_kernel<4, 256>, 256);
     else if (P <= 256 * 8) threshold(cell_threshold_kernel<8, 256>, 256);
+    else if (P <= 256 * 10) threshold(cell_threshold_kernel<10, 256>, 256);
     else if (P <= 256 * 12) threshold(cell_threshold_kernel<12, 256>, 256);
     else if (P <= 256 * 16) threshold(cell_threshold_kernel<16, 256>, 256);
     else threshold(cell_threshold_kernel<32, 256>, 256);
     check_launch("cell_threshold_kernel");
-    if (qorder) {
-        // Queries grouped by nearest cell (a counting sort): neighbouring
-        // queries then share most candidates, which the rerank can reuse.
-        hipLaunchKernelGGL(order_by_cell_kernel, dim3(1), dim3(1024), sizeof(uint32_t) * (size_t)P, s, nearest, Q,
-                           P, qorder);
-        check_launch("order_by_cell_kernel");
-    }
     // First pass: exact distances over the nearest cells, K-th by histogram,
-    // then the second pass's cell counts under the final threshold.
+    // then the second pass's cell counts under the final threshold.  Its
+    // offsets are computed in the launch that also orders the queries by
+    // nearest cell (a counting sort): neighbouring queries then share most
+    // candidates, which the rerank can reuse.
     const unsigned qwaves = (unsigned)((Q + 3) / 4);
-    list_offsets(c, cnt1, coff, choff, cmap, stat + 1, nullptr, 0, s);
+    hipLaunchKernelGGL(offsets_and_order_kernel, dim3(qorder ? 2 : 1), dim3(1024),
+                       qorder ? sizeof(uint32_t) * (size_t)P : 0, s, cnt1, P, c.ctile.get(), coff, choff, cmap,
+                       stat + 1, nearest, Q, qorder);
+    check_launch("offsets_and_order_kernel");
     hipLaunchKernelGGL(place_from_cells_kernel, dim3(qwaves), dim3(256), 0, s, qcells1, n1, Q, P, cnt1, coff, list,
                        nullptr);
     check_launch("place_from_cells_kernel");
@@ -1114,7 +1133,9 @@ const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K
                        c.cmin.get(), c.cmax.get(), cnt2, qcells2, n2);
     check_launch("hist_threshold_kernel");
     // Second pass: the tiles the bounds leave (skipped when too dense).
-    list_offsets(c, cnt2, coff, choff, cmap, stat + 2, stat, dense_limit, s);
+    hipLaunchKernelGGL(cell_offsets_kernel, dim3(1), dim3(1024), 0, s, cnt2, P, c.ctile.get(), coff, choff, cmap,
+                       stat + 2, stat, dense_limit);
+    check_launch("cell_offsets_kernel");
     hipLaunchKernelGGL(place_from_cells_kernel, dim3(qwaves), dim3(256), 0, s, qcells2, n2, Q, P, cnt2, coff, list,
                        stat);
     check_launch("place_from_cells_kernel");
