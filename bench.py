@@ -102,7 +102,11 @@ def main() -> int:
     queries = torch.from_numpy(queries_h).to(dev)
     Q, d, k = queries.shape[0], data.shape[1], args.k
     kq = lsh.calculateK(k)
-    stream = torch.cuda.current_stream()
+    # A stream of its own (default priority): a search queued on a given
+    # stream with device results returns while its last kernels run, so the
+    # host queues the next step meanwhile.  (The default stream's handle is
+    # 0, which the library reads as "its own stream, wait for the results".)
+    stream = torch.cuda.Stream(device=dev)
 
     # ---- index build (constructionTime of the reference; not in the step)
     torch.cuda.synchronize()
@@ -274,7 +278,7 @@ def other_config(args, np, torch, lsh, synth, dev) -> int:
     data = torch.from_numpy(data_h).to(dev)
     queries = torch.from_numpy(queries_h).to(dev)
     Q, d = queries.shape
-    stream = torch.cuda.current_stream()
+    stream = torch.cuda.Stream(device=dev)
     if cfg == 0:
         workload = "exact angular scan, N=100k d=100, 1,000 queries, k=32"
         kw = dict(implementation=2, distanceFunc=1)

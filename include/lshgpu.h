@@ -309,7 +309,9 @@ int lsh_index_create(const lsh_index_config_t* cfg, const float* data,
  * stream: a hipStream_t (NULL = the index's own stream).  With device results
  * on a given stream the results are complete in that stream's order (the
  * call may return while the last kernels run); otherwise the call returns
- * when the results are complete. */
+ * when the results are complete.  Device queries may be read in place and
+ * device results are written in place, so neither buffer may be touched by
+ * other work until the search is complete. */
 int lsh_index_search(lsh_index_t* index, const float* queries, int N_query,
                      int k, int queries_on_device, lsh_point_t* results,
                      int results_on_device, void* stream);

@@ -713,12 +713,24 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
     if (Q == 0) return;
     pt.begin(PH_TOTAL, s);
     pt.begin(PH_PREP, s);
-    q.ensure((size_t)Q * d);
-    LSH_HIP(hipMemcpyAsync(q.get(), queries, sizeof(float) * (size_t)Q * d,
-                           q_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
-    if (framework == 0) apply_steps_query(sketch_steps, q.get(), Q, d, s);
+    // Device queries that need no transform are read where they are; device
+    // results are written in place.
+    if (q_on_device && framework == 0 && sketch_steps.empty()) {
+        qp = queries;
+    } else {
+        q.ensure((size_t)Q * d);
+        LSH_HIP(hipMemcpyAsync(q.get(), queries, sizeof(float) * (size_t)Q * d,
+                               q_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+        if (framework == 0) apply_steps_query(sketch_steps, q.get(), Q, d, s);
+        qp = q.get();
+    }
     pt.end(PH_PREP, s);
-    out.ensure((size_t)Q * kq);
+    if (r_on_device) {
+        outp = results;
+    } else {
+        out.ensure((size_t)Q * kq);
+        outp = out.get();
+    }
     flags.ensure((size_t)Q);
 
     if (framework == 1) {
@@ -728,10 +740,10 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
         LSH_REQUIRE(kq <= kMaxTopK, LSH_ERR_UNSUPPORTED, "exact scan supports k <= 4096");
         int K = std::min(kMaxTopK, pow2_ceil(kq + margin));
         const double eps = fp32_error_bound(kind, d);
-        exact_topk(data.get(), data_inv.get(), N, d, kind, ExactQueries{q.get(), Q}, K, kNormal, st, scratch, 0, s,
+        exact_topk(data.get(), data_inv.get(), N, d, kind, ExactQueries{qp, Q}, K, kNormal, st, scratch, 0, s,
                    pt, scan_launches);
         pt.begin(PH_FINAL, s);
-        launch_exact_finalize(data.get(), q.get(), d, kind, st.view(), kq, eps, out.get(), flags.get(), s);
+        launch_exact_finalize(data.get(), qp, d, kind, st.view(), kq, eps, outp, flags.get(), s);
         pt.end(PH_FINAL, s);
         std::vector<int32_t> bad = flagged(flags.get(), Q, s);
         if (!bad.empty()) {
@@ -743,7 +755,7 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
             DevBuf<int32_t> idx((size_t)m);
             LSH_HIP(hipMemcpyAsync(idx.get(), bad.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, s));
             DevBuf<float> sub((size_t)m * d);
-            launch_gather_rows_f32(q.get(), d, idx.get(), m, sub.get(), s);
+            launch_gather_rows_f32(qp, d, idx.get(), m, sub.get(), s);
             TopKState sst;
             exact_topk(data.get(), data_inv.get(), N, d, kind, ExactQueries{sub.get(), m}, kMaxTopK, kNormal, sst,
                        scratch, 0, s, pt, scan_launches);
@@ -752,16 +764,22 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
             launch_exact_finalize(data.get(), sub.get(), d, kind, sst.view(), kq, eps, sout.get(), sflags.get(), s);
             LSH_REQUIRE(flagged(sflags.get(), m, s).empty(), LSH_ERR_UNSUPPORTED,
                         "more than 4096 points within the FP32 error band of the k-th distance");
-            launch_scatter_rows_pts(sout.get(), kq, idx.get(), m, out.get(), s);
+            launch_scatter_rows_pts(sout.get(), kq, idx.get(), m, outp, s);
             LSH_HIP(hipStreamSynchronize(s));
         }
     } else if (impl >= 3 && impl <= 5) {
         const int Qpad = sketch_query_rows(Q);
         qsk.ensure((size_t)Qpad * wpad);
         pt.begin(PH_PREP, s);
-        LSH_HIP(hipMemsetAsync(qsk.get(), 0, sizeof(uint32_t) * (size_t)Qpad * wpad, s));
-        if (impl == 3) launch_simhash(q.get(), Q, d, planesT.get(), words, 32, qsk.get(), wpad, s);
-        else wmh.sketch(q.get(), Q, d, qsk.get(), wpad, s);
+        // Rows past Q and words past the sketch stay zero: the sketch writes
+        // neither, so the buffer is cleared only when it or Q changes.
+        if (qsk.get() != qsk_zeroed || Q != qsk_zeroed_q) {
+            LSH_HIP(hipMemsetAsync(qsk.get(), 0, sizeof(uint32_t) * (size_t)Qpad * wpad, s));
+            qsk_zeroed = qsk.get();
+            qsk_zeroed_q = Q;
+        }
+        if (impl == 3) launch_simhash(qp, Q, d, planesT.get(), words, 32, qsk.get(), wpad, s);
+        else wmh.sketch(qp, Q, d, qsk.get(), wpad, s);
         pt.end(PH_PREP, s);
         const int Ceff = (int)std::max<int64_t>(1, std::min<int64_t>(candidates, N));
         const int K = Ceff <= kMaxIntTopK ? Ceff : pow2_ceil(Ceff);
@@ -776,17 +794,17 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
             // recheck when it splits.
             const bool split =
                 cells.built()
-                    ? launch_rerank(cdata.get(), q.get(), d, kind, st.view().keys, st.view().stride, nullptr, Ceff, m,
-                                    kq, chalf.get() ? half_error_bound(d) : fp32_error_bound(kind, d), out.get(), s,
+                    ? launch_rerank(cdata.get(), qp, d, kind, st.view().keys, st.view().stride, nullptr, Ceff, m,
+                                    kq, chalf.get() ? half_error_bound(d) : fp32_error_bound(kind, d), outp, s,
                                     order, cpos.get(), chalf.get(), &rband, false)
-                    : launch_rerank(data.get(), q.get(), d, kind, st.view().keys, st.view().stride, nullptr, Ceff, m,
-                                    kq, fp32_error_bound(kind, d), out.get(), s, order, nullptr, nullptr, &rband,
+                    : launch_rerank(data.get(), qp, d, kind, st.view().keys, st.view().stride, nullptr, Ceff, m,
+                                    kq, fp32_error_bound(kind, d), outp, s, order, nullptr, nullptr, &rband,
                                     false);
             pt.end(PH_FINAL, s);
             if (split) {
                 pt.begin(PH_RECHECK, s);
-                launch_rerank_recheck(cells.built() ? cdata.get() : data.get(), q.get(), d, kind, Ceff, m, kq,
-                                      out.get(), s, order, rband);
+                launch_rerank_recheck(cells.built() ? cdata.get() : data.get(), qp, d, kind, Ceff, m, kq,
+                                      outp, s, order, rband);
                 pt.end(PH_RECHECK, s);
             }
         };
@@ -812,7 +830,7 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
     } else {
         qjl.ensure((size_t)Q * sdim);
         pt.begin(PH_PREP, s);
-        launch_jl(q.get(), Q, d, planesT.get(), sdim, qjl.get(), s);
+        launch_jl(qp, Q, d, planesT.get(), sdim, qjl.get(), s);
         pt.end(PH_PREP, s);
         const int Ceff = (int)std::min<int64_t>(candidates, N);
         const int K = pow2_ceil(std::max(Ceff, 1));
@@ -820,13 +838,13 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
         exact_topk(djl.get(), nullptr, N, sdim, kSqL2, ExactQueries{qjl.get(), Q}, K, kNormal, st, scratch, 0, s,
                    pt, scan_launches);
         pt.begin(PH_FINAL, s);
-        launch_rerank(data.get(), q.get(), d, kAngular, st.view().keys, st.view().stride, nullptr, Ceff, Q, kq,
-                      fp32_error_bound(kAngular, d), out.get(), s, nullptr, nullptr, nullptr, &rband);
+        launch_rerank(data.get(), qp, d, kAngular, st.view().keys, st.view().stride, nullptr, Ceff, Q, kq,
+                      fp32_error_bound(kAngular, d), outp, s, nullptr, nullptr, nullptr, &rband);
         pt.end(PH_FINAL, s);
     }
 
-    LSH_HIP(hipMemcpyAsync(results, out.get(), sizeof(lsh_point_t) * (size_t)Q * kq,
-                           r_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    if (!r_on_device)
+        LSH_HIP(hipMemcpyAsync(results, outp, sizeof(lsh_point_t) * (size_t)Q * kq, hipMemcpyDeviceToHost, s));
     pt.end(PH_TOTAL, s);
     // Device results on the caller's stream are ordered on it; otherwise the
     // call returns with the results complete.
@@ -903,7 +921,7 @@ void lsh_index::search_lsh(int Q, int kq, hipStream_t s, PhaseTimer& pt) {
         lsh_topk(a, none, maxS, K, kNormal, int_keys && K <= kMaxIntTopK, st, scratch, 0, s, pt, scan_launches);
         pt.begin(PH_FINAL, s);
         launch_rerank(data.get(), q.get(), d, kind, st.view().keys, st.view().stride, nullptr, Ceff, Q, kq, eps,
-                      out.get(), s, nullptr, nullptr, nullptr, &rband);
+                      outp, s, nullptr, nullptr, nullptr, &rband);
         pt.end(PH_FINAL, s);
         return;
     }
@@ -913,7 +931,7 @@ void lsh_index::search_lsh(int Q, int kq, hipStream_t s, PhaseTimer& pt) {
     const int K = std::min(kMaxTopK, pow2_ceil(kq + margin));
     lsh_topk(a, none, maxS, K, kNormal, false, st, scratch, 0, s, pt, scan_launches);
     pt.begin(PH_FINAL, s);
-    launch_exact_finalize(data.get(), q.get(), d, kind, st.view(), kq, eps, out.get(), flags.get(), s);
+    launch_exact_finalize(data.get(), q.get(), d, kind, st.view(), kq, eps, outp, flags.get(), s);
     pt.end(PH_FINAL, s);
     std::vector<int32_t> bad = flagged(flags.get(), Q, s);
     if (bad.empty()) return;
@@ -934,7 +952,7 @@ void lsh_index::search_lsh(int Q, int kq, hipStream_t s, PhaseTimer& pt) {
     launch_exact_finalize(data.get(), sub.get(), d, kind, sst.view(), kq, eps, sout.get(), sflags.get(), s);
     LSH_REQUIRE(flagged(sflags.get(), m, s).empty(), LSH_ERR_UNSUPPORTED,
                 "more than 4096 candidates within the FP32 error band of the k-th distance");
-    launch_scatter_rows_pts(sout.get(), kq, idx.get(), m, out.get(), s);
+    launch_scatter_rows_pts(sout.get(), kq, idx.get(), m, outp, s);
     LSH_HIP(hipStreamSynchronize(s));
 }
 

@@ -211,7 +211,13 @@ struct lsh_index {
     // Per-search scratch (grow-only).
     lshgpu::DevBuf<float> q, qT, q_inv, qjl, qjlT;
     lshgpu::DevBuf<uint32_t> qsk;
+    const uint32_t* qsk_zeroed = nullptr;  // qsk was cleared at this address for qsk_zeroed_q queries
+    int qsk_zeroed_q = -1;
     lshgpu::DevBuf<lsh_point_t> out;
+    // The current search's queries (the caller's device buffer, or q) and
+    // results (the caller's device buffer, or out).
+    const float* qp = nullptr;
+    lsh_point_t* outp = nullptr;
     lshgpu::DevBuf<uint32_t> flags;
     lshgpu::DevBuf<int32_t> qorder;  // queries grouped by nearest cell (rerank order)
     lshgpu::TopKState st;

@@ -467,6 +467,35 @@ def test_index_searches_on_streams_are_ordered(lsh):
     assert pt["final_ms"] > 0 and pt["scan_ms"] == 0 and pt["recheck_ms"] == 0
 
 
+@pytest.mark.parametrize("kw", [dict(implementation=1), dict(implementation=2, distanceFunc=2),
+                                dict(implementation=3, sketchDim=16), dict(implementation=4, sketchDim=64),
+                                dict(implementation=6, sketchDim=32),
+                                dict(framework=1, implementation=2, keysImplementation=3, bucketKeyBits=6,
+                                     tables=4)])
+def test_index_device_buffers_match_host(lsh, kw):
+    # Device queries are read in place when no transform applies (and copied
+    # otherwise), device results are written in place: both give what host
+    # buffers give, the queries are left unchanged, and a smaller batch
+    # between two full ones (the query-sketch buffer keeps zero padding rows)
+    # changes nothing.
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(31)
+    N, Q, d = 5000, 96, 24
+    data = np.abs(rng.standard_normal((N, d))).astype(np.float32)
+    queries = np.abs(rng.standard_normal((Q, d))).astype(np.float32)
+    dq = torch.from_numpy(queries).cuda()
+    s = torch.cuda.Stream()
+    with lsh.Index(data, **kw) as ix:
+        for nq in (Q, 37, Q):
+            want = ix.search(queries[:nq], 32)
+            out = torch.full((nq, 64), -7, dtype=torch.int32, device="cuda")
+            torch.cuda.synchronize()
+            ix.search(dq[:nq], 32, out=out, stream=s.cuda_stream)
+            s.synchronize()
+            np.testing.assert_array_equal(out.cpu().numpy().view(lsh.POINT_DTYPE).reshape(nq, 32), want)
+    np.testing.assert_array_equal(dq.cpu().numpy(), queries)
+
+
 # ------------------------------------------------------------ weighted MinHash
 
 def _wmh_rows(rng, n, d, density):
