@@ -224,14 +224,14 @@ void sketch_topk(const SketchSet& d, const SketchSet* samples, int nsamples, int
             if (mfma_points) *mfma_points += d.N;
         };
         auto select_and_flag = [&](const uint32_t* extra, uint32_t* h_extra, int n_extra) {
+            // The select writes the flags (and extra words) to pinned memory
+            // ahead of the hook's work, so the host reads them while that
+            // work runs.
+            uint32_t* hb = sc.pinned((size_t)Q + n_extra);
             pt.begin(PH_SELECT, s);
-            launch_select_merge(st.view(), s);
+            launch_select_merge(st.view(), s, hb, extra, extra ? n_extra : 0);
             st.selected();
             pt.end(PH_SELECT, s);
-            // The flags (and extra words) are written to pinned memory ahead
-            // of the hook's work, so the host reads them while that work runs.
-            uint32_t* hb = sc.pinned((size_t)Q + n_extra);
-            launch_incomplete_flags(st.view(), hb, s, extra, extra ? n_extra : 0);
             const hipEvent_t ev = sc.event();
             LSH_HIP(hipEventRecord(ev, s));
             if (d.hook && d.hook->fn) {

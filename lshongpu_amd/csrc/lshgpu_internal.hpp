@@ -214,7 +214,11 @@ int newcap_for(int K);
 // (defined in the .hip files; all launch on `s` and do not synchronise)
 
 // select.hip
-void launch_select_merge(const TopKView& v, hipStream_t s);
+// flags (may be null, may be pinned host memory): after the select,
+// flags[q] = 1 for each query whose list is incomplete, and flags[Q + i] =
+// extra[i] for i < n_extra (n_extra <= 256).
+void launch_select_merge(const TopKView& v, hipStream_t s, uint32_t* flags = nullptr,
+                         const uint32_t* extra = nullptr, int n_extra = 0);
 void launch_sort_rows(const TopKView& v, hipStream_t s);  // orders the best list of every query
 void launch_count_overflow(const TopKView& v, uint32_t* out_count, hipStream_t s);
 // dst.thr[q] = (integer distance of src.thr[q]) + 1 with ID 0: a scan with it

@@ -82,16 +82,25 @@ struct KeysInRegisters {
     }
 };
 
+//
+// flags (may be null, may be pinned host memory): flags[q] = 1 when query q's
+// list is incomplete after the select (overflowed or fewer than K keys),
+// flags[Q + i] = extra[i] for i < n_extra.
 template <int RK>
-__global__ __launch_bounds__(kSelThreads) void select_hist_kernel(TopKView v) {
+__global__ __launch_bounds__(kSelThreads) void select_hist_kernel(TopKView v, uint32_t* flags,
+                                                                  const uint32_t* extra, int n_extra) {
     extern __shared__ uint64_t kept[];  // K
     __shared__ uint32_t hist[kSelBins];
     __shared__ uint64_t ties[kTieCap];
     __shared__ uint32_t sv[kSelScratch];
     __shared__ uint32_t s_nkeep;
     const int q = blockIdx.x;
+    if (flags && blockIdx.x == 0 && (int)threadIdx.x < n_extra) flags[v.Q + threadIdx.x] = extra[threadIdx.x];
     const uint32_t n_raw = v.cnt[q];
-    if (n_raw == 0 && !v.fresh) return;
+    if (n_raw == 0 && !v.fresh) {
+        if (flags && threadIdx.x == 0) flags[q] = (v.ovf[q] != 0u || v.thr[q] == kEmptyKey) ? 1u : 0u;
+        return;
+    }
     const int K = v.K;
     const int n = (int)min(n_raw, (uint32_t)v.newcap);
     uint64_t* row = v.keys + (size_t)q * v.stride;
@@ -121,6 +130,7 @@ __global__ __launch_bounds__(kSelThreads) void select_hist_kernel(TopKView v) {
         v.thr[q] = nkeep == (uint32_t)K ? kth : kEmptyKey;
         v.cnt[q] = 0;
         if (n_raw > (uint32_t)v.newcap) v.ovf[q] = 1;
+        if (flags) flags[q] = (v.ovf[q] != 0u || nkeep != (uint32_t)K) ? 1u : 0u;
     }
 }
 
@@ -218,25 +228,32 @@ void TopKState::reset(int Q, int K, int newcap, hipStream_t s, bool int_keys, bo
     check_launch("top-K state reset");
 }
 
-void launch_select_merge(const TopKView& v, hipStream_t s) {
-    if (v.int_keys) {
+void launch_select_merge(const TopKView& v, hipStream_t s, uint32_t* flags, const uint32_t* extra, int n_extra) {
+    LSH_REQUIRE(n_extra >= 0 && n_extra <= kSelThreads && (n_extra == 0 || extra), LSH_ERR_INTERNAL,
+                "select: bad extra words");
+    if (v.int_keys && v.Q > 0) {
         const size_t lds = (size_t)v.K * sizeof(uint64_t);
         const int most = v.fresh ? v.newcap : v.K + v.newcap;  // keys a query can hold
         constexpr int RK = 12;
         if (most <= kSelThreads * RK) {
             set_dynamic_lds((const void*)select_hist_kernel<RK>, lds + sizeof(uint32_t) * (kSelBins + 8));
-            hipLaunchKernelGGL(select_hist_kernel<RK>, dim3(v.Q), dim3(kSelThreads), lds, s, v);
+            hipLaunchKernelGGL(select_hist_kernel<RK>, dim3(v.Q), dim3(kSelThreads), lds, s, v, flags, extra,
+                               n_extra);
         } else {
             set_dynamic_lds((const void*)select_hist_kernel<0>, lds + sizeof(uint32_t) * (kSelBins + 8));
-            hipLaunchKernelGGL(select_hist_kernel<0>, dim3(v.Q), dim3(kSelThreads), lds, s, v);
+            hipLaunchKernelGGL(select_hist_kernel<0>, dim3(v.Q), dim3(kSelThreads), lds, s, v, flags, extra,
+                               n_extra);
         }
         check_launch("select_hist_kernel");
         return;
     }
-    const size_t lds = (size_t)(v.K + pow2_ceil(std::max(v.newcap, v.K))) * sizeof(uint64_t);
-    set_dynamic_lds((const void*)select_merge_kernel, lds);
-    hipLaunchKernelGGL(select_merge_kernel, dim3(v.Q), dim3(kSelThreads), lds, s, v);
-    check_launch("select_merge_kernel");
+    if (v.Q > 0) {
+        const size_t lds = (size_t)(v.K + pow2_ceil(std::max(v.newcap, v.K))) * sizeof(uint64_t);
+        set_dynamic_lds((const void*)select_merge_kernel, lds);
+        hipLaunchKernelGGL(select_merge_kernel, dim3(v.Q), dim3(kSelThreads), lds, s, v);
+        check_launch("select_merge_kernel");
+    }
+    if (flags) launch_incomplete_flags(v, flags, s, extra, n_extra);
 }
 
 void launch_sort_rows(const TopKView& v, hipStream_t s) {
