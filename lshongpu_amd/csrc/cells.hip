@@ -109,10 +109,20 @@ template <int KS>
 __global__ __launch_bounds__(256) void pivot_matrix_kernel(const uint32_t* __restrict__ qs, int Q,
                                                            const uint32_t* __restrict__ pk, int P, int Pp,
                                                            uint16_t* __restrict__ D, uint32_t* __restrict__ zero,
-                                                           int nzero) {
-    // The first workgroup also zeroes zero[0, nzero) (a search's counters).
+                                                           int nzero, TopKView tv) {
+    // The first workgroup also zeroes zero[0, nzero) (a search's counters),
+    // and the whole grid clears the fresh top-K view tv when it has one (tv.Q
+    // > 0: thresholds empty, counts and overflow flags zero).
     if (blockIdx.x == 0 && blockIdx.y == 0)
         for (int i = threadIdx.x; i < nzero; i += blockDim.x) zero[i] = 0u;
+    {
+        const int nthreads = gridDim.x * gridDim.y * blockDim.x;
+        for (int i = (blockIdx.y * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x; i < tv.Q; i += nthreads) {
+            tv.thr[i] = kEmptyKey;
+            tv.cnt[i] = 0u;
+            tv.ovf[i] = 0u;
+        }
+    }
     constexpr int W = 2 * KS;
     constexpr int BITS = 32 * W;
     constexpr int NV = KS / 4;  // uint4 loads per lane per pivot tile
@@ -863,7 +873,7 @@ __global__ __launch_bounds__(256) void place_from_cells_kernel(const int32_t* __
 
 template <int KS>
 void launch_pivots_ks(const uint32_t* rows, int64_t n, const uint32_t* pfrag, const uint32_t* pk, int P, int Pp,
-                      uint32_t* best, uint16_t* D, uint32_t* zero, int nzero, hipStream_t s) {
+                      uint32_t* best, uint16_t* D, uint32_t* zero, int nzero, const TopKView* tv, hipStream_t s) {
     if (best) {
         const v4i* f = reinterpret_cast<const v4i*>(pfrag);
         hipLaunchKernelGGL((pivot_assign_kernel<KS>), dim3(blocks_for((size_t)((n + 31) / 32), 4)), dim3(256), 0, s,
@@ -872,21 +882,23 @@ void launch_pivots_ks(const uint32_t* rows, int64_t n, const uint32_t* pfrag, co
     } else {
         const unsigned gy = (unsigned)(((P + 31) / 32 + kGroup - 1) / kGroup);
         hipLaunchKernelGGL((pivot_matrix_kernel<KS>), dim3(blocks_for((size_t)((n + 31) / 32), 4), gy), dim3(256), 0,
-                           s, rows, (int)n, pk, P, Pp, D, zero, nzero);
+                           s, rows, (int)n, pk, P, Pp, D, zero, nzero, tv ? *tv : TopKView{});
         check_launch("pivot_matrix_kernel");
     }
 }
 
 // best != nullptr: nearest pivot of each row (from the FP4 image pfrag);
 // otherwise the distance matrix D (from the packed pivot sketches pk), whose
-// launch also zeroes zero[0, nzero).
+// launch also zeroes zero[0, nzero) and clears the fresh view *tv (when
+// given).
 void pivots(int words, const uint32_t* rows, int64_t n, const uint32_t* pfrag, const uint32_t* pk, int P, int Pp,
-            uint32_t* best, uint16_t* D, hipStream_t s, uint32_t* zero = nullptr, int nzero = 0) {
+            uint32_t* best, uint16_t* D, hipStream_t s, uint32_t* zero = nullptr, int nzero = 0,
+            const TopKView* tv = nullptr) {
     switch (words) {
-        case 8: launch_pivots_ks<4>(rows, n, pfrag, pk, P, Pp, best, D, zero, nzero, s); break;
-        case 16: launch_pivots_ks<8>(rows, n, pfrag, pk, P, Pp, best, D, zero, nzero, s); break;
-        case 24: launch_pivots_ks<12>(rows, n, pfrag, pk, P, Pp, best, D, zero, nzero, s); break;
-        case 32: launch_pivots_ks<16>(rows, n, pfrag, pk, P, Pp, best, D, zero, nzero, s); break;
+        case 8: launch_pivots_ks<4>(rows, n, pfrag, pk, P, Pp, best, D, zero, nzero, tv, s); break;
+        case 16: launch_pivots_ks<8>(rows, n, pfrag, pk, P, Pp, best, D, zero, nzero, tv, s); break;
+        case 24: launch_pivots_ks<12>(rows, n, pfrag, pk, P, Pp, best, D, zero, nzero, tv, s); break;
+        case 32: launch_pivots_ks<16>(rows, n, pfrag, pk, P, Pp, best, D, zero, nzero, tv, s); break;
         default: fail(LSH_ERR_INTERNAL, "pivot cells need 8, 16, 24 or 32 sketch words");
     }
 }
@@ -1100,7 +1112,8 @@ const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K
     // of the tiles of a full pass (query tiles x point tiles).
     const uint32_t dense_limit = (uint32_t)std::min<double>(4e9, 0.5 * (double)qgroups * (double)c.ntiles);
 
-    pivots(c.words, qs, Q, c.pfp4.get(), c.pk.get(), P, Pp, nullptr, D, s, meta, nzero);
+    LSH_REQUIRE(v.fresh && v.Q == Q, LSH_ERR_INTERNAL, "cell_search takes a fresh top-K view of its queries");
+    pivots(c.words, qs, Q, c.pfp4.get(), c.pk.get(), P, Pp, nullptr, D, s, meta, nzero, &v);
     // Threshold bound from the tile ranges, the first-pass radii and their
     // cell counts, and the nearest cell of each query.
     auto threshold = [&](auto kernel, int nt) {

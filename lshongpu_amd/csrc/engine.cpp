@@ -246,10 +246,12 @@ void sketch_topk(const SketchSet& d, const SketchSet* samples, int nsamples, int
             return bad;
         };
         // One select follows each reset here, so the best lists need no
-        // clearing (lazy reset).
-        st.reset(Q, K, 2 * K + 512, s, true, true);
+        // clearing (lazy reset).  The cell search clears the view in its
+        // first launch.
+        const bool cells_path = d.cells && !force_full_pass();
+        st.reset(Q, K, 2 * K + 512, s, true, true, !cells_path);
         const uint32_t* cstat = nullptr;
-        if (d.cells && !force_full_pass()) {
+        if (cells_path) {
             // Threshold from the nearest pivot cells, then the cells the
             // triangle inequality leaves.
             pt.begin(PH_SCAN, s);
@@ -259,7 +261,10 @@ void sketch_topk(const SketchSet& d, const SketchSet* samples, int nsamples, int
             pt.end(PH_SCAN, s);
             if (cstat && d.ordered) *d.ordered = d.qorder != nullptr;
         }
-        if (!cstat) sampled_pass();
+        if (!cstat) {
+            if (cells_path) st.clear(s);  // the cell search ran nothing
+            sampled_pass();
+        }
         uint32_t hstat[3] = {0, 0, 0};
         std::vector<int32_t> bad = select_and_flag(cstat, hstat, cstat ? 3 : 0);
         if (cstat) {

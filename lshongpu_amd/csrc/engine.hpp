@@ -133,8 +133,9 @@ struct CellIndex {
 void cell_rows(const CellIndex& c, const float* data, int64_t N, int d, DevBuf<float>& rows, DevBuf<int32_t>& pos,
                hipStream_t s);
 
-// Threshold search over the cells (Q query rows of qs, top-K): sets v.thr
-// to a valid threshold from each query's nearest cells, then appends every
+// Threshold search over the cells (Q query rows of qs, top-K): clears the
+// fresh view v (as TopKState::clear; v.Q == Q) in its first launch, sets
+// v.thr to a valid threshold from each query's nearest cells, then appends every
 // pair below it to v, as launch_hamming_mfma over all points would.  Nothing
 // waits on the host.  Returns a device array stat[3]: stat[0] = 1 when the
 // thresholds left too many tiles for pruning to pay and nothing was appended

@@ -187,7 +187,13 @@ class TopKState {
     // unordered set of exactly K keys and K need not be a power of two.
     // lazy (integer keys only): the best lists are not written; the next
     // select treats them as empty and must be followed by selected().
-    void reset(int Q, int K, int newcap, hipStream_t s, bool int_keys = false, bool lazy = false);
+    // launch = false: only the view is set up; the caller clears it (clear(),
+    // or a kernel of its own that does what clear() does).
+    void reset(int Q, int K, int newcap, hipStream_t s, bool int_keys = false, bool lazy = false,
+               bool launch = true);
+    // The reset's clearing: thresholds empty, counts and overflow flags zero,
+    // and the best lists empty unless the view is fresh.
+    void clear(hipStream_t s);
     TopKView view() const { return v_; }
     // After the select that follows a lazy reset: the best lists are stored.
     void selected() { v_.fresh = 0; }

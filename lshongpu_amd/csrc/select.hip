@@ -210,7 +210,7 @@ int pow2_ceil(int x) {
 
 int newcap_for(int K) { return K <= 1024 ? 2 * K + 256 : K + 512; }
 
-void TopKState::reset(int Q, int K, int newcap, hipStream_t s, bool int_keys, bool lazy) {
+void TopKState::reset(int Q, int K, int newcap, hipStream_t s, bool int_keys, bool lazy, bool launch) {
     LSH_REQUIRE(K >= 1 && K <= kMaxTopK && (int_keys || (K & (K - 1)) == 0), LSH_ERR_INVALID,
                 "top-K size must be a power of two <= 4096");
     LSH_REQUIRE(!lazy || (int_keys && K <= kMaxIntTopK), LSH_ERR_INTERNAL, "lazy reset needs the integer-key select");
@@ -221,10 +221,16 @@ void TopKState::reset(int Q, int K, int newcap, hipStream_t s, bool int_keys, bo
     ovf_.ensure((size_t)Q);
     v_ = TopKView{keys_.get(), cnt_.get(), thr_.get(), ovf_.get(), K, newcap, stride, Q, int_keys ? 1 : 0,
                   lazy ? 1 : 0};
+    if (launch) clear(s);
+}
+
+void TopKState::clear(hipStream_t s) {
     // A fill kernel rather than memsets keeps the sequence capturable in a
     // graph.  Only the best lists need clearing: the selects read the
     // appended slots up to each query's count.
-    hipLaunchKernelGGL(reset_state_kernel, dim3(grid_for(lazy ? (size_t)Q : (size_t)Q * K)), dim3(256), 0, s, v_);
+    if (v_.Q <= 0) return;
+    const size_t n = v_.fresh ? (size_t)v_.Q : (size_t)v_.Q * v_.K;
+    hipLaunchKernelGGL(reset_state_kernel, dim3(grid_for(n)), dim3(256), 0, s, v_);
     check_launch("top-K state reset");
 }
 
