@@ -369,6 +369,7 @@ __device__ inline int block_kth_bin(const uint32_t* hist, int nbins, uint32_t K,
 constexpr int kRadiusMargin = 16;
 constexpr int kWindow = 256;
 constexpr int kHistBins = 128;  // first-pass histogram bins per query (8 bits each)
+constexpr int kBoundBins = 512;  // bins of the D + max bound histogram (the last holds every larger bound)
 template <int CPT, int NT>
 __global__ __launch_bounds__(NT) void cell_threshold_kernel(const uint16_t* __restrict__ D, int Q, int P, int Pp,
                                                             const uint2* __restrict__ crec, int bits, int K,
@@ -380,11 +381,12 @@ __global__ __launch_bounds__(NT) void cell_threshold_kernel(const uint16_t* __re
                                                             uint32_t* __restrict__ n2,
                                                             uint32_t* __restrict__ hist) {
     __shared__ uint32_t h1[kWindow];
-    __shared__ uint32_t h2[2049];
+    __shared__ uint32_t h2[kBoundBins];
     __shared__ uint32_t ws[NT / 64];
     __shared__ int s_bin;
     __shared__ uint32_t best, s_n1, s_n2;
     const int q = blockIdx.x;
+    const int nb2 = min(2 * bits + 1, kBoundBins);  // bound histogram bins; larger bounds share the last
     const uint16_t* __restrict__ drow = D + (size_t)q * Pp;
     uint32_t dq[CPT], sz[CPT], rg[CPT];
 #pragma unroll
@@ -398,7 +400,7 @@ __global__ __launch_bounds__(NT) void cell_threshold_kernel(const uint16_t* __re
         sz[i] = in ? r.x : 0u;
         rg[i] = in ? r.y : 1u;  // min 1 > max 0: empty
     }
-    for (int b = threadIdx.x; b < 2049; b += NT) {
+    for (int b = threadIdx.x; b < kBoundBins; b += NT) {
         if (b < kWindow) h1[b] = 0;
         h2[b] = 0;
     }
@@ -441,10 +443,14 @@ __global__ __launch_bounds__(NT) void cell_threshold_kernel(const uint16_t* __re
             atomicAdd(&cnt1[j], 1u);
             q1[s1] = j;
         }
-        if (pass) atomicAdd(&h2[dq[i] + (rg[i] >> 16)], sz[i]);
+        if (pass) atomicAdd(&h2[min(dq[i] + (rg[i] >> 16), (uint32_t)(nb2 - 1))], sz[i]);
     }
     __syncthreads();
-    const int ub = block_kth_bin<NT>(h2, 2 * bits + 1, (uint32_t)K, ws, &s_bin);
+    int ub = block_kth_bin<NT>(h2, nb2, (uint32_t)K, ws, &s_bin);
+    // A K-th bound in the clamped last bin is only known to be at least its
+    // index: the loosest bound, 2 bits (every pair is within it), is used.
+    if (ub == nb2 - 1 && nb2 < 2 * bits + 1) ub = 2 * bits;
+    else if (ub == nb2) ub = 2 * bits + 1;
     if (threadIdx.x == 0) thr[q] = ub > 2 * bits ? kEmptyKey : make_key((uint32_t)ub + 1u, 0u);
     // The cells the second pass can need: the final threshold is at most this
     // one, so the cells whose bound is below it are a superset.
