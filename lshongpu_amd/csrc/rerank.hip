@@ -502,8 +502,8 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
     while (n < m) n <<= 1;
     if (n <= 64) {
         if (threadIdx.x < 64) {
-            uint64_t kk = threadIdx.x < m ? k64[threadIdx.x] : kEmptyKey;
-            int32_t ii = threadIdx.x < m ? ids[threadIdx.x] : 0x7FFFFFFF;
+            uint64_t kk = (int)threadIdx.x < m ? k64[threadIdx.x] : kEmptyKey;
+            int32_t ii = (int)threadIdx.x < m ? ids[threadIdx.x] : 0x7FFFFFFF;
             wave_sort_pairs(kk, ii);
             // n <= Cp: the first n lanes hold the m entries and padding.
             if ((int)threadIdx.x < n) {
