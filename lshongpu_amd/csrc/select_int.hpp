@@ -13,7 +13,7 @@
 
 namespace lshgpu {
 
-constexpr int kSelBins = 2048;
+constexpr int kSelBins = 1280;  // one bin per distance up to 1278 (1024-bit sketches); the last holds the rest
 constexpr int kTieCap = 512;      // keys of the boundary bin gathered in LDS (more: selected in place)
 constexpr int kSelScratch = 8;    // LDS words int_kth_key uses for its shared scalars
 
