@@ -402,6 +402,18 @@ def bucket_distribution_full_key(hashes, sketchDim: int | None = None):
 
 # ------------------------------------------------------------------ resident index
 
+def _check_tensor(t, name: str, dtype) -> None:
+    """A torch tensor handed to the library as a raw pointer must be what the
+    kernels assume: contiguous, of the right dtype, and (for 2-D operands) 2-D."""
+    if not t.is_contiguous():
+        raise LshError(1, f"{name} tensor must be contiguous")
+    if dtype is not None:
+        if str(t.dtype) != f"torch.{dtype}":
+            raise LshError(1, f"{name} tensor must be {dtype}, not {str(t.dtype).replace('torch.', '')}")
+        if t.dim() != 2:
+            raise LshError(1, f"{name} tensor must be 2-D, not {t.dim()}-D")
+
+
 def _ptr_of(a):
     """(pointer, on_device) of a numpy array or a torch tensor."""
     if hasattr(a, "data_ptr") and hasattr(a, "is_cuda"):
@@ -423,11 +435,12 @@ class Index:
         cfg = _IndexConfig()
         lib.lsh_index_config_default(C.byref(cfg))
         if hasattr(data, "data_ptr"):
-            if not data.is_contiguous():
-                raise LshError(1, "data tensor must be contiguous")
+            _check_tensor(data, "data", "float32")
             n, d = data.shape
         else:
             data = _f32(data)
+            if data.ndim != 2:
+                raise LshError(1, f"data must be a 2-D (N_data, d) array, not {data.ndim}-D")
             n, d = data.shape
         cfg.implementation = int(implementation)
         cfg.distanceFunc = int(distanceFunc)
@@ -454,17 +467,33 @@ class Index:
         """Returns (N_query, calculateK(k)) POINT_DTYPE results (numpy), or
         fills `out` (a device int64/float tensor view of the same bytes)."""
         lib = load()
+        if int(k) < 1:
+            raise LshError(1, "k must be >= 1")
         kq = calculateK(k)
         if hasattr(queries, "data_ptr"):
-            nq = int(queries.shape[0])
+            _check_tensor(queries, "queries", "float32")
         else:
             queries = _f32(queries)
-            nq = int(queries.shape[0])
+            if queries.ndim != 2:
+                raise LshError(1, f"queries must be a 2-D (N_query, d) array, not {queries.ndim}-D")
+        if int(queries.shape[1]) != self.d:
+            raise LshError(1, f"queries have {int(queries.shape[1])} columns; the index holds d = {self.d}")
+        nq = int(queries.shape[0])
         qp, q_dev = _ptr_of(queries)
         if out is None:
             res = np.empty((nq, kq), dtype=POINT_DTYPE)
             rp, r_dev = C.c_void_p(res.ctypes.data), False
         else:
+            need = nq * kq * POINT_DTYPE.itemsize
+            if hasattr(out, "data_ptr"):
+                _check_tensor(out, "out", None)
+                have = out.numel() * out.element_size()
+            else:
+                if not (isinstance(out, np.ndarray) and out.flags.c_contiguous and out.flags.writeable):
+                    raise LshError(1, "out must be a C-contiguous, writeable numpy array or device tensor")
+                have = out.nbytes
+            if have < need:
+                raise LshError(1, f"out holds {have} bytes; {nq} queries x {kq} results need {need}")
             res = out
             rp, r_dev = _ptr_of(out)
         sp = C.c_void_p(stream) if stream else None

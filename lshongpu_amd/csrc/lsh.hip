@@ -286,8 +286,10 @@ __global__ __launch_bounds__(kTPB) void lsh_scan_kernel(LshScanArgs a, TopKView 
                 if (owned) {
                     const float* __restrict__ row = a.djl + (size_t)id * a.sdim;
                     for (int j = 0; j < a.sdim; ++j) {
+                        // distanceFunctions.cuh:87-99: fl(t*t), then a separate add.
                         const float t = s_qf[j] - row[j];
-                        dist = __builtin_fmaf(t, t, dist);
+                        const float sq = t * t;
+                        dist = dist + sq;
                     }
                 }
                 const uint64_t key = make_key(f32_key(dist), id);

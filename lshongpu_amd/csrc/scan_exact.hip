@@ -68,7 +68,16 @@ __global__ __launch_bounds__(TPB) void exact_scan_kernel(ExactScanArgs a, TopKVi
                     } else if constexpr (KIND == kGenJaccard) {
                         acc[q] += fminf(x, y);
                         acc2[q] += fmaxf(x, y);
+                    } else if constexpr (KIND == kSqL2) {
+                        // distanceFunctions.cuh:87-99: pow(q - x, 2) then a
+                        // separate add (the square is exact in the promoted
+                        // type and rounds to fl(t*t)); the JL candidate set is
+                        // ranked by these FP32 sums, so they are not fused.
+                        const float t = y - x;
+                        const float sq = t * t;
+                        acc[q] = acc[q] + sq;
                     } else {
+                        // Euclidean ranking only (an FP64 recheck follows).
                         const float t = y - x;
                         acc[q] = __builtin_fmaf(t, t, acc[q]);
                     }

@@ -16,6 +16,8 @@ namespace lshgpu {
 constexpr int kSelBins = 1280;  // one bin per distance up to 1278 (1024-bit sketches); the last holds the rest
 constexpr int kTieCap = 512;      // keys of the boundary bin gathered in LDS (more: selected in place)
 constexpr int kSelScratch = 8;    // LDS words int_kth_key uses for its shared scalars
+// Wave 0 of int_kth_key scans the histogram in 64 equal slices.
+static_assert(kSelBins % 64 == 0, "the histogram scan splits kSelBins bins over 64 lanes");
 
 // The keys are visited through `each(f)`, which calls f(key) for every key
 // the calling thread owns (every thread of the workgroup together covers the
@@ -141,7 +143,6 @@ __device__ inline uint64_t int_kth_key(const Each& each, uint32_t K, uint32_t* h
                     ties[atomicAdd(&s_nties, 1u)] = key;
             });
             __syncthreads();
-            uint64_t* s_kth = reinterpret_cast<uint64_t*>(hist);  // hist is no longer needed
             const uint32_t want = K - below - 1u;
             for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) {
                 const uint64_t key = ties[t];
@@ -151,10 +152,14 @@ __device__ inline uint64_t int_kth_key(const Each& each, uint32_t K, uint32_t* h
                     lt += x < key ? 1u : 0u;
                     le += x <= key ? 1u : 0u;
                 }
-                if (lt <= want && want < le) *s_kth = key;
+                if (lt <= want && want < le) {
+                    // Exactly one thread finds it; the key goes to two scalar words.
+                    sv[5] = (uint32_t)key;
+                    sv[6] = (uint32_t)(key >> 32);
+                }
             }
             __syncthreads();
-            kth = *s_kth;
+            kth = ((uint64_t)sv[6] << 32) | sv[5];
         } else {
             kth = radix_kth_in_bin(each, T, K - below, last ? 56 : 24, last ? 0ull : (uint64_t)T << 32, hist, &sv[2],
                                    &sv[3]);

@@ -300,8 +300,14 @@ EXPORT void orc_sketch_topc(int kind, const uint32_t* ds, int64_t N,
     });
 }
 
-// Float sketch candidate set (implementation 6, squared L2 of JL sketches,
-// distanceFunctions.cuh:87-99, summed in FP32 as (q-x)*(q-x) with fmaf).
+// Float sketch candidate set (implementation 6): squared L2 of JL sketches as
+// distanceFunctions.cuh:87-99 euclideanDistanceFunc sums it in FP32.  Each
+// term is pow(q - x, 2): the square of a float is exact in double (48
+// significant bits), so whichever precision pow runs in, the term stored in
+// the float `dist` is the correctly rounded fl(t*t); it is then added to the
+// running sum by a separate FP32 add.  (Assumption: the reference's compiler
+// did not fuse that add with the square; the statement boundary and the pow
+// call stand between them.)  Candidates are ordered by (sum, ID).
 EXPORT void orc_jl_topc(const float* ds, int64_t N, const float* qs, int64_t Q,
                         int sdim, int C, int32_t* ids) {
     parallel_for(Q, [&](int64_t q) {
@@ -311,8 +317,9 @@ EXPORT void orc_jl_topc(const float* ds, int64_t N, const float* qs, int64_t Q,
             const float* dv = ds + i * sdim;
             float s = 0.0f;
             for (int j = 0; j < sdim; ++j) {
-                float t = qv[j] - dv[j];
-                s = std::fmaf(t, t, s);
+                const float t = qv[j] - dv[j];
+                const float sq = t * t;  // -ffp-contract=off: not fused with the add
+                s = s + sq;
             }
             c[i].key = f32_key(s);
             c[i].id = (int32_t)i;

@@ -87,3 +87,44 @@ def test_short_buffers_rejected(lsh):
     x = np.zeros((4, 3), np.float32)
     with pytest.raises(lsh.LshError):
         lsh.linearScans(2, 32, 3, 10, 4, x, x, 1, 1)
+
+
+def _index_shell(lsh, d):
+    """An Index object whose argument checks run without a device (its
+    handle is null, so nothing reaches the library)."""
+    ix = lsh.Index.__new__(lsh.Index)
+    ix.d, ix.n, ix._h = d, 0, C.c_void_p()
+    return ix
+
+
+def test_index_search_rejects_bad_host_buffers(lsh):
+    ix = _index_shell(lsh, 8)
+    good = np.zeros((3, 8), np.float32)
+    with pytest.raises(lsh.LshError, match="2-D"):
+        ix.search(np.zeros(8, np.float32), 32)
+    with pytest.raises(lsh.LshError, match="d = 8"):
+        ix.search(np.zeros((3, 7), np.float32), 32)
+    with pytest.raises(lsh.LshError, match="k must be"):
+        ix.search(good, 0)
+    short = np.empty((3, 31), dtype=lsh.POINT_DTYPE)
+    with pytest.raises(lsh.LshError, match="need"):
+        ix.search(good, 32, out=short)
+    strided = np.empty((3, 64), dtype=lsh.POINT_DTYPE)[:, ::2]
+    with pytest.raises(lsh.LshError, match="contiguous"):
+        ix.search(good, 32, out=strided)
+
+
+def test_index_search_rejects_bad_tensors(lsh):
+    torch = pytest.importorskip("torch")
+    ix = _index_shell(lsh, 8)
+    good = torch.zeros((3, 8), dtype=torch.float32)
+    with pytest.raises(lsh.LshError, match="float32"):
+        ix.search(torch.zeros((3, 8), dtype=torch.float16), 32)
+    with pytest.raises(lsh.LshError, match="contiguous"):
+        ix.search(torch.zeros((8, 3), dtype=torch.float32).t(), 32)
+    with pytest.raises(lsh.LshError, match="d = 8"):
+        ix.search(torch.zeros((3, 9), dtype=torch.float32), 32)
+    with pytest.raises(lsh.LshError, match="need"):
+        ix.search(good, 32, out=torch.empty((3, 63), dtype=torch.int32))
+    with pytest.raises(lsh.LshError, match="float32"):
+        lsh.Index(torch.zeros((16, 8), dtype=torch.float64))

@@ -281,13 +281,72 @@ def test_simhash_index_with_pivot_cells_matches_oracle(lsh, orc):
     _close(res["distance"], odist)
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("k", [1, 10, 100])
+def test_simhash_index_cells_other_k(lsh, orc, k):
+    # The pivot-cell index at the k values a serving load draws (1, 10, 100;
+    # rounded to 32, 32, 128).  k = 100 runs the rerank as one kernel with the
+    # FP64 recheck inside (the split form holds at most 64 results).
+    rng = np.random.default_rng(83 + k)
+    N, Q, d, words, C = 120000, 120, 48, 16, 256
+    centres = rng.standard_normal((400, d)).astype(np.float32)
+    data = centres[rng.integers(0, 400, N)] + 0.3 * rng.standard_normal((N, d)).astype(np.float32)
+    queries = centres[rng.integers(0, 400, Q)] + 0.3 * rng.standard_normal((Q, d)).astype(np.float32)
+    data /= np.linalg.norm(data, axis=1, keepdims=True)
+    queries /= np.linalg.norm(queries, axis=1, keepdims=True)
+    res = _index_vs_oracle(lsh, orc, data, queries, words, C, k)
+    assert res.shape == (Q, lsh.calculateK(k))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("words", [48, 64])
+def test_simhash_index_wide_sketches(lsh, orc, words):
+    # 1536- and 2048-bit sketches (the reference's published rows reach 2048
+    # bits).  Queries pointing away from the data put every candidate at a
+    # Hamming distance above 1278, in the select histogram's last bin, which
+    # is resolved by a radix select over the whole key.
+    rng = np.random.default_rng(90 + words)
+    N, Q, d, C, k = 30000, 24, 40, 300, 32
+    centres = rng.standard_normal((50, d)).astype(np.float32)
+    data = centres[rng.integers(0, 50, N)] + 0.3 * rng.standard_normal((N, d)).astype(np.float32)
+    data /= np.linalg.norm(data, axis=1, keepdims=True)
+    queries = _unit(rng, Q, d)
+    queries[:12] = -data[rng.integers(0, N, 12)]
+    res = _index_vs_oracle(lsh, orc, data, queries, words, C, k)
+    assert res.shape == (Q, 32)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("words", [44, 48])
+def test_hamming_topc_last_select_bin(lsh, orc, words):
+    # Distances of 1279 and more share the select histogram's last bin.  The
+    # data are one sketch c with about 3% of its bits flipped per row (so
+    # thousands of rows tie at each distance), the first queries its
+    # complement, so their whole top-C lies in that bin; random queries use
+    # the middle bins.
+    rng = np.random.default_rng(95 + words)
+    N, C = 20000, 300
+    p = 0.03
+    c = rng.integers(0, 2 ** 32, size=words, dtype=np.uint64).astype(np.uint32)
+    flips = np.packbits((rng.random((N, words * 32)) < p).astype(np.uint8), axis=1).view(np.uint32)
+    ds = (c[None, :] ^ flips).astype(np.uint32)
+    qs = rng.integers(0, 2 ** 32, size=(16, words), dtype=np.uint64).astype(np.uint32)
+    qs[:10] = ~c
+    qs[5:10, 0] ^= np.uint32(1) << np.arange(5, dtype=np.uint32)
+    ids, dist = lsh.hamming_topc(ds, qs, C)
+    oids, odist = orc.sketch_topc(0, ds, qs, C)
+    assert (odist[:10] >= 1279).all()
+    np.testing.assert_array_equal(ids, oids)
+    np.testing.assert_array_equal(dist, odist)
+
+
 def _index_vs_oracle(lsh, orc, data, queries, words, C, k):
     d = data.shape[1]
     with lsh.Index(data, implementation=3, sketchDim=words, distanceFunc=1, candidates=C) as ix:
         res = ix.search(queries, k)
     planes = orc.normal_stream(words * 32 * d)
     cand, _ = orc.sketch_topc(0, orc.simhash(data, planes, words), orc.simhash(queries, planes, words), C)
-    oids, odist = orc.rerank(data, queries, cand, k, 1)
+    oids, odist = orc.rerank(data, queries, cand, lsh.calculateK(k), 1)
     np.testing.assert_array_equal(res["ID"], oids)
     _close(res["distance"], odist)
     return res
@@ -389,6 +448,39 @@ def test_simhash_pipeline_matches_oracle(lsh, orc):
     _close(res.distances, dist)
 
 
+@pytest.mark.parametrize("impl", [3, 4, 5])
+@pytest.mark.parametrize("C", [None, 3000])
+def test_sketch_scans_default_candidates(lsh, orc, impl, C):
+    # The drop-in call without options uses C = 4096, the reference's rerank
+    # queue width; candidate sets above 2048 (4096 and 3000) take the sorted
+    # float-key select instead of the integer-key one.
+    rng = np.random.default_rng(60 + impl + (C or 0))
+    N, Q, d, k = 20000, 12, 64, 32
+    Ceff = C or 4096
+    kw = {} if C is None else {"candidates": C}
+    if impl == 3:
+        sk = 8
+        data, queries = _unit(rng, N, d), _unit(rng, Q, d)
+        res = lsh.linearScans(3, k, d, Q, N, data, queries, sk, 1, **kw)
+        planes = orc.normal_stream(sk * 32 * d)
+        ds, qs = orc.simhash(data, planes, sk), orc.simhash(queries, planes, sk)
+        td, tq, func, kind = data, queries, 1, 0
+    else:
+        sk = 24 if impl == 4 else 8
+        data, queries = _wmh_rows(rng, N, d, 0.3), _wmh_rows(rng, Q, d, 0.3)
+        res = lsh.linearScans(impl, k, d, Q, N, data, queries, sk, 2, **kw)
+        td, tq = orc.gj_transform(data, queries)
+        bounds, start, M = orc.wmh_bounds(td, tq)
+        bm = orc.bit_stream(4096)
+        ds = orc.wmh_sketch(td, bounds, start, M, sk, impl - 4, bm)
+        qs = orc.wmh_sketch(tq, bounds, start, M, sk, impl - 4, bm)
+        func, kind = 2, 1 if impl == 4 else 0
+    cand, _ = orc.sketch_topc(kind, ds, qs, Ceff)
+    ids, dist = orc.rerank(td, tq, cand, k, func)
+    np.testing.assert_array_equal(res.ids, ids)
+    _close(res.distances, dist)
+
+
 def test_jl_pipeline_matches_oracle(lsh, orc):
     rng = np.random.default_rng(22)
     N, Q, d, sdim, C, k = 8000, 20, 48, 24, 128, 32
@@ -399,6 +491,28 @@ def test_jl_pipeline_matches_oracle(lsh, orc):
     planes = orc.normal_stream(sdim * d, 1 / sdim)
     cand = orc.jl_topc(orc.jl(nd, planes, sdim), orc.jl(nq, planes, sdim), C)
     ids, dist = orc.rerank(nd, nq, cand, k, 1)
+    np.testing.assert_array_equal(res.ids, ids)
+    _close(res.distances, dist)
+
+
+def test_jl_pipeline_near_ties_match_oracle(lsh, orc):
+    # Rows repeated with last-bit perturbations: their JL sketch distances
+    # differ by a few FP32 roundings, so the top-C set depends on the sums
+    # being fl(t*t) plus a separate add (distanceFunctions.cuh:87-99); a fused
+    # sum changes the set for most of these queries (checked on the CPU).
+    # k = C makes the returned IDs the whole candidate set.
+    rng = np.random.default_rng(41)
+    d, sdim, C = 48, 24, 64
+    base = rng.standard_normal((30, d)).astype(np.float32)
+    data = np.repeat(base, 100, axis=0)
+    data = (data * (1 + rng.integers(-3, 4, size=data.shape) * 2.0 ** -23)).astype(np.float32)
+    queries = np.concatenate([base[:3] * 1.0001, rng.standard_normal((5, d))]).astype(np.float32)
+    N, Q = len(data), len(queries)
+    res = lsh.linearScans(6, C, d, Q, N, data, queries, sdim, 1, candidates=C)
+    nd, nq = orc.normalize_rows(data), orc.normalize_rows(queries)
+    planes = orc.normal_stream(sdim * d, 1 / sdim)
+    cand = orc.jl_topc(orc.jl(nd, planes, sdim), orc.jl(nq, planes, sdim), C)
+    ids, dist = orc.rerank(nd, nq, cand, C, 1)
     np.testing.assert_array_equal(res.ids, ids)
     _close(res.distances, dist)
 

@@ -211,6 +211,83 @@ def test_simhash_generic_bits_per_word(orc):
     assert full.shape == (50, 4)
 
 
+def test_simhash_bits_exact_fma_chain(orc):
+    # Every bit, borderline ones included: the sign of the sequential FP32
+    # fused multiply-add chain (computed here by an exact emulation of fmaf).
+    from _fp32 import dot_fma_chain
+
+    rng = np.random.default_rng(13)
+    d, words = 29, 2
+    x = rng.standard_normal((80, d)).astype(np.float32)
+    planes = rng.standard_normal((words * 32, d)).astype(np.float32)
+    dots = dot_fma_chain(x, planes)
+    want = (dots >= 0).astype(np.uint32)
+    bits = orc.simhash(x, planes, words)
+    got = np.stack([(bits[:, w][:, None] >> np.arange(32)) & 1 for w in range(words)], 1).reshape(80, -1)
+    np.testing.assert_array_equal(got, want)
+
+
+# ---------------------------------------------------------------- SimHash-JL
+
+def test_jl_projection_is_the_fma_chain(orc):
+    # simHashJL.cuh:34-51: sketch value s of row i = sequential FP32 fmaf chain
+    # of x_i . planes[s] in index order, bit for bit; and within the FP32 dot
+    # product error bound gamma_d * sum|x||p| of the FP64 product (a swapped
+    # operand or a wrong stride would be far outside it).
+    from _fp32 import dot_fma_chain
+
+    rng = np.random.default_rng(31)
+    n, d, sdim = 60, 37, 24
+    x = rng.standard_normal((n, d)).astype(np.float32)
+    planes = (rng.standard_normal((sdim, d)) / np.sqrt(sdim)).astype(np.float32)
+    got = orc.jl(x, planes, sdim)
+    np.testing.assert_array_equal(got, dot_fma_chain(x, planes))
+    exact = x.astype(np.float64) @ planes.astype(np.float64).T
+    mag = np.abs(x.astype(np.float64)) @ np.abs(planes.astype(np.float64)).T
+    gamma = d * 2.0 ** -24 / (1 - d * 2.0 ** -24)
+    assert np.all(np.abs(got - exact) <= gamma * mag)
+
+
+def _jl_sq_l2(ds, q, fused):
+    """Squared L2 of every row of ds to q, summed in FP32 in index order:
+    fl(t*t) then a separate add (fused=False, the reference's pow + add), or
+    one fmaf per term (fused=True)."""
+    from _fp32 import fmaf32
+
+    s = np.zeros(ds.shape[0], np.float32)
+    for j in range(ds.shape[1]):
+        t = (q[j] - ds[:, j]).astype(np.float32)
+        s = fmaf32(t, t, s) if fused else (s + t * t).astype(np.float32)
+    return s
+
+
+def test_jl_topc_unfused_squares_and_id_ties(orc):
+    # distanceFunctions.cuh:87-99: each term pow(q - x, 2) is fl(t*t), added
+    # by a separate FP32 add; candidates ordered by (sum, ID).  The rows are
+    # permutations of one vector, so their exact distances to a query at the
+    # origin are equal and only the FP32 roundings order them: a fused sum
+    # gives a different order (checked below), so the pin tells the two apart.
+    rng = np.random.default_rng(32)
+    sdim, N, C = 16, 2000, 150
+    base = rng.uniform(0.5, 2.0, sdim).astype(np.float32)
+    ds = np.stack([rng.permutation(base) for _ in range(N)]).astype(np.float32)
+    ds[1500:] = rng.standard_normal((500, sdim)).astype(np.float32) * 2
+    qs = np.zeros((4, sdim), np.float32)
+    qs[2:] = rng.standard_normal((2, sdim)).astype(np.float32) * 0.01
+    ids = orc.jl_topc(ds, qs, C)
+    differs = 0
+    for q in range(len(qs)):
+        s = _jl_sq_l2(ds, qs[q], fused=False)
+        order = np.lexsort((np.arange(N), s))[:C]
+        np.testing.assert_array_equal(ids[q], order)
+        sf = _jl_sq_l2(ds, qs[q], fused=True)
+        differs += int(not np.array_equal(np.lexsort((np.arange(N), sf))[:C], order))
+    assert differs > 0
+    # fewer data rows than C: the tail is padded with -1
+    few = orc.jl_topc(ds[:10], qs[:1], 16)
+    assert (few[0, 10:] == -1).all() and sorted(few[0, :10]) == list(range(10))
+
+
 def _popcount(a):
     a = a.astype(np.uint32)
     return np.unpackbits(a.view(np.uint8).reshape(a.shape + (4,)), axis=-1).sum(-1).sum(-1)
