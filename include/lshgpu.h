@@ -180,6 +180,23 @@ int lsh_hamming_topc(const uint32_t* data_sketch, int64_t N_data,
                      const uint32_t* query_sketch, int64_t N_query, int words,
                      int C, int32_t* ids, int32_t* distances);
 
+/* No counterpart in the reference.  lsh_hamming_topc for callers that cannot
+ * allocate device memory (code under graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same top-C (IDs and distances, equal
+ * to lsh_hamming_topc's), but every pointer is a device pointer on the
+ * calling thread's current device, the work is queued on `stream` (a
+ * hipStream_t, NULL = the null stream) and the call returns without waiting.
+ * It allocates nothing: its working memory (the query sketches padded to 8
+ * words and 256 rows, and a top-C state of 8*(C + capacity) + 16 bytes per
+ * query, capacity = 2C + 256 for C <= 1024, else C + 512) comes from one
+ * 32 MiB array per GPU that the library holds from its first use until the
+ * process ends.  Calls that need more return LSH_ERR_NOMEM.  Every call on a
+ * GPU shares that array: calls that overlap on one GPU (other streams or
+ * threads) race, so the caller orders them.  C <= 4096 (C <= 2048 keeps an
+ * unordered best list; larger C is rounded up to a power of two inside). */
+int lsh_hamming_topc_noalloc(const uint32_t* data_sketch, int64_t N_data, const uint32_t* query_sketch,
+                             int64_t N_query, int words, int C, int32_t* ids, int32_t* distances, void* stream);
+
 /* weightedMinHash.cuh:41-166 on rows that are already shifted and
  * normalised: bounds from x, then per hash h (seed h*1234 + 92138) the number
  * of red draws before the first green one.  oneBit == 0: out is [n][sketchDim]

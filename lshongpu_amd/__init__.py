@@ -117,6 +117,8 @@ _SIGS = {
     "lsh_random_signs": (C.c_int, [C.POINTER(C.c_int32), C.c_int64]),
     "lsh_simhash_sketch": (C.c_int, [_FP, C.c_int64, C.c_int, C.c_int, C.c_int, _FP, C.POINTER(C.c_uint32)]),
     "lsh_jl_sketch": (C.c_int, [_FP, C.c_int64, C.c_int, C.c_int, _FP, _FP]),
+    "lsh_hamming_topc_noalloc": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_int, C.c_int,
+                                           C.c_void_p, C.c_void_p, C.c_void_p]),
     "lsh_gj_transform": (C.c_int, [_FP, C.c_int64, _FP, C.c_int64, C.c_int]),
     "lsh_hamming_topc": (C.c_int, [C.POINTER(C.c_uint32), C.c_int64, C.POINTER(C.c_uint32), C.c_int64, C.c_int,
                                    C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),

@@ -32,6 +32,7 @@ SOURCES = [
     "lsh.hip",
     "stats.hip",
     "util.hip",
+    "noalloc.hip",
     "io.cpp",
 ]
 

@@ -252,6 +252,14 @@ int lsh_hamming_topc(const uint32_t* data_sketch, int64_t N_data, const uint32_t
     });
 }
 
+int lsh_hamming_topc_noalloc(const uint32_t* data_sketch, int64_t N_data, const uint32_t* query_sketch,
+                             int64_t N_query, int words, int C, int32_t* ids, int32_t* distances, void* stream) {
+    return guarded([&] {
+        lshgpu::hamming_topc_device_noalloc(data_sketch, N_data, query_sketch, N_query, words, C, ids, distances,
+                                            (hipStream_t)stream);
+    });
+}
+
 int lsh_wmh_sketch(const float* x, int64_t n, int d, int sketchDim, int oneBit, uint32_t* out) {
     return guarded([&] {
         LSH_REQUIRE((x && out) || n == 0, LSH_ERR_INVALID, "null pointer");

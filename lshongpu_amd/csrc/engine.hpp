@@ -252,6 +252,10 @@ void gj_transform_host(float* data, int64_t N, float* queries, int64_t Q, int d,
 void hamming_topc_host(const uint32_t* ds, int64_t N, const uint32_t* qs, int64_t Q, int words, int C,
                        int32_t* ids, int32_t* dists, int device);
 void wmh_host(const float* x, int64_t n, int d, int sketchDim, int one_bit, uint32_t* out, int device);
+// Device sketches in, device IDs and distances out, working memory from the
+// per-GPU scratch array of noalloc.hip (no allocation; on the current device).
+void hamming_topc_device_noalloc(const uint32_t* ds, int64_t N, const uint32_t* qs, int64_t Q, int words, int C,
+                                 int32_t* ids, int32_t* dists, hipStream_t s);
 void crosspoly_host(const float* x, int64_t n, int d, int tables, const int32_t* signs, uint16_t* keys,
                     int device);
 

@@ -223,6 +223,8 @@ int newcap_for(int K);
 // flags (may be null, may be pinned host memory): after the select,
 // flags[q] = 1 for each query whose list is incomplete, and flags[Q + i] =
 // extra[i] for i < n_extra (n_extra <= 256).
+// The clearing TopKState::reset does, over a view of memory the caller owns.
+void launch_topk_clear(const TopKView& v, hipStream_t s);
 void launch_select_merge(const TopKView& v, hipStream_t s, uint32_t* flags = nullptr,
                          const uint32_t* extra = nullptr, int n_extra = 0);
 void launch_sort_rows(const TopKView& v, hipStream_t s);  // orders the best list of every query
@@ -308,12 +310,14 @@ void launch_crosspoly(const float* x, int64_t n, int d, int tables, const int32_
 // [p0,p1).  kind 0: Hamming over `words` words; kind 1: words minus the
 // number of equal words (weighted-MinHash match count).
 struct SketchScanArgs {
-    const uint32_t* data;   // [N][words]
+    const uint32_t* data;   // [N][dwords]
     const uint32_t* qs;     // [ceil(Q/64)*64][words], rows past Q zero
-    int words;
+    int words;              // a multiple of 8 (query rows zero padded)
     int Q;
     int64_t p0, p1;
     int kind;
+    int dwords = 0;         // words per data row, <= words (0: words); rows of fewer
+                            // words, or not 16-byte aligned, are read word by word
 };
 void launch_sketch_scan(const SketchScanArgs& a, const TopKView& v, hipStream_t s);
 

@@ -27,7 +27,7 @@ constexpr int QGS = 64;  // queries per group
 constexpr int CH = 8;    // sketch words per register chunk (rows are padded to a multiple of 8)
 constexpr int TPB = 256;
 
-template <int KIND>
+template <int KIND, bool RAGGED>
 __global__ __launch_bounds__(TPB) void sketch_scan_kernel(SketchScanArgs a, TopKView v) {
     const int g = blockIdx.x;
     const int qbase = g * QGS;
@@ -35,8 +35,9 @@ __global__ __launch_bounds__(TPB) void sketch_scan_kernel(SketchScanArgs a, TopK
     const int64_t i = a.p0 + (int64_t)blockIdx.y * TPB + threadIdx.x;
     const bool active = i < a.p1;
     const int W = a.words;  // multiple of CH
+    const int DW = RAGGED ? a.dwords : W;
     const int64_t ri = active ? i : a.p0;
-    const uint32_t* __restrict__ row = a.data + (size_t)ri * W;
+    const uint32_t* __restrict__ row = a.data + (size_t)ri * DW;
     const uint32_t* __restrict__ qg = a.qs + (size_t)qbase * W;
 
     uint32_t h[QGS];
@@ -46,13 +47,19 @@ __global__ __launch_bounds__(TPB) void sketch_scan_kernel(SketchScanArgs a, TopK
 #pragma clang loop unroll(disable) vectorize(disable) interleave(disable)
     for (int w0 = 0; w0 < W; w0 += CH) {
         uint32_t xr[CH];
+        if constexpr (RAGGED) {
+            // words past the row read as zero, as the zero-padded query words
 #pragma unroll
-        for (int c = 0; c < CH / 4; ++c) {
-            const uint4 t = *reinterpret_cast<const uint4*>(row + w0 + 4 * c);
-            xr[4 * c + 0] = t.x;
-            xr[4 * c + 1] = t.y;
-            xr[4 * c + 2] = t.z;
-            xr[4 * c + 3] = t.w;
+            for (int c = 0; c < CH; ++c) xr[c] = w0 + c < DW ? row[w0 + c] : 0u;
+        } else {
+#pragma unroll
+            for (int c = 0; c < CH / 4; ++c) {
+                const uint4 t = *reinterpret_cast<const uint4*>(row + w0 + 4 * c);
+                xr[4 * c + 0] = t.x;
+                xr[4 * c + 1] = t.y;
+                xr[4 * c + 2] = t.z;
+                xr[4 * c + 3] = t.w;
+            }
         }
 #pragma unroll
         for (int q = 0; q < QGS; ++q) {
@@ -86,16 +93,22 @@ template <int KIND>
 void launch_kind(const SketchScanArgs& a, const TopKView& v, hipStream_t s) {
     const int groups = (a.Q + QGS - 1) / QGS;
     const int64_t npts = a.p1 - a.p0;
-    LSH_REQUIRE(a.words % CH == 0 && (reinterpret_cast<uintptr_t>(a.data) % 16) == 0 &&
-                    (reinterpret_cast<uintptr_t>(a.qs) % 16) == 0,
-                LSH_ERR_INTERNAL, "sketch rows must be padded to 8 words and 16-byte aligned");
+    LSH_REQUIRE(a.words % CH == 0 && (reinterpret_cast<uintptr_t>(a.qs) % 16) == 0 && a.dwords >= 0 &&
+                    a.dwords <= a.words,
+                LSH_ERR_INTERNAL, "query sketch rows must be padded to 8 words and 16-byte aligned");
+    const bool ragged = (a.dwords != 0 && a.dwords != a.words) || (reinterpret_cast<uintptr_t>(a.data) % 16) != 0;
+    SketchScanArgs base = a;
+    if (base.dwords == 0) base.dwords = a.words;
     const int64_t max_pts = (int64_t)65535 * TPB;
     for (int64_t off = 0; off < npts; off += max_pts) {
-        SketchScanArgs b = a;
+        SketchScanArgs b = base;
         b.p0 = a.p0 + off;
         b.p1 = std::min(a.p1, b.p0 + max_pts);
         const unsigned by = (unsigned)((b.p1 - b.p0 + TPB - 1) / TPB);
-        hipLaunchKernelGGL((sketch_scan_kernel<KIND>), dim3(groups, by), dim3(TPB), 0, s, b, v);
+        if (ragged)
+            hipLaunchKernelGGL((sketch_scan_kernel<KIND, true>), dim3(groups, by), dim3(TPB), 0, s, b, v);
+        else
+            hipLaunchKernelGGL((sketch_scan_kernel<KIND, false>), dim3(groups, by), dim3(TPB), 0, s, b, v);
     }
     check_launch("sketch_scan_kernel");
 }

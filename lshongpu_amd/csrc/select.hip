@@ -224,13 +224,15 @@ void TopKState::reset(int Q, int K, int newcap, hipStream_t s, bool int_keys, bo
     if (launch) clear(s);
 }
 
-void TopKState::clear(hipStream_t s) {
+void TopKState::clear(hipStream_t s) { launch_topk_clear(v_, s); }
+
+void launch_topk_clear(const TopKView& v, hipStream_t s) {
     // A fill kernel rather than memsets keeps the sequence capturable in a
     // graph.  Only the best lists need clearing: the selects read the
     // appended slots up to each query's count.
-    if (v_.Q <= 0) return;
-    const size_t n = v_.fresh ? (size_t)v_.Q : (size_t)v_.Q * v_.K;
-    hipLaunchKernelGGL(reset_state_kernel, dim3(grid_for(n)), dim3(256), 0, s, v_);
+    if (v.Q <= 0) return;
+    const size_t n = v.fresh ? (size_t)v.Q : (size_t)v.Q * v.K;
+    hipLaunchKernelGGL(reset_state_kernel, dim3(grid_for(n)), dim3(256), 0, s, v);
     check_launch("top-K state reset");
 }
 
