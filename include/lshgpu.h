@@ -107,7 +107,13 @@ int lsh_linear_scans(int implementation, int k, int d, int N_query, int N_data,
  * count (8-bit key type; needs bucketKeyBits >= 8), 5 1-bit weighted MinHash
  * (bucketKeyBits bits, 16-bit key type), 7 cross-polytope (one hash per
  * table; needs 2^bucketKeyBits >= 2*P, P = d rounded up to a power of two,
- * d <= 256); bucketKeyBits 1..24.  Each key family preprocesses the rows
+ * d <= 256), 8 (no reference counterpart) dense-rotation cross-polytope,
+ * K = bucketKeyBits / log2(2P) hashes per table in a 32-bit key (see
+ * lsh_crosspoly_dense_keys; bucketKeyBits a multiple of log2(2P), d <= 128);
+ * bucketKeyBits 1..32.  Tables of up to 16 key bits are 2^bits bucket
+ * offsets as the reference builds them; wider keys and keysImplementation 8
+ * keep each table as (key, ID) pairs sorted on the device, looked up by
+ * binary search.  Each key family preprocesses the rows
  * first (4, 5: shift + normalise; 7: normalise), then the sketch family of
  * `implementation` does (4, 5: shift + normalise; 6: normalise), in that
  * order (lshFramework.cuh:330, 391).  The candidate set of a query is the
@@ -215,6 +221,26 @@ int lsh_wmh_sketch(const float* x, int64_t n, int d, int sketchDim, int oneBit,
  * reference stream (lsh_random_signs over tables*3*P values). */
 int lsh_crosspoly_keys(const float* x, int64_t n, int d, int tables,
                        const int32_t* signs, uint16_t* keys);
+
+/* No counterpart in the reference (keysImplementation 8): dense-rotation
+ * cross-polytope keys.  Rows are L2-normalised, then hash h = t*K + k of row
+ * x is the index of the largest |y_j| of y = G_h x (the first on ties),
+ * encoded j when y_j < 0 and j + P otherwise, as crossPolyTope.cuh:64-76
+ * encodes its HD^3 hash; y_j is the sequential FP32 fused multiply-add chain
+ * over dimensions in index order.  G_h is P x d (P = d rounded up to a power
+ * of two), its entries the reference normal stream (lsh_random_normals with
+ * stddev 1, tables*K*P*d values, row-major [h][j][i]) rounded to bfloat16.
+ * Key t packs hashes t*K .. t*K+K-1 into log2(2P)-bit fields, hash k at bit
+ * k*log2(2P).  keys is [n][tables].  d <= 128, K*log2(2P) <= 32. */
+int lsh_crosspoly_dense_keys(const float* x, int64_t n, int d, int tables, int K, uint32_t* keys);
+
+/* The bucket-table build of keys wider than 16 bits (and of keysImplementation
+ * 8), which replaces the 2^bits offsets of lshFramework.cuh:102-140: per table
+ * t, keys[t][0..N) (the low `bits` bits; 1..32) sorted ascending with their
+ * point IDs, equal keys in ascending ID order (a stable sort).  keys,
+ * sorted_keys and ids are [tables][N]; tables*N < 2^32. */
+int lsh_sort_table_keys(const uint32_t* keys, int64_t N, int tables, int bits, uint32_t* sorted_keys,
+                        uint32_t* ids);
 
 /* statistics.cuh:10-16 bucketDistribution, as weightedMinHash.cuh:215-229
  * runBucketStatistics runs it over byte sketches: counts256[v] = the number

@@ -27,7 +27,8 @@ import numpy as np
 __all__ = [
     "LshError", "Result", "Index", "POINT_DTYPE", "load", "calculateK", "linearScans", "LSH",
     "linear_scans", "lsh_framework", "random_normals", "random_signs", "simhash_sketch",
-    "jl_sketch", "gj_transform", "hamming_topc", "wmh_sketch", "crosspoly_keys", "library_path",
+    "jl_sketch", "gj_transform", "hamming_topc", "wmh_sketch", "crosspoly_keys", "crosspoly_dense_keys",
+    "sort_table_keys", "library_path",
     "read_text_dataset", "read_fvecs", "write_text_dataset", "read_truth", "write_truth", "validate",
     "write_results", "write_performance",
 ]
@@ -125,6 +126,9 @@ _SIGS = {
     "lsh_wmh_sketch": (C.c_int, [_FP, C.c_int64, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint32)]),
     "lsh_crosspoly_keys": (C.c_int, [_FP, C.c_int64, C.c_int, C.c_int, C.POINTER(C.c_int32),
                                      C.POINTER(C.c_uint16)]),
+    "lsh_crosspoly_dense_keys": (C.c_int, [_FP, C.c_int64, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint32)]),
+    "lsh_sort_table_keys": (C.c_int, [C.POINTER(C.c_uint32), C.c_int64, C.c_int, C.c_int, C.POINTER(C.c_uint32),
+                                      C.POINTER(C.c_uint32)]),
     "lsh_bucket_distribution": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
     "lsh_bucket_distribution_full_key": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.POINTER(C.c_void_p),
                                                    C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]),
@@ -363,6 +367,29 @@ def crosspoly_keys(x, tables: int, signs=None) -> np.ndarray:
         sp = signs.ctypes.data_as(C.POINTER(C.c_int32))
     _check(load().lsh_crosspoly_keys(_fp(x), n, d, int(tables), sp, keys.ctypes.data_as(C.POINTER(C.c_uint16))))
     return keys
+
+
+def crosspoly_dense_keys(x, tables: int, K: int) -> np.ndarray:
+    """keysImplementation 8 keys [n][tables] (uint32) of the rows of x (normalised first)."""
+    x = _f32(x)
+    n, d = x.shape
+    keys = np.empty((n, tables), dtype=np.uint32)
+    _check(load().lsh_crosspoly_dense_keys(_fp(x), n, d, int(tables), int(K),
+                                           keys.ctypes.data_as(C.POINTER(C.c_uint32))))
+    return keys
+
+
+def sort_table_keys(keys, bits: int):
+    """Stable device sort of every row of keys [tables][N] (low `bits` bits):
+    (sorted keys, IDs), both [tables][N]."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint32)
+    L, N = keys.shape
+    sk = np.empty_like(keys)
+    ids = np.empty_like(keys)
+    u32 = C.POINTER(C.c_uint32)
+    _check(load().lsh_sort_table_keys(keys.ctypes.data_as(u32), N, L, int(bits), sk.ctypes.data_as(u32),
+                                      ids.ctypes.data_as(u32)))
+    return sk, ids
 
 
 def bucket_distribution(hashes) -> np.ndarray:

@@ -33,6 +33,7 @@ SOURCES = [
     "stats.hip",
     "util.hip",
     "noalloc.hip",
+    "crosspoly_dense.hip",
     "io.cpp",
 ]
 

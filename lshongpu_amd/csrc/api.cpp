@@ -34,8 +34,8 @@ void configure(lsh_index& ix, const lsh_index_config_t& c) {
         LSH_REQUIRE(c.implementation >= 2 && c.implementation <= 6, LSH_ERR_INVALID,
                     "Invalid implementation selected for LSH (2..6)");
         LSH_REQUIRE(c.keysImplementation == 3 || c.keysImplementation == 4 || c.keysImplementation == 5 ||
-                        c.keysImplementation == 7,
-                    LSH_ERR_INVALID, "Invalid keys implementation selected for LSH (3, 4, 5, 7)");
+                        c.keysImplementation == 7 || c.keysImplementation == 8,
+                    LSH_ERR_INVALID, "Invalid keys implementation selected for LSH (3, 4, 5, 7, 8)");
         LSH_REQUIRE(c.tables >= 1, LSH_ERR_INVALID, "tables must be >= 1");
         LSH_REQUIRE(c.bucketKeyBits >= 1 && c.bucketKeyBits <= 32, LSH_ERR_INVALID,
                     "bucketKeyBits must be in 1..32");
@@ -274,6 +274,22 @@ int lsh_crosspoly_keys(const float* x, int64_t n, int d, int tables, const int32
         LSH_REQUIRE(d >= 1 && tables >= 1, LSH_ERR_INVALID, "d and tables must be >= 1");
         if (n == 0) return;
         lshgpu::crosspoly_host(x, n, d, tables, signs, keys, 0);
+    });
+}
+
+int lsh_crosspoly_dense_keys(const float* x, int64_t n, int d, int tables, int K, uint32_t* keys) {
+    return guarded([&] {
+        LSH_REQUIRE((x && keys) || n == 0, LSH_ERR_INVALID, "null pointer");
+        lshgpu::crosspoly_dense_host(x, n, d, tables, K, keys, 0);
+    });
+}
+
+int lsh_sort_table_keys(const uint32_t* keys, int64_t N, int tables, int bits, uint32_t* sorted_keys,
+                        uint32_t* ids) {
+    return guarded([&] {
+        LSH_REQUIRE((keys && sorted_keys && ids) || N == 0, LSH_ERR_INVALID, "null pointer");
+        LSH_REQUIRE((int64_t)tables * N < 0xFFFFFFFFll, LSH_ERR_UNSUPPORTED, "tables * N must be below 2^32");
+        lshgpu::sort_tables_host(keys, N, tables, bits, sorted_keys, ids, 0);
     });
 }
 

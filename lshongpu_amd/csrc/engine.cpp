@@ -499,7 +499,8 @@ void lsh_topk(const LshScanArgs& base, const std::vector<int32_t>& hmap, int64_t
 size_t lsh_index::device_bytes() const {
     size_t b = data.bytes() + data_inv.bytes() + dsk.bytes() + dfp4.bytes() + planesT.bytes() + djl.bytes() + wmh.bytes() +
                kwmh.bytes() + scratch.bytes() + kplanesT.bytes() + ksigns.bytes() + dkeys.bytes() + boff.bytes() + bperm.bytes() +
-               qkeys.bytes() + qlo.bytes() + qpre.bytes() + qmax.bytes() + q.bytes() + qT.bytes() +
+               qkeys.bytes() + qlo.bytes() + qpre.bytes() + qmax.bytes() + qlen.bytes() + bkeys.bytes() +
+               dense.bytes() + q.bytes() + qT.bytes() +
                q_inv.bytes() + qjl.bytes() + qjlT.bytes() + qsk.bytes() + out.bytes() + flags.bytes() + st.bytes();
     for (const SampleLevel& L : samples) b += L.ds.bytes() + L.fp4.bytes();
     b += cells.bytes() + qorder.bytes() + cdata.bytes() + cpos.bytes() + chalf.bytes() + rband.band.bytes() +
@@ -607,18 +608,31 @@ void lsh_index::build_fp4(hipStream_t s) {
 void lsh_index::build_lsh(const float* extra_in, int64_t n_extra, hipStream_t s) {
     float* extra = const_cast<float*>(extra_in);
     const int L = tables;
-    LSH_REQUIRE(bucketKeyBits >= 1 && bucketKeyBits <= 24, LSH_ERR_UNSUPPORTED, "bucketKeyBits must be in 1..24");
+    LSH_REQUIRE(bucketKeyBits >= 1 && bucketKeyBits <= 32, LSH_ERR_UNSUPPORTED, "bucketKeyBits must be in 1..32");
     LSH_REQUIRE((int64_t)L * std::max<int64_t>(N, 1) < 0xFFFFFFFFll, LSH_ERR_UNSUPPORTED,
                 "tables * N_data must be below 2^32");
-    tableSize = 1u << bucketKeyBits;
     if (keysImpl == 4 || keysImpl == 5) key_steps = one_step(kGjStep);
-    else if (keysImpl == 7) key_steps = one_step(kNormStep);
+    else if (keysImpl == 7 || keysImpl == 8) key_steps = one_step(kNormStep);
     apply_steps_build(key_steps, data.get(), N, extra, n_extra, d, s);
 
     // The reference's key types (kernel.cu:67-80): unsigned short for 3, 5
-    // and 7, unsigned char for 4.
-    keyBitsMask = keysImpl == 4 ? 0xFF : 0xFFFF;
-    if (keysImpl == 3) {
+    // and 7, unsigned char for 4.  The dense-rotation keys (8) are 32-bit.
+    keyBitsMask = keysImpl == 4 ? 0xFF : (keysImpl == 8 ? -1 : 0xFFFF);
+    // Tables wider than 16 bits (and the dense-rotation keys) are sorted
+    // (key, ID) arrays searched by binary search; narrower ones keep the
+    // reference's 2^bits bucket offsets (lshFramework.cuh:102-140).
+    sorted = keysImpl == 8 || bucketKeyBits > 16;
+    sortBits = keysImpl == 8 ? bucketKeyBits : std::min(bucketKeyBits, keysImpl == 4 ? 8 : 16);
+    tableSize = sorted ? 0u : 1u << bucketKeyBits;
+    if (keysImpl == 8) {
+        // K = bucketKeyBits / log2(2P) hashes of P outputs per table.
+        const int b = dense_key_field_bits(d);
+        LSH_REQUIRE(d <= 128, LSH_ERR_UNSUPPORTED, "dense cross-polytope keys support d <= 128");
+        LSH_REQUIRE(bucketKeyBits % b == 0, LSH_ERR_UNSUPPORTED,
+                    "dense cross-polytope keys: bucketKeyBits must be a multiple of log2(2P) = " + std::to_string(b) +
+                        " (P = d rounded up to a power of two)");
+        dense.build(d, L, bucketKeyBits / b, s);
+    } else if (keysImpl == 3) {
         // lshFramework.cuh:22-45: tables*d*bucketKeyBits hyperplanes; table t
         // takes bits b < bucketKeyBits from hyperplane t*bucketKeyBits + b.
         LSH_REQUIRE(bucketKeyBits <= 32, LSH_ERR_UNSUPPORTED, "SimHash keys take at most 32 bits");
@@ -639,17 +653,30 @@ void lsh_index::build_lsh(const float* extra_in, int64_t n_extra, hipStream_t s)
         int P = 1;
         while (P < d) P *= 2;
         LSH_REQUIRE(P <= 256, LSH_ERR_UNSUPPORTED, "cross-polytope keys support d <= 256");
-        LSH_REQUIRE((uint32_t)(2 * P) <= tableSize, LSH_ERR_UNSUPPORTED,
+        LSH_REQUIRE(bucketKeyBits >= 31 || (int64_t)(2 * P) <= ((int64_t)1 << bucketKeyBits), LSH_ERR_UNSUPPORTED,
                     "cross-polytope keys need 2^bucketKeyBits >= 2 * (d rounded up to a power of two)");
         std::vector<int32_t> sg = sign_stream((int64_t)L * 3 * P);
         ksigns.alloc(sg.size());
         LSH_HIP(hipMemcpyAsync(ksigns.get(), sg.data(), sizeof(int32_t) * sg.size(), hipMemcpyHostToDevice, s));
     }
     dkeys.alloc((size_t)N * L);
-    lsh_keys(data.get(), N, dkeys.get(), s);
-    boff.alloc((size_t)L * (tableSize + 1));
     bperm.alloc((size_t)L * N);
-    {
+    if (sorted) {
+        // Keys per table ([L][N]), then a stable radix sort of (key, ID).
+        DevBuf<uint32_t> kln((size_t)L * N), tk((size_t)L * N), ti((size_t)L * N),
+            counts(sort_table_counts(N, L));
+        if (keysImpl == 8) {
+            dense.keys(data.get(), N, dkeys.get(), kln.get(), s);
+        } else {
+            lsh_keys(data.get(), N, dkeys.get(), s);
+            launch_transpose_keys(dkeys.get(), N, L, kln.get(), s);
+        }
+        bkeys.alloc((size_t)L * N);
+        launch_sort_tables(kln.get(), N, L, sortBits, bkeys.get(), bperm.get(), tk.get(), ti.get(), counts.get(), s);
+        LSH_HIP(hipStreamSynchronize(s));
+    } else {
+        lsh_keys(data.get(), N, dkeys.get(), s);
+        boff.alloc((size_t)L * (tableSize + 1));
         DevBuf<uint32_t> cursor((size_t)L * (tableSize + 1));
         launch_bucket_build(dkeys.get(), N, L, tableSize, boff.get(), cursor.get(), bperm.get(), s);
         LSH_HIP(hipStreamSynchronize(s));
@@ -689,6 +716,10 @@ void lsh_index::build_lsh(const float* extra_in, int64_t n_extra, hipStream_t s)
 void lsh_index::lsh_keys(const float* x, int64_t n, uint32_t* keys, hipStream_t s) {
     const int L = tables;
     if (n <= 0) return;
+    if (keysImpl == 8) {
+        dense.keys(x, n, keys, nullptr, s);
+        return;
+    }
     if (keysImpl == 3) {
         launch_simhash(x, n, d, kplanesT.get(), L, bucketKeyBits, keys, L, s);
     } else if (keysImpl == 4 || keysImpl == 5) {
@@ -878,7 +909,13 @@ void lsh_index::search_lsh(int Q, int kq, hipStream_t s, PhaseTimer& pt) {
     qpre.ensure((size_t)Q * (L + 1));
     qmax.ensure(1);
     launch_fill_i32(reinterpret_cast<int32_t*>(qmax.get()), 1, 0, s);
-    launch_query_ranges(qkeys.get(), Q, L, tableSize, boff.get(), qlo.get(), qpre.get(), qmax.get(), s);
+    if (sorted) {
+        qlen.ensure((size_t)Q * L);
+        launch_query_ranges_sorted(qkeys.get(), Q, L, N, bkeys.get(), qlo.get(), qlen.get(), qpre.get(), qmax.get(),
+                                   s);
+    } else {
+        launch_query_ranges(qkeys.get(), Q, L, tableSize, boff.get(), qlo.get(), qpre.get(), qmax.get(), s);
+    }
     const bool sketched = withSketch && impl >= 3;
     if (sketched) {
         if (impl == 6) {
@@ -1084,6 +1121,37 @@ void wmh_host(const float* x, int64_t n, int d, int sketchDim, int one_bit, uint
     h.sketch(dx.get(), n, d, dout.get(), sketchDim, nullptr);
     LSH_HIP(hipDeviceSynchronize());
     LSH_HIP(hipMemcpy(out, dout.get(), sizeof(uint32_t) * (size_t)n * sketchDim, hipMemcpyDeviceToHost));
+}
+
+void crosspoly_dense_host(const float* x, int64_t n, int d, int tables, int K, uint32_t* keys, int device) {
+    DeviceGuard g(device);
+    LSH_REQUIRE(n >= 0 && d >= 1 && tables >= 1 && K >= 1, LSH_ERR_INVALID, "bad dense cross-polytope arguments");
+    LSH_REQUIRE(d <= 128, LSH_ERR_UNSUPPORTED, "dense cross-polytope keys support d <= 128");
+    LSH_REQUIRE(K * dense_key_field_bits(d) <= 32, LSH_ERR_UNSUPPORTED, "K hashes do not fit a 32-bit key");
+    if (n == 0) return;
+    DenseRotations dr;
+    dr.build(d, tables, K, nullptr);
+    DevBuf<float> dx((size_t)n * d);
+    DevBuf<uint32_t> dk((size_t)n * tables);
+    LSH_HIP(hipMemcpy(dx.get(), x, sizeof(float) * (size_t)n * d, hipMemcpyHostToDevice));
+    launch_normalize_rows(dx.get(), n, d, nullptr);
+    dr.keys(dx.get(), n, dk.get(), nullptr, nullptr);
+    LSH_HIP(hipDeviceSynchronize());
+    LSH_HIP(hipMemcpy(keys, dk.get(), sizeof(uint32_t) * (size_t)n * tables, hipMemcpyDeviceToHost));
+}
+
+void sort_tables_host(const uint32_t* keys_ln, int64_t N, int L, int bits, uint32_t* skeys, uint32_t* ids,
+                      int device) {
+    DeviceGuard g(device);
+    LSH_REQUIRE(N >= 0 && L >= 1 && bits >= 1 && bits <= 32, LSH_ERR_INVALID, "bad table sort arguments");
+    if (N == 0) return;
+    const size_t n = (size_t)L * N;
+    DevBuf<uint32_t> in(n), ok(n), oi(n), tk(n), ti(n), counts(sort_table_counts(N, L));
+    LSH_HIP(hipMemcpy(in.get(), keys_ln, sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    launch_sort_tables(in.get(), N, L, bits, ok.get(), oi.get(), tk.get(), ti.get(), counts.get(), nullptr);
+    LSH_HIP(hipDeviceSynchronize());
+    LSH_HIP(hipMemcpy(skeys, ok.get(), sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    LSH_HIP(hipMemcpy(ids, oi.get(), sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
 }
 
 void crosspoly_host(const float* x, int64_t n, int d, int tables, const int32_t* signs, uint16_t* keys,

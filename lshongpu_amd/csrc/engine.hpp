@@ -203,10 +203,14 @@ struct lsh_index {
     std::vector<lshgpu::Transform> key_steps, sketch_steps;  // preprocessing before keys / sketches
     lshgpu::DevBuf<float> kplanesT;       // SimHash key hyperplanes [tables][d][bucketKeyBits]
     lshgpu::DevBuf<int32_t> ksigns;       // cross-polytope diagonals [tables][3][P]
+    lshgpu::DenseRotations dense;         // dense-rotation cross-polytope keys (keysImplementation 8)
+    bool sorted = false;                  // tables as sorted (key, ID) runs instead of 2^bits offsets
+    int sortBits = 0;                     // key bits the sorted tables are ordered by
+    lshgpu::DevBuf<uint32_t> bkeys;       // sorted tables: [tables][N] keys in ascending order (bperm alongside)
     lshgpu::DevBuf<uint32_t> dkeys;       // [N][tables]
     lshgpu::DevBuf<uint32_t> boff;        // [tables][tableSize+1] absolute positions into bperm
     lshgpu::DevBuf<uint32_t> bperm;       // [tables][N] point IDs grouped by key
-    lshgpu::DevBuf<uint32_t> qkeys, qlo, qpre, qmax;
+    lshgpu::DevBuf<uint32_t> qkeys, qlo, qpre, qmax, qlen;
     int64_t last_candidates = 0;          // sum over queries of bucket entries walked (last search)
 
     // Per-search scratch (grow-only).
@@ -258,6 +262,9 @@ void hamming_topc_device_noalloc(const uint32_t* ds, int64_t N, const uint32_t* 
                                  int32_t* ids, int32_t* dists, hipStream_t s);
 void crosspoly_host(const float* x, int64_t n, int d, int tables, const int32_t* signs, uint16_t* keys,
                     int device);
+void crosspoly_dense_host(const float* x, int64_t n, int d, int tables, int K, uint32_t* keys, int device);
+void sort_tables_host(const uint32_t* keys_ln, int64_t N, int L, int bits, uint32_t* skeys, uint32_t* ids,
+                      int device);
 
 // Statistics (stats.hip).  summary4: buckets, non-empty buckets, points, largest bucket.
 void byte_distribution_host(const uint8_t* hashes, int64_t n, int64_t* counts, int device);

@@ -138,6 +138,142 @@ __global__ void bucket_scatter_kernel(const uint32_t* __restrict__ keys, int64_t
     }
 }
 
+// ------------------------------------------------ sorted tables (radix sort)
+// One pass of a stable LSD radix sort over 8-bit digits, all tables at once
+// (grid.y = table).  Tiles of kSortTile elements: the histogram kernel counts
+// each tile's digits into counts[t][digit][tile]; bucket_offsets_kernel turns
+// each table's row of counts (digit-major, so digit d of tile i follows every
+// smaller digit and digit d of every earlier tile) into absolute positions;
+// the scatter kernel ranks each element among the equal digits before it in
+// its tile, by 64-lane match masks (eight ballots) per wave and per-wave
+// counts in LDS, in element order, so the sort is stable.
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kTPB * kSortItems;
+
+__global__ __launch_bounds__(kTPB) void radix_hist_kernel(const uint32_t* __restrict__ keys, int64_t N, int shift,
+                                                          int ntiles, uint32_t* __restrict__ counts) {
+    __shared__ uint32_t hist[256];
+    const int t = blockIdx.y;
+    hist[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t* __restrict__ k = keys + (size_t)t * N;
+    const int64_t base = (int64_t)blockIdx.x * kSortTile;
+#pragma unroll
+    for (int it = 0; it < kSortItems; ++it) {
+        const int64_t e = base + it * kTPB + threadIdx.x;
+        if (e < N) atomicAdd(&hist[(k[e] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    counts[((size_t)t * 256 + threadIdx.x) * ntiles + blockIdx.x] = hist[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kTPB) void radix_scatter_kernel(const uint32_t* __restrict__ keys,
+                                                             const uint32_t* __restrict__ ids, int64_t N, int shift,
+                                                             int ntiles, const uint32_t* __restrict__ counts,
+                                                             uint32_t* __restrict__ okeys,
+                                                             uint32_t* __restrict__ oids) {
+    __shared__ uint32_t gofs[256];
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[kTPB / 64][256];
+    const int t = blockIdx.y;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    gofs[threadIdx.x] = counts[((size_t)t * 256 + threadIdx.x) * ntiles + blockIdx.x];
+    run[threadIdx.x] = 0;
+    const uint32_t* __restrict__ k = keys + (size_t)t * N;
+    const uint32_t* __restrict__ id = ids ? ids + (size_t)t * N : nullptr;
+    const int64_t base = (int64_t)blockIdx.x * kSortTile;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (int it = 0; it < kSortItems; ++it) {
+#pragma unroll
+        for (int ww = 0; ww < kTPB / 64; ++ww) wcnt[ww][threadIdx.x] = 0;
+        const int64_t e = base + it * kTPB + threadIdx.x;
+        const bool valid = e < N;
+        const uint32_t key = valid ? k[e] : 0u;
+        const uint32_t pid = valid ? (id ? id[e] : (uint32_t)e) : 0u;
+        const uint32_t dig = (key >> shift) & 255u;
+        uint64_t m = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            const bool on = (dig >> bit) & 1u;
+            const uint64_t bb = __ballot(on);
+            m &= on ? bb : ~bb;
+        }
+        const uint32_t rank = (uint32_t)__popcll(m & lt);
+        __syncthreads();  // wcnt cleared; the previous round's reads of run are done
+        if (valid && rank == 0) wcnt[w][dig] = (uint32_t)__popcll(m);
+        __syncthreads();
+        {
+            uint32_t r = run[threadIdx.x];
+#pragma unroll
+            for (int ww = 0; ww < kTPB / 64; ++ww) {
+                const uint32_t c = wcnt[ww][threadIdx.x];
+                wcnt[ww][threadIdx.x] = r;
+                r += c;
+            }
+            run[threadIdx.x] = r;
+        }
+        __syncthreads();
+        if (valid) {
+            const uint32_t dst = gofs[dig] + wcnt[w][dig] + rank;
+            okeys[dst] = key;
+            oids[dst] = pid;
+        }
+        __syncthreads();  // wcnt is cleared again next round
+    }
+}
+
+__global__ void transpose_keys_kernel(const uint32_t* __restrict__ nl, int64_t n, int L, uint32_t* __restrict__ ln) {
+    const size_t total = (size_t)n * L;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t t = i / (size_t)n, r = i - t * (size_t)n;
+        ln[i] = nl[r * L + t];
+    }
+}
+
+// First position of a value >= key in the ascending array a[0, n).
+__device__ inline uint32_t lower_bound_u32(const uint32_t* __restrict__ a, uint32_t n, uint32_t key) {
+    uint32_t lo = 0, len = n;
+    while (len > 0) {
+        const uint32_t half = len >> 1;
+        if (a[lo + half] < key) {
+            lo += half + 1;
+            len -= half + 1;
+        } else {
+            len = half;
+        }
+    }
+    return lo;
+}
+
+// One thread per (query, table): the run of the query's key in the table.
+__global__ void sorted_ranges_kernel(const uint32_t* __restrict__ qkeys, int Q, int L, int64_t N,
+                                     const uint32_t* __restrict__ skeys, uint32_t* __restrict__ lo,
+                                     uint32_t* __restrict__ len) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)Q * L) return;
+    const int t = (int)(i % L);
+    const uint32_t key = qkeys[i];
+    const uint32_t* __restrict__ a = skeys + (size_t)t * N;
+    const uint32_t b = lower_bound_u32(a, (uint32_t)N, key);
+    const uint32_t e = key == 0xFFFFFFFFu ? (uint32_t)N : lower_bound_u32(a, (uint32_t)N, key + 1u);
+    lo[i] = (uint32_t)((int64_t)t * N) + b;
+    len[i] = e - b;
+}
+
+// Prefix sums of the run lengths per query.
+__global__ void ranges_prefix_kernel(const uint32_t* __restrict__ len, int Q, int L, uint32_t* __restrict__ pre,
+                                     uint32_t* __restrict__ maxS) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Q) return;
+    uint32_t acc = 0;
+    pre[(size_t)q * (L + 1)] = 0;
+    for (int t = 0; t < L; ++t) {
+        acc += len[(size_t)q * L + t];
+        pre[(size_t)q * (L + 1) + t + 1] = acc;
+    }
+    atomicMax(maxS, acc);
+}
+
 __global__ void query_ranges_kernel(const uint32_t* __restrict__ qkeys, int Q, int L, uint32_t T,
                                     const uint32_t* __restrict__ off, uint32_t* __restrict__ lo,
                                     uint32_t* __restrict__ pre, uint32_t* __restrict__ maxS) {
@@ -349,6 +485,56 @@ void launch_query_ranges(const uint32_t* qkeys, int Q, int L, uint32_t T, const 
     hipLaunchKernelGGL(query_ranges_kernel, dim3((unsigned)((Q + 255) / 256)), dim3(256), 0, s, qkeys, Q, L, T,
                        off, lo, pre, maxS);
     check_launch("query_ranges_kernel");
+}
+
+size_t sort_table_counts(int64_t N, int L) {
+    const int64_t ntiles = std::max<int64_t>(1, (N + kSortTile - 1) / kSortTile);
+    return (size_t)L * 256 * ntiles;
+}
+
+void launch_sort_tables(const uint32_t* keys_ln, int64_t N, int L, int bits, uint32_t* skeys, uint32_t* perm,
+                        uint32_t* tk, uint32_t* ti, uint32_t* counts, hipStream_t s) {
+    if (N <= 0 || L <= 0) return;
+    LSH_REQUIRE(bits >= 1 && bits <= 32, LSH_ERR_INTERNAL, "sort: key bits must be 1..32");
+    LSH_REQUIRE((int64_t)L * N < 0xFFFFFFFFll, LSH_ERR_UNSUPPORTED, "tables * N_data must be below 2^32");
+    const int ntiles = (int)((N + kSortTile - 1) / kSortTile);
+    LSH_REQUIRE(ntiles <= 65535 * 64, LSH_ERR_UNSUPPORTED, "too many points for the table sort");
+    const int passes = (bits + 7) / 8;
+    const uint32_t T1 = 256u * (uint32_t)ntiles;
+    for (int p = 0; p < passes; ++p) {
+        // The last pass writes skeys/perm; earlier ones alternate so that it does.
+        const bool to_out = ((passes - 1 - p) & 1) == 0;
+        const uint32_t* ik = p == 0 ? keys_ln : (to_out ? tk : skeys);
+        const uint32_t* ii = p == 0 ? nullptr : (to_out ? ti : perm);
+        uint32_t* ok = to_out ? skeys : tk;
+        uint32_t* oi = to_out ? perm : ti;
+        const dim3 grid((unsigned)ntiles, (unsigned)L);
+        hipLaunchKernelGGL(radix_hist_kernel, grid, dim3(kTPB), 0, s, ik, N, 8 * p, ntiles, counts);
+        check_launch("radix_hist_kernel");
+        // Exclusive scan of each table's counts plus the table base t*N.
+        hipLaunchKernelGGL(bucket_offsets_kernel, dim3((unsigned)L), dim3(1024), 0, s, counts, N, T1);
+        check_launch("bucket_offsets_kernel");
+        hipLaunchKernelGGL(radix_scatter_kernel, grid, dim3(kTPB), 0, s, ik, ii, N, 8 * p, ntiles, counts, ok, oi);
+        check_launch("radix_scatter_kernel");
+    }
+}
+
+void launch_query_ranges_sorted(const uint32_t* qkeys, int Q, int L, int64_t N, const uint32_t* skeys,
+                                uint32_t* lo, uint32_t* len, uint32_t* pre, uint32_t* maxS, hipStream_t s) {
+    if (Q <= 0) return;
+    const size_t n = (size_t)Q * L;
+    hipLaunchKernelGGL(sorted_ranges_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, qkeys, Q, L, N,
+                       skeys, lo, len);
+    check_launch("sorted_ranges_kernel");
+    hipLaunchKernelGGL(ranges_prefix_kernel, dim3((unsigned)((Q + 255) / 256)), dim3(256), 0, s, len, Q, L, pre,
+                       maxS);
+    check_launch("ranges_prefix_kernel");
+}
+
+void launch_transpose_keys(const uint32_t* nl, int64_t n, int L, uint32_t* ln, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(transpose_keys_kernel, dim3(grid_for((size_t)n * L, 256)), dim3(256), 0, s, nl, n, L, ln);
+    check_launch("transpose_keys_kernel");
 }
 
 void launch_lsh_scan(const LshScanArgs& a, const TopKView& v, hipStream_t s) {

@@ -354,6 +354,21 @@ void launch_pad_rows_u32(const uint32_t* src, int64_t n, int cols, int64_t npad,
                          uint32_t* dst, hipStream_t s);
 void launch_fill_i32(int32_t* p, size_t n, int32_t v, hipStream_t s);
 
+// crosspoly_dense.hip: dense-rotation cross-polytope keys (keysImplementation
+// 8).  Rotation h = t*K + k is P x d (P = d rounded up to a power of two),
+// its entries the reference normal stream rounded to bfloat16; a table's key
+// packs K hashes into b = log2(2P)-bit fields.  keys() writes [n][L] and/or
+// [L][n] (either may be null) for unit rows x [n][d].  d <= 128.
+int dense_key_field_bits(int d);
+struct DenseRotations {
+    int d = 0, P = 0, dpad = 0, K = 0, L = 0, H = 0, b = 0;
+    DevBuf<uint16_t> G;   // [H][P][dpad] bfloat16 bits
+    DevBuf<float> gmax;   // [H] largest row norm, rounded up
+    void build(int d, int tables, int K, hipStream_t s);
+    void keys(const float* x, int64_t n, uint32_t* keys_nl, uint32_t* keys_ln, hipStream_t s) const;
+    size_t bytes() const { return G.bytes() + gmax.bytes(); }
+};
+
 // lsh.hip: (K,L) bucket tables and the candidate scan.
 // keys [n][L] -> keys & mask (in place); u16 keys widened to u32.
 void launch_mask_keys(uint32_t* keys, size_t n, uint32_t mask, hipStream_t s);
@@ -369,6 +384,20 @@ void launch_bucket_build(const uint32_t* keys, int64_t N, int L, uint32_t T, uin
 // (maxS must be zero on entry).
 void launch_query_ranges(const uint32_t* qkeys, int Q, int L, uint32_t T, const uint32_t* off, uint32_t* lo,
                          uint32_t* pre, uint32_t* maxS, hipStream_t s);
+// Sorted tables (any key width): a stable LSD radix sort (8-bit digits over
+// the low `bits` bits) of every table's keys keys_ln [L][N] with their point
+// IDs: sorted keys skeys [L][N] and IDs perm [L][N] (equal keys in ID order).
+// tk, ti: [L][N] scratch; counts: sort_table_counts(N, L) words.
+size_t sort_table_counts(int64_t N, int L);
+void launch_sort_tables(const uint32_t* keys_ln, int64_t N, int L, int bits, uint32_t* skeys, uint32_t* perm,
+                        uint32_t* tk, uint32_t* ti, uint32_t* counts, hipStream_t s);
+// launch_query_ranges over sorted tables: the bucket of a key is its run in
+// skeys (binary search).  lo holds absolute positions t*N + start; len
+// ([Q][L]) receives the run lengths.
+void launch_query_ranges_sorted(const uint32_t* qkeys, int Q, int L, int64_t N, const uint32_t* skeys,
+                                uint32_t* lo, uint32_t* len, uint32_t* pre, uint32_t* maxS, hipStream_t s);
+// [n][L] -> [L][n]
+void launch_transpose_keys(const uint32_t* nl, int64_t n, int L, uint32_t* ln, hipStream_t s);
 
 // Candidate positions [p0,p1) of every query row's concatenated L buckets;
 // a point is scored only under the first table in which it shares the

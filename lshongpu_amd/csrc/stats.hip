@@ -230,6 +230,59 @@ __global__ __launch_bounds__(kTPB) void bucket_occupancy_kernel(const uint32_t* 
     }
 }
 
+// The same figures for a sorted table (keys ascending, N of them): each run
+// of equal keys is one non-empty bucket, found at its first position; its
+// size comes from a binary search for the run's end.
+template <bool kLds>
+__global__ __launch_bounds__(kTPB) void sorted_occupancy_kernel(const uint32_t* __restrict__ k, uint32_t N,
+                                                                int hist_len, unsigned long long* __restrict__ hist,
+                                                                unsigned long long* __restrict__ summary) {
+    extern __shared__ uint32_t lhist[];
+    if (kLds) {
+        for (int i = threadIdx.x; i < hist_len; i += kTPB) lhist[i] = 0;
+        __syncthreads();
+    }
+    uint32_t nonempty = 0, maxsz = 0;
+    for (uint32_t i = blockIdx.x * kTPB + threadIdx.x; i < N; i += gridDim.x * kTPB) {
+        const uint32_t key = k[i];
+        if (i > 0 && k[i - 1] == key) continue;
+        uint32_t lo = i + 1, len = N - lo;  // first position past the run
+        while (len > 0) {
+            const uint32_t half = len >> 1;
+            if (k[lo + half] == key) {
+                lo += half + 1;
+                len -= half + 1;
+            } else {
+                len = half;
+            }
+        }
+        const uint32_t sz = lo - i;
+        ++nonempty;
+        maxsz = sz > maxsz ? sz : maxsz;
+        if (hist_len > 0) {
+            const uint32_t bin = sz < (uint32_t)(hist_len - 1) ? sz : (uint32_t)(hist_len - 1);
+            if (kLds)
+                atomicAdd(&lhist[bin], 1u);
+            else
+                atomicAdd(&hist[bin], 1ull);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        nonempty += __shfl_xor(nonempty, o);
+        const uint32_t m = __shfl_xor(maxsz, o);
+        maxsz = m > maxsz ? m : maxsz;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (nonempty) atomicAdd(&summary[0], (unsigned long long)nonempty);
+        atomicMax(&summary[1], (unsigned long long)maxsz);
+    }
+    if (kLds) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < hist_len; i += kTPB)
+            if (lhist[i]) atomicAdd(&hist[i], (unsigned long long)lhist[i]);
+    }
+}
+
 struct DeviceGuard {
     int prev = 0;
     explicit DeviceGuard(int dev) {
@@ -308,12 +361,41 @@ void full_key_distribution_host(const uint8_t* hashes, int64_t n, int sd, std::v
 }
 
 void index_bucket_stats(lsh_index* idx, int table, int64_t* summary4, int64_t* hist, int hist_len) {
-    LSH_REQUIRE(idx->framework == 1 && idx->boff.get(), LSH_ERR_INVALID,
+    LSH_REQUIRE(idx->framework == 1 && (idx->boff.get() || idx->bkeys.get()), LSH_ERR_INVALID,
                 "bucket statistics need an index built with framework 1");
     LSH_REQUIRE(table >= 0 && table < idx->tables, LSH_ERR_INVALID, "table out of range");
     LSH_REQUIRE(hist_len >= 0 && (hist || hist_len == 0), LSH_ERR_INVALID, "bad histogram buffer");
     DeviceGuard g(idx->device);
     hipStream_t s = idx->stream;
+    if (idx->sorted) {
+        const uint32_t N = (uint32_t)idx->N;
+        const uint32_t* k = idx->bkeys.get() + (size_t)table * N;
+        DevBuf<unsigned long long> dh((size_t)hist_len + 2);
+        LSH_HIP(hipMemsetAsync(dh.get(), 0, sizeof(unsigned long long) * ((size_t)hist_len + 2), s));
+        unsigned long long* summary = dh.get() + hist_len;
+        const unsigned grid = blocks_for(N, kTPB * 4, 2048);
+        if (N > 0) {
+            if (hist_len > 0 && hist_len <= kLdsBins)
+                hipLaunchKernelGGL(sorted_occupancy_kernel<true>, dim3(grid), dim3(kTPB), sizeof(uint32_t) * hist_len,
+                                   s, k, N, hist_len, dh.get(), summary);
+            else
+                hipLaunchKernelGGL(sorted_occupancy_kernel<false>, dim3(grid), dim3(kTPB), 0, s, k, N, hist_len,
+                                   dh.get(), summary);
+            check_launch("sorted_occupancy_kernel");
+        }
+        std::vector<unsigned long long> h((size_t)hist_len + 2);
+        LSH_HIP(hipMemcpyAsync(h.data(), dh.get(), sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost, s));
+        LSH_HIP(hipStreamSynchronize(s));
+        const int64_t buckets = (int64_t)1 << idx->sortBits;
+        const int64_t nonempty = (int64_t)h[hist_len];
+        for (int i = 0; i < hist_len; ++i) hist[i] = (int64_t)h[i];
+        if (hist_len > 0) hist[0] += buckets - nonempty;  // empty buckets
+        summary4[0] = buckets;
+        summary4[1] = nonempty;
+        summary4[2] = (int64_t)N;
+        summary4[3] = (int64_t)h[hist_len + 1];
+        return;
+    }
     const uint32_t T = idx->tableSize;
     const uint32_t* off = idx->boff.get() + (size_t)table * (T + 1);
     DevBuf<unsigned long long> dh((size_t)hist_len + 2);

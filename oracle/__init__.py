@@ -56,6 +56,7 @@ def load() -> C.CDLL:
         "orc_hamming": (C.c_int, [_U32, _U32, C.c_int]),
         "orc_sketch_topc": (None, [C.c_int, _U32, C.c_int64, _U32, C.c_int64, C.c_int, C.c_int, _I32, _I32]),
         "orc_jl_topc": (None, [_FP, C.c_int64, _FP, C.c_int64, C.c_int, C.c_int, _I32]),
+        "orc_crosspoly_dense": (None, [_FP, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int, _FP, _U32]),
         "orc_rerank": (None, [_FP, _FP, C.c_int64, C.c_int, C.c_int, _I32, C.c_int, C.c_int, _I32, _FP]),
         "orc_recall": (C.c_float, [_I32, C.c_int, _I32, C.c_int, C.c_int64, C.c_int]),
         "orc_distance_ratio": (C.c_float, [_FP, C.c_int, _I32, _FP, C.c_int, C.c_int64, C.c_int]),
@@ -213,6 +214,41 @@ def crosspoly(x_normalized, tables: int, signs):
     return keys
 
 
+def pow2_at_least(d: int) -> int:
+    P = 1
+    while P < d:
+        P *= 2
+    return P
+
+
+def bf16_round(x):
+    """float32 values rounded to the nearest bfloat16 (ties to even), as float32."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return r.astype(np.uint32).view(np.float32)
+
+
+def dense_rotations(tables: int, K: int, P: int, d: int):
+    """[tables*K][P][d] Gaussian rotations of keysImplementation 8: the
+    reference's default-seeded normal stream (randomVectorGenerator.cpp:32-53),
+    each value rounded to bfloat16 so that it is exact on the matrix cores."""
+    return bf16_round(normal_stream(tables * K * P * d)).reshape(tables * K, P, d)
+
+
+def crosspoly_dense(x_normalized, tables: int, K: int, G=None):
+    """Keys [n][tables] (uint32) of keysImplementation 8 (orc_crosspoly_dense)."""
+    x, xp = _f(x_normalized)
+    n, d = x.shape
+    P = pow2_at_least(d)
+    if G is None:
+        G = dense_rotations(tables, K, P, d)
+    G, gp = _f(G)
+    assert G.shape == (tables * K, P, d)
+    keys = np.empty((n, tables), dtype=np.uint32)
+    load().orc_crosspoly_dense(xp, n, d, P, tables, K, gp, keys.ctypes.data_as(_U32))
+    return keys
+
+
 def philox4x32_10(ctr, key):
     c, cp = _u(ctr)
     k, kp = _u(key)
@@ -276,11 +312,13 @@ def lsh_keys(x, keys_impl: int, tables: int, bits: int, wmh=None):
             return wmh_sketch(x, bounds, start, M, tables, 0, bit_stream(4096)) & np.uint32(0xFF)
         return wmh_sketch(x, bounds, start, M, tables, 1, bit_stream(4096), hbits=bits) & np.uint32(0xFFFF)
     if keys_impl == 7:
-        P = 1
-        while P < d:
-            P *= 2
+        P = pow2_at_least(d)
         return crosspoly(x, tables, sign_stream(tables * 3 * P)).astype(np.uint32)
-    raise ValueError("keys implementation must be 3, 4, 5 or 7")
+    if keys_impl == 8:
+        b = int(np.log2(2 * pow2_at_least(d)))
+        assert bits % b == 0
+        return crosspoly_dense(x, tables, bits // b)
+    raise ValueError("keys implementation must be 3, 4, 5, 7 or 8")
 
 
 def lsh_search(data, queries, k: int, implementation: int, keys_impl: int, sketch_dim: int, func: int,
@@ -300,7 +338,7 @@ def lsh_search(data, queries, k: int, implementation: int, keys_impl: int, sketc
         nonlocal data, queries
         if family in (4, 5):
             data, queries = gj_transform(data, queries)
-        elif family in (6, 7):
+        elif family in (6, 7, 8):
             data, queries = normalize_rows(data), normalize_rows(queries)
 
     preprocess(keys_impl)

@@ -421,6 +421,43 @@ EXPORT int orc_pow2_at_least(int d) {
 // v'[i] = v[i] + v[i^o] when i < i^o, else -v[i] + v[i^o]); the key is the
 // first index of the largest |v| (strictly greater wins), i for negative v and
 // i + P otherwise.
+// Dense-rotation cross-polytope keys (keysImplementation 8; the reference has
+// no counterpart: it is the hash of crossPolyTope.cuh:64-76 applied to a dense
+// Gaussian rotation instead of the HD^3 transform).  Hash h of row x:
+// y_j = sequential FP32 fmaf chain over i < d of G[h][j][i] * x[i] from 0,
+// j < P; the index is the first j of largest |y_j| by the reference's
+// comparison (crossPolyTope.cuh:66-75), encoded j when y_j < 0, else j + P.
+// Table t concatenates hashes h = t*K .. t*K+K-1: key = sum_k hash << (k*b),
+// b = log2(2P).  G is [tables*K][P][d].
+EXPORT void orc_crosspoly_dense(const float* x, int64_t n, int d, int P, int tables, int K,
+                                const float* G, uint32_t* keys) {
+    int b = 0;
+    while ((1 << b) < 2 * P) ++b;
+    parallel_for(n, [&](int64_t i) {
+        const float* r = x + i * d;
+        for (int t = 0; t < tables; ++t) {
+            uint32_t key = 0;
+            for (int k = 0; k < K; ++k) {
+                const float* g = G + (int64_t)(t * K + k) * P * d;
+                int index = P;  // no finite value: j = 0, non-negative
+                float best = (float)INT32_MIN;
+                for (int j = 0; j < P; ++j) {
+                    float y = 0.0f;
+                    for (int q = 0; q < d; ++q) y = std::fmaf(g[(int64_t)j * d + q], r[q], y);
+                    const bool neg = y < 0;
+                    const float m = neg ? std::fmax(best, -y) : std::fmax(best, y);
+                    if (m > best) {
+                        index = neg ? j : j + P;
+                        best = m;
+                    }
+                }
+                key |= (uint32_t)index << (k * b);
+            }
+            keys[i * tables + t] = key;
+        }
+    });
+}
+
 EXPORT void orc_crosspoly(const float* x, int64_t n, int d, int tables,
                           const int32_t* signs, uint16_t* keys) {
     const int P = orc_pow2_at_least(d);

@@ -657,6 +657,44 @@ def test_wmh_pipeline_matches_oracle(lsh, orc, impl):
     _close(res.distances, dist)
 
 
+# ------------------------------------------- dense-rotation cross-polytope
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("d,tables,K", [(128, 4, 2), (100, 3, 2), (128, 2, 4), (17, 2, 3), (64, 3, 1), (5, 2, 2)])
+def test_crosspoly_dense_keys_bit_exact(lsh, orc, d, tables, K):
+    # Matrix-core products with the FP32 recheck of every ambiguous hash
+    # against the oracle's FP32 definition, bit for bit; duplicates, a zero
+    # row (NaN after normalisation: every coordinate rechecked) and rows that
+    # differ in the last bit of one coordinate.
+    rng = np.random.default_rng(d * 10 + tables + K)
+    n = 5003
+    x = rng.standard_normal((n, d)).astype(np.float32)
+    x[10:20] = x[0]
+    x[20] = 0.0
+    x[21:40] = x[1]
+    x[21:40, 0] = np.nextafter(x[21:40, 0], np.float32(np.inf))
+    keys = lsh.crosspoly_dense_keys(x, tables, K)
+    want = orc.crosspoly_dense(orc.normalize_rows(x), tables, K)
+    np.testing.assert_array_equal(keys, want)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("N,L,bits", [(5000, 3, 8), (70001, 2, 16), (4099, 4, 24), (33000, 2, 32), (1, 1, 5)])
+def test_sort_table_keys_stable(lsh, N, L, bits):
+    rng = np.random.default_rng(N + bits)
+    hi = min(2 ** bits, 2 ** 32)
+    keys = rng.integers(0, max(2, hi // 64 if bits > 8 else hi), size=(L, N), dtype=np.uint64).astype(np.uint32)
+    keys[:, ::7] = keys[:, :1]
+    if bits < 32:
+        keys[0, ::5] |= np.uint32(1) << np.uint32(bits)  # bits above the sorted ones keep their order
+    sk, ids = lsh.sort_table_keys(keys, bits)
+    mask = np.uint32((1 << bits) - 1) if bits < 32 else np.uint32(0xFFFFFFFF)
+    for t in range(L):
+        order = np.argsort(keys[t] & mask, kind="stable")
+        np.testing.assert_array_equal(ids[t], order.astype(np.uint32))
+        np.testing.assert_array_equal(sk[t], keys[t][order])
+
+
 # ------------------------------------------------------------ (K,L) framework
 
 LSH_CASES = [
@@ -669,6 +707,11 @@ LSH_CASES = [
     (3, 3, True, 1, 12, 10, 32, "unit"),
     (4, 4, True, 2, 8, 6, 32, "wmh"),           # both families shift and normalise
     (5, 7, True, 2, 8, 4, 32, "wmh"),
+    (2, 8, False, 1, 14, 6, 32, "unit"),        # dense-rotation keys, K = 2 (P = 64, 7-bit fields)
+    (3, 8, True, 1, 14, 8, 32, "unit"),
+    (2, 8, False, 3, 7, 10, 100, "unit"),       # K = 1: 128 buckets per table, large candidate sets
+    (2, 3, False, 1, 20, 5, 32, "unit"),        # wider than 16 bits: sorted tables for a reference family
+    (2, 7, False, 1, 18, 4, 32, "unit"),
     (6, 3, True, 3, 10, 6, 32, "unit"),
 ]
 

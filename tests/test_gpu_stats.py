@@ -46,17 +46,21 @@ def test_bucket_distribution_full_key_empty(lsh):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("keys,bits,tables,hist_len", [(3, 10, 4, 0), (3, 14, 3, 40), (7, 6, 5, 20000)])
+@pytest.mark.parametrize("keys,bits,tables,hist_len", [(3, 10, 4, 0), (3, 14, 3, 40), (7, 6, 5, 20000),
+                                                      (8, 12, 3, 50), (3, 20, 2, 30)])
 def test_index_bucket_stats(lsh, orc, keys, bits, tables, hist_len):
+    # (8, 12): dense-rotation keys in sorted tables; (3, 20): SimHash keys
+    # wider than 16 bits, also sorted (the 16-bit key type keeps 2^16 buckets).
     rng = np.random.default_rng(bits)
     data = rng.standard_normal((30000, 24)).astype(np.float32)
     with lsh.Index(data, implementation=2, framework=1, keysImplementation=keys, bucketKeyBits=bits,
                    tables=tables) as ix:
-        x = orc.normalize_rows(data) if keys == 7 else data
+        x = orc.normalize_rows(data) if keys in (7, 8) else data
         dk = orc.lsh_keys(x, keys, tables, bits)
+        buckets = 2 ** (min(bits, 16) if keys == 3 else bits)
         for t in range(tables):
             s, h = ix.bucket_stats(t, hist_len)
-            os_, oh = orc.bucket_occupancy(dk[:, t], 2 ** bits, hist_len)
+            os_, oh = orc.bucket_occupancy(dk[:, t], buckets, hist_len)
             assert s == os_
             np.testing.assert_array_equal(h, oh)
         with pytest.raises(lsh.LshError):

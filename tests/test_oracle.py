@@ -288,6 +288,78 @@ def test_jl_topc_unfused_squares_and_id_ties(orc):
     assert (few[0, 10:] == -1).all() and sorted(few[0, :10]) == list(range(10))
 
 
+# ------------------------------------------- dense-rotation cross-polytope
+
+def test_bf16_round_matches_torch(orc):
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(50)
+    x = np.concatenate([rng.standard_normal(20000).astype(np.float32),
+                        np.array([1 + 2 ** -8, 1 + 3 * 2 ** -8, -(1 + 2 ** -8), 0.0, -0.0], np.float32)])
+    want = torch.from_numpy(x).to(torch.bfloat16).to(torch.float32).numpy()
+    np.testing.assert_array_equal(orc.bf16_round(x), want)
+    G = orc.dense_rotations(2, 2, 8, 5)
+    np.testing.assert_array_equal(G.ravel(), orc.bf16_round(orc.normal_stream(2 * 2 * 8 * 5)))
+
+
+def _dense_hash_rule(y):
+    """crossPolyTope.cuh:64-76 over the last axis: first j of largest |y_j|,
+    encoded j (negative) or j + P."""
+    P = y.shape[-1]
+    a = np.abs(y)
+    j = np.argmax(a, axis=-1)  # first maximum
+    neg = np.take_along_axis(y, j[..., None], -1)[..., 0] < 0
+    return np.where(neg, j, j + P).astype(np.uint32)
+
+
+@pytest.mark.parametrize("d,tables,K", [(16, 3, 2), (12, 2, 3), (40, 2, 1)])
+def test_crosspoly_dense_is_the_fma_chain_argmax(orc, d, tables, K):
+    # Every hash equals the reference's argmax rule applied to the sequential
+    # FP32 fmaf chain of each rotation row (exact emulation), packed K to a
+    # key in log2(2P)-bit fields.
+    from _fp32 import dot_fma_chain
+
+    rng = np.random.default_rng(51 + d)
+    x = rng.standard_normal((70, d)).astype(np.float32)
+    x /= np.linalg.norm(x, axis=1, keepdims=True)
+    x[5] = x[4]  # duplicates hash alike
+    P = orc.pow2_at_least(d)
+    b = int(np.log2(2 * P))
+    G = orc.dense_rotations(tables, K, P, d)
+    keys = orc.crosspoly_dense(x, tables, K, G)
+    for t in range(tables):
+        want = np.zeros(len(x), np.uint64)
+        for k in range(K):
+            y = dot_fma_chain(x, G[t * K + k])
+            want |= _dense_hash_rule(y).astype(np.uint64) << np.uint64(k * b)
+        np.testing.assert_array_equal(keys[:, t], want.astype(np.uint32))
+    assert (keys < (1 << (b * K))).all() if b * K < 32 else True
+    np.testing.assert_array_equal(keys[4], keys[5])
+
+
+def test_crosspoly_dense_agrees_with_fp64_away_from_ties(orc):
+    # On hashes whose top two |y| differ by more than twice the FP32 error
+    # bound, the argmax of the FP64 product decides (a transposed rotation or a
+    # wrong stride would disagree on most of them).
+    rng = np.random.default_rng(52)
+    d, tables, K = 100, 4, 2
+    x = rng.standard_normal((300, d)).astype(np.float32)
+    x /= np.linalg.norm(x, axis=1, keepdims=True)
+    P = orc.pow2_at_least(d)
+    b = int(np.log2(2 * P))
+    G = orc.dense_rotations(tables, K, P, d)
+    keys = orc.crosspoly_dense(x, tables, K, G)
+    checked = 0
+    for h in range(tables * K):
+        y = x.astype(np.float64) @ G[h].astype(np.float64).T
+        a = np.sort(np.abs(y), axis=1)
+        bound = d * 2.0 ** -23 * np.linalg.norm(G[h].astype(np.float64), axis=1).max()
+        sure = a[:, -1] - a[:, -2] > 2 * bound
+        got = (keys[:, h // K] >> np.uint32((h % K) * b)) & np.uint32((1 << b) - 1)
+        np.testing.assert_array_equal(got[sure], _dense_hash_rule(y)[sure])
+        checked += int(sure.sum())
+    assert checked > 0.95 * 300 * tables * K
+
+
 def _popcount(a):
     a = a.astype(np.uint32)
     return np.unpackbits(a.view(np.uint8).reshape(a.shape + (4,)), axis=-1).sum(-1).sum(-1)
