@@ -331,10 +331,10 @@ def other_config(args, np, torch, lsh, synth, dev) -> int:
         kw = dict(implementation=4, sketchDim=256, distanceFunc=2, candidates=32)
         dfunc = 2
     elif cfg == 3:
-        workload = ("(K,L) framework, N=4M d=128 clustered: 32 tables of 16-bit SimHash keys (one cross-polytope key "
-                    "per table, as the reference builds them, gives only 2P = 256 buckets at d=128), exact angular "
-                    "rescan of the bucket union, query batches log-uniform in 1..4096")
-        kw = dict(implementation=2, distanceFunc=1, framework=1, keysImplementation=3, bucketKeyBits=16, tables=32)
+        workload = ("(K,L) framework, N=4M d=128 clustered: L=32 tables of K=2 dense-rotation cross-polytope hashes "
+                    "(bf16 matrix cores, FP32 recheck), keys sorted on the device, exact angular rescan of the bucket "
+                    "union, query batches log-uniform in 1..4096")
+        kw = dict(implementation=2, distanceFunc=1, framework=1, keysImplementation=8, bucketKeyBits=16, tables=32)
         dfunc = 1
     else:
         workload = "SimHash 512-bit sketch scan, top-1024 by Hamming, exact angular rerank; N=10M d=100, batch of 2,048"

@@ -391,6 +391,28 @@ typedef struct lsh_bucket_stats {
 int lsh_index_bucket_stats(lsh_index_t* index, int table, lsh_bucket_stats_t* out,
                            int64_t* hist, int hist_len);
 
+/* No counterpart in the reference (its run scripts sweep (K,L), sketch sizes
+ * and queue lengths by hand, LinearScan/run.sh).  Builds each of the
+ * n_candidates configurations over `data` (all with the same d, N_data and
+ * device), searches the n_queries host queries with k, and measures recall@k
+ * against the exact scan of candidates[0].distanceFunc (1..3; angular
+ * otherwise) and the median host time of three searches.  *chosen = the
+ * fastest candidate whose recall reaches target_recall (the most accurate one
+ * when none does).  reports (may be NULL) receives one entry per candidate;
+ * status is the candidate's build/search status (0 = measured).  The choice
+ * is cached for the process per (data, queries, k, target, candidate list). */
+typedef struct lsh_tune_report {
+    double recall;
+    double search_ms;
+    double build_ms;
+    int64_t device_bytes;
+    int status;
+} lsh_tune_report_t;
+
+int lsh_autotune(const lsh_index_config_t* candidates, int n_candidates, const float* data, int data_on_device,
+                 const float* queries, int n_queries, int k, double target_recall, lsh_tune_report_t* reports,
+                 int* chosen);
+
 /* Device bytes held by the index. */
 int64_t lsh_index_device_bytes(const lsh_index_t* index);
 

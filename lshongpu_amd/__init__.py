@@ -28,7 +28,7 @@ __all__ = [
     "LshError", "Result", "Index", "POINT_DTYPE", "load", "calculateK", "linearScans", "LSH",
     "linear_scans", "lsh_framework", "random_normals", "random_signs", "simhash_sketch",
     "jl_sketch", "gj_transform", "hamming_topc", "wmh_sketch", "crosspoly_keys", "crosspoly_dense_keys",
-    "sort_table_keys", "library_path",
+    "sort_table_keys", "autotune", "library_path",
     "read_text_dataset", "read_fvecs", "write_text_dataset", "read_truth", "write_truth", "validate",
     "write_results", "write_performance",
 ]
@@ -95,6 +95,11 @@ class _IndexConfig(C.Structure):
 
 
 _FP = C.POINTER(C.c_float)
+class _TuneReport(C.Structure):
+    _fields_ = [("recall", C.c_double), ("search_ms", C.c_double), ("build_ms", C.c_double),
+                ("device_bytes", C.c_int64), ("status", C.c_int)]
+
+
 class _BucketStats(C.Structure):
     _fields_ = [("buckets", C.c_int64), ("nonempty", C.c_int64), ("points", C.c_int64), ("max_size", C.c_int64)]
 
@@ -155,6 +160,8 @@ _SIGS = {
     "lsh_index_bucket_stats": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(_BucketStats), C.POINTER(C.c_int64),
                                          C.c_int]),
     "lsh_index_device_bytes": (C.c_int64, [C.c_void_p]),
+    "lsh_autotune": (C.c_int, [C.POINTER(_IndexConfig), C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int,
+                               C.c_double, C.POINTER(_TuneReport), C.POINTER(C.c_int)]),
     "lsh_index_destroy": (C.c_int, [C.c_void_p]),
 }
 
@@ -431,6 +438,52 @@ def bucket_distribution_full_key(hashes, sketchDim: int | None = None):
 
 # ------------------------------------------------------------------ resident index
 
+def _fill_config(cfg, n, d, *, implementation=2, distanceFunc=1, sketchDim=0, framework=0, keysImplementation=0,
+                 bucketKeyBits=0, tables=0, runWithSketchedData=False, candidates=None, device=0,
+                 rescan_margin=None) -> None:
+    cfg.implementation = int(implementation)
+    cfg.distanceFunc = int(distanceFunc)
+    cfg.d = int(d)
+    cfg.N_data = int(n)
+    cfg.sketchDim = int(sketchDim)
+    cfg.framework = int(framework)
+    cfg.keysImplementation = int(keysImplementation)
+    cfg.bucketKeyBits = int(bucketKeyBits)
+    cfg.tables = int(tables)
+    cfg.runWithSketchedData = int(bool(runWithSketchedData))
+    if candidates is not None:
+        cfg.candidates = int(candidates)
+    if rescan_margin is not None:
+        cfg.rescan_margin = int(rescan_margin)
+    cfg.device = int(device)
+
+
+def autotune(data, queries, k: int, candidates, target_recall: float = 0.9):
+    """Measures each candidate configuration (a list of Index keyword dicts)
+    on `data` (host array or device tensor) with the host query sample, and
+    returns (index of the chosen one, list of reports: recall, search_ms,
+    build_ms, device_bytes, status).  The fastest candidate reaching
+    target_recall is chosen (lsh_autotune)."""
+    lib = load()
+    queries = _f32(queries)
+    if hasattr(data, "data_ptr"):
+        _check_tensor(data, "data", "float32")
+        n, d = data.shape
+    else:
+        data = _f32(data)
+        n, d = data.shape
+    arr = (_IndexConfig * len(candidates))()
+    for i, kw in enumerate(candidates):
+        lib.lsh_index_config_default(C.byref(arr[i]))
+        _fill_config(arr[i], n, d, **kw)
+    reps = (_TuneReport * len(candidates))()
+    chosen = C.c_int(-1)
+    dp, on_dev = _ptr_of(data)
+    _check(lib.lsh_autotune(arr, len(candidates), dp, int(on_dev), C.c_void_p(queries.ctypes.data),
+                            int(queries.shape[0]), int(k), float(target_recall), reps, C.byref(chosen)))
+    return int(chosen.value), [{f: getattr(r, f) for f, _ in _TuneReport._fields_} for r in reps]
+
+
 def _check_tensor(t, name: str, dtype) -> None:
     """A torch tensor handed to the library as a raw pointer must be what the
     kernels assume: contiguous, of the right dtype, and (for 2-D operands) 2-D."""
@@ -471,21 +524,10 @@ class Index:
             if data.ndim != 2:
                 raise LshError(1, f"data must be a 2-D (N_data, d) array, not {data.ndim}-D")
             n, d = data.shape
-        cfg.implementation = int(implementation)
-        cfg.distanceFunc = int(distanceFunc)
-        cfg.d = int(d)
-        cfg.N_data = int(n)
-        cfg.sketchDim = int(sketchDim)
-        cfg.framework = int(framework)
-        cfg.keysImplementation = int(keysImplementation)
-        cfg.bucketKeyBits = int(bucketKeyBits)
-        cfg.tables = int(tables)
-        cfg.runWithSketchedData = int(bool(runWithSketchedData))
-        if candidates is not None:
-            cfg.candidates = int(candidates)
-        if rescan_margin is not None:
-            cfg.rescan_margin = int(rescan_margin)
-        cfg.device = int(device)
+        _fill_config(cfg, n, d, implementation=implementation, distanceFunc=distanceFunc, sketchDim=sketchDim,
+                     framework=framework, keysImplementation=keysImplementation, bucketKeyBits=bucketKeyBits,
+                     tables=tables, runWithSketchedData=runWithSketchedData, candidates=candidates, device=device,
+                     rescan_margin=rescan_margin)
         self.d = int(d)
         self.n = int(n)
         self._h = C.c_void_p()

@@ -20,6 +20,7 @@ LIB = os.path.join(HERE, "liblshgpu.so")
 SOURCES = [
     "common.cpp",
     "api.cpp",
+    "autotune.cpp",
     "engine.cpp",
     "select.hip",
     "scan_exact.hip",

@@ -1,0 +1,167 @@
+// autotune.cpp -- picks an index configuration for a data set by measuring
+// the candidates: recall@k against the exact scan on a query sample, and the
+// search time of that sample.  The reference fixes its (K,L), sketch sizes
+// and queue lengths by hand in its run scripts (LinearScan/run.sh sweeps
+// them); here the fastest candidate that reaches the target recall is chosen
+// and the choice is cached per (data set, k, target, candidate list).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace {
+
+using lshgpu::guarded;
+
+struct Cached {
+    int chosen;
+    std::vector<lsh_tune_report_t> reports;
+};
+
+std::mutex g_mu;
+std::map<std::string, Cached> g_cache;
+
+uint64_t fnv(const void* p, size_t n, uint64_t h = 1469598103934665603ull) {
+    const uint8_t* b = static_cast<const uint8_t*>(p);
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+    return h;
+}
+
+// A fingerprint of the data: its first and last 64 KiB.
+uint64_t data_fingerprint(const float* data, int64_t N, int d, bool on_device) {
+    const size_t total = (size_t)N * d * sizeof(float);
+    const size_t w = std::min<size_t>(total, 65536);
+    std::vector<uint8_t> head(w), tail(w);
+    if (w > 0) {
+        const uint8_t* src = reinterpret_cast<const uint8_t*>(data);
+        const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToHost : hipMemcpyHostToHost;
+        LSH_HIP(hipMemcpy(head.data(), src, w, kind));
+        LSH_HIP(hipMemcpy(tail.data(), src + total - w, w, kind));
+    }
+    uint64_t h = fnv(head.data(), head.size());
+    h = fnv(tail.data(), tail.size(), h);
+    return fnv(&N, sizeof(N), h);
+}
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct IndexGuard {
+    lsh_index_t* ix = nullptr;
+    ~IndexGuard() {
+        if (ix) (void)lsh_index_destroy(ix);
+    }
+};
+
+}  // namespace
+
+extern "C" int lsh_autotune(const lsh_index_config_t* cands, int n_cands, const float* data, int data_on_device,
+                            const float* queries, int n_queries, int k, double target_recall,
+                            lsh_tune_report_t* reports, int* chosen) {
+    return guarded([&] {
+        LSH_REQUIRE(cands && n_cands >= 1 && data && queries && n_queries >= 1 && k >= 1 && chosen, LSH_ERR_INVALID,
+                    "bad autotune arguments");
+        const int d = cands[0].d;
+        const int64_t N = cands[0].N_data;
+        for (int i = 0; i < n_cands; ++i)
+            LSH_REQUIRE(cands[i].d == d && cands[i].N_data == N && cands[i].device == cands[0].device,
+                        LSH_ERR_INVALID, "autotune candidates must share d, N_data and device");
+        int prev = 0;
+        LSH_HIP(hipGetDevice(&prev));
+        LSH_HIP(hipSetDevice(cands[0].device));
+        struct Restore {
+            int dev;
+            ~Restore() { (void)hipSetDevice(dev); }
+        } restore{prev};
+
+        std::string key = std::to_string(data_fingerprint(data, N, d, data_on_device != 0)) + ":" +
+                          std::to_string(fnv(queries, sizeof(float) * (size_t)n_queries * d)) + ":" +
+                          std::to_string(k) + ":" + std::to_string(target_recall) + ":" +
+                          std::to_string(fnv(cands, sizeof(lsh_index_config_t) * (size_t)n_cands));
+        {
+            std::lock_guard<std::mutex> lk(g_mu);
+            auto it = g_cache.find(key);
+            if (it != g_cache.end()) {
+                *chosen = it->second.chosen;
+                if (reports) std::copy(it->second.reports.begin(), it->second.reports.end(), reports);
+                return;
+            }
+        }
+
+        const int kq = lshgpu::calculate_k(k);
+        const int kk = std::min(k, kq);
+        // The truth: the exact scan of the candidates' distance function.
+        std::vector<lsh_point_t> truth((size_t)n_queries * kq), res((size_t)n_queries * kq);
+        {
+            lsh_index_config_t ec;
+            lsh_index_config_default(&ec);
+            ec.implementation = 2;
+            ec.distanceFunc = cands[0].distanceFunc >= 1 && cands[0].distanceFunc <= 3 ? cands[0].distanceFunc : 1;
+            ec.d = d;
+            ec.N_data = N;
+            ec.device = cands[0].device;
+            IndexGuard g;
+            int st = lsh_index_create(&ec, data, data_on_device, &g.ix);
+            if (st) lshgpu::fail(st, std::string("autotune truth: ") + lsh_last_error());
+            st = lsh_index_search(g.ix, queries, n_queries, k, 0, truth.data(), 0, nullptr);
+            if (st) lshgpu::fail(st, std::string("autotune truth: ") + lsh_last_error());
+        }
+        std::vector<lsh_tune_report_t> rep((size_t)n_cands);
+        for (int i = 0; i < n_cands; ++i) {
+            lsh_tune_report_t& r = rep[i];
+            r.recall = NAN;
+            r.search_ms = NAN;
+            r.build_ms = NAN;
+            r.device_bytes = 0;
+            IndexGuard g;
+            const double t0 = now_ms();
+            r.status = lsh_index_create(&cands[i], data, data_on_device, &g.ix);
+            if (r.status) continue;
+            LSH_HIP(hipDeviceSynchronize());
+            r.build_ms = now_ms() - t0;
+            r.device_bytes = lsh_index_device_bytes(g.ix);
+            r.status = lsh_index_search(g.ix, queries, n_queries, k, 0, res.data(), 0, nullptr);  // warm-up
+            if (r.status) continue;
+            int64_t hit = 0;
+            for (int q = 0; q < n_queries; ++q) {
+                const lsh_point_t* a = &res[(size_t)q * kq];
+                const lsh_point_t* b = &truth[(size_t)q * kq];
+                for (int x = 0; x < kk; ++x)
+                    for (int y = 0; y < kk; ++y)
+                        if (a[x].ID >= 0 && a[x].ID == b[y].ID) {
+                            ++hit;
+                            break;
+                        }
+            }
+            r.recall = (double)hit / ((double)n_queries * kk);
+            std::vector<double> t;
+            for (int rep_i = 0; rep_i < 3 && r.status == 0; ++rep_i) {
+                const double s0 = now_ms();
+                r.status = lsh_index_search(g.ix, queries, n_queries, k, 0, res.data(), 0, nullptr);
+                t.push_back(now_ms() - s0);
+            }
+            if (r.status) continue;
+            std::sort(t.begin(), t.end());
+            r.search_ms = t[t.size() / 2];
+        }
+        int best = -1;
+        for (int i = 0; i < n_cands; ++i)
+            if (rep[i].status == 0 && rep[i].recall >= target_recall &&
+                (best < 0 || rep[i].search_ms < rep[best].search_ms))
+                best = i;
+        if (best < 0)
+            for (int i = 0; i < n_cands; ++i)
+                if (rep[i].status == 0 && (best < 0 || rep[i].recall > rep[best].recall)) best = i;
+        LSH_REQUIRE(best >= 0, LSH_ERR_UNSUPPORTED, "autotune: no candidate configuration could be built");
+        *chosen = best;
+        if (reports) std::copy(rep.begin(), rep.end(), reports);
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_cache[key] = Cached{best, rep};
+    });
+}

@@ -58,6 +58,7 @@ struct DenseArgs {
     float c;               // error factor of E
     uint32_t* keys_nl;     // [n][L] or null
     uint32_t* keys_ln;     // [L][n] or null
+    int tpw;               // tables per workgroup (grid.y = ceil(L / tpw) table groups)
 };
 
 __device__ inline float bf16_bits_to_f32(uint16_t h) { return __builtin_bit_cast(float, (uint32_t)h << 16); }
@@ -70,6 +71,9 @@ __global__ __launch_bounds__(kTPB) void crosspoly_dense_kernel(DenseArgs a) {
     uint32_t* key_lds = reinterpret_cast<uint32_t*>(lds_raw + (size_t)PB * 32 * RS * 2);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, kh = lane >> 5;
     const int64_t row0 = (int64_t)blockIdx.x * kRows;
+    // This workgroup's tables [t0, t1): hashes [t0*K, t1*K).
+    const int t0 = blockIdx.y * a.tpw, t1 = min(a.L, t0 + a.tpw), nt = t1 - t0;
+    const int h0 = t0 * a.K, h1 = t1 * a.K;
     const int64_t pt = row0 + wave * 32 + (lane & 31);
     const bool valid = pt < a.n;
     const int KS = a.dpad / 16;
@@ -79,7 +83,7 @@ __global__ __launch_bounds__(kTPB) void crosspoly_dense_kernel(DenseArgs a) {
     // Rotation rows past P (a P below 32) stay zero; keys start at zero.
     for (int i = tid; i < PB * 32 * RS; i += kTPB)
         if (i / RS >= a.P) g_lds[i] = 0;
-    for (int i = tid; i < kRows * a.L; i += kTPB) key_lds[i] = 0u;
+    for (int i = tid; i < kRows * nt; i += kTPB) key_lds[i] = 0u;
 
     // The wave's rows as B operands, split into bfloat16 high and low parts.
     bf16x8 xh[kKS], xl[kKS];
@@ -122,13 +126,13 @@ __global__ __launch_bounds__(kTPB) void crosspoly_dense_kernel(DenseArgs a) {
         }
     };
     uint4 pre[kPre];
-    fetch(0, pre);
+    fetch(h0, pre);
     stage(pre);
     __syncthreads();
 
     const int myrow = wave * 32 + (lane & 31);
-    for (int h = 0; h < a.H; ++h) {
-        if (h + 1 < a.H) fetch(h + 1, pre);
+    for (int h = h0; h < h1; ++h) {
+        if (h + 1 < h1) fetch(h + 1, pre);
         f32x16 acc[PB];
 #pragma unroll
         for (int mb = 0; mb < PB; ++mb) acc[mb] = (f32x16){};
@@ -215,21 +219,24 @@ __global__ __launch_bounds__(kTPB) void crosspoly_dense_kernel(DenseArgs a) {
         }
         if (kh == 0) {
             const int t = h / a.K, k = h - t * a.K;
-            key_lds[myrow * a.L + t] |= hash << (k * a.b);
+            key_lds[myrow * nt + (t - t0)] |= hash << (k * a.b);
         }
         __syncthreads();  // every wave is done with this rotation
-        if (h + 1 < a.H) {
+        if (h + 1 < h1) {
             stage(pre);
             __syncthreads();
         }
     }
     const int nrows = (int)min<int64_t>(kRows, a.n - row0);
     if (a.keys_nl)
-        for (int i = tid; i < nrows * a.L; i += kTPB) a.keys_nl[(size_t)row0 * a.L + i] = key_lds[i];
+        for (int i = tid; i < nrows * nt; i += kTPB) {
+            const int r = i / nt, t = i - r * nt;
+            a.keys_nl[(size_t)(row0 + r) * a.L + t0 + t] = key_lds[i];
+        }
     if (a.keys_ln)
-        for (int i = tid; i < kRows * a.L; i += kTPB) {
+        for (int i = tid; i < kRows * nt; i += kTPB) {
             const int t = i / kRows, r = i - t * kRows;
-            if (r < nrows) a.keys_ln[(size_t)t * a.n + row0 + r] = key_lds[r * a.L + t];
+            if (r < nrows) a.keys_ln[(size_t)(t0 + t) * a.n + row0 + r] = key_lds[r * nt + t];
         }
 }
 
@@ -300,15 +307,19 @@ void DenseRotations::keys(const float* x, int64_t n, uint32_t* keys_nl, uint32_t
     const size_t lds = (size_t)PB * 32 * (dpad + 8) * 2 + (size_t)kRows * L * 4;
     LSH_REQUIRE(lds <= 160 * 1024, LSH_ERR_UNSUPPORTED, "too many tables for the dense cross-polytope kernel");
     const unsigned grid = (unsigned)((n + kRows - 1) / kRows);
+    // Enough workgroups for the 256 CUs: small batches split the tables.
+    const int groups = (int)std::min<int64_t>(L, std::max<int64_t>(1, (1024 + grid - 1) / grid));
+    a.tpw = (L + groups - 1) / groups;
+    const unsigned gy = (unsigned)((L + a.tpw - 1) / a.tpw);
     const void* fn = PB == 1 ? (const void*)crosspoly_dense_kernel<1>
                              : PB == 2 ? (const void*)crosspoly_dense_kernel<2> : (const void*)crosspoly_dense_kernel<4>;
     set_dynamic_lds(fn, lds);
     if (PB == 1)
-        hipLaunchKernelGGL(crosspoly_dense_kernel<1>, dim3(grid), dim3(kTPB), lds, s, a);
+        hipLaunchKernelGGL(crosspoly_dense_kernel<1>, dim3(grid, gy), dim3(kTPB), lds, s, a);
     else if (PB == 2)
-        hipLaunchKernelGGL(crosspoly_dense_kernel<2>, dim3(grid), dim3(kTPB), lds, s, a);
+        hipLaunchKernelGGL(crosspoly_dense_kernel<2>, dim3(grid, gy), dim3(kTPB), lds, s, a);
     else
-        hipLaunchKernelGGL(crosspoly_dense_kernel<4>, dim3(grid), dim3(kTPB), lds, s, a);
+        hipLaunchKernelGGL(crosspoly_dense_kernel<4>, dim3(grid, gy), dim3(kTPB), lds, s, a);
     check_launch("crosspoly_dense_kernel");
 }
 

@@ -68,6 +68,28 @@ std::vector<int32_t> flagged(const uint32_t* dflags, int Q, hipStream_t s, const
     return idx;
 }
 
+// Indices q with a[q] or b[q] nonzero (both device arrays of Q words).
+std::vector<int32_t> flagged_either(const uint32_t* a, const uint32_t* b, int Q, hipStream_t s) {
+    std::vector<uint32_t> ha((size_t)Q), hb((size_t)Q);
+    LSH_HIP(hipMemcpyAsync(ha.data(), a, sizeof(uint32_t) * (size_t)Q, hipMemcpyDeviceToHost, s));
+    LSH_HIP(hipMemcpyAsync(hb.data(), b, sizeof(uint32_t) * (size_t)Q, hipMemcpyDeviceToHost, s));
+    LSH_HIP(hipStreamSynchronize(s));
+    std::vector<int32_t> idx;
+    for (int i = 0; i < Q; ++i)
+        if (ha[i] || hb[i]) idx.push_back(i);
+    return idx;
+}
+
+// LSHGPU_LSH_ROUNDS=1 runs the exact (K,L) rescan in rounds instead of the
+// work list (both give identical results; used to compare them).
+bool force_rounds_lsh() {
+    static const bool on = [] {
+        const char* e = std::getenv("LSHGPU_LSH_ROUNDS");
+        return e && e[0] == '1';
+    }();
+    return on;
+}
+
 // Rounds over points [0, N) (see the file comment).  `scan(p0, p1, view)`
 // launches the scan kernel of one range.  kNormal sizes the ranges for
 // about newcap/2 appends per query, kCautious for newcap/8 (the re-run of
@@ -500,6 +522,7 @@ size_t lsh_index::device_bytes() const {
     size_t b = data.bytes() + data_inv.bytes() + dsk.bytes() + dfp4.bytes() + planesT.bytes() + djl.bytes() + wmh.bytes() +
                kwmh.bytes() + scratch.bytes() + kplanesT.bytes() + ksigns.bytes() + dkeys.bytes() + boff.bytes() + bperm.bytes() +
                qkeys.bytes() + qlo.bytes() + qpre.bytes() + qmax.bytes() + qlen.bytes() + bkeys.bytes() +
+               wl_ibase.bytes() + wl_flags.bytes() + wl_part.bytes() +
                dense.bytes() + q.bytes() + qT.bytes() +
                q_inv.bytes() + qjl.bytes() + qjlT.bytes() + qsk.bytes() + out.bytes() + flags.bytes() + st.bytes();
     for (const SampleLevel& L : samples) b += L.ds.bytes() + L.fp4.bytes();
@@ -927,10 +950,13 @@ void lsh_index::search_lsh(int Q, int kq, hipStream_t s, PhaseTimer& pt) {
             else wmh.sketch(q.get(), Q, d, qsk.get(), wpad, s);
         }
     }
-    uint32_t maxS = 0;
-    LSH_HIP(hipMemcpyAsync(&maxS, qmax.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    LSH_HIP(hipStreamSynchronize(s));
     pt.end(PH_PREP, s);
+    const bool worklist = !sketched && std::min(kMaxTopK, pow2_ceil(kq + margin)) <= 512 && !force_rounds_lsh();
+    uint32_t maxS = 0;
+    if (!worklist) {
+        LSH_HIP(hipMemcpyAsync(&maxS, qmax.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        LSH_HIP(hipStreamSynchronize(s));
+    }
 
     LshScanArgs a{};
     a.perm = bperm.get();
@@ -971,12 +997,35 @@ void lsh_index::search_lsh(int Q, int kq, hipStream_t s, PhaseTimer& pt) {
     a.kind = kind;
     LSH_REQUIRE(kq <= kMaxTopK, LSH_ERR_UNSUPPORTED, "k must be <= 4096");
     const int K = std::min(kMaxTopK, pow2_ceil(kq + margin));
-    lsh_topk(a, none, maxS, K, kNormal, false, st, scratch, 0, s, pt, scan_launches);
-    pt.begin(PH_FINAL, s);
-    launch_exact_finalize(data.get(), q.get(), d, kind, st.view(), kq, eps, outp, flags.get(), s);
-    pt.end(PH_FINAL, s);
-    std::vector<int32_t> bad = flagged(flags.get(), Q, s);
-    if (bad.empty()) return;
+    std::vector<int32_t> bad;
+    if (worklist) {
+        // Every query's whole list in one pass of items, no host reads; the
+        // queries flagged by the merge (item capacity) or the finalisation
+        // (FP32 band) are redone below with the rounds.
+        const uint32_t cap = (uint32_t)((size_t(256) << 20) / (sizeof(uint64_t) * K));
+        wl_ibase.ensure((size_t)Q + 1);
+        wl_part.ensure((size_t)cap * K);
+        wl_flags.ensure((size_t)Q);
+        st.reset(Q, K, newcap_for(K), s, false, false, false);
+        pt.begin(PH_SCAN, s);
+        launch_lsh_exact_worklist(a, K, wl_ibase.get(), cap, wl_part.get(), st.view(), wl_flags.get(), s);
+        pt.end(PH_SCAN, s);
+        ++scan_launches;
+        pt.begin(PH_FINAL, s);
+        launch_exact_finalize(data.get(), q.get(), d, kind, st.view(), kq, eps, outp, flags.get(), s);
+        pt.end(PH_FINAL, s);
+        bad = flagged_either(flags.get(), wl_flags.get(), Q, s);
+        if (bad.empty()) return;
+        LSH_HIP(hipMemcpyAsync(&maxS, qmax.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        LSH_HIP(hipStreamSynchronize(s));
+    } else {
+        lsh_topk(a, none, maxS, K, kNormal, false, st, scratch, 0, s, pt, scan_launches);
+        pt.begin(PH_FINAL, s);
+        launch_exact_finalize(data.get(), q.get(), d, kind, st.view(), kq, eps, outp, flags.get(), s);
+        pt.end(PH_FINAL, s);
+        bad = flagged(flags.get(), Q, s);
+        if (bad.empty()) return;
+    }
     LSH_REQUIRE(K < kMaxTopK, LSH_ERR_UNSUPPORTED,
                 "more than 4096 candidates within the FP32 error band of the k-th distance");
     const int m = (int)bad.size();

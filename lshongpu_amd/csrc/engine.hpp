@@ -211,6 +211,8 @@ struct lsh_index {
     lshgpu::DevBuf<uint32_t> boff;        // [tables][tableSize+1] absolute positions into bperm
     lshgpu::DevBuf<uint32_t> bperm;       // [tables][N] point IDs grouped by key
     lshgpu::DevBuf<uint32_t> qkeys, qlo, qpre, qmax, qlen;
+    lshgpu::DevBuf<uint32_t> wl_ibase, wl_flags;  // exact rescan work list: item offsets, capacity flags
+    lshgpu::DevBuf<uint64_t> wl_part;             // its per-item best lists
     int64_t last_candidates = 0;          // sum over queries of bucket entries walked (last search)
 
     // Per-search scratch (grow-only).

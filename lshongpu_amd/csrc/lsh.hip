@@ -19,6 +19,8 @@
 // already seen), which visits the union once without any shared state.  The
 // reference instead scores every occurrence and drops adjacent duplicates
 // after the final sort (lshFramework.cuh:255-280).
+#include <type_traits>
+
 #include "device_common.hpp"
 
 namespace lshgpu {
@@ -435,6 +437,200 @@ __global__ __launch_bounds__(kTPB) void lsh_scan_kernel(LshScanArgs a, TopKView 
     }
 }
 
+// ------------------------------------------ exact rescan as a work list
+// The exact (K,L) rescan without rounds: every query's list of S positions
+// is cut into items of kItemPos positions; one launch turns the per-query
+// item counts into item offsets (ibase), a persistent grid takes items from
+// that list (a grid-stride loop, so the grid needs no host-side count) and
+// keeps each item's KP best (distance, ID) keys, and one workgroup per query
+// merges its items' lists into the query's sorted best KP.  Nothing is read
+// back by the host; a query whose items do not fit the item capacity is
+// flagged (mflags) for a re-run.
+constexpr int kItemPos = 2048;
+
+__global__ __launch_bounds__(1024) void lsh_items_kernel(const uint32_t* __restrict__ pre, int Q, int L,
+                                                         uint32_t* __restrict__ ibase) {
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t total_s;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t carry = 0;
+    for (int c0 = 0; c0 < Q; c0 += 1024) {
+        const int q = c0 + threadIdx.x;
+        const uint32_t S = q < Q ? pre[(size_t)q * (L + 1) + L] : 0u;
+        const uint32_t n = (S + kItemPos - 1) / kItemPos;
+        uint32_t inc = n;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(inc, o);
+            if (lane >= o) inc += t;
+        }
+        if (lane == 63) wsum[w] = inc;
+        __syncthreads();
+        if (w == 0) {
+            const uint32_t x = lane < 16 ? wsum[lane] : 0u;
+            uint32_t xi = x;
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+                const uint32_t t = __shfl_up(xi, o);
+                if (lane >= o) xi += t;
+            }
+            if (lane < 16) wsum[lane] = xi - x;
+            if (lane == 15) total_s = xi;
+        }
+        __syncthreads();
+        if (q < Q) ibase[q] = carry + wsum[w] + inc - n;
+        carry += total_s;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ibase[Q] = carry;
+}
+
+template <int KP>
+__global__ __launch_bounds__(kTPB) void lsh_partial_kernel(LshScanArgs a, const uint32_t* __restrict__ ibase,
+                                                           uint32_t cap, uint64_t* __restrict__ partials) {
+    extern __shared__ uint32_t lds[];
+    const int L = a.L;
+    uint64_t* s_keys = reinterpret_cast<uint64_t*>(lds);  // kItemPos
+    uint32_t* s_pre = lds + 2 * kItemPos;                  // L+1
+    uint32_t* s_lo = s_pre + (L + 1);                      // L
+    uint32_t* s_qk = s_lo + L;                             // L
+    float* s_qf = reinterpret_cast<float*>(s_qk + L);      // d
+    __shared__ uint32_t s_n;
+    const int lane = threadIdx.x & 63;
+    const uint32_t total = min(ibase[a.Q], cap);
+    for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
+        int lo = 0, hi = a.Q;  // ibase[lo] <= item < ibase[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (ibase[mid] <= item) lo = mid;
+            else hi = mid;
+        }
+        const int q = lo;
+        const uint32_t* pre = a.qpre + (size_t)q * (L + 1);
+        const int64_t S = pre[L];
+        const int64_t c0 = (int64_t)(item - ibase[q]) * kItemPos;
+        const int64_t c1 = min<int64_t>(S, c0 + kItemPos);
+        __syncthreads();  // the previous item's LDS reads are done
+        for (int t = threadIdx.x; t <= L; t += kTPB) s_pre[t] = pre[t];
+        for (int t = threadIdx.x; t < L; t += kTPB) {
+            s_lo[t] = a.qlo[(size_t)q * L + t];
+            s_qk[t] = a.qkeys[(size_t)q * L + t];
+        }
+        for (int j = threadIdx.x; j < a.d; j += kTPB) s_qf[j] = a.queries[(size_t)q * a.d + j];
+        if (threadIdx.x == 0) s_n = 0;
+        __syncthreads();
+        float qn = 0.0f;
+        if (a.kind == kAngular || a.kind == kNegCos) {
+            for (int j = 0; j < a.d; ++j) qn = __builtin_fmaf(s_qf[j], s_qf[j], qn);
+            qn = __builtin_sqrtf(qn);
+        }
+        for (int64_t base = c0; base < c1; base += kTPB) {
+            const int64_t pos = base + threadIdx.x;
+            bool owned = false;
+            uint32_t id = 0;
+            if (pos < c1) {
+                int tl = 0, th = L;  // pre[tl] <= pos < pre[th]
+                while (th - tl > 1) {
+                    const int mid = (tl + th) >> 1;
+                    if ((int64_t)s_pre[mid] <= pos) tl = mid;
+                    else th = mid;
+                }
+                id = a.perm[s_lo[tl] + (uint32_t)(pos - s_pre[tl])];
+                owned = true;
+                const uint32_t* dk = a.dkeys + (size_t)id * L;
+                for (int t2 = 0; t2 < tl; ++t2)
+                    if (dk[t2] == s_qk[t2]) {
+                        owned = false;
+                        break;
+                    }
+            }
+            const uint64_t m = __ballot(owned);
+            const int n = __popcll(m);
+            const int g = lane >> 4, gl = lane & 15;
+            for (int it = 0; it * 4 < n; ++it) {
+                const int j = it * 4 + g;
+                const bool has = j < n;
+                const int src = has ? nth_set_bit(m, j) : 0;
+                const uint32_t cid = (uint32_t)__shfl((int)id, src);
+                float x = 0.0f, y = 0.0f;
+                if (has) {
+                    const float* __restrict__ row = a.data + (size_t)cid * a.d;
+                    if ((a.d & 3) == 0) {
+                        for (int jj = gl * 4; jj < a.d; jj += 64) {
+                            const float4 xv = *reinterpret_cast<const float4*>(row + jj);
+                            group_accumulate(a.kind, xv.x, s_qf[jj], x, y);
+                            group_accumulate(a.kind, xv.y, s_qf[jj + 1], x, y);
+                            group_accumulate(a.kind, xv.z, s_qf[jj + 2], x, y);
+                            group_accumulate(a.kind, xv.w, s_qf[jj + 3], x, y);
+                        }
+                    } else {
+                        for (int jj = gl; jj < a.d; jj += 16) group_accumulate(a.kind, row[jj], s_qf[jj], x, y);
+                    }
+                }
+#pragma unroll
+                for (int o = 8; o > 0; o >>= 1) {
+                    x += __shfl_xor(x, o, 16);
+                    y += __shfl_xor(y, o, 16);
+                }
+                if (has && gl == 0) {
+                    const float dist = group_distance(a.kind, x, y, qn);
+                    s_keys[atomicAdd(&s_n, 1u)] = make_key(f32_key(dist), cid);
+                }
+            }
+        }
+        __syncthreads();
+        // The item's best KP: all of them when there are at most KP, else
+        // the first KP of the sorted list.
+        const int n = (int)s_n;
+        int np = KP;
+        while (np < n) np <<= 1;
+        for (int t = n + threadIdx.x; t < np; t += kTPB) s_keys[t] = kEmptyKey;
+        if (n > KP) bitonic_sort_lds(s_keys, np);
+        else __syncthreads();
+        uint64_t* out = partials + (size_t)item * KP;
+        for (int t = threadIdx.x; t < KP; t += kTPB) out[t] = s_keys[t];
+    }
+}
+
+template <int KP>
+__global__ __launch_bounds__(kTPB) void lsh_merge_kernel(const uint32_t* __restrict__ ibase, uint32_t cap,
+                                                         const uint64_t* __restrict__ partials, TopKView v,
+                                                         uint32_t* __restrict__ mflags) {
+    constexpr int B = 4096;  // LDS buffer: the best KP so far, then the next keys
+    __shared__ uint64_t buf[B];
+    __shared__ uint32_t s_n;
+    const int q = blockIdx.x;
+    const uint32_t i0 = ibase[q], i1 = ibase[q + 1];
+    const uint32_t e1 = min(i1, cap);
+    for (int t = threadIdx.x; t < KP; t += kTPB) buf[t] = kEmptyKey;
+    if (threadIdx.x == 0) s_n = KP;
+    __syncthreads();
+    uint64_t thr = kEmptyKey;  // the current KP-th best: larger keys are dropped
+    const size_t k0 = (size_t)i0 * KP, k1 = e1 > i0 ? (size_t)e1 * KP : k0;
+    for (size_t c = k0; c < k1; c += B - KP) {
+        const size_t ce = min(k1, c + (size_t)(B - KP));
+        for (size_t t = c + threadIdx.x; t < ce; t += kTPB) {
+            const uint64_t key = partials[t];
+            if (key < thr) buf[atomicAdd(&s_n, 1u)] = key;
+        }
+        __syncthreads();
+        const int n = (int)s_n;
+        int np = KP;
+        while (np < n) np <<= 1;
+        for (int t = n + threadIdx.x; t < np; t += kTPB) buf[t] = kEmptyKey;
+        bitonic_sort_lds(buf, np);
+        thr = buf[KP - 1];
+        if (threadIdx.x == 0) s_n = KP;
+        __syncthreads();
+    }
+    uint64_t* row = v.keys + (size_t)q * v.stride;
+    for (int t = threadIdx.x; t < v.K; t += kTPB) row[t] = t < KP ? buf[t] : kEmptyKey;
+    if (threadIdx.x == 0) {
+        v.thr[q] = thr;
+        mflags[q] = i1 > cap ? 1u : 0u;
+    }
+}
+
 }  // namespace
 
 void launch_mask_keys(uint32_t* keys, size_t n, uint32_t mask, hipStream_t s) {
@@ -535,6 +731,36 @@ void launch_transpose_keys(const uint32_t* nl, int64_t n, int L, uint32_t* ln, h
     if (n <= 0) return;
     hipLaunchKernelGGL(transpose_keys_kernel, dim3(grid_for((size_t)n * L, 256)), dim3(256), 0, s, nl, n, L, ln);
     check_launch("transpose_keys_kernel");
+}
+
+int lsh_item_positions() { return kItemPos; }
+
+void launch_lsh_exact_worklist(const LshScanArgs& a, int KP, uint32_t* ibase, uint32_t cap, uint64_t* partials,
+                               const TopKView& v, uint32_t* mflags, hipStream_t s) {
+    if (a.Q <= 0) return;
+    LSH_REQUIRE(!a.sketch && a.qmap == nullptr && v.K >= KP, LSH_ERR_INTERNAL, "work-list rescan: bad arguments");
+    hipLaunchKernelGGL(lsh_items_kernel, dim3(1), dim3(1024), 0, s, a.qpre, a.Q, a.L, ibase);
+    check_launch("lsh_items_kernel");
+    const size_t lds = sizeof(uint64_t) * kItemPos + sizeof(uint32_t) * (3 * (size_t)a.L + 1) + sizeof(float) * a.d;
+    // A persistent grid: 8 workgroups per CU of the 256.
+    const unsigned grid = 2048;
+    auto go = [&](auto kp) {
+        constexpr int K_ = decltype(kp)::value;
+        set_dynamic_lds((const void*)lsh_partial_kernel<K_>, lds);
+        hipLaunchKernelGGL(lsh_partial_kernel<K_>, dim3(grid), dim3(kTPB), lds, s, a, ibase, cap, partials);
+        check_launch("lsh_partial_kernel");
+        hipLaunchKernelGGL(lsh_merge_kernel<K_>, dim3((unsigned)a.Q), dim3(kTPB), 0, s, ibase, cap, partials, v,
+                           mflags);
+        check_launch("lsh_merge_kernel");
+    };
+    switch (KP) {
+        case 32: go(std::integral_constant<int, 32>()); break;
+        case 64: go(std::integral_constant<int, 64>()); break;
+        case 128: go(std::integral_constant<int, 128>()); break;
+        case 256: go(std::integral_constant<int, 256>()); break;
+        case 512: go(std::integral_constant<int, 512>()); break;
+        default: fail(LSH_ERR_INTERNAL, "work-list rescan: KP must be a power of two in 32..512");
+    }
 }
 
 void launch_lsh_scan(const LshScanArgs& a, const TopKView& v, hipStream_t s) {

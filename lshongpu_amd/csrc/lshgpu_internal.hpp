@@ -425,6 +425,16 @@ struct LshScanArgs {
     int sdim;
 };
 void launch_lsh_scan(const LshScanArgs& a, const TopKView& v, hipStream_t s);
+// The exact rescan (a.sketch false, a.qmap null) of every query's whole
+// candidate list without rounds or host reads: positions cut into items of
+// lsh_item_positions(), item offsets in ibase ([Q+1]), at most cap items
+// kept (KP keys each in partials, [cap][KP]), and each query's best KP (a
+// power of two in 32..512, <= v.K) written sorted to v.keys (the rest of
+// its K slots empty) with v.thr its KP-th key.  mflags[q] = 1 when query q
+// had items past cap (its list is then incomplete).
+int lsh_item_positions();
+void launch_lsh_exact_worklist(const LshScanArgs& a, int KP, uint32_t* ibase, uint32_t cap, uint64_t* partials,
+                               const TopKView& v, uint32_t* mflags, hipStream_t s);
 
 // ---------------------------------------------------------- host helpers
 

@@ -163,6 +163,34 @@ __global__ void normalize_rows_kernel(float* __restrict__ x, int64_t n, int d) {
     for (int j = 0; j < d; ++j) r[j] = r[j] / m;
 }
 
+// The same, 64 rows per wave staged through LDS (row stride d + 1, so the
+// per-row chains of the 64 lanes read distinct banks): the loads and stores
+// are coalesced, and each lane's chain reads LDS instead of strided memory.
+__global__ __launch_bounds__(64) void normalize_rows_lds_kernel(float* __restrict__ x, int64_t n, int d) {
+    extern __shared__ float t[];
+    const int64_t r0 = (int64_t)blockIdx.x * 64;
+    const int rows = (int)min<int64_t>(64, n - r0);
+    const int S = d + 1;
+    float* __restrict__ g = x + (size_t)r0 * d;
+    for (int e = threadIdx.x; e < rows * d; e += 64) {
+        const int r = e / d;
+        t[r * S + (e - r * d)] = g[e];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < rows) {
+        float* r = t + threadIdx.x * S;
+        float m = 0.0f;
+        for (int j = 0; j < d; ++j) m = __builtin_fmaf(r[j], r[j], m);
+        m = __builtin_sqrtf(m);
+        for (int j = 0; j < d; ++j) r[j] = r[j] / m;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < rows * d; e += 64) {
+        const int r = e / d;
+        g[e] = t[r * S + (e - r * d)];
+    }
+}
+
 // processingUtils.cuh:12-31: per-dimension min of (int)floor(x), starting
 // from 0, reduced per workgroup and then with one atomic per dimension.
 __global__ void gj_mins_kernel(const float* __restrict__ x, int64_t n, int d, int rows_per_block,
@@ -259,6 +287,12 @@ void launch_jl(const float* x, int64_t n, int d, const float* planesT, int sdim,
 
 void launch_normalize_rows(float* x, int64_t n, int d, hipStream_t s) {
     if (n <= 0) return;
+    if (d <= 255) {
+        const size_t lds = sizeof(float) * 64 * (size_t)(d + 1);
+        hipLaunchKernelGGL(normalize_rows_lds_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), lds, s, x, n, d);
+        check_launch("normalize_rows_lds_kernel");
+        return;
+    }
     hipLaunchKernelGGL(normalize_rows_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, x, n, d);
     check_launch("normalize_rows_kernel");
 }

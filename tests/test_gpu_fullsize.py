@@ -101,3 +101,33 @@ def test_lsh_framework_full_size(lsh, orc):
         ids, dist = orc.rerank(xn, qn[i:i + 1], cand[None, :], k, 1)
         np.testing.assert_array_equal(res.ids[i], ids[0])
         _close(res.distances[i], dist[0])
+
+
+@pytest.mark.timeout(900)
+def test_lsh_framework_dense_keys_full_size(lsh, orc):
+    # cfg 3 as specified: 4M clustered points, d = 128, K = 2 dense-rotation
+    # cross-polytope hashes per table (16 of the 32 key bits), L = 32 tables
+    # sorted on the device, exact angular rescan (16 of the 50,000 queries).
+    # Keys are pinned on 256 sampled rows and on the sampled queries; each
+    # sampled query's result must be the oracle's FP64 top-k of its bucket
+    # union.
+    data, queries = synth.config(3)
+    N, d = data.shape
+    tables, K, bits, k, nsample = 32, 2, 16, 32, 16
+    rng = np.random.default_rng(1004)
+    sample = np.sort(rng.choice(queries.shape[0], nsample, replace=False))
+    qs = queries[sample]
+    G = orc.dense_rotations(tables, K, 128, d)
+    xn, qn = orc.normalize_rows(data), orc.normalize_rows(qs)
+    dk = lsh.crosspoly_dense_keys(data, tables, K)
+    rows = rng.choice(N, 256, replace=False)
+    np.testing.assert_array_equal(dk[rows], orc.crosspoly_dense(xn[rows], tables, K, G))
+    qk = orc.crosspoly_dense(qn, tables, K, G)
+    np.testing.assert_array_equal(lsh.crosspoly_dense_keys(qs, tables, K), qk)
+    res = lsh.LSH(2, 8, k, d, nsample, N, data, qs, 1, 1, bits, tables, False)
+    for i in range(nsample):
+        cand = orc.lsh_candidates(dk, qk, i)
+        assert cand.size >= k
+        ids, dist = orc.rerank(xn, qn[i:i + 1], cand[None, :], k, 1)
+        np.testing.assert_array_equal(res.ids[i], ids[0])
+        _close(res.distances[i], dist[0])
