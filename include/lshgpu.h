@@ -359,6 +359,20 @@ int lsh_index_search(lsh_index_t* index, const float* queries, int N_query,
                      int k, int queries_on_device, lsh_point_t* results,
                      int results_on_device, void* stream);
 
+/* n_batches searches, one per batch: batch i is the next batch_sizes[i] rows
+ * of `queries`, and its results the next rows of `results` ([total][k
+ * rounded]), as consecutive lsh_index_search calls would give them.  On the
+ * exact (K,L) rescan of keysImplementation 3, 4, 5 or 8 (without
+ * runWithSketchedData; rounded k + rescan margin <= 512) each batch replays a
+ * captured hipGraph of its query-count class (classes about 1.25x apart up
+ * to 4096 queries; a larger batch is an ordinary search), with no host wait
+ * between batches; other index kinds run the batches as ordinary searches.
+ * The call returns when every result is complete (stream: as
+ * lsh_index_search). */
+int lsh_index_search_batches(lsh_index_t* index, const float* queries, const int* batch_sizes, int n_batches,
+                             int k, int queries_on_device, lsh_point_t* results, int results_on_device,
+                             void* stream);
+
 /* Per-phase device times (ms), summed over the searches since profiling was
  * turned on or since the previous lsh_index_phase_times(_n) call (which waits
  * for those searches and starts a new sum): [0] query preparation and

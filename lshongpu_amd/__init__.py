@@ -154,6 +154,8 @@ _SIGS = {
     "lsh_index_create": (C.c_int, [C.POINTER(_IndexConfig), C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
     "lsh_index_search": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int,
                                    C.c_void_p]),
+    "lsh_index_search_batches": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_int,
+                                           C.c_int, C.c_void_p, C.c_int, C.c_void_p]),
     "lsh_index_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "lsh_index_phase_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "lsh_index_phase_times_n": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_int]),
@@ -569,6 +571,42 @@ class Index:
             rp, r_dev = _ptr_of(out)
         sp = C.c_void_p(stream) if stream else None
         _check(lib.lsh_index_search(self._h, qp, nq, int(k), int(q_dev), rp, int(r_dev), sp))
+        return res
+
+    def search_batches(self, queries, batch_sizes, k: int, out=None, stream=None):
+        """Each batch (consecutive rows of queries, batch_sizes[i] of them) as
+        its own search, through captured graphs where the index supports
+        them (lsh_index_search_batches); results as search() for all rows."""
+        lib = load()
+        if int(k) < 1:
+            raise LshError(1, "k must be >= 1")
+        kq = calculateK(k)
+        if hasattr(queries, "data_ptr"):
+            _check_tensor(queries, "queries", "float32")
+        else:
+            queries = _f32(queries)
+            if queries.ndim != 2:
+                raise LshError(1, "queries must be a 2-D (N_query, d) array")
+        if int(queries.shape[1]) != self.d:
+            raise LshError(1, f"queries have {int(queries.shape[1])} columns; the index holds d = {self.d}")
+        sizes = np.ascontiguousarray(batch_sizes, dtype=np.int32)
+        nq = int(queries.shape[0])
+        if sizes.ndim != 1 or (sizes < 0).any() or int(sizes.sum()) != nq:
+            raise LshError(1, "batch sizes must be non-negative and sum to the number of queries")
+        if out is None:
+            res = np.empty((nq, kq), dtype=POINT_DTYPE)
+        else:
+            have = out.numel() * out.element_size() if hasattr(out, "data_ptr") else out.nbytes
+            if hasattr(out, "data_ptr"):
+                _check_tensor(out, "out", None)
+            if have < nq * kq * POINT_DTYPE.itemsize:
+                raise LshError(1, "out is too small")
+            res = out
+        qp, q_dev = _ptr_of(queries)
+        rp, r_dev = _ptr_of(res)
+        sp = C.c_void_p(stream) if stream else None
+        _check(lib.lsh_index_search_batches(self._h, qp, sizes.ctypes.data_as(C.POINTER(C.c_int)), len(sizes),
+                                            int(k), int(q_dev), rp, int(r_dev), sp))
         return res
 
     def set_profiling(self, on=True) -> None:

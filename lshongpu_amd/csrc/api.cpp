@@ -360,6 +360,17 @@ int lsh_index_search(lsh_index_t* index, const float* queries, int N_query, int 
     });
 }
 
+int lsh_index_search_batches(lsh_index_t* index, const float* queries, const int* batch_sizes, int n_batches,
+                             int k, int queries_on_device, lsh_point_t* results, int results_on_device,
+                             void* stream) {
+    return guarded([&] {
+        LSH_REQUIRE(index, LSH_ERR_INVALID, "index is null");
+        LSH_REQUIRE(n_batches == 0 || (queries && results && batch_sizes), LSH_ERR_INVALID, "null pointer");
+        index->search_batches(queries, batch_sizes, n_batches, k, queries_on_device != 0, results,
+                              results_on_device != 0, (hipStream_t)stream);
+    });
+}
+
 int lsh_index_set_profiling(lsh_index_t* index, int on) {
     return guarded([&] {
         LSH_REQUIRE(index, LSH_ERR_INVALID, "index is null");

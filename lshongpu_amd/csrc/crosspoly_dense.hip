@@ -105,6 +105,10 @@ __global__ __launch_bounds__(kTPB) void crosspoly_dense_kernel(DenseArgs a) {
     }
     ss += __shfl_xor(ss, 32);
     const float xnorm = __builtin_sqrtf(ss) * (1.0f + 1.0f / 1024.0f);
+    // A NaN in the row makes every FP32 chain NaN (0 * NaN is NaN too), so no
+    // coordinate ever wins the reference's comparison and the hash is P; such
+    // rows (a zero row after normalisation) skip the recheck.
+    const bool xnan = ss != ss;
 
     // Rotation of hash 0 into LDS.
     auto stage = [&](const uint4* pre) {
@@ -192,7 +196,7 @@ __global__ __launch_bounds__(kTPB) void crosspoly_dense_kernel(DenseArgs a) {
             // comparison; then the pair's better one (the lower j on a tie).
             float rb = (float)INT32_MIN;
             int ridx = -1, rj = 0x7FFFFFFF;
-            if (valid) {
+            if (valid && !xnan) {
 #pragma unroll
                 for (int mb = 0; mb < PB; ++mb)
 #pragma unroll

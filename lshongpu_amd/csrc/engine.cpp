@@ -80,6 +80,15 @@ std::vector<int32_t> flagged_either(const uint32_t* a, const uint32_t* b, int Q,
     return idx;
 }
 
+// Items the exact rescan's work list keeps per search (256 MiB of best
+// lists).  LSHGPU_WORKLIST_CAP overrides it (a small value exercises the
+// re-run of the queries past it).
+uint32_t worklist_cap(int K) {
+    const char* e = std::getenv("LSHGPU_WORKLIST_CAP");
+    if (e && *e) return (uint32_t)std::max(1L, std::strtol(e, nullptr, 10));
+    return (uint32_t)((size_t(256) << 20) / (sizeof(uint64_t) * K));
+}
+
 // LSHGPU_LSH_ROUNDS=1 runs the exact (K,L) rescan in rounds instead of the
 // work list (both give identical results; used to compare them).
 bool force_rounds_lsh() {
@@ -917,11 +926,178 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
     }
 }
 
+// ------------------------------------------------------- batched search
+// Query-count classes of the captured graphs: a batch of b queries runs the
+// graph of the smallest class >= b, the rows past b zero (a zero row
+// normalises to NaN, whose keys match no real point, so it costs nearly
+// nothing).  Classes grow by about 1.25x from 1 to 4096.
+namespace {
+
+int graph_class(int b) {
+    int c = 1;
+    while (c < b) c = std::max(c + 1, (int)std::ceil(c * 1.25));
+    return std::min(c, lsh_index::kGraphMaxQ);
+}
+
+}  // namespace
+
+lsh_index::~lsh_index() {
+    drop_graphs();
+    if (bflags) (void)hipHostFree(bflags);
+}
+
+void lsh_index::drop_graphs() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    graphs.clear();
+    graph_sig.clear();
+}
+
+void lsh_index::search_batches(const float* queries, const int* sizes, int n_batches, int k, bool q_on_device,
+                               lsh_point_t* results, bool r_on_device, hipStream_t user_stream) {
+    DeviceGuard g(device);
+    LSH_REQUIRE(n_batches >= 0 && (n_batches == 0 || sizes), LSH_ERR_INVALID, "bad batch list");
+    LSH_REQUIRE(k >= 1, LSH_ERR_INVALID, "k must be >= 1");
+    int64_t total = 0;
+    for (int i = 0; i < n_batches; ++i) {
+        LSH_REQUIRE(sizes[i] >= 0, LSH_ERR_INVALID, "batch sizes must be >= 0");
+        total += sizes[i];
+    }
+    const int kq = calculate_k(k);
+    if (!lsh_graphable(kq)) {
+        // No captured form for this path: one search per batch.
+        int64_t off = 0;
+        for (int i = 0; i < n_batches; ++i) {
+            if (sizes[i] > 0)
+                search(queries + off * d, sizes[i], k, q_on_device, results + off * kq, r_on_device, user_stream);
+            off += sizes[i];
+        }
+        if (r_on_device && user_stream) LSH_HIP(hipStreamSynchronize(user_stream));
+        return;
+    }
+    // Graphs run on the caller's stream, or on the index's own one (the
+    // legacy null stream cannot be captured).
+    hipStream_t s = user_stream ? user_stream : stream;
+    if (done_rec) LSH_HIP(hipStreamWaitEvent(s, done_ev, 0));
+    const int L = tables;
+    const int K = std::min(kMaxTopK, pow2_ceil(kq + margin));
+    const int QM = kGraphMaxQ;
+    // Buffers for the largest class, then the layout the graphs were captured
+    // over: graphs of another layout are dropped.
+    auto prepare = [&] {
+        q.ensure((size_t)QM * d);
+        qkeys.ensure((size_t)QM * L);
+        qlo.ensure((size_t)QM * L);
+        qlen.ensure((size_t)QM * L);
+        qpre.ensure((size_t)QM * (L + 1));
+        qmax.ensure(1);
+        flags.ensure((size_t)QM);
+        wl_ibase.ensure((size_t)QM + 1);
+        wl_flags.ensure((size_t)QM);
+        wl_part.ensure((size_t)worklist_cap(K) * K);
+        out.ensure((size_t)QM * kq);
+        st.reset(QM, K, newcap_for(K), s, false, false, false);
+        const TopKView sv = st.view();
+        const std::vector<uintptr_t> sig = {
+            (uintptr_t)q.get(),      (uintptr_t)qkeys.get(),   (uintptr_t)qlo.get(),     (uintptr_t)qlen.get(),
+            (uintptr_t)qpre.get(),   (uintptr_t)qmax.get(),    (uintptr_t)flags.get(),   (uintptr_t)wl_ibase.get(),
+            (uintptr_t)wl_flags.get(), (uintptr_t)wl_part.get(), (uintptr_t)out.get(),   (uintptr_t)sv.keys,
+            (uintptr_t)sv.cnt,       (uintptr_t)sv.thr,        (uintptr_t)sv.ovf,        (uintptr_t)K,
+            (uintptr_t)worklist_cap(K)};
+        if (sig != graph_sig) {
+            drop_graphs();
+            graph_sig = sig;
+        }
+    };
+    prepare();
+    // Pinned flags of every query: finalisation flags, then capacity flags.
+    if ((size_t)(2 * total) > bflags_n) {
+        LSH_HIP(hipStreamSynchronize(s));
+        if (bflags) (void)hipHostFree(bflags);
+        bflags = nullptr;
+        bflags_n = 0;
+        void* p = nullptr;
+        LSH_HIP(hipHostMalloc(&p, sizeof(uint32_t) * (size_t)std::max<int64_t>(2 * total, 1), hipHostMallocDefault));
+        bflags = static_cast<uint32_t*>(p);
+        bflags_n = (size_t)std::max<int64_t>(2 * total, 1);
+    }
+    const hipMemcpyKind qk = q_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const hipMemcpyKind rk = r_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    int64_t off = 0;
+    for (int i = 0; i < n_batches; ++i) {
+        const int b = sizes[i];
+        if (b == 0) continue;
+        if (b > QM) {
+            // Larger than any class: an ordinary search of its own (complete
+            // when it returns), then the buffers for the graphs again.
+            LSH_HIP(hipStreamSynchronize(s));
+            search(queries + off * d, b, k, q_on_device, results + off * kq, r_on_device, nullptr);
+            for (int j = 0; j < b; ++j) bflags[off + j] = bflags[total + off + j] = 0u;
+            prepare();
+            off += b;
+            continue;
+        }
+        const int Qc = graph_class(b);
+        LSH_HIP(hipMemcpyAsync(q.get(), queries + off * d, sizeof(float) * (size_t)b * d, qk, s));
+        if (Qc > b) LSH_HIP(hipMemsetAsync(q.get() + (size_t)b * d, 0, sizeof(float) * (size_t)(Qc - b) * d, s));
+        auto it = graphs.find({Qc, kq});
+        if (it == graphs.end()) {
+            hipGraph_t graph = nullptr;
+            PhaseTimer none(false);
+            outp = out.get();
+            LSH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            try {
+                search_lsh(Qc, kq, s, none, true);
+            } catch (...) {
+                (void)hipStreamEndCapture(s, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            LSH_HIP(hipStreamEndCapture(s, &graph));
+            hipGraphExec_t exec = nullptr;
+            const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            LSH_HIP(e);
+            it = graphs.emplace(std::make_pair(Qc, kq), exec).first;
+        }
+        LSH_HIP(hipGraphLaunch(it->second, s));
+        LSH_HIP(hipMemcpyAsync(results + off * kq, out.get(), sizeof(lsh_point_t) * (size_t)b * kq, rk, s));
+        LSH_HIP(hipMemcpyAsync(bflags + off, flags.get(), sizeof(uint32_t) * (size_t)b, hipMemcpyDeviceToHost, s));
+        LSH_HIP(hipMemcpyAsync(bflags + total + off, wl_flags.get(), sizeof(uint32_t) * (size_t)b,
+                               hipMemcpyDeviceToHost, s));
+        off += b;
+    }
+    LSH_HIP(hipStreamSynchronize(s));
+    // Queries the work list left incomplete or whose FP32 band was too wide:
+    // redone one by one through the ordinary search (its rounds).
+    std::vector<int64_t> bad;
+    for (int64_t j = 0; j < total; ++j)
+        if (bflags[j] || bflags[total + j]) bad.push_back(j);
+    if (!bad.empty()) {
+        std::vector<float> hq((size_t)d);
+        std::vector<lsh_point_t> hr((size_t)kq);
+        for (int64_t j : bad) {
+            LSH_HIP(hipMemcpy(hq.data(), queries + j * d, sizeof(float) * d,
+                              q_on_device ? hipMemcpyDeviceToHost : hipMemcpyHostToHost));
+            search(hq.data(), 1, k, false, hr.data(), false, nullptr);
+            LSH_HIP(hipMemcpy(results + j * kq, hr.data(), sizeof(lsh_point_t) * kq,
+                              r_on_device ? hipMemcpyHostToDevice : hipMemcpyHostToHost));
+        }
+    }
+    if (!done_ev) LSH_HIP(hipEventCreateWithFlags(&done_ev, hipEventDisableTiming));
+    LSH_HIP(hipEventRecord(done_ev, s));
+    done_rec = true;
+}
+
 // lshFramework.cuh:142-280 scan + removeDuplicates over the unique candidate
 // set of each query: either all of it ranked by the exact distance
 // (distanceFunc), or its top-C by sketch distance reranked exactly
 // (runWithSketchedData, candidateSetScanner.cuh:10-32).
-void lsh_index::search_lsh(int Q, int kq, hipStream_t s, PhaseTimer& pt) {
+bool lsh_index::lsh_graphable(int kq) const {
+    return framework == 1 && !(withSketch && impl >= 3) && keysImpl != 7 &&
+           std::min(kMaxTopK, pow2_ceil(kq + margin)) <= 512 && !force_rounds_lsh();
+}
+
+void lsh_index::search_lsh(int Q, int kq, hipStream_t s, PhaseTimer& pt, bool enqueue_only) {
     const int L = tables;
     pt.begin(PH_PREP, s);
     apply_steps_query(key_steps, q.get(), Q, d, s);
@@ -1002,7 +1178,7 @@ void lsh_index::search_lsh(int Q, int kq, hipStream_t s, PhaseTimer& pt) {
         // Every query's whole list in one pass of items, no host reads; the
         // queries flagged by the merge (item capacity) or the finalisation
         // (FP32 band) are redone below with the rounds.
-        const uint32_t cap = (uint32_t)((size_t(256) << 20) / (sizeof(uint64_t) * K));
+        const uint32_t cap = worklist_cap(K);
         wl_ibase.ensure((size_t)Q + 1);
         wl_part.ensure((size_t)cap * K);
         wl_flags.ensure((size_t)Q);
@@ -1014,6 +1190,7 @@ void lsh_index::search_lsh(int Q, int kq, hipStream_t s, PhaseTimer& pt) {
         pt.begin(PH_FINAL, s);
         launch_exact_finalize(data.get(), q.get(), d, kind, st.view(), kq, eps, outp, flags.get(), s);
         pt.end(PH_FINAL, s);
+        if (enqueue_only) return;  // the caller reads flags and wl_flags later
         bad = flagged_either(flags.get(), wl_flags.get(), Q, s);
         if (bad.empty()) return;
         LSH_HIP(hipMemcpyAsync(&maxS, qmax.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, s));

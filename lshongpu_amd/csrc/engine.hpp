@@ -1,6 +1,9 @@
 // engine.hpp -- the resident index and its search paths.
 #pragma once
 
+#include <map>
+#include <utility>
+
 #include "lshgpu_internal.hpp"
 
 namespace lshgpu {
@@ -233,16 +236,33 @@ struct lsh_index {
     int64_t mfma_launches = 0;   // matrix-core Hamming scan launches of the last search
     int64_t mfma_points = 0;     // data points those launches covered
 
+    // Batched search through captured graphs (search_batches): one graph per
+    // (query count class, rounded k), over buffers sized for kGraphMaxQ
+    // queries; `graph_sig` is the buffers' layout when they were captured.
+    static constexpr int kGraphMaxQ = 4096;
+    std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+    std::vector<uintptr_t> graph_sig;
+    uint32_t* bflags = nullptr;  // pinned: per-query flags of a batch call
+    size_t bflags_n = 0;
+
+    ~lsh_index();
     size_t device_bytes() const;
     void build(const float* src, bool on_device, const float* extra_host, int64_t n_extra);
     void search(const float* queries, int Q, int k, bool q_on_device, lsh_point_t* results,
                 bool r_on_device, hipStream_t s);
+    // n_batches searches of consecutive slices of `queries` (sizes[i] queries
+    // each) into consecutive rows of `results`, each as its own search; with
+    // one host wait for all of them.
+    void search_batches(const float* queries, const int* sizes, int n_batches, int k, bool q_on_device,
+                        lsh_point_t* results, bool r_on_device, hipStream_t s);
+    void drop_graphs();
 
    private:
     void build_fp4(hipStream_t s);
     void build_lsh(const float* extra, int64_t n_extra, hipStream_t s);
     void lsh_keys(const float* x, int64_t n, uint32_t* keys, hipStream_t s);
-    void search_lsh(int Q, int kq, hipStream_t s, lshgpu::PhaseTimer& pt);
+    bool lsh_graphable(int kq) const;
+    void search_lsh(int Q, int kq, hipStream_t s, lshgpu::PhaseTimer& pt, bool enqueue_only = false);
 };
 
 namespace lshgpu {

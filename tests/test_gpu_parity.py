@@ -758,3 +758,83 @@ def test_lsh_rejects_keys_wider_than_the_table(lsh):
         lsh.LSH(2, 7, 32, 64, 4, 100, data, queries, 1, 1, 6, 4, False)  # 2P = 128 > 2^6
     with pytest.raises(lsh.LshError):
         lsh.LSH(2, 4, 32, 64, 4, 100, data, queries, 1, 1, 6, 4, False)  # 8-bit counts > 2^6
+
+
+@pytest.mark.gpu
+def test_autotune_picks_fastest_config_meeting_recall(lsh):
+    # Three (K,L) settings of the dense-rotation keys: the autotuner's recall
+    # must equal the recall the chosen index gives, its choice must reach the
+    # target and be the fastest that does, and a second call is served from
+    # the cache with the same answer.
+    rng = np.random.default_rng(77)
+    N, d, k = 40000, 64, 32
+    centres = rng.standard_normal((100, d)).astype(np.float32)
+    data = centres[rng.integers(0, 100, N)] + 0.25 * rng.standard_normal((N, d)).astype(np.float32)
+    queries = centres[rng.integers(0, 100, 64)] + 0.25 * rng.standard_normal((64, d)).astype(np.float32)
+    base = dict(implementation=2, distanceFunc=1, framework=1, keysImplementation=8)
+    cands = [dict(base, bucketKeyBits=7, tables=8), dict(base, bucketKeyBits=14, tables=8),
+             dict(base, bucketKeyBits=28, tables=2)]
+    best, reps = lsh.autotune(data, queries, k, cands, target_recall=0.8)
+    assert all(r["status"] == 0 for r in reps)
+    truth = lsh.linearScans(2, k, d, len(queries), N, data, queries, 0, 1).ids
+    for c, r in zip(cands, reps):
+        with lsh.Index(data, **c) as ix:
+            got = ix.search(queries, k)["ID"]
+        rec = np.mean([len(set(a) & set(b)) / k for a, b in zip(got, truth)])
+        assert abs(rec - r["recall"]) < 1e-12
+    ok = [i for i, r in enumerate(reps) if r["recall"] >= 0.8]
+    assert best in ok and reps[best]["search_ms"] == min(reps[i]["search_ms"] for i in ok)
+    again, reps2 = lsh.autotune(data, queries, k, cands, target_recall=0.8)
+    assert again == best and reps2 == reps
+
+
+def _dense_lsh_data(rng, N, Q, d):
+    centres = rng.standard_normal((80, d)).astype(np.float32)
+    data = centres[rng.integers(0, 80, N)] + 0.3 * rng.standard_normal((N, d)).astype(np.float32)
+    queries = centres[rng.integers(0, 80, Q)] + 0.3 * rng.standard_normal((Q, d)).astype(np.float32)
+    return data, queries
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("keys,bits", [(8, 14), (3, 12), (7, 8)])
+def test_search_batches_equal_one_search_per_batch(lsh, keys, bits):
+    # Uneven batches (empty, 1, 2, ..., and one above the largest graph
+    # class) through the captured graphs (keys 8 and 3; keys 7 runs them as
+    # ordinary searches) give every query what one search of all of them
+    # gives; a second call replays the graphs; device buffers in and out.
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(500 + keys)
+    N, d = 30000, 64
+    sizes = [1, 0, 2, 37, 5, 4097, 300, 1, 64, 129]
+    data, queries = _dense_lsh_data(rng, N, sum(sizes), d)
+    with lsh.Index(data, implementation=2, distanceFunc=1, framework=1, keysImplementation=keys,
+                   bucketKeyBits=bits, tables=8) as ix:
+        want = ix.search(queries, 32)
+        got = ix.search_batches(queries, sizes, 32)
+        np.testing.assert_array_equal(got, want)
+        again = ix.search_batches(queries, sizes, 32)
+        np.testing.assert_array_equal(again, want)
+        dq = torch.from_numpy(queries).cuda()
+        dout = torch.empty((len(queries), 64), dtype=torch.int32, device="cuda")
+        ix.search_batches(dq, sizes, 32, out=dout)
+        np.testing.assert_array_equal(dout.cpu().numpy().view(lsh.POINT_DTYPE).reshape(len(queries), 32), want)
+        # k = 100 (rounded to 128) captures graphs of its own
+        np.testing.assert_array_equal(ix.search_batches(queries[:500], [250, 250], 100), ix.search(queries[:500], 100))
+
+
+@pytest.mark.gpu
+def test_search_batches_rerun_of_capped_work_list(lsh, orc, monkeypatch):
+    # A work-list capacity of 3 items leaves most queries incomplete: they
+    # are flagged and re-run, and the results still equal the oracle's.
+    rng = np.random.default_rng(510)
+    N, d = 20000, 64
+    data, queries = _dense_lsh_data(rng, N, 40, d)
+    monkeypatch.setenv("LSHGPU_WORKLIST_CAP", "3")
+    res = lsh.LSH(2, 8, 32, d, len(queries), N, data, queries, 1, 1, 14, 6, False)
+    with lsh.Index(data, implementation=2, distanceFunc=1, framework=1, keysImplementation=8, bucketKeyBits=14,
+                   tables=6) as ix:
+        got = ix.search_batches(queries, [7, 33], 32)
+    ids, dist = orc.lsh_search(data, queries, 32, 2, 8, 1, 1, 14, 6, False)
+    np.testing.assert_array_equal(res.ids, ids)
+    np.testing.assert_array_equal(got["ID"], ids)
+    _close(got["distance"], dist)
