@@ -139,6 +139,8 @@ def main() -> int:
                     help="time budget of the single-thread CPU baseline (whole chunks of 4 queries)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-exact", action="store_true")
+    ap.add_argument("--no-fp32-projection", action="store_true",
+                    help="skip the comparison run with the SimHash projections on the FP32 vector units")
     ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2, 3, 4],
                     help="1: the benchmark workload (default); 0, 2, 3, 4: the other configurations, measured for "
                          "information (one JSON line, no roofline or CPU baseline)")
@@ -237,6 +239,44 @@ def main() -> int:
         if phases["mfma_scan_ms"] > 0 else 0.0,
     }
 
+    # ---- the same search with the SimHash projections on the FP32 vector
+    # units (projection 1) beside the default bf16 matrix cores with the FP32
+    # recheck (projection 0): identical sketches, so identical results.
+    projection = None
+    if not args.no_fp32_projection:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ix32 = lsh.Index(data, implementation=3, sketchDim=args.words, candidates=args.candidates, projection=1)
+        torch.cuda.synchronize()
+        build32_ms = (time.perf_counter() - t0) * 1e3
+        out32 = torch.empty_like(out)
+
+        def step32():
+            ix32.search(queries, k, out=out32, stream=stream.cuda_stream)
+
+        for _ in range(args.warmup):
+            step32()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step32()
+        torch.cuda.synchronize()
+        el32 = time.perf_counter() - t0
+        ix32.set_profiling(1)
+        for _ in range(nb):
+            step32()
+        torch.cuda.synchronize()
+        ph32 = ix32.phase_times()
+        ix32.close()
+        same = bool(torch.equal(out32, out))
+        projection = {
+            "bf16_mfma_fp32_recheck": {"queries_per_s": qps, "index_build_ms": build_ms,
+                                       "query_sketch_ms": phases["prep_ms"]},
+            "fp32_valu": {"queries_per_s": Q * args.steps / el32, "index_build_ms": build32_ms,
+                          "query_sketch_ms": ph32["prep_ms"] / nb},
+            "results_identical": same,
+        }
+
     # ---- plain FP32 exact scan on the same GPU (and the truth for recall)
     exact = {}
     recall = None
@@ -287,6 +327,7 @@ def main() -> int:
         "phase_breakdown_note": f"from {nb} fully profiled steps after the timed region",
         "roofline": roofline(args, bytes_launch, avg_launch_ms),
         "threshold_pass": threshold_pass,
+        "simhash_projection": projection,
         "cpu_baseline": cpu,
         "data_gen_s": gen_s,
     }

@@ -338,7 +338,11 @@ typedef struct lsh_index_config {
     int candidates;          /* C; 0 = default */
     int device;
     int rescan_margin;       /* 0 = default 32 */
-    int reserved[8];
+    int projection;          /* SimHash sketches (implementation 3): 0 = bf16 matrix
+                                cores with an FP32 recheck of every bit within the
+                                error bound (d <= 128; bit-identical), 1 = FP32
+                                vector units only */
+    int reserved[7];
 } lsh_index_config_t;
 
 void lsh_index_config_default(lsh_index_config_t* cfg);

@@ -90,7 +90,8 @@ class _IndexConfig(C.Structure):
         ("candidates", C.c_int),
         ("device", C.c_int),
         ("rescan_margin", C.c_int),
-        ("reserved", C.c_int * 8),
+        ("projection", C.c_int),
+        ("reserved", C.c_int * 7),
     ]
 
 
@@ -442,7 +443,7 @@ def bucket_distribution_full_key(hashes, sketchDim: int | None = None):
 
 def _fill_config(cfg, n, d, *, implementation=2, distanceFunc=1, sketchDim=0, framework=0, keysImplementation=0,
                  bucketKeyBits=0, tables=0, runWithSketchedData=False, candidates=None, device=0,
-                 rescan_margin=None) -> None:
+                 rescan_margin=None, projection=0) -> None:
     cfg.implementation = int(implementation)
     cfg.distanceFunc = int(distanceFunc)
     cfg.d = int(d)
@@ -458,6 +459,7 @@ def _fill_config(cfg, n, d, *, implementation=2, distanceFunc=1, sketchDim=0, fr
     if rescan_margin is not None:
         cfg.rescan_margin = int(rescan_margin)
     cfg.device = int(device)
+    cfg.projection = int(projection)
 
 
 def autotune(data, queries, k: int, candidates, target_recall: float = 0.9):
@@ -514,7 +516,7 @@ class Index:
 
     def __init__(self, data, *, implementation=2, distanceFunc=1, sketchDim=0, framework=0,
                  keysImplementation=0, bucketKeyBits=0, tables=0, runWithSketchedData=False, candidates=None,
-                 device=0, rescan_margin=None):
+                 device=0, rescan_margin=None, projection=0):
         lib = load()
         cfg = _IndexConfig()
         lib.lsh_index_config_default(C.byref(cfg))
@@ -529,7 +531,7 @@ class Index:
         _fill_config(cfg, n, d, implementation=implementation, distanceFunc=distanceFunc, sketchDim=sketchDim,
                      framework=framework, keysImplementation=keysImplementation, bucketKeyBits=bucketKeyBits,
                      tables=tables, runWithSketchedData=runWithSketchedData, candidates=candidates, device=device,
-                     rescan_margin=rescan_margin)
+                     rescan_margin=rescan_margin, projection=projection)
         self.d = int(d)
         self.n = int(n)
         self._h = C.c_void_p()

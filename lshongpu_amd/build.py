@@ -35,6 +35,7 @@ SOURCES = [
     "util.hip",
     "noalloc.hip",
     "crosspoly_dense.hip",
+    "simhash_mfma.hip",
     "io.cpp",
 ]
 

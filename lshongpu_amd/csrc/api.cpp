@@ -58,6 +58,8 @@ void configure(lsh_index& ix, const lsh_index_config_t& c) {
     ix.candidates = c.candidates > 0 ? c.candidates : 4096;
     ix.margin = c.rescan_margin > 0 ? c.rescan_margin : 32;
     ix.device = c.device;
+    LSH_REQUIRE(c.projection == 0 || c.projection == 1, LSH_ERR_INVALID, "projection must be 0 or 1");
+    ix.projection = c.projection;
     LSH_REQUIRE(ix.candidates <= lshgpu::kMaxTopK, LSH_ERR_UNSUPPORTED, "candidates must be <= 4096");
 }
 

@@ -194,6 +194,8 @@ struct lsh_index {
     hipEvent_t done_ev = nullptr;
     bool done_rec = false;
     lshgpu::DevBuf<float> planesT;   // SimHash [words][d][32], JL [groups][d][32]
+    int projection = 0;              // 0: SimHash on the matrix cores (FP32 recheck), 1: FP32 only
+    lshgpu::SimhashMfma smf;         // the matrix-core SimHash of planesT, when used
     int sdim = 0;                    // JL sketch length
     lshgpu::DevBuf<float> djl;       // [N][sdim]
 
@@ -261,6 +263,9 @@ struct lsh_index {
     void build_fp4(hipStream_t s);
     void build_lsh(const float* extra, int64_t n_extra, hipStream_t s);
     void lsh_keys(const float* x, int64_t n, uint32_t* keys, hipStream_t s);
+    // SimHash words of rows x (32 hyperplanes a word) by smf when it is built.
+    void sketch_simhash(const float* x, int64_t n, uint32_t* out, int stride, hipStream_t s) const;
+    void build_simhash_planes(hipStream_t s);
     bool lsh_graphable(int kq) const;
     void search_lsh(int Q, int kq, hipStream_t s, lshgpu::PhaseTimer& pt, bool enqueue_only = false);
 };

@@ -295,6 +295,27 @@ int half_row_stride(int d);
 // Error bound of the angular distance computed over that image.
 double half_error_bound(int d);
 
+// simhash_mfma.hip: SimHash bits (32 hyperplanes a word) on the bf16 matrix
+// cores with an FP32 recheck of every bit within the error bound of zero;
+// bit-identical to launch_simhash.  planes is the host [words*32][d]; the
+// device planesT ([words][d][32], FP32) is kept by the caller.  d <= 128.
+bool simhash_mfma_supported(int d, int hashBits);
+struct SimhashMfma {
+    int d = 0, words = 0, KS = 0;
+    const float* planesT = nullptr;
+    DevBuf<uint16_t> ph, pl;  // bfloat16 high and low parts, matrix-core operand order
+    DevBuf<float> pnorm;      // ||p_j||, rounded up
+    DevBuf<float> prow;       // [words*32][d] FP32 hyperplanes (the recheck's rows)
+    mutable DevBuf<uint64_t> fix;   // the recheck list of a sketch call (grow-only)
+    mutable DevBuf<uint32_t> fixn;  // its length
+    void build(const float* planes, int d, int words, const float* planesT_dev, hipStream_t s);
+    // Grows the recheck list when n needs more than it holds (not while a
+    // graph that uses it may replay).
+    void sketch(const float* x, int64_t n, uint32_t* out, int out_stride, hipStream_t s) const;
+    bool built() const { return words > 0; }
+    size_t bytes() const { return ph.bytes() + pl.bytes() + pnorm.bytes() + prow.bytes() + fix.bytes() + fixn.bytes(); }
+};
+
 // sketch.hip (out rows have `out_stride` words; planesT is [words][d][hashBits])
 void launch_simhash(const float* x, int64_t n, int d, const float* planesT, int words,
                     int hashBits, uint32_t* out, int out_stride, hipStream_t s);

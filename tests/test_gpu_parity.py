@@ -126,11 +126,35 @@ def test_exact_scan_near_ties_escalate(lsh, orc):
 @pytest.mark.parametrize("words,bits,n,d", [(32, 32, 3001, 100), (3, 32, 3001, 100), (12, 10, 3001, 100),
                                             (5, 32, 1027, 37), (4, 32, 130, 64), (1, 32, 2, 1),
                                             (7, 32, 515, 129), (2, 17, 70, 300)])
-def test_simhash_bit_exact(lsh, orc, words, bits, n, d):
+@pytest.mark.parametrize("valu", [False, True])
+def test_simhash_bit_exact(lsh, orc, monkeypatch, words, bits, n, d, valu):
+    # 32 bits a word and d <= 128 run on the matrix cores (bf16 splits, FP32
+    # recheck near zero) unless LSHGPU_SIMHASH_VALU=1; the rest on the vector
+    # units.  Both must give the oracle's FP32 bits.
+    if valu:
+        monkeypatch.setenv("LSHGPU_SIMHASH_VALU", "1")
     rng = np.random.default_rng(words * 100 + bits + d)
     x = rng.standard_normal((n, d)).astype(np.float32)
     planes = orc.normal_stream(words * bits * d)
     np.testing.assert_array_equal(lsh.simhash_sketch(x, words, bits), orc.simhash(x, planes, words, bits))
+
+
+@pytest.mark.parametrize("d", [64, 100, 128])
+def test_simhash_matrix_core_borderline_bits(lsh, orc, d):
+    # Rows orthogonal (in FP64) to the first 20 hyperplanes: those projections
+    # are rounding noise around zero, so every such bit takes the FP32
+    # recheck; rows scaled from 2^-60 to 2^60 move the bound with the norm.
+    rng = np.random.default_rng(d)
+    words, n = 2, 700
+    planes = orc.normal_stream(words * 32 * d).reshape(words * 32, d).astype(np.float64)
+    x = rng.standard_normal((n, d))
+    A = planes[:20]
+    x = x - (x @ np.linalg.pinv(A)) @ A
+    x[::7] *= 2.0 ** rng.integers(-60, 60, size=(len(x[::7]), 1))
+    x = x.astype(np.float32)
+    x[3] = 0.0
+    want = orc.simhash(x, planes.astype(np.float32).ravel(), words)
+    np.testing.assert_array_equal(lsh.simhash_sketch(x, words), want)
 
 
 def test_jl_bit_exact(lsh, orc):
@@ -350,6 +374,20 @@ def _index_vs_oracle(lsh, orc, data, queries, words, C, k):
     np.testing.assert_array_equal(res["ID"], oids)
     _close(res["distance"], odist)
     return res
+
+
+@pytest.mark.gpu
+def test_simhash_index_projection_modes_agree(lsh):
+    # The index's data and query sketches on the matrix cores (projection 0)
+    # and on the vector units (1) are the same bits, so searches agree.
+    rng = np.random.default_rng(84)
+    N, Q, d = 100000, 200, 100
+    centres = rng.standard_normal((300, d)).astype(np.float32)
+    data = centres[rng.integers(0, 300, N)] + 0.3 * rng.standard_normal((N, d)).astype(np.float32)
+    queries = centres[rng.integers(0, 300, Q)] + 0.3 * rng.standard_normal((Q, d)).astype(np.float32)
+    with lsh.Index(data, implementation=3, sketchDim=32, candidates=512, projection=0) as a, \
+            lsh.Index(data, implementation=3, sketchDim=32, candidates=512, projection=1) as b:
+        np.testing.assert_array_equal(a.search(queries, 32), b.search(queries, 32))
 
 
 @pytest.mark.gpu
