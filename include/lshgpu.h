@@ -351,7 +351,10 @@ void lsh_index_config_default(lsh_index_config_t* cfg);
 int lsh_index_create(const lsh_index_config_t* cfg, const float* data,
                      int data_on_device, lsh_index_t** out);
 
-/* Search N_query queries for their k nearest (k rounded up to a multiple of
+/* A handle may be used from several threads: the calls on one index are
+ * serialised.
+ *
+ * Search N_query queries for their k nearest (k rounded up to a multiple of
  * 32, as lsh_calculate_k).  results is [N_query][lsh_calculate_k(k)].
  * stream: a hipStream_t (NULL = the index's own stream).  With device results
  * on a given stream the results are complete in that stream's order (the
@@ -430,6 +433,44 @@ typedef struct lsh_tune_report {
 int lsh_autotune(const lsh_index_config_t* candidates, int n_candidates, const float* data, int data_on_device,
                  const float* queries, int n_queries, int k, double target_recall, lsh_tune_report_t* reports,
                  int* chosen);
+
+/* ----------------------------------------------------------------- serving
+ * No counterpart in the reference.  A request loop over a resident index:
+ * requests (host queries and k) are submitted from any thread and returned
+ * through lsh_server_wait.  One worker thread batches every queued request
+ * that fits (continuous batching): at most max_batch_queries queries and, for
+ * the two batches in flight, memory_budget_bytes of device memory (0 = half
+ * of the free device memory; one query is budgeted at 2*(4d + 8*128) + 8*(3C +
+ * 768) + 4096 bytes, C the index's candidates).  A batch runs as one search
+ * at the largest rounded k of its requests; each request receives the first
+ * lsh_calculate_k(k) results of each of its rows, which equal those of
+ * searching its queries alone.  While one batch is on the device the next is
+ * packed and its queries copied on a second stream.  The index must stay
+ * alive until lsh_server_destroy, which serves every request already
+ * submitted before it returns; other calls on the index (from other threads)
+ * are serialised with the server's. */
+typedef struct lsh_server lsh_server_t;
+
+typedef struct lsh_server_config {
+    int max_batch_queries;        /* default 4096 */
+    int max_query_size;           /* largest request accepted; default 2048 */
+    int64_t memory_budget_bytes;  /* 0 = half of the free device memory */
+    int reserved[8];
+} lsh_server_config_t;
+
+void lsh_server_config_default(lsh_server_config_t* cfg);
+int lsh_server_create(lsh_index_t* index, const lsh_server_config_t* cfg, lsh_server_t** out);
+/* queries [n_query][d] and results [n_query][lsh_calculate_k(k)] are host
+ * buffers the caller keeps until lsh_server_wait returns for the ticket.
+ * 1 <= k <= 128. */
+int lsh_server_submit(lsh_server_t* server, const float* queries, int n_query, int k, lsh_point_t* results,
+                      int64_t* ticket);
+/* Blocks until the request is served; returns its status.  latency_ms (may
+ * be NULL): from submission to delivery. */
+int lsh_server_wait(lsh_server_t* server, int64_t ticket, double* latency_ms);
+/* Batches and queries served so far, and the largest batch the budget admits. */
+int lsh_server_stats(lsh_server_t* server, int64_t* batches, int64_t* queries, int* max_batch);
+int lsh_server_destroy(lsh_server_t* server);
 
 /* Device bytes held by the index. */
 int64_t lsh_index_device_bytes(const lsh_index_t* index);

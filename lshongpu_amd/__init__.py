@@ -25,7 +25,7 @@ from typing import Optional
 import numpy as np
 
 __all__ = [
-    "LshError", "Result", "Index", "POINT_DTYPE", "load", "calculateK", "linearScans", "LSH",
+    "LshError", "Result", "Index", "Server", "POINT_DTYPE", "load", "calculateK", "linearScans", "LSH",
     "linear_scans", "lsh_framework", "random_normals", "random_signs", "simhash_sketch",
     "jl_sketch", "gj_transform", "hamming_topc", "wmh_sketch", "crosspoly_keys", "crosspoly_dense_keys",
     "sort_table_keys", "autotune", "library_path",
@@ -96,6 +96,13 @@ class _IndexConfig(C.Structure):
 
 
 _FP = C.POINTER(C.c_float)
+
+
+class _ServerConfig(C.Structure):
+    _fields_ = [("max_batch_queries", C.c_int), ("max_query_size", C.c_int), ("memory_budget_bytes", C.c_int64),
+                ("reserved", C.c_int * 8)]
+
+
 class _TuneReport(C.Structure):
     _fields_ = [("recall", C.c_double), ("search_ms", C.c_double), ("build_ms", C.c_double),
                 ("device_bytes", C.c_int64), ("status", C.c_int)]
@@ -163,6 +170,12 @@ _SIGS = {
     "lsh_index_bucket_stats": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(_BucketStats), C.POINTER(C.c_int64),
                                          C.c_int]),
     "lsh_index_device_bytes": (C.c_int64, [C.c_void_p]),
+    "lsh_server_config_default": (None, [C.POINTER(_ServerConfig)]),
+    "lsh_server_create": (C.c_int, [C.c_void_p, C.POINTER(_ServerConfig), C.POINTER(C.c_void_p)]),
+    "lsh_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_int64)]),
+    "lsh_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_double)]),
+    "lsh_server_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
+    "lsh_server_destroy": (C.c_int, [C.c_void_p]),
     "lsh_autotune": (C.c_int, [C.POINTER(_IndexConfig), C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int,
                                C.c_double, C.POINTER(_TuneReport), C.POINTER(C.c_int)]),
     "lsh_index_destroy": (C.c_int, [C.c_void_p]),
@@ -641,6 +654,69 @@ class Index:
     def close(self) -> None:
         if getattr(self, "_h", None) and self._h.value:
             load().lsh_index_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Server:
+    """Request loop over a resident Index (lsh_server_*): submit() queues a
+    request of host queries and returns a ticket; wait(ticket) returns its
+    (results, latency_ms).  Requests are batched continuously by one worker
+    thread under a device memory budget; submit and wait may be called from
+    any thread."""
+
+    def __init__(self, index: "Index", max_batch_queries: int = 4096, max_query_size=None,
+                 memory_budget_bytes: int = 0):
+        if max_query_size is None:
+            max_query_size = min(2048, int(max_batch_queries))
+        lib = load()
+        cfg = _ServerConfig()
+        lib.lsh_server_config_default(C.byref(cfg))
+        cfg.max_batch_queries = int(max_batch_queries)
+        cfg.max_query_size = int(max_query_size)
+        cfg.memory_budget_bytes = int(memory_budget_bytes)
+        self._index = index  # kept alive while the server runs
+        self._h = C.c_void_p()
+        self._live = {}
+        _check(lib.lsh_server_create(index._h, C.byref(cfg), C.byref(self._h)))
+
+    def submit(self, queries, k: int) -> int:
+        q = _f32(queries)
+        if q.ndim != 2 or q.shape[1] != self._index.d:
+            raise LshError(1, f"queries must be (n, {self._index.d})")
+        res = np.empty((q.shape[0], calculateK(k)), dtype=POINT_DTYPE)
+        t = C.c_int64()
+        _check(load().lsh_server_submit(self._h, C.c_void_p(q.ctypes.data), int(q.shape[0]), int(k),
+                                        C.c_void_p(res.ctypes.data), C.byref(t)))
+        self._live[t.value] = (q, res)
+        return t.value
+
+    def wait(self, ticket: int):
+        lat = C.c_double()
+        st = load().lsh_server_wait(self._h, int(ticket), C.byref(lat))
+        q, res = self._live.pop(int(ticket), (None, None))
+        _check(st)
+        return res, float(lat.value)
+
+    def stats(self) -> dict:
+        b, q, m = C.c_int64(), C.c_int64(), C.c_int()
+        _check(load().lsh_server_stats(self._h, C.byref(b), C.byref(q), C.byref(m)))
+        return {"batches": b.value, "queries": q.value, "max_batch": m.value}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            load().lsh_server_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

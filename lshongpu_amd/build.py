@@ -21,6 +21,7 @@ SOURCES = [
     "common.cpp",
     "api.cpp",
     "autotune.cpp",
+    "serve.cpp",
     "engine.cpp",
     "select.hip",
     "scan_exact.hip",

@@ -357,6 +357,7 @@ int lsh_index_search(lsh_index_t* index, const float* queries, int N_query, int 
     return guarded([&] {
         LSH_REQUIRE(index, LSH_ERR_INVALID, "index is null");
         LSH_REQUIRE(N_query == 0 || (queries && results), LSH_ERR_INVALID, "null pointer");
+        std::lock_guard<std::mutex> lk(index->mu);
         index->search(queries, N_query, k, queries_on_device != 0, results, results_on_device != 0,
                       (hipStream_t)stream);
     });
@@ -368,6 +369,7 @@ int lsh_index_search_batches(lsh_index_t* index, const float* queries, const int
     return guarded([&] {
         LSH_REQUIRE(index, LSH_ERR_INVALID, "index is null");
         LSH_REQUIRE(n_batches == 0 || (queries && results && batch_sizes), LSH_ERR_INVALID, "null pointer");
+        std::lock_guard<std::mutex> lk(index->mu);
         index->search_batches(queries, batch_sizes, n_batches, k, queries_on_device != 0, results,
                               results_on_device != 0, (hipStream_t)stream);
     });
@@ -377,6 +379,7 @@ int lsh_index_set_profiling(lsh_index_t* index, int on) {
     return guarded([&] {
         LSH_REQUIRE(index, LSH_ERR_INVALID, "index is null");
         LSH_REQUIRE(on >= 0 && on <= 2, LSH_ERR_INVALID, "profiling mode must be 0, 1 or 2");
+        std::lock_guard<std::mutex> lk(index->mu);
         // Mode 2 times only the rerank / finalisation phase (index 3).
         index->timer.enable(on == 0 ? 0u : on == 1 ? ~0u : (1u << 3));
         index->prof_mfma_launches = 0;
@@ -387,6 +390,7 @@ int lsh_index_set_profiling(lsh_index_t* index, int on) {
 int lsh_index_phase_times_n(lsh_index_t* index, double* out, int n) {
     return guarded([&] {
         LSH_REQUIRE(index && out && n >= 0, LSH_ERR_INVALID, "null pointer or negative count");
+        std::lock_guard<std::mutex> lk(index->mu);
         double ms[7];
         index->timer.collect(ms, 7);
         const double all[9] = {ms[0], ms[1], ms[2], ms[3], ms[4], ms[5], (double)index->prof_mfma_launches,
@@ -402,6 +406,7 @@ int lsh_index_phase_times(lsh_index_t* index, double* out8) { return lsh_index_p
 int lsh_index_bucket_stats(lsh_index_t* index, int table, lsh_bucket_stats_t* out, int64_t* hist, int hist_len) {
     return guarded([&] {
         LSH_REQUIRE(index && out, LSH_ERR_INVALID, "null pointer");
+        std::lock_guard<std::mutex> lk(index->mu);
         int64_t s4[4];
         lshgpu::index_bucket_stats(index, table, s4, hist, hist_len);
         out->buckets = s4[0];
@@ -412,12 +417,15 @@ int lsh_index_bucket_stats(lsh_index_t* index, int table, lsh_bucket_stats_t* ou
 }
 
 int64_t lsh_index_device_bytes(const lsh_index_t* index) {
-    return index ? (int64_t)index->device_bytes() : -1;
+    if (!index) return -1;
+    std::lock_guard<std::mutex> lk(index->mu);
+    return (int64_t)index->device_bytes();
 }
 
 int lsh_index_destroy(lsh_index_t* index) {
     return guarded([&] {
         if (!index) return;
+        { std::lock_guard<std::mutex> lk(index->mu); }  // no call is inside the index any more
         int prev = 0;
         (void)hipGetDevice(&prev);
         (void)hipSetDevice(index->device);

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <map>
+#include <mutex>
 #include <utility>
 
 #include "lshgpu_internal.hpp"
@@ -152,6 +153,9 @@ const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K
 }  // namespace lshgpu
 
 struct lsh_index {
+    // Held by every entry point that uses the index (searches share its
+    // scratch), so a handle may be used from several threads.
+    mutable std::mutex mu;
     // Resolved configuration.
     int impl = 2;
     int dfunc = 1;
