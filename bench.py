@@ -93,6 +93,66 @@ def roofline(args, bytes_launch: float, avg_launch_ms: float) -> dict:
     }
 
 
+FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix-core peak
+
+
+def exact_scan(lsh, torch, data, queries, k, out_like, stream, valu=False, reps=3) -> dict:
+    """The plain FP32 exact angular scan of `queries` over `data` on this GPU:
+    whole-search time (median of `reps` searches after one warm-up), and the
+    device time of its scan launches alone (events around each, one profiled
+    search).  valu=True builds the index with the scan on the FP32 vector
+    units (LSHGPU_EXACT_VALU=1) instead of the FP32 matrix cores.  Returns the
+    results of the last search as `out`."""
+    old = os.environ.get("LSHGPU_EXACT_VALU")
+    if valu:
+        os.environ["LSHGPU_EXACT_VALU"] = "1"
+    else:
+        os.environ.pop("LSHGPU_EXACT_VALU", None)
+    try:
+        ex = lsh.Index(data, implementation=2, distanceFunc=1)
+    finally:
+        if old is None:
+            os.environ.pop("LSHGPU_EXACT_VALU", None)
+        else:
+            os.environ["LSHGPU_EXACT_VALU"] = old
+    eout = torch.empty_like(out_like)
+    ex.search(queries, k, out=eout, stream=stream.cuda_stream)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        ex.search(queries, k, out=eout, stream=stream.cuda_stream)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    ex.set_profiling(1)
+    ex.search(queries, k, out=eout, stream=stream.cuda_stream)
+    torch.cuda.synchronize()
+    pt = ex.phase_times()
+    ex.set_profiling(False)
+    ex.close()
+    Q, d = queries.shape
+    n = data.shape[0]
+    ms = sorted(times)[len(times) // 2] * 1e3
+    flops = 2.0 * n * Q * d
+    return {"queries_per_s": Q / (ms * 1e-3), "ms": ms, "scan_kernels_ms": pt["scan_ms"],
+            "select_ms": pt["select_ms"], "finalize_ms": pt["final_ms"],
+            "scan_tflops": flops / (pt["scan_ms"] * 1e-3) / 1e12 if pt["scan_ms"] > 0 else None,
+            "search_tflops": flops / (ms * 1e-3) / 1e12, "out": eout}
+
+
+def exact_roofline(ex: dict, n: int, q: int, d: int) -> dict:
+    """The exact scan against the FP32 matrix-core peak: algorithmic flops
+    2*N*Q*d (one multiply-add per point, query and dimension) over the device
+    time of the scan launches of one search."""
+    achieved = ex["scan_tflops"] or 0.0
+    return {"kernel": "exact_mfma_kernel (v_mfma_f32_32x32x2f32 dot products, distances, threshold appends)",
+            "bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+            "algorithmic_flops_per_search": 2.0 * n * q * d, "scan_kernels_ms": ex["scan_kernels_ms"],
+            "whole_search_tflops": ex["search_tflops"],
+            "whole_search_frac": ex["search_tflops"] / FP32_PEAK_TFLOPS}
+
+
 def cpu_baseline(args, data_h, queries_h) -> dict:
     """goldstandard.cpp:136-173 restated (oracle.cpp, with the by-value point
     copies and the full std::sort per query): (a) one thread over a
@@ -284,20 +344,21 @@ def main() -> int:
     exact = {}
     recall = None
     if not args.no_exact:
-        ex = lsh.Index(data, implementation=2, distanceFunc=1)
-        eout = torch.empty_like(out)
-        ex.search(queries, k, out=eout, stream=stream.cuda_stream)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        ex.search(queries, k, out=eout, stream=stream.cuda_stream)
-        torch.cuda.synchronize()
-        ex_s = time.perf_counter() - t0
-        truth = eout.cpu().numpy().view(lsh.POINT_DTYPE).reshape(Q, kq)
-        ex.close()
+        em = exact_scan(lsh, torch, data, queries, k, out, stream)
+        truth = em.pop("out").cpu().numpy().view(lsh.POINT_DTYPE).reshape(Q, kq)
+        ev = exact_scan(lsh, torch, data, queries, k, out, stream, valu=True, reps=1)
+        same = bool(np.array_equal(ev.pop("out").cpu().numpy().view(lsh.POINT_DTYPE).reshape(Q, kq), truth))
         inter = [len(set(a[:k]) & set(b[:k])) / k for a, b in zip(res["ID"], truth["ID"])]
         recall = float(np.mean(inter))
-        exact = {"exact_fp32_scan_qps": Q / ex_s, "exact_fp32_scan_ms": ex_s * 1e3,
-                 "speedup_vs_exact_scan": qps / (Q / ex_s)}
+        # The scan's ceiling: every query against every point at the FP32
+        # matrix-core peak, 2*N*d flops a query.
+        ceiling = FP32_PEAK_TFLOPS * 1e12 / (2.0 * args.n * d)
+        exact = {"exact_fp32_scan_qps": em["queries_per_s"], "exact_fp32_scan_ms": em["ms"],
+                 "speedup_vs_exact_scan": qps / em["queries_per_s"],
+                 "exact_scan_ceiling_qps": ceiling,
+                 "speedup_vs_exact_scan_ceiling": qps / ceiling,
+                 "exact_fp32_scan": {"matrix_cores": em, "vector_units": ev, "results_identical": same,
+                                     "roofline": exact_roofline(em, args.n, Q, d)}}
 
     # ---- reference CPU path (GoldStandard exact k-NN) on query samples
     cpu = None
@@ -369,10 +430,8 @@ def other_config(args, np, torch, lsh, synth, dev) -> int:
     Q, d = queries.shape
     stream = torch.cuda.Stream(device=dev)
     if cfg == 0:
-        workload = "exact angular scan, N=100k d=100, 1,000 queries, k=32"
-        kw = dict(implementation=2, distanceFunc=1)
-        dfunc = 1
-    elif cfg == 2:
+        return exact_config0(args, np, torch, lsh, dev, data_h, queries_h, gen_s)
+    if cfg == 2:
         workload = ("weighted MinHash, 256 values per point, top-32 by matching values, generalized-Jaccard rerank; "
                     "N=1M d=128 (70% zeros), 512 of the 5,000 queries")
         kw = dict(implementation=4, sketchDim=256, distanceFunc=2, candidates=32)
@@ -450,21 +509,49 @@ def other_config(args, np, torch, lsh, synth, dev) -> int:
         same = np.array_equal(out.cpu().numpy().view(lsh.POINT_DTYPE).reshape(Q, kq), res)
         extra["graph_results_equal_eager"] = bool(same)
     ix.close()
-    recall = None
-    if cfg != 0:
-        ex = lsh.Index(data, implementation=2, distanceFunc=dfunc)
-        eout = torch.empty_like(out)
-        ex.search(queries, k, out=eout, stream=stream.cuda_stream)
-        torch.cuda.synchronize()
-        truth = eout.cpu().numpy().view(lsh.POINT_DTYPE).reshape(Q, kq)
-        ex.close()
-        recall = float(np.mean([len(set(a[:k]) & set(b[:k])) / k for a, b in zip(res["ID"], truth["ID"])]))
+    ex = lsh.Index(data, implementation=2, distanceFunc=dfunc)
+    eout = torch.empty_like(out)
+    ex.search(queries, k, out=eout, stream=stream.cuda_stream)
+    torch.cuda.synchronize()
+    truth = eout.cpu().numpy().view(lsh.POINT_DTYPE).reshape(Q, kq)
+    ex.close()
+    recall = float(np.mean([len(set(a[:k]) & set(b[:k])) / k for a, b in zip(res["ID"], truth["ID"])]))
     print(json.dumps({
         "metric": "queries/s (informational configuration)", "value": Q * steps / elapsed, "unit": "queries/s",
         "n_gpus": 1, "steps": steps, "ms_per_step": elapsed * 1e3 / steps, "higher_is_better": True,
         "data": "synthetic, generated in-process", "config": {"workload": workload, "N": int(data_h.shape[0]),
                                                               "d": int(d), "queries": int(Q), "k": k},
         "recall_at_32": recall, "index_build_ms": build_ms, "data_gen_s": gen_s, "autotune": tuning, **extra}))
+    return 0
+
+
+def exact_config0(args, np, torch, lsh, dev, data_h, queries_h, gen_s) -> int:
+    """Configuration 0, the plain exact k-NN: N=100k d=100 unit rows, 1,000
+    queries, k=32 by angular distance, on the FP32 matrix cores and on the
+    FP32 vector units, checked against each other, with the FP32 roofline of
+    the scan launches."""
+    data = torch.from_numpy(data_h).to(dev)
+    queries = torch.from_numpy(queries_h).to(dev)
+    Q, d = queries.shape
+    n = data.shape[0]
+    k = 32
+    kq = lsh.calculateK(k)
+    stream = torch.cuda.Stream(device=dev)
+    out = torch.empty((Q, kq * 2), dtype=torch.int32, device=dev)
+    reps = max(3, min(args.steps, 20))
+    em = exact_scan(lsh, torch, data, queries, k, out, stream, reps=reps)
+    ev = exact_scan(lsh, torch, data, queries, k, out, stream, valu=True, reps=reps)
+    a = em.pop("out").cpu().numpy().view(lsh.POINT_DTYPE).reshape(Q, kq)
+    b = ev.pop("out").cpu().numpy().view(lsh.POINT_DTYPE).reshape(Q, kq)
+    cpu = None if args.no_cpu else cpu_baseline(args, data_h, queries_h)
+    print(json.dumps({
+        "metric": "queries/s (informational configuration)", "value": em["queries_per_s"], "unit": "queries/s",
+        "n_gpus": 1, "steps": reps, "ms_per_step": em["ms"], "higher_is_better": True, "dtype": "f32",
+        "data": "synthetic, generated in-process",
+        "config": {"workload": "exact angular scan, N=100k d=100, 1,000 queries, k=32", "N": int(n), "d": int(d),
+                   "queries": int(Q), "k": k},
+        "matrix_cores": em, "vector_units": ev, "results_identical": bool(np.array_equal(a, b)),
+        "roofline": exact_roofline(em, n, Q, d), "cpu_baseline": cpu, "data_gen_s": gen_s}))
     return 0
 
 

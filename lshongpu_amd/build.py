@@ -25,6 +25,7 @@ SOURCES = [
     "engine.cpp",
     "select.hip",
     "scan_exact.hip",
+    "scan_exact_mfma.hip",
     "scan_sketch.hip",
     "scan_mfma.hip",
     "cells.hip",

@@ -89,6 +89,13 @@ uint32_t worklist_cap(int K) {
     return (uint32_t)((size_t(256) << 20) / (sizeof(uint64_t) * K));
 }
 
+// LSHGPU_EXACT_VALU=1 at index build keeps the exact angular scans on the
+// vector units (the FP32 FMA chain) instead of the matrix cores.
+bool exact_valu_forced() {
+    const char* e = std::getenv("LSHGPU_EXACT_VALU");
+    return e && e[0] == '1';
+}
+
 // LSHGPU_LSH_ROUNDS=1 runs the exact (K,L) rescan in rounds instead of the
 // work list (both give identical results; used to compare them).
 bool force_rounds_lsh() {
@@ -137,20 +144,31 @@ struct ExactQueries {
     int Q;
 };
 
-void exact_topk(const float* data, const float* data_inv, int64_t N, int d, int kind,
+// `dtile` (the rows in matrix-core operand order, or null) selects the
+// matrix-core scan of the angular kinds; its results take mfma_error_bound.
+void exact_topk(const float* data, const float* data_inv, const float* dtile, int64_t N, int d, int kind,
                 const ExactQueries& eq, int K, int mode, TopKState& st, Scratch& sc, int depth, hipStream_t s,
                 PhaseTimer& pt, int64_t& launches) {
-    const int QG = exact_scan_query_group();
-    const int groups = (eq.Q + QG - 1) / QG;
-    float* qT = sc.get<float>(depth, 0, (size_t)groups * d * QG);
     float* qinv = sc.get<float>(depth, 1, (size_t)eq.Q);
-    launch_transpose_queries(eq.rows, eq.Q, d, qT, s);
     if (angular_kind(kind)) launch_row_inv_norms(eq.rows, eq.Q, d, qinv, s);
-    auto scan = [&](int64_t p0, int64_t p1, const TopKView& v) {
-        ExactScanArgs a{data, data_inv, qT, qinv, d, eq.Q, p0, p1, kind};
-        launch_exact_scan(a, v, s);
-    };
-    run_rounds(st, eq.Q, K, N, mode, scan, s, pt, launches);
+    if (dtile) {
+        float* qB = sc.get<float>(depth, 0, (size_t)(eq.Q + 31) / 32 * exact_mfma_steps(d) * 64);
+        launch_tile_rows(eq.rows, eq.Q, d, qB, s);
+        auto scan = [&](int64_t p0, int64_t p1, const TopKView& v) {
+            launch_exact_mfma(dtile, data_inv, d, qB, qinv, eq.Q, p0, p1, kind, v, s);
+        };
+        run_rounds(st, eq.Q, K, N, mode, scan, s, pt, launches);
+    } else {
+        const int QG = exact_scan_query_group();
+        const int groups = (eq.Q + QG - 1) / QG;
+        float* qT = sc.get<float>(depth, 0, (size_t)groups * d * QG);
+        launch_transpose_queries(eq.rows, eq.Q, d, qT, s);
+        auto scan = [&](int64_t p0, int64_t p1, const TopKView& v) {
+            ExactScanArgs a{data, data_inv, qT, qinv, d, eq.Q, p0, p1, kind};
+            launch_exact_scan(a, v, s);
+        };
+        run_rounds(st, eq.Q, K, N, mode, scan, s, pt, launches);
+    }
     if (mode == kFixed) return;
     const std::vector<int32_t> ovf = flagged(st.view().ovf, eq.Q, s);
     if (ovf.empty()) return;
@@ -160,8 +178,8 @@ void exact_topk(const float* data, const float* data_inv, int64_t N, int d, int 
     float* sub = sc.get<float>(depth, 3, (size_t)m * d);
     launch_gather_rows_f32(eq.rows, d, idx, m, sub, s);
     TopKState& sst = sc.tk[depth];
-    exact_topk(data, data_inv, N, d, kind, ExactQueries{sub, m}, K, mode == kNormal ? kCautious : kFixed, sst, sc,
-               depth + 1, s, pt, launches);
+    exact_topk(data, data_inv, dtile, N, d, kind, ExactQueries{sub, m}, K, mode == kNormal ? kCautious : kFixed, sst,
+               sc, depth + 1, s, pt, launches);
     const TopKView sv = sst.view(), v = st.view();
     launch_scatter_rows_u64(sv.keys, sv.stride, K, idx, m, v.keys, v.stride, s);
 }
@@ -528,7 +546,7 @@ void lsh_topk(const LshScanArgs& base, const std::vector<int32_t>& hmap, int64_t
 }  // namespace
 
 size_t lsh_index::device_bytes() const {
-    size_t b = data.bytes() + data_inv.bytes() + dsk.bytes() + dfp4.bytes() + planesT.bytes() + smf.bytes() +
+    size_t b = data.bytes() + data_inv.bytes() + dtile.bytes() + dsk.bytes() + dfp4.bytes() + planesT.bytes() + smf.bytes() +
                djl.bytes() + wmh.bytes() +
                kwmh.bytes() + scratch.bytes() + kplanesT.bytes() + ksigns.bytes() + dkeys.bytes() + boff.bytes() + bperm.bytes() +
                qkeys.bytes() + qlo.bytes() + qpre.bytes() + qmax.bytes() + qlen.bytes() + bkeys.bytes() +
@@ -574,8 +592,15 @@ void lsh_index::build(const float* src, bool on_device, const float* extra_host,
 
     const int kind = impl == 1 ? kNegCos : (impl == 2 ? dfunc : (impl == 4 || impl == 5 ? kGenJaccard : kAngular));
     if (impl <= 2 && angular_kind(kind)) {
-        data_inv.alloc((size_t)N);
+        // Zero past N to a whole tile of 32 rows (the matrix-core scan reads
+        // the inverse norms of whole tiles).
+        data_inv.alloc((size_t)((N + 31) / 32 * 32));
+        if (N > 0) LSH_HIP(hipMemsetAsync(data_inv.get(), 0, data_inv.bytes(), s));
         launch_row_inv_norms(data.get(), N, d, data_inv.get(), s);
+        if (exact_mfma_supported(kind, d) && !exact_valu_forced()) {
+            dtile.alloc((size_t)((N + 31) / 32) * exact_mfma_steps(d) * 64);
+            launch_tile_rows(data.get(), N, d, dtile.get(), s);
+        }
     }
 
     if (impl == 3) {
@@ -825,8 +850,9 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
         const int kind = impl == 1 ? kNegCos : dfunc;
         LSH_REQUIRE(kq <= kMaxTopK, LSH_ERR_UNSUPPORTED, "exact scan supports k <= 4096");
         int K = std::min(kMaxTopK, pow2_ceil(kq + margin));
-        const double eps = fp32_error_bound(kind, d);
-        exact_topk(data.get(), data_inv.get(), N, d, kind, ExactQueries{qp, Q}, K, kNormal, st, scratch, 0, s,
+        const float* dt = dtile.get();
+        const double eps = dt ? mfma_error_bound(d) : fp32_error_bound(kind, d);
+        exact_topk(data.get(), data_inv.get(), dt, N, d, kind, ExactQueries{qp, Q}, K, kNormal, st, scratch, 0, s,
                    pt, scan_launches);
         pt.begin(PH_FINAL, s);
         launch_exact_finalize(data.get(), qp, d, kind, st.view(), kq, eps, outp, flags.get(), s);
@@ -843,8 +869,8 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
             DevBuf<float> sub((size_t)m * d);
             launch_gather_rows_f32(qp, d, idx.get(), m, sub.get(), s);
             TopKState sst;
-            exact_topk(data.get(), data_inv.get(), N, d, kind, ExactQueries{sub.get(), m}, kMaxTopK, kNormal, sst,
-                       scratch, 0, s, pt, scan_launches);
+            exact_topk(data.get(), data_inv.get(), dt, N, d, kind, ExactQueries{sub.get(), m}, kMaxTopK, kNormal,
+                       sst, scratch, 0, s, pt, scan_launches);
             DevBuf<lsh_point_t> sout((size_t)m * kq);
             DevBuf<uint32_t> sflags((size_t)m);
             launch_exact_finalize(data.get(), sub.get(), d, kind, sst.view(), kq, eps, sout.get(), sflags.get(), s);
@@ -921,8 +947,8 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
         const int Ceff = (int)std::min<int64_t>(candidates, N);
         const int K = pow2_ceil(std::max(Ceff, 1));
         LSH_REQUIRE(K <= kMaxTopK, LSH_ERR_UNSUPPORTED, "candidate sets larger than 4096 are not supported");
-        exact_topk(djl.get(), nullptr, N, sdim, kSqL2, ExactQueries{qjl.get(), Q}, K, kNormal, st, scratch, 0, s,
-                   pt, scan_launches);
+        exact_topk(djl.get(), nullptr, nullptr, N, sdim, kSqL2, ExactQueries{qjl.get(), Q}, K, kNormal, st, scratch, 0,
+                   s, pt, scan_launches);
         pt.begin(PH_FINAL, s);
         launch_rerank(data.get(), qp, d, kAngular, st.view().keys, st.view().stride, nullptr, Ceff, Q, kq,
                       fp32_error_bound(kAngular, d), outp, s, nullptr, nullptr, nullptr, &rband);

@@ -181,6 +181,7 @@ struct lsh_index {
     // Working copy of the data after the path's preprocessing.
     lshgpu::DevBuf<float> data;
     lshgpu::DevBuf<float> data_inv;  // 1/||x|| for the angular kinds
+    lshgpu::DevBuf<float> dtile;     // rows in matrix-core operand order (exact angular scans)
 
     // Sketches.
     int words = 0;   // uint32 words per point

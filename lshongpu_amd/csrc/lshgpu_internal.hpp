@@ -248,6 +248,17 @@ struct ExactScanArgs {
     int kind;
 };
 int exact_scan_query_group();
+// scan_exact_mfma.hip: the angular / -cos exact scan on the FP32 matrix
+// cores.  Rows in operand order (launch_tile_rows: tiles of 32 rows,
+// exact_mfma_steps(d) steps of 2 dimensions, 64 floats a step, zero padded);
+// dinv has a multiple of 32 entries (zero past N); the FP64 finalisation of
+// its results takes mfma_error_bound(d).
+bool exact_mfma_supported(int kind, int d);
+int exact_mfma_steps(int d);
+void launch_tile_rows(const float* x, int64_t n, int d, float* out, hipStream_t s);
+void launch_exact_mfma(const float* dT, const float* dinv, int d, const float* qB, const float* qinv, int Q,
+                       int64_t p0, int64_t p1, int kind, const TopKView& v, hipStream_t s);
+double mfma_error_bound(int d);
 void launch_exact_scan(const ExactScanArgs& a, const TopKView& v, hipStream_t s);
 void launch_transpose_queries(const float* q, int Q, int d, float* qT, hipStream_t s);
 void launch_row_inv_norms(const float* x, int64_t n, int d, float* inv, hipStream_t s);

@@ -23,8 +23,20 @@ def _unit(rng, n, d):
 
 # --------------------------------------------------------------- exact scan
 
+# The angular and -cos scans run on the FP32 matrix cores (d <= 128) unless
+# LSHGPU_EXACT_VALU=1 is set when the index is built; both must give the
+# oracle's (FP64, ID) top-k.
+@pytest.fixture(params=["mfma", "valu"])
+def exact_units(request, monkeypatch):
+    if request.param == "valu":
+        monkeypatch.setenv("LSHGPU_EXACT_VALU", "1")
+    else:
+        monkeypatch.delenv("LSHGPU_EXACT_VALU", raising=False)
+    return request.param
+
+
 @pytest.mark.parametrize("impl,func", [(2, 1), (2, 2), (2, 3), (1, 1)])
-def test_exact_scan_matches_oracle(lsh, orc, impl, func):
+def test_exact_scan_matches_oracle(lsh, orc, impl, func, exact_units):
     rng = np.random.default_rng(100 + 10 * impl + func)
     N, Q, d, k = 5000, 70, 100, 32
     data = rng.standard_normal((N, d)).astype(np.float32)
@@ -41,7 +53,7 @@ def test_exact_scan_matches_oracle(lsh, orc, impl, func):
     _close(res.distances, dist)
 
 
-def test_exact_scan_config0_size(lsh, orc):
+def test_exact_scan_config0_size(lsh, orc, exact_units):
     from lshongpu_amd import synth
 
     data, queries = synth.config(0)
@@ -51,7 +63,7 @@ def test_exact_scan_config0_size(lsh, orc):
     _close(res.distances, dist)
 
 
-def test_exact_scan_odd_dims_padding_and_k_rounding(lsh, orc):
+def test_exact_scan_odd_dims_padding_and_k_rounding(lsh, orc, exact_units):
     rng = np.random.default_rng(7)
     N, Q, d = 20, 5, 37
     data = rng.standard_normal((N, d)).astype(np.float32)
@@ -65,7 +77,7 @@ def test_exact_scan_odd_dims_padding_and_k_rounding(lsh, orc):
     _close(res.distances[:, :N], dist[:, :N])
 
 
-def test_exact_scan_duplicates_break_ties_by_index(lsh, orc):
+def test_exact_scan_duplicates_break_ties_by_index(lsh, orc, exact_units):
     rng = np.random.default_rng(8)
     base = _unit(rng, 50, 16)
     data = np.concatenate([base[rng.integers(0, 50, 3000)], _unit(rng, 1000, 16)])
@@ -76,7 +88,7 @@ def test_exact_scan_duplicates_break_ties_by_index(lsh, orc):
     _close(res.distances, dist)
 
 
-def test_exact_scan_large_k(lsh, orc):
+def test_exact_scan_large_k(lsh, orc, exact_units):
     rng = np.random.default_rng(9)
     data = _unit(rng, 6000, 24)
     queries = _unit(rng, 3, 24)
@@ -86,7 +98,7 @@ def test_exact_scan_large_k(lsh, orc):
     _close(res.distances, dist)
 
 
-def test_exact_scan_overflow_fallback_sorted_data(lsh, orc):
+def test_exact_scan_overflow_fallback_sorted_data(lsh, orc, exact_units):
     # Data ordered from farthest to nearest for every query: each scan round
     # appends its whole range, which overflows and takes the re-run path.
     rng = np.random.default_rng(10)
@@ -100,7 +112,7 @@ def test_exact_scan_overflow_fallback_sorted_data(lsh, orc):
     _close(res.distances, dist)
 
 
-def test_exact_scan_near_ties_escalate(lsh, orc):
+def test_exact_scan_near_ties_escalate(lsh, orc, exact_units):
     # 200 points within 1e-7 of each other around the 32nd neighbour: the
     # FP32 margin cannot separate them, so the FP64 recheck has to decide.
     rng = np.random.default_rng(11)
@@ -116,6 +128,33 @@ def test_exact_scan_near_ties_escalate(lsh, orc):
     ids, dist = orc.exact_knn(data, q[None].astype(np.float32), 64, 1)
     np.testing.assert_array_equal(res.ids, ids)
     _close(res.distances, dist)
+
+
+@pytest.mark.parametrize("d", [1, 8, 9, 100, 128, 129])
+def test_exact_scan_matrix_cores_in_use_and_dims(lsh, orc, d, monkeypatch):
+    # The matrix-core scan keeps the rows in operand order (tiles of 32 rows,
+    # 8-dimension steps): its index holds that copy beside the rows, up to
+    # d = 128; past it the vector scan runs.  Ragged N (not a multiple of 32)
+    # and every padded width give the oracle's top-k.
+    rng = np.random.default_rng(300 + d)
+    N, Q, k = 1000 + 7, 45, 32
+    data = rng.standard_normal((N, d)).astype(np.float32)
+    queries = rng.standard_normal((Q, d)).astype(np.float32)
+    sizes = {}
+    for mode in ("mfma", "valu"):
+        if mode == "valu":
+            monkeypatch.setenv("LSHGPU_EXACT_VALU", "1")
+        else:
+            monkeypatch.delenv("LSHGPU_EXACT_VALU", raising=False)
+        ix = lsh.Index(data, implementation=2, distanceFunc=1)
+        sizes[mode] = ix.device_bytes()
+        res = ix.search(queries, k)
+        ids, dist = orc.exact_knn(data, queries, k, 1)
+        np.testing.assert_array_equal(res["ID"], ids)
+        _close(res["distance"], dist)
+        ix.close()
+    tiled = ((N + 31) // 32) * ((d + 7) // 8 * 4) * 64 * 4
+    assert sizes["mfma"] - sizes["valu"] == (tiled if d <= 128 else 0)
 
 
 # ------------------------------------------------------------ sketch stages
