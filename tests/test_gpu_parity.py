@@ -130,10 +130,10 @@ def test_exact_scan_near_ties_escalate(lsh, orc, exact_units):
     _close(res.distances, dist)
 
 
-@pytest.mark.parametrize("d", [1, 8, 9, 100, 128, 129])
+@pytest.mark.parametrize("d", [1, 3, 8, 9, 100, 126, 128, 129])
 def test_exact_scan_matrix_cores_in_use_and_dims(lsh, orc, d, monkeypatch):
     # The matrix-core scan keeps the rows in operand order (tiles of 32 rows,
-    # 8-dimension steps): its index holds that copy beside the rows, up to
+    # widths padded to 4 dimensions): its index holds that copy beside the rows, up to
     # d = 128; past it the vector scan runs.  Ragged N (not a multiple of 32)
     # and every padded width give the oracle's top-k.
     rng = np.random.default_rng(300 + d)
@@ -153,7 +153,7 @@ def test_exact_scan_matrix_cores_in_use_and_dims(lsh, orc, d, monkeypatch):
         np.testing.assert_array_equal(res["ID"], ids)
         _close(res["distance"], dist)
         ix.close()
-    tiled = ((N + 31) // 32) * ((d + 7) // 8 * 4) * 64 * 4
+    tiled = ((N + 31) // 32) * ((d + 3) // 4 * 2) * 64 * 4
     assert sizes["mfma"] - sizes["valu"] == (tiled if d <= 128 else 0)
 
 
