@@ -609,20 +609,23 @@ __global__ __launch_bounds__(1024) void offsets_and_order_kernel(uint32_t* __res
 
 // Coarse distance histogram of the exact first pass: bin = distance / 8,
 // the last bin also holding the distances past it.  In LDS two 16-bit
-// counters share a word; a wave flushes them to the global histogram at the
-// end of its chunk and after every kFlushTiles tiles (32 pairs per query
-// per tile, so no counter reaches 2^16).
-constexpr int kFlushTiles = 2047;
+// counters share a word (a chunk adds at most 32 pairs per query per tile,
+// so no counter reaches 2^16); the workgroup flushes them to the global
+// histogram at the end of its chunk.
+static_assert(32 * kChunkTiles < 65536, "a chunk's pair counts must fit the 16-bit histogram counters");
 __device__ inline int hist_bin(int d) { return min(d >> 3, kHistBins - 1); }
 
-// One wave per chunk of up to 32 queries of one cell.  kHist: counts the
-// distances of every pair in hist[q][bin] (the first pass); otherwise
-// appends the pairs below each query's threshold to v, skipping the tiles
-// whose bound rules them out for every query of the chunk.  The packed
-// point sketches of the next tile are loaded while the current one is
-// multiplied.
+// One workgroup per chunk of up to 32 queries of one cell (chunks taken in a
+// grid-stride loop; their count is on the device).  The chunk's query
+// sketches are expanded once into LDS as the B operand; the tiles that the
+// bounds leave are dealt to the four waves in turn, each wave loading the
+// packed point sketches of its next tile while it multiplies the current
+// one (the A operand expanded through a byte table in LDS).  kHist: counts
+// the distances of every pair in hist[q][bin] (the first pass); otherwise
+// appends the pairs below each query's threshold to v.  Bounded to 128
+// registers, so that four workgroups (16 waves) share a CU.
 template <int KS, bool kHist>
-__global__ __launch_bounds__(256) void cell_scan_kernel(const uint32_t* __restrict__ sk,
+__global__ __launch_bounds__(256, 4) void cell_scan_kernel(const uint32_t* __restrict__ sk,
                                                         const int32_t* __restrict__ ids,
                                                         const uint32_t* __restrict__ tinfo,
                                                         const uint32_t* __restrict__ ctile,
@@ -637,20 +640,21 @@ __global__ __launch_bounds__(256) void cell_scan_kernel(const uint32_t* __restri
     constexpr int W = 2 * KS;
     constexpr int BITS = 32 * W;
     constexpr int HW = kHistBins / 2;  // LDS words per query
-    constexpr int LH = kHist ? 4 * 32 * HW : 1;
+    constexpr int LH = kHist ? 32 * HW : 1;
     constexpr int NV = KS / 4;         // uint4 loads per lane per tile
+    static_assert(kChunkTiles <= 64, "a chunk's tiles must fit one 64-bit mask");
+    __shared__ v4i sb[KS * 64];        // B operand: step s of lane l at [s * 64 + l]
     __shared__ uint32_t lhist[LH];
-    const int lane = threadIdx.x & 63;
-    // Chunks are taken in a grid-stride loop (their count is on the device);
-    // a set skip flag (the second pass found too dense) leaves them all.
+    __shared__ uint32_t lut[256];      // byte -> its 8 FP4 nibbles
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, col = lane & 31;
+    lut[threadIdx.x] = fp4::spread8(threadIdx.x);  // read after the chunk's first barrier
     const uint32_t nchunks = (skip && *skip) ? 0u : choff[P];
-    for (uint32_t c = blockIdx.x * 4 + (threadIdx.x >> 6); c < nchunks; c += gridDim.x * 4) {
+    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
         const uint32_t j = cmap[c];
         const uint32_t parts = cell_parts(ctile, j);
         const uint32_t local = c - choff[j];
         const uint32_t q0 = coff[j] + 32 * (local / parts);
         const uint32_t m = min(32u, coff[j + 1] - q0);
-        const int col = lane & 31;
         const int q = (uint32_t)col < m ? list[q0 + col] : -1;
         uint32_t h = 0;
         float tf = __builtin_inff();  // idle columns never pass
@@ -660,33 +664,34 @@ __global__ __launch_bounds__(256) void cell_scan_kernel(const uint32_t* __restri
             tf = h == 0xFFFFFFFFu ? -__builtin_inff() : (float)BITS - 2.0f * (float)h;
             dq = D[(size_t)q * Pp + j];
         }
-        uint32_t* lh = lhist + (kHist ? (threadIdx.x >> 6) * 32 * HW : 0);
-        auto flush = [&]() {
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            for (int e = lane; e < 32 * HW; e += 64) {
-                const uint32_t x = lh[e];
-                const int cq = e / HW;
-                if (x && (uint32_t)cq < m) {
-                    uint32_t* row = hist + (size_t)list[q0 + cq] * kHistBins + 2 * (e % HW);
-                    if (x & 0xFFFFu) atomicAdd(&row[0], x & 0xFFFFu);
-                    if (x >> 16) atomicAdd(&row[1], x >> 16);
-                }
-                lh[e] = 0;
-            }
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        };
-        if (kHist) {
-            for (int e = lane; e < 32 * HW; e += 64) lh[e] = 0;
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        }
-        v4i b[KS];
-    #pragma unroll
-        for (int s = 0; s < KS; ++s) b[s] = expand_word(q >= 0 ? qs[(size_t)q * W + 2 * s + (lane >> 5)] : 0u);
+        for (int st = wave; st < KS; st += 4)
+            sb[st * 64 + lane] = fp4::expand_word(q >= 0 ? qs[(size_t)q * W + 2 * st + (lane >> 5)] : 0u);
+        if (kHist)
+            for (int e = threadIdx.x; e < LH; e += 256) lhist[e] = 0;
         const uint32_t t0 = ctile[j] + (local % parts) * kChunkTiles;
         const uint32_t nt = min((uint32_t)kChunkTiles, ctile[j + 1] - t0);
+        const uint32_t myinfo = (uint32_t)lane < nt ? tinfo[t0 + lane] : 0u;
+        uint64_t needm = nt == 64 ? ~0ull : ((1ull << nt) - 1ull);
+        if (!kHist) {
+            uint64_t nm = 0;
+            for (uint32_t tt = 0; tt < nt; ++tt) {
+                const uint32_t info = (uint32_t)__shfl((int)myinfo, (int)tt);
+                const uint32_t tlo = info & kDistMask, thi = (info >> kKeyShift) & kDistMask;
+                if (__ballot(q >= 0 && lower_bound_to(dq, tlo, thi) < h)) nm |= 1ull << tt;
+            }
+            needm = nm;
+        }
+        // This wave's share: the needed tiles of rank wave, wave + 4, ...
+        uint64_t mine = 0;
+        {
+            uint64_t mm = needm;
+            for (int r = 0; mm; ++r) {
+                const uint64_t low = mm & (~mm + 1ull);
+                if ((r & 3) == wave) mine |= low;
+                mm ^= low;
+            }
+        }
+        __syncthreads();  // sb, lhist and lut ready
         // Lane l's half of point row l%32 of tile t: W/2 words (and, when
         // appending, the ID of that point).
         auto load = [&](uint32_t t, uint4* x, int32_t& id) {
@@ -719,41 +724,39 @@ __global__ __launch_bounds__(256) void cell_scan_kernel(const uint32_t* __restri
         uint32_t ddist[8] = {};
         uint32_t dpass = 0, dret = 0, dother = 0;
         int32_t did = 0;
-        int since_flush = 0;
-        for (uint32_t i0 = 0; i0 < nt; i0 += 64) {
-            const uint32_t n64 = min(64u, nt - i0);
-            const uint32_t myinfo = (uint32_t)lane < n64 ? tinfo[t0 + i0 + lane] : 0u;
-            uint64_t needm = n64 == 64 ? ~0ull : ((1ull << n64) - 1ull);
-            if (!kHist) {
-                uint64_t nm = 0;
-                for (uint32_t tt = 0; tt < n64; ++tt) {
-                    const uint32_t info = (uint32_t)__shfl((int)myinfo, (int)tt);
-                    const uint32_t tlo = info & kDistMask, thi = (info >> kKeyShift) & kDistMask;
-                    if (__ballot(q >= 0 && lower_bound_to(dq, tlo, thi) < h)) nm |= 1ull << tt;
-                }
-                needm = nm;
-            }
-            if (!needm) continue;
+        if (mine) {
             uint4 nx[NV];
             int32_t nid = 0;
-            int nt_next = __builtin_ctzll(needm);
-            load(t0 + i0 + nt_next, nx, nid);
+            int nt_next = __builtin_ctzll(mine);
+            load(t0 + nt_next, nx, nid);
             while (true) {
                 const int tt = nt_next;
                 uint4 cur[NV];
     #pragma unroll
                 for (int i = 0; i < NV; ++i) cur[i] = nx[i];
                 const int32_t cid = nid;
-                needm &= needm - 1ull;
-                if (needm) {
-                    nt_next = __builtin_ctzll(needm);
-                    load(t0 + i0 + nt_next, nx, nid);
+                mine &= mine - 1ull;
+                if (mine) {
+                    nt_next = __builtin_ctzll(mine);
+                    load(t0 + nt_next, nx, nid);
                 }
                 const uint32_t nv = (uint32_t)__shfl((int)myinfo, tt) >> 22;
                 const uint32_t* cw = reinterpret_cast<const uint32_t*>(cur);
+                // The B offset is made opaque so that the operands are read
+                // from LDS per tile rather than hoisted into registers.
+                uint32_t boff = (uint32_t)lane;
+                asm volatile("" : "+v"(boff));
                 v16f acc = {};
     #pragma unroll
-                for (int s = 0; s < KS; ++s) acc = fp4::mma(expand_word(cw[s]), b[s], acc);
+                for (int st = 0; st < KS; ++st) {
+                    const uint32_t w = cw[st];
+                    v4i a;
+                    a[0] = (int)lut[w & 0xFFu];
+                    a[1] = (int)lut[(w >> 8) & 0xFFu];
+                    a[2] = (int)lut[(w >> 16) & 0xFFu];
+                    a[3] = (int)lut[w >> 24];
+                    acc = fp4::mma(a, sb[boff + st * 64], acc);
+                }
                 uint32_t pass = 0;
     #pragma unroll
                 for (int r = 0; r < 16; ++r)
@@ -763,17 +766,13 @@ __global__ __launch_bounds__(256) void cell_scan_kernel(const uint32_t* __restri
                     for (int r = 0; r < 16; ++r)
                         if ((pass >> r) & 1u) {
                             const int bin = hist_bin((BITS - (int)acc[r]) >> 1);
-                            atomicAdd(&lh[col * HW + (bin >> 1)], 1u << (16 * (bin & 1)));
+                            atomicAdd(&lhist[col * HW + (bin >> 1)], 1u << (16 * (bin & 1)));
                         }
-                    if (++since_flush == kFlushTiles) {
-                        flush();
-                        since_flush = 0;
-                    }
                 } else {
-                    const uint32_t mine = (uint32_t)__popc(pass);
-                    const uint32_t other = (uint32_t)__shfl_xor((int)mine, 32);
+                    const uint32_t mine_n = (uint32_t)__popc(pass);
+                    const uint32_t other = (uint32_t)__shfl_xor((int)mine_n, 32);
                     uint32_t ret = 0;
-                    if (lane < 32 && mine + other > 0) ret = atomicAdd(&v.cnt[q], mine + other);
+                    if (lane < 32 && mine_n + other > 0) ret = atomicAdd(&v.cnt[q], mine_n + other);
                     if (dval) emit(ddist, dpass, dret, dother, did);
                     dval = true;
     #pragma unroll
@@ -785,11 +784,23 @@ __global__ __launch_bounds__(256) void cell_scan_kernel(const uint32_t* __restri
                     dother = other;
                     did = cid;
                 }
-                if (!needm) break;
+                if (!mine) break;
             }
         }
         if (!kHist && dval) emit(ddist, dpass, dret, dother, did);
-        if (kHist && since_flush) flush();
+        __syncthreads();  // every wave is done with sb and lhist
+        if (kHist) {
+            for (int e = threadIdx.x; e < LH; e += 256) {
+                const uint32_t x = lhist[e];
+                const int cq = e / HW;
+                if (x && (uint32_t)cq < m) {
+                    uint32_t* row = hist + (size_t)list[q0 + cq] * kHistBins + 2 * (e % HW);
+                    if (x & 0xFFFFu) atomicAdd(&row[0], x & 0xFFFFu);
+                    if (x >> 16) atomicAdd(&row[1], x >> 16);
+                }
+            }
+            __syncthreads();  // the flush has read lhist before the next chunk clears it
+        }
     }
 }
 
@@ -1113,7 +1124,7 @@ const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K
     const size_t cap_list = (size_t)Q * P, cap_chunks = qgroups * (size_t)c.nparts;
     int32_t* list = sc.get<int32_t>(depth, 2, cap_list + cap_chunks);
     uint32_t* cmap = reinterpret_cast<uint32_t*>(list + cap_list);
-    const unsigned grid = (unsigned)std::max<size_t>(1, std::min<size_t>(4096, (cap_chunks + 3) / 4));
+    const unsigned grid = (unsigned)std::max<size_t>(1, std::min<size_t>(4096, cap_chunks));
     // The second pass is "too dense" when it would multiply more than half
     // of the tiles of a full pass (query tiles x point tiles).
     const uint32_t dense_limit = (uint32_t)std::min<double>(4e9, 0.5 * (double)qgroups * (double)c.ntiles);
