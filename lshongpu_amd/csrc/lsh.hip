@@ -316,6 +316,38 @@ __device__ inline int nth_set_bit(uint64_t m, int n) {
 // MODE 0: exact distance (kind), 16-lane groups per candidate row.
 // MODE 1: integer sketch distance (kind 0 Hamming, 1 unequal words).
 // MODE 2: squared L2 over JL sketches.
+// Whether point id's key equals the query's at a table before t (the point
+// is then scored at that earlier table, so the union is visited once).  The
+// point's keys are one row of L words; with L a multiple of 4 (at most 32)
+// its first t keys are read as 16-byte loads issued together, so the test
+// waits for one memory round trip instead of one per table.
+__device__ inline bool seen_earlier(const uint32_t* __restrict__ dkeys, uint32_t id, int L, int t,
+                                    const uint32_t* s_qk) {
+    const uint32_t* __restrict__ dk = dkeys + (size_t)id * L;
+    if ((L & 3) == 0 && L <= 32) {
+        const uint4* __restrict__ d4 = reinterpret_cast<const uint4*>(dk);
+        const int n4 = (t + 3) >> 2;  // <= L / 4
+        uint4 kv[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (c < n4) kv[c] = d4[c];
+        bool hit = false;
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (c < n4) {
+                const int b = 4 * c;
+                hit |= (b + 0 < t && kv[c].x == s_qk[b + 0]);
+                hit |= (b + 1 < t && kv[c].y == s_qk[b + 1]);
+                hit |= (b + 2 < t && kv[c].z == s_qk[b + 2]);
+                hit |= (b + 3 < t && kv[c].w == s_qk[b + 3]);
+            }
+        return hit;
+    }
+    for (int t2 = 0; t2 < t; ++t2)
+        if (dk[t2] == s_qk[t2]) return true;
+    return false;
+}
+
 template <int MODE>
 __global__ __launch_bounds__(kTPB) void lsh_scan_kernel(LshScanArgs a, TopKView v) {
     extern __shared__ uint32_t lds[];
@@ -366,13 +398,7 @@ __global__ __launch_bounds__(kTPB) void lsh_scan_kernel(LshScanArgs a, TopKView 
                     else hi = mid;
                 }
                 id = a.perm[s_lo[lo] + (uint32_t)(pos - s_pre[lo])];
-                owned = true;
-                const uint32_t* dk = a.dkeys + (size_t)id * L;
-                for (int t2 = 0; t2 < lo; ++t2)
-                    if (dk[t2] == s_qk[t2]) {
-                        owned = false;
-                        break;
-                    }
+                owned = !seen_earlier(a.dkeys, id, L, lo, s_qk);
             }
             if (MODE == 0) {
                 const uint64_t m = __ballot(owned);
@@ -536,13 +562,7 @@ __global__ __launch_bounds__(kTPB) void lsh_partial_kernel(LshScanArgs a, const 
                     else th = mid;
                 }
                 id = a.perm[s_lo[tl] + (uint32_t)(pos - s_pre[tl])];
-                owned = true;
-                const uint32_t* dk = a.dkeys + (size_t)id * L;
-                for (int t2 = 0; t2 < tl; ++t2)
-                    if (dk[t2] == s_qk[t2]) {
-                        owned = false;
-                        break;
-                    }
+                owned = !seen_earlier(a.dkeys, id, L, tl, s_qk);
             }
             const uint64_t m = __ballot(owned);
             const int n = __popcll(m);
