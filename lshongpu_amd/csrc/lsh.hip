@@ -348,6 +348,95 @@ __device__ inline bool seen_earlier(const uint32_t* __restrict__ dkeys, uint32_t
     return false;
 }
 
+// Exact distances of a wave's owned candidates (mask m over the lanes, each
+// lane's candidate in id; n = popcount(m)): 16-lane groups take one
+// candidate each and read its row with coalesced float4 loads.  For rows of
+// at most 128 floats (a multiple of 4) the loads of four candidates per group
+// are issued before any is used, so a round of 16 candidates costs one memory
+// round trip.  Every lane calls emit(leader, cid, dist) for every slot
+// (leader: the slot holds a candidate and this is its group's first lane), so
+// emit may hold wave-wide operations.  The per-candidate arithmetic is the
+// same in both paths (components in index order per lane, then the 16-lane
+// tree), so the distances are too.
+template <class Emit>
+__device__ inline void score_owned(const LshScanArgs& a, const float* s_qf, float qn, uint64_t m, uint32_t id,
+                                   int n, Emit&& emit) {
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, gl = lane & 15;
+    if ((a.d & 3) == 0 && a.d <= 128) {
+        constexpr int U = 4;
+        const int j0 = gl * 4, j1 = gl * 4 + 64;
+        const bool in0 = j0 < a.d, in1 = j1 < a.d;
+        for (int it0 = 0; it0 * 4 < n; it0 += U) {
+            float4 xa[U], xb[U];
+            uint32_t cids[U];
+            bool hs[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int j = (it0 + u) * 4 + g;
+                hs[u] = j < n;
+                const int src = hs[u] ? nth_set_bit(m, j) : 0;
+                cids[u] = (uint32_t)__shfl((int)id, src);
+                const float* __restrict__ row = a.data + (size_t)cids[u] * a.d;
+                xa[u] = hs[u] && in0 ? *reinterpret_cast<const float4*>(row + j0) : make_float4(0.f, 0.f, 0.f, 0.f);
+                xb[u] = hs[u] && in1 ? *reinterpret_cast<const float4*>(row + j1) : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                float x = 0.0f, y = 0.0f;
+                if (hs[u]) {
+                    if (in0) {
+                        group_accumulate(a.kind, xa[u].x, s_qf[j0], x, y);
+                        group_accumulate(a.kind, xa[u].y, s_qf[j0 + 1], x, y);
+                        group_accumulate(a.kind, xa[u].z, s_qf[j0 + 2], x, y);
+                        group_accumulate(a.kind, xa[u].w, s_qf[j0 + 3], x, y);
+                    }
+                    if (in1) {
+                        group_accumulate(a.kind, xb[u].x, s_qf[j1], x, y);
+                        group_accumulate(a.kind, xb[u].y, s_qf[j1 + 1], x, y);
+                        group_accumulate(a.kind, xb[u].z, s_qf[j1 + 2], x, y);
+                        group_accumulate(a.kind, xb[u].w, s_qf[j1 + 3], x, y);
+                    }
+                }
+#pragma unroll
+                for (int o = 8; o > 0; o >>= 1) {
+                    x += __shfl_xor(x, o, 16);
+                    y += __shfl_xor(y, o, 16);
+                }
+                emit(hs[u] && gl == 0, cids[u], group_distance(a.kind, x, y, qn));
+            }
+        }
+        return;
+    }
+    for (int it = 0; it * 4 < n; ++it) {
+        const int j = it * 4 + g;
+        const bool has = j < n;
+        const int src = has ? nth_set_bit(m, j) : 0;
+        const uint32_t cid = (uint32_t)__shfl((int)id, src);
+        float x = 0.0f, y = 0.0f;
+        if (has) {
+            const float* __restrict__ row = a.data + (size_t)cid * a.d;
+            if ((a.d & 3) == 0) {
+                for (int jj = gl * 4; jj < a.d; jj += 64) {
+                    const float4 xv = *reinterpret_cast<const float4*>(row + jj);
+                    group_accumulate(a.kind, xv.x, s_qf[jj], x, y);
+                    group_accumulate(a.kind, xv.y, s_qf[jj + 1], x, y);
+                    group_accumulate(a.kind, xv.z, s_qf[jj + 2], x, y);
+                    group_accumulate(a.kind, xv.w, s_qf[jj + 3], x, y);
+                }
+            } else {
+                for (int jj = gl; jj < a.d; jj += 16) group_accumulate(a.kind, row[jj], s_qf[jj], x, y);
+            }
+        }
+#pragma unroll
+        for (int o = 8; o > 0; o >>= 1) {
+            x += __shfl_xor(x, o, 16);
+            y += __shfl_xor(y, o, 16);
+        }
+        emit(has && gl == 0, cid, group_distance(a.kind, x, y, qn));
+    }
+}
+
 template <int MODE>
 __global__ __launch_bounds__(kTPB) void lsh_scan_kernel(LshScanArgs a, TopKView v) {
     extern __shared__ uint32_t lds[];
@@ -402,37 +491,10 @@ __global__ __launch_bounds__(kTPB) void lsh_scan_kernel(LshScanArgs a, TopKView 
             }
             if (MODE == 0) {
                 const uint64_t m = __ballot(owned);
-                const int n = __popcll(m);
-                const int g = lane >> 4, gl = lane & 15;
-                for (int it = 0; it * 4 < n; ++it) {
-                    const int j = it * 4 + g;
-                    const bool has = j < n;
-                    const int src = has ? nth_set_bit(m, j) : 0;
-                    const uint32_t cid = (uint32_t)__shfl((int)id, src);
-                    float x = 0.0f, y = 0.0f;
-                    if (has) {
-                        const float* __restrict__ row = a.data + (size_t)cid * a.d;
-                        if ((a.d & 3) == 0) {
-                            for (int jj = gl * 4; jj < a.d; jj += 64) {
-                                const float4 xv = *reinterpret_cast<const float4*>(row + jj);
-                                group_accumulate(a.kind, xv.x, s_qf[jj], x, y);
-                                group_accumulate(a.kind, xv.y, s_qf[jj + 1], x, y);
-                                group_accumulate(a.kind, xv.z, s_qf[jj + 2], x, y);
-                                group_accumulate(a.kind, xv.w, s_qf[jj + 3], x, y);
-                            }
-                        } else {
-                            for (int jj = gl; jj < a.d; jj += 16) group_accumulate(a.kind, row[jj], s_qf[jj], x, y);
-                        }
-                    }
-#pragma unroll
-                    for (int o = 8; o > 0; o >>= 1) {
-                        x += __shfl_xor(x, o, 16);
-                        y += __shfl_xor(y, o, 16);
-                    }
-                    const float dist = group_distance(a.kind, x, y, qn);
+                score_owned(a, s_qf, qn, m, id, __popcll(m), [&](bool leader, uint32_t cid, float dist) {
                     const uint64_t key = make_key(f32_key(dist), cid);
-                    append_key(v, r, has && gl == 0 && key < thr, key);
-                }
+                    append_key(v, r, leader && key < thr, key);
+                });
             } else if (MODE == 1) {
                 uint32_t dist = 0;
                 if (owned) {
@@ -565,38 +627,9 @@ __global__ __launch_bounds__(kTPB) void lsh_partial_kernel(LshScanArgs a, const 
                 owned = !seen_earlier(a.dkeys, id, L, tl, s_qk);
             }
             const uint64_t m = __ballot(owned);
-            const int n = __popcll(m);
-            const int g = lane >> 4, gl = lane & 15;
-            for (int it = 0; it * 4 < n; ++it) {
-                const int j = it * 4 + g;
-                const bool has = j < n;
-                const int src = has ? nth_set_bit(m, j) : 0;
-                const uint32_t cid = (uint32_t)__shfl((int)id, src);
-                float x = 0.0f, y = 0.0f;
-                if (has) {
-                    const float* __restrict__ row = a.data + (size_t)cid * a.d;
-                    if ((a.d & 3) == 0) {
-                        for (int jj = gl * 4; jj < a.d; jj += 64) {
-                            const float4 xv = *reinterpret_cast<const float4*>(row + jj);
-                            group_accumulate(a.kind, xv.x, s_qf[jj], x, y);
-                            group_accumulate(a.kind, xv.y, s_qf[jj + 1], x, y);
-                            group_accumulate(a.kind, xv.z, s_qf[jj + 2], x, y);
-                            group_accumulate(a.kind, xv.w, s_qf[jj + 3], x, y);
-                        }
-                    } else {
-                        for (int jj = gl; jj < a.d; jj += 16) group_accumulate(a.kind, row[jj], s_qf[jj], x, y);
-                    }
-                }
-#pragma unroll
-                for (int o = 8; o > 0; o >>= 1) {
-                    x += __shfl_xor(x, o, 16);
-                    y += __shfl_xor(y, o, 16);
-                }
-                if (has && gl == 0) {
-                    const float dist = group_distance(a.kind, x, y, qn);
-                    s_keys[atomicAdd(&s_n, 1u)] = make_key(f32_key(dist), cid);
-                }
-            }
+            score_owned(a, s_qf, qn, m, id, __popcll(m), [&](bool leader, uint32_t cid, float dist) {
+                if (leader) s_keys[atomicAdd(&s_n, 1u)] = make_key(f32_key(dist), cid);
+            });
         }
         __syncthreads();
         // The item's best KP: all of them when there are at most KP, else
