@@ -1172,14 +1172,18 @@ const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K
     cell_scan(c, grid, coff, choff, cmap, Pp, list, D, qs, v, nullptr, stat, s);
     if (debug_cells()) {
         std::vector<uint64_t> ht((size_t)Q);
-        uint32_t hs[3];
+        uint32_t hs[3], pairs2 = 0, chunks2 = 0;
         LSH_HIP(hipMemcpyAsync(ht.data(), v.thr, sizeof(uint64_t) * Q, hipMemcpyDeviceToHost, s));
         LSH_HIP(hipMemcpyAsync(hs, stat, sizeof(hs), hipMemcpyDeviceToHost, s));
+        LSH_HIP(hipMemcpyAsync(&pairs2, coff + P, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        LSH_HIP(hipMemcpyAsync(&chunks2, choff + P, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         LSH_HIP(hipStreamSynchronize(s));
         double mt = 0;
         for (int q = 0; q < Q; ++q) mt += ht[q] == kEmptyKey ? 2.0 * bits : (double)(ht[q] >> 32);
-        std::fprintf(stderr, "cells: P=%d tiles=%lld Q=%d K=%d mean T=%.1f tile products %u + %u%s\n", P,
-                     (long long)c.ntiles, Q, K, mt / Q, hs[1], hs[2], hs[0] ? " (too dense)" : "");
+        std::fprintf(stderr,
+                     "cells: P=%d tiles=%lld Q=%d K=%d mean T=%.1f tile products %u + %u%s; second pass: %u "
+                     "(query, cell) pairs in %u chunks\n",
+                     P, (long long)c.ntiles, Q, K, mt / Q, hs[1], hs[2], hs[0] ? " (too dense)" : "", pairs2, chunks2);
     }
     return stat;
 }

@@ -380,31 +380,52 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
     }
     const float* __restrict__ qv = queries + (size_t)q * d;
 
+    // Candidate IDs and rows first (to LDS), so the row reads below do not
+    // wait on the key reads: each thread issues the loads of four candidates
+    // at a time (keys, then their row-map entries) before it uses any.
+    if (HALF)
+        for (int j = threadIdx.x; j < hs; j += blockDim.x) qs[j] = j < d ? qv[j] : 0.0f;
+    constexpr int B = 4;
+    for (int t0 = threadIdx.x; t0 < Cp; t0 += B * blockDim.x) {
+        int32_t id[B], r[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const int t = t0 + u * (int)blockDim.x;
+            id[u] = -1;
+            if (t < C) {
+                if (cand) {
+                    id[u] = cand[(size_t)q * C + t];
+                } else {
+                    const uint64_t key = keys[(size_t)q * key_stride + t];
+                    id[u] = key == kEmptyKey ? -1 : (int32_t)(uint32_t)key;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < B; ++u) r[u] = id[u] < 0 ? -1 : (rowmap ? rowmap[id[u]] : id[u]);
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const int t = t0 + u * (int)blockDim.x;
+            if (t < Cp) {
+                ids[t] = id[u];
+                rows[t] = r[u];
+            }
+        }
+    }
+    __syncthreads();
+    // The query norm in FP32, the sequential fmaf chain over its components
+    // (with the FP16 image, over the LDS copy: its zero padding adds exact
+    // zeros), and in FP64 for the band.
     float qn = 0.0f;
     double nq64 = 0.0;
     if (KIND == kNegCos || KIND == kAngular) {
-        for (int j = 0; j < d; ++j) qn = __builtin_fmaf(qv[j], qv[j], qn);
+        if (HALF)
+            for (int j = 0; j < hs; ++j) qn = __builtin_fmaf(qs[j], qs[j], qn);
+        else
+            for (int j = 0; j < d; ++j) qn = __builtin_fmaf(qv[j], qv[j], qn);
         qn = __builtin_sqrtf(qn);
-        nq64 = norm2_64(qv, d);
+        if (!BAND) nq64 = norm2_64(qv, d);
     }
-    // Candidate IDs first (to LDS), so the row reads below do not wait on
-    // the key reads.
-    for (int t = threadIdx.x; t < Cp; t += blockDim.x) {
-        int32_t id = -1;
-        if (t < C) {
-            if (cand) {
-                id = cand[(size_t)q * C + t];
-            } else {
-                const uint64_t key = keys[(size_t)q * key_stride + t];
-                id = key == kEmptyKey ? -1 : (int32_t)(uint32_t)key;
-            }
-        }
-        ids[t] = id;
-        rows[t] = id < 0 ? -1 : (rowmap ? rowmap[id] : id);
-    }
-    if (HALF)
-        for (int j = threadIdx.x; j < hs; j += blockDim.x) qs[j] = j < d ? qv[j] : 0.0f;
-    __syncthreads();
     // Approximate distances (k64 serves as scratch for the sums).
     float2* sums = reinterpret_cast<float2*>(k64);
     if (HALF)
