@@ -201,6 +201,9 @@ def main() -> int:
     ap.add_argument("--no-exact", action="store_true")
     ap.add_argument("--no-fp32-projection", action="store_true",
                     help="skip the comparison run with the SimHash projections on the FP32 vector units")
+    ap.add_argument("--no-batches", action="store_true",
+                    help="skip the uneven-batch run (the queries in batches log-uniform in 1..4096, through "
+                         "captured graphs and as one search call per batch)")
     ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2, 3, 4],
                     help="1: the benchmark workload (default); 0, 2, 3, 4: the other configurations, measured for "
                          "information (one JSON line, no roofline or CPU baseline)")
@@ -340,6 +343,48 @@ def main() -> int:
             "results_identical": same,
         }
 
+    # ---- the same queries in uneven batches (sizes log-uniform in 1..4096):
+    # each batch replays the captured graph of its size class with no host
+    # wait between batches, against one ordinary search call per batch.
+    batches = None
+    if not args.no_batches:
+        batches = {}
+        brng = np.random.default_rng(7)
+        for label, top, nq in (("log_uniform_1_4096", 4096.0, Q), ("log_uniform_1_64", 64.0, min(Q, 2000))):
+            sizes, left = [], nq
+            while left > 0:
+                b = min(left, int(round(np.exp(brng.uniform(0.0, np.log(top))))))
+                sizes.append(b)
+                left -= b
+            bounds = np.cumsum([0] + sizes)
+            outb = torch.empty_like(out)
+
+            def step_graphs():
+                ix.search_batches(queries[:nq], sizes, k, out=outb, stream=stream.cuda_stream)
+
+            def step_calls():
+                for a, b in zip(bounds[:-1], bounds[1:]):
+                    ix.search(queries[a:b], k, out=outb[a:b], stream=stream.cuda_stream)
+
+            reps = max(1, min(args.steps, 5))
+            timing = {}
+            for name, fn in (("graphs", step_graphs), ("one_search_call_per_batch", step_calls)):
+                fn()  # warm-up (captures the graphs of the classes)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    fn()
+                torch.cuda.synchronize()
+                timing[name] = nq * reps / (time.perf_counter() - t0)
+                if name == "graphs":
+                    bstats = ix.batch_stats()
+                    same_b = bool(torch.equal(outb[:nq], out[:nq]))
+            batches[label] = {"queries": nq, "batches": len(sizes), "largest": int(max(sizes)),
+                              "graphs_qps": timing["graphs"],
+                              "one_search_call_per_batch_qps": timing["one_search_call_per_batch"],
+                              "graph_batches": bstats["graph_batches"], "redone_queries": bstats["redone_queries"],
+                              "results_equal_one_batch": same_b}
+
     # ---- plain FP32 exact scan on the same GPU (and the truth for recall)
     exact = {}
     recall = None
@@ -392,6 +437,7 @@ def main() -> int:
         "roofline": roofline(args, bytes_launch, avg_launch_ms),
         "threshold_pass": threshold_pass,
         "simhash_projection": projection,
+        "uneven_batches": batches,
         "cpu_baseline": cpu,
         "data_gen_s": gen_s,
     }

@@ -368,17 +368,25 @@ int lsh_index_search(lsh_index_t* index, const float* queries, int N_query,
 
 /* n_batches searches, one per batch: batch i is the next batch_sizes[i] rows
  * of `queries`, and its results the next rows of `results` ([total][k
- * rounded]), as consecutive lsh_index_search calls would give them.  On the
- * exact (K,L) rescan of keysImplementation 3, 4, 5 or 8 (without
- * runWithSketchedData; rounded k + rescan margin <= 512) each batch replays a
- * captured hipGraph of its query-count class (classes about 1.25x apart up
- * to 4096 queries; a larger batch is an ordinary search), with no host wait
- * between batches; other index kinds run the batches as ordinary searches.
- * The call returns when every result is complete (stream: as
- * lsh_index_search). */
+ * rounded]), as consecutive lsh_index_search calls would give them.  Each
+ * batch replays a captured hipGraph of its query-count class (classes about
+ * 1.25x apart up to 4096 queries; a larger batch is an ordinary search),
+ * with no host wait between batches, on the exact (K,L) rescan of
+ * keysImplementation 3, 4, 5 or 8 (without runWithSketchedData; rounded k +
+ * rescan margin <= 512) and on the SimHash index (implementation 3) with
+ * pivot cells (rounded k <= 64, candidates <= 2048); queries a graph leaves
+ * incomplete are redone by ordinary searches before the call returns.  Other
+ * index kinds run the batches as ordinary searches.  The call returns when
+ * every result is complete (stream: as lsh_index_search). */
 int lsh_index_search_batches(lsh_index_t* index, const float* queries, const int* batch_sizes, int n_batches,
                              int k, int queries_on_device, lsh_point_t* results, int results_on_device,
                              void* stream);
+
+/* Diagnostics of the last lsh_index_search_batches call on the index: the
+ * batches that replayed a captured graph, and the queries redone afterwards
+ * by ordinary searches.  No counterpart in the reference (it has no batched
+ * search); for callers that check which path their batches took. */
+int lsh_index_batch_stats(lsh_index_t* index, int64_t* graph_batches, int64_t* redone_queries);
 
 /* Per-phase device times (ms), summed over the searches since profiling was
  * turned on or since the previous lsh_index_phase_times(_n) call (which waits

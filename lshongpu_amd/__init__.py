@@ -165,6 +165,7 @@ _SIGS = {
     "lsh_index_search_batches": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_int,
                                            C.c_int, C.c_void_p, C.c_int, C.c_void_p]),
     "lsh_index_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "lsh_index_batch_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "lsh_index_phase_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "lsh_index_phase_times_n": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_int]),
     "lsh_index_bucket_stats": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(_BucketStats), C.POINTER(C.c_int64),
@@ -623,6 +624,13 @@ class Index:
         _check(lib.lsh_index_search_batches(self._h, qp, sizes.ctypes.data_as(C.POINTER(C.c_int)), len(sizes),
                                             int(k), int(q_dev), rp, int(r_dev), sp))
         return res
+
+    def batch_stats(self) -> dict:
+        """The last search_batches call: batches that replayed a captured
+        graph and queries redone by ordinary searches (lsh_index_batch_stats)."""
+        g, r = C.c_int64(), C.c_int64()
+        _check(load().lsh_index_batch_stats(self._h, C.byref(g), C.byref(r)))
+        return {"graph_batches": g.value, "redone_queries": r.value}
 
     def set_profiling(self, on=True) -> None:
         """True / 1: time every phase; 2: only the rerank / finalisation phase;

@@ -401,6 +401,15 @@ int lsh_index_phase_times_n(lsh_index_t* index, double* out, int n) {
     });
 }
 
+int lsh_index_batch_stats(lsh_index_t* index, int64_t* graph_batches, int64_t* redone_queries) {
+    return guarded([&] {
+        LSH_REQUIRE(index && graph_batches && redone_queries, LSH_ERR_INVALID, "null pointer");
+        std::lock_guard<std::mutex> lk(index->mu);
+        *graph_batches = index->batch_graphs;
+        *redone_queries = index->batch_redone;
+    });
+}
+
 int lsh_index_phase_times(lsh_index_t* index, double* out8) { return lsh_index_phase_times_n(index, out8, 8); }
 
 int lsh_index_bucket_stats(lsh_index_t* index, int table, lsh_bucket_stats_t* out, int64_t* hist, int hist_len) {

@@ -1089,18 +1089,45 @@ size_t CellIndex::bytes() const {
            cmin.bytes() + cmax.bytes() + crec.bytes();
 }
 
+namespace {
+
+// Elements of the four scratch slots a cell search of Q queries takes:
+// D (u16), meta (u32), the lists and chunk maps (i32), the histograms and
+// per-query cell lists (u32).  The layouts are described in cell_search.
+struct CellScratchSizes {
+    size_t d, meta, list, hist;
+};
+CellScratchSizes cell_scratch_sizes(const CellIndex& c, int Q) {
+    const size_t P = (size_t)c.P, Pp = (P + 31) / 32 * 32, q = (size_t)Q;
+    const size_t qgroups = (q + 31) / 32;
+    return {q * Pp, 4 * (P + 1) + 1 + q + 4, q * P + qgroups * (size_t)c.nparts,
+            q * kHistBins + 2 * q * P + 2 * q + (q + 1) / 2};
+}
+
+}  // namespace
+
+void cell_search_reserve(const CellIndex& c, int Q, Scratch& sc, int depth) {
+    if (!c.built() || Q <= 0) return;
+    const CellScratchSizes z = cell_scratch_sizes(c, Q);
+    (void)sc.get<uint16_t>(depth, 0, z.d);
+    (void)sc.get<uint32_t>(depth, 1, z.meta);
+    (void)sc.get<int32_t>(depth, 2, z.list);
+    (void)sc.get<uint32_t>(depth, 3, z.hist);
+}
+
 const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K, const TopKView& v, Scratch& sc,
                             int depth, hipStream_t s, int32_t* qorder) {
     if (!c.built() || Q <= 0) return nullptr;
     const int P = c.P;
     const int Pp = (P + 31) / 32 * 32;
     const int bits = 32 * c.words;
-    uint16_t* D = sc.get<uint16_t>(depth, 0, (size_t)Q * Pp);
+    const CellScratchSizes z = cell_scratch_sizes(c, Q);
+    uint16_t* D = sc.get<uint16_t>(depth, 0, z.d);
     // meta: cnt1, cnt2 [P+1 each], stat [4] = {second pass too dense,
     // first-pass tile products, second-pass tile products}, coff [P+1],
     // choff [P+2], nearest cell [Q].  The first 2 (P+1) + 4 words start at
     // zero (cleared by the pivot-matrix launch).
-    uint32_t* meta = sc.get<uint32_t>(depth, 1, 4 * ((size_t)P + 1) + 1 + Q + 4);
+    uint32_t* meta = sc.get<uint32_t>(depth, 1, z.meta);
     uint32_t* cnt1 = meta;
     uint32_t* cnt2 = meta + (P + 1);
     uint32_t* stat = meta + 2 * (P + 1);
@@ -1111,7 +1138,7 @@ const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K
     // hist [Q][kHistBins], the per-query cell lists [Q][P] x 2 and their
     // lengths [Q] x 2, the radii [Q] (u16).
     const size_t hn = (size_t)Q * kHistBins, qp = (size_t)Q * P;
-    uint32_t* hist = sc.get<uint32_t>(depth, 3, hn + 2 * qp + 2 * (size_t)Q + ((size_t)Q + 1) / 2);
+    uint32_t* hist = sc.get<uint32_t>(depth, 3, z.hist);
     int32_t* qcells1 = reinterpret_cast<int32_t*>(hist + hn);
     int32_t* qcells2 = qcells1 + qp;
     uint32_t* n1 = reinterpret_cast<uint32_t*>(qcells2 + qp);
@@ -1122,7 +1149,7 @@ const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K
     // host before the scans.
     const size_t qgroups = ((size_t)Q + 31) / 32;
     const size_t cap_list = (size_t)Q * P, cap_chunks = qgroups * (size_t)c.nparts;
-    int32_t* list = sc.get<int32_t>(depth, 2, cap_list + cap_chunks);
+    int32_t* list = sc.get<int32_t>(depth, 2, z.list);
     uint32_t* cmap = reinterpret_cast<uint32_t*>(list + cap_list);
     const unsigned grid = (unsigned)std::max<size_t>(1, std::min<size_t>(4096, cap_chunks));
     // The second pass is "too dense" when it would multiply more than half

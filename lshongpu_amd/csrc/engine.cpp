@@ -558,6 +558,9 @@ size_t lsh_index::device_bytes() const {
          rband.cnt.bytes();
     for (const Transform& t : key_steps) b += t.mins.bytes();
     for (const Transform& t : sketch_steps) b += t.mins.bytes();
+    for (const auto& sl : slots)
+        b += sl->q.bytes() + sl->qsk.bytes() + sl->st.bytes() + sl->scratch.bytes() + sl->qorder.bytes() +
+             sl->rband.band.bytes() + sl->rband.cnt.bytes() + sl->gflags.bytes() + sl->out.bytes();
     return b;
 }
 
@@ -1007,6 +1010,12 @@ void lsh_index::search_batches(const float* queries, const int* sizes, int n_bat
         total += sizes[i];
     }
     const int kq = calculate_k(k);
+    batch_graphs = 0;
+    batch_redone = 0;
+    if (sketch_graphable(kq)) {
+        search_batches_sketch(queries, sizes, n_batches, k, total, q_on_device, results, r_on_device, user_stream);
+        return;
+    }
     if (!lsh_graphable(kq)) {
         // No captured form for this path: one search per batch.
         int64_t off = 0;
@@ -1104,6 +1113,7 @@ void lsh_index::search_batches(const float* queries, const int* sizes, int n_bat
             it = graphs.emplace(std::make_pair(Qc, kq), exec).first;
         }
         LSH_HIP(hipGraphLaunch(it->second, s));
+        ++batch_graphs;
         LSH_HIP(hipMemcpyAsync(results + off * kq, out.get(), sizeof(lsh_point_t) * (size_t)b * kq, rk, s));
         LSH_HIP(hipMemcpyAsync(bflags + off, flags.get(), sizeof(uint32_t) * (size_t)b, hipMemcpyDeviceToHost, s));
         LSH_HIP(hipMemcpyAsync(bflags + total + off, wl_flags.get(), sizeof(uint32_t) * (size_t)b,
@@ -1116,6 +1126,7 @@ void lsh_index::search_batches(const float* queries, const int* sizes, int n_bat
     std::vector<int64_t> bad;
     for (int64_t j = 0; j < total; ++j)
         if (bflags[j] || bflags[total + j]) bad.push_back(j);
+    batch_redone = (int64_t)bad.size();
     if (!bad.empty()) {
         std::vector<float> hq((size_t)d);
         std::vector<lsh_point_t> hr((size_t)kq);
@@ -1128,6 +1139,206 @@ void lsh_index::search_batches(const float* queries, const int* sizes, int n_bat
         }
     }
     if (!done_ev) LSH_HIP(hipEventCreateWithFlags(&done_ev, hipEventDisableTiming));
+    LSH_HIP(hipEventRecord(done_ev, s));
+    done_rec = true;
+}
+
+// The SimHash index with pivot cells (impl 3) in captured graphs: the cell
+// search needs no host decision until its select has run, so sketch, cell
+// search, select and rerank are one graph per (query-count class, k).  A
+// batch that leaves a query incomplete (flagged by the select) or finds the
+// cells' second pass too dense is redone afterwards by ordinary searches,
+// whose results replace the graph's.
+bool lsh_index::sketch_graphable(int kq) const {
+    const int64_t ceff = std::max<int64_t>(1, std::min<int64_t>(candidates, N));
+    return framework == 0 && impl == 3 && cells.built() && !cells_dense && sketch_steps.empty() && kq <= 64 &&
+           ceff <= kMaxIntTopK && hamming_mfma_supported(wpad) && !force_valu_scan() && !force_full_pass();
+}
+
+sketch_slot::~sketch_slot() {
+    drop_graphs();
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void sketch_slot::drop_graphs() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    graphs.clear();
+    sig.clear();
+}
+
+void lsh_index::enqueue_sketch_search(sketch_slot& sl, int Qc, int kq) {
+    const hipStream_t s = sl.stream;
+    const int Ceff = (int)std::max<int64_t>(1, std::min<int64_t>(candidates, N));
+    const int K = Ceff;
+    sketch_simhash(sl.q.get(), Qc, sl.qsk.get(), wpad, s);
+    // Lazy reset: the pivot-matrix launch of the cell search clears the view.
+    sl.st.reset(Qc, K, 2 * K + 512, s, true, true, false);
+    const uint32_t* cstat = cell_search(cells, sl.qsk.get(), Qc, K, sl.st.view(), sl.scratch, 0, s, sl.qorder.get());
+    LSH_REQUIRE(cstat != nullptr, LSH_ERR_INTERNAL, "batched search: the cell search did not run");
+    // Flags of the Qc queries, then the cell search's three status words.
+    launch_select_merge(sl.st.view(), s, sl.gflags.get(), cstat, 3);
+    sl.st.selected();
+    const double eps = chalf.get() ? half_error_bound(d) : fp32_error_bound(kAngular, d);
+    const bool split = launch_rerank(cdata.get(), sl.q.get(), d, kAngular, sl.st.view().keys, sl.st.view().stride,
+                                     nullptr, Ceff, Qc, kq, eps, sl.out.get(), s, sl.qorder.get(), cpos.get(),
+                                     chalf.get(), &sl.rband, false);
+    if (split)
+        launch_rerank_recheck(cdata.get(), sl.q.get(), d, kAngular, Ceff, Qc, kq, sl.out.get(), s, sl.qorder.get(),
+                              sl.rband);
+}
+
+void lsh_index::search_batches_sketch(const float* queries, const int* sizes, int n_batches, int k, int64_t total,
+                                      bool q_on_device, lsh_point_t* results, bool r_on_device,
+                                      hipStream_t user_stream) {
+    const int kq = calculate_k(k);
+    hipStream_t s = user_stream ? user_stream : stream;
+    if (done_rec) LSH_HIP(hipStreamWaitEvent(s, done_ev, 0));
+    const int QM = kGraphMaxQ;
+    const int Ceff = (int)std::max<int64_t>(1, std::min<int64_t>(candidates, N));
+    const int Cp = pow2_ceil(std::max(Ceff, 2));
+    while ((int)slots.size() < kSketchSlots) {
+        auto sl = std::make_unique<sketch_slot>();
+        LSH_HIP(hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking));
+        slots.push_back(std::move(sl));
+    }
+    // The slots' streams start after the work queued on s so far.
+    if (!done_ev) LSH_HIP(hipEventCreateWithFlags(&done_ev, hipEventDisableTiming));
+    LSH_HIP(hipEventRecord(done_ev, s));
+    for (auto& sl : slots) LSH_HIP(hipStreamWaitEvent(sl->stream, done_ev, 0));
+    // Every buffer a slot's graphs touch, sized for the largest class
+    // (nothing is allocated while capturing), then the layout its graphs were
+    // captured over: graphs of another layout are dropped.
+    auto prepare = [&](sketch_slot& sl) {
+        sl.q.ensure((size_t)QM * d);
+        const int qrows = sketch_query_rows(QM);
+        const uint32_t* old_qsk = sl.qsk.get();
+        sl.qsk.ensure((size_t)qrows * wpad);
+        // Words past the sketch are never written and must stay zero.
+        if (!sl.qsk_zeroed || sl.qsk.get() != old_qsk) {
+            LSH_HIP(hipMemsetAsync(sl.qsk.get(), 0, sizeof(uint32_t) * (size_t)qrows * wpad, sl.stream));
+            sl.qsk_zeroed = true;
+        }
+        sl.st.reset(QM, Ceff, 2 * Ceff + 512, sl.stream, true, true, false);
+        sl.qorder.ensure((size_t)QM);
+        cell_search_reserve(cells, QM, sl.scratch, 0);
+        sl.rband.band.ensure((size_t)QM * Cp);
+        sl.rband.cnt.ensure((size_t)QM);
+        sl.gflags.ensure((size_t)QM + 3);
+        sl.out.ensure((size_t)QM * 64);
+        const TopKView sv = sl.st.view();
+        const std::vector<uintptr_t> sig = {
+            (uintptr_t)sl.q.get(),      (uintptr_t)sl.qsk.get(),   (uintptr_t)sv.keys,
+            (uintptr_t)sv.cnt,          (uintptr_t)sv.thr,         (uintptr_t)sv.ovf,
+            (uintptr_t)sl.qorder.get(), (uintptr_t)sl.scratch.raw[0].get(), (uintptr_t)sl.scratch.raw[1].get(),
+            (uintptr_t)sl.scratch.raw[2].get(), (uintptr_t)sl.scratch.raw[3].get(), (uintptr_t)sl.rband.band.get(),
+            (uintptr_t)sl.rband.cnt.get(), (uintptr_t)sl.gflags.get(), (uintptr_t)sl.out.get(),
+            (uintptr_t)Ceff};
+        if (sig != sl.sig) {
+            sl.drop_graphs();
+            sl.sig = sig;
+        }
+    };
+    for (auto& sl : slots) prepare(*sl);
+    auto sync_slots = [&] {
+        for (auto& sl : slots) LSH_HIP(hipStreamSynchronize(sl->stream));
+    };
+    // Pinned: the flags of every query, then three status words per batch.
+    const size_t need = (size_t)total + 3 * (size_t)n_batches;
+    if (need > bflags_n) {
+        sync_slots();
+        LSH_HIP(hipStreamSynchronize(s));
+        if (bflags) (void)hipHostFree(bflags);
+        bflags = nullptr;
+        bflags_n = 0;
+        void* p = nullptr;
+        LSH_HIP(hipHostMalloc(&p, sizeof(uint32_t) * std::max<size_t>(need, 1), hipHostMallocDefault));
+        bflags = static_cast<uint32_t*>(p);
+        bflags_n = std::max<size_t>(need, 1);
+    }
+    const hipMemcpyKind qk = q_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const hipMemcpyKind rk = r_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    int64_t off = 0;
+    int next = 0;  // the slot of the next batch
+    for (int i = 0; i < n_batches; ++i) {
+        const int b = sizes[i];
+        uint32_t* ext = bflags + total + 3 * (size_t)i;
+        ext[0] = ext[1] = ext[2] = 0u;
+        if (b == 0) continue;
+        if (b > QM) {
+            // Larger than any class: an ordinary search of its own (complete
+            // when it returns).
+            sync_slots();
+            search(queries + off * d, b, k, q_on_device, results + off * kq, r_on_device, nullptr);
+            for (int j = 0; j < b; ++j) bflags[off + j] = 0u;
+            off += b;
+            continue;
+        }
+        sketch_slot& sl = *slots[next];
+        next = (next + 1) % (int)slots.size();
+        const hipStream_t ss = sl.stream;
+        // The class's rows past the batch repeat its last query (a search of
+        // ordinary cost whose results are not copied out).
+        const int Qc = graph_class(b);
+        LSH_HIP(hipMemcpyAsync(sl.q.get(), queries + off * d, sizeof(float) * (size_t)b * d, qk, ss));
+        launch_repeat_row_f32(sl.q.get(), d, b - 1, Qc - b, ss);
+        auto it = sl.graphs.find({Qc, kq});
+        if (it == sl.graphs.end()) {
+            hipGraph_t graph = nullptr;
+            LSH_HIP(hipStreamBeginCapture(ss, hipStreamCaptureModeThreadLocal));
+            try {
+                enqueue_sketch_search(sl, Qc, kq);
+            } catch (...) {
+                (void)hipStreamEndCapture(ss, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            LSH_HIP(hipStreamEndCapture(ss, &graph));
+            hipGraphExec_t exec = nullptr;
+            const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            LSH_HIP(e);
+            it = sl.graphs.emplace(std::make_pair(Qc, kq), exec).first;
+        }
+        LSH_HIP(hipGraphLaunch(it->second, ss));
+        ++batch_graphs;
+        LSH_HIP(hipMemcpyAsync(results + off * kq, sl.out.get(), sizeof(lsh_point_t) * (size_t)b * kq, rk, ss));
+        LSH_HIP(hipMemcpyAsync(bflags + off, sl.gflags.get(), sizeof(uint32_t) * (size_t)b, hipMemcpyDeviceToHost,
+                               ss));
+        LSH_HIP(hipMemcpyAsync(ext, sl.gflags.get() + Qc, sizeof(uint32_t) * 3, hipMemcpyDeviceToHost, ss));
+        off += b;
+    }
+    sync_slots();
+    // Batches whose second pass was too dense ran no scan: each is redone by
+    // an ordinary search (its sampled threshold pass), and later batch calls
+    // on this index skip the graphs.  Flagged queries of the other batches
+    // are redone together by one ordinary search.
+    std::vector<int64_t> bad;
+    off = 0;
+    for (int i = 0; i < n_batches; ++i) {
+        const int b = sizes[i];
+        if (b > 0 && b <= QM && bflags[total + 3 * (size_t)i]) {
+            cells_dense = true;
+            batch_redone += b;
+            search(queries + off * d, b, k, q_on_device, results + off * kq, r_on_device, nullptr);
+        } else {
+            for (int j = 0; j < b; ++j)
+                if (bflags[off + j]) bad.push_back(off + j);
+        }
+        off += b;
+    }
+    batch_redone += (int64_t)bad.size();
+    if (!bad.empty()) {
+        const int m = (int)bad.size();
+        std::vector<float> hq((size_t)m * d);
+        std::vector<lsh_point_t> hr((size_t)m * kq);
+        for (int i = 0; i < m; ++i)
+            LSH_HIP(hipMemcpy(hq.data() + (size_t)i * d, queries + bad[i] * d, sizeof(float) * d,
+                              q_on_device ? hipMemcpyDeviceToHost : hipMemcpyHostToHost));
+        search(hq.data(), m, k, false, hr.data(), false, nullptr);
+        for (int i = 0; i < m; ++i)
+            LSH_HIP(hipMemcpy(results + bad[i] * kq, hr.data() + (size_t)i * kq, sizeof(lsh_point_t) * kq,
+                              r_on_device ? hipMemcpyHostToDevice : hipMemcpyHostToHost));
+    }
     LSH_HIP(hipEventRecord(done_ev, s));
     done_rec = true;
 }

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <map>
+#include <memory>
 #include <mutex>
 #include <utility>
 
@@ -149,8 +150,35 @@ void cell_rows(const CellIndex& c, const float* data, int64_t N, int d, DevBuf<f
 // entries) receives the queries grouped by their nearest cell.
 const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K, const TopKView& v, Scratch& sc,
                             int depth, hipStream_t s, int32_t* qorder);
+// Sizes the scratch slots of depth `depth` for a cell search of Q queries,
+// so that searches of up to Q queries allocate nothing (graph capture).
+void cell_search_reserve(const CellIndex& c, int Q, Scratch& sc, int depth);
 
 }  // namespace lshgpu
+
+// One batch slot of the SimHash index's captured batches: a stream of its
+// own (default priority), the buffers a graph reads and writes, and the
+// graphs captured over them, one per (query-count class, rounded k).
+// Batches dealt to different slots run concurrently.
+struct sketch_slot {
+    hipStream_t stream = nullptr;
+    lshgpu::DevBuf<float> q;
+    lshgpu::DevBuf<uint32_t> qsk;
+    bool qsk_zeroed = false;
+    lshgpu::TopKState st;
+    lshgpu::Scratch scratch;
+    lshgpu::DevBuf<int32_t> qorder;
+    lshgpu::RerankBand rband;
+    lshgpu::DevBuf<uint32_t> gflags;  // the select's flags, then the cell search's three status words
+    lshgpu::DevBuf<lsh_point_t> out;
+    std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+    std::vector<uintptr_t> sig;  // the buffers' layout when the graphs were captured
+    sketch_slot() = default;
+    sketch_slot(const sketch_slot&) = delete;
+    sketch_slot& operator=(const sketch_slot&) = delete;
+    ~sketch_slot();
+    void drop_graphs();
+};
 
 struct lsh_index {
     // Held by every entry point that uses the index (searches share its
@@ -251,6 +279,15 @@ struct lsh_index {
     std::vector<uintptr_t> graph_sig;
     uint32_t* bflags = nullptr;  // pinned: per-query flags of a batch call
     size_t bflags_n = 0;
+    // SimHash index with pivot cells: the batch slots (created by the first
+    // batch call), and whether a batch found the cells' second pass too
+    // dense (later batch calls then run ordinary searches, whose fallback
+    // handles that).
+    static constexpr int kSketchSlots = 4;
+    std::vector<std::unique_ptr<sketch_slot>> slots;
+    bool cells_dense = false;
+    // The last batch call: batches that replayed a graph, queries redone.
+    int64_t batch_graphs = 0, batch_redone = 0;
 
     ~lsh_index();
     size_t device_bytes() const;
@@ -273,6 +310,13 @@ struct lsh_index {
     void build_simhash_planes(hipStream_t s);
     bool lsh_graphable(int kq) const;
     void search_lsh(int Q, int kq, hipStream_t s, lshgpu::PhaseTimer& pt, bool enqueue_only = false);
+    // The SimHash index's batched search through captured graphs: the cell
+    // search, its select and the rerank of Qc queries held in q, enqueued
+    // with no host wait (flags to gflags, results to out).
+    bool sketch_graphable(int kq) const;
+    void enqueue_sketch_search(sketch_slot& sl, int Qc, int kq);
+    void search_batches_sketch(const float* queries, const int* sizes, int n_batches, int k, int64_t total,
+                               bool q_on_device, lsh_point_t* results, bool r_on_device, hipStream_t s);
 };
 
 namespace lshgpu {

@@ -374,6 +374,9 @@ void launch_wmh_sketch(const float* x, int64_t n, int d, const int* owner, const
                        const int* draw_owner, int R, uint32_t* out, int out_stride, hipStream_t s);
 
 // util.hip
+// Rows src + 1 .. src + n of `rows` (cols floats each) become copies of row
+// src (the padding of a captured batch).
+void launch_repeat_row_f32(float* rows, int cols, int src, int n, hipStream_t s);
 void launch_gather_rows_f32(const float* src, int cols, const int32_t* idx, int m, float* dst,
                             hipStream_t s);
 void launch_gather_rows_u32(const uint32_t* src, int cols, const int32_t* idx, int m, uint32_t* dst,

@@ -43,6 +43,14 @@ __global__ void pad_rows_u32_kernel(const uint32_t* __restrict__ src, int64_t n,
     }
 }
 
+// rows[src + 1 + i] = rows[src] for i < n (rows of `cols` floats).
+__global__ void repeat_row_f32_kernel(float* __restrict__ rows, int cols, int src, int n) {
+    const size_t total = (size_t)n * cols;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (size_t)gridDim.x * blockDim.x)
+        rows[(size_t)(src + 1) * cols + t] = rows[(size_t)src * cols + t % cols];
+}
+
 __global__ void fill_i32_kernel(int32_t* p, size_t n, int32_t v) {
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
          t += (size_t)gridDim.x * blockDim.x)
@@ -61,6 +69,12 @@ void launch_gather_rows_f32(const float* src, int cols, const int32_t* idx, int 
     hipLaunchKernelGGL(gather_rows_kernel<float>, dim3(grid_of((size_t)m * cols)), dim3(256), 0, s, src,
                        cols, idx, m, dst);
     check_launch("gather_rows_kernel<float>");
+}
+
+void launch_repeat_row_f32(float* rows, int cols, int src, int n, hipStream_t s) {
+    if (n <= 0 || cols <= 0) return;
+    hipLaunchKernelGGL(repeat_row_f32_kernel, dim3(grid_of((size_t)n * cols)), dim3(256), 0, s, rows, cols, src, n);
+    check_launch("repeat_row_f32_kernel");
 }
 
 void launch_gather_rows_u32(const uint32_t* src, int cols, const int32_t* idx, int m, uint32_t* dst,

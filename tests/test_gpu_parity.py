@@ -458,6 +458,75 @@ def test_simhash_index_uniform_data_falls_back(lsh, orc):
     _index_vs_oracle(lsh, orc, data, queries, 8, 256, 32)
 
 
+@pytest.mark.gpu
+def test_simhash_index_search_batches_through_graphs(lsh, orc):
+    # Uneven batches (empty, 1, 2, odd sizes, one above the largest graph
+    # class of 4096) through the captured graphs of the SimHash index with
+    # pivot cells give every query what one ordinary search of all of them
+    # gives, and the oracle's.  One query's pass overflows (1,500 copies of
+    # its point at distance 0): the select flags it and it is redone after
+    # the graphs.  A second call replays the graphs; device buffers in and
+    # out; k = 10 shares the k = 32 graphs, k = 100 runs ordinary searches.
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(86)
+    N, d, words, C = 120000, 48, 16, 256
+    sizes = [1, 0, 2, 37, 5, 4097, 300, 1, 64, 129]
+    Q = sum(sizes)
+    centres = rng.standard_normal((400, d)).astype(np.float32)
+    data = centres[rng.integers(0, 400, N)] + 0.3 * rng.standard_normal((N, d)).astype(np.float32)
+    data /= np.linalg.norm(data, axis=1, keepdims=True)
+    copies = rng.choice(N, 1500, replace=False)
+    data[copies] = data[copies[0]]
+    queries = centres[rng.integers(0, 400, Q)] + 0.3 * rng.standard_normal((Q, d)).astype(np.float32)
+    queries /= np.linalg.norm(queries, axis=1, keepdims=True)
+    queries[5] = data[copies[0]]
+    with lsh.Index(data, implementation=3, sketchDim=words, distanceFunc=1, candidates=C) as ix:
+        want = ix.search(queries, 32)
+        got = ix.search_batches(queries, sizes, 32)
+        np.testing.assert_array_equal(got, want)
+        # 8 batches of at most 4096 queries replayed graphs; the overflowing
+        # query was redone
+        st = ix.batch_stats()
+        assert st["graph_batches"] == 8 and st["redone_queries"] >= 1, st
+        np.testing.assert_array_equal(ix.search_batches(queries, sizes, 32), want)
+        dq = torch.from_numpy(queries).cuda()
+        dout = torch.empty((Q, 64), dtype=torch.int32, device="cuda")
+        ix.search_batches(dq, sizes, 32, out=dout)
+        np.testing.assert_array_equal(dout.cpu().numpy().view(lsh.POINT_DTYPE).reshape(Q, 32), want)
+        np.testing.assert_array_equal(ix.search_batches(queries[:300], [150, 7, 143], 10),
+                                      ix.search(queries[:300], 10))
+        np.testing.assert_array_equal(ix.search_batches(queries[:300], [250, 50], 100),
+                                      ix.search(queries[:300], 100))
+        assert ix.batch_stats()["graph_batches"] == 0
+    assert (want["ID"][5] == np.sort(copies)[:32]).all()
+    planes = orc.normal_stream(words * 32 * d)
+    cand, _ = orc.sketch_topc(0, orc.simhash(data, planes, words), orc.simhash(queries[:40], planes, words), C)
+    oids, odist = orc.rerank(data, queries[:40], cand, 32, 1)
+    np.testing.assert_array_equal(got["ID"][:40], oids)
+    _close(got["distance"][:40], odist)
+
+
+@pytest.mark.gpu
+def test_simhash_index_search_batches_dense_cells_redone(lsh, orc):
+    # Unclustered data: a captured batch finds the cells' second pass too
+    # dense and is redone by an ordinary search (the sampled threshold and
+    # the full pass); later calls on the index run ordinary searches.
+    rng = np.random.default_rng(87)
+    data = _unit(rng, 100000, 32)
+    queries = _unit(rng, 300, 32)
+    with lsh.Index(data, implementation=3, sketchDim=8, distanceFunc=1, candidates=256) as ix:
+        want = ix.search(queries, 32)
+        np.testing.assert_array_equal(ix.search_batches(queries, [100, 1, 199], 32), want)
+        st = ix.batch_stats()
+        assert st["graph_batches"] == 3 and st["redone_queries"] > 0, st
+        np.testing.assert_array_equal(ix.search_batches(queries, [3, 297], 32), want)
+        assert ix.batch_stats() == {"graph_batches": 0, "redone_queries": 0}
+    planes = orc.normal_stream(8 * 32 * 32)
+    cand, _ = orc.sketch_topc(0, orc.simhash(data, planes, 8), orc.simhash(queries, planes, 8), 256)
+    oids, _ = orc.rerank(data, queries, cand, 32, 1)
+    np.testing.assert_array_equal(want["ID"], oids)
+
+
 def _rescaled_clusters(rng, N, Q, d, nc=400):
     """Clustered rows with norms spread over 1e-3..1e3 (the angular distance
     does not see the scale; the rerank's FP16 image stores unit rows)."""
