@@ -426,8 +426,9 @@ int lsh_index_bucket_stats(lsh_index_t* index, int table, lsh_bucket_stats_t* ou
  * device), searches the n_queries host queries with k, and measures recall@k
  * against the exact scan of candidates[0].distanceFunc (1..3; angular
  * otherwise) and the median host time of three searches.  *chosen = the
- * fastest candidate whose recall reaches target_recall (the most accurate one
- * when none does).  reports (may be NULL) receives one entry per candidate;
+ * fastest candidate whose recall reaches target_recall, or among the
+ * candidates that reach it within 3% of that time the one of highest recall
+ * (the most accurate one when none reaches it).  reports (may be NULL) receives one entry per candidate;
  * status is the candidate's build/search status (0 = measured).  The choice
  * is cached for the process per (data, queries, k, target, candidate list). */
 typedef struct lsh_tune_report {

@@ -558,9 +558,7 @@ size_t lsh_index::device_bytes() const {
          rband.cnt.bytes();
     for (const Transform& t : key_steps) b += t.mins.bytes();
     for (const Transform& t : sketch_steps) b += t.mins.bytes();
-    for (const auto& sl : slots)
-        b += sl->q.bytes() + sl->qsk.bytes() + sl->st.bytes() + sl->scratch.bytes() + sl->qorder.bytes() +
-             sl->rband.band.bytes() + sl->rband.cnt.bytes() + sl->gflags.bytes() + sl->out.bytes();
+    for (const auto& sl : slots) b += sl->bytes();
     return b;
 }
 
@@ -848,7 +846,7 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
     flags.ensure((size_t)Q);
 
     if (framework == 1) {
-        search_lsh(Q, kq, s, pt);
+        search_lsh(own_work(), Q, kq, s, pt);
     } else if (impl == 1 || impl == 2) {
         const int kind = impl == 1 ? kNegCos : dfunc;
         LSH_REQUIRE(kq <= kMaxTopK, LSH_ERR_UNSUPPORTED, "exact scan supports k <= 4096");
@@ -989,14 +987,29 @@ int graph_class(int b) {
 }  // namespace
 
 lsh_index::~lsh_index() {
-    drop_graphs();
+    slots.clear();
     if (bflags) (void)hipHostFree(bflags);
 }
 
 void lsh_index::drop_graphs() {
+    for (auto& sl : slots) sl->drop_graphs();
+}
+
+batch_slot::~batch_slot() {
+    drop_graphs();
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void batch_slot::drop_graphs() {
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
     graphs.clear();
-    graph_sig.clear();
+    sig.clear();
+}
+
+size_t batch_slot::bytes() const {
+    return q.bytes() + out.bytes() + st.bytes() + flags.bytes() + qsk.bytes() + scratch.bytes() + qorder.bytes() +
+           rband.band.bytes() + rband.cnt.bytes() + qkeys.bytes() + qlo.bytes() + qpre.bytes() + qmax.bytes() +
+           qlen.bytes() + wl_ibase.bytes() + wl_flags.bytes() + wl_part.bytes();
 }
 
 void lsh_index::search_batches(const float* queries, const int* sizes, int n_batches, int k, bool q_on_device,
@@ -1012,11 +1025,8 @@ void lsh_index::search_batches(const float* queries, const int* sizes, int n_bat
     const int kq = calculate_k(k);
     batch_graphs = 0;
     batch_redone = 0;
-    if (sketch_graphable(kq)) {
-        search_batches_sketch(queries, sizes, n_batches, k, total, q_on_device, results, r_on_device, user_stream);
-        return;
-    }
-    if (!lsh_graphable(kq)) {
+    const bool sketch = sketch_graphable(kq);
+    if (!sketch && !lsh_graphable(kq)) {
         // No captured form for this path: one search per batch.
         int64_t off = 0;
         for (int i = 0; i < n_batches; ++i) {
@@ -1027,120 +1037,8 @@ void lsh_index::search_batches(const float* queries, const int* sizes, int n_bat
         if (r_on_device && user_stream) LSH_HIP(hipStreamSynchronize(user_stream));
         return;
     }
-    // Graphs run on the caller's stream, or on the index's own one (the
-    // legacy null stream cannot be captured).
-    hipStream_t s = user_stream ? user_stream : stream;
-    if (done_rec) LSH_HIP(hipStreamWaitEvent(s, done_ev, 0));
-    const int L = tables;
-    const int K = std::min(kMaxTopK, pow2_ceil(kq + margin));
-    const int QM = kGraphMaxQ;
-    // Buffers for the largest class, then the layout the graphs were captured
-    // over: graphs of another layout are dropped.
-    auto prepare = [&] {
-        q.ensure((size_t)QM * d);
-        qkeys.ensure((size_t)QM * L);
-        qlo.ensure((size_t)QM * L);
-        qlen.ensure((size_t)QM * L);
-        qpre.ensure((size_t)QM * (L + 1));
-        qmax.ensure(1);
-        flags.ensure((size_t)QM);
-        wl_ibase.ensure((size_t)QM + 1);
-        wl_flags.ensure((size_t)QM);
-        wl_part.ensure((size_t)worklist_cap(K) * K);
-        out.ensure((size_t)QM * kq);
-        st.reset(QM, K, newcap_for(K), s, false, false, false);
-        const TopKView sv = st.view();
-        const std::vector<uintptr_t> sig = {
-            (uintptr_t)q.get(),      (uintptr_t)qkeys.get(),   (uintptr_t)qlo.get(),     (uintptr_t)qlen.get(),
-            (uintptr_t)qpre.get(),   (uintptr_t)qmax.get(),    (uintptr_t)flags.get(),   (uintptr_t)wl_ibase.get(),
-            (uintptr_t)wl_flags.get(), (uintptr_t)wl_part.get(), (uintptr_t)out.get(),   (uintptr_t)sv.keys,
-            (uintptr_t)sv.cnt,       (uintptr_t)sv.thr,        (uintptr_t)sv.ovf,        (uintptr_t)K,
-            (uintptr_t)worklist_cap(K)};
-        if (sig != graph_sig) {
-            drop_graphs();
-            graph_sig = sig;
-        }
-    };
-    prepare();
-    // Pinned flags of every query: finalisation flags, then capacity flags.
-    if ((size_t)(2 * total) > bflags_n) {
-        LSH_HIP(hipStreamSynchronize(s));
-        if (bflags) (void)hipHostFree(bflags);
-        bflags = nullptr;
-        bflags_n = 0;
-        void* p = nullptr;
-        LSH_HIP(hipHostMalloc(&p, sizeof(uint32_t) * (size_t)std::max<int64_t>(2 * total, 1), hipHostMallocDefault));
-        bflags = static_cast<uint32_t*>(p);
-        bflags_n = (size_t)std::max<int64_t>(2 * total, 1);
-    }
-    const hipMemcpyKind qk = q_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-    const hipMemcpyKind rk = r_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-    int64_t off = 0;
-    for (int i = 0; i < n_batches; ++i) {
-        const int b = sizes[i];
-        if (b == 0) continue;
-        if (b > QM) {
-            // Larger than any class: an ordinary search of its own (complete
-            // when it returns), then the buffers for the graphs again.
-            LSH_HIP(hipStreamSynchronize(s));
-            search(queries + off * d, b, k, q_on_device, results + off * kq, r_on_device, nullptr);
-            for (int j = 0; j < b; ++j) bflags[off + j] = bflags[total + off + j] = 0u;
-            prepare();
-            off += b;
-            continue;
-        }
-        const int Qc = graph_class(b);
-        LSH_HIP(hipMemcpyAsync(q.get(), queries + off * d, sizeof(float) * (size_t)b * d, qk, s));
-        if (Qc > b) LSH_HIP(hipMemsetAsync(q.get() + (size_t)b * d, 0, sizeof(float) * (size_t)(Qc - b) * d, s));
-        auto it = graphs.find({Qc, kq});
-        if (it == graphs.end()) {
-            hipGraph_t graph = nullptr;
-            PhaseTimer none(false);
-            outp = out.get();
-            LSH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            try {
-                search_lsh(Qc, kq, s, none, true);
-            } catch (...) {
-                (void)hipStreamEndCapture(s, &graph);
-                if (graph) (void)hipGraphDestroy(graph);
-                throw;
-            }
-            LSH_HIP(hipStreamEndCapture(s, &graph));
-            hipGraphExec_t exec = nullptr;
-            const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            LSH_HIP(e);
-            it = graphs.emplace(std::make_pair(Qc, kq), exec).first;
-        }
-        LSH_HIP(hipGraphLaunch(it->second, s));
-        ++batch_graphs;
-        LSH_HIP(hipMemcpyAsync(results + off * kq, out.get(), sizeof(lsh_point_t) * (size_t)b * kq, rk, s));
-        LSH_HIP(hipMemcpyAsync(bflags + off, flags.get(), sizeof(uint32_t) * (size_t)b, hipMemcpyDeviceToHost, s));
-        LSH_HIP(hipMemcpyAsync(bflags + total + off, wl_flags.get(), sizeof(uint32_t) * (size_t)b,
-                               hipMemcpyDeviceToHost, s));
-        off += b;
-    }
-    LSH_HIP(hipStreamSynchronize(s));
-    // Queries the work list left incomplete or whose FP32 band was too wide:
-    // redone one by one through the ordinary search (its rounds).
-    std::vector<int64_t> bad;
-    for (int64_t j = 0; j < total; ++j)
-        if (bflags[j] || bflags[total + j]) bad.push_back(j);
-    batch_redone = (int64_t)bad.size();
-    if (!bad.empty()) {
-        std::vector<float> hq((size_t)d);
-        std::vector<lsh_point_t> hr((size_t)kq);
-        for (int64_t j : bad) {
-            LSH_HIP(hipMemcpy(hq.data(), queries + j * d, sizeof(float) * d,
-                              q_on_device ? hipMemcpyDeviceToHost : hipMemcpyHostToHost));
-            search(hq.data(), 1, k, false, hr.data(), false, nullptr);
-            LSH_HIP(hipMemcpy(results + j * kq, hr.data(), sizeof(lsh_point_t) * kq,
-                              r_on_device ? hipMemcpyHostToDevice : hipMemcpyHostToHost));
-        }
-    }
-    if (!done_ev) LSH_HIP(hipEventCreateWithFlags(&done_ev, hipEventDisableTiming));
-    LSH_HIP(hipEventRecord(done_ev, s));
-    done_rec = true;
+    search_batches_graphs(sketch, queries, sizes, n_batches, k, total, q_on_device, results, r_on_device,
+                          user_stream);
 }
 
 // The SimHash index with pivot cells (impl 3) in captured graphs: the cell
@@ -1155,18 +1053,7 @@ bool lsh_index::sketch_graphable(int kq) const {
            ceff <= kMaxIntTopK && hamming_mfma_supported(wpad) && !force_valu_scan() && !force_full_pass();
 }
 
-sketch_slot::~sketch_slot() {
-    drop_graphs();
-    if (stream) (void)hipStreamDestroy(stream);
-}
-
-void sketch_slot::drop_graphs() {
-    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
-    graphs.clear();
-    sig.clear();
-}
-
-void lsh_index::enqueue_sketch_search(sketch_slot& sl, int Qc, int kq) {
+void lsh_index::enqueue_sketch_search(batch_slot& sl, int Qc, int kq) {
     const hipStream_t s = sl.stream;
     const int Ceff = (int)std::max<int64_t>(1, std::min<int64_t>(candidates, N));
     const int K = Ceff;
@@ -1176,7 +1063,7 @@ void lsh_index::enqueue_sketch_search(sketch_slot& sl, int Qc, int kq) {
     const uint32_t* cstat = cell_search(cells, sl.qsk.get(), Qc, K, sl.st.view(), sl.scratch, 0, s, sl.qorder.get());
     LSH_REQUIRE(cstat != nullptr, LSH_ERR_INTERNAL, "batched search: the cell search did not run");
     // Flags of the Qc queries, then the cell search's three status words.
-    launch_select_merge(sl.st.view(), s, sl.gflags.get(), cstat, 3);
+    launch_select_merge(sl.st.view(), s, sl.flags.get(), cstat, 3);
     sl.st.selected();
     const double eps = chalf.get() ? half_error_bound(d) : fp32_error_bound(kAngular, d);
     const bool split = launch_rerank(cdata.get(), sl.q.get(), d, kAngular, sl.st.view().keys, sl.st.view().stride,
@@ -1187,52 +1074,73 @@ void lsh_index::enqueue_sketch_search(sketch_slot& sl, int Qc, int kq) {
                               sl.rband);
 }
 
-void lsh_index::search_batches_sketch(const float* queries, const int* sizes, int n_batches, int k, int64_t total,
-                                      bool q_on_device, lsh_point_t* results, bool r_on_device,
+void lsh_index::search_batches_graphs(bool sketch, const float* queries, const int* sizes, int n_batches, int k,
+                                      int64_t total, bool q_on_device, lsh_point_t* results, bool r_on_device,
                                       hipStream_t user_stream) {
     const int kq = calculate_k(k);
+    // The call's work is ordered after what was queued on the caller's
+    // stream (or the index's own one; the legacy null stream cannot be
+    // captured).
     hipStream_t s = user_stream ? user_stream : stream;
     if (done_rec) LSH_HIP(hipStreamWaitEvent(s, done_ev, 0));
     const int QM = kGraphMaxQ;
+    const int L = tables;
     const int Ceff = (int)std::max<int64_t>(1, std::min<int64_t>(candidates, N));
     const int Cp = pow2_ceil(std::max(Ceff, 2));
-    while ((int)slots.size() < kSketchSlots) {
-        auto sl = std::make_unique<sketch_slot>();
+    const int K = sketch ? Ceff : std::min(kMaxTopK, pow2_ceil(kq + margin));
+    while ((int)slots.size() < kBatchSlots) {
+        auto sl = std::make_unique<batch_slot>();
         LSH_HIP(hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking));
         slots.push_back(std::move(sl));
     }
-    // The slots' streams start after the work queued on s so far.
     if (!done_ev) LSH_HIP(hipEventCreateWithFlags(&done_ev, hipEventDisableTiming));
     LSH_HIP(hipEventRecord(done_ev, s));
     for (auto& sl : slots) LSH_HIP(hipStreamWaitEvent(sl->stream, done_ev, 0));
     // Every buffer a slot's graphs touch, sized for the largest class
     // (nothing is allocated while capturing), then the layout its graphs were
     // captured over: graphs of another layout are dropped.
-    auto prepare = [&](sketch_slot& sl) {
+    auto prepare = [&](batch_slot& sl) {
         sl.q.ensure((size_t)QM * d);
-        const int qrows = sketch_query_rows(QM);
-        const uint32_t* old_qsk = sl.qsk.get();
-        sl.qsk.ensure((size_t)qrows * wpad);
-        // Words past the sketch are never written and must stay zero.
-        if (!sl.qsk_zeroed || sl.qsk.get() != old_qsk) {
-            LSH_HIP(hipMemsetAsync(sl.qsk.get(), 0, sizeof(uint32_t) * (size_t)qrows * wpad, sl.stream));
-            sl.qsk_zeroed = true;
-        }
-        sl.st.reset(QM, Ceff, 2 * Ceff + 512, sl.stream, true, true, false);
-        sl.qorder.ensure((size_t)QM);
-        cell_search_reserve(cells, QM, sl.scratch, 0);
-        sl.rband.band.ensure((size_t)QM * Cp);
-        sl.rband.cnt.ensure((size_t)QM);
-        sl.gflags.ensure((size_t)QM + 3);
         sl.out.ensure((size_t)QM * 64);
+        std::vector<uintptr_t> sig = {(uintptr_t)sketch, (uintptr_t)K, (uintptr_t)sl.q.get(), (uintptr_t)sl.out.get()};
+        if (sketch) {
+            const int qrows = sketch_query_rows(QM);
+            const uint32_t* old_qsk = sl.qsk.get();
+            sl.qsk.ensure((size_t)qrows * wpad);
+            // Words past the sketch are never written and must stay zero.
+            if (!sl.qsk_zeroed || sl.qsk.get() != old_qsk) {
+                LSH_HIP(hipMemsetAsync(sl.qsk.get(), 0, sizeof(uint32_t) * (size_t)qrows * wpad, sl.stream));
+                sl.qsk_zeroed = true;
+            }
+            sl.st.reset(QM, K, 2 * K + 512, sl.stream, true, true, false);
+            sl.qorder.ensure((size_t)QM);
+            cell_search_reserve(cells, QM, sl.scratch, 0);
+            sl.rband.band.ensure((size_t)QM * Cp);
+            sl.rband.cnt.ensure((size_t)QM);
+            sl.flags.ensure((size_t)QM + 3);
+            sig.insert(sig.end(), {(uintptr_t)sl.qsk.get(), (uintptr_t)sl.qorder.get(),
+                                   (uintptr_t)sl.scratch.raw[0].get(), (uintptr_t)sl.scratch.raw[1].get(),
+                                   (uintptr_t)sl.scratch.raw[2].get(), (uintptr_t)sl.scratch.raw[3].get(),
+                                   (uintptr_t)sl.rband.band.get(), (uintptr_t)sl.rband.cnt.get()});
+        } else {
+            sl.qkeys.ensure((size_t)QM * L);
+            sl.qlo.ensure((size_t)QM * L);
+            sl.qlen.ensure((size_t)QM * L);
+            sl.qpre.ensure((size_t)QM * (L + 1));
+            sl.qmax.ensure(1);
+            sl.wl_ibase.ensure((size_t)QM + 1);
+            sl.wl_flags.ensure((size_t)QM);
+            sl.wl_part.ensure((size_t)worklist_cap(K) * K);
+            sl.flags.ensure((size_t)QM);
+            sl.st.reset(QM, K, newcap_for(K), sl.stream, false, false, false);
+            sig.insert(sig.end(), {(uintptr_t)sl.qkeys.get(), (uintptr_t)sl.qlo.get(), (uintptr_t)sl.qlen.get(),
+                                   (uintptr_t)sl.qpre.get(), (uintptr_t)sl.qmax.get(), (uintptr_t)sl.wl_ibase.get(),
+                                   (uintptr_t)sl.wl_flags.get(), (uintptr_t)sl.wl_part.get(),
+                                   (uintptr_t)worklist_cap(K)});
+        }
         const TopKView sv = sl.st.view();
-        const std::vector<uintptr_t> sig = {
-            (uintptr_t)sl.q.get(),      (uintptr_t)sl.qsk.get(),   (uintptr_t)sv.keys,
-            (uintptr_t)sv.cnt,          (uintptr_t)sv.thr,         (uintptr_t)sv.ovf,
-            (uintptr_t)sl.qorder.get(), (uintptr_t)sl.scratch.raw[0].get(), (uintptr_t)sl.scratch.raw[1].get(),
-            (uintptr_t)sl.scratch.raw[2].get(), (uintptr_t)sl.scratch.raw[3].get(), (uintptr_t)sl.rband.band.get(),
-            (uintptr_t)sl.rband.cnt.get(), (uintptr_t)sl.gflags.get(), (uintptr_t)sl.out.get(),
-            (uintptr_t)Ceff};
+        sig.insert(sig.end(), {(uintptr_t)sl.flags.get(), (uintptr_t)sv.keys, (uintptr_t)sv.cnt, (uintptr_t)sv.thr,
+                               (uintptr_t)sv.ovf});
         if (sig != sl.sig) {
             sl.drop_graphs();
             sl.sig = sig;
@@ -1242,8 +1150,10 @@ void lsh_index::search_batches_sketch(const float* queries, const int* sizes, in
     auto sync_slots = [&] {
         for (auto& sl : slots) LSH_HIP(hipStreamSynchronize(sl->stream));
     };
-    // Pinned: the flags of every query, then three status words per batch.
-    const size_t need = (size_t)total + 3 * (size_t)n_batches;
+    // Pinned per call: the flags of every query, then per batch three words
+    // (sketch: the cell search's status; (K,L): none) and, for the (K,L)
+    // rescan, the work list's capacity flags of every query.
+    const size_t need = 2 * (size_t)total + 3 * (size_t)n_batches;
     if (need > bflags_n) {
         sync_slots();
         LSH_HIP(hipStreamSynchronize(s));
@@ -1255,38 +1165,48 @@ void lsh_index::search_batches_sketch(const float* queries, const int* sizes, in
         bflags = static_cast<uint32_t*>(p);
         bflags_n = std::max<size_t>(need, 1);
     }
+    uint32_t* const wflags = bflags + total;                  // (K,L): capacity flags
+    uint32_t* const status = bflags + 2 * (size_t)total;      // three words per batch
     const hipMemcpyKind qk = q_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     const hipMemcpyKind rk = r_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
     int64_t off = 0;
     int next = 0;  // the slot of the next batch
     for (int i = 0; i < n_batches; ++i) {
         const int b = sizes[i];
-        uint32_t* ext = bflags + total + 3 * (size_t)i;
-        ext[0] = ext[1] = ext[2] = 0u;
+        uint32_t* st3 = status + 3 * (size_t)i;
+        st3[0] = st3[1] = st3[2] = 0u;
         if (b == 0) continue;
         if (b > QM) {
             // Larger than any class: an ordinary search of its own (complete
             // when it returns).
             sync_slots();
             search(queries + off * d, b, k, q_on_device, results + off * kq, r_on_device, nullptr);
-            for (int j = 0; j < b; ++j) bflags[off + j] = 0u;
+            for (int j = 0; j < b; ++j) bflags[off + j] = wflags[off + j] = 0u;
             off += b;
             continue;
         }
-        sketch_slot& sl = *slots[next];
+        batch_slot& sl = *slots[next];
         next = (next + 1) % (int)slots.size();
         const hipStream_t ss = sl.stream;
-        // The class's rows past the batch repeat its last query (a search of
-        // ordinary cost whose results are not copied out).
         const int Qc = graph_class(b);
         LSH_HIP(hipMemcpyAsync(sl.q.get(), queries + off * d, sizeof(float) * (size_t)b * d, qk, ss));
-        launch_repeat_row_f32(sl.q.get(), d, b - 1, Qc - b, ss);
+        // The class's rows past the batch: for the SimHash index copies of its
+        // last query (a search of ordinary cost), for the (K,L) rescan zero
+        // rows (they normalise to NaN, whose keys match no real point, so
+        // they cost nearly nothing); their results are not copied out.
+        if (sketch) launch_repeat_row_f32(sl.q.get(), d, b - 1, Qc - b, ss);
+        else if (Qc > b) LSH_HIP(hipMemsetAsync(sl.q.get() + (size_t)b * d, 0, sizeof(float) * (size_t)(Qc - b) * d, ss));
         auto it = sl.graphs.find({Qc, kq});
         if (it == sl.graphs.end()) {
             hipGraph_t graph = nullptr;
             LSH_HIP(hipStreamBeginCapture(ss, hipStreamCaptureModeThreadLocal));
             try {
-                enqueue_sketch_search(sl, Qc, kq);
+                if (sketch) {
+                    enqueue_sketch_search(sl, Qc, kq);
+                } else {
+                    PhaseTimer none(false);
+                    search_lsh(sl.work(), Qc, kq, ss, none, true);
+                }
             } catch (...) {
                 (void)hipStreamEndCapture(ss, &graph);
                 if (graph) (void)hipGraphDestroy(graph);
@@ -1302,27 +1222,31 @@ void lsh_index::search_batches_sketch(const float* queries, const int* sizes, in
         LSH_HIP(hipGraphLaunch(it->second, ss));
         ++batch_graphs;
         LSH_HIP(hipMemcpyAsync(results + off * kq, sl.out.get(), sizeof(lsh_point_t) * (size_t)b * kq, rk, ss));
-        LSH_HIP(hipMemcpyAsync(bflags + off, sl.gflags.get(), sizeof(uint32_t) * (size_t)b, hipMemcpyDeviceToHost,
-                               ss));
-        LSH_HIP(hipMemcpyAsync(ext, sl.gflags.get() + Qc, sizeof(uint32_t) * 3, hipMemcpyDeviceToHost, ss));
+        LSH_HIP(hipMemcpyAsync(bflags + off, sl.flags.get(), sizeof(uint32_t) * (size_t)b, hipMemcpyDeviceToHost, ss));
+        if (sketch)
+            LSH_HIP(hipMemcpyAsync(st3, sl.flags.get() + Qc, sizeof(uint32_t) * 3, hipMemcpyDeviceToHost, ss));
+        else
+            LSH_HIP(hipMemcpyAsync(wflags + off, sl.wl_flags.get(), sizeof(uint32_t) * (size_t)b,
+                                   hipMemcpyDeviceToHost, ss));
         off += b;
     }
     sync_slots();
-    // Batches whose second pass was too dense ran no scan: each is redone by
-    // an ordinary search (its sampled threshold pass), and later batch calls
-    // on this index skip the graphs.  Flagged queries of the other batches
-    // are redone together by one ordinary search.
+    // SimHash batches whose second pass was too dense ran no scan: each is
+    // redone by an ordinary search (its sampled threshold pass), and later
+    // batch calls on this index skip the graphs.  Flagged queries of the
+    // other batches (incomplete, or the (K,L) work list's capacity or FP32
+    // band) are redone together by one ordinary search.
     std::vector<int64_t> bad;
     off = 0;
     for (int i = 0; i < n_batches; ++i) {
         const int b = sizes[i];
-        if (b > 0 && b <= QM && bflags[total + 3 * (size_t)i]) {
+        if (sketch && b > 0 && b <= QM && status[3 * (size_t)i]) {
             cells_dense = true;
             batch_redone += b;
             search(queries + off * d, b, k, q_on_device, results + off * kq, r_on_device, nullptr);
         } else {
             for (int j = 0; j < b; ++j)
-                if (bflags[off + j]) bad.push_back(off + j);
+                if (bflags[off + j] || (!sketch && wflags[off + j])) bad.push_back(off + j);
         }
         off += b;
     }
@@ -1352,54 +1276,54 @@ bool lsh_index::lsh_graphable(int kq) const {
            std::min(kMaxTopK, pow2_ceil(kq + margin)) <= 512 && !force_rounds_lsh();
 }
 
-void lsh_index::search_lsh(int Q, int kq, hipStream_t s, PhaseTimer& pt, bool enqueue_only) {
+void lsh_index::search_lsh(const lsh_work& w, int Q, int kq, hipStream_t s, PhaseTimer& pt, bool enqueue_only) {
     const int L = tables;
     pt.begin(PH_PREP, s);
-    apply_steps_query(key_steps, q.get(), Q, d, s);
-    qkeys.ensure((size_t)Q * L);
-    lsh_keys(q.get(), Q, qkeys.get(), s);
-    apply_steps_query(sketch_steps, q.get(), Q, d, s);
-    qlo.ensure((size_t)Q * L);
-    qpre.ensure((size_t)Q * (L + 1));
-    qmax.ensure(1);
-    launch_fill_i32(reinterpret_cast<int32_t*>(qmax.get()), 1, 0, s);
+    apply_steps_query(key_steps, w.q.get(), Q, d, s);
+    w.qkeys.ensure((size_t)Q * L);
+    lsh_keys(w.q.get(), Q, w.qkeys.get(), s);
+    apply_steps_query(sketch_steps, w.q.get(), Q, d, s);
+    w.qlo.ensure((size_t)Q * L);
+    w.qpre.ensure((size_t)Q * (L + 1));
+    w.qmax.ensure(1);
+    launch_fill_i32(reinterpret_cast<int32_t*>(w.qmax.get()), 1, 0, s);
     if (sorted) {
-        qlen.ensure((size_t)Q * L);
-        launch_query_ranges_sorted(qkeys.get(), Q, L, N, bkeys.get(), qlo.get(), qlen.get(), qpre.get(), qmax.get(),
+        w.qlen.ensure((size_t)Q * L);
+        launch_query_ranges_sorted(w.qkeys.get(), Q, L, N, bkeys.get(), w.qlo.get(), w.qlen.get(), w.qpre.get(), w.qmax.get(),
                                    s);
     } else {
-        launch_query_ranges(qkeys.get(), Q, L, tableSize, boff.get(), qlo.get(), qpre.get(), qmax.get(), s);
+        launch_query_ranges(w.qkeys.get(), Q, L, tableSize, boff.get(), w.qlo.get(), w.qpre.get(), w.qmax.get(), s);
     }
     const bool sketched = withSketch && impl >= 3;
     if (sketched) {
         if (impl == 6) {
             qjl.ensure((size_t)Q * sdim);
-            launch_jl(q.get(), Q, d, planesT.get(), sdim, qjl.get(), s);
+            launch_jl(w.q.get(), Q, d, planesT.get(), sdim, qjl.get(), s);
         } else {
             qsk.ensure((size_t)Q * wpad);
-            if (impl == 3) sketch_simhash(q.get(), Q, qsk.get(), wpad, s);
-            else wmh.sketch(q.get(), Q, d, qsk.get(), wpad, s);
+            if (impl == 3) sketch_simhash(w.q.get(), Q, qsk.get(), wpad, s);
+            else wmh.sketch(w.q.get(), Q, d, qsk.get(), wpad, s);
         }
     }
     pt.end(PH_PREP, s);
     const bool worklist = !sketched && std::min(kMaxTopK, pow2_ceil(kq + margin)) <= 512 && !force_rounds_lsh();
     uint32_t maxS = 0;
     if (!worklist) {
-        LSH_HIP(hipMemcpyAsync(&maxS, qmax.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        LSH_HIP(hipMemcpyAsync(&maxS, w.qmax.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         LSH_HIP(hipStreamSynchronize(s));
     }
 
     LshScanArgs a{};
     a.perm = bperm.get();
     a.dkeys = dkeys.get();
-    a.qkeys = qkeys.get();
-    a.qlo = qlo.get();
-    a.qpre = qpre.get();
+    a.qkeys = w.qkeys.get();
+    a.qlo = w.qlo.get();
+    a.qpre = w.qpre.get();
     a.qmap = nullptr;
     a.L = L;
     a.Q = Q;
     a.data = data.get();
-    a.queries = q.get();
+    a.queries = w.q.get();
     a.d = d;
     a.dsk = dsk.get();
     a.qsk = qsk.get();
@@ -1417,10 +1341,10 @@ void lsh_index::search_lsh(int Q, int kq, hipStream_t s, PhaseTimer& pt, bool en
         const int Ceff = (int)std::max<int64_t>(1, std::min<int64_t>(candidates, N));
         const int K = int_keys && Ceff <= kMaxIntTopK ? Ceff : pow2_ceil(Ceff);
         LSH_REQUIRE(K <= kMaxTopK, LSH_ERR_UNSUPPORTED, "candidate sets larger than 4096 are not supported");
-        lsh_topk(a, none, maxS, K, kNormal, int_keys && K <= kMaxIntTopK, st, scratch, 0, s, pt, scan_launches);
+        lsh_topk(a, none, maxS, K, kNormal, int_keys && K <= kMaxIntTopK, w.st, scratch, 0, s, pt, scan_launches);
         pt.begin(PH_FINAL, s);
-        launch_rerank(data.get(), q.get(), d, kind, st.view().keys, st.view().stride, nullptr, Ceff, Q, kq, eps,
-                      outp, s, nullptr, nullptr, nullptr, &rband);
+        launch_rerank(data.get(), w.q.get(), d, kind, w.st.view().keys, w.st.view().stride, nullptr, Ceff, Q, kq, eps,
+                      w.outp, s, nullptr, nullptr, nullptr, &rband);
         pt.end(PH_FINAL, s);
         return;
     }
@@ -1434,28 +1358,28 @@ void lsh_index::search_lsh(int Q, int kq, hipStream_t s, PhaseTimer& pt, bool en
         // queries flagged by the merge (item capacity) or the finalisation
         // (FP32 band) are redone below with the rounds.
         const uint32_t cap = worklist_cap(K);
-        wl_ibase.ensure((size_t)Q + 1);
-        wl_part.ensure((size_t)cap * K);
-        wl_flags.ensure((size_t)Q);
-        st.reset(Q, K, newcap_for(K), s, false, false, false);
+        w.wl_ibase.ensure((size_t)Q + 1);
+        w.wl_part.ensure((size_t)cap * K);
+        w.wl_flags.ensure((size_t)Q);
+        w.st.reset(Q, K, newcap_for(K), s, false, false, false);
         pt.begin(PH_SCAN, s);
-        launch_lsh_exact_worklist(a, K, wl_ibase.get(), cap, wl_part.get(), st.view(), wl_flags.get(), s);
+        launch_lsh_exact_worklist(a, K, w.wl_ibase.get(), cap, w.wl_part.get(), w.st.view(), w.wl_flags.get(), s);
         pt.end(PH_SCAN, s);
         ++scan_launches;
         pt.begin(PH_FINAL, s);
-        launch_exact_finalize(data.get(), q.get(), d, kind, st.view(), kq, eps, outp, flags.get(), s);
+        launch_exact_finalize(data.get(), w.q.get(), d, kind, w.st.view(), kq, eps, w.outp, w.flags.get(), s);
         pt.end(PH_FINAL, s);
-        if (enqueue_only) return;  // the caller reads flags and wl_flags later
-        bad = flagged_either(flags.get(), wl_flags.get(), Q, s);
+        if (enqueue_only) return;  // the caller reads w.flags and w.wl_flags later
+        bad = flagged_either(w.flags.get(), w.wl_flags.get(), Q, s);
         if (bad.empty()) return;
-        LSH_HIP(hipMemcpyAsync(&maxS, qmax.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        LSH_HIP(hipMemcpyAsync(&maxS, w.qmax.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         LSH_HIP(hipStreamSynchronize(s));
     } else {
-        lsh_topk(a, none, maxS, K, kNormal, false, st, scratch, 0, s, pt, scan_launches);
+        lsh_topk(a, none, maxS, K, kNormal, false, w.st, scratch, 0, s, pt, scan_launches);
         pt.begin(PH_FINAL, s);
-        launch_exact_finalize(data.get(), q.get(), d, kind, st.view(), kq, eps, outp, flags.get(), s);
+        launch_exact_finalize(data.get(), w.q.get(), d, kind, w.st.view(), kq, eps, w.outp, w.flags.get(), s);
         pt.end(PH_FINAL, s);
-        bad = flagged(flags.get(), Q, s);
+        bad = flagged(w.flags.get(), Q, s);
         if (bad.empty()) return;
     }
     LSH_REQUIRE(K < kMaxTopK, LSH_ERR_UNSUPPORTED,
@@ -1464,7 +1388,7 @@ void lsh_index::search_lsh(int Q, int kq, hipStream_t s, PhaseTimer& pt, bool en
     DevBuf<int32_t> idx((size_t)m);
     LSH_HIP(hipMemcpyAsync(idx.get(), bad.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, s));
     DevBuf<float> sub((size_t)m * d);
-    launch_gather_rows_f32(q.get(), d, idx.get(), m, sub.get(), s);
+    launch_gather_rows_f32(w.q.get(), d, idx.get(), m, sub.get(), s);
     LshScanArgs b = a;
     b.qmap = idx.get();
     b.Q = m;
@@ -1475,7 +1399,7 @@ void lsh_index::search_lsh(int Q, int kq, hipStream_t s, PhaseTimer& pt, bool en
     launch_exact_finalize(data.get(), sub.get(), d, kind, sst.view(), kq, eps, sout.get(), sflags.get(), s);
     LSH_REQUIRE(flagged(sflags.get(), m, s).empty(), LSH_ERR_UNSUPPORTED,
                 "more than 4096 candidates within the FP32 error band of the k-th distance");
-    launch_scatter_rows_pts(sout.get(), kq, idx.get(), m, outp, s);
+    launch_scatter_rows_pts(sout.get(), kq, idx.get(), m, w.outp, s);
     LSH_HIP(hipStreamSynchronize(s));
 }
 

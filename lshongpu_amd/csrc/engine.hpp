@@ -156,28 +156,54 @@ void cell_search_reserve(const CellIndex& c, int Q, Scratch& sc, int depth);
 
 }  // namespace lshgpu
 
-// One batch slot of the SimHash index's captured batches: a stream of its
-// own (default priority), the buffers a graph reads and writes, and the
-// graphs captured over them, one per (query-count class, rounded k).
-// Batches dealt to different slots run concurrently.
-struct sketch_slot {
+// The per-search buffers of the exact (K,L) rescan: the index's own, or a
+// batch slot's.
+struct lsh_work {
+    lshgpu::DevBuf<float>& q;
+    lshgpu::DevBuf<uint32_t>& qkeys;
+    lshgpu::DevBuf<uint32_t>& qlo;
+    lshgpu::DevBuf<uint32_t>& qpre;
+    lshgpu::DevBuf<uint32_t>& qmax;
+    lshgpu::DevBuf<uint32_t>& qlen;
+    lshgpu::DevBuf<uint32_t>& wl_ibase;
+    lshgpu::DevBuf<uint32_t>& wl_flags;
+    lshgpu::DevBuf<uint64_t>& wl_part;
+    lshgpu::DevBuf<uint32_t>& flags;
+    lshgpu::TopKState& st;
+    lsh_point_t* outp;
+};
+
+// One batch slot of the captured batch searches: a stream of its own
+// (default priority), the buffers its graphs read and write, and the graphs
+// captured over them, one per (query-count class, rounded k).  Batches dealt
+// to different slots run concurrently.  The SimHash index with pivot cells
+// uses the sketch buffers, the (K,L) rescan the lsh ones.
+struct batch_slot {
     hipStream_t stream = nullptr;
     lshgpu::DevBuf<float> q;
+    lshgpu::DevBuf<lsh_point_t> out;
+    lshgpu::TopKState st;
+    lshgpu::DevBuf<uint32_t> flags;  // sketch: the select's flags, then the cell search's three status words
+    // sketch
     lshgpu::DevBuf<uint32_t> qsk;
     bool qsk_zeroed = false;
-    lshgpu::TopKState st;
     lshgpu::Scratch scratch;
     lshgpu::DevBuf<int32_t> qorder;
     lshgpu::RerankBand rband;
-    lshgpu::DevBuf<uint32_t> gflags;  // the select's flags, then the cell search's three status words
-    lshgpu::DevBuf<lsh_point_t> out;
+    // (K,L) rescan
+    lshgpu::DevBuf<uint32_t> qkeys, qlo, qpre, qmax, qlen, wl_ibase, wl_flags;
+    lshgpu::DevBuf<uint64_t> wl_part;
     std::map<std::pair<int, int>, hipGraphExec_t> graphs;
     std::vector<uintptr_t> sig;  // the buffers' layout when the graphs were captured
-    sketch_slot() = default;
-    sketch_slot(const sketch_slot&) = delete;
-    sketch_slot& operator=(const sketch_slot&) = delete;
-    ~sketch_slot();
+    batch_slot() = default;
+    batch_slot(const batch_slot&) = delete;
+    batch_slot& operator=(const batch_slot&) = delete;
+    ~batch_slot();
     void drop_graphs();
+    size_t bytes() const;
+    lsh_work work() {
+        return {q, qkeys, qlo, qpre, qmax, qlen, wl_ibase, wl_flags, wl_part, flags, st, out.get()};
+    }
 };
 
 struct lsh_index {
@@ -271,21 +297,17 @@ struct lsh_index {
     int64_t mfma_launches = 0;   // matrix-core Hamming scan launches of the last search
     int64_t mfma_points = 0;     // data points those launches covered
 
-    // Batched search through captured graphs (search_batches): one graph per
-    // (query count class, rounded k), over buffers sized for kGraphMaxQ
-    // queries; `graph_sig` is the buffers' layout when they were captured.
+    // Batched search through captured graphs (search_batches): batch slots
+    // (created by the first batch call) whose graphs cover query-count
+    // classes up to kGraphMaxQ; whether a batch of the SimHash index found
+    // the cells' second pass too dense (later batch calls then run ordinary
+    // searches, whose fallback handles that).
     static constexpr int kGraphMaxQ = 4096;
-    std::map<std::pair<int, int>, hipGraphExec_t> graphs;
-    std::vector<uintptr_t> graph_sig;
+    static constexpr int kBatchSlots = 4;
+    std::vector<std::unique_ptr<batch_slot>> slots;
+    bool cells_dense = false;
     uint32_t* bflags = nullptr;  // pinned: per-query flags of a batch call
     size_t bflags_n = 0;
-    // SimHash index with pivot cells: the batch slots (created by the first
-    // batch call), and whether a batch found the cells' second pass too
-    // dense (later batch calls then run ordinary searches, whose fallback
-    // handles that).
-    static constexpr int kSketchSlots = 4;
-    std::vector<std::unique_ptr<sketch_slot>> slots;
-    bool cells_dense = false;
     // The last batch call: batches that replayed a graph, queries redone.
     int64_t batch_graphs = 0, batch_redone = 0;
 
@@ -309,14 +331,22 @@ struct lsh_index {
     void sketch_simhash(const float* x, int64_t n, uint32_t* out, int stride, hipStream_t s) const;
     void build_simhash_planes(hipStream_t s);
     bool lsh_graphable(int kq) const;
-    void search_lsh(int Q, int kq, hipStream_t s, lshgpu::PhaseTimer& pt, bool enqueue_only = false);
-    // The SimHash index's batched search through captured graphs: the cell
-    // search, its select and the rerank of Qc queries held in q, enqueued
-    // with no host wait (flags to gflags, results to out).
+    // The index's own buffers for search_lsh.
+    lsh_work own_work() {
+        return {q, qkeys, qlo, qpre, qmax, qlen, wl_ibase, wl_flags, wl_part, flags, st, outp};
+    }
+    void search_lsh(const lsh_work& w, int Q, int kq, hipStream_t s, lshgpu::PhaseTimer& pt,
+                    bool enqueue_only = false);
+    // The SimHash index's search of Qc queries held in a slot's q, enqueued
+    // with no host wait: sketch, cell search, select (flags to the slot's
+    // flags) and rerank (results to its out).
     bool sketch_graphable(int kq) const;
-    void enqueue_sketch_search(sketch_slot& sl, int Qc, int kq);
-    void search_batches_sketch(const float* queries, const int* sizes, int n_batches, int k, int64_t total,
-                               bool q_on_device, lsh_point_t* results, bool r_on_device, hipStream_t s);
+    void enqueue_sketch_search(batch_slot& sl, int Qc, int kq);
+    // The batches through the slots' graphs (sketch: the SimHash index path,
+    // else the (K,L) rescan).
+    void search_batches_graphs(bool sketch, const float* queries, const int* sizes, int n_batches, int k,
+                               int64_t total, bool q_on_device, lsh_point_t* results, bool r_on_device,
+                               hipStream_t s);
 };
 
 namespace lshgpu {

@@ -910,8 +910,9 @@ def test_lsh_rejects_keys_wider_than_the_table(lsh):
 def test_autotune_picks_fastest_config_meeting_recall(lsh):
     # Three (K,L) settings of the dense-rotation keys: the autotuner's recall
     # must equal the recall the chosen index gives, its choice must reach the
-    # target and be the fastest that does, and a second call is served from
-    # the cache with the same answer.
+    # target and be the fastest that does or, among those within 3% of the
+    # fastest, the most accurate; a second call is served from the cache with
+    # the same answer.
     rng = np.random.default_rng(77)
     N, d, k = 40000, 64, 32
     centres = rng.standard_normal((100, d)).astype(np.float32)
@@ -929,7 +930,9 @@ def test_autotune_picks_fastest_config_meeting_recall(lsh):
         rec = np.mean([len(set(a) & set(b)) / k for a, b in zip(got, truth)])
         assert abs(rec - r["recall"]) < 1e-12
     ok = [i for i, r in enumerate(reps) if r["recall"] >= 0.8]
-    assert best in ok and reps[best]["search_ms"] == min(reps[i]["search_ms"] for i in ok)
+    fastest = min(reps[i]["search_ms"] for i in ok)
+    near = [i for i in ok if reps[i]["search_ms"] <= fastest * 1.03]
+    assert best in near and reps[best]["recall"] == max(reps[i]["recall"] for i in near)
     again, reps2 = lsh.autotune(data, queries, k, cands, target_recall=0.8)
     assert again == best and reps2 == reps
 
