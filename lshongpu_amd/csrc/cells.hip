@@ -954,7 +954,7 @@ void cell_scan(const CellIndex& c, unsigned grid, const uint32_t* coff, const ui
 constexpr int64_t kPointsPerPivot = 768;
 int cell_pivots_for(int64_t N) {
     if (N < kCellMinPoints) return 0;
-    return (int)std::min<int64_t>(kMaxPivots * 7 / 8, std::max<int64_t>(64, N / kPointsPerPivot));
+    return (int)std::min<int64_t>(kMaxPivots / 2, std::max<int64_t>(64, N / kPointsPerPivot));
 }
 
 void CellIndex::build(const uint32_t* ds, int64_t N, int wpad, hipStream_t s) {
@@ -963,16 +963,20 @@ void CellIndex::build(const uint32_t* ds, int64_t N, int wpad, hipStream_t s) {
     if (np == 0 || !hamming_mfma_supported(wpad)) return;
     LSH_REQUIRE(N < ((int64_t)1 << 31), LSH_ERR_UNSUPPORTED, "pivot cells need N < 2^31");
     words = wpad;
-    // Pivots: every (N/np)-th row.  A group of points that drew no pivot of
-    // its own ends up far from the pivot it joins and widens that cell's
-    // range.  So a point is "far" beyond the smaller of twice the median and
-    // three times the 10th percentile of the distances to the pivots (and
-    // beyond bits/16); far points seed more pivots (a stride over them, up to
-    // a quarter of the pivots a round), and the points are assigned again,
-    // until at most one point in a thousand is far (or eight rounds, or
-    // kMaxPivots).  The median keeps well-seeded data from drawing pivots for
-    // the tails of its groups; the low percentile keeps the rule working when
-    // most points start without a near pivot (the median is then far too).
+    // Pivots: every (N/np)-th row (np at most half of kMaxPivots).  A group
+    // of points that drew no pivot of its own ends up far from the pivot it
+    // joins and widens that cell's range.  So a point is "far" beyond the
+    // smaller of twice the median and three times the 10th percentile of the
+    // distances to the pivots (and beyond bits/16); far points seed more
+    // pivots, and the points are assigned again, until at most one point in
+    // a thousand is far (or eight rounds, or kMaxPivots).  A round's seeds
+    // come from a stride over the far points (four candidates per seed it
+    // may add, up to half the pivots a round), thinned greedily: a candidate
+    // within the far distance of a seed already taken this round belongs to
+    // the same group and is skipped, so each seed covers a group of its own.
+    // The median keeps well-seeded data from drawing pivots for the tails of
+    // its groups; the low percentile keeps the rule working when most points
+    // start without a near pivot (the median is then far too).
     const int bits = 32 * wpad;
     std::vector<int32_t> prow((size_t)np);
     const int64_t step = N / np;
@@ -1006,8 +1010,34 @@ void CellIndex::build(const uint32_t* ds, int64_t N, int wpad, hipStream_t s) {
         for (int64_t i = 0; i < N; ++i)
             if ((int)(hk[i] & kDistMask) > far_at) add.push_back((int32_t)i);
         if ((int64_t)add.size() * 1000 <= N) break;
-        const size_t take = std::min<size_t>({add.size(), (size_t)(kMaxPivots - n), (size_t)std::max(1, n / 4)});
-        for (size_t i = 0; i < take; ++i) prow.push_back(add[i * add.size() / take]);
+        const size_t take = std::min<size_t>({add.size(), (size_t)(kMaxPivots - n), (size_t)std::max(1, n / 2)});
+        const size_t ncand = std::min(add.size(), 4 * take);
+        std::vector<int32_t> cand(ncand);
+        for (size_t i = 0; i < ncand; ++i) cand[i] = add[i * add.size() / ncand];
+        DevBuf<int32_t> cidx(ncand);
+        DevBuf<uint32_t> csk(ncand * wpad);
+        LSH_HIP(hipMemcpyAsync(cidx.get(), cand.data(), sizeof(int32_t) * ncand, hipMemcpyHostToDevice, s));
+        launch_gather_rows_u32(ds, wpad, cidx.get(), (int)ncand, csk.get(), s);
+        std::vector<uint64_t> hsk(ncand * wpad / 2);
+        LSH_HIP(hipMemcpyAsync(hsk.data(), csk.get(), sizeof(uint32_t) * ncand * wpad, hipMemcpyDeviceToHost, s));
+        LSH_HIP(hipStreamSynchronize(s));
+        const int w2 = wpad / 2;  // wpad is a multiple of 8
+        std::vector<size_t> kept;
+        for (size_t i = 0; i < ncand && kept.size() < take; ++i) {
+            const uint64_t* a = &hsk[i * w2];
+            bool fresh = true;
+            for (size_t j : kept) {
+                const uint64_t* b = &hsk[j * w2];
+                int h = 0;
+                for (int w = 0; w < w2 && h <= far_at; ++w) h += __builtin_popcountll(a[w] ^ b[w]);
+                if (h <= far_at) {
+                    fresh = false;
+                    break;
+                }
+            }
+            if (fresh) kept.push_back(i);
+        }
+        for (size_t i : kept) prow.push_back(cand[i]);
     }
     np = (int)prow.size();
     {
