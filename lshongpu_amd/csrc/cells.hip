@@ -1227,7 +1227,9 @@ const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K
                        stat);
     check_launch("place_from_cells_kernel");
     cell_scan(c, grid, coff, choff, cmap, Pp, list, D, qs, v, nullptr, stat, s);
-    if (debug_cells()) {
+    hipStreamCaptureStatus capturing = hipStreamCaptureStatusNone;
+    if (debug_cells()) LSH_HIP(hipStreamIsCapturing(s, &capturing));
+    if (debug_cells() && capturing == hipStreamCaptureStatusNone) {
         std::vector<uint64_t> ht((size_t)Q);
         uint32_t hs[3], pairs2 = 0, chunks2 = 0;
         LSH_HIP(hipMemcpyAsync(ht.data(), v.thr, sizeof(uint64_t) * Q, hipMemcpyDeviceToHost, s));
