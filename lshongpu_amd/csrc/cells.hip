@@ -625,7 +625,7 @@ __device__ inline int hist_bin(int d) { return min(d >> 3, kHistBins - 1); }
 // appends the pairs below each query's threshold to v.  Bounded to 128
 // registers, so that four workgroups (16 waves) share a CU.
 template <int KS, bool kHist>
-__global__ __launch_bounds__(256, 4) void cell_scan_kernel(const uint32_t* __restrict__ sk,
+__global__ __launch_bounds__(256, 5) void cell_scan_kernel(const uint32_t* __restrict__ sk,
                                                         const int32_t* __restrict__ ids,
                                                         const uint32_t* __restrict__ tinfo,
                                                         const uint32_t* __restrict__ ctile,
@@ -720,10 +720,6 @@ __global__ __launch_bounds__(256, 4) void cell_scan_kernel(const uint32_t* __res
                 }
             }
         };
-        bool dval = false;  // a tile's appends are pending
-        uint32_t ddist[8] = {};
-        uint32_t dpass = 0, dret = 0, dother = 0;
-        int32_t did = 0;
         if (mine) {
             uint4 nx[NV];
             int32_t nid = 0;
@@ -773,21 +769,17 @@ __global__ __launch_bounds__(256, 4) void cell_scan_kernel(const uint32_t* __res
                     const uint32_t other = (uint32_t)__shfl_xor((int)mine_n, 32);
                     uint32_t ret = 0;
                     if (lane < 32 && mine_n + other > 0) ret = atomicAdd(&v.cnt[q], mine_n + other);
-                    if (dval) emit(ddist, dpass, dret, dother, did);
-                    dval = true;
+                    uint32_t dist[8];
     #pragma unroll
                     for (int r = 0; r < 16; r += 2)
-                        ddist[r >> 1] = (uint32_t)((BITS - (int)acc[r]) >> 1) |
-                                        ((uint32_t)((BITS - (int)acc[r + 1]) >> 1) << 16);
-                    dpass = pass;
-                    dret = ret;
-                    dother = other;
-                    did = cid;
+                        dist[r >> 1] = (uint32_t)((BITS - (int)acc[r]) >> 1) |
+                                       ((uint32_t)((BITS - (int)acc[r + 1]) >> 1) << 16);
+                    // emit shuffles across the wave: every lane calls it, or none.
+                    if (__ballot(pass != 0)) emit(dist, pass, ret, other, cid);
                 }
                 if (!mine) break;
             }
         }
-        if (!kHist && dval) emit(ddist, dpass, dret, dother, did);
         __syncthreads();  // every wave is done with sb and lhist
         if (kHist) {
             for (int e = threadIdx.x; e < LH; e += 256) {
