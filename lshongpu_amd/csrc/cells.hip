@@ -101,13 +101,17 @@ __global__ __launch_bounds__(256) void pivot_assign_kernel(const uint32_t* __res
 }
 
 // D[q][p] (row stride Pp, a multiple of 32) = Hamming distance of query q
-// to pivot p.  Each wave takes 32 queries (blockIdx.x, wave) and a group of
-// kGroup pivot tiles (blockIdx.y); a 32x32 block of distances goes through
-// LDS so that every lane stores 32 contiguous bytes of one query's row.
+// to pivot p.  Each wave takes 32 queries (blockIdx.x, wave), their FP4
+// operands expanded once in registers, and the workgroup a group of kGroup
+// pivot tiles (blockIdx.y): each tile's FP4 image (the A operand, KS KB) is
+// copied into LDS once for the four waves with direct global-to-LDS loads,
+// the next tile while the current one is multiplied.  A 32x32 block of
+// distances goes through LDS so that every lane stores 32 contiguous bytes
+// of one query's row.
 constexpr int kGroup = 8;
 template <int KS>
 __global__ __launch_bounds__(256) void pivot_matrix_kernel(const uint32_t* __restrict__ qs, int Q,
-                                                           const uint32_t* __restrict__ pk, int P, int Pp,
+                                                           const v4i* __restrict__ pfrag, int P, int Pp,
                                                            uint16_t* __restrict__ D, uint32_t* __restrict__ zero,
                                                            int nzero, TopKView tv) {
     // The first workgroup also zeroes zero[0, nzero) (a search's counters),
@@ -125,11 +129,13 @@ __global__ __launch_bounds__(256) void pivot_matrix_kernel(const uint32_t* __res
     }
     constexpr int W = 2 * KS;
     constexpr int BITS = 32 * W;
-    constexpr int NV = KS / 4;  // uint4 loads per lane per pivot tile
     __shared__ uint32_t tb[4][32][33];
+    __shared__ v4i pa[2][KS * 64];  // a pivot tile's A operand, two buffers
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int q0 = (blockIdx.x * 4 + wave) * 32;
-    if (q0 >= Q) return;
+    // A wave past the queries still stages its share of the tiles and meets
+    // the barriers; it multiplies and stores nothing.
+    const bool live = q0 < Q;
     const int q = q0 + (lane & 31);
     v4i b[KS];
 #pragma unroll
@@ -138,23 +144,24 @@ __global__ __launch_bounds__(256) void pivot_matrix_kernel(const uint32_t* __res
     const int pt0 = blockIdx.y * kGroup;
     const int pt1 = min(ptiles, pt0 + kGroup);
     uint32_t(*t)[33] = tb[wave];
-    // Lane l's half of pivot row l%32 of tile pt: W/2 packed words.
-    auto load = [&](int pt, uint4* x) {
-        const uint4* src = reinterpret_cast<const uint4*>(pk + ((size_t)pt * 32 + (lane & 31)) * W + (lane >> 5) * KS);
-#pragma unroll
-        for (int i = 0; i < NV; ++i) x[i] = src[i];
+    // Wave w copies K-steps w, w + 4, ... of pivot tile pt (1 KB each).
+    auto stage = [&](int pt, int buf) {
+        for (int st = wave; st < KS; st += 4)
+            __builtin_amdgcn_global_load_lds((const void*)(pfrag + ((size_t)pt * KS + st) * 64 + lane),
+                                             (__attribute__((address_space(3))) void*)&pa[buf][st * 64], 16, 0, 0);
     };
-    uint4 nx[NV];
-    if (pt0 < pt1) load(pt0, nx);
+    if (pt0 < pt1) stage(pt0, 0);
     for (int pt = pt0; pt < pt1; ++pt) {
-        uint4 cur[NV];
-#pragma unroll
-        for (int i = 0; i < NV; ++i) cur[i] = nx[i];
-        if (pt + 1 < pt1) load(pt + 1, nx);
-        const uint32_t* cw = reinterpret_cast<const uint32_t*>(cur);
+        const int buf = (pt - pt0) & 1;
+        // This wave's copies of tile pt have landed; the barrier makes every
+        // wave's visible and frees the other buffer (read for tile pt - 1).
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        if (pt + 1 < pt1) stage(pt + 1, buf ^ 1);
+        if (!live) continue;
         v16f acc = {};
 #pragma unroll
-        for (int s = 0; s < KS; ++s) acc = fp4::mma(expand_word(cw[s]), b[s], acc);
+        for (int s = 0; s < KS; ++s) acc = fp4::mma(pa[buf][s * 64 + lane], b[s], acc);
 #pragma unroll
         for (int r = 0; r < 16; ++r) t[lane & 31][fp4::acc_row(r, lane)] = (uint32_t)((BITS - (int)acc[r]) >> 1);
         __builtin_amdgcn_wave_barrier();
@@ -881,7 +888,7 @@ __global__ __launch_bounds__(256) void place_from_cells_kernel(const int32_t* __
 }
 
 template <int KS>
-void launch_pivots_ks(const uint32_t* rows, int64_t n, const uint32_t* pfrag, const uint32_t* pk, int P, int Pp,
+void launch_pivots_ks(const uint32_t* rows, int64_t n, const uint32_t* pfrag, int P, int Pp,
                       uint32_t* best, uint16_t* D, uint32_t* zero, int nzero, const TopKView* tv, hipStream_t s) {
     if (best) {
         const v4i* f = reinterpret_cast<const v4i*>(pfrag);
@@ -891,23 +898,24 @@ void launch_pivots_ks(const uint32_t* rows, int64_t n, const uint32_t* pfrag, co
     } else {
         const unsigned gy = (unsigned)(((P + 31) / 32 + kGroup - 1) / kGroup);
         hipLaunchKernelGGL((pivot_matrix_kernel<KS>), dim3(blocks_for((size_t)((n + 31) / 32), 4), gy), dim3(256), 0,
-                           s, rows, (int)n, pk, P, Pp, D, zero, nzero, tv ? *tv : TopKView{});
+                           s, rows, (int)n, reinterpret_cast<const v4i*>(pfrag), P, Pp, D, zero, nzero,
+                           tv ? *tv : TopKView{});
         check_launch("pivot_matrix_kernel");
     }
 }
 
-// best != nullptr: nearest pivot of each row (from the FP4 image pfrag);
-// otherwise the distance matrix D (from the packed pivot sketches pk), whose
+// best != nullptr: nearest pivot of each row; otherwise the distance matrix
+// D, whose
 // launch also zeroes zero[0, nzero) and clears the fresh view *tv (when
 // given).
-void pivots(int words, const uint32_t* rows, int64_t n, const uint32_t* pfrag, const uint32_t* pk, int P, int Pp,
+void pivots(int words, const uint32_t* rows, int64_t n, const uint32_t* pfrag, int P, int Pp,
             uint32_t* best, uint16_t* D, hipStream_t s, uint32_t* zero = nullptr, int nzero = 0,
             const TopKView* tv = nullptr) {
     switch (words) {
-        case 8: launch_pivots_ks<4>(rows, n, pfrag, pk, P, Pp, best, D, zero, nzero, tv, s); break;
-        case 16: launch_pivots_ks<8>(rows, n, pfrag, pk, P, Pp, best, D, zero, nzero, tv, s); break;
-        case 24: launch_pivots_ks<12>(rows, n, pfrag, pk, P, Pp, best, D, zero, nzero, tv, s); break;
-        case 32: launch_pivots_ks<16>(rows, n, pfrag, pk, P, Pp, best, D, zero, nzero, tv, s); break;
+        case 8: launch_pivots_ks<4>(rows, n, pfrag, P, Pp, best, D, zero, nzero, tv, s); break;
+        case 16: launch_pivots_ks<8>(rows, n, pfrag, P, Pp, best, D, zero, nzero, tv, s); break;
+        case 24: launch_pivots_ks<12>(rows, n, pfrag, P, Pp, best, D, zero, nzero, tv, s); break;
+        case 32: launch_pivots_ks<16>(rows, n, pfrag, P, Pp, best, D, zero, nzero, tv, s); break;
         default: fail(LSH_ERR_INTERNAL, "pivot cells need 8, 16, 24 or 32 sketch words");
     }
 }
@@ -983,7 +991,7 @@ void CellIndex::build(const uint32_t* ds, int64_t N, int wpad, hipStream_t s) {
         pfp4.alloc(hamming_fp4_words(n, wpad));
         launch_expand_fp4(psk.get(), n, wpad, pfp4.get(), s);
         // Nearest pivot of every point.
-        pivots(wpad, ds, N, pfp4.get(), nullptr, n, 0, ckey.get(), nullptr, s);
+        pivots(wpad, ds, N, pfp4.get(), n, 0, ckey.get(), nullptr, s);
         if (round == 8 || n >= kMaxPivots) break;
         std::vector<uint32_t> hk((size_t)N);
         LSH_HIP(hipMemcpyAsync(hk.data(), ckey.get(), sizeof(uint32_t) * N, hipMemcpyDeviceToHost, s));
@@ -1032,20 +1040,6 @@ void CellIndex::build(const uint32_t* ds, int64_t N, int wpad, hipStream_t s) {
         for (size_t i : kept) prow.push_back(cand[i]);
     }
     np = (int)prow.size();
-    {
-        // The final pivots' sketches, packed as the cells' tiles are (even
-        // words, then odd), zero rows up to a whole tile.
-        const int Ppad = (np + 31) / 32 * 32;
-        std::vector<int32_t> pr(prow);
-        pr.resize((size_t)Ppad, -1);
-        DevBuf<int32_t> pidx((size_t)Ppad);
-        LSH_HIP(hipMemcpyAsync(pidx.get(), pr.data(), sizeof(int32_t) * Ppad, hipMemcpyHostToDevice, s));
-        pk.alloc((size_t)Ppad * wpad);
-        hipLaunchKernelGGL(pack_cells_kernel, dim3(blocks_for((size_t)Ppad * wpad, 256)), dim3(256), 0, s, ds, wpad,
-                           pidx.get(), (int64_t)Ppad, pk.get());
-        check_launch("pack_cells_kernel");
-        LSH_HIP(hipStreamSynchronize(s));
-    }
 
     // Counting sort by (pivot, distance).
     const uint32_t T = (uint32_t)np << kKeyShift;
@@ -1107,7 +1101,7 @@ void cell_rows(const CellIndex& c, const float* data, int64_t N, int d, DevBuf<f
 }
 
 size_t CellIndex::bytes() const {
-    return pfp4.bytes() + pk.bytes() + sk.bytes() + ids.bytes() + tinfo.bytes() + ctile.bytes() + csize.bytes() +
+    return pfp4.bytes() + sk.bytes() + ids.bytes() + tinfo.bytes() + ctile.bytes() + csize.bytes() +
            cmin.bytes() + cmax.bytes() + crec.bytes();
 }
 
@@ -1179,7 +1173,7 @@ const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K
     const uint32_t dense_limit = (uint32_t)std::min<double>(4e9, 0.5 * (double)qgroups * (double)c.ntiles);
 
     LSH_REQUIRE(v.fresh && v.Q == Q, LSH_ERR_INTERNAL, "cell_search takes a fresh top-K view of its queries");
-    pivots(c.words, qs, Q, c.pfp4.get(), c.pk.get(), P, Pp, nullptr, D, s, meta, nzero, &v);
+    pivots(c.words, qs, Q, c.pfp4.get(), P, Pp, nullptr, D, s, meta, nzero, &v);
     // Threshold bound from the tile ranges, the first-pass radii and their
     // cell counts, and the nearest cell of each query.
     auto threshold = [&](auto kernel, int nt) {

@@ -120,7 +120,6 @@ struct CellIndex {
     int64_t ntiles = 0;
     int64_t nparts = 0;            // sum over cells of their tile ranges (chunks of one query group)
     DevBuf<uint32_t> pfp4;         // FP4 image of the pivot sketches
-    DevBuf<uint32_t> pk;           // [P rounded up to 32][W] pivot sketches, packed as sk
     DevBuf<uint32_t> sk;           // [ntiles*32][W] packed sketches in cell order, even words then odd
     DevBuf<int32_t> ids;           // [ntiles*32] data row of each position, -1 for padding
     DevBuf<uint32_t> tinfo;        // per tile: min | max << 11 | rows << 22 (distances to the pivot)
