@@ -1181,8 +1181,12 @@ const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K
                            nearest, cnt1, qcells1, n1, qcells2, n2, hist);
     };
     LSH_REQUIRE(P >= 1 && P <= 256 * 32, LSH_ERR_INTERNAL, "cell_search: pivot count outside the threshold kernel's range");
-    if (P <= 256 * 4) threshold(cell_threshold_kernel<4, 256>, 256);
-    else if (P <= 256 * 8) threshold(cell_threshold_kernel<8, 256>, 256);
+    // Up to 2,048 pivots two waves own a query (one wave holding 32 cells a
+    // lane needs 151 registers, 3 waves a SIMD, and was slower); beyond, a
+    // 256-thread workgroup.
+    if (P <= 128 * 8) threshold(cell_threshold_kernel<8, 128>, 128);
+    else if (P <= 128 * 16) threshold(cell_threshold_kernel<16, 128>, 128);
+    else if (P <= 256 * 10) threshold(cell_threshold_kernel<10, 256>, 256);
     else if (P <= 256 * 10) threshold(cell_threshold_kernel<10, 256>, 256);
     else if (P <= 256 * 12) threshold(cell_threshold_kernel<12, 256>, 256);
     else if (P <= 256 * 16) threshold(cell_threshold_kernel<16, 256>, 256);
