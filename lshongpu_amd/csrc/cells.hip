@@ -1040,6 +1040,51 @@ void CellIndex::build(const uint32_t* ds, int64_t N, int wpad, hipStream_t s) {
         for (size_t i : kept) prow.push_back(cand[i]);
     }
     np = (int)prow.size();
+    {
+        // Pivots renumbered along a greedy nearest-neighbour chain (each next
+        // pivot the nearest not yet taken), then the points assigned again:
+        // cells with consecutive numbers lie near each other, so queries that
+        // run side by side in nearest-cell order (and the cells' rows, laid
+        // out in cell order) share candidate cells in L2.  Any assignment to
+        // the nearest pivot keeps the search exact.
+        DevBuf<int32_t> didx((size_t)np);
+        DevBuf<uint32_t> psk((size_t)np * wpad);
+        LSH_HIP(hipMemcpyAsync(didx.get(), prow.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, s));
+        launch_gather_rows_u32(ds, wpad, didx.get(), np, psk.get(), s);
+        std::vector<uint64_t> hp((size_t)np * wpad / 2);
+        LSH_HIP(hipMemcpyAsync(hp.data(), psk.get(), sizeof(uint32_t) * (size_t)np * wpad, hipMemcpyDeviceToHost, s));
+        LSH_HIP(hipStreamSynchronize(s));
+        const int w2 = wpad / 2;
+        std::vector<int32_t> chain;
+        chain.reserve((size_t)np);
+        std::vector<char> taken((size_t)np, 0);
+        int cur = 0;
+        for (int step = 0; step < np; ++step) {
+            chain.push_back(prow[cur]);
+            taken[cur] = 1;
+            int best = -1, bd = 1 << 30;
+            const uint64_t* a = &hp[(size_t)cur * w2];
+            for (int j = 0; j < np; ++j) {
+                if (taken[j]) continue;
+                const uint64_t* b = &hp[(size_t)j * w2];
+                int h = 0;
+                for (int w = 0; w < w2 && h < bd; ++w) h += __builtin_popcountll(a[w] ^ b[w]);
+                if (h < bd) {
+                    bd = h;
+                    best = j;
+                }
+            }
+            if (best < 0) break;
+            cur = best;
+        }
+        prow.swap(chain);
+        LSH_HIP(hipMemcpyAsync(didx.get(), prow.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, s));
+        launch_gather_rows_u32(ds, wpad, didx.get(), np, psk.get(), s);
+        pfp4.alloc(hamming_fp4_words(np, wpad));
+        launch_expand_fp4(psk.get(), np, wpad, pfp4.get(), s);
+        pivots(wpad, ds, N, pfp4.get(), np, 0, ckey.get(), nullptr, s);
+        LSH_HIP(hipStreamSynchronize(s));
+    }
 
     // Counting sort by (pivot, distance).
     const uint32_t T = (uint32_t)np << kKeyShift;
