@@ -86,9 +86,10 @@ def roofline(args, bytes_launch: float, avg_launch_ms: float) -> dict:
         "algorithmic_bytes_per_launch": bytes_launch,
         "gathered_gbs": gathered_gbs,
         "gathered_note": "every candidate row of every query in the filter's format, over the launch time; above "
-                         "HBM peak because rows shared by queries of one cell are served from L2/MALL; the limit "
-                         "is the rate of the random row gather through L2 and the MALL, not HBM bandwidth (the "
-                         "launch time is the same with 6, 7 or 8 resident workgroups per CU)",
+                         "HBM peak because rows shared by queries of one cell and its neighbours are served from "
+                         "L2; the limit is the rate of the random row gather through L2, not HBM bandwidth (the "
+                         "launch time is the same with 6, 7 or 8 resident workgroups per CU, and 36% fewer bytes "
+                         "past L2 took 4% off it)",
         "avg_launch_ms": avg_launch_ms,
         "launches_per_step": 1,
     }
