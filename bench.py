@@ -345,6 +345,27 @@ def main() -> int:
             "results_identical": same,
         }
 
+    # ---- the same search with host buffers: the queries copied in and the
+    # results copied out over PCIe inside every step (reported beside the
+    # device-resident rate, never as it).
+    nh = max(1, min(args.steps, 5))
+    host_buffers = {"note": "host queries in and host results out, the PCIe copies and the host wait inside the "
+                            "step: pageable numpy arrays (staged by the driver) and pinned host tensors (DMA)"}
+    hq = np.ascontiguousarray(queries_h)
+    pq = torch.from_numpy(hq).pin_memory()
+    pres = torch.empty((Q, kq * 2), dtype=torch.int32, pin_memory=True)
+    for name, fn in (("pageable", lambda: ix.search(hq, k)), ("pinned", lambda: ix.search(pq, k, out=pres))):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(nh):
+            hres = fn()
+        host_s = time.perf_counter() - t0
+        if name == "pinned":
+            hres = pres.numpy().view(lsh.POINT_DTYPE).reshape(Q, kq)
+        host_buffers[name] = {"queries_per_s": Q * nh / host_s, "ms_per_step": host_s * 1e3 / nh,
+                              "results_equal_device": bool(np.array_equal(hres, res))}
+
     # ---- the same queries in uneven batches (sizes log-uniform in 1..4096):
     # each batch replays the captured graph of its size class with no host
     # wait between batches, against one ordinary search call per batch.
@@ -440,6 +461,7 @@ def main() -> int:
         "threshold_pass": threshold_pass,
         "simhash_projection": projection,
         "uneven_batches": batches,
+        "host_buffers": host_buffers,
         "cpu_baseline": cpu,
         "data_gen_s": gen_s,
     }
