@@ -209,7 +209,7 @@ def main() -> int:
     ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2, 3, 4],
                     help="1: the benchmark workload (default); 0, 2, 3, 4: the other configurations, measured for "
                          "information (one JSON line, no roofline or CPU baseline)")
-    ap.add_argument("--serve-rates", default="50,200,800,2000,4000,8000,16000",
+    ap.add_argument("--serve-rates", default="50,200,800,2000,4000,8000,16000,32000",
                     help="configuration 4: offered request rates (requests/s) to sweep")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="configuration 4: seconds per rate")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",

@@ -374,7 +374,7 @@ int lsh_index_search(lsh_index_t* index, const float* queries, int N_query,
  * with no host wait between batches, on the exact (K,L) rescan of
  * keysImplementation 3, 4, 5 or 8 (without runWithSketchedData; rounded k +
  * rescan margin <= 512) and on the SimHash index (implementation 3) with
- * pivot cells (rounded k <= 64, candidates <= 2048); queries a graph leaves
+ * pivot cells (rounded k <= 128, candidates <= 2048); queries a graph leaves
  * incomplete are redone by ordinary searches before the call returns.  Other
  * index kinds run the batches as ordinary searches.  The call returns when
  * every result is complete (stream: as lsh_index_search). */
@@ -382,9 +382,9 @@ int lsh_index_search_batches(lsh_index_t* index, const float* queries, const int
                              int k, int queries_on_device, lsh_point_t* results, int results_on_device,
                              void* stream);
 
-/* Diagnostics of the last lsh_index_search_batches call on the index: the
- * batches that replayed a captured graph, and the queries redone afterwards
- * by ordinary searches.  No counterpart in the reference (it has no batched
+/* Diagnostics of the last lsh_index_search_batches call on the index (and of
+ * the request loops' batches since): the batches that replayed a captured
+ * graph, and the queries redone afterwards by ordinary searches.  No counterpart in the reference (it has no batched
  * search); for callers that check which path their batches took. */
 int lsh_index_batch_stats(lsh_index_t* index, int64_t* graph_batches, int64_t* redone_queries);
 
@@ -448,12 +448,17 @@ int lsh_autotune(const lsh_index_config_t* candidates, int n_candidates, const f
  * requests (host queries and k) are submitted from any thread and returned
  * through lsh_server_wait.  One worker thread batches every queued request
  * that fits (continuous batching): at most max_batch_queries queries and, for
- * the two batches in flight, memory_budget_bytes of device memory (0 = half
+ * two batches of its size, memory_budget_bytes of device memory (0 = half
  * of the free device memory; one query is budgeted at 2*(4d + 8*128) + 8*(3C +
  * 768) + 4096 bytes, C the index's candidates).  A batch runs as one search
  * at the largest rounded k of its requests; each request receives the first
  * lsh_calculate_k(k) results of each of its rows, which equal those of
- * searching its queries alone.  While one batch is on the device the next is
+ * searching its queries alone.  Up to four batches are in flight.  On the
+ * SimHash index with pivot cells (rounded k <= 128, at most 4096 queries a
+ * batch) each is enqueued as a replay of the index's captured graphs, with
+ * no host wait, so several run on the device together; queries a replay
+ * leaves incomplete are redone by an ordinary search before their request
+ * is answered.  Otherwise one search runs at a time while the next batch is
  * packed and its queries copied on a second stream.  The index must stay
  * alive until lsh_server_destroy, which serves every request already
  * submitted before it returns; other calls on the index (from other threads)

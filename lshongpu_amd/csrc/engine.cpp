@@ -1049,7 +1049,7 @@ void lsh_index::search_batches(const float* queries, const int* sizes, int n_bat
 // whose results replace the graph's.
 bool lsh_index::sketch_graphable(int kq) const {
     const int64_t ceff = std::max<int64_t>(1, std::min<int64_t>(candidates, N));
-    return framework == 0 && impl == 3 && cells.built() && !cells_dense && sketch_steps.empty() && kq <= 64 &&
+    return framework == 0 && impl == 3 && cells.built() && !cells_dense && sketch_steps.empty() && kq <= kGraphMaxK &&
            ceff <= kMaxIntTopK && hamming_mfma_supported(wpad) && !force_valu_scan() && !force_full_pass();
 }
 
@@ -1074,34 +1074,28 @@ void lsh_index::enqueue_sketch_search(batch_slot& sl, int Qc, int kq) {
                               sl.rband);
 }
 
-void lsh_index::search_batches_graphs(bool sketch, const float* queries, const int* sizes, int n_batches, int k,
-                                      int64_t total, bool q_on_device, lsh_point_t* results, bool r_on_device,
-                                      hipStream_t user_stream) {
-    const int kq = calculate_k(k);
-    // The call's work is ordered after what was queued on the caller's
-    // stream (or the index's own one; the legacy null stream cannot be
-    // captured).
-    hipStream_t s = user_stream ? user_stream : stream;
-    if (done_rec) LSH_HIP(hipStreamWaitEvent(s, done_ev, 0));
+void lsh_index::prepare_slots(bool sketch, int kq) {
     const int QM = kGraphMaxQ;
     const int L = tables;
     const int Ceff = (int)std::max<int64_t>(1, std::min<int64_t>(candidates, N));
     const int Cp = pow2_ceil(std::max(Ceff, 2));
     const int K = sketch ? Ceff : std::min(kMaxTopK, pow2_ceil(kq + margin));
+    // Sized already for this layout: nothing to do (the common case, once a
+    // batch per call of a request loop).  Otherwise batches still in flight
+    // on the slots finish before any buffer or graph is replaced.
+    const std::vector<int64_t> key = {sketch, K, Cp, d, wpad, L, QM, cells.built() ? (int64_t)cells.P : -1,
+                                      cells.built() ? (int64_t)cells.words : -1};
+    if (key == slots_key && (int)slots.size() == kBatchSlots) return;
+    for (auto& sl : slots) LSH_HIP(hipStreamSynchronize(sl->stream));
     while ((int)slots.size() < kBatchSlots) {
         auto sl = std::make_unique<batch_slot>();
         LSH_HIP(hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking));
         slots.push_back(std::move(sl));
     }
-    if (!done_ev) LSH_HIP(hipEventCreateWithFlags(&done_ev, hipEventDisableTiming));
-    LSH_HIP(hipEventRecord(done_ev, s));
-    for (auto& sl : slots) LSH_HIP(hipStreamWaitEvent(sl->stream, done_ev, 0));
-    // Every buffer a slot's graphs touch, sized for the largest class
-    // (nothing is allocated while capturing), then the layout its graphs were
-    // captured over: graphs of another layout are dropped.
-    auto prepare = [&](batch_slot& sl) {
+    for (auto& slp : slots) {
+        batch_slot& sl = *slp;
         sl.q.ensure((size_t)QM * d);
-        sl.out.ensure((size_t)QM * 64);
+        sl.out.ensure((size_t)QM * kGraphMaxK);
         std::vector<uintptr_t> sig = {(uintptr_t)sketch, (uintptr_t)K, (uintptr_t)sl.q.get(), (uintptr_t)sl.out.get()};
         if (sketch) {
             const int qrows = sketch_query_rows(QM);
@@ -1145,8 +1139,85 @@ void lsh_index::search_batches_graphs(bool sketch, const float* queries, const i
             sl.drop_graphs();
             sl.sig = sig;
         }
-    };
-    for (auto& sl : slots) prepare(*sl);
+    }
+    slots_key = key;
+}
+
+void lsh_index::enqueue_slot_batch(batch_slot& sl, bool sketch, const float* q, hipMemcpyKind qk, int b, int kq,
+                                   lsh_point_t* res, hipMemcpyKind rk, uint32_t* flags_out, uint32_t* extra_out) {
+    const hipStream_t ss = sl.stream;
+    const int Qc = graph_class(b);
+    LSH_HIP(hipMemcpyAsync(sl.q.get(), q, sizeof(float) * (size_t)b * d, qk, ss));
+    // The class's rows past the batch: for the SimHash index copies of its
+    // last query (a search of ordinary cost), for the (K,L) rescan zero rows
+    // (they normalise to NaN, whose keys match no real point, so they cost
+    // nearly nothing); their results are not copied out.
+    if (sketch) launch_repeat_row_f32(sl.q.get(), d, b - 1, Qc - b, ss);
+    else if (Qc > b) LSH_HIP(hipMemsetAsync(sl.q.get() + (size_t)b * d, 0, sizeof(float) * (size_t)(Qc - b) * d, ss));
+    auto it = sl.graphs.find({Qc, kq});
+    if (it == sl.graphs.end()) {
+        hipGraph_t graph = nullptr;
+        LSH_HIP(hipStreamBeginCapture(ss, hipStreamCaptureModeThreadLocal));
+        try {
+            if (sketch) {
+                enqueue_sketch_search(sl, Qc, kq);
+            } else {
+                PhaseTimer none(false);
+                search_lsh(sl.work(), Qc, kq, ss, none, true);
+            }
+        } catch (...) {
+            (void)hipStreamEndCapture(ss, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        LSH_HIP(hipStreamEndCapture(ss, &graph));
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        LSH_HIP(e);
+        it = sl.graphs.emplace(std::make_pair(Qc, kq), exec).first;
+    }
+    LSH_HIP(hipGraphLaunch(it->second, ss));
+    ++batch_graphs;
+    LSH_HIP(hipMemcpyAsync(res, sl.out.get(), sizeof(lsh_point_t) * (size_t)b * kq, rk, ss));
+    LSH_HIP(hipMemcpyAsync(flags_out, sl.flags.get(), sizeof(uint32_t) * (size_t)b, hipMemcpyDeviceToHost, ss));
+    if (sketch)
+        LSH_HIP(hipMemcpyAsync(extra_out, sl.flags.get() + Qc, sizeof(uint32_t) * 3, hipMemcpyDeviceToHost, ss));
+    else
+        LSH_HIP(hipMemcpyAsync(extra_out, sl.wl_flags.get(), sizeof(uint32_t) * (size_t)b, hipMemcpyDeviceToHost, ss));
+}
+
+bool lsh_index::enqueue_batch(const float* q, bool q_on_device, int n, int k, lsh_point_t* res, bool r_on_device,
+                              uint32_t* flags_out, hipEvent_t ready, hipEvent_t done) {
+    DeviceGuard g(device);
+    if (n < 1 || n > kGraphMaxQ || k < 1) return false;
+    const int kq = calculate_k(k);
+    if (!sketch_graphable(kq)) return false;
+    prepare_slots(true, kq);
+    batch_slot& sl = *slots[next_slot];
+    next_slot = (next_slot + 1) % (int)slots.size();
+    if (ready) LSH_HIP(hipStreamWaitEvent(sl.stream, ready, 0));
+    if (done_rec) LSH_HIP(hipStreamWaitEvent(sl.stream, done_ev, 0));
+    enqueue_slot_batch(sl, true, q, q_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, n, kq, res,
+                       r_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, flags_out, flags_out + n);
+    if (done) LSH_HIP(hipEventRecord(done, sl.stream));
+    return true;
+}
+
+void lsh_index::search_batches_graphs(bool sketch, const float* queries, const int* sizes, int n_batches, int k,
+                                      int64_t total, bool q_on_device, lsh_point_t* results, bool r_on_device,
+                                      hipStream_t user_stream) {
+    const int kq = calculate_k(k);
+    const int QM = kGraphMaxQ;
+    // The call's work is ordered after what was queued on the caller's
+    // stream (or the index's own one; the legacy null stream cannot be
+    // captured).
+    hipStream_t s = user_stream ? user_stream : stream;
+    if (done_rec) LSH_HIP(hipStreamWaitEvent(s, done_ev, 0));
+    prepare_slots(sketch, kq);
+    if (!done_ev) LSH_HIP(hipEventCreateWithFlags(&done_ev, hipEventDisableTiming));
+    LSH_HIP(hipEventRecord(done_ev, s));
+    for (auto& sl : slots) LSH_HIP(hipStreamWaitEvent(sl->stream, done_ev, 0));
     auto sync_slots = [&] {
         for (auto& sl : slots) LSH_HIP(hipStreamSynchronize(sl->stream));
     };
@@ -1170,7 +1241,6 @@ void lsh_index::search_batches_graphs(bool sketch, const float* queries, const i
     const hipMemcpyKind qk = q_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     const hipMemcpyKind rk = r_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
     int64_t off = 0;
-    int next = 0;  // the slot of the next batch
     for (int i = 0; i < n_batches; ++i) {
         const int b = sizes[i];
         uint32_t* st3 = status + 3 * (size_t)i;
@@ -1185,49 +1255,10 @@ void lsh_index::search_batches_graphs(bool sketch, const float* queries, const i
             off += b;
             continue;
         }
-        batch_slot& sl = *slots[next];
-        next = (next + 1) % (int)slots.size();
-        const hipStream_t ss = sl.stream;
-        const int Qc = graph_class(b);
-        LSH_HIP(hipMemcpyAsync(sl.q.get(), queries + off * d, sizeof(float) * (size_t)b * d, qk, ss));
-        // The class's rows past the batch: for the SimHash index copies of its
-        // last query (a search of ordinary cost), for the (K,L) rescan zero
-        // rows (they normalise to NaN, whose keys match no real point, so
-        // they cost nearly nothing); their results are not copied out.
-        if (sketch) launch_repeat_row_f32(sl.q.get(), d, b - 1, Qc - b, ss);
-        else if (Qc > b) LSH_HIP(hipMemsetAsync(sl.q.get() + (size_t)b * d, 0, sizeof(float) * (size_t)(Qc - b) * d, ss));
-        auto it = sl.graphs.find({Qc, kq});
-        if (it == sl.graphs.end()) {
-            hipGraph_t graph = nullptr;
-            LSH_HIP(hipStreamBeginCapture(ss, hipStreamCaptureModeThreadLocal));
-            try {
-                if (sketch) {
-                    enqueue_sketch_search(sl, Qc, kq);
-                } else {
-                    PhaseTimer none(false);
-                    search_lsh(sl.work(), Qc, kq, ss, none, true);
-                }
-            } catch (...) {
-                (void)hipStreamEndCapture(ss, &graph);
-                if (graph) (void)hipGraphDestroy(graph);
-                throw;
-            }
-            LSH_HIP(hipStreamEndCapture(ss, &graph));
-            hipGraphExec_t exec = nullptr;
-            const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            LSH_HIP(e);
-            it = sl.graphs.emplace(std::make_pair(Qc, kq), exec).first;
-        }
-        LSH_HIP(hipGraphLaunch(it->second, ss));
-        ++batch_graphs;
-        LSH_HIP(hipMemcpyAsync(results + off * kq, sl.out.get(), sizeof(lsh_point_t) * (size_t)b * kq, rk, ss));
-        LSH_HIP(hipMemcpyAsync(bflags + off, sl.flags.get(), sizeof(uint32_t) * (size_t)b, hipMemcpyDeviceToHost, ss));
-        if (sketch)
-            LSH_HIP(hipMemcpyAsync(st3, sl.flags.get() + Qc, sizeof(uint32_t) * 3, hipMemcpyDeviceToHost, ss));
-        else
-            LSH_HIP(hipMemcpyAsync(wflags + off, sl.wl_flags.get(), sizeof(uint32_t) * (size_t)b,
-                                   hipMemcpyDeviceToHost, ss));
+        batch_slot& sl = *slots[next_slot];
+        next_slot = (next_slot + 1) % (int)slots.size();
+        enqueue_slot_batch(sl, sketch, queries + off * d, qk, b, kq, results + off * kq, rk, bflags + off,
+                           sketch ? st3 : wflags + off);
         off += b;
     }
     sync_slots();

@@ -302,8 +302,11 @@ struct lsh_index {
     // the cells' second pass too dense (later batch calls then run ordinary
     // searches, whose fallback handles that).
     static constexpr int kGraphMaxQ = 4096;
+    static constexpr int kGraphMaxK = 128;  // rounded k of the SimHash index's graphs
     static constexpr int kBatchSlots = 4;
     std::vector<std::unique_ptr<batch_slot>> slots;
+    int next_slot = 0;               // the slot of the next batch
+    std::vector<int64_t> slots_key;  // the layout the slots are sized for
     bool cells_dense = false;
     uint32_t* bflags = nullptr;  // pinned: per-query flags of a batch call
     size_t bflags_n = 0;
@@ -321,6 +324,18 @@ struct lsh_index {
     void search_batches(const float* queries, const int* sizes, int n_batches, int k, bool q_on_device,
                         lsh_point_t* results, bool r_on_device, hipStream_t s);
     void drop_graphs();
+    // One batch of n <= kGraphMaxQ queries of the SimHash index with pivot
+    // cells, enqueued on the next batch slot with no host wait (for a
+    // request loop; host buffers should be pinned): it starts after `ready`
+    // (may be null), its results go to res (n x calculateK(k)) and its flags
+    // to flags_out (pinned host: n incomplete-query flags, then the cell
+    // search's three status words, the first nonzero when the second pass was
+    // too dense); `done` (may be null) is recorded once all are complete.  A
+    // flagged query, or every query of a too-dense batch, needs an ordinary
+    // search.  Returns false, enqueuing nothing, when the index has no
+    // captured form for the batch.
+    bool enqueue_batch(const float* q, bool q_on_device, int n, int k, lsh_point_t* res, bool r_on_device,
+                       uint32_t* flags_out, hipEvent_t ready, hipEvent_t done);
 
    private:
     void build_fp4(hipStream_t s);
@@ -341,6 +356,17 @@ struct lsh_index {
     // flags) and rerank (results to its out).
     bool sketch_graphable(int kq) const;
     void enqueue_sketch_search(batch_slot& sl, int Qc, int kq);
+    // Creates the batch slots on first use and sizes every buffer their
+    // graphs touch for the largest class (nothing is allocated while
+    // capturing); graphs captured over another layout are dropped.
+    void prepare_slots(bool sketch, int kq);
+    // One batch of b <= kGraphMaxQ queries (q, copied with qk) through slot
+    // sl's graph of its class (captured on first use): its results to res
+    // (rk), its flags to flags_out and to extra_out (sketch: the cell
+    // search's three status words; (K,L): the work list's capacity flags),
+    // both pinned host memory; all on the slot's stream, with no host wait.
+    void enqueue_slot_batch(batch_slot& sl, bool sketch, const float* q, hipMemcpyKind qk, int b, int kq,
+                            lsh_point_t* res, hipMemcpyKind rk, uint32_t* flags_out, uint32_t* extra_out);
     // The batches through the slots' graphs (sketch: the SimHash index path,
     // else the (K,L) rescan).
     void search_batches_graphs(bool sketch, const float* queries, const int* sizes, int n_batches, int k,

@@ -8,11 +8,17 @@
 // memory budget -- packs them into one search at the largest rounded k among
 // them, and hands each request its rows (the first calculateK(k) results of
 // each: the top k by (distance, ID) does not depend on the other queries of
-// the batch, nor on a larger k).  Two batch slots alternate, each with its
-// own pinned and device buffers: while the device searches batch i, the
-// worker delivers batch i-1's results and packs batch i+1, whose query copy
-// runs on a copy stream beside the search.  A request's latency is from its
-// submission to the delivery of its results.
+// the batch, nor on a larger k).  Four batch slots take turns, each with its
+// own pinned and device buffers.  Where the index has a captured form for
+// the batch (the SimHash index with pivot cells, rounded k <= 128, at most
+// 4096 queries), the worker enqueues it on one of the index's batch slots, with
+// no host wait, so up to four batches are on the device together; the
+// worker delivers each as it finishes, redoing by an ordinary search the
+// queries its select flagged (or the whole batch, when its cells' second
+// pass was too dense).  Otherwise one search runs at a time: while the
+// device searches batch i, the worker delivers the batches before it and
+// packs batch i+1, whose query copy runs on a copy stream beside the search.
+// A request's latency is from its submission to the delivery of its results.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -46,15 +52,20 @@ struct Request {
     bool done = false;
 };
 
+constexpr int kSlots = 4;
+
 struct Slot {
     float* hq = nullptr;          // pinned packed queries
     lsh_point_t* hres = nullptr;  // pinned results
+    uint32_t* hflags = nullptr;   // pinned: a captured batch's query flags and status words
     lshgpu::DevBuf<float> dq;
     lshgpu::DevBuf<lsh_point_t> dres;
     hipEvent_t ev_in = nullptr, ev_done = nullptr, ev_out = nullptr;
     std::vector<Request*> reqs;
     int kq = 0;
+    int total = 0;
     bool busy = false;
+    bool graph = false;  // enqueued through the index's captured form
     int status = LSH_OK;
     std::string error;
 };
@@ -72,7 +83,7 @@ struct lsh_server {
     std::unordered_map<int64_t, std::unique_ptr<Request>> reqs;
     int64_t next_id = 1;
     bool stop = false;
-    Slot slot[2];
+    Slot slot[kSlots];
     hipStream_t s_comp = nullptr, s_copy = nullptr;
     std::thread worker;
     // statistics
@@ -81,6 +92,8 @@ struct lsh_server {
 
     void run();
     void launch(Slot& sl);
+    bool finished(Slot& sl);
+    void redo_flagged(Slot& sl);
     void deliver(Slot& sl);
 };
 
@@ -91,9 +104,19 @@ void lsh_server::launch(Slot& sl) {
         std::memcpy(sl.hq + (size_t)total * d, r->q, sizeof(float) * (size_t)r->n * d);
         total += r->n;
     }
+    sl.total = total;
+    sl.graph = false;
     sl.status = LSH_OK;
     sl.error.clear();
     try {
+        {
+            std::lock_guard<std::mutex> lk(ix->mu);
+            sl.graph = ix->enqueue_batch(sl.hq, false, total, sl.kq, sl.hres, false, sl.hflags, nullptr, sl.ev_out);
+        }
+        if (sl.graph) {
+            sl.busy = true;
+            return;
+        }
         LSH_HIP(hipMemcpyAsync(sl.dq.get(), sl.hq, sizeof(float) * (size_t)total * d, hipMemcpyHostToDevice, s_copy));
         LSH_HIP(hipEventRecord(sl.ev_in, s_copy));
         LSH_HIP(hipStreamWaitEvent(s_comp, sl.ev_in, 0));
@@ -116,6 +139,36 @@ void lsh_server::launch(Slot& sl) {
     sl.busy = true;
 }
 
+bool lsh_server::finished(Slot& sl) { return sl.status != LSH_OK || hipEventQuery(sl.ev_out) != hipErrorNotReady; }
+
+// A captured batch's queries that its select left incomplete, or all of them
+// when the cells' second pass was too dense (the index then stops using the
+// captured form), searched again by an ordinary search whose results
+// replace the batch's.
+void lsh_server::redo_flagged(Slot& sl) {
+    const int d = ix->d;
+    const int n = sl.total;
+    std::lock_guard<std::mutex> lk(ix->mu);
+    if (sl.hflags[n]) {
+        ix->cells_dense = true;
+        ix->batch_redone += n;
+        ix->search(sl.hq, n, sl.kq, false, sl.hres, false, nullptr);
+        return;
+    }
+    std::vector<int> bad;
+    for (int i = 0; i < n; ++i)
+        if (sl.hflags[i]) bad.push_back(i);
+    if (bad.empty()) return;
+    const int m = (int)bad.size();
+    std::vector<float> q((size_t)m * d);
+    std::vector<lsh_point_t> r((size_t)m * sl.kq);
+    for (int i = 0; i < m; ++i) std::memcpy(q.data() + (size_t)i * d, sl.hq + (size_t)bad[i] * d, sizeof(float) * d);
+    ix->search(q.data(), m, sl.kq, false, r.data(), false, nullptr);
+    for (int i = 0; i < m; ++i)
+        std::memcpy(sl.hres + (size_t)bad[i] * sl.kq, r.data() + (size_t)i * sl.kq, sizeof(lsh_point_t) * sl.kq);
+    ix->batch_redone += m;
+}
+
 void lsh_server::deliver(Slot& sl) {
     if (!sl.busy) return;
     if (sl.status == LSH_OK) {
@@ -123,6 +176,17 @@ void lsh_server::deliver(Slot& sl) {
         if (e != hipSuccess) {
             sl.status = LSH_ERR_DEVICE;
             sl.error = std::string("hipEventSynchronize: ") + hipGetErrorString(e);
+        }
+    }
+    if (sl.status == LSH_OK && sl.graph) {
+        try {
+            redo_flagged(sl);
+        } catch (const lshgpu::Error& e) {
+            sl.status = e.code;
+            sl.error = e.what();
+        } catch (const std::exception& e) {
+            sl.status = LSH_ERR_INTERNAL;
+            sl.error = e.what();
         }
     }
     int row = 0;
@@ -152,18 +216,28 @@ void lsh_server::deliver(Slot& sl) {
 
 void lsh_server::run() {
     (void)hipSetDevice(ix->device);
-    int cur = 0;
+    std::deque<int> inflight;  // slots on the device, oldest first
     for (;;) {
+        // Every slot on the device: the oldest batch is delivered first.
+        if ((int)inflight.size() == kSlots) {
+            deliver(slot[inflight.front()]);
+            inflight.pop_front();
+        }
+        int cur = 0;
+        while (slot[cur].busy) ++cur;
         Slot& sl = slot[cur];
+        bool have = false;
         {
             std::unique_lock<std::mutex> lk(mu);
-            // While the other slot is on the device, wait for work; with
-            // nothing in flight, also wait (or stop when asked and drained).
-            cv_work.wait(lk, [&] { return stop || !queue.empty(); });
-            if (queue.empty() && stop) break;
+            // Nothing in flight: wait for work (or stop when asked and
+            // drained).
+            if (inflight.empty()) {
+                cv_work.wait(lk, [&] { return stop || !queue.empty(); });
+                if (queue.empty() && stop) break;
+            }
             // Admission: whole requests in arrival order while the batch
             // stays within max_batch queries and the memory budget, counted
-            // for both slots at this batch's size.
+            // for two batches at this batch's size.
             int total = 0, kq = 0;
             while (!queue.empty()) {
                 Request* r = queue.front();
@@ -175,23 +249,28 @@ void lsh_server::run() {
                 kq = std::max(kq, lshgpu::calculate_k(r->k));
             }
             sl.kq = kq;
+            have = total > 0;
         }
         const double t0 = now_ms();
-        launch(sl);
-        // The previous batch's results are copied out while this one runs.
-        deliver(slot[cur ^ 1]);
-        busy_ms += now_ms() - t0;
-        cur ^= 1;
-        // Nothing queued: finish this batch too before sleeping.
-        bool idle;
-        {
-            std::lock_guard<std::mutex> lk(mu);
-            idle = queue.empty();
+        if (have) {
+            launch(sl);
+            inflight.push_back(cur);
+            // Batches that have finished meanwhile are delivered now.
+            while (!inflight.empty() && finished(slot[inflight.front()])) {
+                deliver(slot[inflight.front()]);
+                inflight.pop_front();
+            }
+        } else {
+            // Nothing queued: finish the oldest batch before looking again.
+            deliver(slot[inflight.front()]);
+            inflight.pop_front();
         }
-        if (idle) deliver(slot[cur ^ 1]);
+        busy_ms += now_ms() - t0;
     }
-    deliver(slot[0]);
-    deliver(slot[1]);
+    while (!inflight.empty()) {
+        deliver(slot[inflight.front()]);
+        inflight.pop_front();
+    }
 }
 
 extern "C" {
@@ -239,6 +318,8 @@ int lsh_server_create(lsh_index_t* index, const lsh_server_config_t* cfg_in, lsh
             sl.hq = static_cast<float*>(p);
             LSH_HIP(hipHostMalloc(&p, sizeof(lsh_point_t) * (size_t)sv->max_batch * 128, hipHostMallocDefault));
             sl.hres = static_cast<lsh_point_t*>(p);
+            LSH_HIP(hipHostMalloc(&p, sizeof(uint32_t) * ((size_t)sv->max_batch + 3), hipHostMallocDefault));
+            sl.hflags = static_cast<uint32_t*>(p);
             sl.dq.alloc((size_t)sv->max_batch * index->d);
             sl.dres.alloc((size_t)sv->max_batch * 128);
             LSH_HIP(hipEventCreateWithFlags(&sl.ev_in, hipEventDisableTiming));
@@ -322,6 +403,7 @@ int lsh_server_destroy(lsh_server_t* sv) {
         for (Slot& sl : sv->slot) {
             if (sl.hq) (void)hipHostFree(sl.hq);
             if (sl.hres) (void)hipHostFree(sl.hres);
+            if (sl.hflags) (void)hipHostFree(sl.hflags);
             for (hipEvent_t e : {sl.ev_in, sl.ev_done, sl.ev_out})
                 if (e) (void)hipEventDestroy(e);
         }

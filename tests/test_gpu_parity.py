@@ -466,7 +466,9 @@ def test_simhash_index_search_batches_through_graphs(lsh, orc):
     # gives, and the oracle's.  One query's pass overflows (1,500 copies of
     # its point at distance 0): the select flags it and it is redone after
     # the graphs.  A second call replays the graphs; device buffers in and
-    # out; k = 10 shares the k = 32 graphs, k = 100 runs ordinary searches.
+    # out; k = 10 shares the k = 32 graphs, k = 100 (rounded 128, the
+    # rerank without the FP64 recheck kernel) has graphs of its own and
+    # k = 200 runs ordinary searches.
     torch = pytest.importorskip("torch")
     rng = np.random.default_rng(86)
     N, d, words, C = 120000, 48, 16, 256
@@ -497,6 +499,9 @@ def test_simhash_index_search_batches_through_graphs(lsh, orc):
                                       ix.search(queries[:300], 10))
         np.testing.assert_array_equal(ix.search_batches(queries[:300], [250, 50], 100),
                                       ix.search(queries[:300], 100))
+        assert ix.batch_stats()["graph_batches"] == 2
+        np.testing.assert_array_equal(ix.search_batches(queries[:300], [250, 50], 200),
+                                      ix.search(queries[:300], 200))
         assert ix.batch_stats()["graph_batches"] == 0
     assert (want["ID"][5] == np.sort(copies)[:32]).all()
     planes = orc.normal_stream(words * 32 * d)
