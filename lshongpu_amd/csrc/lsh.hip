@@ -446,7 +446,6 @@ __global__ __launch_bounds__(kTPB) void lsh_scan_kernel(LshScanArgs a, TopKView 
     uint32_t* s_qk = s_lo + L;          // L
     float* s_qf = reinterpret_cast<float*>(s_qk + L);  // query vector (d / sdim floats)
     uint32_t* s_qw = s_qk + L;                          // query sketch (words)
-    const int lane = threadIdx.x & 63;
 
     for (int r = blockIdx.y; r < a.Q; r += gridDim.y) {
         const int q = a.qmap ? a.qmap[r] : r;
@@ -584,7 +583,6 @@ __global__ __launch_bounds__(kTPB) void lsh_partial_kernel(LshScanArgs a, const 
     uint32_t* s_qk = s_lo + L;                             // L
     float* s_qf = reinterpret_cast<float*>(s_qk + L);      // d
     __shared__ uint32_t s_n;
-    const int lane = threadIdx.x & 63;
     const uint32_t total = min(ibase[a.Q], cap);
     for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
         int lo = 0, hi = a.Q;  // ibase[lo] <= item < ibase[hi]

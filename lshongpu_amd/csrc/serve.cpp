@@ -3,22 +3,24 @@
 // cudaHelpers.cu:12-14).
 //
 // Continuous batching: callers submit requests (a block of host queries and
-// its k) from any thread; one worker thread takes every queued request that
+// its k) from any thread; a launch thread takes every queued request that
 // fits the next batch -- at most max_batch_queries queries and a device
 // memory budget -- packs them into one search at the largest rounded k among
-// them, and hands each request its rows (the first calculateK(k) results of
-// each: the top k by (distance, ID) does not depend on the other queries of
-// the batch, nor on a larger k).  Four batch slots take turns, each with its
-// own pinned and device buffers.  Where the index has a captured form for
-// the batch (the SimHash index with pivot cells, rounded k <= 128, at most
-// 4096 queries), the worker enqueues it on one of the index's batch slots, with
-// no host wait, so up to four batches are on the device together; the
-// worker delivers each as it finishes, redoing by an ordinary search the
-// queries its select flagged (or the whole batch, when its cells' second
-// pass was too dense).  Otherwise one search runs at a time: while the
-// device searches batch i, the worker delivers the batches before it and
-// packs batch i+1, whose query copy runs on a copy stream beside the search.
-// A request's latency is from its submission to the delivery of its results.
+// them, and each request receives its rows (the first calculateK(k) results
+// of each: the top k by (distance, ID) does not depend on the other queries
+// of the batch, nor on a larger k).  Four batch slots take turns, each with
+// its own pinned and device buffers.  Where the index has a captured form
+// for the batch (the SimHash index with pivot cells, rounded k <= 128, at
+// most 4096 queries), the batch is enqueued on one of the index's batch
+// slots with no host wait, so up to four batches are on the device
+// together; otherwise one search runs at a time, and the next batch's query
+// copy runs on a copy stream beside it.  A delivery thread takes the
+// batches in launch order: it waits for each, redoes by an ordinary search
+// the queries a captured batch's select flagged (or the whole batch, when
+// its cells' second pass was too dense), copies the rows out to the
+// requests and frees the slot, while the launch thread packs and launches
+// the next.  A request's latency is from its submission to the delivery of
+// its results.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -64,7 +66,7 @@ struct Slot {
     std::vector<Request*> reqs;
     int kq = 0;
     int total = 0;
-    bool busy = false;
+    bool busy = false;   // admitted, until delivered (guarded by the server's mutex)
     bool graph = false;  // enqueued through the index's captured form
     int status = LSH_OK;
     std::string error;
@@ -78,21 +80,22 @@ struct lsh_server {
     int64_t budget = 0;        // device bytes the in-flight batches may use
     int64_t per_query = 0;     // device bytes one query of a batch may use
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
+    std::condition_variable cv_work, cv_done, cv_slot, cv_inflight;
     std::deque<Request*> queue;
+    std::deque<int> inflight;  // launched slots, oldest first
+    bool launcher_done = false;
     std::unordered_map<int64_t, std::unique_ptr<Request>> reqs;
     int64_t next_id = 1;
     bool stop = false;
     Slot slot[kSlots];
     hipStream_t s_comp = nullptr, s_copy = nullptr;
-    std::thread worker;
+    std::thread worker, deliverer;
     // statistics
     int64_t batches = 0, queries = 0;
-    double busy_ms = 0;
 
     void run();
+    void run_delivery();
     void launch(Slot& sl);
-    bool finished(Slot& sl);
     void redo_flagged(Slot& sl);
     void deliver(Slot& sl);
 };
@@ -113,10 +116,7 @@ void lsh_server::launch(Slot& sl) {
             std::lock_guard<std::mutex> lk(ix->mu);
             sl.graph = ix->enqueue_batch(sl.hq, false, total, sl.kq, sl.hres, false, sl.hflags, nullptr, sl.ev_out);
         }
-        if (sl.graph) {
-            sl.busy = true;
-            return;
-        }
+        if (sl.graph) return;
         LSH_HIP(hipMemcpyAsync(sl.dq.get(), sl.hq, sizeof(float) * (size_t)total * d, hipMemcpyHostToDevice, s_copy));
         LSH_HIP(hipEventRecord(sl.ev_in, s_copy));
         LSH_HIP(hipStreamWaitEvent(s_comp, sl.ev_in, 0));
@@ -136,10 +136,7 @@ void lsh_server::launch(Slot& sl) {
         sl.status = LSH_ERR_INTERNAL;
         sl.error = e.what();
     }
-    sl.busy = true;
 }
-
-bool lsh_server::finished(Slot& sl) { return sl.status != LSH_OK || hipEventQuery(sl.ev_out) != hipErrorNotReady; }
 
 // A captured batch's queries that its select left incomplete, or all of them
 // when the cells' second pass was too dense (the index then stops using the
@@ -170,7 +167,6 @@ void lsh_server::redo_flagged(Slot& sl) {
 }
 
 void lsh_server::deliver(Slot& sl) {
-    if (!sl.busy) return;
     if (sl.status == LSH_OK) {
         const hipError_t e = hipEventSynchronize(sl.ev_out);
         if (e != hipSuccess) {
@@ -211,33 +207,28 @@ void lsh_server::deliver(Slot& sl) {
     }
     cv_done.notify_all();
     sl.reqs.clear();
-    sl.busy = false;
 }
 
 void lsh_server::run() {
     (void)hipSetDevice(ix->device);
-    std::deque<int> inflight;  // slots on the device, oldest first
     for (;;) {
-        // Every slot on the device: the oldest batch is delivered first.
-        if ((int)inflight.size() == kSlots) {
-            deliver(slot[inflight.front()]);
-            inflight.pop_front();
-        }
         int cur = 0;
-        while (slot[cur].busy) ++cur;
-        Slot& sl = slot[cur];
-        bool have = false;
         {
             std::unique_lock<std::mutex> lk(mu);
-            // Nothing in flight: wait for work (or stop when asked and
-            // drained).
-            if (inflight.empty()) {
-                cv_work.wait(lk, [&] { return stop || !queue.empty(); });
-                if (queue.empty() && stop) break;
-            }
+            cv_work.wait(lk, [&] { return stop || !queue.empty(); });
+            if (queue.empty() && stop) break;
+            // A free slot (with all four on the device, the oldest one's
+            // delivery frees it); requests keep queueing meanwhile.
+            cv_slot.wait(lk, [&] {
+                for (const Slot& x : slot)
+                    if (!x.busy) return true;
+                return false;
+            });
+            while (slot[cur].busy) ++cur;
             // Admission: whole requests in arrival order while the batch
             // stays within max_batch queries and the memory budget, counted
             // for two batches at this batch's size.
+            Slot& sl = slot[cur];
             int total = 0, kq = 0;
             while (!queue.empty()) {
                 Request* r = queue.front();
@@ -249,27 +240,39 @@ void lsh_server::run() {
                 kq = std::max(kq, lshgpu::calculate_k(r->k));
             }
             sl.kq = kq;
-            have = total > 0;
+            sl.busy = true;
         }
-        const double t0 = now_ms();
-        if (have) {
-            launch(sl);
+        launch(slot[cur]);
+        {
+            std::lock_guard<std::mutex> lk(mu);
             inflight.push_back(cur);
-            // Batches that have finished meanwhile are delivered now.
-            while (!inflight.empty() && finished(slot[inflight.front()])) {
-                deliver(slot[inflight.front()]);
-                inflight.pop_front();
-            }
-        } else {
-            // Nothing queued: finish the oldest batch before looking again.
-            deliver(slot[inflight.front()]);
+        }
+        cv_inflight.notify_one();
+    }
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        launcher_done = true;
+    }
+    cv_inflight.notify_one();
+}
+
+void lsh_server::run_delivery() {
+    (void)hipSetDevice(ix->device);
+    for (;;) {
+        int cur;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv_inflight.wait(lk, [&] { return !inflight.empty() || launcher_done; });
+            if (inflight.empty()) break;
+            cur = inflight.front();
             inflight.pop_front();
         }
-        busy_ms += now_ms() - t0;
-    }
-    while (!inflight.empty()) {
-        deliver(slot[inflight.front()]);
-        inflight.pop_front();
+        deliver(slot[cur]);
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            slot[cur].busy = false;
+        }
+        cv_slot.notify_one();
     }
 }
 
@@ -329,7 +332,18 @@ int lsh_server_create(lsh_index_t* index, const lsh_server_config_t* cfg_in, lsh
         LSH_HIP(hipStreamCreateWithFlags(&sv->s_comp, hipStreamNonBlocking));
         LSH_HIP(hipStreamCreateWithFlags(&sv->s_copy, hipStreamNonBlocking));
         lsh_server* raw = sv.get();
-        sv->worker = std::thread([raw] { raw->run(); });
+        sv->deliverer = std::thread([raw] { raw->run_delivery(); });
+        try {
+            sv->worker = std::thread([raw] { raw->run(); });
+        } catch (...) {
+            {
+                std::lock_guard<std::mutex> lk(raw->mu);
+                raw->launcher_done = true;
+            }
+            raw->cv_inflight.notify_one();
+            raw->deliverer.join();
+            throw;
+        }
         *out = sv.release();
     });
 }
@@ -399,6 +413,7 @@ int lsh_server_destroy(lsh_server_t* sv) {
         }
         sv->cv_work.notify_all();
         if (sv->worker.joinable()) sv->worker.join();
+        if (sv->deliverer.joinable()) sv->deliverer.join();
         (void)hipSetDevice(sv->ix->device);
         for (Slot& sl : sv->slot) {
             if (sl.hq) (void)hipHostFree(sl.hq);
