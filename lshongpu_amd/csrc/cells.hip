@@ -705,21 +705,23 @@ __global__ __launch_bounds__(256, 5) void cell_scan_kernel(const uint32_t* __res
             const uint4* src = reinterpret_cast<const uint4*>(sk + ((size_t)t * 32 + col) * W + (lane >> 5) * KS);
     #pragma unroll
             for (int i = 0; i < NV; ++i) x[i] = src[i];
-            if (!kHist) id = ids[(size_t)t * 32 + col];
+            if (!kHist && !v.row_keys) id = ids[(size_t)t * 32 + col];
         };
         // Appends of one tile: slots from the count atomic's old value `ret`
         // (lanes < 32; `other` = the upper half's hits), IDs from the tile's
-        // ID lanes.  Called one tile late, so the atomic's round trip
-        // overlaps the next tile's products.
+        // ID lanes, or with v.row_keys the points' positions in the layout
+        // (the tile's first is pos0) in their place.
         // dist: the 16 distances, two 16-bit halves per word.
-        auto emit = [&](const uint32_t* dist, uint32_t pass, uint32_t ret, uint32_t other, int32_t tid) {
+        auto emit = [&](const uint32_t* dist, uint32_t pass, uint32_t ret, uint32_t other, int32_t tid,
+                        uint32_t pos0) {
             uint32_t base = (uint32_t)__shfl((int)ret, col);
             if (lane >= 32) base += other;
             const size_t row0 = (size_t)q * v.stride + v.K;
     #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 if (!__ballot((pass >> r) & 1u)) continue;
-                const int32_t id = __shfl(tid, fp4::acc_row(r, lane));
+                const int32_t id = v.row_keys ? (int32_t)(pos0 + (uint32_t)fp4::acc_row(r, lane))
+                                              : __shfl(tid, fp4::acc_row(r, lane));
                 if ((pass >> r) & 1u) {
                     if (base < (uint32_t)v.newcap)
                         v.keys[row0 + base] = make_key((dist[r >> 1] >> (16 * (r & 1))) & 0xFFFFu, (uint32_t)id);
@@ -782,7 +784,7 @@ __global__ __launch_bounds__(256, 5) void cell_scan_kernel(const uint32_t* __res
                         dist[r >> 1] = (uint32_t)((BITS - (int)acc[r]) >> 1) |
                                        ((uint32_t)((BITS - (int)acc[r + 1]) >> 1) << 16);
                     // emit shuffles across the wave: every lane calls it, or none.
-                    if (__ballot(pass != 0)) emit(dist, pass, ret, other, cid);
+                    if (__ballot(pass != 0)) emit(dist, pass, ret, other, cid, (t0 + (uint32_t)tt) * 32u);
                 }
                 if (!mine) break;
             }

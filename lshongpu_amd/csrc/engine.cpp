@@ -201,6 +201,11 @@ struct SketchSet {
         std::function<void()> fn;
         bool ran = false;
         std::vector<int32_t> redone;
+        // The pivot-cell layout's IDs: with them the cell scan appends rows
+        // in place of IDs, and the select leaves each kept key's row beside
+        // the best list (rows = true for fn).
+        const int32_t* row_ids = nullptr;
+        bool rows = false;
     }* hook = nullptr;
 };
 
@@ -278,8 +283,10 @@ void sketch_topk(const SketchSet& d, const SketchSet* samples, int nsamples, int
             // work runs.
             uint32_t* hb = sc.pinned((size_t)Q + n_extra);
             pt.begin(PH_SELECT, s);
-            launch_select_merge(st.view(), s, hb, extra, extra ? n_extra : 0);
+            const bool rk = st.view().row_keys != 0;
+            launch_select_merge(st.view(), s, hb, extra, extra ? n_extra : 0, rk ? d.hook->row_ids : nullptr);
             st.selected();
+            if (d.hook) d.hook->rows = rk;
             pt.end(PH_SELECT, s);
             const hipEvent_t ev = sc.event();
             LSH_HIP(hipEventRecord(ev, s));
@@ -299,6 +306,7 @@ void sketch_topk(const SketchSet& d, const SketchSet* samples, int nsamples, int
         // first launch.
         const bool cells_path = d.cells && !force_full_pass();
         st.reset(Q, K, 2 * K + 512, s, true, true, !cells_path);
+        if (cells_path && d.hook && d.hook->row_ids && row_keys_supported(K, 2 * K + 512)) st.rows_in_keys();
         const uint32_t* cstat = nullptr;
         if (cells_path) {
             // Threshold from the nearest pivot cells, then the cells the
@@ -900,16 +908,19 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
         bool ordered = false;
         if (cells.built()) qorder.ensure((size_t)Q);
         const int kind = impl == 3 ? kAngular : kGenJaccard;
-        // The rerank of queries `order` (all queries when null; m of them).
-        auto rerank = [&](const int32_t* order, int m) {
+        // The rerank of queries `order` (all queries when null; m of them);
+        // rows: the select left each best-list key's row beside the list.
+        auto rerank = [&](const int32_t* order, int m, bool rows) {
             pt.begin(PH_FINAL, s);
             // The rerank's filter (timed as the rerank phase), then its FP64
             // recheck when it splits.
+            const TopKView v = st.view();
+            const int32_t* rows_in = rows ? reinterpret_cast<const int32_t*>(v.keys) + 2 * (size_t)v.K : nullptr;
             const bool split =
                 cells.built()
-                    ? launch_rerank(cdata.get(), qp, d, kind, st.view().keys, st.view().stride, nullptr, Ceff, m,
-                                    kq, chalf.get() ? half_error_bound(d) : fp32_error_bound(kind, d), outp, s,
-                                    order, cpos.get(), chalf.get(), &rband, false)
+                    ? launch_rerank(cdata.get(), qp, d, kind, v.keys, v.stride, nullptr, Ceff, m, kq,
+                                    chalf.get() ? half_error_bound(d) : fp32_error_bound(kind, d), outp, s, order,
+                                    cpos.get(), chalf.get(), &rband, false, rows_in, 2 * v.stride)
                     : launch_rerank(data.get(), qp, d, kind, st.view().keys, st.view().stride, nullptr, Ceff, m,
                                     kq, fp32_error_bound(kind, d), outp, s, order, nullptr, nullptr, &rband,
                                     false);
@@ -922,7 +933,8 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
             }
         };
         SketchSet::Hook hook;
-        hook.fn = [&] { rerank(ordered ? qorder.get() : nullptr, Q); };
+        hook.row_ids = cells.built() ? cells.ids.get() : nullptr;
+        hook.fn = [&] { rerank(ordered ? qorder.get() : nullptr, Q, hook.rows); };
         const SketchSet full{dsk.get(), dfp4.get(), N, &cells, cells.built() ? qorder.get() : nullptr, &ordered,
                              &hook};
         const std::vector<SketchSet> sv = sample_views(samples);
@@ -930,7 +942,7 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
                     scratch, 0, s, pt, scan_launches,
                     &mfma_launches, &mfma_points);
         if (!hook.ran) {
-            rerank(ordered ? qorder.get() : nullptr, Q);
+            rerank(ordered ? qorder.get() : nullptr, Q, false);
         } else if (!hook.redone.empty()) {
             // Queries whose candidates were redone after the rerank was
             // enqueued: rerank them again.
@@ -938,7 +950,7 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
             flags.ensure((size_t)m);
             int32_t* idx = reinterpret_cast<int32_t*>(flags.get());
             LSH_HIP(hipMemcpyAsync(idx, hook.redone.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, s));
-            rerank(idx, m);
+            rerank(idx, m, false);
         }
     } else {
         qjl.ensure((size_t)Q * sdim);
@@ -1059,16 +1071,22 @@ void lsh_index::enqueue_sketch_search(batch_slot& sl, int Qc, int kq) {
     const int K = Ceff;
     sketch_simhash(sl.q.get(), Qc, sl.qsk.get(), wpad, s);
     // Lazy reset: the pivot-matrix launch of the cell search clears the view.
+    // The cell scan appends rows in place of IDs where the select can turn
+    // them back; the select then leaves the rows beside the best lists.
     sl.st.reset(Qc, K, 2 * K + 512, s, true, true, false);
+    const bool rk = row_keys_supported(K, 2 * K + 512);
+    if (rk) sl.st.rows_in_keys();
     const uint32_t* cstat = cell_search(cells, sl.qsk.get(), Qc, K, sl.st.view(), sl.scratch, 0, s, sl.qorder.get());
     LSH_REQUIRE(cstat != nullptr, LSH_ERR_INTERNAL, "batched search: the cell search did not run");
     // Flags of the Qc queries, then the cell search's three status words.
-    launch_select_merge(sl.st.view(), s, sl.flags.get(), cstat, 3);
+    launch_select_merge(sl.st.view(), s, sl.flags.get(), cstat, 3, rk ? cells.ids.get() : nullptr);
     sl.st.selected();
     const double eps = chalf.get() ? half_error_bound(d) : fp32_error_bound(kAngular, d);
-    const bool split = launch_rerank(cdata.get(), sl.q.get(), d, kAngular, sl.st.view().keys, sl.st.view().stride,
-                                     nullptr, Ceff, Qc, kq, eps, sl.out.get(), s, sl.qorder.get(), cpos.get(),
-                                     chalf.get(), &sl.rband, false);
+    const TopKView v = sl.st.view();
+    const bool split = launch_rerank(cdata.get(), sl.q.get(), d, kAngular, v.keys, v.stride, nullptr, Ceff, Qc, kq,
+                                     eps, sl.out.get(), s, sl.qorder.get(), cpos.get(), chalf.get(), &sl.rband, false,
+                                     rk ? reinterpret_cast<const int32_t*>(v.keys) + 2 * (size_t)v.K : nullptr,
+                                     2 * v.stride);
     if (split)
         launch_rerank_recheck(cdata.get(), sl.q.get(), d, kAngular, Ceff, Qc, kq, sl.out.get(), s, sl.qorder.get(),
                               sl.rband);

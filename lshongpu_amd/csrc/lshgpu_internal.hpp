@@ -179,6 +179,11 @@ struct TopKView {
     int Q;
     int int_keys;  // 1: integer distances, best list unordered (select_hist_kernel)
     int fresh;     // 1: the best lists are empty and not stored (lazy reset; the next select reads only appends)
+    // 1: an appended key's low word is the point's data row (its position in
+    // the pivot-cell layout) in place of its ID; the select, given the
+    // layout's IDs, turns them into (distance, ID) keys.  Set after a lazy
+    // reset, by the pivot-cell search's caller.
+    int row_keys;
 };
 
 class TopKState {
@@ -197,6 +202,9 @@ class TopKState {
     TopKView view() const { return v_; }
     // After the select that follows a lazy reset: the best lists are stored.
     void selected() { v_.fresh = 0; }
+    // After a lazy reset: the appends will carry rows (TopKView::row_keys);
+    // the next reset clears it.
+    void rows_in_keys() { v_.row_keys = 1; }
     size_t bytes() const { return keys_.bytes() + cnt_.bytes() + thr_.bytes() + ovf_.bytes(); }
 
    private:
@@ -222,11 +230,17 @@ int newcap_for(int K);
 // select.hip
 // flags (may be null, may be pinned host memory): after the select,
 // flags[q] = 1 for each query whose list is incomplete, and flags[Q + i] =
-// extra[i] for i < n_extra (n_extra <= 256).
+// extra[i] for i < n_extra (n_extra <= 256).  row_ids: with v.row_keys, the
+// pivot-cell layout's IDs (row -> ID); the select then also writes the data
+// row of each best-list key i as int32 i of the query's append space (-1
+// for an empty key), for the rerank.
 // The clearing TopKState::reset does, over a view of memory the caller owns.
 void launch_topk_clear(const TopKView& v, hipStream_t s);
 void launch_select_merge(const TopKView& v, hipStream_t s, uint32_t* flags = nullptr,
-                         const uint32_t* extra = nullptr, int n_extra = 0);
+                         const uint32_t* extra = nullptr, int n_extra = 0, const int32_t* row_ids = nullptr);
+// Whether a view of K best keys and newcap appends can carry rows in its
+// keys (the select then holds every key in registers).
+bool row_keys_supported(int K, int newcap);
 void launch_sort_rows(const TopKView& v, hipStream_t s);  // orders the best list of every query
 void launch_count_overflow(const TopKView& v, uint32_t* out_count, hipStream_t s);
 // dst.thr[q] = (integer distance of src.thr[q]) + 1 with ID 0: a scan with it
@@ -284,6 +298,9 @@ void launch_exact_finalize(const float* data, const float* queries, int d, int k
 // one-wave-per-query FP64 recheck, the filter then needing less LDS.
 // Returns whether it split; with recheck_too false the caller launches the
 // recheck itself (launch_rerank_recheck, same queries and order).
+// rows_in (may be null): the candidates' rows, rows_in[q * rows_stride + i]
+// for key i of query q (as the select writes them with row_ids), in place
+// of rowmap lookups.
 struct RerankBand {
     DevBuf<uint64_t> band;  // [queries][Cp]: (row << 32 | ID) of the band's members
     DevBuf<uint32_t> cnt;   // [queries]: members
@@ -292,7 +309,7 @@ bool launch_rerank(const float* data, const float* queries, int d, int kind, con
                    int key_stride, const int32_t* cand, int C, int Q, int k, double eps,
                    lsh_point_t* out, hipStream_t s, const int32_t* qorder = nullptr,
                    const int32_t* rowmap = nullptr, const _Float16* hdata = nullptr, RerankBand* rb = nullptr,
-                   bool recheck_too = true);
+                   bool recheck_too = true, const int32_t* rows_in = nullptr, int rows_stride = 0);
 void launch_rerank_recheck(const float* data, const float* queries, int d, int kind, int C, int Q, int k,
                            lsh_point_t* out, hipStream_t s, const int32_t* qorder, RerankBand& rb);
 

@@ -354,7 +354,8 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
                                                      lsh_point_t* __restrict__ out,
                                                      const int32_t* __restrict__ qorder, int Q, int per,
                                                      const int32_t* __restrict__ rowmap,
-                                                     uint64_t* __restrict__ band, uint32_t* __restrict__ bcnt) {
+                                                     uint64_t* __restrict__ band, uint32_t* __restrict__ bcnt,
+                                                     const int32_t* __restrict__ rows_in, int rows_stride) {
     extern __shared__ uint64_t sm[];
     // LDS, in bytes from sm: k64 [0, 8Cp) (absent with BAND: it aliases
     // k32), ids [8Cp, 12Cp), k32 [12Cp, 20Cp), rows [20Cp, 24Cp), qs [24Cp,
@@ -382,7 +383,8 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
 
     // Candidate IDs and rows first (to LDS), so the row reads below do not
     // wait on the key reads: each thread issues the loads of four candidates
-    // at a time (keys, then their row-map entries) before it uses any.
+    // at a time (keys, then their row-map entries, or with rows_in the rows
+    // beside the keys) before it uses any.
     if (HALF)
         for (int j = threadIdx.x; j < hs; j += blockDim.x) qs[j] = j < d ? qv[j] : 0.0f;
     constexpr int B = 4;
@@ -392,7 +394,9 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
         for (int u = 0; u < B; ++u) {
             const int t = t0 + u * (int)blockDim.x;
             id[u] = -1;
+            r[u] = -1;
             if (t < C) {
+                if (rows_in) r[u] = rows_in[(size_t)q * rows_stride + t];
                 if (cand) {
                     id[u] = cand[(size_t)q * C + t];
                 } else {
@@ -402,7 +406,8 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
             }
         }
 #pragma unroll
-        for (int u = 0; u < B; ++u) r[u] = id[u] < 0 ? -1 : (rowmap ? rowmap[id[u]] : id[u]);
+        for (int u = 0; u < B; ++u)
+            r[u] = id[u] < 0 ? -1 : (rows_in ? r[u] : (rowmap ? rowmap[id[u]] : id[u]));
 #pragma unroll
         for (int u = 0; u < B; ++u) {
             const int t = t0 + u * (int)blockDim.x;
@@ -687,8 +692,10 @@ bool build_unit_rows_half(const float* rows, const int32_t* ids, int64_t npos, i
 bool launch_rerank(const float* data, const float* queries, int d, int kind, const uint64_t* keys,
                    int key_stride, const int32_t* cand, int C, int Q, int k, double eps,
                    lsh_point_t* out, hipStream_t s, const int32_t* qorder, const int32_t* rowmap,
-                   const _Float16* hdata, RerankBand* rb, bool recheck_too) {
+                   const _Float16* hdata, RerankBand* rb, bool recheck_too, const int32_t* rows_in,
+                   int rows_stride) {
     if (Q <= 0) return false;
+    LSH_REQUIRE(!rows_in || (!cand && rows_stride >= C), LSH_ERR_INTERNAL, "rerank: bad candidate rows");
     const int Cp = pow2_ceil(std::max(C, 2));
     LSH_REQUIRE(Cp <= kMaxTopK, LSH_ERR_INVALID, "candidate set larger than 4096");
     LSH_REQUIRE(!hdata || kind == kAngular || kind == kNegCos, LSH_ERR_INTERNAL,
@@ -711,7 +718,8 @@ bool launch_rerank(const float* data, const float* queries, int d, int kind, con
     auto go = [&](auto kernel) {
         set_dynamic_lds((const void*)kernel, lds);
         hipLaunchKernelGGL(kernel, dim3(qorder ? 8 * per : Q), dim3(256), lds, s, data, hdata, hs, queries, d, keys,
-                           key_stride, cand, C, Cp, k, eps, out, qorder, Q, per, rowmap, band, bcnt);
+                           key_stride, cand, C, Cp, k, eps, out, qorder, Q, per, rowmap, band, bcnt, rows_in,
+                           rows_stride);
     };
     auto pick = [&](auto band_tag) {
         constexpr bool B = decltype(band_tag)::value;

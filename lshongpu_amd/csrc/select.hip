@@ -88,7 +88,8 @@ struct KeysInRegisters {
 // flags[Q + i] = extra[i] for i < n_extra.
 template <int RK>
 __global__ __launch_bounds__(kSelThreads) void select_hist_kernel(TopKView v, uint32_t* flags,
-                                                                  const uint32_t* extra, int n_extra) {
+                                                                  const uint32_t* extra, int n_extra,
+                                                                  const int32_t* __restrict__ row_ids) {
     extern __shared__ uint64_t kept[];  // K
     __shared__ uint32_t hist[kSelBins];
     __shared__ uint64_t ties[kTieCap];
@@ -119,13 +120,47 @@ __global__ __launch_bounds__(kSelThreads) void select_hist_kernel(TopKView v, ui
         });
         return kth;
     };
+    // Row keys (fresh views only, held in registers): each key's row is
+    // looked up in the layout's IDs (a query's rows come from a few cells, so
+    // the lookups are local) and the key becomes (distance, ID); each kept
+    // key's row is stored at its slot of the best list, as int32 from the
+    // list's end (the appends were all read before the k-th search's
+    // barriers, so that space is free by then).
+    int32_t* rws = reinterpret_cast<int32_t*>(row + K);
     uint64_t kth;
-    if constexpr (RK > 0) kth = run(KeysInRegisters<RK>(in, total));
-    else kth = run(KeysInMemory{in, total});
+    if constexpr (RK > 0) {
+        if (v.row_keys) {
+            KeysInRegisters<RK> kr(in, total);
+            int32_t rw[RK];
+#pragma unroll
+            for (int r = 0; r < RK; ++r) rw[r] = kr.key[r] != kEmptyKey ? (int32_t)(uint32_t)kr.key[r] : -1;
+#pragma unroll
+            for (int r = 0; r < RK; ++r)
+                if (rw[r] >= 0) kr.key[r] = (kr.key[r] & 0xFFFFFFFF00000000ull) | (uint32_t)row_ids[rw[r]];
+            kth = int_kth_key(kr, (uint32_t)K, hist, ties, sv);
+#pragma unroll
+            for (int r = 0; r < RK; ++r) {
+                const uint64_t key = kr.key[r];
+                if (key != kEmptyKey && key <= kth) {
+                    const uint32_t at = atomicAdd(&s_nkeep, 1u);
+                    if (at < (uint32_t)K) {
+                        kept[at] = key;
+                        rws[at] = rw[r];
+                    }
+                }
+            }
+        } else {
+            kth = run(KeysInRegisters<RK>(in, total));
+        }
+    } else {
+        kth = run(KeysInMemory{in, total});
+    }
     __syncthreads();
     const uint32_t nkeep = min(s_nkeep, (uint32_t)K);
     for (uint32_t i = threadIdx.x; i < nkeep; i += blockDim.x) row[i] = kept[i];
     for (uint32_t i = nkeep + threadIdx.x; i < (uint32_t)K; i += blockDim.x) row[i] = kEmptyKey;
+    if (RK > 0 && v.row_keys)
+        for (uint32_t i = nkeep + threadIdx.x; i < (uint32_t)K; i += blockDim.x) rws[i] = -1;
     if (threadIdx.x == 0) {
         v.thr[q] = nkeep == (uint32_t)K ? kth : kEmptyKey;
         v.cnt[q] = 0;
@@ -220,7 +255,7 @@ void TopKState::reset(int Q, int K, int newcap, hipStream_t s, bool int_keys, bo
     thr_.ensure((size_t)Q);
     ovf_.ensure((size_t)Q);
     v_ = TopKView{keys_.get(), cnt_.get(), thr_.get(), ovf_.get(), K, newcap, stride, Q, int_keys ? 1 : 0,
-                  lazy ? 1 : 0};
+                  lazy ? 1 : 0, 0};
     if (launch) clear(s);
 }
 
@@ -236,21 +271,31 @@ void launch_topk_clear(const TopKView& v, hipStream_t s) {
     check_launch("top-K state reset");
 }
 
-void launch_select_merge(const TopKView& v, hipStream_t s, uint32_t* flags, const uint32_t* extra, int n_extra) {
+// Keys per thread the register select holds.
+constexpr int kSelRegKeys = 12;
+
+bool row_keys_supported(int K, int newcap) {
+    return 2 * (size_t)newcap >= (size_t)K && newcap <= kSelThreads * kSelRegKeys;
+}
+
+void launch_select_merge(const TopKView& v, hipStream_t s, uint32_t* flags, const uint32_t* extra, int n_extra,
+                         const int32_t* row_ids) {
     LSH_REQUIRE(n_extra >= 0 && n_extra <= kSelThreads && (n_extra == 0 || extra), LSH_ERR_INTERNAL,
                 "select: bad extra words");
+    LSH_REQUIRE(!v.row_keys || (v.int_keys && v.fresh && row_ids && row_keys_supported(v.K, v.newcap)),
+                LSH_ERR_INTERNAL, "select: row keys need a fresh integer-key view, the layout's IDs and room");
     if (v.int_keys && v.Q > 0) {
         const size_t lds = (size_t)v.K * sizeof(uint64_t);
         const int most = v.fresh ? v.newcap : v.K + v.newcap;  // keys a query can hold
-        constexpr int RK = 12;
+        constexpr int RK = kSelRegKeys;
         if (most <= kSelThreads * RK) {
             set_dynamic_lds((const void*)select_hist_kernel<RK>, lds + sizeof(uint32_t) * (kSelBins + 8));
             hipLaunchKernelGGL(select_hist_kernel<RK>, dim3(v.Q), dim3(kSelThreads), lds, s, v, flags, extra,
-                               n_extra);
+                               n_extra, row_ids);
         } else {
             set_dynamic_lds((const void*)select_hist_kernel<0>, lds + sizeof(uint32_t) * (kSelBins + 8));
             hipLaunchKernelGGL(select_hist_kernel<0>, dim3(v.Q), dim3(kSelThreads), lds, s, v, flags, extra,
-                               n_extra);
+                               n_extra, row_ids);
         }
         check_launch("select_hist_kernel");
         return;
