@@ -165,6 +165,29 @@ __device__ inline float row_sum16(float x) {
     return x;
 }
 
+template <int CTRL>
+__device__ inline float dpp_f(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xF, 0xF, false));
+}
+
+// The sums over the 16 lanes of a DPP row of four values a0..a3 at once: lane
+// l of the row returns the sum of a_{l / 4}.  The halves of the row trade
+// the pairs they drop (rotation by 8), the lanes of each half then trade
+// the value they drop with their mirror in the half (l <-> 7 - l), and each
+// quad, holding one value summed over four lanes that cover all 16, sums
+// it with two quad permutations: 11 VALU operations for the four sums
+// instead of 16 for four rotation chains.
+__device__ inline float row_sums4(float a0, float a1, float a2, float a3, int l) {
+    const bool hi = l >= 8;
+    const float b0 = (hi ? a2 : a0) + dpp_f<0x128>(hi ? a0 : a2);  // row_ror:8
+    const float b1 = (hi ? a3 : a1) + dpp_f<0x128>(hi ? a1 : a3);
+    const bool up = (l & 7) >= 4;
+    float c = (up ? b1 : b0) + dpp_f<0x141>(up ? b0 : b1);  // row_half_mirror
+    c += dpp_f<0xB1>(c);                                    // quad_perm [1,0,3,2]
+    c += dpp_f<0x4E>(c);                                    // quad_perm [2,3,0,1]
+    return c;
+}
+
 // group_accumulate / group_distance with the distance kind fixed at compile
 // time (the same operations).
 template <int KIND>
@@ -270,6 +293,35 @@ __device__ inline void sums_f16(const _Float16* __restrict__ hdata, int hs, cons
         float4 y0 = *reinterpret_cast<const float4*>(qs + j);
         float4 y1 = *reinterpret_cast<const float4*>(qs + j + 4);
         if (!act) y0 = y1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (U == 4 && Cp >= 4) {
+            // Four consecutive candidates per group and step (Cp, a power of
+            // two, is a multiple of 4): their rows come in two 8-byte LDS
+            // reads, and lanes 4u..4u+3 end with candidate u's sum.
+            for (int g4 = 4 * (threadIdx.x >> 4); g4 < Cp; g4 += 4 * ngroups) {
+                const int2 r01 = *reinterpret_cast<const int2*>(rows + g4);
+                const int2 r23 = *reinterpret_cast<const int2*>(rows + g4 + 2);
+                const int r[4] = {r01.x, r01.y, r23.x, r23.y};
+                half8 xv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    xv[u] = *reinterpret_cast<const half8*>(hdata + (size_t)(r[u] < 0 ? 0 : r[u]) * hs + j);
+                float a[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    a[u] = (float)xv[u][0] * y0.x;
+                    a[u] = __builtin_fmaf((float)xv[u][1], y0.y, a[u]);
+                    a[u] = __builtin_fmaf((float)xv[u][2], y0.z, a[u]);
+                    a[u] = __builtin_fmaf((float)xv[u][3], y0.w, a[u]);
+                    a[u] = __builtin_fmaf((float)xv[u][4], y1.x, a[u]);
+                    a[u] = __builtin_fmaf((float)xv[u][5], y1.y, a[u]);
+                    a[u] = __builtin_fmaf((float)xv[u][6], y1.z, a[u]);
+                    a[u] = __builtin_fmaf((float)xv[u][7], y1.w, a[u]);
+                }
+                const float as = row_sums4(a[0], a[1], a[2], a[3], gl);
+                if ((gl & 3) == 0) sums[g4 + (gl >> 2)] = make_float2(as, 0.0f);
+            }
+            return;
+        }
         for (int t0 = threadIdx.x >> 4; t0 < Cp; t0 += ngroups * U) {
             half8 xv[U];
 #pragma unroll
@@ -289,14 +341,22 @@ __device__ inline void sums_f16(const _Float16* __restrict__ hdata, int hs, cons
                 a[u] = __builtin_fmaf((float)xv[u][5], y1.y, a[u]);
                 a[u] = __builtin_fmaf((float)xv[u][6], y1.z, a[u]);
                 a[u] = __builtin_fmaf((float)xv[u][7], y1.w, a[u]);
-                a[u] = row_sum16(a[u]);
             }
-            float as = a[0];
+            if constexpr (U == 4) {
+                // Lanes 4u..4u+3 of the group hold row u's sum.
+                const float as = row_sums4(a[0], a[1], a[2], a[3], gl);
+                const int t = t0 + (gl >> 2) * ngroups;
+                if ((gl & 3) == 0 && t < Cp) sums[t] = make_float2(as, 0.0f);
+            } else {
 #pragma unroll
-            for (int u = 1; u < U; ++u)
-                if (gl == u) as = a[u];
-            const int t = t0 + gl * ngroups;
-            if (gl < U && t < Cp) sums[t] = make_float2(as, 0.0f);
+                for (int u = 0; u < U; ++u) a[u] = row_sum16(a[u]);
+                float as = a[0];
+#pragma unroll
+                for (int u = 1; u < U; ++u)
+                    if (gl == u) as = a[u];
+                const int t = t0 + gl * ngroups;
+                if (gl < U && t < Cp) sums[t] = make_float2(as, 0.0f);
+            }
         }
         return;
     }
