@@ -678,7 +678,8 @@ void lsh_index::build_fp4(hipStream_t s) {
         if (cells.built()) {
             cell_rows(cells, data.get(), N, d, cdata, cpos, s);
             // The angular rerank (SimHash) filters on an FP16 image of the
-            // unit rows; data with a zero or non-finite row keeps FP32.
+            // unit rows; data with a zero or non-finite row, or more than
+            // 2^32 sixteen-byte pieces of image, keeps FP32.
             // LSHGPU_RERANK_F32=1 keeps the FP32 filter (for comparisons).
             const char* f32 = std::getenv("LSHGPU_RERANK_F32");
             if (impl == 3 && !(f32 && f32[0] == '1'))
@@ -920,7 +921,7 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
                 cells.built()
                     ? launch_rerank(cdata.get(), qp, d, kind, v.keys, v.stride, nullptr, Ceff, m, kq,
                                     chalf.get() ? half_error_bound(d) : fp32_error_bound(kind, d), outp, s, order,
-                                    cpos.get(), chalf.get(), &rband, false, rows_in, 2 * v.stride)
+                                    cpos.get(), chalf.get(), &rband, false, rows_in, 2 * v.stride, chalf.size())
                     : launch_rerank(data.get(), qp, d, kind, st.view().keys, st.view().stride, nullptr, Ceff, m,
                                     kq, fp32_error_bound(kind, d), outp, s, order, nullptr, nullptr, &rband,
                                     false);
@@ -1086,7 +1087,7 @@ void lsh_index::enqueue_sketch_search(batch_slot& sl, int Qc, int kq) {
     const bool split = launch_rerank(cdata.get(), sl.q.get(), d, kAngular, v.keys, v.stride, nullptr, Ceff, Qc, kq,
                                      eps, sl.out.get(), s, sl.qorder.get(), cpos.get(), chalf.get(), &sl.rband, false,
                                      rk ? reinterpret_cast<const int32_t*>(v.keys) + 2 * (size_t)v.K : nullptr,
-                                     2 * v.stride);
+                                     2 * v.stride, chalf.size());
     if (split)
         launch_rerank_recheck(cdata.get(), sl.q.get(), d, kAngular, Ceff, Qc, kq, sl.out.get(), s, sl.qorder.get(),
                               sl.rband);

@@ -301,6 +301,8 @@ void launch_exact_finalize(const float* data, const float* queries, int d, int k
 // rows_in (may be null): the candidates' rows, rows_in[q * rows_stride + i]
 // for key i of query q (as the select writes them with row_ids), in place
 // of rowmap lookups.
+// hdata_elems: the FP16 image's length in halves (0: unknown); an image
+// under 4 GiB is read through 32-bit offsets.
 struct RerankBand {
     DevBuf<uint64_t> band;  // [queries][Cp]: (row << 32 | ID) of the band's members
     DevBuf<uint32_t> cnt;   // [queries]: members
@@ -309,7 +311,8 @@ bool launch_rerank(const float* data, const float* queries, int d, int kind, con
                    int key_stride, const int32_t* cand, int C, int Q, int k, double eps,
                    lsh_point_t* out, hipStream_t s, const int32_t* qorder = nullptr,
                    const int32_t* rowmap = nullptr, const _Float16* hdata = nullptr, RerankBand* rb = nullptr,
-                   bool recheck_too = true, const int32_t* rows_in = nullptr, int rows_stride = 0);
+                   bool recheck_too = true, const int32_t* rows_in = nullptr, int rows_stride = 0,
+                   size_t hdata_elems = 0);
 void launch_rerank_recheck(const float* data, const float* queries, int d, int kind, int C, int Q, int k,
                            lsh_point_t* out, hipStream_t s, const int32_t* qorder, RerankBand& rb);
 

@@ -277,6 +277,59 @@ __device__ inline void sums_f32(const float* __restrict__ data, const float* __r
     }
 }
 
+// Whether sums_f16<4> takes rows[] as offsets into the FP16 image, in
+// 16-byte pieces (row * hs / 8; the image is built only while these fit 32
+// bits), rather than as row numbers.
+__device__ inline bool rows_as_offsets(int hs, int Cp) { return hs <= 128 && Cp >= 4; }
+
+// The FP16 dot products of sums_f16 for hs <= 128 and Cp >= 4: four
+// consecutive candidates per 16-lane group and step (Cp, a power of two, is
+// a multiple of 4).  rows[] holds each candidate's row offset in 16-byte
+// pieces (rows_as_offsets), read in two 8-byte LDS reads; a lane's row piece
+// (jp) is one add from there, addressed in bytes from the image base while
+// the image is under 4 GiB (B32: 32-bit offsets from a uniform base), else
+// as a 64-bit piece index.  Lanes past the row (act false) issue no loads
+// and add zeros: the data path that returns loaded bytes to the lanes
+// bounds this loop, and rows of hs <= 104 halves leave at least three of
+// the 16 lanes idle.  Lanes 4u..4u+3 end with candidate u's sum.  The first
+// product is a fused multiply-add onto zero: the product's own rounding,
+// in one mixed-precision instruction.
+template <bool B32>
+__device__ inline void rows4_sums(const _Float16* __restrict__ hdata, const int32_t* rows, int Cp, uint32_t jp,
+                                  bool act, float4 y0, float4 y1, float2* sums) {
+    const int gl = threadIdx.x & 15;
+    const int ngroups = blockDim.x >> 4;
+    const char* __restrict__ bytes = reinterpret_cast<const char*>(hdata);
+    const half8* __restrict__ base = reinterpret_cast<const half8*>(hdata) + jp;
+    for (int g4 = 4 * (threadIdx.x >> 4); g4 < Cp; g4 += 4 * ngroups) {
+        const uint2 r01 = *reinterpret_cast<const uint2*>(rows + g4);
+        const uint2 r23 = *reinterpret_cast<const uint2*>(rows + g4 + 2);
+        const uint32_t r[4] = {r01.x, r01.y, r23.x, r23.y};
+        float a[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (act) {
+            half8 xv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (B32) xv[u] = *reinterpret_cast<const half8*>(bytes + ((r[u] + jp) << 4));
+                else xv[u] = base[r[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                a[u] = __builtin_fmaf((float)xv[u][0], y0.x, 0.0f);
+                a[u] = __builtin_fmaf((float)xv[u][1], y0.y, a[u]);
+                a[u] = __builtin_fmaf((float)xv[u][2], y0.z, a[u]);
+                a[u] = __builtin_fmaf((float)xv[u][3], y0.w, a[u]);
+                a[u] = __builtin_fmaf((float)xv[u][4], y1.x, a[u]);
+                a[u] = __builtin_fmaf((float)xv[u][5], y1.y, a[u]);
+                a[u] = __builtin_fmaf((float)xv[u][6], y1.z, a[u]);
+                a[u] = __builtin_fmaf((float)xv[u][7], y1.w, a[u]);
+            }
+        }
+        const float as = row_sums4(a[0], a[1], a[2], a[3], gl);
+        if ((gl & 3) == 0) sums[g4 + (gl >> 2)] = make_float2(as, 0.0f);
+    }
+}
+
 // The same over the FP16 image (rows of hs halves, hs a multiple of 8, zero
 // padded): one 16-byte load per lane covers 8 components, the query comes
 // from LDS (qs, zero padded to hs), and only the dot product is summed.
@@ -284,7 +337,7 @@ __device__ inline void sums_f32(const float* __restrict__ data, const float* __r
 // registers; the U loads of a step are issued before any is used.
 template <int U>
 __device__ inline void sums_f16(const _Float16* __restrict__ hdata, int hs, const float* qs, const int32_t* rows,
-                                int Cp, float2* sums) {
+                                int Cp, float2* sums, bool img32) {
     const int gl = threadIdx.x & 15;
     const int ngroups = blockDim.x >> 4;
     if (hs <= 128) {
@@ -294,34 +347,11 @@ __device__ inline void sums_f16(const _Float16* __restrict__ hdata, int hs, cons
         float4 y1 = *reinterpret_cast<const float4*>(qs + j + 4);
         if (!act) y0 = y1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (U == 4 && Cp >= 4) {
-            // Four consecutive candidates per group and step (Cp, a power of
-            // two, is a multiple of 4): their rows come in two 8-byte LDS
-            // reads, and lanes 4u..4u+3 end with candidate u's sum.
-            for (int g4 = 4 * (threadIdx.x >> 4); g4 < Cp; g4 += 4 * ngroups) {
-                const int2 r01 = *reinterpret_cast<const int2*>(rows + g4);
-                const int2 r23 = *reinterpret_cast<const int2*>(rows + g4 + 2);
-                const int r[4] = {r01.x, r01.y, r23.x, r23.y};
-                half8 xv[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    xv[u] = *reinterpret_cast<const half8*>(hdata + (size_t)(r[u] < 0 ? 0 : r[u]) * hs + j);
-                float a[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    a[u] = (float)xv[u][0] * y0.x;
-                    a[u] = __builtin_fmaf((float)xv[u][1], y0.y, a[u]);
-                    a[u] = __builtin_fmaf((float)xv[u][2], y0.z, a[u]);
-                    a[u] = __builtin_fmaf((float)xv[u][3], y0.w, a[u]);
-                    a[u] = __builtin_fmaf((float)xv[u][4], y1.x, a[u]);
-                    a[u] = __builtin_fmaf((float)xv[u][5], y1.y, a[u]);
-                    a[u] = __builtin_fmaf((float)xv[u][6], y1.z, a[u]);
-                    a[u] = __builtin_fmaf((float)xv[u][7], y1.w, a[u]);
-                }
-                const float as = row_sums4(a[0], a[1], a[2], a[3], gl);
-                if ((gl & 3) == 0) sums[g4 + (gl >> 2)] = make_float2(as, 0.0f);
-            }
+            if (img32) rows4_sums<true>(hdata, rows, Cp, (uint32_t)j >> 3, act, y0, y1, sums);
+            else rows4_sums<false>(hdata, rows, Cp, (uint32_t)j >> 3, act, y0, y1, sums);
             return;
         }
+        // Any other U, or fewer than four candidates: rows[] holds rows.
         for (int t0 = threadIdx.x >> 4; t0 < Cp; t0 += ngroups * U) {
             half8 xv[U];
 #pragma unroll
@@ -415,7 +445,8 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
                                                      const int32_t* __restrict__ qorder, int Q, int per,
                                                      const int32_t* __restrict__ rowmap,
                                                      uint64_t* __restrict__ band, uint32_t* __restrict__ bcnt,
-                                                     const int32_t* __restrict__ rows_in, int rows_stride) {
+                                                     const int32_t* __restrict__ rows_in, int rows_stride,
+                                                     int img32) {
     extern __shared__ uint64_t sm[];
     // LDS, in bytes from sm: k64 [0, 8Cp) (absent with BAND: it aliases
     // k32), ids [8Cp, 12Cp), k32 [12Cp, 20Cp), rows [20Cp, 24Cp), qs [24Cp,
@@ -447,6 +478,10 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
     // beside the keys) before it uses any.
     if (HALF)
         for (int j = threadIdx.x; j < hs; j += blockDim.x) qs[j] = j < d ? qv[j] : 0.0f;
+    // With offsets in rows[], a candidate's row is its offset / pieces.
+    const bool roff = HALF && rows_as_offsets(hs, Cp);
+    const uint32_t pieces = (uint32_t)(hs >> 3);
+    auto row_of = [&](int t) -> uint32_t { return roff ? (uint32_t)rows[t] / pieces : (uint32_t)rows[t]; };
     constexpr int B = 4;
     for (int t0 = threadIdx.x; t0 < Cp; t0 += B * blockDim.x) {
         int32_t id[B], r[B];
@@ -473,40 +508,50 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
             const int t = t0 + u * (int)blockDim.x;
             if (t < Cp) {
                 ids[t] = id[u];
-                rows[t] = r[u];
+                // An empty slot reads row 0 in the filter; its sum is not used.
+                rows[t] = roff ? (int32_t)((uint32_t)max(r[u], 0) * pieces) : r[u];
             }
         }
     }
     __syncthreads();
-    // The query norm in FP32, the sequential fmaf chain over its components
-    // (with the FP16 image, over the LDS copy: its zero padding adds exact
-    // zeros), and in FP64 for the band.
+    // The query norm in FP32 and in FP64 for the band.  Over the FP16 image
+    // each wave sums the LDS copy (its zero padding adds exact zeros) with
+    // one multiply-add chain per lane and a butterfly; every lane of every
+    // wave ends with the same value (each step adds the same two operands in
+    // both lanes of a pair), and the error bound holds for any order.  The
+    // FP32 rows keep the sequential chain.
     float qn = 0.0f;
     double nq64 = 0.0;
     if (KIND == kNegCos || KIND == kAngular) {
-        if (HALF)
-            for (int j = 0; j < hs; ++j) qn = __builtin_fmaf(qs[j], qs[j], qn);
-        else
+        if (HALF) {
+            for (int j = threadIdx.x & 63; j < hs; j += 64) qn = __builtin_fmaf(qs[j], qs[j], qn);
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) qn += __shfl_xor(qn, o);
+        } else {
             for (int j = 0; j < d; ++j) qn = __builtin_fmaf(qv[j], qv[j], qn);
+        }
         qn = __builtin_sqrtf(qn);
         if (!BAND) nq64 = norm2_64(qv, d);
     }
     // Approximate distances (k64 serves as scratch for the sums).
     float2* sums = reinterpret_cast<float2*>(k64);
     if (HALF)
-        sums_f16<4>(hdata, hs, qs, rows, Cp, sums);
+        sums_f16<4>(hdata, hs, qs, rows, Cp, sums, img32 != 0);
     else
         sums_f32<KIND, 6>(data, qv, d, rows, Cp, sums);
     __syncthreads();
     // Queries of extreme norm leave the FP16 bound's range assumptions;
     // every candidate is then rechecked.
     const bool all = HALF && !(qn >= 0x1p-60f && qn <= 0x1p60f);
+    // The image's scale and the query norm as one reciprocal (one rounding
+    // more than a division per candidate; the bound allows for it).
+    const float inv = HALF ? (1.0f / kHalfScale) / qn : 0.0f;
     for (int t = threadIdx.x; t < Cp; t += blockDim.x) {
         const int32_t idt = ids[t];
         const float2 ab = sums[t];
         float dist;
         if (HALF) {
-            const float c = (ab.x * (1.0f / kHalfScale)) / qn;
+            const float c = ab.x * inv;
             dist = KIND == kNegCos ? -c : 1.0f - c;
         } else {
             dist = distance_k<KIND>(ab.x, ab.y, qn);
@@ -541,7 +586,7 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
         uint64_t* __restrict__ bq = band + (size_t)pos * Cp;
         for (int s = threadIdx.x; s < m; s += blockDim.x) {
             const int t = ids[s];
-            bq[s] = ((uint64_t)(uint32_t)rows[t] << 32) | (uint32_t)k32[t];
+            bq[s] = ((uint64_t)row_of(t) << 32) | (uint32_t)k32[t];
         }
         if (threadIdx.x == 0) bcnt[pos] = (uint32_t)m;
         return;
@@ -552,7 +597,7 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
     // coalesced loads, so that each FP64 sum reads its row from LDS.
     for (int s = threadIdx.x; s < m; s += blockDim.x) {
         const int t = ids[s];
-        k64[s] = ((uint64_t)(uint32_t)rows[t] << 32) | (uint32_t)k32[t];
+        k64[s] = ((uint64_t)row_of(t) << 32) | (uint32_t)k32[t];
     }
     __syncthreads();
     // Staging: k32 and rows, 12 Cp bytes from byte 12 Cp (16-byte aligned
@@ -722,10 +767,12 @@ double half_error_bound(int d) {
     // (2^-11 + 2^-23) of its magnitude (two roundings, FP64 -> FP32 -> FP16)
     // plus 2^-14 (a subnormal flushed), i.e. 2^-28 of the unit row.  Over
     // the dot product with q that is (2^-11 + 2^-23) + sqrt(d) 2^-28 of
-    // ||q||; the FP32 sums, norm and division add at most the FP32 bound.
+    // ||q||; the FP32 sums, norm and division add at most the FP32 bound,
+    // and the filter's reciprocal (2^-14 / ||q||, then one product per
+    // candidate in place of a division) one rounding more.
     const double u = 1.0 / 16777216.0;
     return 1.01 * (1.0 / 2048.0 + 1.0 / 8388608.0) + std::sqrt((double)d) * (1.0 / 268435456.0) +
-           (2.0 * d + 16.0) * u;
+           (2.0 * d + 17.0) * u;
 }
 
 int half_row_stride(int d) { return (d + 7) / 8 * 8; }
@@ -733,6 +780,11 @@ int half_row_stride(int d) { return (d + 7) / 8 * 8; }
 bool build_unit_rows_half(const float* rows, const int32_t* ids, int64_t npos, int d, DevBuf<_Float16>& out,
                           hipStream_t s) {
     const int hs = half_row_stride(d);
+    // The filter addresses the image by 32-bit offsets in 16-byte pieces.
+    if ((uint64_t)npos * (uint64_t)(hs / 8) > 0xFFFFFFFFull) {
+        out.release();
+        return false;
+    }
     out.alloc((size_t)npos * hs);
     DevBuf<uint32_t> bad(1);
     LSH_HIP(hipMemsetAsync(bad.get(), 0, sizeof(uint32_t), s));
@@ -753,7 +805,7 @@ bool launch_rerank(const float* data, const float* queries, int d, int kind, con
                    int key_stride, const int32_t* cand, int C, int Q, int k, double eps,
                    lsh_point_t* out, hipStream_t s, const int32_t* qorder, const int32_t* rowmap,
                    const _Float16* hdata, RerankBand* rb, bool recheck_too, const int32_t* rows_in,
-                   int rows_stride) {
+                   int rows_stride, size_t hdata_elems) {
     if (Q <= 0) return false;
     LSH_REQUIRE(!rows_in || (!cand && rows_stride >= C), LSH_ERR_INTERNAL, "rerank: bad candidate rows");
     const int Cp = pow2_ceil(std::max(C, 2));
@@ -761,6 +813,8 @@ bool launch_rerank(const float* data, const float* queries, int d, int kind, con
     LSH_REQUIRE(!hdata || kind == kAngular || kind == kNegCos, LSH_ERR_INTERNAL,
                 "rerank: the FP16 image serves the angular kinds only");
     const int hs = hdata ? half_row_stride(d) : 0;
+    // An image under 4 GiB is addressed by 32-bit byte offsets.
+    const int img32 = hdata && hdata_elems > 0 && hdata_elems * sizeof(_Float16) <= 0xFFFFFFFFull ? 1 : 0;
     // Split into the filter and the recheck when a band workspace is given
     // and k fits one wave.
     const bool split = rb != nullptr && k <= 64;
@@ -779,7 +833,7 @@ bool launch_rerank(const float* data, const float* queries, int d, int kind, con
         set_dynamic_lds((const void*)kernel, lds);
         hipLaunchKernelGGL(kernel, dim3(qorder ? 8 * per : Q), dim3(256), lds, s, data, hdata, hs, queries, d, keys,
                            key_stride, cand, C, Cp, k, eps, out, qorder, Q, per, rowmap, band, bcnt, rows_in,
-                           rows_stride);
+                           rows_stride, img32);
     };
     auto pick = [&](auto band_tag) {
         constexpr bool B = decltype(band_tag)::value;
