@@ -15,6 +15,7 @@
 // set.
 #include "device_common.hpp"
 
+#include <cstdlib>
 #include <type_traits>
 
 namespace lshgpu {
@@ -813,8 +814,13 @@ bool launch_rerank(const float* data, const float* queries, int d, int kind, con
     LSH_REQUIRE(!hdata || kind == kAngular || kind == kNegCos, LSH_ERR_INTERNAL,
                 "rerank: the FP16 image serves the angular kinds only");
     const int hs = hdata ? half_row_stride(d) : 0;
-    // An image under 4 GiB is addressed by 32-bit byte offsets.
-    const int img32 = hdata && hdata_elems > 0 && hdata_elems * sizeof(_Float16) <= 0xFFFFFFFFull ? 1 : 0;
+    // An image under 4 GiB is addressed by 32-bit byte offsets
+    // (LSHGPU_RERANK_OFF64=1 takes the 64-bit path of larger images).
+    const char* off64 = std::getenv("LSHGPU_RERANK_OFF64");
+    const int img32 = hdata && hdata_elems > 0 && hdata_elems * sizeof(_Float16) <= 0xFFFFFFFFull &&
+                              !(off64 && off64[0] == '1')
+                          ? 1
+                          : 0;
     // Split into the filter and the recheck when a band workspace is given
     // and k fits one wave.
     const bool split = rb != nullptr && k <= 64;

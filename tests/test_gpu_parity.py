@@ -569,6 +569,19 @@ def test_simhash_index_rerank_filters_match_oracle(lsh, orc, monkeypatch, f32, d
 
 
 @pytest.mark.gpu
+def test_simhash_index_rerank_64bit_image_offsets(lsh, orc, monkeypatch):
+    # The FP16 filter addresses rows by 32-bit byte offsets below 4 GiB of
+    # image and by 64-bit piece indices above; LSHGPU_RERANK_OFF64=1 takes
+    # the second path on a small index, which must give the same result.
+    monkeypatch.delenv("LSHGPU_RERANK_F32", raising=False)
+    monkeypatch.setenv("LSHGPU_RERANK_OFF64", "1")
+    rng = np.random.default_rng(97)
+    N, Q, d, words, C, k = 110000, 160, 100, 16, 512, 32
+    data, queries = _rescaled_clusters(rng, N, Q, d)
+    _index_vs_oracle(lsh, orc, data, queries, words, C, k)
+
+
+@pytest.mark.gpu
 def test_simhash_index_zero_row_keeps_fp32_filter(lsh, orc, monkeypatch):
     # A zero row has no unit direction: the index keeps the FP32 filter (no
     # FP16 image, so fewer device bytes) and still matches the oracle.
