@@ -622,6 +622,17 @@ __global__ __launch_bounds__(1024) void offsets_and_order_kernel(uint32_t* __res
 static_assert(32 * kChunkTiles < 65536, "a chunk's pair counts must fit the 16-bit histogram counters");
 __device__ inline int hist_bin(int d) { return min(d >> 3, kHistBins - 1); }
 
+// Fused appending pass: at most kMaxFuseSteps K-steps of 16 halves (rows of
+// up to 128 halves), and the 16-bit approximate angular distance of a pair
+// from its product sum p (unit rows and unit query, both scaled by 2^14):
+// floor((1 - p 2^-28) 2^15), clamped to [0, 65535].
+constexpr int kMaxFuseSteps = 8;
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+__device__ inline uint32_t fused_q16(float p) {
+    const float t = (1.0f - p * 0x1p-28f) * 32768.0f;
+    return t <= 0.0f ? 0u : (t >= 65535.0f ? 65535u : (uint32_t)t);
+}
+
 // One workgroup per chunk of up to 32 queries of one cell (chunks taken in a
 // grid-stride loop; their count is on the device).  The chunk's query
 // sketches are expanded once into LDS as the B operand; the tiles that the
@@ -629,10 +640,22 @@ __device__ inline int hist_bin(int d) { return min(d >> 3, kHistBins - 1); }
 // packed point sketches of its next tile while it multiplies the current
 // one (the A operand expanded through a byte table in LDS).  kHist: counts
 // the distances of every pair in hist[q][bin] (the first pass); otherwise
-// appends the pairs below each query's threshold to v.  Bounded to 128
-// registers, so that four workgroups (16 waves) share a CU.
-template <int KS, bool kHist>
-__global__ __launch_bounds__(256, 5) void cell_scan_kernel(const uint32_t* __restrict__ sk,
+// appends the pairs below each query's threshold to v.  Bounded to 96
+// registers, so that five workgroups (20 waves) share a CU.
+//
+// kFuse (appending pass, angular index with an FP16 image of the unit rows,
+// v.fused_keys): each tile's pairs also get their approximate angular
+// distance on the matrix cores, the tile's FP16 rows (chalf, hs halves a
+// row, zero padded) against the queries' FP16 high and low parts (q16,
+// query_half_kernel: 2 x 16 ksteps halves a query, staged in dynamic LDS
+// as the B operands), and an appended key's high word becomes (distance <<
+// 16 | q), q = floor((1 - c) * 2^15) clamped to 16 bits, with c = the
+// product sum * 2^-28: the rerank's filter then reads the distance from the
+// key instead of gathering the row.  The tile's rows are read once per
+// chunk of up to 32 queries.  Bounded to 128 registers (four workgroups a
+// CU).
+template <int KS, bool kHist, bool kFuse>
+__global__ __launch_bounds__(256, kFuse ? 4 : 5) void cell_scan_kernel(const uint32_t* __restrict__ sk,
                                                         const int32_t* __restrict__ ids,
                                                         const uint32_t* __restrict__ tinfo,
                                                         const uint32_t* __restrict__ ctile,
@@ -643,7 +666,9 @@ __global__ __launch_bounds__(256, 5) void cell_scan_kernel(const uint32_t* __res
                                                         const uint16_t* __restrict__ D,
                                                         const uint32_t* __restrict__ qs, TopKView v,
                                                         uint32_t* __restrict__ hist,
-                                                        const uint32_t* __restrict__ skip) {
+                                                        const uint32_t* __restrict__ skip,
+                                                        const _Float16* __restrict__ chalf, int hs,
+                                                        const _Float16* __restrict__ q16, int ksteps) {
     constexpr int W = 2 * KS;
     constexpr int BITS = 32 * W;
     constexpr int HW = kHistBins / 2;  // LDS words per query
@@ -653,6 +678,8 @@ __global__ __launch_bounds__(256, 5) void cell_scan_kernel(const uint32_t* __res
     __shared__ v4i sb[KS * 64];        // B operand: step s of lane l at [s * 64 + l]
     __shared__ uint32_t lhist[LH];
     __shared__ uint32_t lut[256];      // byte -> its 8 FP4 nibbles
+    // kFuse: [2][ksteps][64] FP16 B operands (high parts, then low parts).
+    extern __shared__ h8 qb[];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, col = lane & 31;
     lut[threadIdx.x] = fp4::spread8(threadIdx.x);  // read after the chunk's first barrier
     const uint32_t nchunks = (skip && *skip) ? 0u : choff[P];
@@ -673,6 +700,16 @@ __global__ __launch_bounds__(256, 5) void cell_scan_kernel(const uint32_t* __res
         }
         for (int st = wave; st < KS; st += 4)
             sb[st * 64 + lane] = fp4::expand_word(q >= 0 ? qs[(size_t)q * W + 2 * st + (lane >> 5)] : 0u);
+        if (kFuse)
+            for (int st = wave; st < 2 * ksteps; st += 4) {
+                // Step st % ksteps of part st / ksteps: halves 16 (st % ksteps)
+                // + 8 (lane / 32) .. + 7 of the part.
+                h8 z = {};
+                if (q >= 0)
+                    z = *reinterpret_cast<const h8*>(q16 + (size_t)q * (32 * ksteps) + (st / ksteps) * (16 * ksteps) +
+                                                     16 * (st % ksteps) + 8 * (lane >> 5));
+                qb[st * 64 + lane] = z;
+            }
         if (kHist)
             for (int e = threadIdx.x; e < LH; e += 256) lhist[e] = 0;
         const uint32_t t0 = ctile[j] + (local % parts) * kChunkTiles;
@@ -711,9 +748,10 @@ __global__ __launch_bounds__(256, 5) void cell_scan_kernel(const uint32_t* __res
         // (lanes < 32; `other` = the upper half's hits), IDs from the tile's
         // ID lanes, or with v.row_keys the points' positions in the layout
         // (the tile's first is pos0) in their place.
-        // dist: the 16 distances, two 16-bit halves per word.
-        auto emit = [&](const uint32_t* dist, uint32_t pass, uint32_t ret, uint32_t other, int32_t tid,
-                        uint32_t pos0) {
+        // dist: the 16 distances, two 16-bit halves per word; dqs (kFuse):
+        // their 16-bit approximate angular distances, packed the same way.
+        auto emit = [&](const uint32_t* dist, const uint32_t* dqs, uint32_t pass, uint32_t ret, uint32_t other,
+                        int32_t tid, uint32_t pos0) {
             uint32_t base = (uint32_t)__shfl((int)ret, col);
             if (lane >= 32) base += other;
             const size_t row0 = (size_t)q * v.stride + v.K;
@@ -723,8 +761,9 @@ __global__ __launch_bounds__(256, 5) void cell_scan_kernel(const uint32_t* __res
                 const int32_t id = v.row_keys ? (int32_t)(pos0 + (uint32_t)fp4::acc_row(r, lane))
                                               : __shfl(tid, fp4::acc_row(r, lane));
                 if ((pass >> r) & 1u) {
-                    if (base < (uint32_t)v.newcap)
-                        v.keys[row0 + base] = make_key((dist[r >> 1] >> (16 * (r & 1))) & 0xFFFFu, (uint32_t)id);
+                    uint32_t hi = (dist[r >> 1] >> (16 * (r & 1))) & 0xFFFFu;
+                    if (kFuse) hi = (hi << 16) | ((dqs[r >> 1] >> (16 * (r & 1))) & 0xFFFFu);
+                    if (base < (uint32_t)v.newcap) v.keys[row0 + base] = make_key(hi, (uint32_t)id);
                     ++base;
                 }
             }
@@ -783,8 +822,42 @@ __global__ __launch_bounds__(256, 5) void cell_scan_kernel(const uint32_t* __res
                     for (int r = 0; r < 16; r += 2)
                         dist[r >> 1] = (uint32_t)((BITS - (int)acc[r]) >> 1) |
                                        ((uint32_t)((BITS - (int)acc[r + 1]) >> 1) << 16);
+                    uint32_t dqs[kFuse ? 8 : 1];
+                    if (kFuse) {
+                        // This lane's pieces of tile row col: halves 16 st + 8
+                        // (lane / 32) .. + 7, zero past the row.
+                        const _Float16* __restrict__ rp =
+                            chalf + ((size_t)(t0 + (uint32_t)tt) * 32 + col) * hs + 8 * (lane >> 5);
+                        // Two groups of four steps, each group's loads issued together.
+                        v16f ac = {};
+    #pragma unroll
+                        for (int g = 0; g < kMaxFuseSteps; g += 4) {
+                            h8 ah[4];
+    #pragma unroll
+                            for (int i = 0; i < 4; ++i) {
+                                const int st = g + i;
+                                h8 z = {};
+                                if (st < ksteps && 16 * st + 8 * (lane >> 5) < hs)
+                                    z = *reinterpret_cast<const h8*>(rp + 16 * st);
+                                ah[i] = z;
+                            }
+    #pragma unroll
+                            for (int i = 0; i < 4; ++i) {
+                                const int st = g + i;
+                                if (st < ksteps) {
+                                    ac = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[i], qb[st * 64 + lane], ac, 0, 0,
+                                                                                0);
+                                    ac = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[i], qb[(ksteps + st) * 64 + lane],
+                                                                                ac, 0, 0, 0);
+                                }
+                            }
+                        }
+    #pragma unroll
+                        for (int r = 0; r < 16; r += 2)
+                            dqs[r >> 1] = fused_q16(ac[r]) | (fused_q16(ac[r + 1]) << 16);
+                    }
                     // emit shuffles across the wave: every lane calls it, or none.
-                    if (__ballot(pass != 0)) emit(dist, pass, ret, other, cid, (t0 + (uint32_t)tt) * 32u);
+                    if (__ballot(pass != 0)) emit(dist, dqs, pass, ret, other, cid, (t0 + (uint32_t)tt) * 32u);
                 }
                 if (!mine) break;
             }
@@ -925,25 +998,35 @@ void pivots(int words, const uint32_t* rows, int64_t n, const uint32_t* pfrag, i
 template <int KS>
 void launch_cell_scan_ks(const CellIndex& c, unsigned grid, const uint32_t* coff, const uint32_t* choff,
                          const uint32_t* cmap, int Pp, const int32_t* list, const uint16_t* D, const uint32_t* qs,
-                         const TopKView& v, uint32_t* hist, const uint32_t* skip, hipStream_t s) {
-    if (hist)
-        hipLaunchKernelGGL((cell_scan_kernel<KS, true>), dim3(grid), dim3(256), 0, s, c.sk.get(), c.ids.get(),
-                           c.tinfo.get(), c.ctile.get(), coff, choff, cmap, c.P, Pp, list, D, qs, v, hist, skip);
-    else
-        hipLaunchKernelGGL((cell_scan_kernel<KS, false>), dim3(grid), dim3(256), 0, s, c.sk.get(), c.ids.get(),
-                           c.tinfo.get(), c.ctile.get(), coff, choff, cmap, c.P, Pp, list, D, qs, v, hist, skip);
+                         const TopKView& v, uint32_t* hist, const uint32_t* skip, const CellFuse* fz, hipStream_t s) {
+    auto go = [&](auto kernel, size_t lds, const _Float16* chalf, int hs, const _Float16* q16, int ksteps) {
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(256), lds, s, c.sk.get(), c.ids.get(), c.tinfo.get(),
+                           c.ctile.get(), coff, choff, cmap, c.P, Pp, list, D, qs, v, hist, skip, chalf, hs, q16,
+                           ksteps);
+    };
+    if (hist) {
+        go(cell_scan_kernel<KS, true, false>, 0, nullptr, 0, nullptr, 0);
+    } else if (fz) {
+        LSH_REQUIRE(fz->ksteps >= 1 && fz->ksteps <= kMaxFuseSteps && 16 * fz->ksteps >= fz->hs && v.row_keys &&
+                        v.fused_keys,
+                    LSH_ERR_INTERNAL, "cell scan: fused distances need rows of at most 128 halves and row keys");
+        go(cell_scan_kernel<KS, false, true>, sizeof(h8) * 2 * 64 * (size_t)fz->ksteps, fz->chalf, fz->hs, fz->q16,
+           fz->ksteps);
+    } else {
+        go(cell_scan_kernel<KS, false, false>, 0, nullptr, 0, nullptr, 0);
+    }
     check_launch("cell_scan_kernel");
 }
 
 // The scan over a grid of `grid` workgroups that loops over the chunks.
 void cell_scan(const CellIndex& c, unsigned grid, const uint32_t* coff, const uint32_t* choff, uint32_t* cmap,
                int Pp, const int32_t* list, const uint16_t* D, const uint32_t* qs, const TopKView& v, uint32_t* hist,
-               const uint32_t* skip, hipStream_t s) {
+               const uint32_t* skip, const CellFuse* fz, hipStream_t s) {
     switch (c.words) {
-        case 8: launch_cell_scan_ks<4>(c, grid, coff, choff, cmap, Pp, list, D, qs, v, hist, skip, s); break;
-        case 16: launch_cell_scan_ks<8>(c, grid, coff, choff, cmap, Pp, list, D, qs, v, hist, skip, s); break;
-        case 24: launch_cell_scan_ks<12>(c, grid, coff, choff, cmap, Pp, list, D, qs, v, hist, skip, s); break;
-        case 32: launch_cell_scan_ks<16>(c, grid, coff, choff, cmap, Pp, list, D, qs, v, hist, skip, s); break;
+        case 8: launch_cell_scan_ks<4>(c, grid, coff, choff, cmap, Pp, list, D, qs, v, hist, skip, fz, s); break;
+        case 16: launch_cell_scan_ks<8>(c, grid, coff, choff, cmap, Pp, list, D, qs, v, hist, skip, fz, s); break;
+        case 24: launch_cell_scan_ks<12>(c, grid, coff, choff, cmap, Pp, list, D, qs, v, hist, skip, fz, s); break;
+        case 32: launch_cell_scan_ks<16>(c, grid, coff, choff, cmap, Pp, list, D, qs, v, hist, skip, fz, s); break;
         default: fail(LSH_ERR_INTERNAL, "pivot cells need 8, 16, 24 or 32 sketch words");
     }
 }
@@ -1179,7 +1262,7 @@ void cell_search_reserve(const CellIndex& c, int Q, Scratch& sc, int depth) {
 }
 
 const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K, const TopKView& v, Scratch& sc,
-                            int depth, hipStream_t s, int32_t* qorder) {
+                            int depth, hipStream_t s, int32_t* qorder, const CellFuse* fuse) {
     if (!c.built() || Q <= 0) return nullptr;
     const int P = c.P;
     const int Pp = (P + 31) / 32 * 32;
@@ -1252,7 +1335,7 @@ const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K
     hipLaunchKernelGGL(place_from_cells_kernel, dim3(qwaves), dim3(256), 0, s, qcells1, n1, Q, P, cnt1, coff, list,
                        nullptr);
     check_launch("place_from_cells_kernel");
-    cell_scan(c, grid, coff, choff, cmap, Pp, list, D, qs, v, hist, nullptr, s);
+    cell_scan(c, grid, coff, choff, cmap, Pp, list, D, qs, v, hist, nullptr, nullptr, s);
     hipLaunchKernelGGL(hist_threshold_kernel, dim3(qwaves), dim3(256), 0, s, hist, Q, bits, K, v.thr, D, P, Pp,
                        c.cmin.get(), c.cmax.get(), cnt2, qcells2, n2);
     check_launch("hist_threshold_kernel");
@@ -1263,7 +1346,7 @@ const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K
     hipLaunchKernelGGL(place_from_cells_kernel, dim3(qwaves), dim3(256), 0, s, qcells2, n2, Q, P, cnt2, coff, list,
                        stat);
     check_launch("place_from_cells_kernel");
-    cell_scan(c, grid, coff, choff, cmap, Pp, list, D, qs, v, nullptr, stat, s);
+    cell_scan(c, grid, coff, choff, cmap, Pp, list, D, qs, v, nullptr, stat, fuse, s);
     hipStreamCaptureStatus capturing = hipStreamCaptureStatusNone;
     if (debug_cells()) LSH_HIP(hipStreamIsCapturing(s, &capturing));
     if (debug_cells() && capturing == hipStreamCaptureStatusNone) {

@@ -206,6 +206,11 @@ struct SketchSet {
         // the best list (rows = true for fn).
         const int32_t* row_ids = nullptr;
         bool rows = false;
+        // With fuse (and row_ids), the cell scan also writes each appended
+        // pair's approximate angular distance into its key (fused = true
+        // for fn when it did).
+        const CellFuse* fuse = nullptr;
+        bool fused = false;
     }* hook = nullptr;
 };
 
@@ -285,8 +290,11 @@ void sketch_topk(const SketchSet& d, const SketchSet* samples, int nsamples, int
             pt.begin(PH_SELECT, s);
             const bool rk = st.view().row_keys != 0;
             launch_select_merge(st.view(), s, hb, extra, extra ? n_extra : 0, rk ? d.hook->row_ids : nullptr);
+            if (d.hook) {
+                d.hook->rows = rk;
+                d.hook->fused = rk && st.view().fused_keys != 0;
+            }
             st.selected();
-            if (d.hook) d.hook->rows = rk;
             pt.end(PH_SELECT, s);
             const hipEvent_t ev = sc.event();
             LSH_HIP(hipEventRecord(ev, s));
@@ -306,14 +314,18 @@ void sketch_topk(const SketchSet& d, const SketchSet* samples, int nsamples, int
         // first launch.
         const bool cells_path = d.cells && !force_full_pass();
         st.reset(Q, K, 2 * K + 512, s, true, true, !cells_path);
-        if (cells_path && d.hook && d.hook->row_ids && row_keys_supported(K, 2 * K + 512)) st.rows_in_keys();
+        if (cells_path && d.hook && d.hook->row_ids && row_keys_supported(K, 2 * K + 512)) {
+            st.rows_in_keys();
+            if (d.hook->fuse) st.fused_in_keys();
+        }
         const uint32_t* cstat = nullptr;
         if (cells_path) {
             // Threshold from the nearest pivot cells, then the cells the
             // triangle inequality leaves.
             pt.begin(PH_SCAN, s);
             pt.begin(PH_MFMA, s);
-            cstat = cell_search(*d.cells, qs, Q, K, st.view(), sc, depth, s, d.qorder);
+            cstat = cell_search(*d.cells, qs, Q, K, st.view(), sc, depth, s, d.qorder,
+                                st.view().fused_keys ? d.hook->fuse : nullptr);
             pt.end(PH_MFMA, s);
             pt.end(PH_SCAN, s);
             if (cstat && d.ordered) *d.ordered = d.qorder != nullptr;
@@ -562,7 +574,7 @@ size_t lsh_index::device_bytes() const {
                dense.bytes() + q.bytes() + qT.bytes() +
                q_inv.bytes() + qjl.bytes() + qjlT.bytes() + qsk.bytes() + out.bytes() + flags.bytes() + st.bytes();
     for (const SampleLevel& L : samples) b += L.ds.bytes() + L.fp4.bytes();
-    b += cells.bytes() + qorder.bytes() + cdata.bytes() + cpos.bytes() + chalf.bytes() + rband.band.bytes() +
+    b += cells.bytes() + qorder.bytes() + cdata.bytes() + cpos.bytes() + chalf.bytes() + q16.bytes() + rband.band.bytes() +
          rband.cnt.bytes();
     for (const Transform& t : key_steps) b += t.mins.bytes();
     for (const Transform& t : sketch_steps) b += t.mins.bytes();
@@ -902,6 +914,12 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
         }
         if (impl == 3) sketch_simhash(qp, Q, qsk.get(), wpad, s);
         else wmh.sketch(qp, Q, d, qsk.get(), wpad, s);
+        // The queries' FP16 parts for the fused cell scan.
+        const bool fuse = fuse_ok(kq);
+        if (fuse) {
+            q16.ensure((size_t)Q * 32 * fused_k_steps(d));
+            launch_query_half(qp, Q, d, fused_k_steps(d), q16.get(), s);
+        }
         pt.end(PH_PREP, s);
         const int Ceff = (int)std::max<int64_t>(1, std::min<int64_t>(candidates, N));
         const int K = Ceff <= kMaxIntTopK ? Ceff : pow2_ceil(Ceff);
@@ -910,18 +928,21 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
         if (cells.built()) qorder.ensure((size_t)Q);
         const int kind = impl == 3 ? kAngular : kGenJaccard;
         // The rerank of queries `order` (all queries when null; m of them);
-        // rows: the select left each best-list key's row beside the list.
-        auto rerank = [&](const int32_t* order, int m, bool rows) {
+        // rows: the select left each best-list key's row beside the list;
+        // fused: the keys are fused keys (approximate distance and row).
+        auto rerank = [&](const int32_t* order, int m, bool rows, bool fused) {
             pt.begin(PH_FINAL, s);
             // The rerank's filter (timed as the rerank phase), then its FP64
             // recheck when it splits.
             const TopKView v = st.view();
-            const int32_t* rows_in = rows ? reinterpret_cast<const int32_t*>(v.keys) + 2 * (size_t)v.K : nullptr;
+            const int32_t* rows_in =
+                rows && !fused ? reinterpret_cast<const int32_t*>(v.keys) + 2 * (size_t)v.K : nullptr;
+            const double eps = fused ? fused_error_bound(d) : chalf.get() ? half_error_bound(d) : fp32_error_bound(kind, d);
             const bool split =
                 cells.built()
-                    ? launch_rerank(cdata.get(), qp, d, kind, v.keys, v.stride, nullptr, Ceff, m, kq,
-                                    chalf.get() ? half_error_bound(d) : fp32_error_bound(kind, d), outp, s, order,
-                                    cpos.get(), chalf.get(), &rband, false, rows_in, 2 * v.stride, chalf.size())
+                    ? launch_rerank(cdata.get(), qp, d, kind, v.keys, v.stride, nullptr, Ceff, m, kq, eps, outp, s,
+                                    order, cpos.get(), chalf.get(), &rband, false, rows_in, 2 * v.stride,
+                                    chalf.size(), fused ? cells.ids.get() : nullptr)
                     : launch_rerank(data.get(), qp, d, kind, st.view().keys, st.view().stride, nullptr, Ceff, m,
                                     kq, fp32_error_bound(kind, d), outp, s, order, nullptr, nullptr, &rband,
                                     false);
@@ -935,7 +956,14 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
         };
         SketchSet::Hook hook;
         hook.row_ids = cells.built() ? cells.ids.get() : nullptr;
-        hook.fn = [&] { rerank(ordered ? qorder.get() : nullptr, Q, hook.rows); };
+        // Approximate distances fused into the cell scan's keys: the angular
+        // index with its FP16 image, k within the filter's split form.
+        CellFuse cfuse{};
+        if (fuse) {
+            cfuse = CellFuse{chalf.get(), half_row_stride(d), q16.get(), fused_k_steps(d)};
+            hook.fuse = &cfuse;
+        }
+        hook.fn = [&] { rerank(ordered ? qorder.get() : nullptr, Q, hook.rows, hook.fused); };
         const SketchSet full{dsk.get(), dfp4.get(), N, &cells, cells.built() ? qorder.get() : nullptr, &ordered,
                              &hook};
         const std::vector<SketchSet> sv = sample_views(samples);
@@ -943,7 +971,7 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
                     scratch, 0, s, pt, scan_launches,
                     &mfma_launches, &mfma_points);
         if (!hook.ran) {
-            rerank(ordered ? qorder.get() : nullptr, Q, false);
+            rerank(ordered ? qorder.get() : nullptr, Q, false, false);
         } else if (!hook.redone.empty()) {
             // Queries whose candidates were redone after the rerank was
             // enqueued: rerank them again.
@@ -951,7 +979,7 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
             flags.ensure((size_t)m);
             int32_t* idx = reinterpret_cast<int32_t*>(flags.get());
             LSH_HIP(hipMemcpyAsync(idx, hook.redone.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, s));
-            rerank(idx, m, false);
+            rerank(idx, m, false, false);
         }
     } else {
         qjl.ensure((size_t)Q * sdim);
@@ -1022,7 +1050,7 @@ void batch_slot::drop_graphs() {
 size_t batch_slot::bytes() const {
     return q.bytes() + out.bytes() + st.bytes() + flags.bytes() + qsk.bytes() + scratch.bytes() + qorder.bytes() +
            rband.band.bytes() + rband.cnt.bytes() + qkeys.bytes() + qlo.bytes() + qpre.bytes() + qmax.bytes() +
-           qlen.bytes() + wl_ibase.bytes() + wl_flags.bytes() + wl_part.bytes();
+           qlen.bytes() + wl_ibase.bytes() + wl_flags.bytes() + wl_part.bytes() + q16.bytes();
 }
 
 void lsh_index::search_batches(const float* queries, const int* sizes, int n_batches, int k, bool q_on_device,
@@ -1060,6 +1088,16 @@ void lsh_index::search_batches(const float* queries, const int* sizes, int n_bat
 // batch that leaves a query incomplete (flagged by the select) or finds the
 // cells' second pass too dense is redone afterwards by ordinary searches,
 // whose results replace the graph's.
+// Whether a search with k rounded to kq fuses approximate distances into the
+// cell scan's keys: the angular index's FP16 image, d within the fused
+// products' 128 halves, k within the filter's split form (LSHGPU_FUSED=0
+// keeps the filter's row gather).
+bool lsh_index::fuse_ok(int kq) const {
+    const char* off = std::getenv("LSHGPU_FUSED");
+    return framework == 0 && impl == 3 && cells.built() && chalf.get() && kq <= 64 && fused_k_steps(d) > 0 &&
+           sketch_steps.empty() && !(off && off[0] == '0');
+}
+
 bool lsh_index::sketch_graphable(int kq) const {
     const int64_t ceff = std::max<int64_t>(1, std::min<int64_t>(candidates, N));
     return framework == 0 && impl == 3 && cells.built() && !cells_dense && sketch_steps.empty() && kq <= kGraphMaxK &&
@@ -1074,20 +1112,31 @@ void lsh_index::enqueue_sketch_search(batch_slot& sl, int Qc, int kq) {
     // Lazy reset: the pivot-matrix launch of the cell search clears the view.
     // The cell scan appends rows in place of IDs where the select can turn
     // them back; the select then leaves the rows beside the best lists.
+    // With fused keys the scan also writes each pair's approximate distance
+    // into its key, and the rerank's filter reads no rows.
     sl.st.reset(Qc, K, 2 * K + 512, s, true, true, false);
     const bool rk = row_keys_supported(K, 2 * K + 512);
+    const bool fused = rk && fuse_ok(kq) && sl.q16.get();
+    CellFuse cf{};
     if (rk) sl.st.rows_in_keys();
-    const uint32_t* cstat = cell_search(cells, sl.qsk.get(), Qc, K, sl.st.view(), sl.scratch, 0, s, sl.qorder.get());
+    if (fused) {
+        launch_query_half(sl.q.get(), Qc, d, fused_k_steps(d), sl.q16.get(), s);
+        sl.st.fused_in_keys();
+        cf = CellFuse{chalf.get(), half_row_stride(d), sl.q16.get(), fused_k_steps(d)};
+    }
+    const uint32_t* cstat = cell_search(cells, sl.qsk.get(), Qc, K, sl.st.view(), sl.scratch, 0, s, sl.qorder.get(),
+                                        fused ? &cf : nullptr);
     LSH_REQUIRE(cstat != nullptr, LSH_ERR_INTERNAL, "batched search: the cell search did not run");
     // Flags of the Qc queries, then the cell search's three status words.
     launch_select_merge(sl.st.view(), s, sl.flags.get(), cstat, 3, rk ? cells.ids.get() : nullptr);
     sl.st.selected();
-    const double eps = chalf.get() ? half_error_bound(d) : fp32_error_bound(kAngular, d);
+    const double eps = fused ? fused_error_bound(d) : chalf.get() ? half_error_bound(d) : fp32_error_bound(kAngular, d);
     const TopKView v = sl.st.view();
     const bool split = launch_rerank(cdata.get(), sl.q.get(), d, kAngular, v.keys, v.stride, nullptr, Ceff, Qc, kq,
                                      eps, sl.out.get(), s, sl.qorder.get(), cpos.get(), chalf.get(), &sl.rband, false,
-                                     rk ? reinterpret_cast<const int32_t*>(v.keys) + 2 * (size_t)v.K : nullptr,
-                                     2 * v.stride, chalf.size());
+                                     rk && !fused ? reinterpret_cast<const int32_t*>(v.keys) + 2 * (size_t)v.K
+                                                  : nullptr,
+                                     2 * v.stride, chalf.size(), fused ? cells.ids.get() : nullptr);
     if (split)
         launch_rerank_recheck(cdata.get(), sl.q.get(), d, kAngular, Ceff, Qc, kq, sl.out.get(), s, sl.qorder.get(),
                               sl.rband);
@@ -1131,7 +1180,8 @@ void lsh_index::prepare_slots(bool sketch, int kq) {
             sl.rband.band.ensure((size_t)QM * Cp);
             sl.rband.cnt.ensure((size_t)QM);
             sl.flags.ensure((size_t)QM + 3);
-            sig.insert(sig.end(), {(uintptr_t)sl.qsk.get(), (uintptr_t)sl.qorder.get(),
+            if (fuse_ok(1)) sl.q16.ensure((size_t)QM * 32 * fused_k_steps(d));
+            sig.insert(sig.end(), {(uintptr_t)sl.q16.get(), (uintptr_t)sl.qsk.get(), (uintptr_t)sl.qorder.get(),
                                    (uintptr_t)sl.scratch.raw[0].get(), (uintptr_t)sl.scratch.raw[1].get(),
                                    (uintptr_t)sl.scratch.raw[2].get(), (uintptr_t)sl.scratch.raw[3].get(),
                                    (uintptr_t)sl.rband.band.get(), (uintptr_t)sl.rband.cnt.get()});

@@ -147,8 +147,19 @@ void cell_rows(const CellIndex& c, const float* data, int64_t N, int d, DevBuf<f
 // stat[2] = the tile products of the two passes (32-query units), an upper
 // bound.  Uses scratch slots 0-3 of `depth`.  qorder (may be null; Q
 // entries) receives the queries grouped by their nearest cell.
+// fuse (may be null; needs v.row_keys and v.fused_keys): the appending pass
+// also writes each pair's approximate angular distance into its key (see
+// TopKView::fused_keys) from the FP16 image of the unit rows in cell order
+// (chalf, hs halves a row) and the queries' FP16 parts (q16,
+// launch_query_half with 16 ksteps halves a part).
+struct CellFuse {
+    const _Float16* chalf;
+    int hs;
+    const _Float16* q16;
+    int ksteps;
+};
 const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K, const TopKView& v, Scratch& sc,
-                            int depth, hipStream_t s, int32_t* qorder);
+                            int depth, hipStream_t s, int32_t* qorder, const CellFuse* fuse = nullptr);
 // Sizes the scratch slots of depth `depth` for a cell search of Q queries,
 // so that searches of up to Q queries allocate nothing (graph capture).
 void cell_search_reserve(const CellIndex& c, int Q, Scratch& sc, int depth);
@@ -189,6 +200,7 @@ struct batch_slot {
     lshgpu::Scratch scratch;
     lshgpu::DevBuf<int32_t> qorder;
     lshgpu::RerankBand rband;
+    lshgpu::DevBuf<_Float16> q16;  // the queries' FP16 parts of the fused cell scan
     // (K,L) rescan
     lshgpu::DevBuf<uint32_t> qkeys, qlo, qpre, qmax, qlen, wl_ibase, wl_flags;
     lshgpu::DevBuf<uint64_t> wl_part;
@@ -246,6 +258,7 @@ struct lsh_index {
     lshgpu::DevBuf<float> cdata;     // data rows in cell order (rerank reads)
     lshgpu::DevBuf<int32_t> cpos;    // position of each data row in cdata
     lshgpu::DevBuf<_Float16> chalf;  // FP16 image of the unit rows of cdata (angular rerank filter)
+    lshgpu::DevBuf<_Float16> q16;    // per search: the queries' FP16 parts for the fused cell scan
     lshgpu::RerankBand rband;        // the rerank's band workspace (grow-only)
     // Recorded at the end of each search: the next search's stream waits on it,
     // since a search may return while its last kernels still run.
@@ -355,6 +368,7 @@ struct lsh_index {
     // with no host wait: sketch, cell search, select (flags to the slot's
     // flags) and rerank (results to its out).
     bool sketch_graphable(int kq) const;
+    bool fuse_ok(int kq) const;
     void enqueue_sketch_search(batch_slot& sl, int Qc, int kq);
     // Creates the batch slots on first use and sizes every buffer their
     // graphs touch for the largest class (nothing is allocated while

@@ -184,6 +184,11 @@ struct TopKView {
     // layout's IDs, turns them into (distance, ID) keys.  Set after a lazy
     // reset, by the pivot-cell search's caller.
     int row_keys;
+    // 1 (with row_keys): an appended key's high word is (distance << 16 |
+    // q), q the pair's 16-bit approximate angular distance from the fused
+    // cell scan; the select orders by (distance, ID) as ever and keeps such
+    // keys unchanged in the best list, for the rerank's filter.
+    int fused_keys;
 };
 
 class TopKState {
@@ -205,6 +210,9 @@ class TopKState {
     // After a lazy reset: the appends will carry rows (TopKView::row_keys);
     // the next reset clears it.
     void rows_in_keys() { v_.row_keys = 1; }
+    // After rows_in_keys: the appends will carry approximate distances
+    // (TopKView::fused_keys); the next reset clears it.
+    void fused_in_keys() { v_.fused_keys = 1; }
     size_t bytes() const { return keys_.bytes() + cnt_.bytes() + thr_.bytes() + ovf_.bytes(); }
 
    private:
@@ -303,6 +311,11 @@ void launch_exact_finalize(const float* data, const float* queries, int d, int k
 // of rowmap lookups.
 // hdata_elems: the FP16 image's length in halves (0: unknown); an image
 // under 4 GiB is read through 32-bit offsets.
+// fused_ids (angular kinds, k <= 64 with rb): the keys are fused keys
+// (TopKView::fused_keys) whose low word is the candidate's row, and
+// fused_ids[row] its ID; the filter takes each candidate's approximate
+// distance from its key (no row is read) and eps must be
+// fused_error_bound(d).
 struct RerankBand {
     DevBuf<uint64_t> band;  // [queries][Cp]: (row << 32 | ID) of the band's members
     DevBuf<uint32_t> cnt;   // [queries]: members
@@ -312,7 +325,7 @@ bool launch_rerank(const float* data, const float* queries, int d, int kind, con
                    lsh_point_t* out, hipStream_t s, const int32_t* qorder = nullptr,
                    const int32_t* rowmap = nullptr, const _Float16* hdata = nullptr, RerankBand* rb = nullptr,
                    bool recheck_too = true, const int32_t* rows_in = nullptr, int rows_stride = 0,
-                   size_t hdata_elems = 0);
+                   size_t hdata_elems = 0, const int32_t* fused_ids = nullptr);
 void launch_rerank_recheck(const float* data, const float* queries, int d, int kind, int C, int Q, int k,
                            lsh_point_t* out, hipStream_t s, const int32_t* qorder, RerankBand& rb);
 
@@ -325,6 +338,15 @@ bool build_unit_rows_half(const float* rows, const int32_t* ids, int64_t npos, i
 int half_row_stride(int d);
 // Error bound of the angular distance computed over that image.
 double half_error_bound(int d);
+// K-steps of 16 halves of the fused cell scan's products at dimension d
+// (0 when d is too large for it), the bound of its approximate angular
+// distances (the 16-bit keys included), and the queries' FP16 parts it
+// reads: per query 2 x 16 ksteps halves, the high parts of the unit query
+// scaled by 2^14, then the low parts (zero past d; all zero for a query
+// whose norm is zero, not finite or outside [2^-60, 2^60]).
+int fused_k_steps(int d);
+double fused_error_bound(int d);
+void launch_query_half(const float* q, int Q, int d, int ksteps, _Float16* out, hipStream_t s);
 
 // simhash_mfma.hip: SimHash bits (32 hyperplanes a word) on the bf16 matrix
 // cores with an FP32 recheck of every bit within the error bound of zero;

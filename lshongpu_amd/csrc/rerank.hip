@@ -447,7 +447,7 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
                                                      const int32_t* __restrict__ rowmap,
                                                      uint64_t* __restrict__ band, uint32_t* __restrict__ bcnt,
                                                      const int32_t* __restrict__ rows_in, int rows_stride,
-                                                     int img32) {
+                                                     int img32, const int32_t* __restrict__ fused_ids) {
     extern __shared__ uint64_t sm[];
     // LDS, in bytes from sm: k64 [0, 8Cp) (absent with BAND: it aliases
     // k32), ids [8Cp, 12Cp), k32 [12Cp, 20Cp), rows [20Cp, 24Cp), qs [24Cp,
@@ -479,31 +479,49 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
     // beside the keys) before it uses any.
     if (HALF)
         for (int j = threadIdx.x; j < hs; j += blockDim.x) qs[j] = j < d ? qv[j] : 0.0f;
+    // Fused keys (BAND only): (distance << 16 | q, row), q the candidate's
+    // 16-bit approximate distance from the fused cell scan; no row is read
+    // and the IDs come from fused_ids for the band alone.
+    const bool fused = BAND && HALF && fused_ids != nullptr;
     // With offsets in rows[], a candidate's row is its offset / pieces.
-    const bool roff = HALF && rows_as_offsets(hs, Cp);
+    const bool roff = HALF && rows_as_offsets(hs, Cp) && !fused;
     const uint32_t pieces = (uint32_t)(hs >> 3);
     auto row_of = [&](int t) -> uint32_t { return roff ? (uint32_t)rows[t] / pieces : (uint32_t)rows[t]; };
     constexpr int B = 4;
+    // The approximate sums (or, with fused keys, distances) go to `sums`,
+    // which aliases k64 (k32 with BAND) and is filled below.
+    float2* sums = reinterpret_cast<float2*>(k64);
     for (int t0 = threadIdx.x; t0 < Cp; t0 += B * blockDim.x) {
         int32_t id[B], r[B];
+        float fd[B];
 #pragma unroll
         for (int u = 0; u < B; ++u) {
             const int t = t0 + u * (int)blockDim.x;
             id[u] = -1;
             r[u] = -1;
+            fd[u] = 0.0f;
             if (t < C) {
                 if (rows_in) r[u] = rows_in[(size_t)q * rows_stride + t];
                 if (cand) {
                     id[u] = cand[(size_t)q * C + t];
                 } else {
                     const uint64_t key = keys[(size_t)q * key_stride + t];
-                    id[u] = key == kEmptyKey ? -1 : (int32_t)(uint32_t)key;
+                    if (fused) {
+                        // ID unknown (0 marks a candidate); the row and q from the key.
+                        id[u] = key == kEmptyKey ? -1 : 0;
+                        r[u] = (int32_t)(uint32_t)key;
+                        fd[u] = (float)((uint32_t)(key >> 32) & 0xFFFFu) * 0x1p-15f - (KIND == kNegCos ? 1.0f : 0.0f);
+                    } else {
+                        id[u] = key == kEmptyKey ? -1 : (int32_t)(uint32_t)key;
+                    }
                 }
             }
         }
+        if (!fused) {
 #pragma unroll
-        for (int u = 0; u < B; ++u)
-            r[u] = id[u] < 0 ? -1 : (rows_in ? r[u] : (rowmap ? rowmap[id[u]] : id[u]));
+            for (int u = 0; u < B; ++u)
+                r[u] = id[u] < 0 ? -1 : (rows_in ? r[u] : (rowmap ? rowmap[id[u]] : id[u]));
+        }
 #pragma unroll
         for (int u = 0; u < B; ++u) {
             const int t = t0 + u * (int)blockDim.x;
@@ -511,6 +529,7 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
                 ids[t] = id[u];
                 // An empty slot reads row 0 in the filter; its sum is not used.
                 rows[t] = roff ? (int32_t)((uint32_t)max(r[u], 0) * pieces) : r[u];
+                if (fused) sums[t] = make_float2(fd[u], 0.0f);
             }
         }
     }
@@ -534,11 +553,11 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
         qn = __builtin_sqrtf(qn);
         if (!BAND) nq64 = norm2_64(qv, d);
     }
-    // Approximate distances (k64 serves as scratch for the sums).
-    float2* sums = reinterpret_cast<float2*>(k64);
-    if (HALF)
+    // Approximate distances (k64 serves as scratch for the sums; fused keys
+    // brought theirs).
+    if (HALF && !fused)
         sums_f16<4>(hdata, hs, qs, rows, Cp, sums, img32 != 0);
-    else
+    else if (!HALF)
         sums_f32<KIND, 6>(data, qv, d, rows, Cp, sums);
     __syncthreads();
     // Queries of extreme norm leave the FP16 bound's range assumptions;
@@ -551,13 +570,16 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
         const int32_t idt = ids[t];
         const float2 ab = sums[t];
         float dist;
-        if (HALF) {
+        if (fused) {
+            dist = ab.x;
+        } else if (HALF) {
             const float c = ab.x * inv;
             dist = KIND == kNegCos ? -c : 1.0f - c;
         } else {
             dist = distance_k<KIND>(ab.x, ab.y, qn);
         }
-        k32[t] = idt < 0 ? kEmptyKey : make_key(f32_key(dist), (uint32_t)idt);
+        // Fused keys carry the row in place of the ID until the band.
+        k32[t] = idt < 0 ? kEmptyKey : make_key(f32_key(dist), fused ? (uint32_t)rows[t] : (uint32_t)idt);
     }
     __syncthreads();
     // The band: approximate distance at most ub (as a key's high word).
@@ -587,7 +609,8 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ d
         uint64_t* __restrict__ bq = band + (size_t)pos * Cp;
         for (int s = threadIdx.x; s < m; s += blockDim.x) {
             const int t = ids[s];
-            bq[s] = ((uint64_t)row_of(t) << 32) | (uint32_t)k32[t];
+            const uint32_t idb = fused ? (uint32_t)fused_ids[rows[t]] : (uint32_t)k32[t];
+            bq[s] = ((uint64_t)row_of(t) << 32) | idb;
         }
         if (threadIdx.x == 0) bcnt[pos] = (uint32_t)m;
         return;
@@ -761,6 +784,33 @@ __global__ void unit_rows_half_kernel(const float* __restrict__ rows, const int3
     }
 }
 
+// Per query (one wave each): the unit query scaled by 2^14, c = q * (2^14 /
+// ||q||) in FP32, split into FP16 parts hi = (half)c and lo = (half)(c -
+// hi) (c - hi is exact in FP32), 16 ksteps halves each, zero past d; a
+// query whose norm is zero, not finite or outside [2^-60, 2^60] gets zeros
+// (the filter rechecks all its candidates).
+__global__ __launch_bounds__(256) void query_half_kernel(const float* __restrict__ q, int Q, int d, int ksteps,
+                                                         _Float16* __restrict__ out) {
+    const int qi = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (qi >= Q) return;
+    const float* __restrict__ x = q + (size_t)qi * d;
+    float s = 0.0f;
+    for (int j = lane; j < d; j += 64) s = __builtin_fmaf(x[j], x[j], s);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    const float n = __builtin_sqrtf(s);
+    const float r = (n >= 0x1p-60f && n <= 0x1p60f) ? kHalfScale / n : 0.0f;
+    const int kp = 16 * ksteps;
+    _Float16* __restrict__ o = out + (size_t)qi * 2 * kp;
+    for (int j = lane; j < kp; j += 64) {
+        const float c = j < d ? x[j] * r : 0.0f;
+        const _Float16 hi = (_Float16)c;
+        o[j] = hi;
+        o[kp + j] = (_Float16)(c - (float)hi);
+    }
+}
+
 }  // namespace
 
 double half_error_bound(int d) {
@@ -777,6 +827,34 @@ double half_error_bound(int d) {
 }
 
 int half_row_stride(int d) { return (d + 7) / 8 * 8; }
+
+int fused_k_steps(int d) {
+    const int ks = (d + 15) / 16;
+    return ks >= 1 && ks <= 8 ? ks : 0;
+}
+
+double fused_error_bound(int d) {
+    // The image's part as in half_error_bound.  The query's FP16 parts
+    // differ from q 2^14 / ||q|| by the FP32 norm, quotient and product
+    // (a relative (d/2 + 3) 2^-24, mostly a common factor) and the split's
+    // residue (2^-22 of each component, plus flushed subnormals of the low
+    // part, far below that).  The 2 x 16 ksteps exact FP16 products are
+    // summed in FP32 in the matrix cores' order: at most 32 ksteps 2^-24 of
+    // the sum of their magnitudes, at most the product of the norms; then
+    // 1 - c rounds once.  Last, the 16-bit key floors (1 - c) 2^15: half of
+    // 2^-15 more on each side of the band, 2^-16 here.
+    const double u = 1.0 / 16777216.0;
+    const int ks = fused_k_steps(d);
+    return 1.01 * (1.0 / 2048.0 + 1.0 / 8388608.0) + std::sqrt((double)d) * (1.0 / 268435456.0) +
+           (32.0 * ks + 0.5 * d + 8.0) * u + 1.0 / 4194304.0 + 1.0 / 65536.0;
+}
+
+void launch_query_half(const float* q, int Q, int d, int ksteps, _Float16* out, hipStream_t s) {
+    if (Q <= 0) return;
+    LSH_REQUIRE(ksteps >= 1 && ksteps <= 8 && 16 * ksteps >= d, LSH_ERR_INTERNAL, "query parts: bad K-steps");
+    hipLaunchKernelGGL(query_half_kernel, dim3((unsigned)((Q + 3) / 4)), dim3(256), 0, s, q, Q, d, ksteps, out);
+    check_launch("query_half_kernel");
+}
 
 bool build_unit_rows_half(const float* rows, const int32_t* ids, int64_t npos, int d, DevBuf<_Float16>& out,
                           hipStream_t s) {
@@ -806,13 +884,15 @@ bool launch_rerank(const float* data, const float* queries, int d, int kind, con
                    int key_stride, const int32_t* cand, int C, int Q, int k, double eps,
                    lsh_point_t* out, hipStream_t s, const int32_t* qorder, const int32_t* rowmap,
                    const _Float16* hdata, RerankBand* rb, bool recheck_too, const int32_t* rows_in,
-                   int rows_stride, size_t hdata_elems) {
+                   int rows_stride, size_t hdata_elems, const int32_t* fused_ids) {
     if (Q <= 0) return false;
     LSH_REQUIRE(!rows_in || (!cand && rows_stride >= C), LSH_ERR_INTERNAL, "rerank: bad candidate rows");
     const int Cp = pow2_ceil(std::max(C, 2));
     LSH_REQUIRE(Cp <= kMaxTopK, LSH_ERR_INVALID, "candidate set larger than 4096");
     LSH_REQUIRE(!hdata || kind == kAngular || kind == kNegCos, LSH_ERR_INTERNAL,
                 "rerank: the FP16 image serves the angular kinds only");
+    LSH_REQUIRE(!fused_ids || (hdata && rb && k <= 64 && !cand && !rows_in), LSH_ERR_INTERNAL,
+                "rerank: fused keys need the FP16 filter's split form");
     const int hs = hdata ? half_row_stride(d) : 0;
     // An image under 4 GiB is addressed by 32-bit byte offsets
     // (LSHGPU_RERANK_OFF64=1 takes the 64-bit path of larger images).
@@ -839,7 +919,7 @@ bool launch_rerank(const float* data, const float* queries, int d, int kind, con
         set_dynamic_lds((const void*)kernel, lds);
         hipLaunchKernelGGL(kernel, dim3(qorder ? 8 * per : Q), dim3(256), lds, s, data, hdata, hs, queries, d, keys,
                            key_stride, cand, C, Cp, k, eps, out, qorder, Q, per, rowmap, band, bcnt, rows_in,
-                           rows_stride, img32);
+                           rows_stride, img32, fused_ids);
     };
     auto pick = [&](auto band_tag) {
         constexpr bool B = decltype(band_tag)::value;

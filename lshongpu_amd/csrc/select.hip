@@ -125,18 +125,29 @@ __global__ __launch_bounds__(kSelThreads) void select_hist_kernel(TopKView v, ui
     // the lookups are local) and the key becomes (distance, ID); each kept
     // key's row is stored at its slot of the best list, as int32 from the
     // list's end (the appends were all read before the k-th search's
-    // barriers, so that space is free by then).
+    // barriers, so that space is free by then).  Fused keys (high word
+    // distance << 16 | q) are ordered by (distance, ID) and kept as they
+    // came: (distance << 16 | q, row).
     int32_t* rws = reinterpret_cast<int32_t*>(row + K);
     uint64_t kth;
     if constexpr (RK > 0) {
         if (v.row_keys) {
             KeysInRegisters<RK> kr(in, total);
+            const bool fused = v.fused_keys != 0;
             int32_t rw[RK];
+            uint32_t fq[(RK + 1) / 2] = {};  // fused: the 16-bit q of each key, two a word
 #pragma unroll
             for (int r = 0; r < RK; ++r) rw[r] = kr.key[r] != kEmptyKey ? (int32_t)(uint32_t)kr.key[r] : -1;
 #pragma unroll
             for (int r = 0; r < RK; ++r)
-                if (rw[r] >= 0) kr.key[r] = (kr.key[r] & 0xFFFFFFFF00000000ull) | (uint32_t)row_ids[rw[r]];
+                if (rw[r] >= 0) {
+                    uint32_t hi = (uint32_t)(kr.key[r] >> 32);
+                    if (fused) {
+                        fq[r >> 1] |= (hi & 0xFFFFu) << (16 * (r & 1));
+                        hi >>= 16;
+                    }
+                    kr.key[r] = make_key(hi, (uint32_t)row_ids[rw[r]]);
+                }
             kth = int_kth_key(kr, (uint32_t)K, hist, ties, sv);
 #pragma unroll
             for (int r = 0; r < RK; ++r) {
@@ -144,7 +155,10 @@ __global__ __launch_bounds__(kSelThreads) void select_hist_kernel(TopKView v, ui
                 if (key != kEmptyKey && key <= kth) {
                     const uint32_t at = atomicAdd(&s_nkeep, 1u);
                     if (at < (uint32_t)K) {
-                        kept[at] = key;
+                        kept[at] = fused ? make_key(((uint32_t)(key >> 32) << 16) | ((fq[r >> 1] >> (16 * (r & 1))) &
+                                                                                     0xFFFFu),
+                                                    (uint32_t)rw[r])
+                                         : key;
                         rws[at] = rw[r];
                     }
                 }
@@ -255,7 +269,7 @@ void TopKState::reset(int Q, int K, int newcap, hipStream_t s, bool int_keys, bo
     thr_.ensure((size_t)Q);
     ovf_.ensure((size_t)Q);
     v_ = TopKView{keys_.get(), cnt_.get(), thr_.get(), ovf_.get(), K, newcap, stride, Q, int_keys ? 1 : 0,
-                  lazy ? 1 : 0, 0};
+                  lazy ? 1 : 0, 0, 0};
     if (launch) clear(s);
 }
 
