@@ -836,16 +836,17 @@ int fused_k_steps(int d) {
 double fused_error_bound(int d) {
     // The image's part as in half_error_bound.  The query's FP16 parts
     // differ from q 2^14 / ||q|| by the FP32 norm, quotient and product
-    // (a relative (d/2 + 3) 2^-24, mostly a common factor) and the split's
-    // residue (2^-22 of each component, plus flushed subnormals of the low
-    // part, far below that).  The 2 x 16 ksteps exact FP16 products are
+    // (a relative (d/2 + 3) 2^-24, mostly a common factor), the split's
+    // residue (2^-22 of each component) and, should the matrix cores flush
+    // FP16 subnormals, parts below 2^-14 (2^-28 of the unit query each, for
+    // the high and the low part).  The 2 x 16 ksteps exact FP16 products are
     // summed in FP32 in the matrix cores' order: at most 32 ksteps 2^-24 of
     // the sum of their magnitudes, at most the product of the norms; then
-    // 1 - c rounds once.  Last, the 16-bit key floors (1 - c) 2^15: half of
-    // 2^-15 more on each side of the band, 2^-16 here.
+    // 1 - c rounds once.  Last, the 16-bit key floors (1 - c) 2^15: the
+    // filter takes the midpoint of that step, 2^-16 either way.
     const double u = 1.0 / 16777216.0;
     const int ks = fused_k_steps(d);
-    return 1.01 * (1.0 / 2048.0 + 1.0 / 8388608.0) + std::sqrt((double)d) * (1.0 / 268435456.0) +
+    return 1.01 * (1.0 / 2048.0 + 1.0 / 8388608.0) + 3.0 * std::sqrt((double)d) * (1.0 / 268435456.0) +
            (32.0 * ks + 0.5 * d + 8.0) * u + 1.0 / 4194304.0 + 1.0 / 65536.0;
 }
 

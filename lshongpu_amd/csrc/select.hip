@@ -125,29 +125,56 @@ __global__ __launch_bounds__(kSelThreads) void select_hist_kernel(TopKView v, ui
     // the lookups are local) and the key becomes (distance, ID); each kept
     // key's row is stored at its slot of the best list, as int32 from the
     // list's end (the appends were all read before the k-th search's
-    // barriers, so that space is free by then).  Fused keys (high word
-    // distance << 16 | q) are ordered by (distance, ID) and kept as they
-    // came: (distance << 16 | q, row).
+    // barriers, so that space is free by then).
     int32_t* rws = reinterpret_cast<int32_t*>(row + K);
     uint64_t kth;
     if constexpr (RK > 0) {
-        if (v.row_keys) {
+        if (v.row_keys && v.fused_keys) {
+            // Fused keys: the K-th by (distance, ID) with the IDs looked up
+            // for the boundary bin's keys alone (the others are kept or
+            // dropped by distance, whatever their low word); the kept keys
+            // stay (distance << 16 | q, row), and their IDs are looked up
+            // later only where needed.
             KeysInRegisters<RK> kr(in, total);
-            const bool fused = v.fused_keys != 0;
+            uint32_t fq[(RK + 1) / 2] = {};  // the 16-bit q of each key, two a word
+#pragma unroll
+            for (int r = 0; r < RK; ++r)
+                if (kr.key[r] != kEmptyKey) {
+                    const uint32_t hi = (uint32_t)(kr.key[r] >> 32);
+                    fq[r >> 1] |= (hi & 0xFFFFu) << (16 * (r & 1));
+                    kr.key[r] = make_key(hi >> 16, (uint32_t)kr.key[r]);
+                }
+            // The boundary bin's keys get their IDs (rows kept aside), all
+            // lookups issued together.
+            uint32_t brow[RK];
+            const auto to_ids = [&](uint32_t T) {
+#pragma unroll
+                for (int r = 0; r < RK; ++r) {
+                    const uint64_t key = kr.key[r];
+                    const bool b = key != kEmptyKey && min((uint32_t)(key >> 32), (uint32_t)(kSelBins - 1)) == T;
+                    brow[r] = (uint32_t)key;
+                    if (b) kr.key[r] = (key & 0xFFFFFFFF00000000ull) | (uint32_t)row_ids[(uint32_t)key];
+                }
+            };
+            kth = int_kth_key(kr, (uint32_t)K, hist, ties, sv, to_ids);
+#pragma unroll
+            for (int r = 0; r < RK; ++r) {
+                const uint64_t key = kr.key[r];
+                if (key != kEmptyKey && key <= kth) {
+                    const uint32_t at = atomicAdd(&s_nkeep, 1u);
+                    if (at < (uint32_t)K)
+                        kept[at] = make_key(((uint32_t)(key >> 32) << 16) | ((fq[r >> 1] >> (16 * (r & 1))) & 0xFFFFu),
+                                            brow[r]);
+                }
+            }
+        } else if (v.row_keys) {
+            KeysInRegisters<RK> kr(in, total);
             int32_t rw[RK];
-            uint32_t fq[(RK + 1) / 2] = {};  // fused: the 16-bit q of each key, two a word
 #pragma unroll
             for (int r = 0; r < RK; ++r) rw[r] = kr.key[r] != kEmptyKey ? (int32_t)(uint32_t)kr.key[r] : -1;
 #pragma unroll
             for (int r = 0; r < RK; ++r)
-                if (rw[r] >= 0) {
-                    uint32_t hi = (uint32_t)(kr.key[r] >> 32);
-                    if (fused) {
-                        fq[r >> 1] |= (hi & 0xFFFFu) << (16 * (r & 1));
-                        hi >>= 16;
-                    }
-                    kr.key[r] = make_key(hi, (uint32_t)row_ids[rw[r]]);
-                }
+                if (rw[r] >= 0) kr.key[r] = (kr.key[r] & 0xFFFFFFFF00000000ull) | (uint32_t)row_ids[rw[r]];
             kth = int_kth_key(kr, (uint32_t)K, hist, ties, sv);
 #pragma unroll
             for (int r = 0; r < RK; ++r) {
@@ -155,10 +182,7 @@ __global__ __launch_bounds__(kSelThreads) void select_hist_kernel(TopKView v, ui
                 if (key != kEmptyKey && key <= kth) {
                     const uint32_t at = atomicAdd(&s_nkeep, 1u);
                     if (at < (uint32_t)K) {
-                        kept[at] = fused ? make_key(((uint32_t)(key >> 32) << 16) | ((fq[r >> 1] >> (16 * (r & 1))) &
-                                                                                     0xFFFFu),
-                                                    (uint32_t)rw[r])
-                                         : key;
+                        kept[at] = key;
                         rws[at] = rw[r];
                     }
                 }

@@ -79,12 +79,22 @@ __device__ inline uint64_t radix_kth_in_bin(const Each& each, uint32_t T, uint32
     return prefix;
 }
 
+// The caller's hook of int_kth_key: nothing.
+struct NoBoundaryPrep {
+    __device__ void operator()(uint32_t) const {}
+};
+
 // The K-th smallest non-empty key of the set `each` visits (K >= 1), or
 // kEmptyKey when there are fewer than K.  LDS: hist[kSelBins],
 // ties[kTieCap], sv[kSelScratch].  Every thread of the workgroup must call
 // it; it ends with a barrier, so the caller may reuse that LDS at once.
-template <class Each>
-__device__ inline uint64_t int_kth_key(const Each& each, uint32_t K, uint32_t* hist, uint64_t* ties, uint32_t* sv) {
+// prep(T) runs in every thread once the boundary bin T is known (T =
+// kSelBins: fewer than K keys), before the keys of that bin are compared:
+// the caller may then rewrite the low words of its keys in bin T (the
+// order of equal distances), which `each` must then visit.
+template <class Each, class Prep = NoBoundaryPrep>
+__device__ inline uint64_t int_kth_key(const Each& each, uint32_t K, uint32_t* hist, uint64_t* ties, uint32_t* sv,
+                                       const Prep& prep = Prep{}) {
     uint32_t& s_T = sv[0];
     uint32_t& s_below = sv[1];
     uint32_t& s_nties = sv[4];
@@ -128,6 +138,7 @@ __device__ inline uint64_t int_kth_key(const Each& each, uint32_t K, uint32_t* h
     }
     __syncthreads();
     const uint32_t T = s_T;
+    prep(T);
     uint64_t kth = kEmptyKey;
     if (T < (uint32_t)kSelBins) {
         const bool last = T == (uint32_t)(kSelBins - 1);

@@ -569,6 +569,31 @@ def test_simhash_index_rerank_filters_match_oracle(lsh, orc, monkeypatch, f32, d
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("d", [37, 100, 128, 130])
+def test_simhash_index_fused_distances_match_oracle(lsh, orc, monkeypatch, d):
+    # The cell scan packs each appended pair's approximate angular distance
+    # (FP16 rows against the query's FP16 parts on the matrix cores, 16-bit
+    # fixed point) into its key, and the filter reads it there instead of
+    # gathering rows (d <= 128; d = 130 keeps the gather).  Both forms, and
+    # k beyond the filter's split form (100), must give the oracle's result.
+    monkeypatch.delenv("LSHGPU_RERANK_F32", raising=False)
+    rng = np.random.default_rng(98 + d)
+    N, Q, words, C = 110000, 160, 16, 512
+    data, queries = _rescaled_clusters(rng, N, Q, d)
+    for j, b in enumerate(rng.integers(0, N, 4)):
+        grp = rng.choice(N, 200, replace=False)
+        data[grp] = data[b] * (1 + 1e-4 * rng.standard_normal((200, d)).astype(np.float32))
+        queries[j] = data[b] * 0.5
+    got = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("LSHGPU_FUSED", fused)
+        got[fused] = _index_vs_oracle(lsh, orc, data, queries, words, C, 32)
+    np.testing.assert_array_equal(got["1"], got["0"])
+    monkeypatch.setenv("LSHGPU_FUSED", "1")
+    _index_vs_oracle(lsh, orc, data, queries, words, C, 100)
+
+
+@pytest.mark.gpu
 def test_simhash_index_rerank_64bit_image_offsets(lsh, orc, monkeypatch):
     # The FP16 filter addresses rows by 32-bit byte offsets below 4 GiB of
     # image and by 64-bit piece indices above; LSHGPU_RERANK_OFF64=1 takes
