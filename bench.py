@@ -26,9 +26,12 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# HBM bytes per rerank_kernel launch measured with FETCH_SIZE/WRITE_SIZE
-# counters at the default workload, recorded in this summary.
+# HBM bytes per launch of the dominant kernel measured with FETCH_SIZE /
+# WRITE_SIZE counters at the default workload, recorded in these summaries:
+# the pivot-cell search's appending pass (with the approximate distances
+# fused into its keys, the default) and the rerank's filter.
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "rerank_kernel_simhash1024_N1.2M_Q10k.md")
+PASS2_PROFILE = os.path.join(ROOT, "profiles", "cell_scan_kernel_fused_simhash1024_N1.2M_Q10k.md")
 DEFAULT_WORKLOAD = dict(n=1_200_000, queries=10_000, words=32, candidates=1024, k=32)
 
 # MI355X published peaks.
@@ -45,12 +48,12 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
-def measured_traffic(args):
+def measured_traffic(args, path=TRAFFIC_PROFILE):
     """Counter-measured bytes per launch, when this run is the profiled workload."""
     if any(getattr(args, key) != val for key, val in DEFAULT_WORKLOAD.items()):
         return None
     try:
-        for line in open(TRAFFIC_PROFILE):
+        for line in open(path):
             if line.startswith("TRAFFIC_BYTES_PER_LAUNCH:"):
                 return float(line.split(":", 1)[1])
     except OSError:
@@ -90,6 +93,52 @@ def roofline(args, bytes_launch: float, avg_launch_ms: float) -> dict:
                          "L2; the limit is the rate of the random row gather through L2, not HBM bandwidth (the "
                          "launch time is the same with 6, 7 or 8 resident workgroups per CU, and 36% fewer bytes "
                          "past L2 took 4% off it)",
+        "avg_launch_ms": avg_launch_ms,
+        "launches_per_step": 1,
+    }
+
+
+def roofline_pass2(args, avg_launch_ms: float, tiles: float, bits: int, d: int) -> dict:
+    """The pivot-cell search's appending pass with fused distances against
+    HBM: `achieved` is the counter-measured HBM traffic of one launch over
+    its live launch time.  The algorithmic bytes are the data of every tile
+    product (its 32 points' packed sketches and FP16 unit rows, read once per
+    chunk of up to 32 queries); the appended keys it writes are in the
+    counters but not in that figure.  Its matrix-core rate (FP4 Hamming
+    products and FP16 hi/lo angular products) is reported beside it."""
+    traffic = measured_traffic(args, PASS2_PROFILE)
+    ks = (d + 15) // 16
+    tile_bytes = 32.0 * (bits / 8.0 + 2.0 * ((d + 7) // 8 * 8))
+    algo = tiles * tile_bytes
+    secs = avg_launch_ms * 1e-3
+    if traffic is not None and avg_launch_ms > 0:
+        achieved = traffic / secs / 1e9
+        basis = "counter-measured HBM bytes per launch (traffic) / avg_launch_ms"
+    else:
+        achieved = algo / secs / 1e9 if secs > 0 else 0.0
+        basis = "no counter profile for this workload: algorithmic bytes / avg_launch_ms"
+    fp4_flops = tiles * 32 * 32 * 2.0 * bits
+    fp16_flops = tiles * 32 * 32 * 2.0 * 2 * 16 * ks
+    return {
+        "kernel": "cell_scan_kernel, appending pass (FP4 Hamming tile products, threshold appends, FP16 "
+                  "approximate angular distances fused into the keys)",
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": traffic,
+        "achieved_basis": basis,
+        "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a plain copy kernel; "
+                          "profiles/cell_scan_kernel_fused_simhash1024_N1.2M_Q10k.md",
+        "algorithmic_bytes_per_launch": algo,
+        "tile_products_per_launch": tiles,
+        "mfma": {"fp4_tflops": fp4_flops / secs / 1e12 if secs > 0 else 0.0,
+                 "fp4_frac_of_dense_peak": fp4_flops / secs / 10.1e15 if secs > 0 else 0.0,
+                 "fp16_tflops": fp16_flops / secs / 1e12 if secs > 0 else 0.0,
+                 "fp16_frac_of_dense_peak": fp16_flops / secs / 2.5e15 if secs > 0 else 0.0},
+        "bound_note": "neither: each chunk of up to 32 queries x 8 tiles lives 11-25 us, most of it dependent "
+                      "set-up loads, operand staging and the appends' atomics; 4 workgroups a CU (128 registers)",
         "avg_launch_ms": avg_launch_ms,
         "launches_per_step": 1,
     }
@@ -258,16 +307,22 @@ def main() -> int:
 
     for _ in range(args.warmup):
         step()
-    # Timed region: HIP events around the rerank launch only (profiling mode
-    # 2), read once after the steps.
-    ix.set_profiling(2)
+    # The dominant kernel: with the approximate distances fused into the cell
+    # scan's keys (the default for k <= 64 over the FP16 image) the appending
+    # pass of the cell search, else the rerank's filter.
+    half_filter = os.environ.get("LSHGPU_RERANK_F32", "0") != "1"
+    fused = half_filter and kq <= 64 and os.environ.get("LSHGPU_FUSED", "1") != "0" and d <= 128
+    # Timed region: HIP events around that kernel's launch only (profiling
+    # mode 3 or 2), read once after the steps.
+    ix.set_profiling(3 if fused else 2)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    rerank_ms_total = ix.phase_times()["final_ms"]
+    live = ix.phase_times()
+    dominant_ms_total = live["pass2_ms"] if fused else live["final_ms"]
     # Per-phase breakdown from a separate, fully profiled pass (events around
     # every phase cost time, so it stays out of the timed region).
     nb = max(1, min(args.steps, 5))
@@ -291,8 +346,7 @@ def main() -> int:
     # entries of 8 bytes).
     steps = args.steps
     C = min(args.candidates, args.n)
-    avg_launch_ms = rerank_ms_total / steps
-    half_filter = os.environ.get("LSHGPU_RERANK_F32", "0") != "1"
+    avg_launch_ms = dominant_ms_total / steps
     row_bytes = 2.0 * ((d + 7) // 8 * 8) if half_filter else 4.0 * d
     bytes_launch = float(Q) * (row_bytes * C + 8.0 * C + 4.0 * d + 8.0 * kq)
     # The threshold pass over the pivot cells: matrix-core tile products
@@ -444,7 +498,9 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u32 Hamming distances; angular rerank filter in f32 arithmetic over f16 unit rows with a proven "
+        "dtype": "u32 Hamming distances; approximate angular distances from f16 matrix-core products of unit rows "
+                 "(f32 sums, 16-bit fixed point) with a proven error band, f64 recheck of the band" if fused else
+                 "u32 Hamming distances; angular rerank filter in f32 arithmetic over f16 unit rows with a proven "
                  "error band, f64 recheck of the band" if half_filter else
                  "u32 Hamming distances; f32 rerank filter, f64 recheck of the error band",
         "data": "synthetic (seeded Gaussian mixture), generated in-process",
@@ -457,7 +513,13 @@ def main() -> int:
         "index_build_ms": build_ms,
         "phase_ms_per_step": phases,
         "phase_breakdown_note": f"from {nb} fully profiled steps after the timed region",
-        "roofline": roofline(args, bytes_launch, avg_launch_ms),
+        "roofline": roofline_pass2(args, avg_launch_ms, phases["pass2_tiles"], bits, d) if fused
+        else roofline(args, bytes_launch, avg_launch_ms),
+        # The rerank's filter beside it, timed in the fully profiled steps.
+        "rerank_filter": {"ms": phases["final_ms"], "fused_distances": fused,
+                          "note": "with fused distances the filter reads each candidate's approximate distance "
+                                  "from its key; without, it gathers the candidate rows (the roofline of that "
+                                  "form is the rerank_kernel profile's)"},
         "threshold_pass": threshold_pass,
         "simhash_projection": projection,
         "uneven_batches": batches,

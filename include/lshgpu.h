@@ -396,10 +396,12 @@ int lsh_index_batch_stats(lsh_index_t* index, int64_t* graph_batches, int64_t* r
  * recheck runs as its own kernel), [4] total, [5] the matrix-core Hamming
  * scan launches among [1]; [6] the number of those launches and [7] the
  * (query, point) pairs they multiplied per query (summed over launches); [8]
- * the rerank's separate FP64 recheck.  Recorded only while profiling is on:
- * on = 1 times every phase, on = 2 only [3] (two events per search), 0 turns
- * it off.  lsh_index_phase_times fills [0, 8), lsh_index_phase_times_n the
- * first n (entries past [8] read 0). */
+ * the rerank's separate FP64 recheck; [9] the pivot-cell search's appending
+ * pass (among [1]) and [10] its tile products (32 queries x 32 points each).
+ * Recorded only while profiling is on: on = 1 times every phase, on = 2 only
+ * [3] and on = 3 only [9] (two events per search each), 0 turns it off.
+ * lsh_index_phase_times fills [0, 8), lsh_index_phase_times_n the first n
+ * (entries past [10] read 0). */
 int lsh_index_set_profiling(lsh_index_t* index, int on);
 int lsh_index_phase_times(lsh_index_t* index, double* out8);
 int lsh_index_phase_times_n(lsh_index_t* index, double* out, int n);

@@ -634,18 +634,19 @@ class Index:
 
     def set_profiling(self, on=True) -> None:
         """True / 1: time every phase; 2: only the rerank / finalisation phase;
-        False / 0: off."""
+        3: only the pivot-cell search's appending pass; False / 0: off."""
         _check(load().lsh_index_set_profiling(self._h, int(on)))
 
     def phase_times(self) -> dict:
         """Device time per phase (ms) and matrix-core scan counters, summed over
         the searches since profiling was turned on or since the last call
         (which waits for those searches)."""
-        arr = (C.c_double * 9)()
-        _check(load().lsh_index_phase_times_n(self._h, arr, 9))
+        arr = (C.c_double * 11)()
+        _check(load().lsh_index_phase_times_n(self._h, arr, 11))
         v = list(arr)
         return {"prep_ms": v[0], "scan_ms": v[1], "select_ms": v[2], "final_ms": v[3], "total_ms": v[4],
-                "mfma_scan_ms": v[5], "mfma_launches": int(v[6]), "mfma_points": int(v[7]), "recheck_ms": v[8]}
+                "mfma_scan_ms": v[5], "mfma_launches": int(v[6]), "mfma_points": int(v[7]), "recheck_ms": v[8],
+                "pass2_ms": v[9], "pass2_tiles": int(v[10])}
 
     def bucket_stats(self, table: int, hist_len: int = 0):
         """Occupancy of one (K,L) table: ({buckets, nonempty, points, max_size},

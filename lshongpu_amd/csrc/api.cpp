@@ -378,12 +378,14 @@ int lsh_index_search_batches(lsh_index_t* index, const float* queries, const int
 int lsh_index_set_profiling(lsh_index_t* index, int on) {
     return guarded([&] {
         LSH_REQUIRE(index, LSH_ERR_INVALID, "index is null");
-        LSH_REQUIRE(on >= 0 && on <= 2, LSH_ERR_INVALID, "profiling mode must be 0, 1 or 2");
+        LSH_REQUIRE(on >= 0 && on <= 3, LSH_ERR_INVALID, "profiling mode must be 0, 1, 2 or 3");
         std::lock_guard<std::mutex> lk(index->mu);
-        // Mode 2 times only the rerank / finalisation phase (index 3).
-        index->timer.enable(on == 0 ? 0u : on == 1 ? ~0u : (1u << 3));
+        // Mode 2 times only the rerank / finalisation phase (index 3), mode 3
+        // only the pivot-cell search's appending pass (index 7).
+        index->timer.enable(on == 0 ? 0u : on == 1 ? ~0u : on == 2 ? (1u << 3) : (1u << 7));
         index->prof_mfma_launches = 0;
         index->prof_mfma_points = 0;
+        index->prof_pass2_tiles = 0;
     });
 }
 
@@ -391,13 +393,14 @@ int lsh_index_phase_times_n(lsh_index_t* index, double* out, int n) {
     return guarded([&] {
         LSH_REQUIRE(index && out && n >= 0, LSH_ERR_INVALID, "null pointer or negative count");
         std::lock_guard<std::mutex> lk(index->mu);
-        double ms[7];
-        index->timer.collect(ms, 7);
-        const double all[9] = {ms[0], ms[1], ms[2], ms[3], ms[4], ms[5], (double)index->prof_mfma_launches,
-                               (double)index->prof_mfma_points, ms[6]};
-        for (int i = 0; i < n; ++i) out[i] = i < 9 ? all[i] : 0.0;
+        double ms[8];
+        index->timer.collect(ms, 8);
+        const double all[11] = {ms[0], ms[1], ms[2], ms[3], ms[4], ms[5], (double)index->prof_mfma_launches,
+                                (double)index->prof_mfma_points, ms[6], ms[7], (double)index->prof_pass2_tiles};
+        for (int i = 0; i < n; ++i) out[i] = i < 11 ? all[i] : 0.0;
         index->prof_mfma_launches = 0;
         index->prof_mfma_points = 0;
+        index->prof_pass2_tiles = 0;
     });
 }
 

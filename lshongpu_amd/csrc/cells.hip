@@ -1262,7 +1262,8 @@ void cell_search_reserve(const CellIndex& c, int Q, Scratch& sc, int depth) {
 }
 
 const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K, const TopKView& v, Scratch& sc,
-                            int depth, hipStream_t s, int32_t* qorder, const CellFuse* fuse) {
+                            int depth, hipStream_t s, int32_t* qorder, const CellFuse* fuse, PhaseTimer* pt,
+                            int phase) {
     if (!c.built() || Q <= 0) return nullptr;
     const int P = c.P;
     const int Pp = (P + 31) / 32 * 32;
@@ -1346,7 +1347,9 @@ const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K
     hipLaunchKernelGGL(place_from_cells_kernel, dim3(qwaves), dim3(256), 0, s, qcells2, n2, Q, P, cnt2, coff, list,
                        stat);
     check_launch("place_from_cells_kernel");
+    if (pt) pt->begin(phase, s);
     cell_scan(c, grid, coff, choff, cmap, Pp, list, D, qs, v, nullptr, stat, fuse, s);
+    if (pt) pt->end(phase, s);
     hipStreamCaptureStatus capturing = hipStreamCaptureStatusNone;
     if (debug_cells()) LSH_HIP(hipStreamIsCapturing(s, &capturing));
     if (debug_cells() && capturing == hipStreamCaptureStatusNone) {

@@ -19,7 +19,16 @@ using namespace lshgpu;
 
 namespace {
 
-enum Phase { PH_PREP = 0, PH_SCAN = 1, PH_SELECT = 2, PH_FINAL = 3, PH_TOTAL = 4, PH_MFMA = 5, PH_RECHECK = 6 };
+enum Phase {
+    PH_PREP = 0,
+    PH_SCAN = 1,
+    PH_SELECT = 2,
+    PH_FINAL = 3,
+    PH_TOTAL = 4,
+    PH_MFMA = 5,
+    PH_RECHECK = 6,
+    PH_PASS2 = 7  // the pivot-cell search's appending pass
+};
 
 struct DeviceGuard {
     int prev = 0;
@@ -211,6 +220,8 @@ struct SketchSet {
         // for fn when it did).
         const CellFuse* fuse = nullptr;
         bool fused = false;
+        // Out: the appending pass's tile products (32 x 32 pairs each).
+        int64_t pass2_tiles = 0;
     }* hook = nullptr;
 };
 
@@ -325,7 +336,7 @@ void sketch_topk(const SketchSet& d, const SketchSet* samples, int nsamples, int
             pt.begin(PH_SCAN, s);
             pt.begin(PH_MFMA, s);
             cstat = cell_search(*d.cells, qs, Q, K, st.view(), sc, depth, s, d.qorder,
-                                st.view().fused_keys ? d.hook->fuse : nullptr);
+                                st.view().fused_keys ? d.hook->fuse : nullptr, &pt, PH_PASS2);
             pt.end(PH_MFMA, s);
             pt.end(PH_SCAN, s);
             if (cstat && d.ordered) *d.ordered = d.qorder != nullptr;
@@ -347,6 +358,7 @@ void sketch_topk(const SketchSet& d, const SketchSet* samples, int nsamples, int
                 if (mfma_launches) ++*mfma_launches;
                 // Pairs multiplied per query: the cells' tile products cover 32 x 32 pairs.
                 if (mfma_points) *mfma_points += ((int64_t)hstat[1] + hstat[2]) * 1024 / std::max(Q, 1);
+                if (d.hook) d.hook->pass2_tiles += hstat[2];
             }
         }
         if (std::getenv("LSHGPU_DEBUG_CELLS"))
@@ -842,6 +854,7 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
     scan_launches = 0;
     mfma_launches = 0;
     mfma_points = 0;
+    pass2_tiles = 0;
     last_candidates = 0;
     if (Q == 0) return;
     pt.begin(PH_TOTAL, s);
@@ -970,6 +983,7 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
         sketch_topk(full, sv.data(), (int)sv.size(), wpad, impl == 4 ? 1 : 0, qsk.get(), Q, K, kNormal, st,
                     scratch, 0, s, pt, scan_launches,
                     &mfma_launches, &mfma_points);
+        pass2_tiles = hook.pass2_tiles;
         if (!hook.ran) {
             rerank(ordered ? qorder.get() : nullptr, Q, false, false);
         } else if (!hook.redone.empty()) {
@@ -1009,6 +1023,7 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
     if (pt.on()) {
         prof_mfma_launches += mfma_launches;
         prof_mfma_points += mfma_points;
+        prof_pass2_tiles += pass2_tiles;
     }
 }
 

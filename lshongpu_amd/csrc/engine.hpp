@@ -158,8 +158,10 @@ struct CellFuse {
     const _Float16* q16;
     int ksteps;
 };
+// pt (may be null) times the appending pass as phase `phase`.
 const uint32_t* cell_search(const CellIndex& c, const uint32_t* qs, int Q, int K, const TopKView& v, Scratch& sc,
-                            int depth, hipStream_t s, int32_t* qorder, const CellFuse* fuse = nullptr);
+                            int depth, hipStream_t s, int32_t* qorder, const CellFuse* fuse = nullptr,
+                            PhaseTimer* pt = nullptr, int phase = 0);
 // Sizes the scratch slots of depth `depth` for a cell search of Q queries,
 // so that searches of up to Q queries allocate nothing (graph capture).
 void cell_search_reserve(const CellIndex& c, int Q, Scratch& sc, int depth);
@@ -242,6 +244,7 @@ struct lsh_index {
     lshgpu::PhaseTimer timer{false};
     int64_t prof_mfma_launches = 0;
     int64_t prof_mfma_points = 0;
+    int64_t prof_pass2_tiles = 0;
 
     // Working copy of the data after the path's preprocessing.
     lshgpu::DevBuf<float> data;
@@ -308,6 +311,7 @@ struct lsh_index {
     int64_t scan_launches = 0;
     int64_t mfma_launches = 0;   // matrix-core Hamming scan launches of the last search
     int64_t mfma_points = 0;     // data points those launches covered
+    int64_t pass2_tiles = 0;     // the pivot-cell appending pass's tile products (last search)
 
     // Batched search through captured graphs (search_batches): batch slots
     // (created by the first batch call) whose graphs cover query-count
