@@ -517,9 +517,10 @@ def main() -> int:
         else roofline(args, bytes_launch, avg_launch_ms),
         # The rerank's filter beside it, timed in the fully profiled steps.
         "rerank_filter": {"ms": phases["final_ms"], "fused_distances": fused,
-                          "note": "with fused distances the filter reads each candidate's approximate distance "
-                                  "from its key; without, it gathers the candidate rows (the roofline of that "
-                                  "form is the rerank_kernel profile's)"},
+                          "note": "with fused distances the select does the filter's work from the keys (the "
+                                  "k-th 16-bit distance and the band) and no filter kernel runs; without, the "
+                                  "filter gathers the candidate rows (the roofline of that form is the "
+                                  "rerank_kernel profile's)"},
         "threshold_pass": threshold_pass,
         "simhash_projection": projection,
         "uneven_batches": batches,

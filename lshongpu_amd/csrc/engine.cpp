@@ -222,6 +222,10 @@ struct SketchSet {
         bool fused = false;
         // Out: the appending pass's tile products (32 x 32 pairs each).
         int64_t pass2_tiles = 0;
+        // With fused keys the select also writes the rerank's band (band
+        // may be null); banded = true for fn when it did.
+        const FusedBand* band = nullptr;
+        bool banded = false;
     }* hook = nullptr;
 };
 
@@ -300,10 +304,13 @@ void sketch_topk(const SketchSet& d, const SketchSet* samples, int nsamples, int
             uint32_t* hb = sc.pinned((size_t)Q + n_extra);
             pt.begin(PH_SELECT, s);
             const bool rk = st.view().row_keys != 0;
-            launch_select_merge(st.view(), s, hb, extra, extra ? n_extra : 0, rk ? d.hook->row_ids : nullptr);
+            const bool fk = rk && st.view().fused_keys != 0;
+            const FusedBand* fb = fk ? d.hook->band : nullptr;
+            launch_select_merge(st.view(), s, hb, extra, extra ? n_extra : 0, rk ? d.hook->row_ids : nullptr, fb);
             if (d.hook) {
                 d.hook->rows = rk;
-                d.hook->fused = rk && st.view().fused_keys != 0;
+                d.hook->fused = fk;
+                d.hook->banded = fb != nullptr;
             }
             st.selected();
             pt.end(PH_SELECT, s);
@@ -930,7 +937,7 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
         // The queries' FP16 parts for the fused cell scan.
         const bool fuse = fuse_ok(kq);
         if (fuse) {
-            q16.ensure((size_t)Q * 32 * fused_k_steps(d));
+            q16.ensure(query_half_elems(Q, fused_k_steps(d)));
             launch_query_half(qp, Q, d, fused_k_steps(d), q16.get(), s);
         }
         pt.end(PH_PREP, s);
@@ -942,8 +949,16 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
         const int kind = impl == 3 ? kAngular : kGenJaccard;
         // The rerank of queries `order` (all queries when null; m of them);
         // rows: the select left each best-list key's row beside the list;
-        // fused: the keys are fused keys (approximate distance and row).
-        auto rerank = [&](const int32_t* order, int m, bool rows, bool fused) {
+        // fused: the keys are fused keys (approximate distance and row);
+        // banded: the select wrote the band too (indexed by query), and only
+        // the FP64 recheck is left.
+        auto rerank = [&](const int32_t* order, int m, bool rows, bool fused, bool banded) {
+            if (banded) {
+                pt.begin(PH_RECHECK, s);
+                launch_rerank_recheck(cdata.get(), qp, d, kind, Ceff, m, kq, outp, s, nullptr, rband, cells.ids.get());
+                pt.end(PH_RECHECK, s);
+                return;
+            }
             pt.begin(PH_FINAL, s);
             // The rerank's filter (timed as the rerank phase), then its FP64
             // recheck when it splits.
@@ -972,11 +987,18 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
         // Approximate distances fused into the cell scan's keys: the angular
         // index with its FP16 image, k within the filter's split form.
         CellFuse cfuse{};
+        FusedBand fband{};
         if (fuse) {
             cfuse = CellFuse{chalf.get(), half_row_stride(d), q16.get(), fused_k_steps(d)};
             hook.fuse = &cfuse;
+            const int Cp = pow2_ceil(std::max(Ceff, 2));
+            rband.band.ensure((size_t)Q * Cp);
+            rband.cnt.ensure((size_t)Q);
+            fband = FusedBand{rband.band.get(), rband.cnt.get(), query_half_flags(q16.get(), Q, fused_k_steps(d)),
+                              Cp, kq, fused_band_delta(d)};
+            hook.band = &fband;
         }
-        hook.fn = [&] { rerank(ordered ? qorder.get() : nullptr, Q, hook.rows, hook.fused); };
+        hook.fn = [&] { rerank(ordered ? qorder.get() : nullptr, Q, hook.rows, hook.fused, hook.banded); };
         const SketchSet full{dsk.get(), dfp4.get(), N, &cells, cells.built() ? qorder.get() : nullptr, &ordered,
                              &hook};
         const std::vector<SketchSet> sv = sample_views(samples);
@@ -985,7 +1007,7 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
                     &mfma_launches, &mfma_points);
         pass2_tiles = hook.pass2_tiles;
         if (!hook.ran) {
-            rerank(ordered ? qorder.get() : nullptr, Q, false, false);
+            rerank(ordered ? qorder.get() : nullptr, Q, false, false, false);
         } else if (!hook.redone.empty()) {
             // Queries whose candidates were redone after the rerank was
             // enqueued: rerank them again.
@@ -993,7 +1015,7 @@ void lsh_index::search(const float* queries, int Q, int k, bool q_on_device, lsh
             flags.ensure((size_t)m);
             int32_t* idx = reinterpret_cast<int32_t*>(flags.get());
             LSH_HIP(hipMemcpyAsync(idx, hook.redone.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, s));
-            rerank(idx, m, false, false);
+            rerank(idx, m, false, false, false);
         }
     } else {
         qjl.ensure((size_t)Q * sdim);
@@ -1134,17 +1156,27 @@ void lsh_index::enqueue_sketch_search(batch_slot& sl, int Qc, int kq) {
     const bool fused = rk && fuse_ok(kq) && sl.q16.get();
     CellFuse cf{};
     if (rk) sl.st.rows_in_keys();
+    FusedBand fb{};
     if (fused) {
         launch_query_half(sl.q.get(), Qc, d, fused_k_steps(d), sl.q16.get(), s);
         sl.st.fused_in_keys();
         cf = CellFuse{chalf.get(), half_row_stride(d), sl.q16.get(), fused_k_steps(d)};
+        fb = FusedBand{sl.rband.band.get(), sl.rband.cnt.get(), query_half_flags(sl.q16.get(), Qc, fused_k_steps(d)),
+                       pow2_ceil(std::max(Ceff, 2)), kq, fused_band_delta(d)};
     }
     const uint32_t* cstat = cell_search(cells, sl.qsk.get(), Qc, K, sl.st.view(), sl.scratch, 0, s, sl.qorder.get(),
                                         fused ? &cf : nullptr);
     LSH_REQUIRE(cstat != nullptr, LSH_ERR_INTERNAL, "batched search: the cell search did not run");
     // Flags of the Qc queries, then the cell search's three status words.
-    launch_select_merge(sl.st.view(), s, sl.flags.get(), cstat, 3, rk ? cells.ids.get() : nullptr);
+    launch_select_merge(sl.st.view(), s, sl.flags.get(), cstat, 3, rk ? cells.ids.get() : nullptr,
+                        fused ? &fb : nullptr);
     sl.st.selected();
+    if (fused) {
+        // The select wrote the band (indexed by query): the FP64 recheck is left.
+        launch_rerank_recheck(cdata.get(), sl.q.get(), d, kAngular, Ceff, Qc, kq, sl.out.get(), s, nullptr,
+                              sl.rband, cells.ids.get());
+        return;
+    }
     const double eps = fused ? fused_error_bound(d) : chalf.get() ? half_error_bound(d) : fp32_error_bound(kAngular, d);
     const TopKView v = sl.st.view();
     const bool split = launch_rerank(cdata.get(), sl.q.get(), d, kAngular, v.keys, v.stride, nullptr, Ceff, Qc, kq,
@@ -1195,7 +1227,7 @@ void lsh_index::prepare_slots(bool sketch, int kq) {
             sl.rband.band.ensure((size_t)QM * Cp);
             sl.rband.cnt.ensure((size_t)QM);
             sl.flags.ensure((size_t)QM + 3);
-            if (fuse_ok(1)) sl.q16.ensure((size_t)QM * 32 * fused_k_steps(d));
+            if (fuse_ok(1)) sl.q16.ensure(query_half_elems(QM, fused_k_steps(d)));
             sig.insert(sig.end(), {(uintptr_t)sl.q16.get(), (uintptr_t)sl.qsk.get(), (uintptr_t)sl.qorder.get(),
                                    (uintptr_t)sl.scratch.raw[0].get(), (uintptr_t)sl.scratch.raw[1].get(),
                                    (uintptr_t)sl.scratch.raw[2].get(), (uintptr_t)sl.scratch.raw[3].get(),

@@ -244,8 +244,24 @@ int newcap_for(int K);
 // for an empty key), for the rerank.
 // The clearing TopKState::reset does, over a view of memory the caller owns.
 void launch_topk_clear(const TopKView& v, hipStream_t s);
+// fb (may be null; needs v.fused_keys and row_ids): the select also does the
+// rerank filter's work for fused keys (TopKView::fused_keys): with q_k the
+// k-th smallest 16-bit distance among a query's kept keys, every kept key
+// whose q is at most q_k + delta goes to band[query][0, m) as row << 32 (the
+// recheck looks the IDs up), and bcnt[query] = m; every kept key when qbad[query] (the query's
+// FP16 parts are zero) or fewer than k are kept.  delta = floor(2 eps
+// 2^15), eps the fused keys' error bound: the band of launch_rerank.
+struct FusedBand {
+    uint64_t* band;  // [queries][stride]
+    uint32_t* bcnt;
+    const uint32_t* qbad;
+    int stride;
+    int k;
+    int delta;
+};
 void launch_select_merge(const TopKView& v, hipStream_t s, uint32_t* flags = nullptr,
-                         const uint32_t* extra = nullptr, int n_extra = 0, const int32_t* row_ids = nullptr);
+                         const uint32_t* extra = nullptr, int n_extra = 0, const int32_t* row_ids = nullptr,
+                         const FusedBand* fb = nullptr);
 // Whether a view of K best keys and newcap appends can carry rows in its
 // keys (the select then holds every key in registers).
 bool row_keys_supported(int K, int newcap);
@@ -326,8 +342,11 @@ bool launch_rerank(const float* data, const float* queries, int d, int kind, con
                    const int32_t* rowmap = nullptr, const _Float16* hdata = nullptr, RerankBand* rb = nullptr,
                    bool recheck_too = true, const int32_t* rows_in = nullptr, int rows_stride = 0,
                    size_t hdata_elems = 0, const int32_t* fused_ids = nullptr);
+// band_ids (may be null): the band's entries hold rows alone and
+// band_ids[row] is the member's ID.
 void launch_rerank_recheck(const float* data, const float* queries, int d, int kind, int C, int Q, int k,
-                           lsh_point_t* out, hipStream_t s, const int32_t* qorder, RerankBand& rb);
+                           lsh_point_t* out, hipStream_t s, const int32_t* qorder, RerankBand& rb,
+                           const int32_t* band_ids = nullptr);
 
 // FP16 image of npos rows for the angular rerank: row i is x_i * 2^14 /
 // ||x_i|| (FP64 norm), rounded to FP16 and zero padded to
@@ -344,9 +363,17 @@ double half_error_bound(int d);
 // reads: per query 2 x 16 ksteps halves, the high parts of the unit query
 // scaled by 2^14, then the low parts (zero past d; all zero for a query
 // whose norm is zero, not finite or outside [2^-60, 2^60]).
+// After the parts, launch_query_half writes one uint32 per query, 1 when its
+// parts are zero (query_half_flags); query_half_elems is the buffer's length
+// in halves for Q queries.
 int fused_k_steps(int d);
 double fused_error_bound(int d);
+int fused_band_delta(int d);  // floor(2 fused_error_bound(d) 2^15)
 void launch_query_half(const float* q, int Q, int d, int ksteps, _Float16* out, hipStream_t s);
+inline size_t query_half_elems(int Q, int ksteps) { return (size_t)Q * 32 * ksteps + 2 * (size_t)Q; }
+inline uint32_t* query_half_flags(_Float16* q16, int Q, int ksteps) {
+    return reinterpret_cast<uint32_t*>(q16 + (size_t)Q * 32 * ksteps);
+}
 
 // simhash_mfma.hip: SimHash bits (32 hyperplanes a word) on the bf16 matrix
 // cores with an FP32 recheck of every bit within the error bound of zero;

@@ -720,7 +720,8 @@ __global__ __launch_bounds__(64) void rerank_recheck_kernel(const float* __restr
                                                             const uint64_t* __restrict__ band,
                                                             const uint32_t* __restrict__ bcnt, int Cp, int k,
                                                             lsh_point_t* __restrict__ out,
-                                                            const int32_t* __restrict__ qlist) {
+                                                            const int32_t* __restrict__ qlist,
+                                                            const int32_t* __restrict__ band_ids) {
     const int lane = threadIdx.x;
     const int pos = blockIdx.x;
     const int q = qlist ? qlist[pos] : pos;
@@ -735,7 +736,7 @@ __global__ __launch_bounds__(64) void rerank_recheck_kernel(const float* __restr
         int32_t ci = 0x7FFFFFFF;
         if (c0 + lane < m) {
             const uint64_t v = bq[c0 + lane];
-            ci = (int32_t)(uint32_t)v;
+            ci = band_ids ? band_ids[(uint32_t)(v >> 32)] : (int32_t)(uint32_t)v;
             ck = f64_key(dist64_row(KIND, qv, data + (size_t)(uint32_t)(v >> 32) * d, d, nq64));
         }
         wave_sort_pairs(ck, ci);
@@ -790,7 +791,7 @@ __global__ void unit_rows_half_kernel(const float* __restrict__ rows, const int3
 // query whose norm is zero, not finite or outside [2^-60, 2^60] gets zeros
 // (the filter rechecks all its candidates).
 __global__ __launch_bounds__(256) void query_half_kernel(const float* __restrict__ q, int Q, int d, int ksteps,
-                                                         _Float16* __restrict__ out) {
+                                                         _Float16* __restrict__ out, uint32_t* __restrict__ bad) {
     const int qi = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (qi >= Q) return;
@@ -800,7 +801,9 @@ __global__ __launch_bounds__(256) void query_half_kernel(const float* __restrict
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
     const float n = __builtin_sqrtf(s);
-    const float r = (n >= 0x1p-60f && n <= 0x1p60f) ? kHalfScale / n : 0.0f;
+    const bool ok = n >= 0x1p-60f && n <= 0x1p60f;
+    const float r = ok ? kHalfScale / n : 0.0f;
+    if (lane == 0) bad[qi] = ok ? 0u : 1u;
     const int kp = 16 * ksteps;
     _Float16* __restrict__ o = out + (size_t)qi * 2 * kp;
     for (int j = lane; j < kp; j += 64) {
@@ -850,10 +853,13 @@ double fused_error_bound(int d) {
            (32.0 * ks + 0.5 * d + 8.0) * u + 1.0 / 4194304.0 + 1.0 / 65536.0;
 }
 
+int fused_band_delta(int d) { return (int)std::floor(2.0 * fused_error_bound(d) * 32768.0); }
+
 void launch_query_half(const float* q, int Q, int d, int ksteps, _Float16* out, hipStream_t s) {
     if (Q <= 0) return;
     LSH_REQUIRE(ksteps >= 1 && ksteps <= 8 && 16 * ksteps >= d, LSH_ERR_INTERNAL, "query parts: bad K-steps");
-    hipLaunchKernelGGL(query_half_kernel, dim3((unsigned)((Q + 3) / 4)), dim3(256), 0, s, q, Q, d, ksteps, out);
+    hipLaunchKernelGGL(query_half_kernel, dim3((unsigned)((Q + 3) / 4)), dim3(256), 0, s, q, Q, d, ksteps, out,
+                       query_half_flags(out, Q, ksteps));
     check_launch("query_half_kernel");
 }
 
@@ -945,14 +951,15 @@ bool launch_rerank(const float* data, const float* queries, int d, int kind, con
 }
 
 void launch_rerank_recheck(const float* data, const float* queries, int d, int kind, int C, int Q, int k,
-                           lsh_point_t* out, hipStream_t s, const int32_t* qorder, RerankBand& rb) {
+                           lsh_point_t* out, hipStream_t s, const int32_t* qorder, RerankBand& rb,
+                           const int32_t* band_ids) {
     if (Q <= 0) return;
     const int Cp = pow2_ceil(std::max(C, 2));
     LSH_REQUIRE(k <= 64 && rb.band.size() >= (size_t)Q * Cp && rb.cnt.size() >= (size_t)Q, LSH_ERR_INTERNAL,
                 "rerank recheck: no band for these queries");
     auto recheck = [&](auto kernel) {
         hipLaunchKernelGGL(kernel, dim3(Q), dim3(64), 0, s, data, queries, d, rb.band.get(), rb.cnt.get(), Cp, k,
-                           out, qorder);
+                           out, qorder, band_ids);
     };
     switch (kind) {
         case kNegCos: recheck(rerank_recheck_kernel<kNegCos>); break;

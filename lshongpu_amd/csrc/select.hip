@@ -86,12 +86,79 @@ struct KeysInRegisters {
 // flags (may be null, may be pinned host memory): flags[q] = 1 when query q's
 // list is incomplete after the select (overflowed or fewer than K keys),
 // flags[Q + i] = extra[i] for i < n_extra.
+// Wave 0 of the workgroup: the bin of hist[0, 256) at which the running
+// count reaches rank (1-based; rank <= the total), and rank within it, to
+// out[0] and out[1].
+__device__ inline void wave_bin_of_rank(const uint32_t* hist, uint32_t rank, uint32_t* out) {
+    if (threadIdx.x >= 64) return;
+    const int lane = threadIdx.x;
+    uint32_t c[4], sum = 0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        c[e] = hist[4 * lane + e];
+        sum += c[e];
+    }
+    uint32_t inc = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t x = __shfl_up(inc, o);
+        if (lane >= o) inc += x;
+    }
+    uint32_t run = inc - sum;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        if (run < rank && run + c[e] >= rank) {
+            out[0] = 4u * lane + e;
+            out[1] = rank - run;
+        }
+        run += c[e];
+    }
+}
+
+// The rerank filter's work over the nkeep fused keys in kept[] (LDS) of
+// query q: the k-th smallest 16-bit distance q_k (two 8-bit histogram
+// rounds; qh[0, 256) already counts the kept keys' high bytes, qh[256, 512)
+// is zero), then the band, every key with q <= q_k + delta, as (row << 32 |
+// 0), the IDs left to the recheck.  sv: 4 words of LDS.  Every thread must
+// call it, after a barrier that follows the counts.
+__device__ __forceinline__ void fused_band(const uint64_t* kept, uint32_t nkeep, int q, const FusedBand& fb,
+                           uint32_t* qh, uint32_t* sv) {
+    const int tid = threadIdx.x, bs = blockDim.x;
+    auto qof = [](uint64_t key) { return (uint32_t)(key >> 32) & 0xFFFFu; };
+    const bool all = fb.qbad[q] != 0u || nkeep < (uint32_t)fb.k;
+    uint32_t qub = 0xFFFFFFFFu;
+    if (!all) {
+        wave_bin_of_rank(qh, (uint32_t)fb.k, sv);
+        __syncthreads();
+        const uint32_t hb = sv[0], rank = sv[1];
+        for (uint32_t i = tid; i < nkeep; i += bs) {
+            const uint32_t x = qof(kept[i]);
+            if ((x >> 8) == hb) atomicAdd(&qh[256 + (x & 255u)], 1u);
+        }
+        __syncthreads();
+        wave_bin_of_rank(qh + 256, rank, sv + 2);
+        __syncthreads();
+        qub = ((hb << 8) | sv[2]) + (uint32_t)fb.delta;
+    }
+    if (tid == 0) sv[3] = 0;
+    __syncthreads();
+    uint64_t* __restrict__ bq = fb.band + (size_t)q * fb.stride;
+    for (uint32_t i = tid; i < nkeep; i += bs) {
+        const uint64_t key = kept[i];
+        if (qof(key) <= qub) bq[atomicAdd(&sv[3], 1u)] = (uint64_t)(uint32_t)key << 32;
+    }
+    __syncthreads();
+    if (tid == 0) fb.bcnt[q] = sv[3];
+}
+
 template <int RK>
 __global__ __launch_bounds__(kSelThreads) void select_hist_kernel(TopKView v, uint32_t* flags,
                                                                   const uint32_t* extra, int n_extra,
-                                                                  const int32_t* __restrict__ row_ids) {
+                                                                  const int32_t* __restrict__ row_ids,
+                                                                  FusedBand fb) {
     extern __shared__ uint64_t kept[];  // K
     __shared__ uint32_t hist[kSelBins];
+    __shared__ uint32_t qh[512];        // fused band: histograms of the kept keys' 16-bit distances
     __shared__ uint64_t ties[kTieCap];
     __shared__ uint32_t sv[kSelScratch];
     __shared__ uint32_t s_nkeep;
@@ -110,6 +177,10 @@ __global__ __launch_bounds__(kSelThreads) void select_hist_kernel(TopKView v, ui
     const uint64_t* in = v.fresh ? row + K : row;
     const int total = v.fresh ? n : K + n;
     if (threadIdx.x == 0) s_nkeep = 0;
+    // Cleared here, counted in the keep loop: the K-th search's barriers
+    // order the two.
+    if (fb.band)
+        for (int i = threadIdx.x; i < 512; i += blockDim.x) qh[i] = 0;
     auto run = [&](const auto& each) {
         const uint64_t kth = int_kth_key(each, (uint32_t)K, hist, ties, sv);  // the K-th key (none: keep all)
         each([&](uint64_t key) {
@@ -162,9 +233,11 @@ __global__ __launch_bounds__(kSelThreads) void select_hist_kernel(TopKView v, ui
                 const uint64_t key = kr.key[r];
                 if (key != kEmptyKey && key <= kth) {
                     const uint32_t at = atomicAdd(&s_nkeep, 1u);
-                    if (at < (uint32_t)K)
-                        kept[at] = make_key(((uint32_t)(key >> 32) << 16) | ((fq[r >> 1] >> (16 * (r & 1))) & 0xFFFFu),
-                                            brow[r]);
+                    const uint32_t fqr = (fq[r >> 1] >> (16 * (r & 1))) & 0xFFFFu;
+                    if (at < (uint32_t)K) {
+                        kept[at] = make_key(((uint32_t)(key >> 32) << 16) | fqr, brow[r]);
+                        if (fb.band) atomicAdd(&qh[fqr >> 8], 1u);
+                    }
                 }
             }
         } else if (v.row_keys) {
@@ -205,6 +278,9 @@ __global__ __launch_bounds__(kSelThreads) void select_hist_kernel(TopKView v, ui
         if (n_raw > (uint32_t)v.newcap) v.ovf[q] = 1;
         if (flags) flags[q] = (v.ovf[q] != 0u || nkeep != (uint32_t)K) ? 1u : 0u;
     }
+    // sv is free (int_kth_key ended with a barrier); the barrier after the
+    // keep loop ordered the counts in qh.
+    if (fb.band) fused_band(kept, nkeep, q, fb, qh, sv);
 }
 
 // Sorts the first K keys of every row ascending (K <= 4096).
@@ -317,11 +393,16 @@ bool row_keys_supported(int K, int newcap) {
 }
 
 void launch_select_merge(const TopKView& v, hipStream_t s, uint32_t* flags, const uint32_t* extra, int n_extra,
-                         const int32_t* row_ids) {
+                         const int32_t* row_ids, const FusedBand* fbp) {
     LSH_REQUIRE(n_extra >= 0 && n_extra <= kSelThreads && (n_extra == 0 || extra), LSH_ERR_INTERNAL,
                 "select: bad extra words");
     LSH_REQUIRE(!v.row_keys || (v.int_keys && v.fresh && row_ids && row_keys_supported(v.K, v.newcap)),
                 LSH_ERR_INTERNAL, "select: row keys need a fresh integer-key view, the layout's IDs and room");
+    LSH_REQUIRE(!fbp || (v.fused_keys && v.K <= kMaxIntTopK && row_ids && fbp->band && fbp->bcnt && fbp->qbad &&
+                         fbp->k >= 1 &&
+                         fbp->stride >= v.K && fbp->delta >= 0),
+                LSH_ERR_INTERNAL, "select: the fused band needs fused keys and its buffers");
+    const FusedBand fb = fbp ? *fbp : FusedBand{};
     if (v.int_keys && v.Q > 0) {
         const size_t lds = (size_t)v.K * sizeof(uint64_t);
         const int most = v.fresh ? v.newcap : v.K + v.newcap;  // keys a query can hold
@@ -329,11 +410,12 @@ void launch_select_merge(const TopKView& v, hipStream_t s, uint32_t* flags, cons
         if (most <= kSelThreads * RK) {
             set_dynamic_lds((const void*)select_hist_kernel<RK>, lds + sizeof(uint32_t) * (kSelBins + 8));
             hipLaunchKernelGGL(select_hist_kernel<RK>, dim3(v.Q), dim3(kSelThreads), lds, s, v, flags, extra,
-                               n_extra, row_ids);
+                               n_extra, row_ids, fb);
         } else {
             set_dynamic_lds((const void*)select_hist_kernel<0>, lds + sizeof(uint32_t) * (kSelBins + 8));
+            LSH_REQUIRE(!fbp, LSH_ERR_INTERNAL, "select: the fused band needs the register select");
             hipLaunchKernelGGL(select_hist_kernel<0>, dim3(v.Q), dim3(kSelThreads), lds, s, v, flags, extra,
-                               n_extra, row_ids);
+                               n_extra, row_ids, fb);
         }
         check_launch("select_hist_kernel");
         return;

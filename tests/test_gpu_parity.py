@@ -740,12 +740,16 @@ def test_index_device_pointers(lsh):
     np.testing.assert_array_equal(got, host)
 
 
-def test_index_searches_on_streams_are_ordered(lsh):
+@pytest.mark.parametrize("fused", ["1", "0"])
+def test_index_searches_on_streams_are_ordered(lsh, monkeypatch, fused):
     # Device results on a caller's stream: the search may return before its
     # rerank has run.  Back-to-back searches on two streams into two outputs
     # are ordered by the index, and each matches the synchronous search; the
-    # rerank's split filter and FP64 recheck (profiling mode 2 times the
-    # filter only) run on the pivot-cell index.
+    # FP64 recheck runs as its own kernel on the pivot-cell index, after the
+    # split filter (fused = "0", profiling mode 2 times the filter only) or
+    # after the select that wrote the band from fused keys ("1", no filter
+    # kernel; mode 3 times the cell search's appending pass only).
+    monkeypatch.setenv("LSHGPU_FUSED", fused)
     torch = pytest.importorskip("torch")
     rng = np.random.default_rng(25)
     N, Q, d = 120000, 300, 48
@@ -756,7 +760,7 @@ def test_index_searches_on_streams_are_ordered(lsh):
     s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
     with lsh.Index(dd, implementation=3, sketchDim=16, distanceFunc=1, candidates=256) as ix:
         want = ix.search(queries, 32)
-        ix.set_profiling(2)
+        ix.set_profiling(2 if fused == "0" else 3)
         outs = []
         for rep in range(3):
             o1 = torch.full((Q, 64), -7, dtype=torch.int32, device="cuda")
@@ -771,7 +775,11 @@ def test_index_searches_on_streams_are_ordered(lsh):
         st.synchronize()
         got = o.cpu().numpy().view(lsh.POINT_DTYPE).reshape(Q, 32)
         np.testing.assert_array_equal(got, want)
-    assert pt["final_ms"] > 0 and pt["scan_ms"] == 0 and pt["recheck_ms"] == 0
+    assert pt["scan_ms"] == 0 and pt["recheck_ms"] == 0
+    if fused == "0":
+        assert pt["final_ms"] > 0 and pt["pass2_ms"] == 0
+    else:
+        assert pt["final_ms"] == 0 and pt["pass2_ms"] > 0 and pt["pass2_tiles"] > 0
 
 
 @pytest.mark.parametrize("kw", [dict(implementation=1), dict(implementation=2, distanceFunc=2),
