@@ -268,7 +268,7 @@ def main() -> int:
 
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
-                                   "(sharding over GPUs is described in DESIGN.md, not implemented)"}))
+                                   "(sharding the index over several GPUs is not implemented)"}))
         return 2
 
     import numpy as np
